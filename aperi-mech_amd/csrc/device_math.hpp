@@ -1,0 +1,148 @@
+// Device-side FP64 math of the force step: 3x3 algebra and the stress updates.
+//
+// Stress updates restate the reference StressFunctor::GetStress implementations:
+//   linear elastic  include/Materials/LinearElastic.h:70-88
+//   neo-Hookean     include/Materials/NeoHooke.h:67-92 (J-1 from DetApIm1, MathUtils.h:333-339;
+//                   F^-1 by cofactors, as the reference's CUDA build does, MathUtils.h:314-316)
+//   J2              include/Materials/Plastic.h:88-140
+//   SVK             include/Materials/Elastic.h:61-79
+// The material is a compile-time template parameter; the reference's device
+// virtual call (ComputeInternalForceGaussian.h:107) is resolved on the host.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "aperi_hip.h"
+
+namespace aphip {
+
+#define APHIP_DEV __device__ __forceinline__
+
+// det(I + A) - 1 by principal invariants (MathUtils.h:333-339); A row-major.
+APHIP_DEV double det_ap_im1(const double *A) {
+    return A[0] + A[4] + A[8] - A[1] * A[3] * (1 + A[8]) + A[0] * A[4] * (1 + A[8]) - A[2] * A[6] * (1 + A[4]) -
+           A[5] * A[7] * (1 + A[0]) + A[0] * A[8] + A[4] * A[8] + A[1] * A[5] * A[6] + A[2] * A[3] * A[7];
+}
+
+// Cofactor inverse of a row-major 3x3; returns det.
+APHIP_DEV double inverse3(const double *A, double *Ai) {
+    const double c00 = A[4] * A[8] - A[5] * A[7];
+    const double c10 = A[5] * A[6] - A[3] * A[8];
+    const double c20 = A[3] * A[7] - A[4] * A[6];
+    const double det = A[0] * c00 + A[1] * c10 + A[2] * c20;
+    const double id = 1.0 / det;
+    Ai[0] = c00 * id;
+    Ai[1] = (A[2] * A[7] - A[1] * A[8]) * id;
+    Ai[2] = (A[1] * A[5] - A[2] * A[4]) * id;
+    Ai[3] = c10 * id;
+    Ai[4] = (A[0] * A[8] - A[2] * A[6]) * id;
+    Ai[5] = (A[2] * A[3] - A[0] * A[5]) * id;
+    Ai[6] = c20 * id;
+    Ai[7] = (A[1] * A[6] - A[0] * A[7]) * id;
+    Ai[8] = (A[0] * A[4] - A[1] * A[3]) * id;
+    return det;
+}
+
+struct MatParams {
+    double lambda, mu, yield_stress, hardening;
+};
+
+// P = stress(H); sn / snp1: J2 equivalent plastic strain in / out.
+template <int KIND>
+APHIP_DEV void stress(const double *H, const MatParams &m, double sn, double &snp1, double *P) {
+    if constexpr (KIND == APHIP_LINEAR_ELASTIC) {
+        const double two_mu = m.mu * 2.0;
+        double eps[9];
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) eps[3 * i + j] = 0.5 * (H[3 * i + j] + H[3 * j + i]);
+        const double tr = eps[0] + eps[4] + eps[8];
+#pragma unroll
+        for (int i = 0; i < 9; ++i) P[i] = two_mu * eps[i];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) P[4 * i] = m.lambda * tr + two_mu * eps[4 * i];
+        snp1 = sn;
+    } else if constexpr (KIND == APHIP_NEO_HOOKEAN) {
+        double tau[9], F[9], Fi[9];
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                double hh = H[3 * i] * H[3 * j] + H[3 * i + 1] * H[3 * j + 1] + H[3 * i + 2] * H[3 * j + 2];
+                tau[3 * i + j] = m.mu * (hh + H[3 * j + i] + H[3 * i + j]);
+            }
+        const double lnJ = log1p(det_ap_im1(H));
+#pragma unroll
+        for (int i = 0; i < 3; ++i) tau[4 * i] += m.lambda * lnJ;
+#pragma unroll
+        for (int i = 0; i < 9; ++i) F[i] = H[i];
+        F[0] += 1.0;
+        F[4] += 1.0;
+        F[8] += 1.0;
+        inverse3(F, Fi);
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+                P[3 * i + j] = tau[3 * i] * Fi[3 * j] + tau[3 * i + 1] * Fi[3 * j + 1] + tau[3 * i + 2] * Fi[3 * j + 2];
+        snp1 = sn;
+    } else if constexpr (KIND == APHIP_J2) {
+        double ep = sn;
+        double dev[9];
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) dev[3 * i + j] = 0.5 * (H[3 * i + j] + H[3 * j + i]);
+        const double tr = dev[0] + dev[4] + dev[8];
+        const double K = m.lambda + 2.0 * m.mu / 3.0;
+        const double p = K * tr;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) dev[4 * i] -= (1.0 / 3.0) * tr;
+        double n2 = 0.0;
+#pragma unroll
+        for (int i = 0; i < 9; ++i) {
+            dev[i] *= 2.0 * m.mu;
+            n2 += dev[i] * dev[i];
+        }
+        const double eta_norm = sqrt(n2);
+        const double q = sqrt(3.0 / 2.0) * eta_norm;
+        const double phi = q - (m.yield_stress + m.hardening * ep);
+        if (phi > 0.0) {
+            const double dep = phi / (3.0 * m.mu + m.hardening);
+            const double s = sqrt(6.0) * m.mu * dep / eta_norm;
+#pragma unroll
+            for (int i = 0; i < 9; ++i) dev[i] -= s * dev[i];
+            ep += dep;
+        }
+        snp1 = ep;
+#pragma unroll
+        for (int i = 0; i < 9; ++i) P[i] = dev[i];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) P[4 * i] += p;
+    } else {  // SVK
+        const double two_mu = m.mu * 2.0;
+        double E[9], S[9];
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                const double hth = H[i] * H[j] + H[3 + i] * H[3 + j] + H[6 + i] * H[6 + j];
+                E[3 * i + j] = 0.5 * (H[3 * i + j] + H[3 * j + i] + hth);
+            }
+        const double tr = E[0] + E[4] + E[8];
+#pragma unroll
+        for (int i = 0; i < 9; ++i) S[i] = two_mu * E[i];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) S[4 * i] += m.lambda * tr;
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+                P[3 * i + j] = S[j] * ((i == 0 ? 1.0 : 0.0) + H[3 * i]) + S[3 + j] * ((i == 1 ? 1.0 : 0.0) + H[3 * i + 1]) +
+                               S[6 + j] * ((i == 2 ? 1.0 : 0.0) + H[3 * i + 2]);
+        snp1 = sn;
+    }
+}
+
+}  // namespace aphip

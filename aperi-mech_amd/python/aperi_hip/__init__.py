@@ -1,0 +1,245 @@
+"""Python host bindings of the aperi-hip C ABI (include/aperi_hip.h).
+
+The compute lives in libaperi_hip.so (hand-written gfx950 HIP kernels); this
+module only marshals numpy arrays through ctypes.  There is no CPU fallback:
+if the library is missing, loading fails with an error.
+
+The `Solver` class mirrors the calling sequence of aperi-mech's ExplicitSolver
+(src/Solvers/Explicit.cpp:308-452): mesh + material (element block and
+Material), mass (BuildMassMatrix), initial force/acceleration, then fixed-dt
+steps; `internal_force` mirrors ElementBase::ComputeInternalForceAllElements.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from . import meshgen  # noqa: F401
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.normpath(os.path.join(_PKG, "..", "..", "lib", "libaperi_hip.so"))
+
+TET4, HEX8 = 0, 1
+LINEAR_ELASTIC, NEO_HOOKEAN, J2, SVK = 0, 1, 2, 3
+FIELD_U, FIELD_V, FIELD_A, FIELD_F, FIELD_MASS, FIELD_H, FIELD_P, FIELD_STATE, FIELD_VOLUME = range(9)
+MAX_NODE_NUM_NEIGHBORS = 40
+
+_d = ctypes.POINTER(ctypes.c_double)
+_i32 = ctypes.POINTER(ctypes.c_int32)
+_i64 = ctypes.POINTER(ctypes.c_int64)
+_u8 = ctypes.POINTER(ctypes.c_uint8)
+
+EXPORTED = [
+    "aphip_create", "aphip_destroy", "aphip_last_error", "aphip_device_count", "aphip_set_mesh", "aphip_get_node_permutation",
+    "aphip_set_material", "aphip_set_material_fields", "aphip_compute_mass", "aphip_set_fixed",
+    "aphip_set_field", "aphip_get_field", "aphip_internal_force", "aphip_initialize", "aphip_step",
+    "aphip_synchronize", "aphip_rk_setup", "aphip_rk_get_neighbors", "aphip_rk_get_function_values",
+    "aphip_rk_get_cells", "aphip_set_profiling", "aphip_get_profile", "aphip_reset_profile",
+]
+
+
+class Config(ctypes.Structure):
+    _fields_ = [("device", ctypes.c_int32), ("flags", ctypes.c_int32)]
+
+
+class MaterialParams(ctypes.Structure):
+    _fields_ = [("density", ctypes.c_double), ("youngs_modulus", ctypes.c_double),
+                ("poissons_ratio", ctypes.c_double), ("yield_stress", ctypes.c_double),
+                ("hardening_modulus", ctypes.c_double), ("state_per_qp", ctypes.c_int32),
+                ("reserved", ctypes.c_int32)]
+
+
+class AphipError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"aperi-hip error {code}: {msg}")
+        self.code = code
+
+
+_LIB = None
+
+
+def load_library(path=LIB_PATH):
+    """Load libaperi_hip.so (raises if it has not been built)."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(path):
+        raise FileNotFoundError(f"{path} not built; run __graft_entry__.build() or make -C aperi-mech_amd")
+    L = ctypes.CDLL(path)
+    L.aphip_last_error.restype = ctypes.c_char_p
+    for name in EXPORTED:
+        fn = getattr(L, name)
+        if name != "aphip_last_error":
+            fn.restype = ctypes.c_int
+    _LIB = L
+    return L
+
+
+def device_count():
+    """Visible HIP devices, as reported by the library's own HIP runtime."""
+    L = load_library()
+    n = ctypes.c_int32()
+    _check(L.aphip_device_count(ctypes.byref(n)))
+    return n.value
+
+
+def _check(rc):
+    if rc != 0:
+        raise AphipError(rc, _LIB.aphip_last_error().decode())
+
+
+def _p(a, t=_d):
+    return None if a is None else a.ctypes.data_as(t)
+
+
+class Solver:
+    """One element block on one GPU behind the C ABI."""
+
+    def __init__(self, device=0):
+        self.L = load_library()
+        self.ctx = ctypes.c_void_p()
+        cfg = Config(device, 0)
+        _check(self.L.aphip_create(ctypes.byref(self.ctx), ctypes.byref(cfg)))
+        self.nn = self.ne = 0
+        self.etype = None
+        self.kind = None
+        self.state_per_qp = 0
+
+    def close(self):
+        if self.ctx:
+            self.L.aphip_destroy(self.ctx)
+            self.ctx = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # ---- set-up ----
+    def set_mesh(self, etype, X, conn):
+        X = np.ascontiguousarray(X, dtype=np.float64)
+        conn = np.ascontiguousarray(conn, dtype=np.int32)
+        self.nn, self.ne, self.etype = X.shape[0], conn.shape[0], etype
+        _check(self.L.aphip_set_mesh(self.ctx, etype, self.nn, _p(X), self.ne, _p(conn, _i32)))
+        return self
+
+    def node_permutation(self):
+        out = np.zeros(self.nn, dtype=np.int32)
+        _check(self.L.aphip_get_node_permutation(self.ctx, _p(out, _i32)))
+        return out
+
+    def set_material(self, kind, density, E, nu, yield_stress=0.0, hardening=0.0, state_per_qp=0):
+        p = MaterialParams(density, E, nu, yield_stress, hardening, state_per_qp, 0)
+        _check(self.L.aphip_set_material(self.ctx, kind, ctypes.byref(p)))
+        self.kind, self.state_per_qp = kind, state_per_qp
+        return self
+
+    def set_material_fields(self, E=None, nu=None):
+        if E is None:
+            _check(self.L.aphip_set_material_fields(self.ctx, None, None))
+            return self
+        E = np.ascontiguousarray(E, dtype=np.float64)
+        nu = np.ascontiguousarray(nu, dtype=np.float64)
+        _check(self.L.aphip_set_material_fields(self.ctx, _p(E), _p(nu)))
+        return self
+
+    def compute_mass(self):
+        tot = ctypes.c_double()
+        _check(self.L.aphip_compute_mass(self.ctx, ctypes.byref(tot)))
+        return tot.value
+
+    def set_fixed(self, mask):
+        if mask is None:
+            _check(self.L.aphip_set_fixed(self.ctx, None))
+            return self
+        m = np.ascontiguousarray(mask, dtype=np.uint8)
+        _check(self.L.aphip_set_fixed(self.ctx, _p(m, _u8)))
+        return self
+
+    def set_field(self, field, data):
+        d = np.ascontiguousarray(data, dtype=np.float64)
+        _check(self.L.aphip_set_field(self.ctx, field, _p(d)))
+        return self
+
+    def _shape(self, field):
+        if field in (FIELD_U, FIELD_V, FIELD_A, FIELD_F):
+            return (self.nn, 3)
+        if field == FIELD_MASS:
+            return (self.nn,)
+        if field in (FIELD_H, FIELD_P):
+            return (self.ne, 9)
+        if field == FIELD_STATE:
+            return (self.ne * (8 if (self.etype == HEX8 and self.state_per_qp) else 1),)
+        return (self.ne,)
+
+    def get(self, field):
+        out = np.zeros(self._shape(field))
+        _check(self.L.aphip_get_field(self.ctx, field, _p(out)))
+        return out
+
+    # ---- hot path ----
+    def internal_force(self, time=0.0, dt=0.0):
+        _check(self.L.aphip_internal_force(self.ctx, ctypes.c_double(time), ctypes.c_double(dt)))
+        return self
+
+    def initialize(self):
+        _check(self.L.aphip_initialize(self.ctx))
+        return self
+
+    def step(self, dt, n_steps=1):
+        _check(self.L.aphip_step(self.ctx, ctypes.c_double(dt), int(n_steps)))
+        return self
+
+    def synchronize(self):
+        _check(self.L.aphip_synchronize(self.ctx))
+        return self
+
+    # ---- RKPM ----
+    def rk_setup(self, kernel_radius_scale):
+        _check(self.L.aphip_rk_setup(self.ctx, ctypes.c_double(kernel_radius_scale)))
+        return self
+
+    def rk_neighbors(self):
+        num = np.zeros(self.nn, dtype=np.int32)
+        nbr = np.zeros((self.nn, MAX_NODE_NUM_NEIGHBORS), dtype=np.int32)
+        tr = ctypes.c_int64()
+        _check(self.L.aphip_rk_get_neighbors(self.ctx, _p(num, _i32), _p(nbr, _i32), ctypes.byref(tr)))
+        return num, nbr, tr.value
+
+    def rk_function_values(self):
+        phi = np.zeros((self.nn, MAX_NODE_NUM_NEIGHBORS))
+        _check(self.L.aphip_rk_get_function_values(self.ctx, _p(phi)))
+        return phi
+
+    def rk_cells(self):
+        nnz = ctypes.c_int64()
+        _check(self.L.aphip_rk_get_cells(self.ctx, ctypes.byref(nnz), None, None, None, None))
+        start = np.zeros(self.ne + 1, dtype=np.int64)
+        nodes = np.zeros(nnz.value, dtype=np.int32)
+        dphi = np.zeros((nnz.value, 3))
+        vol = np.zeros(self.ne)
+        _check(self.L.aphip_rk_get_cells(self.ctx, ctypes.byref(nnz), _p(start, _i64), _p(nodes, _i32), _p(dphi),
+                                         _p(vol)))
+        return start, nodes, dphi, vol
+
+    # ---- profiling ----
+    def set_profiling(self, on=True):
+        _check(self.L.aphip_set_profiling(self.ctx, 1 if on else 0))
+        return self
+
+    def profile(self, which):
+        n = ctypes.c_int64()
+        ms = ctypes.c_double()
+        _check(self.L.aphip_get_profile(self.ctx, which, ctypes.byref(n), ctypes.byref(ms)))
+        return n.value, ms.value
+
+    def reset_profile(self):
+        _check(self.L.aphip_reset_profile(self.ctx))
+        return self

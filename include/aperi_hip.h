@@ -1,0 +1,176 @@
+/*
+ * aperi_hip.h — C ABI of the MI355X (gfx950) explicit internal-force step.
+ *
+ * This library is a drop-in for the hot path of aperi-mech's explicit solver:
+ * the per-step internal-force evaluation of an element block and the
+ * central-difference update around it.  Every entry point names the
+ * reference interface it replaces (paths relative to the aperi-mech tree).
+ *
+ * Conventions
+ *   - Every call returns APHIP_OK (0) or a negative error code; the message of
+ *     the last failure on the calling thread is returned by aphip_last_error().
+ *   - Host arrays are in the caller's node / element numbering and in the
+ *     reference's field layout (include/Field.h:129-138): node vectors are
+ *     n_nodes x 3 row-major, element tensors n_elem x 9 row-major (3*i+j),
+ *     connectivity n_elem x nodes_per_element.  The library renumbers nodes
+ *     and elements internally (Morton order) and converts on the way in/out.
+ *   - A context owns one HIP stream and all device memory it allocates.  Calls
+ *     on one context are stream-ordered and must come from one thread at a time.
+ *   - All arithmetic is FP64.
+ */
+#ifndef APERI_HIP_H
+#define APERI_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define APHIP_OK 0
+#define APHIP_ERR_INVALID (-1)   /* bad argument or call order            */
+#define APHIP_ERR_HIP (-2)       /* HIP runtime error                      */
+#define APHIP_ERR_ALLOC (-3)     /* device or host allocation failed       */
+#define APHIP_ERR_NUMERIC (-4)   /* inverted element / non-finite result   */
+#define APHIP_ERR_UNSUPPORTED (-5)
+
+typedef struct aphip_ctx aphip_ctx;
+
+/* Element topology: include/Constants.h:23-25 (ElementTopology). */
+enum { APHIP_TET4 = 0, APHIP_HEX8 = 1 };
+
+/* Material kinds: include/Materials/{LinearElastic,NeoHooke,Plastic,Elastic}.h */
+enum { APHIP_LINEAR_ELASTIC = 0, APHIP_NEO_HOOKEAN = 1, APHIP_J2 = 2, APHIP_SVK = 3 };
+
+/* Fields for aphip_get_field / aphip_set_field.
+ * Node fields (n_nodes x 3): U, V, A, F; MASS is n_nodes x 1.
+ * Element fields (n_elem x 9): H (displacement_gradient), P (pk1_stress);
+ * STATE is n_elem x 1 (J2 equivalent plastic strain; n_elem x 8 with state_per_qp);
+ * VOLUME is n_elem x 1. */
+enum {
+    APHIP_FIELD_U = 0,
+    APHIP_FIELD_V = 1,
+    APHIP_FIELD_A = 2,
+    APHIP_FIELD_F = 3,
+    APHIP_FIELD_MASS = 4,
+    APHIP_FIELD_H = 5,
+    APHIP_FIELD_P = 6,
+    APHIP_FIELD_STATE = 7,
+    APHIP_FIELD_VOLUME = 8
+};
+
+typedef struct {
+    int32_t device; /* HIP device ordinal */
+    int32_t flags;  /* reserved, 0 */
+} aphip_config;
+
+/* Material parameters.  Replaces the YAML-built MaterialProperties
+ * (include/Materials/Base.h:163-168) consumed by CreateMaterial
+ * (include/Materials/Factory.h:43-160). */
+typedef struct {
+    double density;
+    double youngs_modulus;
+    double poissons_ratio;
+    double yield_stress;      /* J2 only */
+    double hardening_modulus; /* J2 only */
+    int32_t state_per_qp;     /* J2 + Hex8: one state per Gauss point (extension; 0 = reference) */
+    int32_t reserved;
+} aphip_material_params;
+
+/* ---- lifetime ---------------------------------------------------------- */
+
+/* Replaces construction of ElementBase + ComputeInternalForceBase
+ * (include/ElementBase.h:27-117, include/ComputeInternalForceBase.h:34-89). */
+int aphip_create(aphip_ctx **ctx, const aphip_config *cfg);
+int aphip_destroy(aphip_ctx *ctx);
+const char *aphip_last_error(void);
+/* Number of visible HIP devices (0 when none); no reference counterpart. */
+int aphip_device_count(int32_t *count);
+
+/* ---- mesh, material, state ----------------------------------------------- */
+
+/* Element block: replaces the STK mesh/field binding of
+ * NgpMeshData::ForEachElementAndConnectedNodes (include/NgpMeshData.h:131-153)
+ * and the element volume functor (include/ComputeElementVolumeFunctor.h:47-69). */
+int aphip_set_mesh(aphip_ctx *ctx, int32_t elem_type, int64_t n_nodes, const double *X, int64_t n_elem,
+                   const int32_t *conn);
+/* perm[i] = internal (Morton) index of caller node i. */
+int aphip_get_node_permutation(aphip_ctx *ctx, int32_t *perm);
+
+/* Material of the block: replaces CreateMaterial (include/Materials/Factory.h:43-160)
+ * and LinearElasticProperties::CompleteProperties (include/Materials/Base.h:47-126). */
+int aphip_set_material(aphip_ctx *ctx, int32_t kind, const aphip_material_params *params);
+/* Per-element Young's modulus / Poisson ratio (extension; no reference counterpart:
+ * the reference holds one material per part).  NULL, NULL clears them. */
+int aphip_set_material_fields(aphip_ctx *ctx, const double *E, const double *nu);
+
+/* Lumped mass rho V_e / n_nodes per node: replaces ComputeMassMatrixForPart +
+ * FinishComputingMassMatrix (src/MassUtils.cpp:57-112).  After aphip_rk_setup the
+ * mass is scattered through the RK functions (MassUtils.cpp:87-97). */
+int aphip_compute_mass(aphip_ctx *ctx, double *total_mass);
+
+/* Velocity/acceleration held at zero on masked DOFs (n_nodes x 3 bytes, 1 = fixed):
+ * replaces BoundaryCondition::ApplyVelocity/ApplyAcceleration for a fixed set
+ * (src/BoundaryCondition.cpp:33-72).  NULL clears the mask. */
+int aphip_set_fixed(aphip_ctx *ctx, const uint8_t *mask);
+
+/* Upload / download a field in caller numbering and reference layout. */
+int aphip_set_field(aphip_ctx *ctx, int32_t field, const double *data);
+int aphip_get_field(aphip_ctx *ctx, int32_t field, double *out);
+
+/* ---- the hot path -------------------------------------------------------- */
+
+/* f = -sum_e int B^T P dV from the current displacement; writes H, P and the NP1
+ * state.  Replaces ElementTetrahedron4/ElementHexahedron8::ComputeInternalForceAllElements
+ * (include/ElementTetrahedron4.h:184-192, include/ElementHexahedron8.h:174-182) and,
+ * after aphip_rk_setup, ElementReproducingKernel::ComputeInternalForceAllElements
+ * (include/ElementReproducingKernel.h:214-224). */
+int aphip_internal_force(aphip_ctx *ctx, double time, double time_increment);
+
+/* Initial force and acceleration a = f/m, as before the time loop of
+ * ExplicitSolver::Solve (src/Solvers/Explicit.cpp:362-376). */
+int aphip_initialize(aphip_ctx *ctx);
+
+/* n_steps of the explicit loop body (src/Solvers/Explicit.cpp:383-452) with a fixed
+ * increment (DirectTimeStepper, include/TimeStepper.h:97-128): state swap, v += dt/2 a,
+ * velocity BC, u += dt v, internal force, a = f/m, acceleration BC, v += dt/2 a. */
+int aphip_step(aphip_ctx *ctx, double time_increment, int32_t n_steps);
+
+/* Wait for all work queued on the context's stream. */
+int aphip_synchronize(aphip_ctx *ctx);
+
+/* ---- RKPM (one-pass, element-as-cell strain smoothing) ------------------- */
+
+/* Max edge length, kernel radius r = scale * l, variable-radius ball search with the
+ * 40-neighbor cap, RK shape functions and the smoothed-cell CSR.  Replaces
+ * MaxEdgeLengthProcessor::ComputeMaxEdgeLength (src/MaxEdgeLengthProcessor.cpp:87-104),
+ * NeighborSearchProcessor::AddNeighborsWithinVariableSizedBall
+ * (include/NeighborSearchProcessor.h:196-317, src/NeighborSearchProcessor.cpp:154-240),
+ * FunctionCreationProcessor::compute_and_store_function_values
+ * (include/FunctionCreationProcessor.h:73-134) and
+ * SmoothedCellDataProcessor::SetSubcellNodeIndices / ComputeFunctionDerivatives
+ * (include/SmoothedCellDataProcessor.h:407-675).  Requires a Tet4 mesh. */
+int aphip_rk_setup(aphip_ctx *ctx, double kernel_radius_scale);
+/* Neighbor lists in caller numbering: num[n_nodes], nbr[n_nodes x 40] (-1 padded),
+ * the 40-cap truncation count (NeighborSearchProcessor.h:307-316). */
+int aphip_rk_get_neighbors(aphip_ctx *ctx, int32_t *num, int32_t *nbr, int64_t *n_truncated);
+/* RK function values phi[n_nodes x 40] aligned with the neighbor lists. */
+int aphip_rk_get_function_values(aphip_ctx *ctx, double *phi);
+/* Cell CSR in caller numbering: start[n_elem+1], nodes[nnz], dphi[nnz x 3], volume[n_elem].
+ * Pass NULL arrays to query nnz only. */
+int aphip_rk_get_cells(aphip_ctx *ctx, int64_t *nnz, int64_t *start, int32_t *nodes, double *dphi,
+                       double *volume);
+
+/* ---- profiling ----------------------------------------------------------- */
+
+/* Record HIP events around every launch of the step kernels on the context's stream. */
+int aphip_set_profiling(aphip_ctx *ctx, int32_t enable);
+/* Accumulated launches and device milliseconds of kernel class `which`
+ * (0 = whole step, 1 = element/force kernel, 2 = node update kernel) since the last reset. */
+int aphip_get_profile(aphip_ctx *ctx, int32_t which, int64_t *launches, double *milliseconds);
+int aphip_reset_profile(aphip_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

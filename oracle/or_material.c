@@ -1,0 +1,100 @@
+/*
+ * or_material.c — stress updates.  TEST INFRASTRUCTURE (oracle).
+ *
+ * Each branch restates one reference StressFunctor::GetStress:
+ *   linear elastic  include/Materials/LinearElastic.h:70-88
+ *   neo-Hookean     include/Materials/NeoHooke.h:67-92 (InvertMatrix = FullPivLU, MathUtils.h:307-319)
+ *   J2 plasticity   include/Materials/Plastic.h:88-140 (small strain, Simo-Hughes box 7.5)
+ *   SVK "elastic"   include/Materials/Elastic.h:61-79
+ *
+ * Pinning (tests/test_oracle_materials.py):
+ *   LE   textbook Hooke's law on random symmetric strains; P independent of skew(H)
+ *   NH   notebook closed forms (test/material_tests/neohookean.ipynb): uniaxial
+ *        stretch and simple shear Cauchy stress tau = lambda ln J I + mu (B - I);
+ *        rotation invariance P(RF) = R P(F); small-strain limit -> LE
+ *   J2   1-element uniaxial radial return with hardening, analytic plastic strain
+ *   SVK  PatchTestFixture.h:138-169 closed form P = F S with E = 1, nu = 0
+ */
+#include <math.h>
+
+#include "oracle.h"
+
+void or_stress(int kind, double lambda, double mu, double yield_stress, double hardening,
+               const double H[9], const double *state_n, double *state_np1, double P[9]) {
+    if (kind == OR_LINEAR_ELASTIC) {
+        const double two_mu = mu * 2.0;
+        double eps[9];
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) eps[3 * i + j] = 0.5 * (H[3 * i + j] + H[3 * j + i]);
+        const double tr = eps[0] + eps[4] + eps[8];
+        for (int i = 0; i < 9; ++i) P[i] = two_mu * eps[i];
+        for (int i = 0; i < 3; ++i) P[4 * i] = lambda * tr + two_mu * eps[4 * i];
+    } else if (kind == OR_NEO_HOOKEAN) {
+        const double m = (mu * 2.0) / 2.0;
+        double BmI[9], F[9], Finv[9], tau[9];
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) {
+                double hh = 0.0;
+                for (int k = 0; k < 3; ++k) hh += H[3 * i + k] * H[3 * j + k];
+                BmI[3 * i + j] = hh + H[3 * j + i] + H[3 * i + j];
+            }
+        const double Jm1 = or_det_ap_im1(H);
+        const double lnJ = log1p(Jm1);
+        for (int i = 0; i < 9; ++i) tau[i] = m * BmI[i];
+        for (int i = 0; i < 3; ++i) tau[4 * i] += lambda * lnJ;
+        for (int i = 0; i < 9; ++i) F[i] = H[i];
+        F[0] += 1.0; F[4] += 1.0; F[8] += 1.0;
+        or_fullpivlu_inverse(3, F, Finv);
+        /* P = tau * Finv^T  ->  P_ij = sum_k tau_ik Finv_jk */
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) {
+                double s = 0.0;
+                for (int k = 0; k < 3; ++k) s += tau[3 * i + k] * Finv[3 * j + k];
+                P[3 * i + j] = s;
+            }
+    } else if (kind == OR_J2) {
+        const double m = (mu * 2.0) / 2.0;
+        double ep = state_n ? state_n[0] : 0.0;
+        double eps[9], dev[9];
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) eps[3 * i + j] = 0.5 * (H[3 * i + j] + H[3 * j + i]);
+        const double tr = eps[0] + eps[4] + eps[8];
+        const double K = lambda + 2.0 * m / 3.0;
+        const double p = K * tr;
+        for (int i = 0; i < 9; ++i) dev[i] = eps[i];
+        for (int i = 0; i < 3; ++i) dev[4 * i] -= (1.0 / 3.0) * tr;
+        for (int i = 0; i < 9; ++i) dev[i] *= 2.0 * m;
+        double nrm2 = 0.0;
+        for (int i = 0; i < 9; ++i) nrm2 += dev[i] * dev[i];
+        const double eta_norm = sqrt(nrm2);
+        const double q = sqrt(3.0 / 2.0) * eta_norm;
+        const double phi = q - (yield_stress + hardening * ep);
+        if (phi > 0.0) {
+            const double dep = phi / (3.0 * m + hardening);
+            const double s = sqrt(6.0) * m * dep / eta_norm;
+            for (int i = 0; i < 9; ++i) dev[i] -= s * dev[i];
+            ep += dep;
+        }
+        if (state_np1) state_np1[0] = ep;
+        for (int i = 0; i < 9; ++i) P[i] = dev[i];
+        for (int i = 0; i < 3; ++i) P[4 * i] += p;
+    } else { /* OR_SVK */
+        const double two_mu = mu * 2.0;
+        double E[9], S[9];
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) {
+                double hth = 0.0;
+                for (int k = 0; k < 3; ++k) hth += H[3 * k + i] * H[3 * k + j];
+                E[3 * i + j] = 0.5 * (H[3 * i + j] + H[3 * j + i] + hth);
+            }
+        const double tr = E[0] + E[4] + E[8];
+        for (int i = 0; i < 9; ++i) S[i] = two_mu * E[i];
+        for (int i = 0; i < 3; ++i) S[4 * i] += lambda * tr;
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) {
+                double s = 0.0;
+                for (int k = 0; k < 3; ++k) s += ((i == k ? 1.0 : 0.0) + H[3 * i + k]) * S[3 * k + j];
+                P[3 * i + j] = s;
+            }
+    }
+}

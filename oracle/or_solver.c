@@ -1,0 +1,153 @@
+/*
+ * or_solver.c — lumped mass and the explicit central-difference loop.
+ * TEST INFRASTRUCTURE (oracle).
+ *
+ * Restates:
+ *   lumped mass        src/MassUtils.cpp:26-39 (m_I += rho V_e / n_nodes), :79-112
+ *   solver loop        src/Solvers/Explicit.cpp:308-452 (initial force + a, then per step:
+ *                      v1/2 = v + dt/2 a; velocity BC; u += dt v1/2; f(u); a = f/m;
+ *                      acceleration BC; v = v1/2 + dt/2 a)
+ *   node functors      include/ExplicitTimeIntegrator.h:116-126, :267-276, :308-318, :348-357
+ *   N/NP1 state swap   Explicit.cpp:393 (UpdateFieldStates) for the element state field
+ *
+ * Pinning (tests/test_oracle_solver.py): sum m = rho V (MassUtilsTest.cpp:202-265);
+ * free flight (zero stiffness path: rigid translation gives f = 0 and u = v0 t exactly);
+ * a one-element spring-mass vibration against the closed-form central-difference
+ * recurrence; linear-elastic energy conservation to O(dt^2) over 100 steps.
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "oracle.h"
+
+int or_lumped_mass(int etype, int64_t n_nodes, const double *X, int64_t n_elem, const int32_t *conn,
+                   double density, const double *elem_density, double *mass) {
+    const int npe = etype == OR_TET4 ? 4 : 8;
+    double *vol = (double *)malloc(sizeof(double) * (size_t)(n_elem > 0 ? n_elem : 1));
+    if (!vol) return -1;
+    or_element_volumes(etype, n_elem, conn, X, vol);
+    memset(mass, 0, sizeof(double) * n_nodes);
+    for (int64_t e = 0; e < n_elem; ++e) {
+        const double rho = elem_density ? elem_density[e] : density;
+        const double m = vol[e] * rho;
+        const double mn = m / npe;
+        for (int k = 0; k < npe; ++k) mass[conn[e * npe + k]] += mn;
+    }
+    free(vol);
+    return 0;
+}
+
+static int compute_force(const or_problem *p, const double *u, const double *sn, double *snp1, double *H,
+                         double *P, double *f, int nthreads) {
+    if (p->etype == -1)
+        return or_cell_internal_force(p->n_nodes, p->n_elem, p->cell_start, p->cell_nodes, p->cell_dphi,
+                                      p->cell_volume, &p->mat, u, sn, snp1, H, P, f, nthreads);
+    return or_internal_force(p->etype, p->n_nodes, p->X, p->n_elem, p->conn, &p->mat, u, sn, snp1, H, P, f,
+                             nthreads);
+}
+
+int or_solve(const or_problem *p, double dt, int n_steps, int do_initial, double *u, double *v, double *a,
+             double *f, double *H, double *P, double *state_a, double *state_b, int *state_out_is_b,
+             int nthreads) {
+    const int64_t N = p->n_nodes;
+    const int nt = nthreads > 0 ? nthreads : 1;
+    double *sn = state_a, *snp1 = state_b;
+    int rc;
+    if (do_initial) {
+        /* Explicit.cpp:363-376: force and acceleration at the initial state (written to NP1) */
+        rc = compute_force(p, u, snp1, snp1, H, P, f, nt);
+        if (rc) return rc;
+#pragma omp parallel for schedule(static) num_threads(nt)
+        for (int64_t i = 0; i < N; ++i)
+            for (int d = 0; d < 3; ++d) a[3 * i + d] = f[3 * i + d] / p->mass[i];
+    }
+    for (int s = 0; s < n_steps; ++s) {
+        /* UpdateFieldStates: N <- NP1 */
+        double *t = sn; sn = snp1; snp1 = t;
+#pragma omp parallel for schedule(static) num_threads(nt)
+        for (int64_t i = 0; i < N; ++i) {
+            const double dti = p->node_dt ? p->node_dt[i] : dt;
+            const double h = 0.5 * dti;
+            for (int d = 0; d < 3; ++d) v[3 * i + d] = v[3 * i + d] + h * a[3 * i + d];
+        }
+        if (p->fixed)
+            for (int64_t i = 0; i < 3 * N; ++i)
+                if (p->fixed[i]) v[i] = 0.0;
+#pragma omp parallel for schedule(static) num_threads(nt)
+        for (int64_t i = 0; i < N; ++i) {
+            const double dti = p->node_dt ? p->node_dt[i] : dt;
+            for (int d = 0; d < 3; ++d) u[3 * i + d] = u[3 * i + d] + dti * v[3 * i + d];
+        }
+        rc = compute_force(p, u, sn, snp1, H, P, f, nt);
+        if (rc) return rc;
+#pragma omp parallel for schedule(static) num_threads(nt)
+        for (int64_t i = 0; i < N; ++i)
+            for (int d = 0; d < 3; ++d) a[3 * i + d] = f[3 * i + d] / p->mass[i];
+        if (p->fixed)
+            for (int64_t i = 0; i < 3 * N; ++i)
+                if (p->fixed[i]) a[i] = 0.0;
+#pragma omp parallel for schedule(static) num_threads(nt)
+        for (int64_t i = 0; i < N; ++i) {
+            const double dti = p->node_dt ? p->node_dt[i] : dt;
+            const double h = 0.5 * dti;
+            for (int d = 0; d < 3; ++d) v[3 * i + d] += h * a[3 * i + d];
+        }
+    }
+    if (state_out_is_b) *state_out_is_b = (snp1 == state_b);
+    return 0;
+}
+
+static uint64_t splitmix64(uint64_t x) {
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+
+/*
+ * Stable time increment by power iteration on M^-1 K, with K applied as the
+ * finite difference of the internal force (include/PowerMethodProcessor.h:
+ * 331-415): v <- -(f(u + eps v) - f(u)) / (eps m), lambda = |v|, dt = 2/sqrt(lambda).
+ * Runs n_iter iterations from a seeded start vector; fixed DOFs are zeroed.
+ */
+double or_power_method_dt(const or_problem *p, int n_iter, double epsilon, uint64_t seed, int nthreads) {
+    const int64_t N = p->n_nodes, M = 3 * N;
+    double *vec = (double *)calloc((size_t)M, sizeof(double));
+    double *up = (double *)calloc((size_t)M, sizeof(double));
+    double *f0 = (double *)calloc((size_t)M, sizeof(double));
+    double *f1 = (double *)calloc((size_t)M, sizeof(double));
+    const int64_t ns = p->mat.kind == OR_J2 ? (p->etype == OR_HEX8 && p->mat.state_per_qp ? 8 : 1) * p->n_elem : 1;
+    double *st = (double *)calloc((size_t)ns, sizeof(double));
+    double *st2 = (double *)calloc((size_t)ns, sizeof(double));
+    double lambda = 0.0;
+    if (!vec || !up || !f0 || !f1 || !st || !st2) goto done;
+    for (int64_t i = 0; i < M; ++i) {
+        vec[i] = (double)(splitmix64(seed ^ (uint64_t)i) >> 11) * (1.0 / 9007199254740992.0) - 0.5;
+        if (p->fixed && p->fixed[i]) vec[i] = 0.0;
+    }
+    {
+        double nrm = 0.0;
+        for (int64_t i = 0; i < M; ++i) nrm += vec[i] * vec[i];
+        nrm = sqrt(nrm);
+        for (int64_t i = 0; i < M; ++i) vec[i] /= nrm;
+    }
+    compute_force(p, up, st, st2, NULL, NULL, f0, nthreads);
+    for (int it = 0; it < n_iter; ++it) {
+        for (int64_t i = 0; i < M; ++i) up[i] = epsilon * vec[i];
+        compute_force(p, up, st, st2, NULL, NULL, f1, nthreads);
+        double nrm = 0.0;
+        for (int64_t i = 0; i < M; ++i) {
+            double w = -(f1[i] - f0[i]) / (epsilon * p->mass[i / 3]);
+            if (p->fixed && p->fixed[i]) w = 0.0;
+            vec[i] = w;
+            nrm += w * w;
+        }
+        nrm = sqrt(nrm);
+        lambda = nrm;
+        for (int64_t i = 0; i < M; ++i) vec[i] /= nrm;
+    }
+done:
+    free(vec); free(up); free(f0); free(f1); free(st); free(st2);
+    return lambda > 0.0 ? 2.0 / sqrt(lambda) : 0.0;
+}

@@ -1,0 +1,124 @@
+/*
+ * oracle.h — CPU restatement of aperi-mech's explicit internal-force step.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the parity checker for the HIP
+ * implementation in aperi-mech_amd/.  Only tests/, __graft_entry__.smoke() and
+ * bench.py's cpu_baseline leg may load it.  The product never links it.
+ *
+ * Conventions follow the reference's field layout (include/Field.h:129-138):
+ *   node vectors   n_nodes x 3, row-major (x,y,z per node)
+ *   tensors        n_elem x 9, row-major (component 3*i+j)
+ *   connectivity   n_elem x nodes_per_elem
+ *
+ * Pinning status of each function (what fixes it independent of this code)
+ * is listed in the header comment of the .c file that defines it.  Functions
+ * whose values are pinned only at rounding level (never bit level):
+ *   - Eigen 3x3 inverse / FullPivLU (third party; tolerance-level parity)
+ *   - summation order of the reference's atomic scatters
+ */
+#ifndef APERI_ORACLE_H
+#define APERI_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { OR_TET4 = 0, OR_HEX8 = 1 };
+enum { OR_LINEAR_ELASTIC = 0, OR_NEO_HOOKEAN = 1, OR_J2 = 2, OR_SVK = 3 };
+
+/* Material: uniform parameters, optionally overridden per element. */
+typedef struct {
+    int32_t kind;
+    int32_t state_per_qp;      /* Hex8 + J2 only: 1 = one state slot per Gauss point (extension) */
+    double lambda, mu;         /* Lame parameters */
+    double yield_stress, hardening_modulus;
+    const double *elem_lambda; /* optional [n_elem] */
+    const double *elem_mu;     /* optional [n_elem] */
+} or_material;
+
+/* ---- math (or_math.c) ---- */
+int or_lame_from_E_nu(double E, double nu, double *lambda, double *mu, double *K);
+double or_det_ap_im1(const double A[9]);
+double or_kernel_value(const double d[3], double R);
+void or_inverse3_cofactor(const double A[9], double Ainv[9], double *det);
+int or_fullpivlu_inverse(int n, const double *A, double *Ainv);
+int or_elastic_properties(int given_a, double a, int given_b, double b, double out[5]);
+
+/* ---- materials (or_material.c) ---- */
+void or_stress(int kind, double lambda, double mu, double yield_stress, double hardening,
+               const double H[9], const double *state_n, double *state_np1, double P[9]);
+
+/* ---- element kinematics (or_element.c) ---- */
+void or_tet4_bmatrix(const double X[12], double B[12], double *w);
+void or_hex8_bmatrix(const double X[24], int gauss_id, double B[24], double *w);
+void or_smoothed_tet4_bmatrix(const double X[12], double B[12], double *V);
+void or_smoothed_hex8_bmatrix(const double X[24], double B[24], double *V);
+void or_element_volumes(int etype, int64_t n_elem, const int32_t *conn, const double *X, double *vol);
+
+/* Gaussian internal force over all elements.  f is ZEROED then accumulated.
+ * state arrays may be NULL for stateless materials; for J2 they hold one value
+ * per element (or 8 per element with state_per_qp).  H, P may be NULL. */
+int or_internal_force(int etype, int64_t n_nodes, const double *X, int64_t n_elem, const int32_t *conn,
+                      const or_material *mat, const double *u, const double *state_n, double *state_np1,
+                      double *H, double *P, double *f, int nthreads);
+
+/* ---- mass and time integration (or_solver.c) ---- */
+int or_lumped_mass(int etype, int64_t n_nodes, const double *X, int64_t n_elem, const int32_t *conn,
+                   double density, const double *elem_density, double *mass);
+
+/* Problem description for the explicit solver loop (src/Solvers/Explicit.cpp:383-452). */
+typedef struct {
+    int32_t etype;        /* OR_TET4 / OR_HEX8, or -1 for RK smoothed cells */
+    int64_t n_nodes;
+    const double *X;
+    int64_t n_elem;
+    const int32_t *conn;
+    or_material mat;
+    const double *mass;            /* [n_nodes] one scalar per node */
+    const uint8_t *fixed;          /* [n_nodes*3] 1 = velocity/accel held at 0, or NULL */
+    const double *node_dt;         /* optional [n_nodes] per-node time increment (ensembles) */
+    /* RK smoothed cells (etype == -1) */
+    const int64_t *cell_start;     /* [n_elem+1] */
+    const int32_t *cell_nodes;     /* [nnz] */
+    const double *cell_dphi;       /* [nnz*3] */
+    const double *cell_volume;     /* [n_elem] */
+} or_problem;
+
+/* Initial force and acceleration (Explicit.cpp:363-376), then n_steps of the
+ * central-difference loop.  u, v are inout; a, f, H, P outputs; state_a/state_b
+ * are the N/NP1 state buffers (swapped each step); on return *state_out_is_b
+ * tells which buffer holds the final NP1 state.  dt is the fixed increment
+ * (DirectTimeStepper); with node_dt set, node_dt overrides it per node. */
+int or_solve(const or_problem *p, double dt, int n_steps, int do_initial,
+             double *u, double *v, double *a, double *f, double *H, double *P,
+             double *state_a, double *state_b, int *state_out_is_b, int nthreads);
+
+/* Power-method stable time increment (include/PowerMethodProcessor.h:331-415). */
+double or_power_method_dt(const or_problem *p, int n_iter, double epsilon, uint64_t seed, int nthreads);
+
+/* ---- RKPM (or_rk.c) ---- */
+int or_max_edge_length(int64_t n_nodes, const double *X, int64_t n_elem, const int32_t *conn, int npe,
+                       const uint8_t *active, double *max_edge);
+int or_neighbor_search(int64_t n_nodes, const double *X, const double *radius, int max_neighbors,
+                       int brute_force, int32_t *num_neighbors, int32_t *neighbors, int64_t *n_truncated);
+int or_rk_function_values(int64_t n_nodes, const double *X, const double *radius, int max_neighbors,
+                          const int32_t *num_neighbors, const int32_t *neighbors, double *phi);
+int64_t or_rk_cells_count(int64_t n_elem, const int32_t *conn, int max_neighbors,
+                          const int32_t *num_neighbors, const int32_t *neighbors, int64_t *cell_start);
+int or_rk_cells_build(int64_t n_nodes, const double *X, int64_t n_elem, const int32_t *conn, int max_neighbors,
+                      const int32_t *num_neighbors, const int32_t *neighbors, const double *phi,
+                      const int64_t *cell_start, int32_t *cell_nodes, double *cell_dphi, double *cell_volume);
+int or_cell_internal_force(int64_t n_nodes, int64_t n_cells, const int64_t *cell_start, const int32_t *cell_nodes,
+                           const double *cell_dphi, const double *cell_volume, const or_material *mat,
+                           const double *u, const double *state_n, double *state_np1,
+                           double *H, double *P, double *f, int nthreads);
+int or_rk_mass(int64_t n_nodes, int64_t n_elem, const int32_t *conn, const double *elem_volume, double density,
+               int max_neighbors, const int32_t *num_neighbors, const int32_t *neighbors, const double *phi,
+               double *mass);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

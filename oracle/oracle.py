@@ -1,0 +1,340 @@
+"""ctypes front end of the CPU oracle (liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py.  The product (aperi-mech_amd/) never imports it.
+Every function here is a thin marshalling layer over oracle/*.c, which cites
+the reference file:line it restates.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+TET4, HEX8 = 0, 1
+LINEAR_ELASTIC, NEO_HOOKEAN, J2, SVK = 0, 1, 2, 3
+MAX_NODE_NUM_NEIGHBORS = 40  # include/Constants.h:15
+
+_d = ctypes.POINTER(ctypes.c_double)
+_i32 = ctypes.POINTER(ctypes.c_int32)
+_i64 = ctypes.POINTER(ctypes.c_int64)
+_u8 = ctypes.POINTER(ctypes.c_uint8)
+
+
+class Material(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int32), ("state_per_qp", ctypes.c_int32),
+                ("lam", ctypes.c_double), ("mu", ctypes.c_double),
+                ("yield_stress", ctypes.c_double), ("hardening_modulus", ctypes.c_double),
+                ("elem_lambda", _d), ("elem_mu", _d)]
+
+
+class Problem(ctypes.Structure):
+    _fields_ = [("etype", ctypes.c_int32), ("n_nodes", ctypes.c_int64), ("X", _d),
+                ("n_elem", ctypes.c_int64), ("conn", _i32), ("mat", Material),
+                ("mass", _d), ("fixed", _u8), ("node_dt", _d),
+                ("cell_start", _i64), ("cell_nodes", _i32), ("cell_dphi", _d), ("cell_volume", _d)]
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            build()
+        L = ctypes.CDLL(path)
+        L.or_det_ap_im1.restype = ctypes.c_double
+        L.or_kernel_value.restype = ctypes.c_double
+        L.or_power_method_dt.restype = ctypes.c_double
+        L.or_rk_cells_count.restype = ctypes.c_int64
+        _LIB = L
+    return _LIB
+
+
+def _p(a, t=_d):
+    return None if a is None else a.ctypes.data_as(t)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _i32a(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def lame(E, nu):
+    """(lambda, mu) from Young's modulus and Poisson's ratio (Materials/Base.h:97-107)."""
+    lam, mu, K = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+    if lib().or_lame_from_E_nu(ctypes.c_double(E), ctypes.c_double(nu), ctypes.byref(lam), ctypes.byref(mu),
+                               ctypes.byref(K)):
+        raise ValueError("invalid E/nu")
+    return lam.value, mu.value
+
+
+def elastic_properties(code_a, a, code_b, b):
+    out = np.zeros(5)
+    rc = lib().or_elastic_properties(code_a, ctypes.c_double(a), code_b, ctypes.c_double(b), _p(out))
+    if rc:
+        raise ValueError("invalid property pair or bounds")
+    return dict(zip(["E", "nu", "G", "lambda", "K"], out))
+
+
+def det_ap_im1(A):
+    return lib().or_det_ap_im1(_p(_f64(A).reshape(9)))
+
+
+def kernel_value(d, R):
+    return lib().or_kernel_value(_p(_f64(d)), ctypes.c_double(R))
+
+
+def inverse3_cofactor(A):
+    out = np.zeros(9)
+    det = ctypes.c_double()
+    lib().or_inverse3_cofactor(_p(_f64(A).reshape(9)), _p(out), ctypes.byref(det))
+    return out.reshape(3, 3), det.value
+
+
+def fullpivlu_inverse(A):
+    A = _f64(A)
+    n = A.shape[0]
+    out = np.zeros((n, n))
+    rank = lib().or_fullpivlu_inverse(n, _p(A), _p(out))
+    return out, rank
+
+
+def stress(kind, lam, mu, H, state_n=0.0, yield_stress=0.0, hardening=0.0):
+    H = _f64(H).reshape(9)
+    P = np.zeros(9)
+    sn = np.array([state_n], dtype=np.float64)
+    snp1 = np.zeros(1)
+    lib().or_stress(kind, ctypes.c_double(lam), ctypes.c_double(mu), ctypes.c_double(yield_stress),
+                    ctypes.c_double(hardening), _p(H), _p(sn), _p(snp1), _p(P))
+    return P.reshape(3, 3), snp1[0]
+
+
+def tet4_bmatrix(X):
+    B = np.zeros(12)
+    w = ctypes.c_double()
+    lib().or_tet4_bmatrix(_p(_f64(X).reshape(12)), _p(B), ctypes.byref(w))
+    return B.reshape(4, 3), w.value
+
+
+def hex8_bmatrix(X, gauss_id):
+    B = np.zeros(24)
+    w = ctypes.c_double()
+    lib().or_hex8_bmatrix(_p(_f64(X).reshape(24)), gauss_id, _p(B), ctypes.byref(w))
+    return B.reshape(8, 3), w.value
+
+
+def smoothed_tet4_bmatrix(X):
+    B = np.zeros(12)
+    V = ctypes.c_double()
+    lib().or_smoothed_tet4_bmatrix(_p(_f64(X).reshape(12)), _p(B), ctypes.byref(V))
+    return B.reshape(4, 3), V.value
+
+
+def smoothed_hex8_bmatrix(X):
+    B = np.zeros(24)
+    V = ctypes.c_double()
+    lib().or_smoothed_hex8_bmatrix(_p(_f64(X).reshape(24)), _p(B), ctypes.byref(V))
+    return B.reshape(8, 3), V.value
+
+
+def _material(kind, lam, mu, yield_stress=0.0, hardening=0.0, elem_lambda=None, elem_mu=None, state_per_qp=0):
+    m = Material()
+    m.kind, m.state_per_qp = kind, state_per_qp
+    m.lam, m.mu, m.yield_stress, m.hardening_modulus = lam, mu, yield_stress, hardening
+    m.elem_lambda = _p(elem_lambda)
+    m.elem_mu = _p(elem_mu)
+    return m
+
+
+def state_size(etype, kind, n_elem, state_per_qp=0):
+    if kind != J2:
+        return 1
+    return n_elem * (8 if (etype == HEX8 and state_per_qp) else 1)
+
+
+def internal_force(etype, X, conn, kind, lam, mu, u, state_n=None, yield_stress=0.0, hardening=0.0,
+                   elem_lambda=None, elem_mu=None, state_per_qp=0, nthreads=1):
+    """f = -sum_e int B^T P dV, with H, P (last Gauss point) and the new state."""
+    X, conn, u = _f64(X), _i32a(conn), _f64(u)
+    ne, nn = conn.shape[0], X.shape[0]
+    el = None if elem_lambda is None else _f64(elem_lambda)
+    em = None if elem_mu is None else _f64(elem_mu)
+    mat = _material(kind, lam, mu, yield_stress, hardening, el, em, state_per_qp)
+    ns = state_size(etype, kind, ne, state_per_qp)
+    sn = np.zeros(ns) if state_n is None else _f64(state_n).copy()
+    snp1 = np.zeros(ns)
+    H = np.zeros((ne, 9))
+    P = np.zeros((ne, 9))
+    f = np.zeros((nn, 3))
+    rc = lib().or_internal_force(etype, nn, _p(X), ne, _p(conn, _i32), ctypes.byref(mat), _p(u), _p(sn), _p(snp1),
+                                 _p(H), _p(P), _p(f), nthreads)
+    if rc:
+        raise RuntimeError("or_internal_force failed")
+    return f, H, P, snp1
+
+
+def element_volumes(etype, X, conn):
+    X, conn = _f64(X), _i32a(conn)
+    vol = np.zeros(conn.shape[0])
+    lib().or_element_volumes(etype, conn.shape[0], _p(conn, _i32), _p(X), _p(vol))
+    return vol
+
+
+def lumped_mass(etype, X, conn, density, elem_density=None):
+    X, conn = _f64(X), _i32a(conn)
+    m = np.zeros(X.shape[0])
+    ed = None if elem_density is None else _f64(elem_density)
+    lib().or_lumped_mass(etype, X.shape[0], _p(X), conn.shape[0], _p(conn, _i32), ctypes.c_double(density),
+                         _p(ed), _p(m))
+    return m
+
+
+class Solver:
+    """The explicit loop of src/Solvers/Explicit.cpp:308-452 on the CPU oracle."""
+
+    def __init__(self, etype, X, conn, kind, lam, mu, mass, fixed=None, yield_stress=0.0, hardening=0.0,
+                 elem_lambda=None, elem_mu=None, node_dt=None, state_per_qp=0, cells=None):
+        self.X = _f64(X)
+        self.conn = None if conn is None else _i32a(conn)
+        self.mass = _f64(mass)
+        self.fixed = None if fixed is None else np.ascontiguousarray(fixed, dtype=np.uint8)
+        self.el = None if elem_lambda is None else _f64(elem_lambda)
+        self.em = None if elem_mu is None else _f64(elem_mu)
+        self.node_dt = None if node_dt is None else _f64(node_dt)
+        self.nn = self.X.shape[0]
+        p = Problem()
+        p.etype = etype
+        p.n_nodes = self.nn
+        p.X = _p(self.X)
+        if cells is not None:
+            self.cs, self.cn, self.cd, self.cv = (np.ascontiguousarray(cells[0], dtype=np.int64), _i32a(cells[1]),
+                                                  _f64(cells[2]), _f64(cells[3]))
+            p.etype = -1
+            p.n_elem = self.cv.shape[0]
+            p.cell_start, p.cell_nodes = _p(self.cs, _i64), _p(self.cn, _i32)
+            p.cell_dphi, p.cell_volume = _p(self.cd), _p(self.cv)
+            p.conn = None
+        else:
+            p.n_elem = self.conn.shape[0]
+            p.conn = _p(self.conn, _i32)
+        p.mat = _material(kind, lam, mu, yield_stress, hardening, self.el, self.em, state_per_qp)
+        p.mass = _p(self.mass)
+        p.fixed = _p(self.fixed, _u8)
+        p.node_dt = _p(self.node_dt)
+        self.p = p
+        self.ne = p.n_elem
+        ns = state_size(etype, kind, self.ne, state_per_qp)
+        self.sa, self.sb = np.zeros(ns), np.zeros(ns)
+        self.u = np.zeros((self.nn, 3))
+        self.v = np.zeros((self.nn, 3))
+        self.a = np.zeros((self.nn, 3))
+        self.f = np.zeros((self.nn, 3))
+        self.H = np.zeros((self.ne, 9))
+        self.P = np.zeros((self.ne, 9))
+        self.state = self.sb
+        self.initialized = False
+
+    def run(self, dt, n_steps, nthreads=1):
+        flag = ctypes.c_int()
+        rc = lib().or_solve(ctypes.byref(self.p), ctypes.c_double(dt), n_steps, 0 if self.initialized else 1,
+                            _p(self.u), _p(self.v), _p(self.a), _p(self.f), _p(self.H), _p(self.P),
+                            _p(self.sa), _p(self.sb), ctypes.byref(flag), nthreads)
+        if rc:
+            raise RuntimeError("or_solve failed")
+        self.initialized = True
+        if flag.value == 0:  # keep "sb" as the NP1 buffer across calls
+            self.sa, self.sb = self.sb, self.sa
+        self.state = self.sb
+        return self
+
+    def power_method_dt(self, n_iter=50, epsilon=1e-5, seed=20261202, nthreads=1):
+        return lib().or_power_method_dt(ctypes.byref(self.p), n_iter, ctypes.c_double(epsilon),
+                                        ctypes.c_uint64(seed), nthreads)
+
+
+# ---------------- RKPM ----------------
+
+def max_edge_length(X, conn, active=None):
+    X, conn = _f64(X), _i32a(conn)
+    out = np.zeros(X.shape[0])
+    act = None if active is None else np.ascontiguousarray(active, dtype=np.uint8)
+    lib().or_max_edge_length(X.shape[0], _p(X), conn.shape[0], _p(conn, _i32), conn.shape[1], _p(act, _u8), _p(out))
+    return out
+
+
+def kernel_radius(max_edge, scale):
+    """src/NeighborSearchProcessor.cpp:154-185: r = scale * l, scale 1.0 -> 1.0 + 1e-6."""
+    if scale < 0:
+        raise ValueError("scale must be non-negative")
+    if scale == 1.0:
+        scale += 1.0e-6
+    return np.asarray(max_edge) * scale
+
+
+def neighbor_search(X, radius, max_neighbors=MAX_NODE_NUM_NEIGHBORS, brute_force=False):
+    X, radius = _f64(X), _f64(radius)
+    n = X.shape[0]
+    num = np.zeros(n, dtype=np.int32)
+    nbr = np.zeros((n, max_neighbors), dtype=np.int32)
+    trunc = ctypes.c_int64()
+    rc = lib().or_neighbor_search(n, _p(X), _p(radius), max_neighbors, 1 if brute_force else 0, _p(num, _i32),
+                                  _p(nbr, _i32), ctypes.byref(trunc))
+    if rc:
+        raise RuntimeError("or_neighbor_search failed")
+    return num, nbr, trunc.value
+
+
+def rk_function_values(X, radius, num, nbr):
+    X, radius = _f64(X), _f64(radius)
+    n, mx = nbr.shape
+    phi = np.zeros((n, mx))
+    lib().or_rk_function_values(n, _p(X), _p(radius), mx, _p(_i32a(num), _i32), _p(_i32a(nbr), _i32), _p(phi))
+    return phi
+
+
+def rk_cells(X, conn, num, nbr, phi):
+    X, conn, num, nbr, phi = _f64(X), _i32a(conn), _i32a(num), _i32a(nbr), _f64(phi)
+    ne, mx = conn.shape[0], nbr.shape[1]
+    start = np.zeros(ne + 1, dtype=np.int64)
+    nnz = lib().or_rk_cells_count(ne, _p(conn, _i32), mx, _p(num, _i32), _p(nbr, _i32), _p(start, _i64))
+    nodes = np.zeros(nnz, dtype=np.int32)
+    dphi = np.zeros((nnz, 3))
+    vol = np.zeros(ne)
+    rc = lib().or_rk_cells_build(X.shape[0], _p(X), ne, _p(conn, _i32), mx, _p(num, _i32), _p(nbr, _i32), _p(phi),
+                                 _p(start, _i64), _p(nodes, _i32), _p(dphi), _p(vol))
+    if rc:
+        raise RuntimeError("or_rk_cells_build failed")
+    return start, nodes, dphi, vol
+
+
+def cell_internal_force(n_nodes, cells, kind, lam, mu, u, state_n=None, yield_stress=0.0, hardening=0.0,
+                        nthreads=1):
+    start, nodes, dphi, vol = cells
+    nc = vol.shape[0]
+    mat = _material(kind, lam, mu, yield_stress, hardening)
+    sn = np.zeros(nc) if state_n is None else _f64(state_n).copy()
+    snp1 = np.zeros(nc)
+    H, P, f = np.zeros((nc, 9)), np.zeros((nc, 9)), np.zeros((n_nodes, 3))
+    lib().or_cell_internal_force(n_nodes, nc, _p(np.ascontiguousarray(start, dtype=np.int64), _i64),
+                                 _p(_i32a(nodes), _i32), _p(_f64(dphi)), _p(_f64(vol)), ctypes.byref(mat),
+                                 _p(_f64(u)), _p(sn), _p(snp1), _p(H), _p(P), _p(f), nthreads)
+    return f, H, P, snp1
+
+
+def rk_mass(conn, elem_volume, density, num, nbr, phi):
+    conn, num, nbr, phi = _i32a(conn), _i32a(num), _i32a(nbr), _f64(phi)
+    n, mx = nbr.shape
+    m = np.zeros(n)
+    lib().or_rk_mass(n, conn.shape[0], _p(conn, _i32), _p(_f64(elem_volume)), ctypes.c_double(density), mx,
+                     _p(num, _i32), _p(nbr, _i32), _p(phi), _p(m))
+    return m

@@ -1,0 +1,165 @@
+"""Pins the oracle's RKPM set-up and smoothed-cell force (oracle/or_rk.c) against
+the neighbor counts of test/unit_tests/NeighborSearchProcessorTest.cpp and the
+reproducing conditions of the RK functions."""
+from collections import Counter
+
+import numpy as np
+import pytest
+
+import oracle as O
+from aperi_hip import meshgen as M
+
+
+def generated(nx, ny, nz):
+    # "generated:NXxNYxNZ|tets": unit spacing, 6 tets per hex
+    X, hc = M.hex_lattice(nx, ny, nz, nx, ny, nz)
+    return X, M.split_tets(X, hc)
+
+
+def test_ball_search_small_and_large():
+    X, _ = generated(1, 1, 4)
+    num, nbr, tr = O.neighbor_search(X, np.full(len(X), 0.5))
+    assert (num == 1).all() and (nbr[:, 0] == np.arange(len(X))).all()
+    diag = np.sqrt(1 + 1 + 16)
+    num, nbr, tr = O.neighbor_search(X, np.full(len(X), 1.01 * diag))
+    assert (num == len(X)).all() and tr == 0
+
+
+def test_ball_search_mid_counts():
+    # NeighborSearchProcessorTest.cpp:123-158
+    X, _ = generated(2, 2, 4)
+    num, _, _ = O.neighbor_search(X, np.full(len(X), 1.01))
+    assert sorted(Counter(num.tolist()).items()) == [(4, 8), (5, 20), (6, 14), (7, 3)]
+
+
+def test_kernel_radius_counts():
+    # NeighborSearchProcessorTest.cpp:183-207
+    X, t = generated(1, 1, 4)
+    r = O.kernel_radius(O.max_edge_length(X, t), 1.01)
+    radii = Counter(np.round(r, 12).tolist())
+    assert radii == {round(np.sqrt(2) * 1.01, 12): 12, round(np.sqrt(3) * 1.01, 12): 8}
+    num, _, _ = O.neighbor_search(X, r)
+    assert sorted(Counter(num.tolist()).items()) == [(7, 4), (8, 4), (10, 6), (11, 2), (12, 4)]
+
+
+def test_variable_ball_search_stats():
+    # NeighborSearchProcessorTest.cpp:160-181: min 8, max 27, avg 18.713
+    X, t = generated(5, 5, 5)
+    num, _, _ = O.neighbor_search(X, O.kernel_radius(O.max_edge_length(X, t), 1.1))
+    assert num.min() == 8 and num.max() == 27
+    assert num.mean() == pytest.approx(18.713, abs=1e-3)
+
+
+def test_kernel_radius_scale_one_is_nudged():
+    assert O.kernel_radius(np.array([2.0]), 1.0)[0] == 2.0 * (1.0 + 1.0e-6)
+    with pytest.raises(ValueError):
+        O.kernel_radius(np.array([2.0]), -1.0)
+
+
+def test_inactive_nodes_double_edge_length():
+    X, hc = M.hex_lattice(1, 1, 1)
+    t = M.split_tets(X, hc)
+    n0, n6 = hc[0, 0], hc[0, 6]  # ends of the body diagonal every tet shares
+    active = np.ones(len(X), dtype=np.uint8)
+    active[n0] = 0
+    l_all = O.max_edge_length(X, t)
+    l_act = O.max_edge_length(X, t, active)
+    assert l_all[n6] == pytest.approx(np.sqrt(3))
+    assert l_act[n6] == pytest.approx(2 * np.sqrt(3))  # the inactive end counts double
+
+
+def jittered(n, seed=3):
+    X, hc = M.hex_lattice(n, n, n)
+    X = M.jitter(X, 1.0 / n, 0.1, seed, 1)
+    return X, M.split_tets(X, hc)
+
+
+@pytest.mark.parametrize("scale", [1.1, 1.5, 2.2])
+def test_grid_search_equals_brute_force(scale):
+    X, t = jittered(5)
+    r = O.kernel_radius(O.max_edge_length(X, t), scale)
+    a = O.neighbor_search(X, r)
+    b = O.neighbor_search(X, r, brute_force=True)
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]) and a[2] == b[2]
+
+
+def test_neighbor_cap_sorting_and_truncation():
+    X, t = jittered(6)
+    r = O.kernel_radius(O.max_edge_length(X, t), 2.2)
+    num, nbr, tr = O.neighbor_search(X, r)
+    full, fnbr, _ = O.neighbor_search(X, r, max_neighbors=400)
+    assert (num <= 40).all() and tr == int(np.maximum(full - 40, 0).sum()) and tr > 0
+    tw = np.array([1 + 1e-6, 1 + 1e-7, 1 + 1e-8])
+    for i in range(0, len(X), 7):
+        ids = fnbr[i, :full[i]]
+        d2 = ((X[ids] * tw - X[i]) ** 2).sum(1)
+        order = np.lexsort((ids, d2))
+        assert np.array_equal(nbr[i, :num[i]], ids[order][:40])  # nearest (tweaked) 40, ties by id
+        assert np.all(np.linalg.norm(X[ids] - X[i], axis=1) <= r[ids] * (1 + 1e-15))
+
+
+def _rk(n=6, scale=1.5, seed=3):
+    X, t = jittered(n, seed)
+    r = O.kernel_radius(O.max_edge_length(X, t), scale)
+    num, nbr, _ = O.neighbor_search(X, r)
+    phi = O.rk_function_values(X, r, num, nbr)
+    return X, t, r, num, nbr, phi
+
+
+def test_rk_partition_of_unity_and_linear_reproduction():
+    # ShapeFunctionsFunctorsTest.cpp:182-272 (tolerance 4e-12 * 10^(order-1))
+    X, t, r, num, nbr, phi = _rk()
+    for i in range(len(X)):
+        ids = nbr[i, :num[i]]
+        assert phi[i, :num[i]].sum() == pytest.approx(1.0, abs=4e-12)
+        assert np.allclose(phi[i, :num[i]] @ X[ids], X[i], atol=4e-12)
+        assert not phi[i, num[i]:].any()
+
+
+def test_rk_single_neighbor_and_shepard_fallback():
+    X = np.array([[0.0, 0, 0], [10.0, 0, 0]])
+    r = np.array([0.5, 0.5])
+    num, nbr, _ = O.neighbor_search(X, r)
+    phi = O.rk_function_values(X, r, num, nbr)
+    assert (num == 1).all() and (phi[:, 0] == 1.0).all()
+    # collinear neighbours: the moment matrix is rank deficient; Eigen's FullPivLU
+    # zero-fills the null space and the functions still reproduce constants
+    X = np.array([[0.0, 0, 0], [0.5, 0, 0], [1.0, 0, 0]])
+    r = np.full(3, 1.2)
+    num, nbr, _ = O.neighbor_search(X, r)
+    phi = O.rk_function_values(X, r, num, nbr)
+    assert phi[1, :num[1]].sum() == pytest.approx(1.0, abs=1e-12)
+
+
+def test_cells_partition_of_nullity_and_linear_patch():
+    # KinematicsTestFixture.h:338-357: a linear displacement gives the exact gradient
+    X, t, r, num, nbr, phi = _rk()
+    cells = O.rk_cells(X, t, num, nbr, phi)
+    start, nodes, dphi, vol = cells
+    assert vol.sum() == pytest.approx(O.element_volumes(0, X, t).sum(), rel=1e-12)
+    for c in range(0, len(vol), 13):
+        assert np.allclose(dphi[start[c]:start[c + 1]].sum(0), 0.0, atol=1e-10)
+    Hu = np.array([[0.01, 0.002, 0.0], [0.0, -0.004, 0.003], [0.001, 0.0, 0.02]])
+    u = X @ Hu.T
+    lam, mu = O.lame(70e9, 0.33)
+    f, H, P, _ = O.cell_internal_force(len(X), cells, O.LINEAR_ELASTIC, lam, mu, u)
+    assert np.allclose(H, Hu.ravel(), atol=1e-12)
+
+
+def test_cell_lists_are_first_seen_unions():
+    X, t, r, num, nbr, phi = _rk(n=4)
+    start, nodes, dphi, vol = O.rk_cells(X, t, num, nbr, phi)
+    for c in range(len(t)):
+        seen = []
+        for a in t[c]:
+            for j in nbr[a, :num[a]]:
+                if j not in seen:
+                    seen.append(j)
+        assert nodes[start[c]:start[c + 1]].tolist() == seen
+
+
+def test_rk_mass_sums_to_rho_v():
+    X, t, r, num, nbr, phi = _rk(n=4)
+    start, nodes, dphi, vol = O.rk_cells(X, t, num, nbr, phi)
+    m = O.rk_mass(t, vol, 2700.0, num, nbr, phi)
+    assert m.sum() == pytest.approx(2700.0 * vol.sum(), rel=1e-11)

@@ -1,0 +1,135 @@
+"""Pins the oracle's mass and explicit loop (oracle/or_solver.c)."""
+import math
+
+import numpy as np
+import pytest
+
+import oracle as O
+from aperi_hip import meshgen as M
+
+
+@pytest.mark.parametrize("etype", [0, 1])
+def test_total_mass_is_rho_v(etype):
+    # MassUtilsTest.cpp:202-265: sum of nodal mass = rho * volume
+    X, hc = M.hex_lattice(4, 3, 2, 2.0, 1.5, 0.5)
+    X = M.jitter(X, 0.1, 0.2, 5, 1)
+    conn = hc if etype == 1 else M.split_tets(X, hc)
+    m = O.lumped_mass(etype, X, conn, 7850.0)
+    vol = O.element_volumes(etype, X, conn).sum()
+    assert m.sum() == pytest.approx(7850.0 * vol, rel=1e-13)
+    assert (m > 0).all()
+
+
+def test_rigid_translation_free_flight():
+    X, hc = M.hex_lattice(3, 3, 3)
+    tets = M.split_tets(X, hc)
+    lam, mu = O.lame(200e9, 0.3)
+    m = O.lumped_mass(0, X, tets, 7850.0)
+    s = O.Solver(0, X, tets, O.NEO_HOOKEAN, lam, mu, m)
+    v0 = np.array([1.5, -0.5, 2.0])
+    s.v[:] = v0
+    dt = 1e-7
+    s.run(dt, 40)
+    assert np.allclose(s.u, v0 * 40 * dt, rtol=1e-9)
+    assert np.allclose(s.v, v0, rtol=1e-9)
+
+
+def test_single_dof_oscillator_matches_discrete_recurrence():
+    # One Tet4 with every DOF fixed except x of node 1; linear elasticity makes the
+    # force exactly linear, f = -k u, so velocity-Verlet gives
+    # u_{n+1} - 2 cos(W) u_n + u_{n-1} = 0 with cos W = 1 - (w dt)^2 / 2.
+    X = np.array([[0.0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]])
+    conn = np.array([[0, 1, 2, 3]])
+    lam, mu = O.lame(1.0, 0.25)
+    fixed = np.ones((4, 3), dtype=np.uint8)
+    fixed[1, 0] = 0
+    mass = np.full(4, 2.0)
+    # stiffness of the free DOF from one force evaluation
+    u = np.zeros((4, 3))
+    u[1, 0] = 1e-3
+    f, *_ = O.internal_force(0, X, conn, O.LINEAR_ELASTIC, lam, mu, u)
+    k = -f[1, 0] / 1e-3
+    w = math.sqrt(k / 2.0)
+    dt = 0.3 / w
+    s = O.Solver(0, X, conn, O.LINEAR_ELASTIC, lam, mu, mass, fixed)
+    s.v[1, 0] = 0.01
+    hist = [0.0]
+    for _ in range(60):
+        s.run(dt, 1)
+        hist.append(s.u[1, 0])
+    c = 1 - 0.5 * (w * dt) ** 2
+    W = math.acos(c)
+    # u_0 = 0, u_1 = dt v0 (a_0 = 0), closed form u_n = u_1 sin(n W) / sin(W)
+    for n, un in enumerate(hist):
+        assert un == pytest.approx(dt * 0.01 * math.sin(n * W) / math.sin(W), rel=1e-9, abs=1e-18)
+    assert np.all(s.u[[0, 2, 3]] == 0.0) and np.all(s.u[1, 1:] == 0.0)
+
+
+def test_linear_elastic_energy_is_conserved():
+    X, hc = M.hex_lattice(3, 3, 3)
+    X = M.jitter(X, 1 / 3, 0.1, 2, 1)
+    tets = M.split_tets(X, hc)
+    E, nu, rho = 200e9, 0.3, 7850.0
+    lam, mu = O.lame(E, nu)
+    m = O.lumped_mass(0, X, tets, rho)
+    dt = 0.05 * M.stable_dt_estimate(X, tets, E, nu, rho)
+    s = O.Solver(0, X, tets, O.LINEAR_ELASTIC, lam, mu, m)
+    s.v[:] = M.normal(3, 3, X.size).reshape(X.shape)
+    vol = O.element_volumes(0, X, tets)
+
+    def energy():
+        ke = 0.5 * np.sum(m[:, None] * s.v ** 2)
+        eps = s.H.reshape(-1, 3, 3)
+        eps = 0.5 * (eps + eps.transpose(0, 2, 1))
+        se = 0.5 * np.sum(vol * np.einsum("eij,eij->e", eps, s.P.reshape(-1, 3, 3)))
+        return ke + se
+
+    s.run(dt, 0)
+    e0 = energy()
+    for _ in range(10):
+        s.run(dt, 10)
+        assert energy() == pytest.approx(e0, rel=5e-3)
+
+
+def test_fixed_dofs_hold():
+    X, hc = M.hex_lattice(2, 2, 2)
+    lam, mu = O.lame(200e9, 0.3)
+    m = O.lumped_mass(1, X, hc, 7850.0)
+    fixed = np.zeros(X.shape, dtype=np.uint8)
+    fixed[X[:, 2] == 0.0] = 1
+    s = O.Solver(1, X, hc, O.NEO_HOOKEAN, lam, mu, m, fixed)
+    s.v[:] = 1.0
+    s.v[fixed.astype(bool)] = 0.0
+    s.run(1e-7, 20)
+    assert np.all(s.u[fixed.astype(bool)] == 0.0)
+    assert np.all(s.a[fixed.astype(bool)] == 0.0)
+    assert np.abs(s.u[~fixed.astype(bool)]).max() > 0
+
+
+@pytest.mark.parametrize("etype", [0, 1])
+@pytest.mark.parametrize("jit", [0.0, 0.1])
+def test_power_method_dt_bounds_the_estimate(etype, jit):
+    X, hc = M.hex_lattice(4, 4, 4)
+    X = M.jitter(X, 0.25, jit, 3, 1)
+    conn = hc if etype == 1 else M.split_tets(X, hc)
+    E, nu, rho = 200e9, 0.3, 7850.0
+    lam, mu = O.lame(E, nu)
+    m = O.lumped_mass(etype, X, conn, rho)
+    s = O.Solver(etype, X, conn, O.LINEAR_ELASTIC, lam, mu, m)
+    dt_pm = s.power_method_dt(n_iter=300)
+    dt_est = M.stable_dt_estimate(X, conn, E, nu, rho)
+    # the element estimate is conservative and within a small factor of 2/sqrt(lambda_max)
+    assert dt_est < dt_pm < 3 * dt_est
+    # single tet check of the power method itself against the dense eigenproblem
+    Xt = np.array([[0.0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]])
+    ct = np.array([[0, 1, 2, 3]])
+    mt = O.lumped_mass(0, Xt, ct, rho)
+    K = np.zeros((12, 12))
+    for j in range(12):
+        u = np.zeros(12)
+        u[j] = 1e-6
+        f, *_ = O.internal_force(0, Xt, ct, O.LINEAR_ELASTIC, lam, mu, u.reshape(4, 3))
+        K[:, j] = -f.ravel() / 1e-6
+    lmax = np.linalg.eigvalsh(K / np.repeat(mt, 3)[:, None]).max()
+    st = O.Solver(0, Xt, ct, O.LINEAR_ELASTIC, lam, mu, mt)
+    assert st.power_method_dt(n_iter=500) == pytest.approx(2 / np.sqrt(lmax), rel=1e-6)
