@@ -128,6 +128,8 @@ static void resolve_material(int kind, const aphip_material_params *p, MaterialC
     APHIP_REQUIRE(m.lambda > -(2.0 / 3.0) * m.mu, "lambda must exceed -2G/3");
     m.yield_stress = p->yield_stress;
     m.hardening = p->hardening_modulus;
+    m.E = E;
+    m.nu = nu;
     m.state_per_qp = p->state_per_qp ? 1 : 0;
 }
 
@@ -150,6 +152,142 @@ static void require_ready(Ctx &c, bool need_mass) {
     if (need_mass) APHIP_REQUIRE(c.has_mass, "mass not computed or set");
 }
 
+// node -> (element, slot) incidences in ascending internal element order (two-pass path, mass)
+static void upload_node_csr(Ctx &c, const std::vector<int32_t> &ci) {
+    const int64_t n_nodes = c.nn, n_inc = c.ne * c.npe;
+    std::vector<int32_t> start(n_nodes + 1, 0), inc((size_t)std::max<int64_t>(n_inc, 1));
+    for (int64_t t = 0; t < n_inc; ++t) start[ci[t] + 1]++;
+    for (int64_t i = 0; i < n_nodes; ++i) start[i + 1] += start[i];
+    std::vector<int32_t> fill(start.begin(), start.end() - 1);
+    for (int64_t t = 0; t < n_inc; ++t) inc[fill[ci[t]]++] = (int32_t)t;
+    c.node_start.upload(start.data(), start.size(), c.stream);
+    c.node_inc.upload(inc.data(), inc.size(), c.stream);
+    APHIP_HIP(hipStreamSynchronize(c.stream));
+}
+
+template <typename T>
+static void permute_device(Ctx &c, DevBuf<T> &b, int comps, const std::vector<int32_t> &old_of_new) {
+    if (!b.p) return;
+    const int64_t n = c.nn;
+    std::vector<T> h((size_t)n * comps), o((size_t)n * comps);
+    b.download(h.data(), h.size(), c.stream);
+    APHIP_HIP(hipStreamSynchronize(c.stream));
+    for (int64_t i = 0; i < n; ++i)
+        for (int d = 0; d < comps; ++d) o[i * comps + d] = h[(int64_t)old_of_new[i] * comps + d];
+    b.upload(o.data(), o.size(), c.stream);
+    APHIP_HIP(hipStreamSynchronize(c.stream));
+}
+
+void renumber_nodes(Ctx &c, const std::vector<int32_t> &old_of_new) {
+    const int64_t n = c.nn;
+    std::vector<int32_t> new_of_old(n);
+    for (int64_t i = 0; i < n; ++i) new_of_old[old_of_new[i]] = (int32_t)i;
+    std::vector<int32_t> iperm(n);
+    for (int64_t i = 0; i < n; ++i) iperm[i] = c.node_iperm[old_of_new[i]];
+    c.node_iperm.swap(iperm);
+    for (int64_t i = 0; i < n; ++i) c.node_perm[c.node_iperm[i]] = (int32_t)i;
+    for (auto &x : c.conn_h) x = new_of_old[x];
+    c.conn.upload(c.conn_h.data(), c.conn_h.size(), c.stream);
+    upload_node_csr(c, c.conn_h);
+    permute_device(c, c.X, 3, old_of_new);
+    permute_device(c, c.u, 3, old_of_new);
+    permute_device(c, c.v, 3, old_of_new);
+    permute_device(c, c.a, 3, old_of_new);
+    permute_device(c, c.f, 3, old_of_new);
+    if (c.has_mass) permute_device(c, c.mass, 1, old_of_new);
+    if (c.has_fixed) permute_device(c, c.fixed, 1, old_of_new);
+}
+
+static bool use_tiles(const Ctx &c) { return !c.cells.ready && c.tiles.ready && !(c.flags & APHIP_FLAG_TWO_PASS); }
+
+static void default_tiling(Ctx &c) {
+    if (c.ne == 0 || (c.flags & APHIP_FLAG_TWO_PASS)) return;
+    if (c.etype == APHIP_TET4) tile_build(c, 128, 128);
+    else tile_build(c, 32, 256);
+}
+
+
+// Mesh set-up shared by aphip_set_mesh and the ensemble builder.  node_seg / elem_seg
+// delimit blocks of caller nodes / elements (one block per specimen) that are
+// Morton-ordered independently and never share a tile.
+static void setup_mesh(Ctx &c, int32_t elem_type, int64_t n_nodes, const double *X, int64_t n_elem,
+                       const int32_t *conn, const std::vector<int64_t> &node_seg, const std::vector<int64_t> &elem_seg) {
+    APHIP_REQUIRE(elem_type == APHIP_TET4 || elem_type == APHIP_HEX8, "element type must be TET4 or HEX8");
+    APHIP_REQUIRE(n_nodes > 0 && n_nodes < (int64_t)INT32_MAX, "n_nodes out of range");
+    APHIP_REQUIRE(n_elem >= 0, "n_elem out of range");
+    APHIP_REQUIRE(X != nullptr && (n_elem == 0 || conn != nullptr), "mesh arrays are NULL");
+    const int npe = elem_type == APHIP_TET4 ? 4 : 8;
+    APHIP_REQUIRE(n_elem * npe < (int64_t)INT32_MAX, "n_elem too large for int32 incidences");
+    for (int64_t i = 0; i < n_elem * npe; ++i)
+        APHIP_REQUIRE(conn[i] >= 0 && conn[i] < n_nodes, "connectivity references a node out of range");
+    c.cells.reset();
+    c.tiles.reset();
+    c.graph_reset();
+    c.etype = elem_type;
+    c.npe = npe;
+    c.nn = n_nodes;
+    c.ne = n_elem;
+    c.elem_seg = elem_seg;
+
+    // Morton renumbering per block: nodes by position, elements by centroid
+    c.node_iperm.assign(n_nodes, 0);
+    c.elem_iperm.assign(n_elem, 0);
+    for (size_t sgm = 0; sgm + 1 < node_seg.size(); ++sgm) {
+        const int64_t n0 = node_seg[sgm], n1 = node_seg[sgm + 1];
+        std::vector<double> xs(X + 3 * n0, X + 3 * n1);
+        const std::vector<int32_t> ord = morton_order(xs, n1 - n0);
+        for (int64_t i = 0; i < n1 - n0; ++i) c.node_iperm[n0 + i] = (int32_t)(n0 + ord[i]);
+        const int64_t e0 = elem_seg[sgm], e1 = elem_seg[sgm + 1];
+        std::vector<double> cent(3 * (e1 - e0), 0.0);
+        for (int64_t e = e0; e < e1; ++e) {
+            for (int k = 0; k < npe; ++k) {
+                APHIP_REQUIRE(conn[e * npe + k] >= n0 && conn[e * npe + k] < n1, "an element references another specimen's node");
+                for (int d = 0; d < 3; ++d) cent[3 * (e - e0) + d] += X[3 * (int64_t)conn[e * npe + k] + d] / npe;
+            }
+        }
+        const std::vector<int32_t> eord = morton_order(cent, e1 - e0);
+        for (int64_t e = 0; e < e1 - e0; ++e) c.elem_iperm[e0 + e] = (int32_t)(e0 + eord[e]);
+    }
+    c.node_perm.assign(n_nodes, 0);
+    for (int64_t i = 0; i < n_nodes; ++i) c.node_perm[c.node_iperm[i]] = (int32_t)i;
+    c.elem_perm.assign(n_elem, 0);
+    for (int64_t e = 0; e < n_elem; ++e) c.elem_perm[c.elem_iperm[e]] = (int32_t)e;
+
+    std::vector<double> Xi(3 * n_nodes);
+    for (int64_t i = 0; i < n_nodes; ++i)
+        for (int d = 0; d < 3; ++d) Xi[3 * i + d] = X[3 * (int64_t)c.node_iperm[i] + d];
+    std::vector<int32_t> ci((size_t)n_elem * npe);
+    for (int64_t e = 0; e < n_elem; ++e)
+        for (int k = 0; k < npe; ++k) ci[e * npe + k] = c.node_perm[conn[(int64_t)c.elem_iperm[e] * npe + k]];
+
+    c.X.upload(Xi.data(), Xi.size(), c.stream);
+    c.conn.upload(ci.data(), ci.size(), c.stream);
+    upload_node_csr(c, ci);
+    for (DevBuf<double> *b : {&c.u, &c.v, &c.a, &c.f}) {
+        b->alloc(3 * n_nodes);
+        b->zero(c.stream);
+    }
+    c.H.alloc(9 * std::max<int64_t>(n_elem, 1));
+    c.P.alloc(9 * std::max<int64_t>(n_elem, 1));
+    c.H.zero(c.stream);
+    c.P.zero(c.stream);
+    c.partial.alloc((size_t)std::max<int64_t>(n_elem, 1) * npe * 3);
+    c.mass.release();
+    c.has_mass = false;
+    c.fixed.release();
+    c.has_fixed = false;
+    c.has_fields = false;
+    c.elem_lambda.release();
+    c.elem_mu.release();
+    c.elem_rho.release();
+    c.node_dt.release();
+    fem_volumes(c);
+    if (c.mat.kind >= 0) alloc_state(c);
+    c.conn_h.swap(ci);
+    default_tiling(c);
+    APHIP_HIP(hipStreamSynchronize(c.stream));
+}
+
 static void force_pass(Ctx &c) {
     prof_begin(c, 1);
     if (c.cells.ready) rk_cell_force(c);
@@ -157,11 +295,45 @@ static void force_pass(Ctx &c) {
     prof_end(c, 1);
 }
 
+static void step_once(Ctx &c, double dt);
+
 static void node_pass(Ctx &c, double dt, int mode) {
     prof_begin(c, 2);
     if (c.cells.ready) rk_node_update(c, dt, mode);
     else fem_node_update(c, dt, mode);
     prof_end(c, 2);
+}
+
+void step_once_tiles(Ctx &c, double dt);
+
+// one step of the explicit loop body (src/Solvers/Explicit.cpp:391-452)
+static void step_once(Ctx &c, double dt) {
+    c.state_np1 = 1 - c.state_np1;  // UpdateFieldStates (Explicit.cpp:393)
+    if (use_tiles(c)) {
+        tile_run(c, dt, 2);
+    } else {
+        fem_predict(c, dt);
+        force_pass(c);
+        node_pass(c, dt, 2);
+    }
+}
+
+void step_once_tiles(Ctx &c, double dt) { step_once(c, dt); }
+
+static void get_node_field(Ctx &c, const DevBuf<double> &b, double *out) {
+    std::vector<double> h(3 * c.nn);
+    b.download(h.data(), h.size(), c.stream);
+    APHIP_HIP(hipStreamSynchronize(c.stream));
+    for (int64_t i = 0; i < c.nn; ++i)
+        for (int d = 0; d < 3; ++d) out[3 * (int64_t)c.node_iperm[i] + d] = h[3 * i + d];
+}
+
+static void set_node_field(Ctx &c, DevBuf<double> &b, const double *in) {
+    std::vector<double> h(3 * c.nn);
+    for (int64_t i = 0; i < c.nn; ++i)
+        for (int d = 0; d < 3; ++d) h[3 * i + d] = in[3 * (int64_t)c.node_iperm[i] + d];
+    b.upload(h.data(), h.size(), c.stream);
+    APHIP_HIP(hipStreamSynchronize(c.stream));
 }
 
 }  // namespace aphip
@@ -211,6 +383,7 @@ int aphip_create(aphip_ctx **out, const aphip_config *cfg) {
     APHIP_HIP(hipSetDevice(dev));
     aphip_ctx *c = new aphip_ctx();
     c->device = dev;
+    c->flags = cfg ? cfg->flags : 0;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
         delete c;
         throw Error(APHIP_ERR_HIP, "hipStreamCreate failed");
@@ -224,6 +397,7 @@ int aphip_destroy(aphip_ctx *c) {
     if (!c) return APHIP_OK;
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
+    c->graph_reset();
     for (hipEvent_t e : c->prof.ev) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(c->stream);
     delete c;
@@ -234,71 +408,20 @@ int aphip_set_mesh(aphip_ctx *c, int32_t elem_type, int64_t n_nodes, const doubl
                    const int32_t *conn) {
     APHIP_API_BEGIN
     APHIP_REQUIRE(c != nullptr, "ctx is NULL");
-    APHIP_REQUIRE(elem_type == APHIP_TET4 || elem_type == APHIP_HEX8, "element type must be TET4 or HEX8");
-    APHIP_REQUIRE(n_nodes > 0 && n_nodes < (int64_t)INT32_MAX, "n_nodes out of range");
-    APHIP_REQUIRE(n_elem >= 0, "n_elem out of range");
-    APHIP_REQUIRE(X != nullptr && (n_elem == 0 || conn != nullptr), "mesh arrays are NULL");
-    const int npe = elem_type == APHIP_TET4 ? 4 : 8;
-    APHIP_REQUIRE(n_elem * npe < (int64_t)INT32_MAX, "n_elem too large for int32 incidences");
-    for (int64_t i = 0; i < n_elem * npe; ++i)
-        APHIP_REQUIRE(conn[i] >= 0 && conn[i] < n_nodes, "connectivity references a node out of range");
     APHIP_HIP(hipSetDevice(c->device));
-    c->cells.reset();
-    c->etype = elem_type;
-    c->npe = npe;
-    c->nn = n_nodes;
-    c->ne = n_elem;
+    c->batch = BatchBuild();
+    setup_mesh(*c, elem_type, n_nodes, X, n_elem, conn, {0, n_nodes}, {0, n_elem});
+    APHIP_API_END
+}
 
-    // Morton renumbering: nodes by position, elements by centroid
-    std::vector<double> xs(X, X + 3 * n_nodes);
-    c->node_iperm = morton_order(xs, n_nodes);
-    c->node_perm.assign(n_nodes, 0);
-    for (int64_t i = 0; i < n_nodes; ++i) c->node_perm[c->node_iperm[i]] = (int32_t)i;
-    std::vector<double> cent(3 * n_elem, 0.0);
-    for (int64_t e = 0; e < n_elem; ++e)
-        for (int k = 0; k < npe; ++k)
-            for (int d = 0; d < 3; ++d) cent[3 * e + d] += X[3 * (int64_t)conn[e * npe + k] + d] / npe;
-    c->elem_iperm = morton_order(cent, n_elem);
-    c->elem_perm.assign(n_elem, 0);
-    for (int64_t e = 0; e < n_elem; ++e) c->elem_perm[c->elem_iperm[e]] = (int32_t)e;
-
-    std::vector<double> Xi(3 * n_nodes);
-    for (int64_t i = 0; i < n_nodes; ++i)
-        for (int d = 0; d < 3; ++d) Xi[3 * i + d] = X[3 * (int64_t)c->node_iperm[i] + d];
-    std::vector<int32_t> ci((size_t)n_elem * npe);
-    for (int64_t e = 0; e < n_elem; ++e)
-        for (int k = 0; k < npe; ++k) ci[e * npe + k] = c->node_perm[conn[(int64_t)c->elem_iperm[e] * npe + k]];
-
-    // node -> (element, slot) incidences in ascending internal element order
-    std::vector<int32_t> start(n_nodes + 1, 0), inc((size_t)n_elem * npe);
-    for (int64_t t = 0; t < n_elem * npe; ++t) start[ci[t] + 1]++;
-    for (int64_t i = 0; i < n_nodes; ++i) start[i + 1] += start[i];
-    std::vector<int32_t> fill(start.begin(), start.end() - 1);
-    for (int64_t t = 0; t < n_elem * npe; ++t) inc[fill[ci[t]]++] = (int32_t)t;
-
-    c->X.upload(Xi.data(), Xi.size(), c->stream);
-    c->conn.upload(ci.data(), ci.size(), c->stream);
-    c->node_start.upload(start.data(), start.size(), c->stream);
-    c->node_inc.upload(inc.data(), inc.size(), c->stream);
-    for (DevBuf<double> *b : {&c->u, &c->v, &c->a, &c->f}) {
-        b->alloc(3 * n_nodes);
-        b->zero(c->stream);
-    }
-    c->H.alloc(9 * std::max<int64_t>(n_elem, 1));
-    c->P.alloc(9 * std::max<int64_t>(n_elem, 1));
-    c->H.zero(c->stream);
-    c->P.zero(c->stream);
-    c->partial.alloc((size_t)std::max<int64_t>(n_elem, 1) * npe * 3);
-    c->mass.release();
-    c->has_mass = false;
-    c->fixed.release();
-    c->has_fixed = false;
-    c->has_fields = false;
-    c->elem_lambda.release();
-    c->elem_mu.release();
-    fem_volumes(*c);
-    if (c->mat.kind >= 0) alloc_state(*c);
+int aphip_set_tiling(aphip_ctx *c, int32_t te, int32_t threads) {
+    APHIP_API_BEGIN
+    APHIP_REQUIRE(c != nullptr, "ctx is NULL");
+    APHIP_REQUIRE(c->etype >= 0, "mesh not set");
+    APHIP_HIP(hipSetDevice(c->device));
     APHIP_HIP(hipStreamSynchronize(c->stream));
+    c->graph_reset();
+    tile_build(*c, te, threads);
     APHIP_API_END
 }
 
@@ -316,6 +439,8 @@ int aphip_set_material(aphip_ctx *c, int32_t kind, const aphip_material_params *
     MaterialConst m;
     resolve_material(kind, p, m);
     APHIP_HIP(hipSetDevice(c->device));
+    APHIP_HIP(hipStreamSynchronize(c->stream));
+    c->graph_reset();
     c->mat = m;
     if (c->etype >= 0) alloc_state(*c);
     APHIP_HIP(hipStreamSynchronize(c->stream));
@@ -327,20 +452,28 @@ int aphip_set_material_fields(aphip_ctx *c, const double *E, const double *nu) {
     APHIP_REQUIRE(c != nullptr, "ctx is NULL");
     APHIP_REQUIRE(c->etype >= 0, "mesh not set");
     APHIP_HIP(hipSetDevice(c->device));
+    APHIP_HIP(hipStreamSynchronize(c->stream));
+    c->graph_reset();
     if (!E && !nu) {
         c->has_fields = false;
+        c->elem_E_h.clear();
+        c->elem_nu_h.clear();
         c->elem_lambda.release();
         c->elem_mu.release();
         return APHIP_OK;
     }
     APHIP_REQUIRE(E && nu, "E and nu must both be given");
     std::vector<double> lam(c->ne), mu(c->ne);
+    c->elem_E_h.assign(c->ne, 0.0);
+    c->elem_nu_h.assign(c->ne, 0.0);
     for (int64_t e = 0; e < c->ne; ++e) {
         const int64_t src = c->elem_iperm[e];
         const double Ee = E[src], ne = nu[src];
         APHIP_REQUIRE(ne >= -1.0 && ne < 0.5 && Ee > 0.0, "per-element E/nu out of bounds");
         mu[e] = Ee / (2 * (1 + ne));
         lam[e] = (Ee * ne) / ((1 + ne) * (1 - 2 * ne));
+        c->elem_E_h[e] = Ee;
+        c->elem_nu_h[e] = ne;
     }
     c->elem_lambda.upload(lam.data(), lam.size(), c->stream);
     c->elem_mu.upload(mu.data(), mu.size(), c->stream);
@@ -354,6 +487,8 @@ int aphip_compute_mass(aphip_ctx *c, double *total) {
     APHIP_REQUIRE(c != nullptr, "ctx is NULL");
     require_ready(*c, false);
     APHIP_HIP(hipSetDevice(c->device));
+    APHIP_HIP(hipStreamSynchronize(c->stream));
+    c->graph_reset();
     double tot = 0.0;
     if (c->cells.ready) {
         rk_mass(*c, &tot);
@@ -374,6 +509,8 @@ int aphip_set_fixed(aphip_ctx *c, const uint8_t *mask) {
     APHIP_REQUIRE(c != nullptr, "ctx is NULL");
     APHIP_REQUIRE(c->etype >= 0, "mesh not set");
     APHIP_HIP(hipSetDevice(c->device));
+    APHIP_HIP(hipStreamSynchronize(c->stream));
+    c->graph_reset();
     if (!mask) {
         c->fixed.release();
         c->has_fixed = false;
@@ -396,6 +533,8 @@ int aphip_set_field(aphip_ctx *c, int32_t field, const double *data) {
     APHIP_REQUIRE(c != nullptr && data != nullptr, "NULL argument");
     APHIP_REQUIRE(c->etype >= 0, "mesh not set");
     APHIP_HIP(hipSetDevice(c->device));
+    APHIP_HIP(hipStreamSynchronize(c->stream));
+    c->graph_reset();
     if (field <= APHIP_FIELD_F) {
         DevBuf<double> *b = field == APHIP_FIELD_U ? &c->u : field == APHIP_FIELD_V ? &c->v
                             : field == APHIP_FIELD_A ? &c->a : &c->f;
@@ -480,8 +619,12 @@ int aphip_internal_force(aphip_ctx *c, double time, double dt) {
     (void)time;
     (void)dt;
     APHIP_HIP(hipSetDevice(c->device));
-    force_pass(*c);
-    node_pass(*c, 0.0, 0);
+    if (use_tiles(*c)) {
+        tile_run(*c, 0.0, 0);
+    } else {
+        force_pass(*c);
+        node_pass(*c, 0.0, 0);
+    }
     APHIP_HIP(hipStreamSynchronize(c->stream));
     APHIP_API_END
 }
@@ -491,8 +634,12 @@ int aphip_initialize(aphip_ctx *c) {
     APHIP_REQUIRE(c != nullptr, "ctx is NULL");
     require_ready(*c, true);
     APHIP_HIP(hipSetDevice(c->device));
-    force_pass(*c);
-    node_pass(*c, 0.0, 1);
+    if (use_tiles(*c)) {
+        tile_run(*c, 0.0, 1);
+    } else {
+        force_pass(*c);
+        node_pass(*c, 0.0, 1);
+    }
     APHIP_HIP(hipStreamSynchronize(c->stream));
     APHIP_API_END
 }
@@ -504,14 +651,209 @@ int aphip_step(aphip_ctx *c, double dt, int32_t n_steps) {
     APHIP_REQUIRE(n_steps >= 0, "n_steps must be non-negative");
     APHIP_REQUIRE(std::isfinite(dt) && dt >= 0.0, "time increment must be finite and non-negative");
     APHIP_HIP(hipSetDevice(c->device));
-    for (int32_t s = 0; s < n_steps; ++s) {
+    int32_t done = 0;
+    if (c->graph_enabled && !c->prof.enabled && n_steps >= 2) {
+        // two steps per graph: the N/NP1 state buffers swap twice and return to their roles
+        if (!c->graph_exec || c->graph_dt != dt || c->graph_parity != c->state_np1) {
+            c->graph_reset();
+            const int parity = c->state_np1;
+            hipGraph_t g = nullptr;
+            APHIP_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            try {
+                step_once(*c, dt);
+                step_once(*c, dt);
+            } catch (...) {
+                (void)hipStreamEndCapture(c->stream, &g);
+                if (g) (void)hipGraphDestroy(g);
+                c->state_np1 = parity;
+                throw;
+            }
+            APHIP_HIP(hipStreamEndCapture(c->stream, &g));
+            const hipError_t ie = hipGraphInstantiate(&c->graph_exec, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            APHIP_HIP(ie);
+            c->graph_dt = dt;
+            c->graph_parity = parity;
+        }
+        for (; done + 2 <= n_steps; done += 2) APHIP_HIP(hipGraphLaunch(c->graph_exec, c->stream));
+    }
+    for (; done < n_steps; ++done) {
         prof_begin(*c, 0);
-        c->state_np1 = 1 - c->state_np1;  // UpdateFieldStates (Explicit.cpp:393)
-        fem_predict(*c, dt);
-        force_pass(*c);
-        node_pass(*c, dt, 2);
+        step_once(*c, dt);
         prof_end(*c, 0);
     }
+    APHIP_API_END
+}
+
+int aphip_set_graph(aphip_ctx *c, int32_t enable) {
+    APHIP_API_BEGIN
+    APHIP_REQUIRE(c != nullptr, "ctx is NULL");
+    APHIP_HIP(hipSetDevice(c->device));
+    APHIP_HIP(hipStreamSynchronize(c->stream));
+    c->graph_reset();
+    c->graph_enabled = enable != 0;
+    APHIP_API_END
+}
+
+int aphip_autotune(aphip_ctx *c, double dt, int32_t trial_steps, int32_t *chosen) {
+    APHIP_API_BEGIN
+    APHIP_REQUIRE(c != nullptr, "ctx is NULL");
+    require_ready(*c, true);
+    APHIP_REQUIRE(!c->cells.ready && !(c->flags & APHIP_FLAG_TWO_PASS), "autotuning applies to the tiled element path");
+    APHIP_REQUIRE(trial_steps > 0, "trial_steps must be positive");
+    APHIP_HIP(hipSetDevice(c->device));
+    // save the simulation state in caller numbering (tilings renumber the nodes)
+    std::vector<double> su(3 * c->nn), sv(3 * c->nn), sa(3 * c->nn), sf(3 * c->nn);
+    std::vector<double> sH(9 * c->ne), sP(9 * c->ne);
+    const int64_t ns = c->state_count();
+    std::vector<double> s0(ns), s1(ns);
+    const int parity = c->state_np1;
+    get_node_field(*c, c->u, su.data());
+    get_node_field(*c, c->v, sv.data());
+    get_node_field(*c, c->a, sa.data());
+    get_node_field(*c, c->f, sf.data());
+    const int prev_te = c->tiles.te;
+    std::vector<std::vector<double>> ssave;
+    {
+        std::vector<double> h(9 * c->ne);
+        c->H.download(sH.data(), sH.size(), c->stream);
+        c->P.download(sP.data(), sP.size(), c->stream);
+        if (ns) {
+            c->state[0].download(s0.data(), ns, c->stream);
+            c->state[1].download(s1.data(), ns, c->stream);
+        }
+        APHIP_HIP(hipStreamSynchronize(c->stream));
+    }
+    std::vector<std::pair<int, int>> cand;
+    if (c->etype == APHIP_TET4) cand = {{64, 64}, {128, 128}, {256, 256}, {512, 512}};
+    else cand = {{64, 64}, {128, 128}, {16, 128}, {32, 256}, {64, 512}};
+    const bool was_prof = c->prof.enabled;
+    std::pair<int, int> best{prev_te, c->tiles.threads};
+    double best_ms = 1e300;
+    hipEvent_t e0, e1;
+    APHIP_HIP(hipEventCreate(&e0));
+    APHIP_HIP(hipEventCreate(&e1));
+    c->prof.enabled = false;
+    for (const auto &tt : cand) {
+        tile_build(*c, tt.first, tt.second);
+        step_once(*c, dt);  // warm-up
+        APHIP_HIP(hipEventRecord(e0, c->stream));
+        for (int k = 0; k < trial_steps; ++k) step_once(*c, dt);
+        APHIP_HIP(hipEventRecord(e1, c->stream));
+        APHIP_HIP(hipEventSynchronize(e1));
+        float ms = 0.0f;
+        APHIP_HIP(hipEventElapsedTime(&ms, e0, e1));
+        if (ms < best_ms) {
+            best_ms = ms;
+            best = tt;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    c->prof.enabled = was_prof;
+    tile_build(*c, best.first, best.second);
+    set_node_field(*c, c->u, su.data());
+    set_node_field(*c, c->v, sv.data());
+    set_node_field(*c, c->a, sa.data());
+    set_node_field(*c, c->f, sf.data());
+    c->H.upload(sH.data(), sH.size(), c->stream);
+    c->P.upload(sP.data(), sP.size(), c->stream);
+    if (ns) {
+        c->state[0].upload(s0.data(), ns, c->stream);
+        c->state[1].upload(s1.data(), ns, c->stream);
+    }
+    c->state_np1 = parity;
+    APHIP_HIP(hipStreamSynchronize(c->stream));
+    if (chosen) *chosen = best.first * 10000 + best.second;
+    APHIP_API_END
+}
+
+int aphip_batch_begin(aphip_ctx *c, int32_t elem_type, int32_t kind) {
+    APHIP_API_BEGIN
+    APHIP_REQUIRE(c != nullptr, "ctx is NULL");
+    APHIP_REQUIRE(elem_type == APHIP_TET4 || elem_type == APHIP_HEX8, "element type must be TET4 or HEX8");
+    APHIP_REQUIRE(kind >= APHIP_LINEAR_ELASTIC && kind <= APHIP_SVK, "unknown material kind");
+    c->batch = BatchBuild();
+    c->batch.open = true;
+    c->batch.etype = elem_type;
+    c->batch.kind = kind;
+    APHIP_API_END
+}
+
+int aphip_batch_add(aphip_ctx *c, int64_t n_nodes, const double *X, int64_t n_elem, const int32_t *conn,
+                    const aphip_material_params *p, double dt, int32_t *specimen) {
+    APHIP_API_BEGIN
+    APHIP_REQUIRE(c != nullptr && X != nullptr && conn != nullptr && p != nullptr, "NULL argument");
+    BatchBuild &B = c->batch;
+    APHIP_REQUIRE(B.open, "aphip_batch_begin has not been called");
+    APHIP_REQUIRE(n_nodes > 0 && n_elem > 0, "empty specimen");
+    APHIP_REQUIRE(std::isfinite(dt) && dt >= 0.0, "time increment must be finite and non-negative");
+    MaterialConst m;
+    resolve_material(B.kind, p, m);  // validates the parameters
+    const int npe = B.etype == APHIP_TET4 ? 4 : 8;
+    const int64_t base = B.node_seg.back();
+    APHIP_REQUIRE(base + n_nodes < (int64_t)INT32_MAX, "ensemble too large");
+    for (int64_t i = 0; i < n_elem * npe; ++i)
+        APHIP_REQUIRE(conn[i] >= 0 && conn[i] < n_nodes, "connectivity references a node out of range");
+    B.X.insert(B.X.end(), X, X + 3 * n_nodes);
+    for (int64_t i = 0; i < n_elem * npe; ++i) B.conn.push_back((int32_t)(base + conn[i]));
+    B.node_seg.push_back(base + n_nodes);
+    B.elem_seg.push_back(B.elem_seg.back() + n_elem);
+    B.params.push_back(*p);
+    B.dt.push_back(dt);
+    if (specimen) *specimen = (int32_t)(B.params.size() - 1);
+    APHIP_API_END
+}
+
+int aphip_batch_end(aphip_ctx *c) {
+    APHIP_API_BEGIN
+    APHIP_REQUIRE(c != nullptr, "ctx is NULL");
+    BatchBuild B = std::move(c->batch);
+    c->batch = BatchBuild();
+    APHIP_REQUIRE(B.open, "aphip_batch_begin has not been called");
+    APHIP_REQUIRE(!B.params.empty(), "the ensemble has no specimens");
+    APHIP_HIP(hipSetDevice(c->device));
+    // material: kind and J2 constants of the first specimen, E / nu / rho per element
+    MaterialConst m;
+    resolve_material(B.kind, &B.params[0], m);
+    c->mat = m;
+    const int64_t nn = B.node_seg.back(), ne = B.elem_seg.back();
+    setup_mesh(*c, B.etype, nn, B.X.data(), ne, B.conn.data(), B.node_seg, B.elem_seg);
+    const int64_t nspec = (int64_t)B.params.size();
+    std::vector<double> lam(ne), mu(ne), rho(ne), ndt(nn);
+    for (int64_t sp = 0; sp < nspec; ++sp) {
+        MaterialConst ms;
+        resolve_material(B.kind, &B.params[sp], ms);
+        for (int64_t e = B.elem_seg[sp]; e < B.elem_seg[sp + 1]; ++e) {
+            const int64_t ie = c->elem_perm[e];
+            lam[ie] = ms.lambda;
+            mu[ie] = ms.mu;
+            rho[ie] = ms.density;
+        }
+        for (int64_t i = B.node_seg[sp]; i < B.node_seg[sp + 1]; ++i) ndt[c->node_perm[i]] = B.dt[sp];
+    }
+    c->elem_lambda.upload(lam.data(), ne, c->stream);
+    c->elem_mu.upload(mu.data(), ne, c->stream);
+    c->elem_rho.upload(rho.data(), ne, c->stream);
+    c->node_dt.upload(ndt.data(), nn, c->stream);
+    c->has_fields = true;
+    c->n_specimens = nspec;
+    c->spec_node_seg = B.node_seg;
+    c->spec_elem_seg = B.elem_seg;
+    alloc_state(*c);
+    APHIP_HIP(hipStreamSynchronize(c->stream));
+    APHIP_API_END
+}
+
+int aphip_batch_ranges(aphip_ctx *c, int32_t sp, int64_t *first_node, int64_t *n_nodes, int64_t *first_elem,
+                       int64_t *n_elem) {
+    APHIP_API_BEGIN
+    APHIP_REQUIRE(c != nullptr, "ctx is NULL");
+    APHIP_REQUIRE(sp >= 0 && sp < c->n_specimens, "specimen index out of range");
+    if (first_node) *first_node = c->spec_node_seg[sp];
+    if (n_nodes) *n_nodes = c->spec_node_seg[sp + 1] - c->spec_node_seg[sp];
+    if (first_elem) *first_elem = c->spec_elem_seg[sp];
+    if (n_elem) *n_elem = c->spec_elem_seg[sp + 1] - c->spec_elem_seg[sp];
     APHIP_API_END
 }
 
@@ -531,6 +873,8 @@ int aphip_rk_setup(aphip_ctx *c, double scale) {
     APHIP_REQUIRE(c->etype == APHIP_TET4, "RKPM element-as-cell smoothing needs a Tet4 background mesh");
     APHIP_REQUIRE(scale >= 0.0, "Kernel radius scale factor must be non-negative.");
     APHIP_HIP(hipSetDevice(c->device));
+    APHIP_HIP(hipStreamSynchronize(c->stream));
+    c->graph_reset();
     rk_setup(*c, scale);
     c->has_mass = false;
     APHIP_HIP(hipStreamSynchronize(c->stream));
@@ -604,6 +948,32 @@ int aphip_rk_get_cells(aphip_ctx *c, int64_t *nnz, int64_t *start, int32_t *node
         }
         if (volume) volume[ce] = hv[e];
     }
+    APHIP_API_END
+}
+
+int aphip_internal_force_vjp(aphip_ctx *c, const double *f_bar, double *u_bar, double *E_bar, double *nu_bar) {
+    APHIP_API_BEGIN
+    APHIP_REQUIRE(c != nullptr && f_bar != nullptr, "NULL argument");
+    APHIP_HIP(hipSetDevice(c->device));
+    force_vjp(*c, f_bar, u_bar, E_bar, nu_bar);
+    APHIP_API_END
+}
+
+int aphip_adjoint_forward(aphip_ctx *c, double dt, int32_t n_steps, int32_t interval) {
+    APHIP_API_BEGIN
+    APHIP_REQUIRE(c != nullptr, "ctx is NULL");
+    APHIP_REQUIRE(std::isfinite(dt) && dt >= 0.0, "time increment must be finite and non-negative");
+    APHIP_HIP(hipSetDevice(c->device));
+    adjoint_forward(*c, dt, n_steps, interval);
+    APHIP_API_END
+}
+
+int aphip_adjoint_backward(aphip_ctx *c, const double *ub_T, const double *vb_T, double *E_bar, double *nu_bar,
+                           double *v0_bar) {
+    APHIP_API_BEGIN
+    APHIP_REQUIRE(c != nullptr, "ctx is NULL");
+    APHIP_HIP(hipSetDevice(c->device));
+    adjoint_backward(*c, ub_T, vb_T, E_bar, nu_bar, v0_bar);
     APHIP_API_END
 }
 

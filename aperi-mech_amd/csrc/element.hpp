@@ -1,0 +1,244 @@
+// Per-element force math shared by the element kernels (fem.hip, tile.hip).
+//
+//   Tet4 1-point: B from the constant derivatives, w = detJ / 6
+//     (include/ShapeFunctionsFunctorTet4.h:27-35, include/QuadratureGaussian.h:17-38,
+//      include/ElementTetrahedron4.h:125-127)
+//   Hex8 2x2x2: Gauss points in the order of include/ElementHexahedron8.h:107-115, w = detJ;
+//     per point H = (u^T dN) J^-1 and f_k -= dN_k (w J^-1 P^T), i.e. -B P^T w without
+//     forming B (include/ComputeInternalForceGaussian.h:85-124)
+//   Stored H / P are those of the last Gauss point (ComputeInternalForceGaussian.h:89, :99-107).
+#pragma once
+
+#include "device_math.hpp"
+
+namespace aphip {
+
+// Central-difference node updates shared by every kernel that finishes a node, so
+// a node finished in the tile kernel and one finished in the boundary pass see
+// bit-identical arithmetic.
+//   predictor  v1/2 = v + dt/2 a, velocity BC, u1 = u + dt v1/2  (ExplicitTimeIntegrator.h:116-126, :348-357)
+//   corrector  a = f/m, acceleration BC, v = v1/2 + dt/2 a       (ExplicitTimeIntegrator.h:267-276, :308-318)
+APHIP_DEV void cd_predict(const double *u, const double *v, const double *a, unsigned fixed, double dt, double *vh,
+                          double *u1) {
+    const double h = 0.5 * dt;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        double w = __fma_rn(h, a[d], v[d]);
+        if (fixed & (1u << d)) w = 0.0;
+        vh[d] = w;
+        u1[d] = __fma_rn(dt, w, u[d]);
+    }
+}
+
+APHIP_DEV void cd_correct(const double *f, double m, unsigned fixed, double dt, const double *vh, double *a1,
+                          double *v1) {
+    const double h = 0.5 * dt;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        double acc = f[d] / m;
+        if (fixed & (1u << d)) acc = 0.0;
+        a1[d] = acc;
+        v1[d] = __fma_rn(h, acc, vh[d]);
+    }
+}
+
+// Tet4: fe[k][j] = -(B_k . P_j) w
+template <int KIND>
+APHIP_DEV void tet4_element(const double (&X)[4][3], const double (&U)[4][3], const MatParams &m, double sn,
+                            double &snp1, double *H, double *P, double (&fe)[4][3]) {
+    double J[9], Ji[9];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        J[3 * i + 0] = X[1][i] - X[0][i];
+        J[3 * i + 1] = X[2][i] - X[0][i];
+        J[3 * i + 2] = X[3][i] - X[0][i];
+    }
+    const double det = inverse3(J, Ji);
+    const double w = (1.0 / 6.0) * det;
+    double B[4][3];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        B[1][j] = Ji[j];
+        B[2][j] = Ji[3 + j];
+        B[3][j] = Ji[6 + j];
+        B[0][j] = ((-Ji[j]) - Ji[3 + j]) - Ji[6 + j];
+    }
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            H[3 * i + j] = U[0][i] * B[0][j] + U[1][i] * B[1][j] + U[2][i] * B[2][j] + U[3][i] * B[3][j];
+    stress<KIND>(H, m, sn, snp1, P);
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            fe[k][j] = -((B[k][0] * P[3 * j] + B[k][1] * P[3 * j + 1] + B[k][2] * P[3 * j + 2]) * w);
+}
+
+APHIP_DEV void hex8_sign(int k, int &s0, int &s1, int &s2) {
+    s0 = (k == 1 || k == 2 || k == 5 || k == 6) ? 1 : -1;
+    s1 = (k == 2 || k == 3 || k == 6 || k == 7) ? 1 : -1;
+    s2 = (k >= 4) ? 1 : -1;
+}
+
+// Hex8 2x2x2; state_n / state_np1 point at the element's state slot(s) (per_qp: 8 slots).
+template <int KIND, bool PER_QP>
+APHIP_DEV void hex8_element(const double (&X)[8][3], const double (&U)[8][3], const MatParams &m, double ga,
+                            const double *state_n, double *state_np1, double *H, double *P, double (&fe)[8][3]) {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) fe[k][0] = fe[k][1] = fe[k][2] = 0.0;
+    double sn = 0.0;
+    if constexpr (KIND == APHIP_J2 && !PER_QP) sn = state_n[0];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        int sq0, sq1, sq2;
+        hex8_sign(q, sq0, sq1, sq2);
+        const double p0 = sq0 * ga, p1 = sq1 * ga, p2 = sq2 * ga;
+        double J[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, G[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        double dN[8][3];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            int s0, s1, s2;
+            hex8_sign(k, s0, s1, s2);
+            const double f0 = s0 < 0 ? (1.0 - p0) : (1.0 + p0);
+            const double f1 = s1 < 0 ? (1.0 - p1) : (1.0 + p1);
+            const double f2 = s2 < 0 ? (1.0 - p2) : (1.0 + p2);
+            dN[k][0] = (s0 * 0.125) * f1 * f2;
+            dN[k][1] = (s1 * 0.125) * f0 * f2;
+            dN[k][2] = (s2 * 0.125) * f0 * f1;
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int d = 0; d < 3; ++d) {
+                    J[3 * i + d] += X[k][i] * dN[k][d];
+                    G[3 * i + d] += U[k][i] * dN[k][d];
+                }
+        }
+        double Ji[9];
+        const double w = inverse3(J, Ji);
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+                H[3 * i + j] = G[3 * i] * Ji[j] + G[3 * i + 1] * Ji[3 + j] + G[3 * i + 2] * Ji[6 + j];
+        double snp1 = 0.0;
+        if constexpr (KIND == APHIP_J2 && PER_QP) sn = state_n[q];
+        stress<KIND>(H, m, sn, snp1, P);
+        if constexpr (KIND == APHIP_J2) {
+            if constexpr (PER_QP) state_np1[q] = snp1;
+            else if (q == 7) state_np1[0] = snp1;
+        }
+        double Q[9];
+#pragma unroll
+        for (int n = 0; n < 3; ++n)
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+                Q[3 * n + j] = w * (Ji[3 * n] * P[3 * j] + Ji[3 * n + 1] * P[3 * j + 1] + Ji[3 * n + 2] * P[3 * j + 2]);
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) fe[k][j] -= dN[k][0] * Q[j] + dN[k][1] * Q[3 + j] + dN[k][2] * Q[6 + j];
+    }
+}
+
+// Hex8 with one lane per Gauss point: lane q (= lane id mod 8) evaluates point q
+// and the eight lanes of an element combine their nodal contributions with a
+// three-step exchange, after which lane k holds node k's force (summed as
+// ((q0+q4)+(q2+q6)) + ((q1+q5)+(q3+q7)), a fixed order).  Node coordinates and
+// displacements are streamed from LDS (sX/sU, 3 doubles per local node, lc =
+// local node of each corner), so no lane keeps the 48-double element state.
+// H, P and a per-element J2 state are those of Gauss point 7 (the last point,
+// ComputeInternalForceGaussian.h:89, :99-107); with per-point state each lane
+// owns its slot.
+template <int KIND, bool PER_QP>
+APHIP_DEV void hex8_qp_lane(int q, const int (&lc)[8], const double *sX, const double *sU, const MatParams &m,
+                            double ga, const double *state_n, double *state_np1, double *H, double *P,
+                            double (&fk)[3]) {
+    int sq0, sq1, sq2;
+    hex8_sign(q, sq0, sq1, sq2);
+    const double p0 = sq0 * ga, p1 = sq1 * ga, p2 = sq2 * ga;
+    // parametric derivatives of node k at this lane's Gauss point
+    auto dn = [&](int k, double (&d)[3]) {
+        int s0, s1, s2;
+        hex8_sign(k, s0, s1, s2);
+        const double f0 = s0 < 0 ? (1.0 - p0) : (1.0 + p0);
+        const double f1 = s1 < 0 ? (1.0 - p1) : (1.0 + p1);
+        const double f2 = s2 < 0 ? (1.0 - p2) : (1.0 + p2);
+        d[0] = (s0 * 0.125) * f1 * f2;
+        d[1] = (s1 * 0.125) * f0 * f2;
+        d[2] = (s2 * 0.125) * f0 * f1;
+    };
+    double J[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, G[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        double d[3];
+        dn(k, d);
+        const double *x = sX + 3 * lc[k];
+        const double *u = sU + 3 * lc[k];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            const double xi = x[i], ui = u[i];
+#pragma unroll
+            for (int dd = 0; dd < 3; ++dd) {
+                J[3 * i + dd] += xi * d[dd];
+                G[3 * i + dd] += ui * d[dd];
+            }
+        }
+    }
+    double Ji[9];
+    const double w = inverse3(J, Ji);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            H[3 * i + j] = G[3 * i] * Ji[j] + G[3 * i + 1] * Ji[3 + j] + G[3 * i + 2] * Ji[6 + j];
+    double sn = 0.0, snp1 = 0.0;
+    if constexpr (KIND == APHIP_J2) sn = PER_QP ? state_n[q] : state_n[0];
+    stress<KIND>(H, m, sn, snp1, P);
+    if constexpr (KIND == APHIP_J2) {
+        if constexpr (PER_QP) state_np1[q] = snp1;
+        else if (q == 7) state_np1[0] = snp1;
+    }
+    double Q[9];
+#pragma unroll
+    for (int n = 0; n < 3; ++n)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            Q[3 * n + j] = w * (Ji[3 * n] * P[3 * j] + Ji[3 * n + 1] * P[3 * j + 1] + Ji[3 * n + 2] * P[3 * j + 2]);
+    // reduce-scatter over the 8 lanes: at each step keep the half of the nodes whose
+    // index bit matches this lane's bit and add the partner's copy of that half
+    const bool b2 = (q >> 2) & 1, b1 = (q >> 1) & 1, b0 = q & 1;
+    double r4[4][3];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        double dl[3], dh[3];
+        dn(k, dl);
+        dn(k + 4, dh);
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            const double cl = -(dl[0] * Q[j] + dl[1] * Q[3 + j] + dl[2] * Q[6 + j]);
+            const double ch = -(dh[0] * Q[j] + dh[1] * Q[3 + j] + dh[2] * Q[6 + j]);
+            const double keep = b2 ? ch : cl;
+            const double send = b2 ? cl : ch;
+            r4[k][j] = keep + __shfl_xor(send, 4, 8);
+        }
+    }
+    double r2[2][3];
+#pragma unroll
+    for (int k = 0; k < 2; ++k)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            const double keep = b1 ? r4[k + 2][j] : r4[k][j];
+            const double send = b1 ? r4[k][j] : r4[k + 2][j];
+            r2[k][j] = keep + __shfl_xor(send, 2, 8);
+        }
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        const double keep = b0 ? r2[1][j] : r2[0][j];
+        const double send = b0 ? r2[0][j] : r2[1][j];
+        fk[j] = keep + __shfl_xor(send, 1, 8);
+    }
+}
+
+}  // namespace aphip

@@ -16,7 +16,7 @@
 //   connectivity  AoS  npe int32 per element (one 16 B / 32 B load)
 //   H, P          SoA  9 planes of n_elem doubles (coalesced stores)
 //   partials      AoS  npe*3 doubles per element, read back by the node gather
-#include "device_math.hpp"
+#include "element.hpp"
 #include "internal.hpp"
 
 namespace aphip {
@@ -70,44 +70,20 @@ __global__ __launch_bounds__(kElemBlock) void k_tet4_force(ElemArgs A) {
     if (e >= A.ne) return;
     const int4 c = reinterpret_cast<const int4 *>(A.conn)[e];
     const int cn[4] = {c.x, c.y, c.z, c.w};
-    double X[4][3], U[4][3];
+    double X[4][3], U[4][3], fe[4][3];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
         load3(A.X, cn[k], X[k]);
         load3(A.u, cn[k], U[k]);
     }
-    // J = X^T dN with the constant Tet4 derivatives: columns X1-X0, X2-X0, X3-X0
-    double J[9], Ji[9];
-#pragma unroll
-    for (int i = 0; i < 3; ++i) {
-        J[3 * i + 0] = X[1][i] - X[0][i];
-        J[3 * i + 1] = X[2][i] - X[0][i];
-        J[3 * i + 2] = X[3][i] - X[0][i];
-    }
-    const double det = inverse3(J, Ji);
-    const double w = (1.0 / 6.0) * det;
-    double B[4][3];
-#pragma unroll
-    for (int j = 0; j < 3; ++j) {
-        B[1][j] = Ji[j];
-        B[2][j] = Ji[3 + j];
-        B[3][j] = Ji[6 + j];
-        B[0][j] = ((-Ji[j]) - Ji[3 + j]) - Ji[6 + j];
-    }
-    double H[9], P[9];
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-        for (int j = 0; j < 3; ++j)
-            H[3 * i + j] = U[0][i] * B[0][j] + U[1][i] * B[1][j] + U[2][i] * B[2][j] + U[3][i] * B[3][j];
     MatParams m = A.mat;
     if (A.elem_lambda) {
         m.lambda = A.elem_lambda[e];
         m.mu = A.elem_mu[e];
     }
-    double sn = 0.0, snp1 = 0.0;
+    double sn = 0.0, snp1 = 0.0, H[9], P[9];
     if constexpr (KIND == APHIP_J2) sn = A.state_n[e];
-    stress<KIND>(H, m, sn, snp1, P);
+    tet4_element<KIND>(X, U, m, sn, snp1, H, P, fe);
     if constexpr (KIND == APHIP_J2) A.state_np1[e] = snp1;
 #pragma unroll
     for (int i = 0; i < 9; ++i) {
@@ -118,12 +94,9 @@ __global__ __launch_bounds__(kElemBlock) void k_tet4_force(ElemArgs A) {
 #pragma unroll
     for (int k = 0; k < 4; ++k)
 #pragma unroll
-        for (int j = 0; j < 3; ++j)
-            fp[3 * k + j] = -((B[k][0] * P[3 * j] + B[k][1] * P[3 * j + 1] + B[k][2] * P[3 * j + 2]) * w);
+        for (int j = 0; j < 3; ++j) fp[3 * k + j] = fe[k][j];
 }
 
-// Hex8 2x2x2.  Per Gauss point the B matrix is never formed: with dN the
-// parametric derivatives, H = (u^T dN) J^-1 and f_k -= dN_k (w J^-1 P^T).
 template <int KIND, bool PER_QP>
 __global__ __launch_bounds__(128) void k_hex8_force(ElemArgs A) {
     const int64_t e = xcd_block(A.nblocks) * 128 + threadIdx.x;
@@ -136,71 +109,16 @@ __global__ __launch_bounds__(128) void k_hex8_force(ElemArgs A) {
     for (int k = 0; k < 8; ++k) {
         load3(A.X, cn[k], X[k]);
         load3(A.u, cn[k], U[k]);
-        fe[k][0] = fe[k][1] = fe[k][2] = 0.0;
     }
     MatParams m = A.mat;
     if (A.elem_lambda) {
         m.lambda = A.elem_lambda[e];
         m.mu = A.elem_mu[e];
     }
-    double sn = 0.0;
-    if constexpr (KIND == APHIP_J2 && !PER_QP) sn = A.state_n[e];
+    const int64_t so = PER_QP ? e * 8 : e;
     double H[9], P[9];
-    const double ga = A.gauss_a;
-#pragma unroll
-    for (int q = 0; q < 8; ++q) {
-        // Gauss point order of include/ElementHexahedron8.h:107-115
-        const int sq0 = (q == 1 || q == 2 || q == 5 || q == 6) ? 1 : -1;
-        const int sq1 = (q == 2 || q == 3 || q == 6 || q == 7) ? 1 : -1;
-        const int sq2 = (q >= 4) ? 1 : -1;
-        const double p0 = sq0 * ga, p1 = sq1 * ga, p2 = sq2 * ga;
-        double J[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, G[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-        double dN[8][3];
-#pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            const int s0 = (k == 1 || k == 2 || k == 5 || k == 6) ? 1 : -1;
-            const int s1 = (k == 2 || k == 3 || k == 6 || k == 7) ? 1 : -1;
-            const int s2 = (k >= 4) ? 1 : -1;
-            const double f0 = s0 < 0 ? (1.0 - p0) : (1.0 + p0);
-            const double f1 = s1 < 0 ? (1.0 - p1) : (1.0 + p1);
-            const double f2 = s2 < 0 ? (1.0 - p2) : (1.0 + p2);
-            dN[k][0] = (s0 * 0.125) * f1 * f2;
-            dN[k][1] = (s1 * 0.125) * f0 * f2;
-            dN[k][2] = (s2 * 0.125) * f0 * f1;
-#pragma unroll
-            for (int i = 0; i < 3; ++i)
-#pragma unroll
-                for (int d = 0; d < 3; ++d) {
-                    J[3 * i + d] += X[k][i] * dN[k][d];
-                    G[3 * i + d] += U[k][i] * dN[k][d];
-                }
-        }
-        double Ji[9];
-        const double w = inverse3(J, Ji);  // gauss weight 1
-#pragma unroll
-        for (int i = 0; i < 3; ++i)
-#pragma unroll
-            for (int j = 0; j < 3; ++j)
-                H[3 * i + j] = G[3 * i] * Ji[j] + G[3 * i + 1] * Ji[3 + j] + G[3 * i + 2] * Ji[6 + j];
-        double snp1 = 0.0;
-        if constexpr (KIND == APHIP_J2 && PER_QP) sn = A.state_n[e * 8 + q];
-        stress<KIND>(H, m, sn, snp1, P);
-        if constexpr (KIND == APHIP_J2) {
-            if constexpr (PER_QP) A.state_np1[e * 8 + q] = snp1;
-            else if (q == 7) A.state_np1[e] = snp1;
-        }
-        double Q[9];
-#pragma unroll
-        for (int n = 0; n < 3; ++n)
-#pragma unroll
-            for (int j = 0; j < 3; ++j)
-                Q[3 * n + j] = w * (Ji[3 * n] * P[3 * j] + Ji[3 * n + 1] * P[3 * j + 1] + Ji[3 * n + 2] * P[3 * j + 2]);
-#pragma unroll
-        for (int k = 0; k < 8; ++k)
-#pragma unroll
-            for (int j = 0; j < 3; ++j) fe[k][j] -= dN[k][0] * Q[j] + dN[k][1] * Q[3 + j] + dN[k][2] * Q[6 + j];
-    }
-    // stored H/P are those of the last Gauss point (ComputeInternalForceGaussian.h:89, :99-107)
+    hex8_element<KIND, PER_QP>(X, U, m, A.gauss_a, KIND == APHIP_J2 ? A.state_n + so : nullptr,
+                               KIND == APHIP_J2 ? A.state_np1 + so : nullptr, H, P, fe);
 #pragma unroll
     for (int i = 0; i < 9; ++i) {
         A.H[i * A.ne + e] = H[i];
@@ -224,6 +142,7 @@ struct NodeArgs {
     double *a;
     double *v;
     double *u;
+    const double *node_dt;
     double dt;
 };
 
@@ -231,14 +150,15 @@ struct NodeArgs {
 __global__ __launch_bounds__(kNodeBlock) void k_predict(NodeArgs A) {
     const int64_t i = (int64_t)blockIdx.x * kNodeBlock + threadIdx.x;
     if (i >= A.nn) return;
-    const double h = 0.5 * A.dt;
-    const uint8_t fx = A.fixed ? A.fixed[i] : 0;
+    double u[3], v[3], a[3], vh[3], u1[3];
+    load3(A.u, i, u);
+    load3(A.v, i, v);
+    load3(A.a, i, a);
+    cd_predict(u, v, a, A.fixed ? A.fixed[i] : 0u, A.node_dt ? A.node_dt[i] : A.dt, vh, u1);
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-        double vv = A.v[3 * i + d] + h * A.a[3 * i + d];
-        if (fx & (1u << d)) vv = 0.0;
-        A.v[3 * i + d] = vv;
-        A.u[3 * i + d] = A.u[3 * i + d] + A.dt * vv;
+        A.v[3 * i + d] = vh[d];
+        A.u[3 * i + d] = u1[d];
     }
 }
 
@@ -260,22 +180,21 @@ __global__ __launch_bounds__(kNodeBlock) void k_node(NodeArgs A) {
     A.f[3 * i] = f0;
     A.f[3 * i + 1] = f1;
     A.f[3 * i + 2] = f2;
-    if constexpr (MODE >= 1) {
+    if constexpr (MODE == 1) {
         const double m = A.mass[i];
-        double a0 = f0 / m, a1 = f1 / m, a2 = f2 / m;
-        if constexpr (MODE == 2) {
-            const uint8_t fx = A.fixed ? A.fixed[i] : 0;
-            if (fx & 1u) a0 = 0.0;
-            if (fx & 2u) a1 = 0.0;
-            if (fx & 4u) a2 = 0.0;
-            const double h = 0.5 * A.dt;
-            A.v[3 * i] += h * a0;
-            A.v[3 * i + 1] += h * a1;
-            A.v[3 * i + 2] += h * a2;
+        A.a[3 * i] = f0 / m;
+        A.a[3 * i + 1] = f1 / m;
+        A.a[3 * i + 2] = f2 / m;
+    } else if constexpr (MODE == 2) {
+        const double f[3] = {f0, f1, f2};
+        double vh[3], a1[3], v1[3];
+        load3(A.v, i, vh);
+        cd_correct(f, A.mass[i], A.fixed ? A.fixed[i] : 0u, A.node_dt ? A.node_dt[i] : A.dt, vh, a1, v1);
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            A.a[3 * i + d] = a1[d];
+            A.v[3 * i + d] = v1[d];
         }
-        A.a[3 * i] = a0;
-        A.a[3 * i + 1] = a1;
-        A.a[3 * i + 2] = a2;
     }
 }
 
@@ -332,11 +251,14 @@ __global__ __launch_bounds__(128) void k_volume(VolArgs A) {
 // m_I = sum over incident elements (ascending) of rho V_e / npe  (src/MassUtils.cpp:26-39)
 __global__ __launch_bounds__(kNodeBlock) void k_lumped_mass(int64_t nn, const int32_t *node_start,
                                                             const int32_t *node_inc, int npe, const double *vol,
-                                                            double rho, double *mass) {
+                                                            double rho, const double *elem_rho, double *mass) {
     const int64_t i = (int64_t)blockIdx.x * kNodeBlock + threadIdx.x;
     if (i >= nn) return;
     double m = 0.0;
-    for (int32_t t = node_start[i]; t < node_start[i + 1]; ++t) m += (vol[node_inc[t] / npe] * rho) / npe;
+    for (int32_t t = node_start[i]; t < node_start[i + 1]; ++t) {
+        const int64_t e = node_inc[t] / npe;
+        m += (vol[e] * (elem_rho ? elem_rho[e] : rho)) / npe;
+    }
     mass[i] = m;
 }
 
@@ -392,6 +314,7 @@ static NodeArgs node_args(Ctx &c, double dt) {
     A.a = c.a.p;
     A.v = c.v.p;
     A.u = c.u.p;
+    A.node_dt = c.node_dt.p;
     A.dt = dt;
     return A;
 }
@@ -426,7 +349,7 @@ void fem_lumped_mass(Ctx &c, double rho) {
     c.mass.alloc(c.nn);
     if (c.nn == 0) return;
     hipLaunchKernelGGL(k_lumped_mass, dim3((unsigned)((c.nn + kNodeBlock - 1) / kNodeBlock)), dim3(kNodeBlock), 0,
-                       c.stream, c.nn, c.node_start.p, c.node_inc.p, c.npe, c.volume.p, rho, c.mass.p);
+                       c.stream, c.nn, c.node_start.p, c.node_inc.p, c.npe, c.volume.p, rho, c.elem_rho.p, c.mass.p);
     APHIP_HIP(hipGetLastError());
 }
 

@@ -71,7 +71,29 @@ struct MaterialConst {
     double density = 0.0;
     double lambda = 0.0, mu = 0.0;
     double yield_stress = 0.0, hardening = 0.0;
+    double E = 0.0, nu = 0.0;
     int state_per_qp = 0;
+};
+
+// Checkpointed forward run for the adjoint (adjoint.hip).
+struct AdjointData {
+    bool ready = false;
+    double dt = 0.0;
+    int n_steps = 0, interval = 0;
+    std::vector<DevBuf<double> *> ckpt;  // (u, v, a) at steps 0, K, 2K, ...; 9 doubles per node
+    DevBuf<double> final_state;          // (u, v, a) at the last step
+    DevBuf<double> seg_u;                // recomputed u of one segment, K x 3 x nn
+    DevBuf<double> ub, vb, ab;           // adjoint node state
+    DevBuf<double> lam_bar, mu_bar;      // per-element parameter adjoints
+    void reset() {
+        for (auto *b : ckpt) delete b;
+        ckpt.clear();
+        final_state.release();
+        seg_u.release();
+        ub.release(); vb.release(); ab.release(); lam_bar.release(); mu_bar.release();
+        ready = false;
+    }
+    ~AdjointData() { reset(); }
 };
 
 // Smoothed-cell (RKPM) data in internal numbering.
@@ -83,19 +105,54 @@ struct CellData {
     DevBuf<int32_t> nbr;        // [nn * 40] internal ids, -1 padded
     DevBuf<double> phi;         // [nn * 40]
     DevBuf<double> radius;      // [nn]
-    DevBuf<int64_t> start;      // [ne + 1]
+    DevBuf<int64_t> start;      // [ne + 1] cell entry ranges
     DevBuf<int32_t> nodes;      // [nnz] internal node ids
     DevBuf<double> dphi;        // [nnz * 3]
     DevBuf<double> volume;      // [ne]
-    DevBuf<int64_t> node_start; // [nn + 1] node -> (cell entry) incidences
-    DevBuf<int64_t> node_inc;   // [nnz] entry index t (cell found by binary search in start)
-    DevBuf<int32_t> inc_cell;   // [nnz] owning cell of each incidence
+    DevBuf<int32_t> inc_cell;   // [nnz] cell owning each entry
+    DevBuf<int64_t> node_start; // [nn + 1] node -> entries naming it
+    DevBuf<int32_t> node_inc;   // unused (kept for layout stability)
+    DevBuf<int32_t> inc_entry;  // [nnz] entries sorted by node, ascending entry
+    DevBuf<double> st;          // [ne * 9] -V P^T of the last force evaluation
     void reset() {
         ready = false;
         nnz = n_truncated = 0;
         num_nbr.release(); nbr.release(); phi.release(); radius.release(); start.release();
-        nodes.release(); dphi.release(); volume.release(); node_start.release(); node_inc.release();
-        inc_cell.release();
+        nodes.release(); dphi.release(); volume.release(); inc_cell.release(); node_start.release();
+        node_inc.release(); inc_entry.release(); st.release();
+    }
+};
+
+// Element-tile plan of the fused step (tile.hip), internal numbering.  The plan
+// renumbers nodes: interior nodes of each tile are contiguous, shared nodes follow.
+struct TilePlan {
+    bool ready = false;
+    int te = 0;                       // elements per tile (= threads)
+    int threads = 0;                  // workgroup size
+    int64_t ntiles = 0;
+    int maxtn = 0;                    // most nodes in one tile
+    int64_t nb = 0;                   // shared nodes (finished by the second launch)
+    int64_t n_interior = 0;           // first shared node id
+    int64_t total_tile_nodes = 0;
+    size_t lds_bytes = 0;
+    DevBuf<int32_t> tn_start;         // [ntiles + 1] offsets into the tile-node arrays
+    DevBuf<int32_t> tile_e0;          // [ntiles + 1] first element of each tile
+    DevBuf<int32_t> tn_ninterior;     // [ntiles] interior nodes come first in each tile
+    DevBuf<int32_t> int_base;         // [ntiles] id of the tile's first interior node
+    DevBuf<int32_t> tn_node;          // [total] global node id
+    DevBuf<uint32_t> tn_csr;          // [total] (local CSR start << 16) | count
+    DevBuf<int32_t> tn_bdst;          // [total] slot of a shared node's tile sum (-1 interior)
+    DevBuf<uint16_t> lcsr;            // [ntiles * te * npe] local element * npe + slot
+    DevBuf<uint16_t> lconn;           // [ne * npe] local node of each element slot
+    DevBuf<double> bpart;             // [slots * 3] tile sums of shared nodes
+    DevBuf<int32_t> bstart;           // [nb + 1] slot range of each shared node
+    void reset() {
+        ready = false;
+        te = threads = maxtn = 0;
+        ntiles = nb = n_interior = total_tile_nodes = 0;
+        lds_bytes = 0;
+        tn_start.release(); tile_e0.release(); tn_ninterior.release(); int_base.release(); tn_node.release(); tn_csr.release();
+        tn_bdst.release(); lcsr.release(); lconn.release(); bpart.release(); bstart.release();
     }
 };
 
@@ -105,6 +162,18 @@ struct Profile {
     std::vector<int> tag;
     int64_t launches[3] = {0, 0, 0};
     double ms[3] = {0, 0, 0};
+};
+
+// Specimens collected between aphip_batch_begin / aphip_batch_end (caller numbering).
+struct BatchBuild {
+    bool open = false;
+    int etype = -1;
+    int kind = -1;
+    std::vector<double> X;
+    std::vector<int32_t> conn;
+    std::vector<int64_t> node_seg{0}, elem_seg{0};
+    std::vector<aphip_material_params> params;
+    std::vector<double> dt;
 };
 
 struct Ctx {
@@ -119,6 +188,12 @@ struct Ctx {
     std::vector<int32_t> node_iperm;  // internal -> caller
     std::vector<int32_t> elem_perm;   // caller -> internal
     std::vector<int32_t> elem_iperm;  // internal -> caller
+    std::vector<int32_t> conn_h;      // internal connectivity (host copy, for tile plans)
+    std::vector<int64_t> elem_seg{0}; // element blocks (specimens); tiles never cross them
+    int flags = 0;
+    BatchBuild batch;
+    int64_t n_specimens = 0;
+    std::vector<int64_t> spec_node_seg, spec_elem_seg;  // caller-numbering ranges per specimen
 
     DevBuf<double> X, u, v, a, f, mass;  // node vectors AoS n x 3 (mass n)
     DevBuf<uint8_t> fixed;               // bit d set = DOF d held at zero
@@ -134,10 +209,27 @@ struct Ctx {
     MaterialConst mat;
     bool has_fields = false;             // per-element lambda / mu
     DevBuf<double> elem_lambda, elem_mu;
+    DevBuf<double> elem_rho;             // per-element density (ensembles), else mat.density
+    std::vector<double> elem_E_h, elem_nu_h;  // per-element E / nu (internal order) when fields are set
+    DevBuf<double> node_dt;              // per-node time increment (ensembles), else the step argument
+
+    // captured step sequences (aphip_set_graph)
+    bool graph_enabled = false;
+    hipGraphExec_t graph_exec = nullptr;
+    double graph_dt = 0.0;
+    int graph_parity = -1;
+    int graph_steps = 0;
+    void graph_reset() {
+        if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+        graph_exec = nullptr;
+        graph_steps = 0;
+    }
     bool has_mass = false;
     bool has_fixed = false;
 
     CellData cells;
+    TilePlan tiles;
+    AdjointData adj;
     Profile prof;
 
     int64_t state_count() const {
@@ -155,6 +247,20 @@ void fem_predict(Ctx &c, double dt);
 void fem_node_update(Ctx &c, double dt, int mode);
 void fem_volumes(Ctx &c);
 void fem_lumped_mass(Ctx &c, double rho);
+
+// ---- fused tile step (tile.hip) ----
+void tile_build(Ctx &c, int te, int threads);
+// Reorders the internal nodes (old_of_new[i] = current id of new node i) and every
+// node-indexed array of the context (capi.hip).
+void renumber_nodes(Ctx &c, const std::vector<int32_t> &old_of_new);
+void tile_run(Ctx &c, double dt, int mode);
+
+// ---- adjoint (adjoint.hip) ----
+void adjoint_forward(Ctx &c, double dt, int n_steps, int interval);
+void adjoint_backward(Ctx &c, const double *ub_T, const double *vb_T, double *E_bar, double *nu_bar,
+                      double *v0_bar);
+void force_vjp(Ctx &c, const double *f_bar, double *u_bar, double *E_bar, double *nu_bar);
+void step_once_tiles(Ctx &c, double dt);
 
 // ---- RK (rk.hip) ----
 void rk_setup(Ctx &c, double scale);

@@ -1,9 +1,800 @@
-// RKPM set-up and smoothed-cell force (placeholder until the RK kernels land).
+// RKPM on gfx950: one-pass reproducing-kernel functions with element-as-cell
+// strain smoothing over a Tet4 background mesh.
+//
+// Set-up (once, Total Lagrangian):
+//   max edge length     src/MaxEdgeLengthProcessor.cpp:36-60
+//   kernel radius       src/NeighborSearchProcessor.cpp:154-185 (r = scale * l; scale 1 -> 1 + 1e-6)
+//   ball search         include/NeighborSearchProcessor.h:196-317, src/NeighborSearchProcessor.cpp:185-240,
+//                       include/NeighborSelectorFunctor.h:63-99 — here a uniform cell list (cell edge =
+//                       largest radius) replaces STK's LBVH; each point keeps its 40 nearest accepted
+//                       candidates by (tweaked d^2, id) in LDS
+//   RK functions        include/FunctionCreationProcessor.h:85-130,
+//                       include/ShapeFunctionsFunctorReproducingKernel.h:190-257 (4x4 moment matrix
+//                       inverted by complete-pivoting LU in registers)
+//   cell node lists     include/SmoothedCellDataProcessor.h:407-539 (first-seen union)
+//   cell derivatives    include/SmoothedCellDataProcessor.h:542-675, include/QuadratureSmoothed.h:32-62
+//   RK mass             src/MassUtils.cpp:79-112 via FunctionEvaluationProcessor::ScatterValues
+// Per step:
+//   cell kernel         include/ComputeInternalForceSmoothedCell.h:86-117, :186-224 (H = sum u (x) grad phi,
+//                       stress; the stress term -V P^T of :325-326 is kept per cell)
+//   node kernel         :328-349 as a node gather over the cell entries that name the node, in ascending
+//                       (cell, entry) order, fused with the central-difference node update
+#include <algorithm>
+#include <cmath>
+#include <numeric>
+#include <vector>
+
+#include "element.hpp"
 #include "internal.hpp"
 
 namespace aphip {
-void rk_setup(Ctx &, double) { throw Error(APHIP_ERR_UNSUPPORTED, "RKPM not built yet"); }
-void rk_cell_force(Ctx &) { throw Error(APHIP_ERR_UNSUPPORTED, "RKPM not built yet"); }
-void rk_node_update(Ctx &, double, int) { throw Error(APHIP_ERR_UNSUPPORTED, "RKPM not built yet"); }
-void rk_mass(Ctx &, double *) { throw Error(APHIP_ERR_UNSUPPORTED, "RKPM not built yet"); }
+
+void stable_sort_pairs(Ctx &c, const int32_t *keys_in, int32_t *keys_out, const int32_t *vals_in, int32_t *vals_out,
+                       int64_t n, int key_bits);
+
+namespace {
+
+constexpr int kMaxNbr = 40;  // include/Constants.h:15
+constexpr int kSearchBlock = 64;
+constexpr int kCellBlock = 64;
+constexpr int kMaxCellNodes = 4 * kMaxNbr;
+
+__device__ __forceinline__ void ld3(const double *p, int64_t i, double *o) {
+    o[0] = p[3 * i];
+    o[1] = p[3 * i + 1];
+    o[2] = p[3 * i + 2];
+}
+
+// |a - b|^2 with a fixed fused order (the oracle evaluates the same expression)
+__device__ __forceinline__ double dist2(const double *a, const double *b) {
+    const double dx = a[0] - b[0], dy = a[1] - b[1], dz = a[2] - b[2];
+    return __fma_rn(dz, dz, __fma_rn(dy, dy, dx * dx));
+}
+
+__global__ __launch_bounds__(256) void k_max_edge(int64_t nn, const int32_t *node_start, const int32_t *node_inc,
+                                                  const int32_t *conn, const double *X, double scale, double *radius) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= nn) return;
+    double xi[3];
+    ld3(X, i, xi);
+    double l = 0.0;
+    for (int32_t t = node_start[i]; t < node_start[i + 1]; ++t) {
+        const int64_t e = node_inc[t] / 4;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            double xj[3];
+            ld3(X, conn[4 * e + k], xj);
+            l = fmax(l, sqrt(dist2(xi, xj)));
+        }
+    }
+    radius[i] = l * scale;
+}
+
+struct Grid {
+    double lo[3];
+    double cell;
+    int nb[3];
+};
+
+__device__ __forceinline__ int bin_coord(double x, double lo, double cell, int nb) {
+    int b = (int)floor((x - lo) / cell);
+    return b < 0 ? 0 : (b >= nb ? nb - 1 : b);
+}
+
+__global__ __launch_bounds__(256) void k_bin_count(int64_t nn, const double *X, Grid G, int32_t *bin_of,
+                                                   int32_t *count) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= nn) return;
+    const int bx = bin_coord(X[3 * i], G.lo[0], G.cell, G.nb[0]);
+    const int by = bin_coord(X[3 * i + 1], G.lo[1], G.cell, G.nb[1]);
+    const int bz = bin_coord(X[3 * i + 2], G.lo[2], G.cell, G.nb[2]);
+    const int b = (bz * G.nb[1] + by) * G.nb[0] + bx;
+    bin_of[i] = b;
+    atomicAdd(&count[b], 1);
+}
+
+// order inside a bin is irrelevant: accepted candidates are ranked by (d^2, id)
+__global__ __launch_bounds__(256) void k_bin_fill(int64_t nn, const int32_t *bin_of, const int32_t *bstart,
+                                                  int32_t *fill, int32_t *items) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= nn) return;
+    const int b = bin_of[i];
+    items[bstart[b] + atomicAdd(&fill[b], 1)] = (int32_t)i;
+}
+
+struct SearchArgs {
+    int64_t nn;
+    const double *X;
+    const double *radius;
+    const int32_t *caller_id;
+    Grid G;
+    const int32_t *bstart;
+    const int32_t *items;
+    int32_t *num;
+    int32_t *nbr;
+    int32_t *ncand;
+};
+
+__device__ __forceinline__ bool before(double da, int ia, double db, int ib) {
+    return da < db || (da == db && ia < ib);
+}
+
+__global__ __launch_bounds__(kSearchBlock) void k_search(SearchArgs A) {
+    __shared__ double sd[kMaxNbr * kSearchBlock];
+    __shared__ int32_t sk[kMaxNbr * kSearchBlock];  // ranking key: caller id
+    __shared__ int32_t sn[kMaxNbr * kSearchBlock];  // internal id
+    const int tid = threadIdx.x;
+    const int64_t i = (int64_t)blockIdx.x * kSearchBlock + tid;
+    if (i >= A.nn) return;
+    double xi[3];
+    ld3(A.X, i, xi);
+    const int bx = bin_coord(xi[0], A.G.lo[0], A.G.cell, A.G.nb[0]);
+    const int by = bin_coord(xi[1], A.G.lo[1], A.G.cell, A.G.nb[1]);
+    const int bz = bin_coord(xi[2], A.G.lo[2], A.G.cell, A.G.nb[2]);
+    int cnt = 0, seen = 0;
+    for (int z = bz - 1; z <= bz + 1; ++z) {
+        if (z < 0 || z >= A.G.nb[2]) continue;
+        for (int y = by - 1; y <= by + 1; ++y) {
+            if (y < 0 || y >= A.G.nb[1]) continue;
+            for (int x = bx - 1; x <= bx + 1; ++x) {
+                if (x < 0 || x >= A.G.nb[0]) continue;
+                const int b = (z * A.G.nb[1] + y) * A.G.nb[0] + x;
+                for (int p = A.bstart[b]; p < A.bstart[b + 1]; ++p) {
+                    const int32_t j = A.items[p];
+                    double xj[3];
+                    ld3(A.X, j, xj);
+                    const double rj = A.radius[j];
+                    // the sphere of neighbor j contains point i
+                    if (!(dist2(xj, xi) <= rj * rj)) continue;
+                    ++seen;
+                    // ranking distance with the neighbor coordinates tweaked by (1+1e-6, 1+1e-7, 1+1e-8)
+                    const double tw[3] = {xj[0] * (1 + 1.0e-6), xj[1] * (1 + 1.0e-7), xj[2] * (1 + 1.0e-8)};
+                    const double d2 = dist2(tw, xi);
+                    const int32_t key = A.caller_id[j];
+                    if (cnt == kMaxNbr && !before(d2, key, sd[(kMaxNbr - 1) * kSearchBlock + tid],
+                                                  sk[(kMaxNbr - 1) * kSearchBlock + tid]))
+                        continue;
+                    int pos = cnt < kMaxNbr ? cnt : kMaxNbr - 1;
+                    while (pos > 0 && before(d2, key, sd[(pos - 1) * kSearchBlock + tid], sk[(pos - 1) * kSearchBlock + tid])) {
+                        sd[pos * kSearchBlock + tid] = sd[(pos - 1) * kSearchBlock + tid];
+                        sk[pos * kSearchBlock + tid] = sk[(pos - 1) * kSearchBlock + tid];
+                        sn[pos * kSearchBlock + tid] = sn[(pos - 1) * kSearchBlock + tid];
+                        --pos;
+                    }
+                    sd[pos * kSearchBlock + tid] = d2;
+                    sk[pos * kSearchBlock + tid] = key;
+                    sn[pos * kSearchBlock + tid] = j;
+                    if (cnt < kMaxNbr) ++cnt;
+                }
+            }
+        }
+    }
+    A.num[i] = cnt;
+    A.ncand[i] = seen;
+    for (int k = 0; k < kMaxNbr; ++k) A.nbr[i * kMaxNbr + k] = k < cnt ? sn[k * kSearchBlock + tid] : -1;
+}
+
+// Cubic B-spline kernel, include/MathUtils.h:321-330
+__device__ __forceinline__ double kernel_value(const double *d, double R) {
+    const double q = sqrt(__fma_rn(d[2], d[2], __fma_rn(d[1], d[1], d[0] * d[0]))) / R;
+    if (q < 0.5) return 1.0 + 6.0 * q * q * (-1.0 + q);
+    if (q < 1.0) return 2.0 * (1.0 + q * (-3.0 + 3.0 * q - 1.0 * q * q));
+    return 0.0;
+}
+
+__device__ __forceinline__ void swap_d(double &a, double &b) {
+    const double t = a;
+    a = b;
+    b = t;
+}
+
+// Row 0 of the inverse of the symmetric 4x4 moment matrix by LU with complete
+// pivoting (the published FullPivLU algorithm of Eigen 3.4, which InvertMatrix uses on
+// CPU builds, include/MathUtils.h:307-319): first maximum in column-major scan,
+// rank threshold |max pivot| * eps * 4, unknowns beyond the rank set to zero.
+// All indices are compile-time after unrolling, so the matrix stays in registers.
+// Returns the Frobenius norm of the full inverse.
+__device__ double inverse4_row0(double (&a)[4][4], double (&row0)[4]) {
+    int prow[4] = {0, 1, 2, 3}, pcol[4] = {0, 1, 2, 3};
+    int nonzero = 4;
+    double maxpiv = 0.0;
+    bool stop = false;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        if (!stop) {
+            double big = -1.0;
+            int bi = k, bj = k;
+#pragma unroll
+            for (int j = k; j < 4; ++j)
+#pragma unroll
+                for (int i = k; i < 4; ++i) {
+                    const double v = fabs(a[i][j]);
+                    if (v > big) {
+                        big = v;
+                        bi = i;
+                        bj = j;
+                    }
+                }
+            if (big == 0.0) {
+                nonzero = k;
+                stop = true;
+            } else {
+                maxpiv = fmax(maxpiv, big);
+#pragma unroll
+                for (int r = k + 1; r < 4; ++r)
+                    if (bi == r) {
+#pragma unroll
+                        for (int j = 0; j < 4; ++j) swap_d(a[k][j], a[r][j]);
+                        const int tmp = prow[k];
+                        prow[k] = prow[r];
+                        prow[r] = tmp;
+                    }
+#pragma unroll
+                for (int r = k + 1; r < 4; ++r)
+                    if (bj == r) {
+#pragma unroll
+                        for (int i = 0; i < 4; ++i) swap_d(a[i][k], a[i][r]);
+                        const int tmp = pcol[k];
+                        pcol[k] = pcol[r];
+                        pcol[r] = tmp;
+                    }
+                const double piv = a[k][k];
+#pragma unroll
+                for (int i = k + 1; i < 4; ++i) a[i][k] /= piv;
+#pragma unroll
+                for (int i = k + 1; i < 4; ++i)
+#pragma unroll
+                    for (int j = k + 1; j < 4; ++j) a[i][j] -= a[i][k] * a[k][j];
+            }
+        }
+    }
+    const double thr = maxpiv * (2.220446049250313e-16 * 4.0);
+    int rank = 0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+        if (i < nonzero && fabs(a[i][i]) > thr) ++rank;
+    double fro = 0.0;
+#pragma unroll
+    for (int col = 0; col < 4; ++col) row0[col] = 0.0;
+    if (rank == 0) return 0.0;
+#pragma unroll
+    for (int col = 0; col < 4; ++col) {
+        double cv[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) cv[i] = (prow[i] == col) ? 1.0 : 0.0;
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < i; ++j) cv[i] -= a[i][j] * cv[j];
+#pragma unroll
+        for (int i = 3; i >= 0; --i) {
+            if (i < rank) {
+#pragma unroll
+                for (int j = i + 1; j < 4; ++j)
+                    if (j < rank) cv[i] -= a[i][j] * cv[j];
+                cv[i] /= a[i][i];
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const double x = i < rank ? cv[i] : 0.0;
+            fro += x * x;
+            if (pcol[i] == 0) row0[col] = x;
+        }
+    }
+    return sqrt(fro);
+}
+
+__global__ __launch_bounds__(128) void k_rk_phi(int64_t nn, const double *X, const double *radius, const int32_t *num,
+                                                const int32_t *nbr, double *phi) {
+    const int64_t i = (int64_t)blockIdx.x * 128 + threadIdx.x;
+    if (i >= nn) return;
+    const int n = num[i];
+    double *ph = phi + i * kMaxNbr;
+    for (int k = 0; k < kMaxNbr; ++k) ph[k] = 0.0;
+    if (n == 0) return;
+    if (n == 1) {
+        ph[0] = 1.0;
+        return;
+    }
+    double xi[3];
+    ld3(X, i, xi);
+    double M[4][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}, {0, 0, 0, 0}, {0, 0, 0, 0}};
+    double wsum = 0.0;
+    for (int k = 0; k < n; ++k) {
+        const int32_t j = nbr[i * kMaxNbr + k];
+        double xj[3];
+        ld3(X, j, xj);
+        const double d[3] = {xi[0] - xj[0], xi[1] - xj[1], xi[2] - xj[2]};
+        const double w = kernel_value(d, radius[j]);
+        if (w == 0.0) continue;
+        const double Hb[4] = {1.0, d[0], d[1], d[2]};
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b) M[a][b] += (w * Hb[a]) * Hb[b];
+        wsum += w;
+    }
+    double r0[4];
+    const double fro = inverse4_row0(M, r0);
+    const bool shepard = fro < 1e-15;
+    for (int k = 0; k < n; ++k) {
+        const int32_t j = nbr[i * kMaxNbr + k];
+        double xj[3];
+        ld3(X, j, xj);
+        const double d[3] = {xi[0] - xj[0], xi[1] - xj[1], xi[2] - xj[2]};
+        const double w = kernel_value(d, radius[j]);
+        if (shepard) {
+            ph[k] = w / wsum;
+        } else if (w != 0.0) {
+            ph[k] = (r0[0] + r0[1] * d[0] + r0[2] * d[1] + r0[3] * d[2]) * w;
+        }
+    }
+}
+
+// First-seen union of the neighbor lists of a cell's four nodes, in LDS.
+__device__ __forceinline__ int cell_union(const int32_t *en, const int32_t *num, const int32_t *nbr, int32_t *list,
+                                          int stride) {
+    int len = 0;
+    for (int k = 0; k < 4; ++k) {
+        const int32_t a = en[k];
+        const int na = num[a];
+        for (int l = 0; l < na; ++l) {
+            const int32_t j = nbr[(int64_t)a * kMaxNbr + l];
+            bool found = false;
+            for (int t = 0; t < len; ++t)
+                if (list[t * stride] == j) {
+                    found = true;
+                    break;
+                }
+            if (!found) list[(len++) * stride] = j;
+        }
+    }
+    return len;
+}
+
+__global__ __launch_bounds__(kCellBlock) void k_cell_count(int64_t ne, const int32_t *conn, const int32_t *num,
+                                                           const int32_t *nbr, int32_t *len_out) {
+    __shared__ int32_t list[kMaxCellNodes * kCellBlock];
+    const int64_t e = (int64_t)blockIdx.x * kCellBlock + threadIdx.x;
+    if (e >= ne) return;
+    const int32_t en[4] = {conn[4 * e], conn[4 * e + 1], conn[4 * e + 2], conn[4 * e + 3]};
+    len_out[e] = cell_union(en, num, nbr, list + threadIdx.x, kCellBlock);
+}
+
+// Smoothed Tet4 B and volume from the faces (include/QuadratureSmoothed.h:32-62)
+__device__ void smoothed_tet4(const double (&X)[4][3], double (&B)[4][3], double &V) {
+    const int faces[4][3] = {{0, 1, 3}, {1, 2, 3}, {0, 3, 2}, {0, 2, 1}};
+    double vol = 0.0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) B[k][0] = B[k][1] = B[k][2] = 0.0;
+#pragma unroll
+    for (int f = 0; f < 4; ++f) {
+        double e1[3], e2[3], n[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            e1[d] = X[faces[f][1]][d] - X[faces[f][0]][d];
+            e2[d] = X[faces[f][2]][d] - X[faces[f][0]][d];
+        }
+        n[0] = (e1[1] * e2[2] - e1[2] * e2[1]) / 2.0;
+        n[1] = (e1[2] * e2[0] - e1[0] * e2[2]) / 2.0;
+        n[2] = (e1[0] * e2[1] - e1[1] * e2[0]) / 2.0;
+        const double *x0 = X[faces[f][0]];
+        vol += x0[0] * n[0] + x0[1] * n[1] + x0[2] * n[2];
+#pragma unroll
+        for (int l = 0; l < 3; ++l)
+#pragma unroll
+            for (int d = 0; d < 3; ++d) B[faces[f][l]][d] += n[d] / 3.0;
+    }
+    vol /= 3.0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) B[k][d] /= vol;
+    V = vol;
+}
+
+struct CellFillArgs {
+    int64_t ne;
+    const int32_t *conn;
+    const double *X;
+    const int32_t *num;
+    const int32_t *nbr;
+    const double *phi;
+    const int64_t *start;
+    int32_t *nodes;
+    double *dphi;
+    double *volume;
+    int32_t *entry_cell;
+};
+
+__global__ __launch_bounds__(kCellBlock) void k_cell_fill(CellFillArgs A) {
+    __shared__ int32_t list[kMaxCellNodes * kCellBlock];
+    const int tid = threadIdx.x;
+    const int64_t e = (int64_t)blockIdx.x * kCellBlock + tid;
+    if (e >= A.ne) return;
+    const int32_t en[4] = {A.conn[4 * e], A.conn[4 * e + 1], A.conn[4 * e + 2], A.conn[4 * e + 3]};
+    int32_t *my = list + tid;
+    const int len = cell_union(en, A.num, A.nbr, my, kCellBlock);
+    const int64_t s0 = A.start[e];
+    double X[4][3], B[4][3], V;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) ld3(A.X, en[k], X[k]);
+    smoothed_tet4(X, B, V);
+    double *dp = A.dphi + 3 * s0;
+    for (int t = 0; t < len; ++t) {
+        A.nodes[s0 + t] = my[t * kCellBlock];
+        A.entry_cell[s0 + t] = (int32_t)e;
+        dp[3 * t] = dp[3 * t + 1] = dp[3 * t + 2] = 0.0;
+    }
+    // grad phi_J = sum_k B_k V phi_{k,J} / V_sub (SmoothedCellDataProcessor.h:623-662)
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int32_t a = en[k];
+        const int na = A.num[a];
+        for (int l = 0; l < na; ++l) {
+            const int32_t j = A.nbr[(int64_t)a * kMaxNbr + l];
+            const double fv = A.phi[(int64_t)a * kMaxNbr + l];
+            int t = 0;
+            while (my[t * kCellBlock] != j) ++t;
+#pragma unroll
+            for (int m = 0; m < 3; ++m) dp[3 * t + m] += B[k][m] * V * fv;
+        }
+    }
+    for (int t = 0; t < len; ++t) {
+        dp[3 * t] /= V;
+        dp[3 * t + 1] /= V;
+        dp[3 * t + 2] /= V;
+    }
+    A.volume[e] = V;
+}
+
+__global__ __launch_bounds__(256) void k_iota(int64_t n, int32_t *v) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) v[i] = (int32_t)i;
+}
+
+__global__ __launch_bounds__(256) void k_count_keys(int64_t n, const int32_t *keys, int32_t *count) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) atomicAdd(&count[keys[i]], 1);
+}
+
+struct CellForceArgs {
+    int64_t ne;
+    const int64_t *start;
+    const int32_t *nodes;
+    const double *dphi;
+    const double *volume;
+    const double *u;
+    double *H, *P;
+    double *st;  // -V P^T per cell (AoS 9)
+    const double *state_n;
+    double *state_np1;
+    MatParams mat;
+    const double *elem_lambda, *elem_mu;
+};
+
+template <int KIND>
+__global__ __launch_bounds__(128) void k_rk_cells(CellForceArgs A) {
+    const int64_t c = (int64_t)blockIdx.x * 128 + threadIdx.x;
+    if (c >= A.ne) return;
+    double H[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, P[9];
+    const int64_t t1 = A.start[c + 1];
+    for (int64_t t = A.start[c]; t < t1; ++t) {
+        double uj[3], g[3];
+        ld3(A.u, A.nodes[t], uj);
+        ld3(A.dphi, t, g);
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int b = 0; b < 3; ++b) H[3 * a + b] += uj[a] * g[b];
+    }
+    MatParams m = A.mat;
+    if (A.elem_lambda) {
+        m.lambda = A.elem_lambda[c];
+        m.mu = A.elem_mu[c];
+    }
+    double sn = 0.0, snp1 = 0.0;
+    if constexpr (KIND == APHIP_J2) sn = A.state_n[c];
+    stress<KIND>(H, m, sn, snp1, P);
+    if constexpr (KIND == APHIP_J2) A.state_np1[c] = snp1;
+    const double V = A.volume[c];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        A.H[i * A.ne + c] = H[i];
+        A.P[i * A.ne + c] = P[i];
+    }
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int b = 0; b < 3; ++b) A.st[9 * c + 3 * a + b] = P[3 * b + a] * -V;
+}
+
+struct RkNodeArgs {
+    int64_t nn;
+    const int64_t *inc_start;
+    const int32_t *inc_entry;
+    const int32_t *entry_cell;
+    const double *dphi;
+    const double *st;
+    const double *mass;
+    const uint8_t *fixed;
+    double *f, *a, *v;
+    double dt;
+};
+
+template <int MODE>
+__global__ __launch_bounds__(256) void k_rk_nodes(RkNodeArgs A) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= A.nn) return;
+    double f[3] = {0.0, 0.0, 0.0};
+    const int64_t t1 = A.inc_start[i + 1];
+    for (int64_t t = A.inc_start[i]; t < t1; ++t) {
+        const int64_t en = A.inc_entry[t];
+        const double *s = A.st + 9 * (int64_t)A.entry_cell[en];
+        double g[3];
+        ld3(A.dphi, en, g);
+#pragma unroll
+        for (int b = 0; b < 3; ++b) f[b] += g[0] * s[b] + g[1] * s[3 + b] + g[2] * s[6 + b];
+    }
+    A.f[3 * i] = f[0];
+    A.f[3 * i + 1] = f[1];
+    A.f[3 * i + 2] = f[2];
+    if constexpr (MODE == 1) {
+        const double m = A.mass[i];
+        A.a[3 * i] = f[0] / m;
+        A.a[3 * i + 1] = f[1] / m;
+        A.a[3 * i + 2] = f[2] / m;
+    } else if constexpr (MODE == 2) {
+        double vh[3], a1[3], v1[3];
+        ld3(A.v, i, vh);
+        cd_correct(f, A.mass[i], A.fixed ? A.fixed[i] : 0u, A.dt, vh, a1, v1);
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            A.a[3 * i + d] = a1[d];
+            A.v[3 * i + d] = v1[d];
+        }
+    }
+}
+
+// mass_from_elements: rho V / 4 from every cell touching the node, ascending cell
+__global__ __launch_bounds__(256) void k_rk_mfe(int64_t nn, const int32_t *node_start, const int32_t *node_inc,
+                                                const double *vol, double rho, double *mfe) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= nn) return;
+    double m = 0.0;
+    for (int32_t t = node_start[i]; t < node_start[i + 1]; ++t) m += (vol[node_inc[t] / 4] * rho) / 4;
+    mfe[i] = m;
+}
+
+// m_J = sum over points i that list J of phi_iJ * mfe_i, ascending i (ScatterValues)
+__global__ __launch_bounds__(256) void k_rk_mass(int64_t nn, const int32_t *num, const int64_t *rstart,
+                                                 const int32_t *rval, const double *phi, const double *mfe,
+                                                 double *mass) {
+    const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (j >= nn) return;
+    if (num[j] == 0) {
+        mass[j] = mfe[j];
+        return;
+    }
+    double m = 0.0;
+    for (int64_t t = rstart[j]; t < rstart[j + 1]; ++t) {
+        const int64_t slot = rval[t];
+        m += mfe[slot / kMaxNbr] * phi[slot];
+    }
+    mass[j] = m;
+}
+
+inline dim3 grid1(int64_t n, int block) { return dim3((unsigned)((n + block - 1) / block)); }
+
+// counts -> exclusive prefix (host) -> device start array of n+1 entries
+template <typename T>
+void scan_counts(Ctx &c, const DevBuf<int32_t> &counts, int64_t n, DevBuf<T> &start, int64_t *total) {
+    std::vector<int32_t> h(n);
+    counts.download(h.data(), n, c.stream);
+    APHIP_HIP(hipStreamSynchronize(c.stream));
+    std::vector<T> s(n + 1, 0);
+    for (int64_t i = 0; i < n; ++i) s[i + 1] = s[i] + (T)h[i];
+    start.upload(s.data(), s.size(), c.stream);
+    APHIP_HIP(hipStreamSynchronize(c.stream));
+    if (total) *total = (int64_t)s[n];
+}
+
+}  // namespace
+
+void rk_setup(Ctx &c, double scale) {
+    CellData &D = c.cells;
+    D.reset();
+    const int64_t nn = c.nn, ne = c.ne;
+    if (scale == 1.0) scale += 1.0e-6;  // src/NeighborSearchProcessor.cpp:161-163
+
+    // kernel radius from the max edge length
+    D.radius.alloc(nn);
+    hipLaunchKernelGGL(k_max_edge, grid1(nn, 256), dim3(256), 0, c.stream, nn, c.node_start.p, c.node_inc.p, c.conn.p,
+                       c.X.p, scale, D.radius.p);
+    APHIP_HIP(hipGetLastError());
+
+    // uniform cell list with cell edge = largest radius
+    std::vector<double> hX(3 * nn), hr(nn);
+    c.X.download(hX.data(), hX.size(), c.stream);
+    D.radius.download(hr.data(), hr.size(), c.stream);
+    APHIP_HIP(hipStreamSynchronize(c.stream));
+    Grid G;
+    double hi[3];
+    for (int d = 0; d < 3; ++d) {
+        G.lo[d] = INFINITY;
+        hi[d] = -INFINITY;
+    }
+    double rmax = 0.0;
+    for (int64_t i = 0; i < nn; ++i) {
+        for (int d = 0; d < 3; ++d) {
+            G.lo[d] = std::min(G.lo[d], hX[3 * i + d]);
+            hi[d] = std::max(hi[d], hX[3 * i + d]);
+        }
+        rmax = std::max(rmax, hr[i]);
+    }
+    double ext = 0.0;
+    for (int d = 0; d < 3; ++d) ext = std::max(ext, hi[d] - G.lo[d]);
+    G.cell = rmax > 0.0 ? rmax : (ext > 0.0 ? ext : 1.0);
+    // keep the grid modest; a coarser cell only adds candidates, never drops one
+    while (true) {
+        int64_t tot = 1;
+        for (int d = 0; d < 3; ++d) {
+            G.nb[d] = (int)std::floor((hi[d] - G.lo[d]) / G.cell) + 1;
+            tot *= G.nb[d];
+        }
+        if (tot <= (int64_t)1 << 24) break;
+        G.cell *= 1.26;
+    }
+    const int64_t nbins = (int64_t)G.nb[0] * G.nb[1] * G.nb[2];
+    DevBuf<int32_t> bin_of, bcount, bstart, bfill, items;
+    bin_of.alloc(nn);
+    bcount.alloc(nbins);
+    bcount.zero(c.stream);
+    hipLaunchKernelGGL(k_bin_count, grid1(nn, 256), dim3(256), 0, c.stream, nn, c.X.p, G, bin_of.p, bcount.p);
+    APHIP_HIP(hipGetLastError());
+    scan_counts(c, bcount, nbins, bstart, nullptr);
+    bfill.alloc(nbins);
+    bfill.zero(c.stream);
+    items.alloc(nn);
+    hipLaunchKernelGGL(k_bin_fill, grid1(nn, 256), dim3(256), 0, c.stream, nn, bin_of.p, bstart.p, bfill.p, items.p);
+    APHIP_HIP(hipGetLastError());
+
+    // ball search, ranked by caller ids for ties
+    DevBuf<int32_t> caller, ncand;
+    caller.upload(c.node_iperm.data(), nn, c.stream);
+    D.num_nbr.alloc(nn);
+    D.nbr.alloc(nn * kMaxNbr);
+    ncand.alloc(nn);
+    SearchArgs S{nn, c.X.p, D.radius.p, caller.p, G, bstart.p, items.p, D.num_nbr.p, D.nbr.p, ncand.p};
+    hipLaunchKernelGGL(k_search, grid1(nn, kSearchBlock), dim3(kSearchBlock), 0, c.stream, S);
+    APHIP_HIP(hipGetLastError());
+    {
+        std::vector<int32_t> hn(nn), hc(nn);
+        D.num_nbr.download(hn.data(), nn, c.stream);
+        ncand.download(hc.data(), nn, c.stream);
+        APHIP_HIP(hipStreamSynchronize(c.stream));
+        int64_t tr = 0;
+        for (int64_t i = 0; i < nn; ++i) tr += hc[i] - hn[i];
+        D.n_truncated = tr;
+    }
+
+    // RK function values
+    D.phi.alloc(nn * kMaxNbr);
+    hipLaunchKernelGGL(k_rk_phi, grid1(nn, 128), dim3(128), 0, c.stream, nn, c.X.p, D.radius.p, D.num_nbr.p, D.nbr.p,
+                       D.phi.p);
+    APHIP_HIP(hipGetLastError());
+
+    // smoothed cells: counts, offsets, lists and derivatives
+    DevBuf<int32_t> clen;
+    clen.alloc(std::max<int64_t>(ne, 1));
+    if (ne) {
+        hipLaunchKernelGGL(k_cell_count, grid1(ne, kCellBlock), dim3(kCellBlock), 0, c.stream, ne, c.conn.p,
+                           D.num_nbr.p, D.nbr.p, clen.p);
+        APHIP_HIP(hipGetLastError());
+    }
+    scan_counts(c, clen, ne, D.start, &D.nnz);
+    APHIP_REQUIRE(D.nnz < (int64_t)INT32_MAX, "too many cell entries for int32 incidences");
+    D.nodes.alloc(std::max<int64_t>(D.nnz, 1));
+    D.dphi.alloc(3 * std::max<int64_t>(D.nnz, 1));
+    D.volume.alloc(std::max<int64_t>(ne, 1));
+    D.inc_cell.alloc(std::max<int64_t>(D.nnz, 1));
+    if (ne) {
+        CellFillArgs F{ne, c.conn.p, c.X.p, D.num_nbr.p, D.nbr.p, D.phi.p, D.start.p, D.nodes.p, D.dphi.p,
+                       D.volume.p, D.inc_cell.p};
+        hipLaunchKernelGGL(k_cell_fill, grid1(ne, kCellBlock), dim3(kCellBlock), 0, c.stream, F);
+        APHIP_HIP(hipGetLastError());
+    }
+
+    // node -> entries naming it, ascending entry (stable radix sort by node)
+    {
+        DevBuf<int32_t> vals, keys_sorted, ccount;
+        vals.alloc(std::max<int64_t>(D.nnz, 1));
+        keys_sorted.alloc(std::max<int64_t>(D.nnz, 1));
+        D.node_inc.release();
+        DevBuf<int32_t> inc;
+        inc.alloc(std::max<int64_t>(D.nnz, 1));
+        hipLaunchKernelGGL(k_iota, grid1(D.nnz, 256), dim3(256), 0, c.stream, D.nnz, vals.p);
+        int bits = 1;
+        while (((int64_t)1 << bits) < nn) ++bits;
+        stable_sort_pairs(c, D.nodes.p, keys_sorted.p, vals.p, inc.p, D.nnz, bits);
+        ccount.alloc(nn);
+        ccount.zero(c.stream);
+        hipLaunchKernelGGL(k_count_keys, grid1(D.nnz, 256), dim3(256), 0, c.stream, D.nnz, D.nodes.p, ccount.p);
+        scan_counts(c, ccount, nn, D.node_start, nullptr);
+        D.inc_entry.alloc(std::max<int64_t>(D.nnz, 1));
+        APHIP_HIP(hipMemcpyAsync(D.inc_entry.p, inc.p, sizeof(int32_t) * std::max<int64_t>(D.nnz, 1),
+                                 hipMemcpyDeviceToDevice, c.stream));
+    }
+    D.st.alloc(9 * std::max<int64_t>(ne, 1));
+    APHIP_HIP(hipStreamSynchronize(c.stream));
+    D.ready = true;
+}
+
+void rk_cell_force(Ctx &c) {
+    CellData &D = c.cells;
+    if (c.ne == 0) return;
+    CellForceArgs A{c.ne, D.start.p, D.nodes.p, D.dphi.p, D.volume.p, c.u.p, c.H.p, c.P.p, D.st.p,
+                    c.state[1 - c.state_np1].p, c.state[c.state_np1].p,
+                    MatParams{c.mat.lambda, c.mat.mu, c.mat.yield_stress, c.mat.hardening},
+                    c.has_fields ? c.elem_lambda.p : nullptr, c.has_fields ? c.elem_mu.p : nullptr};
+    const dim3 g = grid1(c.ne, 128);
+    switch (c.mat.kind) {
+        case APHIP_LINEAR_ELASTIC: hipLaunchKernelGGL(k_rk_cells<APHIP_LINEAR_ELASTIC>, g, dim3(128), 0, c.stream, A); break;
+        case APHIP_NEO_HOOKEAN: hipLaunchKernelGGL(k_rk_cells<APHIP_NEO_HOOKEAN>, g, dim3(128), 0, c.stream, A); break;
+        case APHIP_J2: hipLaunchKernelGGL(k_rk_cells<APHIP_J2>, g, dim3(128), 0, c.stream, A); break;
+        case APHIP_SVK: hipLaunchKernelGGL(k_rk_cells<APHIP_SVK>, g, dim3(128), 0, c.stream, A); break;
+        default: throw Error(APHIP_ERR_INVALID, "material not set");
+    }
+    APHIP_HIP(hipGetLastError());
+}
+
+void rk_node_update(Ctx &c, double dt, int mode) {
+    CellData &D = c.cells;
+    RkNodeArgs A{c.nn, D.node_start.p, D.inc_entry.p, D.inc_cell.p, D.dphi.p, D.st.p, c.mass.p,
+                 c.has_fixed ? c.fixed.p : nullptr, c.f.p, c.a.p, c.v.p, dt};
+    const dim3 g = grid1(c.nn, 256);
+    if (mode == 0) hipLaunchKernelGGL(k_rk_nodes<0>, g, dim3(256), 0, c.stream, A);
+    else if (mode == 1) hipLaunchKernelGGL(k_rk_nodes<1>, g, dim3(256), 0, c.stream, A);
+    else hipLaunchKernelGGL(k_rk_nodes<2>, g, dim3(256), 0, c.stream, A);
+    APHIP_HIP(hipGetLastError());
+}
+
+void rk_mass(Ctx &c, double *total) {
+    CellData &D = c.cells;
+    const int64_t nn = c.nn;
+    DevBuf<double> mfe;
+    mfe.alloc(nn);
+    hipLaunchKernelGGL(k_rk_mfe, grid1(nn, 256), dim3(256), 0, c.stream, nn, c.node_start.p, c.node_inc.p,
+                       D.volume.p, c.mat.density, mfe.p);
+    APHIP_HIP(hipGetLastError());
+    // reverse neighbor lists: (neighbor J, slot i*40+l) sorted stably by J
+    std::vector<int32_t> hn(nn), hb(nn * kMaxNbr);
+    D.num_nbr.download(hn.data(), nn, c.stream);
+    D.nbr.download(hb.data(), hb.size(), c.stream);
+    APHIP_HIP(hipStreamSynchronize(c.stream));
+    std::vector<int64_t> rstart(nn + 1, 0);
+    for (int64_t i = 0; i < nn; ++i)
+        for (int l = 0; l < hn[i]; ++l) rstart[hb[i * kMaxNbr + l] + 1]++;
+    for (int64_t j = 0; j < nn; ++j) rstart[j + 1] += rstart[j];
+    std::vector<int32_t> rval(std::max<int64_t>(rstart[nn], 1));
+    std::vector<int64_t> fill(rstart.begin(), rstart.end() - 1);
+    for (int64_t i = 0; i < nn; ++i)
+        for (int l = 0; l < hn[i]; ++l) rval[fill[hb[i * kMaxNbr + l]]++] = (int32_t)(i * kMaxNbr + l);
+    DevBuf<int64_t> drs;
+    DevBuf<int32_t> drv;
+    drs.upload(rstart.data(), rstart.size(), c.stream);
+    drv.upload(rval.data(), rval.size(), c.stream);
+    c.mass.alloc(nn);
+    hipLaunchKernelGGL(k_rk_mass, grid1(nn, 256), dim3(256), 0, c.stream, nn, D.num_nbr.p, drs.p, drv.p, D.phi.p,
+                       mfe.p, c.mass.p);
+    APHIP_HIP(hipGetLastError());
+    std::vector<double> hm(nn);
+    c.mass.download(hm.data(), nn, c.stream);
+    APHIP_HIP(hipStreamSynchronize(c.stream));
+    double tot = 0.0;
+    for (double x : hm) tot += x;
+    if (total) *total = tot;
+}
+
 }  // namespace aphip

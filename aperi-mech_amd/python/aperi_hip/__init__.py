@@ -23,6 +23,7 @@ TET4, HEX8 = 0, 1
 LINEAR_ELASTIC, NEO_HOOKEAN, J2, SVK = 0, 1, 2, 3
 FIELD_U, FIELD_V, FIELD_A, FIELD_F, FIELD_MASS, FIELD_H, FIELD_P, FIELD_STATE, FIELD_VOLUME = range(9)
 MAX_NODE_NUM_NEIGHBORS = 40
+FLAG_TWO_PASS = 1
 
 _d = ctypes.POINTER(ctypes.c_double)
 _i32 = ctypes.POINTER(ctypes.c_int32)
@@ -33,7 +34,9 @@ EXPORTED = [
     "aphip_create", "aphip_destroy", "aphip_last_error", "aphip_device_count", "aphip_set_mesh", "aphip_get_node_permutation",
     "aphip_set_material", "aphip_set_material_fields", "aphip_compute_mass", "aphip_set_fixed",
     "aphip_set_field", "aphip_get_field", "aphip_internal_force", "aphip_initialize", "aphip_step",
-    "aphip_synchronize", "aphip_rk_setup", "aphip_rk_get_neighbors", "aphip_rk_get_function_values",
+    "aphip_synchronize", "aphip_set_tiling", "aphip_set_graph", "aphip_autotune", "aphip_batch_begin",
+    "aphip_batch_add", "aphip_batch_end", "aphip_batch_ranges", "aphip_internal_force_vjp",
+    "aphip_adjoint_forward", "aphip_adjoint_backward", "aphip_rk_setup", "aphip_rk_get_neighbors", "aphip_rk_get_function_values",
     "aphip_rk_get_cells", "aphip_set_profiling", "aphip_get_profile", "aphip_reset_profile",
 ]
 
@@ -95,10 +98,10 @@ def _p(a, t=_d):
 class Solver:
     """One element block on one GPU behind the C ABI."""
 
-    def __init__(self, device=0):
+    def __init__(self, device=0, flags=0):
         self.L = load_library()
         self.ctx = ctypes.c_void_p()
-        cfg = Config(device, 0)
+        cfg = Config(device, flags)
         _check(self.L.aphip_create(ctypes.byref(self.ctx), ctypes.byref(cfg)))
         self.nn = self.ne = 0
         self.etype = None
@@ -197,9 +200,72 @@ class Solver:
         _check(self.L.aphip_step(self.ctx, ctypes.c_double(dt), int(n_steps)))
         return self
 
+    def set_tiling(self, elems_per_tile, threads):
+        _check(self.L.aphip_set_tiling(self.ctx, int(elems_per_tile), int(threads)))
+        return self
+
     def synchronize(self):
         _check(self.L.aphip_synchronize(self.ctx))
         return self
+
+    def set_graph(self, on=True):
+        _check(self.L.aphip_set_graph(self.ctx, 1 if on else 0))
+        return self
+
+    def autotune(self, dt, trial_steps=5):
+        chosen = ctypes.c_int32()
+        _check(self.L.aphip_autotune(self.ctx, ctypes.c_double(dt), int(trial_steps), ctypes.byref(chosen)))
+        return chosen.value
+
+    # ---- adjoints ----
+    def internal_force_vjp(self, f_bar):
+        fb = np.ascontiguousarray(f_bar, dtype=np.float64)
+        ub = np.zeros((self.nn, 3))
+        Eb = np.zeros(self.ne)
+        nb = np.zeros(self.ne)
+        _check(self.L.aphip_internal_force_vjp(self.ctx, _p(fb), _p(ub), _p(Eb), _p(nb)))
+        return ub, Eb, nb
+
+    def adjoint_forward(self, dt, n_steps, checkpoint_interval):
+        _check(self.L.aphip_adjoint_forward(self.ctx, ctypes.c_double(dt), int(n_steps), int(checkpoint_interval)))
+        return self
+
+    def adjoint_backward(self, u_bar_T, v_bar_T=None):
+        ub = np.ascontiguousarray(u_bar_T, dtype=np.float64)
+        vb = None if v_bar_T is None else np.ascontiguousarray(v_bar_T, dtype=np.float64)
+        Eb = np.zeros(self.ne)
+        nb = np.zeros(self.ne)
+        v0b = np.zeros((self.nn, 3))
+        _check(self.L.aphip_adjoint_backward(self.ctx, _p(ub), _p(vb), _p(Eb), _p(nb), _p(v0b)))
+        return Eb, nb, v0b
+
+    # ---- ensembles ----
+    def batch_begin(self, etype, kind):
+        _check(self.L.aphip_batch_begin(self.ctx, etype, kind))
+        self.etype, self.kind, self._nn, self._ne = etype, kind, 0, 0
+        return self
+
+    def batch_add(self, X, conn, density, E, nu, dt, yield_stress=0.0, hardening=0.0):
+        X = np.ascontiguousarray(X, dtype=np.float64)
+        conn = np.ascontiguousarray(conn, dtype=np.int32)
+        p = MaterialParams(density, E, nu, yield_stress, hardening, 0, 0)
+        sp = ctypes.c_int32()
+        _check(self.L.aphip_batch_add(self.ctx, X.shape[0], _p(X), conn.shape[0], _p(conn, _i32), ctypes.byref(p),
+                                      ctypes.c_double(dt), ctypes.byref(sp)))
+        self._nn += X.shape[0]
+        self._ne += conn.shape[0]
+        return sp.value
+
+    def batch_end(self):
+        _check(self.L.aphip_batch_end(self.ctx))
+        self.nn, self.ne = self._nn, self._ne
+        return self
+
+    def batch_ranges(self, specimen):
+        a, b, c, d = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+        _check(self.L.aphip_batch_ranges(self.ctx, specimen, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c),
+                                         ctypes.byref(d)))
+        return a.value, b.value, c.value, d.value
 
     # ---- RKPM ----
     def rk_setup(self, kernel_radius_scale):

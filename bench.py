@@ -22,8 +22,9 @@ import numpy as np  # noqa: E402
 
 METRIC = "FP64 element-updates/s + % HBM roofline; batched & adjoint vs plain, 1 GPU"
 HBM_PEAK_GBS = 8000.0
-# Algorithmic bytes per element-update of the full-output step:
-# element streams conn 16 B + H 72 B + P 72 B, node streams 200 B per node.
+# Algorithmic bytes per element-update of the full-output step: element streams
+# conn 16 B + H 72 B + P 72 B; node streams 200 B per node (read X, u, v, a as
+# 4 x 24 B and the mass 8 B; write u, v, a, f as 4 x 24 B).
 ELEM_BYTES_TET4 = 4 * 4 + 72 + 72
 NODE_BYTES = 200
 
@@ -113,7 +114,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
-              f"(the force step shards by Morton element ranges with a halo force sum; see DESIGN.md)",
+              f"(the force step would shard by Morton tile ranges with a shared-node force exchange)",
               file=sys.stderr)
         sys.exit(2)
     import aperi_hip as A
@@ -145,9 +146,9 @@ def main():
                    "elements": ne, "nodes": nn, "material": args.material},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "whole step (predict + element force + node gather/update)",
+                     "kernel": "whole step (fused element-tile kernel + shared-node kernel)",
                      "bytes_per_update": bytes_per_update},
-        "kernels_ms": {"step": r["step_ms"], "element_force": r["force_ms"], "node_update": r["node_ms"]},
+        "kernels_ms": {"step": r["step_ms"], "tile_kernel": r["force_ms"], "shared_node_kernel": r["node_ms"]},
     }
     if not r["finite"]:
         line["error"] = "non-finite displacement"

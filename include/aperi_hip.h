@@ -59,9 +59,12 @@ enum {
     APHIP_FIELD_VOLUME = 8
 };
 
+/* aphip_config.flags */
+#define APHIP_FLAG_TWO_PASS 1 /* element partials through HBM + node CSR gather (no LDS tiles) */
+
 typedef struct {
     int32_t device; /* HIP device ordinal */
-    int32_t flags;  /* reserved, 0 */
+    int32_t flags;  /* APHIP_FLAG_* */
 } aphip_config;
 
 /* Material parameters.  Replaces the YAML-built MaterialProperties
@@ -136,6 +139,25 @@ int aphip_initialize(aphip_ctx *ctx);
  * velocity BC, u += dt v, internal force, a = f/m, acceleration BC, v += dt/2 a. */
 int aphip_step(aphip_ctx *ctx, double time_increment, int32_t n_steps);
 
+/* Element-tile shape of the fused step: elements per tile and workgroup size.
+ * Tet4: one element per thread (64, 128, 256 or 512).  Hex8: one element per
+ * thread (64 or 128), or eight lanes per element, one per Gauss point
+ * (threads = 8 x elements, 8..64 elements).  Rebuilds the tile plan; results do
+ * not depend on it beyond the rounding of the summation grouping.  No reference
+ * counterpart. */
+int aphip_set_tiling(aphip_ctx *ctx, int32_t elems_per_tile, int32_t threads);
+
+/* Replay aphip_step through a captured HIP graph (two steps per graph, so the
+ * alternating state buffers return to their captured roles).  No reference
+ * counterpart (the reference launches every kernel eagerly). */
+int aphip_set_graph(aphip_ctx *ctx, int32_t enable);
+
+/* Times the candidate tile shapes on the context's own workload (trial_steps steps
+ * each, the simulation state is restored afterwards) and keeps the fastest.
+ * Writes elements_per_tile * 10000 + threads of the choice to *chosen.  No
+ * reference counterpart. */
+int aphip_autotune(aphip_ctx *ctx, double time_increment, int32_t trial_steps, int32_t *chosen);
+
 /* Wait for all work queued on the context's stream. */
 int aphip_synchronize(aphip_ctx *ctx);
 
@@ -160,6 +182,43 @@ int aphip_rk_get_function_values(aphip_ctx *ctx, double *phi);
  * Pass NULL arrays to query nnz only. */
 int aphip_rk_get_cells(aphip_ctx *ctx, int64_t *nnz, int64_t *start, int32_t *nodes, double *dphi,
                        double *volume);
+
+/* ---- ensembles (extension; no reference counterpart) ---------------------- */
+
+/* Uneven-size specimens with their own material parameters and time increment,
+ * advanced together by the same kernels as one element block.  Tiles never
+ * span two specimens, so every specimen's results are bitwise identical to
+ * running it alone in its own context.  Between begin and end, add specimens
+ * in caller numbering local to the specimen; afterwards the context behaves as
+ * one block whose caller numbering is the concatenation of the specimens (see
+ * aphip_batch_ranges).  All specimens share the element type and material kind;
+ * J2 yield stress and hardening are taken from the first specimen.  With a
+ * batch, aphip_step ignores its time_increment argument. */
+int aphip_batch_begin(aphip_ctx *ctx, int32_t elem_type, int32_t material_kind);
+int aphip_batch_add(aphip_ctx *ctx, int64_t n_nodes, const double *X, int64_t n_elem, const int32_t *conn,
+                    const aphip_material_params *params, double time_increment, int32_t *specimen);
+int aphip_batch_end(aphip_ctx *ctx);
+int aphip_batch_ranges(aphip_ctx *ctx, int32_t specimen, int64_t *first_node, int64_t *n_nodes,
+                       int64_t *first_elem, int64_t *n_elem);
+
+/* ---- adjoints (extension; no reference counterpart) ----------------------- */
+
+/* Vector-Jacobian product of the internal force at the current displacement:
+ * given f_bar (n_nodes x 3) returns u_bar = (df/du)^T f_bar (n_nodes x 3) and the
+ * per-element E_bar, nu_bar = (df/dE_e)^T f_bar, (df/dnu_e)^T f_bar (n_elem each).
+ * Linear elastic and neo-Hookean only.  Any output may be NULL. */
+int aphip_internal_force_vjp(aphip_ctx *ctx, const double *f_bar, double *u_bar, double *E_bar, double *nu_bar);
+
+/* Runs n_steps of aphip_step from the context's current state (u, v, a after
+ * aphip_initialize) keeping (u, v, a) checkpoints every checkpoint_interval steps. */
+int aphip_adjoint_forward(aphip_ctx *ctx, double time_increment, int32_t n_steps, int32_t checkpoint_interval);
+
+/* Reverse sweep of the last aphip_adjoint_forward: given the adjoint of the final
+ * displacement (and optionally velocity), returns dL/dE_e, dL/dnu_e (n_elem, caller
+ * order) and dL/dv0 (n_nodes x 3), recomputing each segment from its checkpoint.
+ * The initial acceleration is taken as f(u0)/m.  Leaves the final forward state. */
+int aphip_adjoint_backward(aphip_ctx *ctx, const double *u_bar_T, const double *v_bar_T, double *E_bar,
+                           double *nu_bar, double *v0_bar);
 
 /* ---- profiling ----------------------------------------------------------- */
 

@@ -97,7 +97,7 @@ double or_det_ap_im1(const double A[9]) {
 
 /* Cubic B-spline kernel, include/MathUtils.h:321-330. */
 double or_kernel_value(const double d[3], double R) {
-    const double q = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]) / R;
+    const double q = sqrt(fma(d[2], d[2], fma(d[1], d[1], d[0] * d[0]))) / R;
     if (q < 0.5) return 1.0 + 6.0 * q * q * (-1.0 + q);
     if (q < 1.0) return 2.0 * (1.0 + q * (-3.0 + 3.0 * q - 1.0 * q * q));
     return 0.0;

@@ -35,6 +35,13 @@
 
 #include "oracle.h"
 
+/* |a - b|^2 with one fixed fused evaluation order, so that every build of the
+ * search agrees to the bit on inclusion and ranking: fma(dz, dz, fma(dy, dy, dx*dx)). */
+static double dist2(const double *a, const double *b) {
+    const double dx = a[0] - b[0], dy = a[1] - b[1], dz = a[2] - b[2];
+    return fma(dz, dz, fma(dy, dy, dx * dx));
+}
+
 int or_max_edge_length(int64_t n_nodes, const double *X, int64_t n_elem, const int32_t *conn, int npe,
                        const uint8_t *active, double *max_edge) {
     memset(max_edge, 0, sizeof(double) * n_nodes);
@@ -43,9 +50,8 @@ int or_max_edge_length(int64_t n_nodes, const double *X, int64_t n_elem, const i
             const int64_t i = conn[e * npe + a];
             for (int b = 0; b < npe; ++b) {
                 const int64_t j = conn[e * npe + b];
-                const double dx = X[3 * i] - X[3 * j], dy = X[3 * i + 1] - X[3 * j + 1], dz = X[3 * i + 2] - X[3 * j + 2];
                 const double s = (active && !active[j]) ? 2.0 : 1.0;
-                const double len = sqrt(dx * dx + dy * dy + dz * dz) * s;
+                const double len = sqrt(dist2(&X[3 * i], &X[3 * j])) * s;
                 if (len > max_edge[i]) max_edge[i] = len;
             }
         }
@@ -64,14 +70,12 @@ static int cand_cmp(const void *pa, const void *pb) {
 }
 
 static double tweaked_d2(const double *xi, const double *xj) {
-    const double nx = xj[0] * (1 + 1.0e-6), ny = xj[1] * (1 + 1.0e-7), nz = xj[2] * (1 + 1.0e-8);
-    const double dx = nx - xi[0], dy = ny - xi[1], dz = nz - xi[2];
-    return dx * dx + dy * dy + dz * dz;
+    const double tw[3] = {xj[0] * (1 + 1.0e-6), xj[1] * (1 + 1.0e-7), xj[2] * (1 + 1.0e-8)};
+    return dist2(tw, xi);
 }
 
 static int contains(const double *xi, const double *xj, double rj) {
-    const double dx = xj[0] - xi[0], dy = xj[1] - xi[1], dz = xj[2] - xi[2];
-    return dx * dx + dy * dy + dz * dz <= rj * rj;
+    return dist2(xj, xi) <= rj * rj;
 }
 
 int or_neighbor_search(int64_t n_nodes, const double *X, const double *radius, int max_neighbors,

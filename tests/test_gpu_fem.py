@@ -32,8 +32,8 @@ def _mesh(etype, n, seed):
     return X, conn
 
 
-def _gpu(etype, X, conn, kind, p, state_per_qp=0):
-    s = A.Solver()
+def _gpu(etype, X, conn, kind, p, state_per_qp=0, flags=0):
+    s = A.Solver(flags=flags)
     s.set_mesh(etype, X, conn)
     s.set_material(kind, p["rho"], p["E"], p["nu"], p.get("yield_stress", 0.0), p.get("hardening", 0.0),
                    state_per_qp)
@@ -168,3 +168,52 @@ def test_errors_are_reported_not_crashes(gpu_available):
     with pytest.raises(A.AphipError):
         s.step(1e-6, 1)  # no mass yet
     s.close()
+
+
+TILINGS = [(A.TET4, (64, 64)), (A.TET4, (128, 128)), (A.TET4, (256, 256)), (A.TET4, (512, 512)),
+           (A.TET4, "two_pass"), (A.HEX8, (64, 64)), (A.HEX8, (128, 128)), (A.HEX8, (8, 64)),
+           (A.HEX8, (16, 128)), (A.HEX8, (32, 256)), (A.HEX8, (64, 512)), (A.HEX8, "two_pass")]
+
+
+@pytest.mark.parametrize("etype,tiling", TILINGS, ids=[f"{'tet4' if e == 0 else 'hex8'}-{t}" for e, t in TILINGS])
+def test_tilings_and_two_pass_path(gpu_available, etype, tiling):
+    """Every tile shape (incl. a partial last tile) and the two-pass path give the oracle's
+    20-step trajectory; the tile sums differ only in summation grouping."""
+    kind, p = MATS["nh"]
+    X, conn = _mesh(etype, 5, 31)
+    lam, mu = O.lame(p["E"], p["nu"])
+    v0 = 5.0 * (2.0 * M.uniform01(4, 2, X.size).reshape(X.shape) - 1.0)
+    mass = O.lumped_mass(etype, X, conn, p["rho"])
+    dt = 0.5 * M.stable_dt_estimate(X, conn, p["E"], p["nu"], p["rho"])
+    ref = O.Solver(etype, X, conn, kind, lam, mu, mass)
+    ref.v[:] = v0
+    ref.run(dt, 20)
+    flags = A.FLAG_TWO_PASS if tiling == "two_pass" else 0
+    print(f"tiling {tiling} etype {etype}", flush=True)
+    with _gpu(etype, X, conn, kind, p, flags=flags) as s:
+        if tiling != "two_pass":
+            s.set_tiling(*tiling)
+        s.compute_mass()
+        s.set_field(A.FIELD_V, v0)
+        s.initialize()
+        s.step(dt, 20)
+        for fld, r in ((A.FIELD_U, ref.u), (A.FIELD_V, ref.v), (A.FIELD_A, ref.a), (A.FIELD_F, ref.f),
+                       (A.FIELD_H, ref.H), (A.FIELD_P, ref.P)):
+            assert rel_err(s.get(fld), r) < 1e-11, fld
+
+
+def test_tile_results_are_deterministic(gpu_available):
+    kind, p = MATS["nh"]
+    X, conn = _mesh(A.TET4, 6, 2)
+    v0 = 5.0 * (2.0 * M.uniform01(4, 2, X.size).reshape(X.shape) - 1.0)
+    dt = 0.5 * M.stable_dt_estimate(X, conn, p["E"], p["nu"], p["rho"])
+    outs = []
+    for _ in range(2):
+        with _gpu(A.TET4, X, conn, kind, p) as s:
+            s.compute_mass()
+            s.set_field(A.FIELD_V, v0)
+            s.initialize()
+            s.step(dt, 10)
+            outs.append((s.get(A.FIELD_U), s.get(A.FIELD_F), s.get(A.FIELD_P)))
+    for a, b in zip(*outs):
+        assert np.array_equal(a, b)  # bitwise: no atomics anywhere in the step
