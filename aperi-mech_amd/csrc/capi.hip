@@ -1006,4 +1006,11 @@ int aphip_reset_profile(aphip_ctx *c) {
     APHIP_API_END
 }
 
+int aphip_bandwidth_probe(int32_t device, int64_t nbytes, int32_t reps, double *ms_per_copy) {
+    APHIP_API_BEGIN
+    APHIP_REQUIRE(ms_per_copy != nullptr, "ms_per_copy is NULL");
+    *ms_per_copy = bandwidth_probe(device, nbytes, reps);
+    APHIP_API_END
+}
+
 }  // extern "C"

@@ -18,28 +18,62 @@ namespace aphip {
 
 #define APHIP_DEV __device__ __forceinline__
 
-// det(I + A) - 1 by principal invariants (MathUtils.h:333-339); A row-major.
+// Rounding is fixed by the source: every device function of the force math turns
+// implicit multiply-add contraction off and writes its fused multiply-adds
+// explicitly, so an element gives bit-identical results in every kernel that
+// inlines it (the compiler's fusion choices otherwise depend on the surrounding
+// code).
+
+// a0 b0 + a1 b1 + a2 b2, accumulated left to right with fused multiply-adds
+APHIP_DEV double dot3(double a0, double b0, double a1, double b1, double a2, double b2) {
+#pragma clang fp contract(off)
+    return __fma_rn(a2, b2, __fma_rn(a1, b1, a0 * b0));
+}
+
+APHIP_DEV double dot4(double a0, double b0, double a1, double b1, double a2, double b2, double a3, double b3) {
+#pragma clang fp contract(off)
+    return __fma_rn(a3, b3, __fma_rn(a2, b2, __fma_rn(a1, b1, a0 * b0)));
+}
+
+// det(I + A) - 1 by principal invariants (MathUtils.h:333-339); A row-major.  The
+// terms are accumulated in the reference's order.
 APHIP_DEV double det_ap_im1(const double *A) {
-    return A[0] + A[4] + A[8] - A[1] * A[3] * (1 + A[8]) + A[0] * A[4] * (1 + A[8]) - A[2] * A[6] * (1 + A[4]) -
-           A[5] * A[7] * (1 + A[0]) + A[0] * A[8] + A[4] * A[8] + A[1] * A[5] * A[6] + A[2] * A[3] * A[7];
+#pragma clang fp contract(off)
+    double s = A[0] + A[4] + A[8];
+    s = __fma_rn(-(A[1] * A[3]), 1.0 + A[8], s);
+    s = __fma_rn(A[0] * A[4], 1.0 + A[8], s);
+    s = __fma_rn(-(A[2] * A[6]), 1.0 + A[4], s);
+    s = __fma_rn(-(A[5] * A[7]), 1.0 + A[0], s);
+    s = __fma_rn(A[0], A[8], s);
+    s = __fma_rn(A[4], A[8], s);
+    s = __fma_rn(A[1] * A[5], A[6], s);
+    s = __fma_rn(A[2] * A[3], A[7], s);
+    return s;
+}
+
+// a b - c d with one rounding of the product difference's second term
+APHIP_DEV double dif2(double a, double b, double c, double d) {
+#pragma clang fp contract(off)
+    return __fma_rn(a, b, -(c * d));
 }
 
 // Cofactor inverse of a row-major 3x3; returns det.
 APHIP_DEV double inverse3(const double *A, double *Ai) {
-    const double c00 = A[4] * A[8] - A[5] * A[7];
-    const double c10 = A[5] * A[6] - A[3] * A[8];
-    const double c20 = A[3] * A[7] - A[4] * A[6];
-    const double det = A[0] * c00 + A[1] * c10 + A[2] * c20;
+#pragma clang fp contract(off)
+    const double c00 = dif2(A[4], A[8], A[5], A[7]);
+    const double c10 = dif2(A[5], A[6], A[3], A[8]);
+    const double c20 = dif2(A[3], A[7], A[4], A[6]);
+    const double det = dot3(A[0], c00, A[1], c10, A[2], c20);
     const double id = 1.0 / det;
     Ai[0] = c00 * id;
-    Ai[1] = (A[2] * A[7] - A[1] * A[8]) * id;
-    Ai[2] = (A[1] * A[5] - A[2] * A[4]) * id;
+    Ai[1] = dif2(A[2], A[7], A[1], A[8]) * id;
+    Ai[2] = dif2(A[1], A[5], A[2], A[4]) * id;
     Ai[3] = c10 * id;
-    Ai[4] = (A[0] * A[8] - A[2] * A[6]) * id;
-    Ai[5] = (A[2] * A[3] - A[0] * A[5]) * id;
+    Ai[4] = dif2(A[0], A[8], A[2], A[6]) * id;
+    Ai[5] = dif2(A[2], A[3], A[0], A[5]) * id;
     Ai[6] = c20 * id;
-    Ai[7] = (A[1] * A[6] - A[0] * A[7]) * id;
-    Ai[8] = (A[0] * A[4] - A[1] * A[3]) * id;
+    Ai[7] = dif2(A[1], A[6], A[0], A[7]) * id;
+    Ai[8] = dif2(A[0], A[4], A[1], A[3]) * id;
     return det;
 }
 
@@ -50,6 +84,7 @@ struct MatParams {
 // P = stress(H); sn / snp1: J2 equivalent plastic strain in / out.
 template <int KIND>
 APHIP_DEV void stress(const double *H, const MatParams &m, double sn, double &snp1, double *P) {
+#pragma clang fp contract(off)
     if constexpr (KIND == APHIP_LINEAR_ELASTIC) {
         const double two_mu = m.mu * 2.0;
         double eps[9];
@@ -61,7 +96,7 @@ APHIP_DEV void stress(const double *H, const MatParams &m, double sn, double &sn
 #pragma unroll
         for (int i = 0; i < 9; ++i) P[i] = two_mu * eps[i];
 #pragma unroll
-        for (int i = 0; i < 3; ++i) P[4 * i] = m.lambda * tr + two_mu * eps[4 * i];
+        for (int i = 0; i < 3; ++i) P[4 * i] = __fma_rn(m.lambda, tr, two_mu * eps[4 * i]);
         snp1 = sn;
     } else if constexpr (KIND == APHIP_NEO_HOOKEAN) {
         double tau[9], F[9], Fi[9];
@@ -69,12 +104,12 @@ APHIP_DEV void stress(const double *H, const MatParams &m, double sn, double &sn
         for (int i = 0; i < 3; ++i)
 #pragma unroll
             for (int j = 0; j < 3; ++j) {
-                double hh = H[3 * i] * H[3 * j] + H[3 * i + 1] * H[3 * j + 1] + H[3 * i + 2] * H[3 * j + 2];
+                const double hh = dot3(H[3 * i], H[3 * j], H[3 * i + 1], H[3 * j + 1], H[3 * i + 2], H[3 * j + 2]);
                 tau[3 * i + j] = m.mu * (hh + H[3 * j + i] + H[3 * i + j]);
             }
         const double lnJ = log1p(det_ap_im1(H));
 #pragma unroll
-        for (int i = 0; i < 3; ++i) tau[4 * i] += m.lambda * lnJ;
+        for (int i = 0; i < 3; ++i) tau[4 * i] = __fma_rn(m.lambda, lnJ, tau[4 * i]);
 #pragma unroll
         for (int i = 0; i < 9; ++i) F[i] = H[i];
         F[0] += 1.0;
@@ -85,7 +120,7 @@ APHIP_DEV void stress(const double *H, const MatParams &m, double sn, double &sn
         for (int i = 0; i < 3; ++i)
 #pragma unroll
             for (int j = 0; j < 3; ++j)
-                P[3 * i + j] = tau[3 * i] * Fi[3 * j] + tau[3 * i + 1] * Fi[3 * j + 1] + tau[3 * i + 2] * Fi[3 * j + 2];
+                P[3 * i + j] = dot3(tau[3 * i], Fi[3 * j], tau[3 * i + 1], Fi[3 * j + 1], tau[3 * i + 2], Fi[3 * j + 2]);
         snp1 = sn;
     } else if constexpr (KIND == APHIP_J2) {
         double ep = sn;
@@ -95,7 +130,7 @@ APHIP_DEV void stress(const double *H, const MatParams &m, double sn, double &sn
 #pragma unroll
             for (int j = 0; j < 3; ++j) dev[3 * i + j] = 0.5 * (H[3 * i + j] + H[3 * j + i]);
         const double tr = dev[0] + dev[4] + dev[8];
-        const double K = m.lambda + 2.0 * m.mu / 3.0;
+        const double K = m.lambda + 2.0 * m.mu / 3.0;  // (no product to fuse: 2 mu / 3 rounds first)
         const double p = K * tr;
 #pragma unroll
         for (int i = 0; i < 3; ++i) dev[4 * i] -= (1.0 / 3.0) * tr;
@@ -103,16 +138,16 @@ APHIP_DEV void stress(const double *H, const MatParams &m, double sn, double &sn
 #pragma unroll
         for (int i = 0; i < 9; ++i) {
             dev[i] *= 2.0 * m.mu;
-            n2 += dev[i] * dev[i];
+            n2 = __fma_rn(dev[i], dev[i], n2);
         }
         const double eta_norm = sqrt(n2);
         const double q = sqrt(3.0 / 2.0) * eta_norm;
-        const double phi = q - (m.yield_stress + m.hardening * ep);
+        const double phi = q - __fma_rn(m.hardening, ep, m.yield_stress);
         if (phi > 0.0) {
             const double dep = phi / (3.0 * m.mu + m.hardening);
             const double s = sqrt(6.0) * m.mu * dep / eta_norm;
 #pragma unroll
-            for (int i = 0; i < 9; ++i) dev[i] -= s * dev[i];
+            for (int i = 0; i < 9; ++i) dev[i] = __fma_rn(-s, dev[i], dev[i]);
             ep += dep;
         }
         snp1 = ep;
@@ -127,20 +162,20 @@ APHIP_DEV void stress(const double *H, const MatParams &m, double sn, double &sn
         for (int i = 0; i < 3; ++i)
 #pragma unroll
             for (int j = 0; j < 3; ++j) {
-                const double hth = H[i] * H[j] + H[3 + i] * H[3 + j] + H[6 + i] * H[6 + j];
+                const double hth = dot3(H[i], H[j], H[3 + i], H[3 + j], H[6 + i], H[6 + j]);
                 E[3 * i + j] = 0.5 * (H[3 * i + j] + H[3 * j + i] + hth);
             }
         const double tr = E[0] + E[4] + E[8];
 #pragma unroll
         for (int i = 0; i < 9; ++i) S[i] = two_mu * E[i];
 #pragma unroll
-        for (int i = 0; i < 3; ++i) S[4 * i] += m.lambda * tr;
+        for (int i = 0; i < 3; ++i) S[4 * i] = __fma_rn(m.lambda, tr, S[4 * i]);
 #pragma unroll
         for (int i = 0; i < 3; ++i)
 #pragma unroll
             for (int j = 0; j < 3; ++j)
-                P[3 * i + j] = S[j] * ((i == 0 ? 1.0 : 0.0) + H[3 * i]) + S[3 + j] * ((i == 1 ? 1.0 : 0.0) + H[3 * i + 1]) +
-                               S[6 + j] * ((i == 2 ? 1.0 : 0.0) + H[3 * i + 2]);
+                P[3 * i + j] = dot3(S[j], (i == 0 ? 1.0 : 0.0) + H[3 * i], S[3 + j], (i == 1 ? 1.0 : 0.0) + H[3 * i + 1],
+                                    S[6 + j], (i == 2 ? 1.0 : 0.0) + H[3 * i + 2]);
         snp1 = sn;
     }
 }

@@ -20,6 +20,7 @@ namespace aphip {
 //   corrector  a = f/m, acceleration BC, v = v1/2 + dt/2 a       (ExplicitTimeIntegrator.h:267-276, :308-318)
 APHIP_DEV void cd_predict(const double *u, const double *v, const double *a, unsigned fixed, double dt, double *vh,
                           double *u1) {
+#pragma clang fp contract(off)
     const double h = 0.5 * dt;
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
@@ -32,6 +33,7 @@ APHIP_DEV void cd_predict(const double *u, const double *v, const double *a, uns
 
 APHIP_DEV void cd_correct(const double *f, double m, unsigned fixed, double dt, const double *vh, double *a1,
                           double *v1) {
+#pragma clang fp contract(off)
     const double h = 0.5 * dt;
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
@@ -46,6 +48,7 @@ APHIP_DEV void cd_correct(const double *f, double m, unsigned fixed, double dt, 
 template <int KIND>
 APHIP_DEV void tet4_element(const double (&X)[4][3], const double (&U)[4][3], const MatParams &m, double sn,
                             double &snp1, double *H, double *P, double (&fe)[4][3]) {
+#pragma clang fp contract(off)
     double J[9], Ji[9];
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
@@ -67,13 +70,13 @@ APHIP_DEV void tet4_element(const double (&X)[4][3], const double (&U)[4][3], co
     for (int i = 0; i < 3; ++i)
 #pragma unroll
         for (int j = 0; j < 3; ++j)
-            H[3 * i + j] = U[0][i] * B[0][j] + U[1][i] * B[1][j] + U[2][i] * B[2][j] + U[3][i] * B[3][j];
+            H[3 * i + j] = dot4(U[0][i], B[0][j], U[1][i], B[1][j], U[2][i], B[2][j], U[3][i], B[3][j]);
     stress<KIND>(H, m, sn, snp1, P);
 #pragma unroll
     for (int k = 0; k < 4; ++k)
 #pragma unroll
         for (int j = 0; j < 3; ++j)
-            fe[k][j] = -((B[k][0] * P[3 * j] + B[k][1] * P[3 * j + 1] + B[k][2] * P[3 * j + 2]) * w);
+            fe[k][j] = -(dot3(B[k][0], P[3 * j], B[k][1], P[3 * j + 1], B[k][2], P[3 * j + 2]) * w);
 }
 
 APHIP_DEV void hex8_sign(int k, int &s0, int &s1, int &s2) {
@@ -86,6 +89,7 @@ APHIP_DEV void hex8_sign(int k, int &s0, int &s1, int &s2) {
 template <int KIND, bool PER_QP>
 APHIP_DEV void hex8_element(const double (&X)[8][3], const double (&U)[8][3], const MatParams &m, double ga,
                             const double *state_n, double *state_np1, double *H, double *P, double (&fe)[8][3]) {
+#pragma clang fp contract(off)
 #pragma unroll
     for (int k = 0; k < 8; ++k) fe[k][0] = fe[k][1] = fe[k][2] = 0.0;
     double sn = 0.0;
@@ -111,8 +115,8 @@ APHIP_DEV void hex8_element(const double (&X)[8][3], const double (&U)[8][3], co
             for (int i = 0; i < 3; ++i)
 #pragma unroll
                 for (int d = 0; d < 3; ++d) {
-                    J[3 * i + d] += X[k][i] * dN[k][d];
-                    G[3 * i + d] += U[k][i] * dN[k][d];
+                    J[3 * i + d] = __fma_rn(X[k][i], dN[k][d], J[3 * i + d]);
+                    G[3 * i + d] = __fma_rn(U[k][i], dN[k][d], G[3 * i + d]);
                 }
         }
         double Ji[9];
@@ -121,7 +125,7 @@ APHIP_DEV void hex8_element(const double (&X)[8][3], const double (&U)[8][3], co
         for (int i = 0; i < 3; ++i)
 #pragma unroll
             for (int j = 0; j < 3; ++j)
-                H[3 * i + j] = G[3 * i] * Ji[j] + G[3 * i + 1] * Ji[3 + j] + G[3 * i + 2] * Ji[6 + j];
+                H[3 * i + j] = dot3(G[3 * i], Ji[j], G[3 * i + 1], Ji[3 + j], G[3 * i + 2], Ji[6 + j]);
         double snp1 = 0.0;
         if constexpr (KIND == APHIP_J2 && PER_QP) sn = state_n[q];
         stress<KIND>(H, m, sn, snp1, P);
@@ -134,11 +138,11 @@ APHIP_DEV void hex8_element(const double (&X)[8][3], const double (&U)[8][3], co
         for (int n = 0; n < 3; ++n)
 #pragma unroll
             for (int j = 0; j < 3; ++j)
-                Q[3 * n + j] = w * (Ji[3 * n] * P[3 * j] + Ji[3 * n + 1] * P[3 * j + 1] + Ji[3 * n + 2] * P[3 * j + 2]);
+                Q[3 * n + j] = w * dot3(Ji[3 * n], P[3 * j], Ji[3 * n + 1], P[3 * j + 1], Ji[3 * n + 2], P[3 * j + 2]);
 #pragma unroll
         for (int k = 0; k < 8; ++k)
 #pragma unroll
-            for (int j = 0; j < 3; ++j) fe[k][j] -= dN[k][0] * Q[j] + dN[k][1] * Q[3 + j] + dN[k][2] * Q[6 + j];
+            for (int j = 0; j < 3; ++j) fe[k][j] -= dot3(dN[k][0], Q[j], dN[k][1], Q[3 + j], dN[k][2], Q[6 + j]);
     }
 }
 
@@ -155,6 +159,7 @@ template <int KIND, bool PER_QP>
 APHIP_DEV void hex8_qp_lane(int q, const int (&lc)[8], const double *sX, const double *sU, const MatParams &m,
                             double ga, const double *state_n, double *state_np1, double *H, double *P,
                             double (&fk)[3]) {
+#pragma clang fp contract(off)
     int sq0, sq1, sq2;
     hex8_sign(q, sq0, sq1, sq2);
     const double p0 = sq0 * ga, p1 = sq1 * ga, p2 = sq2 * ga;
@@ -181,8 +186,8 @@ APHIP_DEV void hex8_qp_lane(int q, const int (&lc)[8], const double *sX, const d
             const double xi = x[i], ui = u[i];
 #pragma unroll
             for (int dd = 0; dd < 3; ++dd) {
-                J[3 * i + dd] += xi * d[dd];
-                G[3 * i + dd] += ui * d[dd];
+                J[3 * i + dd] = __fma_rn(xi, d[dd], J[3 * i + dd]);
+                G[3 * i + dd] = __fma_rn(ui, d[dd], G[3 * i + dd]);
             }
         }
     }
@@ -192,7 +197,7 @@ APHIP_DEV void hex8_qp_lane(int q, const int (&lc)[8], const double *sX, const d
     for (int i = 0; i < 3; ++i)
 #pragma unroll
         for (int j = 0; j < 3; ++j)
-            H[3 * i + j] = G[3 * i] * Ji[j] + G[3 * i + 1] * Ji[3 + j] + G[3 * i + 2] * Ji[6 + j];
+            H[3 * i + j] = dot3(G[3 * i], Ji[j], G[3 * i + 1], Ji[3 + j], G[3 * i + 2], Ji[6 + j]);
     double sn = 0.0, snp1 = 0.0;
     if constexpr (KIND == APHIP_J2) sn = PER_QP ? state_n[q] : state_n[0];
     stress<KIND>(H, m, sn, snp1, P);
@@ -205,7 +210,7 @@ APHIP_DEV void hex8_qp_lane(int q, const int (&lc)[8], const double *sX, const d
     for (int n = 0; n < 3; ++n)
 #pragma unroll
         for (int j = 0; j < 3; ++j)
-            Q[3 * n + j] = w * (Ji[3 * n] * P[3 * j] + Ji[3 * n + 1] * P[3 * j + 1] + Ji[3 * n + 2] * P[3 * j + 2]);
+            Q[3 * n + j] = w * dot3(Ji[3 * n], P[3 * j], Ji[3 * n + 1], P[3 * j + 1], Ji[3 * n + 2], P[3 * j + 2]);
     // reduce-scatter over the 8 lanes: at each step keep the half of the nodes whose
     // index bit matches this lane's bit and add the partner's copy of that half
     const bool b2 = (q >> 2) & 1, b1 = (q >> 1) & 1, b0 = q & 1;
@@ -217,8 +222,8 @@ APHIP_DEV void hex8_qp_lane(int q, const int (&lc)[8], const double *sX, const d
         dn(k + 4, dh);
 #pragma unroll
         for (int j = 0; j < 3; ++j) {
-            const double cl = -(dl[0] * Q[j] + dl[1] * Q[3 + j] + dl[2] * Q[6 + j]);
-            const double ch = -(dh[0] * Q[j] + dh[1] * Q[3 + j] + dh[2] * Q[6 + j]);
+            const double cl = -dot3(dl[0], Q[j], dl[1], Q[3 + j], dl[2], Q[6 + j]);
+            const double ch = -dot3(dh[0], Q[j], dh[1], Q[3 + j], dh[2], Q[6 + j]);
             const double keep = b2 ? ch : cl;
             const double send = b2 ? cl : ch;
             r4[k][j] = keep + __shfl_xor(send, 4, 8);

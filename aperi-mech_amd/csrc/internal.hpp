@@ -134,7 +134,10 @@ struct TilePlan {
     int64_t nb = 0;                   // shared nodes (finished by the second launch)
     int64_t n_interior = 0;           // first shared node id
     int64_t total_tile_nodes = 0;
+    int64_t nslot = 0;                // shared-node slots (tile sums parked for the second launch)
     size_t lds_bytes = 0;
+    int stream_grid[4] = {0, 0, 0, 0};  // persistent Tet4 step grid per material kind (0 = not sized yet)
+    bool use_stream = true;            // persistent pipelined Tet4 step when the plan allows it
     DevBuf<int32_t> tn_start;         // [ntiles + 1] offsets into the tile-node arrays
     DevBuf<int32_t> tile_e0;          // [ntiles + 1] first element of each tile
     DevBuf<int32_t> tn_ninterior;     // [ntiles] interior nodes come first in each tile
@@ -146,13 +149,17 @@ struct TilePlan {
     DevBuf<uint16_t> lconn;           // [ne * npe] local node of each element slot
     DevBuf<double> bpart;             // [slots * 3] tile sums of shared nodes
     DevBuf<int32_t> bstart;           // [nb + 1] slot range of each shared node
+    DevBuf<double> zero;              // [8] zeros: stands in for absent optional arrays
+    DevBuf<int32_t> meta;             // [ntiles * 8] packed per-tile scalars of the pipelined step
     void reset() {
         ready = false;
         te = threads = maxtn = 0;
-        ntiles = nb = n_interior = total_tile_nodes = 0;
+        ntiles = nb = n_interior = total_tile_nodes = nslot = 0;
         lds_bytes = 0;
+        for (int &g : stream_grid) g = 0;
         tn_start.release(); tile_e0.release(); tn_ninterior.release(); int_base.release(); tn_node.release(); tn_csr.release();
         tn_bdst.release(); lcsr.release(); lconn.release(); bpart.release(); bstart.release();
+        zero.release(); meta.release();
     }
 };
 
@@ -268,8 +275,11 @@ void rk_cell_force(Ctx &c);
 void rk_node_update(Ctx &c, double dt, int mode);
 void rk_mass(Ctx &c, double *total);
 
-// ---- profiling helpers (capi.cpp) ----
+// ---- profiling helpers (capi.hip) ----
 void prof_begin(Ctx &c, int which);
 void prof_end(Ctx &c, int which);
+
+// ---- copy yardstick (probe.hip): device milliseconds per copy of nbytes ----
+double bandwidth_probe(int device, int64_t nbytes, int reps);
 
 }  // namespace aphip

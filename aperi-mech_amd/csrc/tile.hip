@@ -70,15 +70,18 @@ struct TileArgs {
     MatParams mat;
     const double *elem_lambda, *elem_mu;
     const double *node_dt;  // per-node increment (ensembles) or null
+    const double *zero;     // 8 zero doubles: stands in for an absent optional array
+    const int32_t *meta;    // [ntiles * 8] packed per-tile scalars (TileMeta)
     double gauss_a;
     int64_t ne, ntiles, nblocks;
+    int64_t nn, nslot;      // nodes; shared-node slots (bpart holds 3 doubles each)
     int te, maxtn;
     double dt;
 };
 
 // LDS carve-up (bytes), sized on the host from maxtn and te.
 struct TileLds {
-    double *X, *U, *V, *F, *M;
+    double *X, *U, *V, *F, *M, *DT;
     uint32_t *csr;
     int32_t *bdst;
     uint16_t *ent;
@@ -91,12 +94,37 @@ __device__ __forceinline__ TileLds carve(char *base, int maxtn, int te, int npe)
     L.U = L.X + 3 * maxtn;
     L.V = L.U + 3 * maxtn;
     L.M = L.V + 3 * maxtn;
-    L.F = L.M + maxtn;
+    L.DT = L.M + maxtn;
+    L.F = L.DT + maxtn;
     L.csr = reinterpret_cast<uint32_t *>(L.F + 3 * te * npe);
     L.bdst = reinterpret_cast<int32_t *>(L.csr + maxtn);
     L.ent = reinterpret_cast<uint16_t *>(L.bdst + maxtn);
     L.fx = reinterpret_cast<uint8_t *>(L.ent + te * npe);
     return L;
+}
+
+// Optional arrays are read without branches: an absent array is replaced by the
+// zero buffer at index 0 (a uniform select), and the value is then selected.  A
+// load under a branch ends a basic block, and at the join the compiler waits for
+// every outstanding memory operation of the wave, stores included.
+__device__ __forceinline__ unsigned load_fixed(const TileArgs &A, int64_t g) {
+    const uint8_t *p = A.fixed ? A.fixed : reinterpret_cast<const uint8_t *>(A.zero);
+    return p[A.fixed ? g : 0];
+}
+
+__device__ __forceinline__ double load_node_dt(const TileArgs &A, int64_t g) {
+    const double *p = A.node_dt ? A.node_dt : A.zero;
+    const double d = p[A.node_dt ? g : 0];
+    return A.node_dt ? d : A.dt;
+}
+
+__device__ __forceinline__ void load_elem_lame(const TileArgs &A, int64_t e, MatParams &m) {
+    const double *pl = A.elem_lambda ? A.elem_lambda : A.zero;
+    const double *pm = A.elem_lambda ? A.elem_mu : A.zero;
+    const int64_t i = A.elem_lambda ? e : 0;
+    const double l = pl[i], mu = pm[i];
+    m.lambda = A.elem_lambda ? l : m.lambda;
+    m.mu = A.elem_lambda ? mu : m.mu;
 }
 
 // MODE 0: force only; 1: force + a = f/m (initial); 2: full central-difference step.
@@ -119,52 +147,60 @@ __global__ __launch_bounds__(NPE == 4 ? 512 : (QPAR ? 512 : 128)) void k_tile(Ti
     const int le = QPAR ? (tid >> 3) : tid;
     const int64_t e = te0 + le;
     const bool has_elem = le < nel;
+    const int64_t ec = te0 + min(le, nel - 1);  // clamped: the loads below need no branch
     int lc[NPE];
     MatParams m = A.mat;
     double sn = 0.0;
-    if (has_elem) {
-        if constexpr (NPE == 4) {
-            const ushort4 c = reinterpret_cast<const ushort4 *>(A.lconn)[e];
-            lc[0] = c.x; lc[1] = c.y; lc[2] = c.z; lc[3] = c.w;
-        } else {
-            const ushort4 c0 = reinterpret_cast<const ushort4 *>(A.lconn)[2 * e];
-            const ushort4 c1 = reinterpret_cast<const ushort4 *>(A.lconn)[2 * e + 1];
-            lc[0] = c0.x; lc[1] = c0.y; lc[2] = c0.z; lc[3] = c0.w;
-            lc[4] = c1.x; lc[5] = c1.y; lc[6] = c1.z; lc[7] = c1.w;
-        }
-        if (A.elem_lambda) {
-            m.lambda = A.elem_lambda[e];
-            m.mu = A.elem_mu[e];
-        }
-        if constexpr (KIND == APHIP_J2 && !PER_QP && !QPAR) sn = A.state_n[e];
+    if constexpr (NPE == 4) {
+        const ushort4 c = reinterpret_cast<const ushort4 *>(A.lconn)[ec];
+        lc[0] = c.x; lc[1] = c.y; lc[2] = c.z; lc[3] = c.w;
+    } else {
+        const ushort4 c0 = reinterpret_cast<const ushort4 *>(A.lconn)[2 * ec];
+        const ushort4 c1 = reinterpret_cast<const ushort4 *>(A.lconn)[2 * ec + 1];
+        lc[0] = c0.x; lc[1] = c0.y; lc[2] = c0.z; lc[3] = c0.w;
+        lc[4] = c1.x; lc[5] = c1.y; lc[6] = c1.z; lc[7] = c1.w;
     }
+    load_elem_lame(A, ec, m);
+    if constexpr (KIND == APHIP_J2 && !PER_QP && !QPAR) sn = A.state_n[ec];
 
     // A: tile CSR entries and node data into LDS
     {
         const int nent = A.te * NPE;  // entries of a full tile; a partial tile reads zeros past its end
         const uint32_t *src = reinterpret_cast<const uint32_t *>(A.lcsr + t * (int64_t)nent);
         uint32_t *dst = reinterpret_cast<uint32_t *>(L.ent);
-        for (int i = tid; i < nent / 2; i += blockDim.x) dst[i] = src[i];
+        if (nent / 2 == 2 * (int)blockDim.x) {  // one element per thread, Tet4: two words each
+            const uint32_t w0 = src[tid], w1 = src[tid + blockDim.x];
+            dst[tid] = w0;
+            dst[tid + blockDim.x] = w1;
+        } else {
+            for (int i = tid; i < nent / 2; i += blockDim.x) dst[i] = src[i];
+        }
     }
     for (int ln = tid; ln < tn; ln += blockDim.x) {
-        const bool interior = ln < nint;
-        const int64_t g = interior ? ibase + ln : (int64_t)A.tn_node[tb + ln];
-        L.csr[ln] = A.tn_csr[tb + ln];
-        if (!interior) L.bdst[ln] = A.tn_bdst[tb + ln];
+        // all loads unconditional (tn_node also lists the interior nodes, as ibase + ln)
+        const int64_t g = A.tn_node[tb + ln];
+        const uint32_t csr = A.tn_csr[tb + ln];
+        const int32_t bdst = A.tn_bdst[tb + ln];
         double x[3], u[3];
         ld3(A.X, g, x);
         ld3(A.u, g, u);
-        st3(L.X, ln, x);
-        const unsigned fx = A.fixed ? A.fixed[g] : 0u;
-        L.fx[ln] = (uint8_t)fx;
-        if constexpr (MODE >= 1) {
-            if (interior) L.M[ln] = A.mass[g];
-        }
+        const unsigned fx = load_fixed(A, g);
+        double mm = 1.0, v[3], a[3], nd = A.dt;
+        if constexpr (MODE >= 1) mm = A.mass[g];
         if constexpr (MODE == 2) {
-            double v[3], a[3], vh[3], u1[3];
             ld3(A.v, g, v);
             ld3(A.a, g, a);
-            cd_predict(u, v, a, fx, A.node_dt ? A.node_dt[g] : A.dt, vh, u1);
+            nd = load_node_dt(A, g);
+        }
+        L.csr[ln] = csr;
+        L.bdst[ln] = bdst;
+        st3(L.X, ln, x);
+        L.fx[ln] = (uint8_t)fx;
+        L.M[ln] = mm;
+        L.DT[ln] = nd;
+        if constexpr (MODE == 2) {
+            double vh[3], u1[3];
+            cd_predict(u, v, a, fx, nd, vh, u1);
             st3(L.U, ln, u1);
             st3(L.V, ln, vh);
         } else {
@@ -239,7 +275,7 @@ __global__ __launch_bounds__(NPE == 4 ? 512 : (QPAR ? 512 : 128)) void k_tile(Ti
                 double vh[3], u1[3], a1[3], v1[3];
                 ld3(L.V, ln, vh);
                 ld3(L.U, ln, u1);
-                cd_correct(f, L.M[ln], L.fx[ln], A.node_dt ? A.node_dt[g] : A.dt, vh, a1, v1);
+                cd_correct(f, L.M[ln], L.fx[ln], L.DT[ln], vh, a1, v1);
                 st3(A.u, g, u1);
                 st3(A.v, g, v1);
                 st3(A.a, g, a1);
@@ -248,6 +284,297 @@ __global__ __launch_bounds__(NPE == 4 ? 512 : (QPAR ? 512 : 128)) void k_tile(Ti
             st3(A.bpart, L.bdst[ln], f);
         }
     }
+}
+
+// ---- persistent, software-pipelined Tet4 step -------------------------------
+// Same arithmetic and summation order as k_tile<4, KIND, false, 2, false>, so the
+// results are bitwise identical.  A block owns the tile sequence
+//   tbeg(xcd) + kx, + bx, + 2 bx, ...   (xcd = block % 8, kx = block / 8)
+// inside the Morton range of its XCD and overlaps the global loads of the next
+// tiles with the work of the current one: while tile j is evaluated, the node
+// data of tile j+1, the node slots of tile j+2 and the scalars of tile j+3 are in
+// flight.  Node staging is double-buffered in LDS and tile j+1 is committed to the
+// idle buffer after tile j's element arithmetic but before tile j's H / P stores.
+// gfx9 counts loads and stores on one in-order counter, so this order lets the
+// wait for the prefetched data leave the fresh stores in flight.  Every global
+// access in the loop is unconditional (idle lanes use clamped loads and dropped
+// buffer stores): a memory instruction under a branch would make the compiler's
+// wait counts conservative at the join, and a conservative count drains the stores.
+// Requires one element per thread and at most one tile node per thread.
+
+// Per-tile scalars, packed so that one 32-byte load brings all of them.
+struct TileMeta {
+    int tb, tn, nint, nel;  // first tile-node slot, tile nodes, interior nodes, elements
+    int ibase, e0, pad0, pad1;  // first interior node id, first element
+};
+
+__device__ __forceinline__ TileMeta load_meta(const TileArgs &A, int64_t t) {
+    const int4 *p = reinterpret_cast<const int4 *>(A.meta + 8 * t);
+    const int4 a = p[0], b = p[1];
+    return TileMeta{a.x, a.y, a.z, a.w, b.x, b.y, 0, 0};
+}
+
+// Raw buffer stores (stride 0): a store whose offset lies past the resource's
+// num_records is discarded by the hardware's range check, which turns a
+// per-lane condition into an unconditional instruction with no memory traffic.
+typedef unsigned int u32x2_t __attribute__((ext_vector_type(2)));
+constexpr uint32_t kDropOffset = 0x80000000u;  // beyond every resource (all are < 2 GiB)
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer_rsrc(const void *p, int64_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, (int)bytes, 0x00020000);
+}
+
+__device__ __forceinline__ void bstore(__amdgpu_buffer_rsrc_t r, uint32_t off, double v) {
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2_t, v), r, off, 0, 0);
+}
+
+__device__ __forceinline__ void bstore3(__amdgpu_buffer_rsrc_t r, uint32_t off, const double *v) {
+    bstore(r, off, v[0]);
+    bstore(r, off + 8, v[1]);
+    bstore(r, off + 16, v[2]);
+}
+
+struct StageIdx {  // per-thread node slot of a tile
+    int g;
+    uint32_t csr;
+    int bdst;
+};
+
+struct StageData {  // one thread's share of a tile, from global memory
+    TileMeta mt;
+    uint32_t csr;
+    int bdst;
+    double x[3], u[3], v[3], a[3];
+    double m, ndt, lam, mu, sn;
+    unsigned fx;
+    ushort4 lc;
+    uint32_t w0, w1;
+};
+
+// One staging buffer: the tile's nodes and elements as the element and node phases read them.
+struct StageLds {
+    double *X, *U, *V, *M, *DT, *LAM, *MU, *SN;
+    uint32_t *csr;
+    int32_t *bdst;
+    ushort4 *lc;
+    uint32_t *ent;  // the tile's CSR entries (u16 pairs)
+    uint8_t *fx;
+};
+
+__host__ __device__ inline size_t stage_bytes(int te, int maxtn) {
+    const size_t d = 8 * (11 * (size_t)maxtn + 3 * (size_t)te);  // X U V M DT LAM MU SN
+    const size_t w = 4 * (2 * (size_t)maxtn) + 8 * (size_t)te + 8 * (size_t)te;  // csr bdst lc ent
+    return (d + w + (size_t)maxtn + 15) & ~(size_t)15;
+}
+
+__host__ __device__ inline size_t stream_lds_bytes(int te, int maxtn) {
+    return 96 * (size_t)te + 2 * stage_bytes(te, maxtn);
+}
+
+__device__ __forceinline__ StageLds stage_carve(char *base, int te, int maxtn) {
+    StageLds S;
+    S.X = reinterpret_cast<double *>(base);
+    S.U = S.X + 3 * maxtn;
+    S.V = S.U + 3 * maxtn;
+    S.M = S.V + 3 * maxtn;
+    S.DT = S.M + maxtn;
+    S.LAM = S.DT + maxtn;
+    S.MU = S.LAM + te;
+    S.SN = S.MU + te;
+    S.lc = reinterpret_cast<ushort4 *>(S.SN + te);
+    S.ent = reinterpret_cast<uint32_t *>(S.lc + te);
+    S.csr = S.ent + 2 * te;
+    S.bdst = reinterpret_cast<int32_t *>(S.csr + maxtn);
+    S.fx = reinterpret_cast<uint8_t *>(S.bdst + maxtn);
+    return S;
+}
+
+__device__ __forceinline__ void stage_idx(const TileArgs &A, const TileMeta &mt, int tid, StageIdx &s) {
+    const int slot = mt.tb + min(tid, mt.tn - 1);  // clamped: no branch around the loads
+    s.g = A.tn_node[slot];
+    s.csr = A.tn_csr[slot];
+    s.bdst = A.tn_bdst[slot];
+}
+
+// Optional per-node increment and per-element moduli are read from the zero buffer
+// when absent and added to the uniform value, so the loaded value is used on both
+// paths and the compiler keeps the loads unconditional.
+template <int KIND>
+__device__ __forceinline__ void stage_data(const TileArgs &A, int64_t t, const TileMeta &mt, int tid,
+                                           const StageIdx &si, StageData &d) {
+    d.mt = mt;
+    d.csr = si.csr;
+    d.bdst = si.bdst;
+    const int64_t g = si.g;
+    ld3(A.X, g, d.x);
+    ld3(A.u, g, d.u);
+    ld3(A.v, g, d.v);
+    ld3(A.a, g, d.a);
+    d.fx = load_fixed(A, g);
+    d.m = A.mass[g];
+    const double *pd = A.node_dt ? A.node_dt : A.zero;
+    d.ndt = pd[A.node_dt ? g : 0] + (A.node_dt ? 0.0 : A.dt);
+    const int64_t e = (int64_t)mt.e0 + min(tid, mt.nel - 1);
+    d.lc = reinterpret_cast<const ushort4 *>(A.lconn)[e];
+    const int64_t ie = A.elem_lambda ? e : 0;
+    const double *pl = A.elem_lambda ? A.elem_lambda : A.zero;
+    const double *pm = A.elem_lambda ? A.elem_mu : A.zero;
+    d.lam = pl[ie] + (A.elem_lambda ? 0.0 : A.mat.lambda);
+    d.mu = pm[ie] + (A.elem_lambda ? 0.0 : A.mat.mu);
+    d.sn = 0.0;
+    if constexpr (KIND == APHIP_J2) d.sn = A.state_n[e];
+    // the tile's CSR entries: te * 4 u16 = 2 words per thread
+    const uint32_t *src = reinterpret_cast<const uint32_t *>(A.lcsr + t * (int64_t)(A.te * 4));
+    d.w0 = src[tid];
+    d.w1 = src[tid + A.te];
+}
+
+// Writes one thread's share of a staged tile into an LDS buffer (predictor applied).
+__device__ __forceinline__ void stage_commit(const StageData &d, const StageLds &S, int tid, int te) {
+    if (tid < d.mt.tn) {
+        S.csr[tid] = d.csr;
+        S.bdst[tid] = d.bdst;
+        st3(S.X, tid, d.x);
+        S.fx[tid] = (uint8_t)d.fx;
+        S.M[tid] = d.m;
+        S.DT[tid] = d.ndt;
+        double vh[3], u1[3];
+        cd_predict(d.u, d.v, d.a, d.fx, d.ndt, vh, u1);
+        st3(S.U, tid, u1);
+        st3(S.V, tid, vh);
+    }
+    S.lc[tid] = d.lc;
+    S.LAM[tid] = d.lam;
+    S.MU[tid] = d.mu;
+    S.SN[tid] = d.sn;
+    S.ent[tid] = d.w0;
+    S.ent[tid + te] = d.w1;
+}
+
+template <int KIND>
+__global__ __launch_bounds__(256) void k_tile_stream(TileArgs A, int nblk) {
+    extern __shared__ __align__(16) char lds_raw[];
+    const int te = A.te;
+    double *F = reinterpret_cast<double *>(lds_raw);
+    const size_t sb = stage_bytes(te, A.maxtn);
+    const StageLds S0 = stage_carve(lds_raw + 96 * (size_t)te, te, A.maxtn);
+    const StageLds S1 = stage_carve(lds_raw + 96 * (size_t)te + sb, te, A.maxtn);
+    const int tid = threadIdx.x;
+    const int xcd = blockIdx.x % kNumXcd, kx = blockIdx.x / kNumXcd;
+    const int bx = nblk / kNumXcd;
+    const int64_t per = (A.ntiles + kNumXcd - 1) / kNumXcd;
+    const int64_t tend = min(A.ntiles, (int64_t)(xcd + 1) * per);
+    int64_t t = (int64_t)xcd * per + kx;
+    if (t >= tend) return;
+    // every prefetch is unconditional, on tile ids clamped to the block's range: a
+    // branch around a load would make the compiler wait for it at the join
+    const int64_t tlast = tend - 1;
+    const __amdgpu_buffer_rsrc_t rH = buffer_rsrc(A.H, 72 * A.ne), rP = buffer_rsrc(A.P, 72 * A.ne);
+    const __amdgpu_buffer_rsrc_t rS = buffer_rsrc(A.state_np1, KIND == APHIP_J2 ? 8 * A.ne : 0);
+    const __amdgpu_buffer_rsrc_t rF = buffer_rsrc(A.f, 24 * A.nn), rU = buffer_rsrc(A.u, 24 * A.nn);
+    const __amdgpu_buffer_rsrc_t rV = buffer_rsrc(A.v, 24 * A.nn), rA = buffer_rsrc(A.a, 24 * A.nn);
+    const __amdgpu_buffer_rsrc_t rB = buffer_rsrc(A.bpart, 24 * A.nslot);
+
+    StageIdx si;
+    StageData d;
+    {
+        const TileMeta m0 = load_meta(A, t);
+        stage_idx(A, m0, tid, si);
+        stage_data<KIND>(A, t, m0, tid, si, d);
+        stage_commit(d, S0, tid, te);
+    }
+    TileMeta m1 = load_meta(A, min(t + bx, tlast));
+    stage_idx(A, m1, tid, si);
+    TileMeta m2 = load_meta(A, min(t + 2 * bx, tlast));
+    TileMeta c = d.mt;  // the tile in the current buffer
+    bool odd = false;
+    __syncthreads();
+
+    // One tile; returns false after the block's last tile.  The first call is peeled
+    // off the loop so that the loop header merges two identical states and the
+    // compiler's wait counts for the in-flight prefetch stay exact.
+    auto tile = [&]() __attribute__((always_inline)) -> bool {
+        const StageLds &S = odd ? S1 : S0;
+        const StageLds &Sn = odd ? S0 : S1;
+        const int64_t t1 = t + bx;
+        // prefetch (data of t1, slots of t2, scalars of t3)
+        stage_data<KIND>(A, min(t1, tlast), m1, tid, si, d);
+        stage_idx(A, m2, tid, si);
+        m1 = m2;
+        m2 = load_meta(A, min(t + 3 * bx, tlast));
+        // keep the scheduler from sinking the prefetch towards its use
+        __builtin_amdgcn_sched_barrier(0);
+        // B: element (every lane computes; the stores of lanes past the last element are dropped)
+        const bool has_elem = tid < c.nel;
+        const int64_t e = (int64_t)c.e0 + tid;
+        {
+            const ushort4 q = S.lc[tid];
+            const int lc[4] = {q.x, q.y, q.z, q.w};
+            MatParams m = A.mat;
+            m.lambda = S.LAM[tid];
+            m.mu = S.MU[tid];
+            double X[4][3], U[4][3], fe[4][3];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                ld3(S.X, lc[k], X[k]);
+                ld3(S.U, lc[k], U[k]);
+            }
+            double H[9], P[9], snp1 = 0.0;
+            tet4_element<KIND>(X, U, m, S.SN[tid], snp1, H, P, fe);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) st3(F, tid * 4 + k, fe[k]);
+            // commit the prefetched tile before this tile's stores are issued
+            stage_commit(d, Sn, tid, te);
+            const uint32_t oe = has_elem ? (uint32_t)(8 * e) : kDropOffset;
+            const uint32_t oi = (uint32_t)(8 * A.ne);
+#pragma unroll
+            for (int i = 0; i < 9; ++i) {
+                bstore(rH, oe + i * oi, H[i]);
+                bstore(rP, oe + i * oi, P[i]);
+            }
+            if constexpr (KIND == APHIP_J2) bstore(rS, oe, snp1);
+        }
+        __syncthreads();
+
+        // C: gather and finish (same order as k_tile).  Stores are unconditional; those
+        // of lanes past the tile's nodes, and the u, v, a of a shared node (finished by
+        // the shared-node pass), are dropped.
+        {
+            const bool live = tid < c.tn, interior = tid < c.nint;
+            const int ln = min(tid, c.tn - 1);
+            const uint32_t pk = S.csr[ln];
+            const int st = (int)(pk >> 16), cnt = (int)(pk & 0xffffu);
+            const uint16_t *ent = reinterpret_cast<const uint16_t *>(S.ent);
+            double f[3] = {0.0, 0.0, 0.0};
+            for (int j = 0; j < cnt; ++j) {
+                const int s = ent[st + j];
+                f[0] += F[3 * s];
+                f[1] += F[3 * s + 1];
+                f[2] += F[3 * s + 2];
+            }
+            double vh[3], u1[3], a1[3], v1[3];
+            ld3(S.V, ln, vh);
+            ld3(S.U, ln, u1);
+            cd_correct(f, S.M[ln], S.fx[ln], S.DT[ln], vh, a1, v1);
+            const int64_t g = (int64_t)c.ibase + tid;
+            const uint32_t og = (live && interior) ? (uint32_t)(24 * g) : kDropOffset;
+            const uint32_t ob = (live && !interior) ? (uint32_t)(24 * (int64_t)S.bdst[ln]) : kDropOffset;
+            bstore3(rF, og, f);
+            bstore3(rB, ob, f);
+            bstore3(rU, og, u1);
+            bstore3(rV, og, v1);
+            bstore3(rA, og, a1);
+        }
+        if (t1 >= tend) return false;
+        c = d.mt;
+        odd = !odd;
+        t = t1;
+        __syncthreads();
+        return true;
+    };
+    if (tile())
+        while (tile()) {
+        }
 }
 
 struct BoundArgs {
@@ -313,8 +640,47 @@ void launch_tile(Ctx &c, const TileArgs &A, size_t lds) {
     hipLaunchKernelGGL(kern, dim3((unsigned)A.nblocks), dim3(c.tiles.threads), lds, c.stream, A);
 }
 
+template <int KIND>
+void launch_stream(Ctx &c, const TileArgs &A, size_t) {
+    auto kern = k_tile_stream<KIND>;
+    TilePlan &P = c.tiles;
+    const size_t lds = stream_lds_bytes(P.te, P.maxtn);
+    int &grid = P.stream_grid[KIND];
+    if (grid == 0) {
+        if (lds > 65536)
+            APHIP_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        int per_cu = 0, cus = 0;
+        APHIP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, P.threads, lds));
+        APHIP_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c.device));
+        const int64_t want = std::max<int64_t>(1, std::min<int64_t>((P.ntiles + kNumXcd - 1) / kNumXcd,
+                                                                   (int64_t)std::max(per_cu, 1) * cus / kNumXcd));
+        grid = (int)(want * kNumXcd);
+    }
+    hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(P.threads), lds, c.stream, A, grid);
+}
+
+bool stream_eligible(const Ctx &c) {
+    const TilePlan &P = c.tiles;
+    // buffer-store offsets are 32-bit and must stay below the drop offset (2 GiB)
+    const int64_t lim = (int64_t)1 << 31;
+    return P.use_stream && c.npe == 4 && P.te == P.threads && P.threads <= 256 && P.maxtn <= P.threads &&
+           stream_lds_bytes(P.te, P.maxtn) <= 160 * 1024 && 72 * c.ne < lim && 24 * c.nn < lim &&
+           24 * P.nslot < lim;
+}
+
 template <int NPE, int MODE>
 void dispatch_kind(Ctx &c, const TileArgs &A, size_t lds) {
+    if constexpr (NPE == 4 && MODE == 2) {
+        if (stream_eligible(c)) {
+            switch (c.mat.kind) {
+                case APHIP_LINEAR_ELASTIC: launch_stream<APHIP_LINEAR_ELASTIC>(c, A, lds); return;
+                case APHIP_NEO_HOOKEAN: launch_stream<APHIP_NEO_HOOKEAN>(c, A, lds); return;
+                case APHIP_J2: launch_stream<APHIP_J2>(c, A, lds); return;
+                case APHIP_SVK: launch_stream<APHIP_SVK>(c, A, lds); return;
+                default: break;
+            }
+        }
+    }
     const bool pq = c.mat.state_per_qp && NPE == 8;
     switch (c.mat.kind) {
         case APHIP_LINEAR_ELASTIC: launch_tile<NPE, APHIP_LINEAR_ELASTIC, false, MODE>(c, A, lds); break;
@@ -460,12 +826,29 @@ void tile_build(Ctx &c, int te, int threads) {
     P.tn_bdst.upload(tbdst.data(), tbdst.size(), c.stream);
     P.lcsr.upload(lcsr.data(), lcsr.size(), c.stream);
     P.lconn.upload(lconn.data(), lconn.size(), c.stream);
+    P.nslot = bstart[nb];
     P.bpart.alloc(3 * std::max<int64_t>(1, (int64_t)bstart[nb]));
     P.bstart.upload(bstart.data(), bstart.size(), c.stream);
-    // LDS: X, U, V (3 doubles each) + mass per node, partials, csr/bdst per node, entries, flags
-    P.lds_bytes = sizeof(double) * (10 * (size_t)maxtn + 3 * (size_t)te * npe) + 8 * (size_t)maxtn +
+    P.zero.alloc(8);
+    P.zero.zero(c.stream);
+    {
+        std::vector<int32_t> meta((size_t)nt * 8, 0);
+        for (int64_t t = 0; t < nt; ++t) {
+            int32_t *m = &meta[(size_t)t * 8];
+            m[0] = tstart[t];
+            m[1] = tstart[t + 1] - tstart[t];
+            m[2] = ninterior[t];
+            m[3] = te0v[t + 1] - te0v[t];
+            m[4] = int_base[t];
+            m[5] = te0v[t];
+        }
+        P.meta.upload(meta.data(), meta.size(), c.stream);
+    }
+    // LDS: X, U, V (3 doubles each) + mass + increment per node, partials, csr/bdst per node, entries, flags
+    P.lds_bytes = sizeof(double) * (11 * (size_t)maxtn + 3 * (size_t)te * npe) + 8 * (size_t)maxtn +
                   2 * (size_t)te * npe + (size_t)maxtn + 16;
     APHIP_REQUIRE(P.lds_bytes <= 160 * 1024, "tile needs more LDS than a CU has; use a smaller tile");
+    P.use_stream = !(c.flags & APHIP_FLAG_NO_PIPELINE);
     P.ready = true;
     APHIP_HIP(hipStreamSynchronize(c.stream));
 }
@@ -500,8 +883,12 @@ void tile_run(Ctx &c, double dt, int mode) {
     A.elem_lambda = c.has_fields ? c.elem_lambda.p : nullptr;
     A.elem_mu = c.has_fields ? c.elem_mu.p : nullptr;
     A.node_dt = c.node_dt.p;
+    A.zero = P.zero.p;
+    A.meta = P.meta.p;
     A.gauss_a = 1.0 / std::sqrt(3.0);
     A.ne = c.ne;
+    A.nn = c.nn;
+    A.nslot = P.nslot;
     A.ntiles = P.ntiles;
     A.nblocks = padded(P.ntiles);
     A.te = P.te;

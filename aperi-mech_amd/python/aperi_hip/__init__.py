@@ -24,6 +24,7 @@ LINEAR_ELASTIC, NEO_HOOKEAN, J2, SVK = 0, 1, 2, 3
 FIELD_U, FIELD_V, FIELD_A, FIELD_F, FIELD_MASS, FIELD_H, FIELD_P, FIELD_STATE, FIELD_VOLUME = range(9)
 MAX_NODE_NUM_NEIGHBORS = 40
 FLAG_TWO_PASS = 1
+FLAG_NO_PIPELINE = 2
 
 _d = ctypes.POINTER(ctypes.c_double)
 _i32 = ctypes.POINTER(ctypes.c_int32)
@@ -38,6 +39,7 @@ EXPORTED = [
     "aphip_batch_add", "aphip_batch_end", "aphip_batch_ranges", "aphip_internal_force_vjp",
     "aphip_adjoint_forward", "aphip_adjoint_backward", "aphip_rk_setup", "aphip_rk_get_neighbors", "aphip_rk_get_function_values",
     "aphip_rk_get_cells", "aphip_set_profiling", "aphip_get_profile", "aphip_reset_profile",
+    "aphip_bandwidth_probe",
 ]
 
 
@@ -84,6 +86,15 @@ def device_count():
     n = ctypes.c_int32()
     _check(L.aphip_device_count(ctypes.byref(n)))
     return n.value
+
+
+def bandwidth_probe(nbytes, reps=20, device=0):
+    """Device milliseconds per device-to-device copy of `nbytes` (multiple of 16)."""
+    L = load_library()
+    ms = ctypes.c_double()
+    _check(L.aphip_bandwidth_probe(ctypes.c_int32(device), ctypes.c_int64(nbytes), ctypes.c_int32(reps),
+                                   ctypes.byref(ms)))
+    return ms.value
 
 
 def _check(rc):

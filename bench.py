@@ -111,6 +111,8 @@ def main():
     ap.add_argument("--n", type=int, default=100, help="hexes per edge (x6 tets)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--material", default="nh", choices=["nh", "le"])
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
+                    help="only time a device copy of BYTES (HBM yardstick) and print it")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
@@ -118,6 +120,11 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     import aperi_hip as A
+    if args.copy_probe:
+        ms = A.bandwidth_probe(args.copy_probe, reps=max(args.steps, 1))
+        print(json.dumps({"copy_bytes": args.copy_probe, "ms_per_copy": ms,
+                          "GBps": 2 * args.copy_probe / (ms * 1e-3) / 1e9}), flush=True)
+        return
     kind = A.NEO_HOOKEAN if args.material == "nh" else A.LINEAR_ELASTIC
     t0 = time.time()
     w = workload(args.n)

@@ -61,6 +61,8 @@ enum {
 
 /* aphip_config.flags */
 #define APHIP_FLAG_TWO_PASS 1 /* element partials through HBM + node CSR gather (no LDS tiles) */
+#define APHIP_FLAG_NO_PIPELINE 2 /* one tile per workgroup instead of the persistent pipelined
+                                    Tet4 step (bitwise identical results) */
 
 typedef struct {
     int32_t device; /* HIP device ordinal */
@@ -228,6 +230,11 @@ int aphip_set_profiling(aphip_ctx *ctx, int32_t enable);
  * (0 = whole step, 1 = element/force kernel, 2 = node update kernel) since the last reset. */
 int aphip_get_profile(aphip_ctx *ctx, int32_t which, int64_t *launches, double *milliseconds);
 int aphip_reset_profile(aphip_ctx *ctx);
+/* Measurement yardstick with no counterpart in the reference: times `reps`
+ * device-to-device copies of `nbytes` (a multiple of 16) on `device` and returns
+ * the device milliseconds per copy (read + write of nbytes each).  Used to state
+ * achievable HBM bandwidth and to scale the FETCH_SIZE / WRITE_SIZE counters. */
+int aphip_bandwidth_probe(int32_t device, int64_t nbytes, int32_t reps, double *ms_per_copy);
 
 #ifdef __cplusplus
 }
