@@ -202,7 +202,7 @@ static bool use_tiles(const Ctx &c) { return !c.cells.ready && c.tiles.ready && 
 
 static void default_tiling(Ctx &c) {
     if (c.ne == 0 || (c.flags & APHIP_FLAG_TWO_PASS)) return;
-    if (c.etype == APHIP_TET4) tile_build(c, 128, 128);
+    if (c.etype == APHIP_TET4) tile_build(c, 512, 512);
     else tile_build(c, 32, 256);
 }
 
@@ -724,18 +724,26 @@ int aphip_autotune(aphip_ctx *c, double dt, int32_t trial_steps, int32_t *chosen
         }
         APHIP_HIP(hipStreamSynchronize(c->stream));
     }
-    std::vector<std::pair<int, int>> cand;
-    if (c->etype == APHIP_TET4) cand = {{64, 64}, {128, 128}, {256, 256}, {512, 512}};
-    else cand = {{64, 64}, {128, 128}, {16, 128}, {32, 256}, {64, 512}};
+    struct Cand {
+        int te, threads;
+        bool pipelined;
+    };
+    std::vector<Cand> cand;
+    if (c->etype == APHIP_TET4)
+        cand = {{64, 64, false},  {128, 128, false}, {256, 256, false}, {512, 512, false},
+                {128, 128, true}, {256, 256, true}};
+    else
+        cand = {{64, 64, false}, {128, 128, false}, {16, 128, false}, {32, 256, false}, {64, 512, false}};
     const bool was_prof = c->prof.enabled;
-    std::pair<int, int> best{prev_te, c->tiles.threads};
+    Cand best{prev_te, c->tiles.threads, c->tiles.use_stream};
     double best_ms = 1e300;
     hipEvent_t e0, e1;
     APHIP_HIP(hipEventCreate(&e0));
     APHIP_HIP(hipEventCreate(&e1));
     c->prof.enabled = false;
     for (const auto &tt : cand) {
-        tile_build(*c, tt.first, tt.second);
+        tile_build(*c, tt.te, tt.threads);
+        c->tiles.use_stream = tt.pipelined;
         step_once(*c, dt);  // warm-up
         APHIP_HIP(hipEventRecord(e0, c->stream));
         for (int k = 0; k < trial_steps; ++k) step_once(*c, dt);
@@ -751,7 +759,8 @@ int aphip_autotune(aphip_ctx *c, double dt, int32_t trial_steps, int32_t *chosen
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     c->prof.enabled = was_prof;
-    tile_build(*c, best.first, best.second);
+    tile_build(*c, best.te, best.threads);
+    c->tiles.use_stream = best.pipelined;
     set_node_field(*c, c->u, su.data());
     set_node_field(*c, c->v, sv.data());
     set_node_field(*c, c->a, sa.data());
@@ -764,7 +773,7 @@ int aphip_autotune(aphip_ctx *c, double dt, int32_t trial_steps, int32_t *chosen
     }
     c->state_np1 = parity;
     APHIP_HIP(hipStreamSynchronize(c->stream));
-    if (chosen) *chosen = best.first * 10000 + best.second;
+    if (chosen) *chosen = (best.pipelined ? 100000000 : 0) + best.te * 10000 + best.threads;
     APHIP_API_END
 }
 

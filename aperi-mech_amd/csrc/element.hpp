@@ -44,6 +44,11 @@ APHIP_DEV void cd_correct(const double *f, double m, unsigned fixed, double dt, 
     }
 }
 
+// H, P and the state are written once per step and not read again in it: a
+// non-temporal store keeps that stream (144 B per element) from evicting the node
+// data that neighbouring tiles re-read from L2.
+APHIP_DEV void store_once(double *p, double v) { __builtin_nontemporal_store(v, p); }
+
 // Tet4: fe[k][j] = -(B_k . P_j) w
 template <int KIND>
 APHIP_DEV void tet4_element(const double (&X)[4][3], const double (&U)[4][3], const MatParams &m, double sn,

@@ -87,8 +87,8 @@ __global__ __launch_bounds__(kElemBlock) void k_tet4_force(ElemArgs A) {
     if constexpr (KIND == APHIP_J2) A.state_np1[e] = snp1;
 #pragma unroll
     for (int i = 0; i < 9; ++i) {
-        A.H[i * A.ne + e] = H[i];
-        A.P[i * A.ne + e] = P[i];
+        store_once(A.H + i * A.ne + e, H[i]);
+        store_once(A.P + i * A.ne + e, P[i]);
     }
     double *fp = A.partial + e * 12;
 #pragma unroll
@@ -121,8 +121,8 @@ __global__ __launch_bounds__(128) void k_hex8_force(ElemArgs A) {
                                KIND == APHIP_J2 ? A.state_np1 + so : nullptr, H, P, fe);
 #pragma unroll
     for (int i = 0; i < 9; ++i) {
-        A.H[i * A.ne + e] = H[i];
-        A.P[i * A.ne + e] = P[i];
+        store_once(A.H + i * A.ne + e, H[i]);
+        store_once(A.P + i * A.ne + e, P[i]);
     }
     double *fp = A.partial + e * 24;
 #pragma unroll

@@ -137,7 +137,7 @@ struct TilePlan {
     int64_t nslot = 0;                // shared-node slots (tile sums parked for the second launch)
     size_t lds_bytes = 0;
     int stream_grid[4] = {0, 0, 0, 0};  // persistent Tet4 step grid per material kind (0 = not sized yet)
-    bool use_stream = true;            // persistent pipelined Tet4 step when the plan allows it
+    bool use_stream = false;           // persistent pipelined Tet4 step (APHIP_FLAG_PIPELINED)
     DevBuf<int32_t> tn_start;         // [ntiles + 1] offsets into the tile-node arrays
     DevBuf<int32_t> tile_e0;          // [ntiles + 1] first element of each tile
     DevBuf<int32_t> tn_ninterior;     // [ntiles] interior nodes come first in each tile

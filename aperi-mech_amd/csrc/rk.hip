@@ -501,8 +501,8 @@ __global__ __launch_bounds__(128) void k_rk_cells(CellForceArgs A) {
     const double V = A.volume[c];
 #pragma unroll
     for (int i = 0; i < 9; ++i) {
-        A.H[i * A.ne + c] = H[i];
-        A.P[i * A.ne + c] = P[i];
+        store_once(A.H + i * A.ne + c, H[i]);
+        store_once(A.P + i * A.ne + c, P[i]);
     }
 #pragma unroll
     for (int a = 0; a < 3; ++a)

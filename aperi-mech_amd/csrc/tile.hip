@@ -220,8 +220,8 @@ __global__ __launch_bounds__(NPE == 4 ? 512 : (QPAR ? 512 : 128)) void k_tile(Ti
             if (q == 7) {
 #pragma unroll
                 for (int i = 0; i < 9; ++i) {
-                    A.H[i * A.ne + e] = H[i];
-                    A.P[i * A.ne + e] = P[i];
+                    store_once(A.H + i * A.ne + e, H[i]);
+                    store_once(A.P + i * A.ne + e, P[i]);
                 }
             }
             st3(L.F, le * NPE + q, fk);
@@ -245,8 +245,8 @@ __global__ __launch_bounds__(NPE == 4 ? 512 : (QPAR ? 512 : 128)) void k_tile(Ti
         }
 #pragma unroll
         for (int i = 0; i < 9; ++i) {
-            A.H[i * A.ne + e] = H[i];
-            A.P[i * A.ne + e] = P[i];
+            store_once(A.H + i * A.ne + e, H[i]);
+            store_once(A.P + i * A.ne + e, P[i]);
         }
 #pragma unroll
         for (int k = 0; k < NPE; ++k) st3(L.F, tid * NPE + k, fe[k]);
@@ -324,8 +324,10 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer_rsrc(const void *p, int
     return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, (int)bytes, 0x00020000);
 }
 
+// aux: cache policy (2 = nt, the non-temporal hint used for the H / P stream)
+template <int AUX = 0>
 __device__ __forceinline__ void bstore(__amdgpu_buffer_rsrc_t r, uint32_t off, double v) {
-    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2_t, v), r, off, 0, 0);
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2_t, v), r, off, 0, AUX);
 }
 
 __device__ __forceinline__ void bstore3(__amdgpu_buffer_rsrc_t r, uint32_t off, const double *v) {
@@ -529,10 +531,10 @@ __global__ __launch_bounds__(256) void k_tile_stream(TileArgs A, int nblk) {
             const uint32_t oi = (uint32_t)(8 * A.ne);
 #pragma unroll
             for (int i = 0; i < 9; ++i) {
-                bstore(rH, oe + i * oi, H[i]);
-                bstore(rP, oe + i * oi, P[i]);
+                bstore<2>(rH, oe + i * oi, H[i]);
+                bstore<2>(rP, oe + i * oi, P[i]);
             }
-            if constexpr (KIND == APHIP_J2) bstore(rS, oe, snp1);
+            if constexpr (KIND == APHIP_J2) bstore<2>(rS, oe, snp1);
         }
         __syncthreads();
 
@@ -848,7 +850,8 @@ void tile_build(Ctx &c, int te, int threads) {
     P.lds_bytes = sizeof(double) * (11 * (size_t)maxtn + 3 * (size_t)te * npe) + 8 * (size_t)maxtn +
                   2 * (size_t)te * npe + (size_t)maxtn + 16;
     APHIP_REQUIRE(P.lds_bytes <= 160 * 1024, "tile needs more LDS than a CU has; use a smaller tile");
-    P.use_stream = !(c.flags & APHIP_FLAG_NO_PIPELINE);
+    P.use_stream = (c.flags & APHIP_FLAG_PIPELINED) != 0;
+    for (int &g : P.stream_grid) g = 0;  // sized (and LDS attribute set) at the first launch
     P.ready = true;
     APHIP_HIP(hipStreamSynchronize(c.stream));
 }

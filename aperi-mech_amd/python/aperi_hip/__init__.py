@@ -24,7 +24,7 @@ LINEAR_ELASTIC, NEO_HOOKEAN, J2, SVK = 0, 1, 2, 3
 FIELD_U, FIELD_V, FIELD_A, FIELD_F, FIELD_MASS, FIELD_H, FIELD_P, FIELD_STATE, FIELD_VOLUME = range(9)
 MAX_NODE_NUM_NEIGHBORS = 40
 FLAG_TWO_PASS = 1
-FLAG_NO_PIPELINE = 2
+FLAG_PIPELINED = 2
 
 _d = ctypes.POINTER(ctypes.c_double)
 _i32 = ctypes.POINTER(ctypes.c_int32)

@@ -61,8 +61,8 @@ enum {
 
 /* aphip_config.flags */
 #define APHIP_FLAG_TWO_PASS 1 /* element partials through HBM + node CSR gather (no LDS tiles) */
-#define APHIP_FLAG_NO_PIPELINE 2 /* one tile per workgroup instead of the persistent pipelined
-                                    Tet4 step (bitwise identical results) */
+#define APHIP_FLAG_PIPELINED 2 /* Tet4 step as a persistent, software-pipelined kernel instead of
+                                  one tile per workgroup (bitwise identical results) */
 
 typedef struct {
     int32_t device; /* HIP device ordinal */
@@ -154,10 +154,11 @@ int aphip_set_tiling(aphip_ctx *ctx, int32_t elems_per_tile, int32_t threads);
  * counterpart (the reference launches every kernel eagerly). */
 int aphip_set_graph(aphip_ctx *ctx, int32_t enable);
 
-/* Times the candidate tile shapes on the context's own workload (trial_steps steps
- * each, the simulation state is restored afterwards) and keeps the fastest.
- * Writes elements_per_tile * 10000 + threads of the choice to *chosen.  No
- * reference counterpart. */
+/* Times the candidate tile shapes (and, for Tet4, the pipelined kernel) on the
+ * context's own workload (trial_steps steps each, the simulation state is restored
+ * afterwards) and keeps the fastest.  Writes elements_per_tile * 10000 + threads
+ * (+ 100000000 for the pipelined kernel) of the choice to *chosen.  No reference
+ * counterpart. */
 int aphip_autotune(aphip_ctx *ctx, double time_increment, int32_t trial_steps, int32_t *chosen);
 
 /* Wait for all work queued on the context's stream. */

@@ -239,7 +239,7 @@ def test_pipelined_step_matches_one_tile_per_workgroup(gpu_available, mat, te):
     dt = 0.4 * M.stable_dt_estimate(X, conn, p["E"], p["nu"], p["rho"])
     print(f"pipelined vs one-tile mat={mat} te={te} ne={ne}", flush=True)
     outs = []
-    for flags in (0, A.FLAG_NO_PIPELINE):
+    for flags in (A.FLAG_PIPELINED, 0):
         with _gpu(A.TET4, X, conn, kind, p, flags=flags) as s:
             s.set_tiling(te, te)
             s.set_material_fields(E, nu)
