@@ -1,0 +1,397 @@
+// C++ host adapter over the C ABI of libaperi_hip (include/aperi_hip.h).
+//
+// Mirrors the host-side interfaces of aperi-mech's explicit force path so a caller
+// (and its tests) can drive the MI355X library the way it drives aperi-mech:
+//   MaterialType, LinearElasticProperties, MaterialProperties  include/Materials/Base.h:20-169
+//   Material, CreateMaterial                                   include/Materials/Base.h:174-323,
+//                                                              include/Materials/Factory.h:43-160
+//   ElementBase                                                include/ElementBase.h:27-117
+//   CreateElement (supported combinations, error messages)     include/Element.h:32-86
+//   InternalForceContribution::ComputeForce                    include/InternalForceContribution.h:26-146
+//   ExplicitSolver loop body with a fixed increment            src/Solvers/Explicit.cpp:308-498
+// Errors are reported the way the reference reports them: std::runtime_error.
+// Header-only; link against libaperi_hip.so.
+#pragma once
+
+#include <aperi_hip.h>
+
+#include <cmath>
+#include <cstdint>
+#include <limits>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace aperi_hip {
+
+inline void Check(int rc) {
+    if (rc != APHIP_OK) throw std::runtime_error(aphip_last_error());
+}
+
+// ---- materials -------------------------------------------------------------
+
+// Same enumerators and order as include/Materials/Base.h:20-29.
+enum MaterialType {
+    ELASTIC,
+    LINEAR_ELASTIC,
+    NEO_HOOKEAN,
+    NEO_HOOKEAN_WITH_DAMAGE,
+    PLASTIC,
+    DRUCKER_PRAGER,
+    POWER_LAW_CREEP,
+    NONE
+};
+
+// Exactly two of the five isotropic moduli define the other three; the physical
+// bounds are checked afterwards (include/Materials/Base.h:40-160).  The conversion
+// formulas are the textbook isotropic-elasticity relations.
+struct LinearElasticProperties {
+    static constexpr double kUnset = std::numeric_limits<double>::quiet_NaN();
+    double shear_modulus = kUnset;
+    double lambda = kUnset;
+    double youngs_modulus = kUnset;
+    double poisson_ratio = kUnset;
+    double bulk_modulus = kUnset;
+
+    void CompleteProperties() {
+        const bool K = !std::isnan(bulk_modulus), E = !std::isnan(youngs_modulus), L = !std::isnan(lambda),
+                   G = !std::isnan(shear_modulus), nu = !std::isnan(poisson_ratio);
+        const int defined = K + E + L + G + nu;
+        if (defined != 2)
+            throw std::runtime_error("Exactly two properties must be defined. Defined: " + std::to_string(defined));
+        double &k = bulk_modulus, &e = youngs_modulus, &l = lambda, &g = shear_modulus, &v = poisson_ratio;
+        if (K && E) {
+            v = (3 * k - e) / (6 * k);
+            g = e / (2 * (1 + v));
+            l = (e * v) / ((1 + v) * (1 - 2 * v));
+        } else if (K && L) {
+            g = (3 * (k - l)) / 2;
+            v = l / (2 * (l + g));
+            e = 2 * g * (1 + v);
+        } else if (K && G) {
+            l = k - (2.0 / 3.0) * g;
+            v = l / (2 * (l + g));
+            e = 2 * g * (1 + v);
+        } else if (K && nu) {
+            e = 3 * k * (1 - 2 * v);
+            g = e / (2 * (1 + v));
+            l = (e * v) / ((1 + v) * (1 - 2 * v));
+        } else if (E && L) {
+            // nu from 2 l nu^2 + (l + E) nu - l = 0, the root in (-1, 0.5)
+            const double a = 2 * l, b = l + e, c = -l;
+            v = (-b + std::sqrt(b * b - 4 * a * c)) / (2 * a);
+            g = e / (2 * (1 + v));
+            k = l + (2.0 / 3.0) * g;
+        } else if (E && G) {
+            v = e / (2 * g) - 1;
+            l = (e * v) / ((1 + v) * (1 - 2 * v));
+            k = l + (2.0 / 3.0) * g;
+        } else if (E && nu) {
+            g = e / (2 * (1 + v));
+            l = (e * v) / ((1 + v) * (1 - 2 * v));
+            k = e / (3 * (1 - 2 * v));
+        } else if (L && G) {
+            k = l + (2.0 / 3.0) * g;
+            v = l / (2 * (l + g));
+            e = 2 * g * (1 + v);
+        } else if (L && nu) {
+            e = (l * (1 + v) * (1 - 2 * v)) / v;
+            g = e / (2 * (1 + v));
+            k = l + (2.0 / 3.0) * g;
+        } else {  // G and nu
+            e = 2 * g * (1 + v);
+            l = (2 * g * v) / (1 - 2 * v);
+            k = (2 * g * (1 + v)) / (3 * (1 - 2 * v));
+        }
+        CheckPhysicalBounds();
+    }
+
+   private:
+    void CheckPhysicalBounds() const {
+        if (poisson_ratio < -1.0 || poisson_ratio >= 0.5)
+            throw std::runtime_error("Poisson's ratio violation: " + std::to_string(poisson_ratio) +
+                                     " must be in [-1, 0.5)");
+        auto positive = [](double x, const char *name) {
+            if (x <= 0.0) throw std::runtime_error(std::string(name) + " must be positive: " + std::to_string(x));
+        };
+        positive(youngs_modulus, "Young's modulus");
+        positive(shear_modulus, "Shear modulus");
+        positive(bulk_modulus, "Bulk modulus");
+        if (lambda <= -(2.0 / 3.0) * shear_modulus)
+            throw std::runtime_error("Lamé's parameter violation: λ=" + std::to_string(lambda) +
+                                     " must be > -2G/3=" + std::to_string(-(2.0 / 3.0) * shear_modulus));
+    }
+};
+
+struct MaterialProperties {
+    MaterialType material_type = NONE;
+    double density = std::numeric_limits<double>::quiet_NaN();
+    LinearElasticProperties linear_elastic_properties;
+    std::map<std::string, double> properties;  // "yield_stress", "hardening_modulus" for PLASTIC
+};
+
+// Host side of a material: its kind on the device and its completed constants.
+class Material {
+   public:
+    explicit Material(std::shared_ptr<MaterialProperties> p) : m_props(std::move(p)) {
+        if (!m_props) throw std::runtime_error("Material properties are null");
+        m_props->linear_elastic_properties.CompleteProperties();
+        switch (m_props->material_type) {
+            case LINEAR_ELASTIC: m_kind = APHIP_LINEAR_ELASTIC; break;
+            case NEO_HOOKEAN: m_kind = APHIP_NEO_HOOKEAN; break;
+            case PLASTIC: m_kind = APHIP_J2; break;
+            case ELASTIC: m_kind = APHIP_SVK; break;
+            default: throw std::runtime_error("Material type not supported by the MI355X force step");
+        }
+        if (m_kind == APHIP_J2) {
+            m_yield = Required("yield_stress");
+            m_hardening = Required("hardening_modulus");
+        }
+    }
+    MaterialType GetMaterialType() const { return m_props->material_type; }
+    std::shared_ptr<MaterialProperties> GetMaterialProperties() const { return m_props; }
+    double GetDensity() const { return m_props->density; }
+    bool HasState() const { return m_kind == APHIP_J2; }
+    size_t NumberOfStateVariables() const { return HasState() ? 1 : 0; }
+    int Kind() const { return m_kind; }
+    aphip_material_params Params(int state_per_qp = 0) const {
+        const LinearElasticProperties &le = m_props->linear_elastic_properties;
+        return aphip_material_params{m_props->density, le.youngs_modulus, le.poisson_ratio, m_yield, m_hardening,
+                                     state_per_qp, 0};
+    }
+
+   private:
+    double Required(const char *name) const {
+        const auto it = m_props->properties.find(name);
+        if (it == m_props->properties.end()) throw std::runtime_error(std::string("Missing material property: ") + name);
+        return it->second;
+    }
+    std::shared_ptr<MaterialProperties> m_props;
+    int m_kind = -1;
+    double m_yield = 0.0, m_hardening = 0.0;
+};
+
+inline std::shared_ptr<Material> CreateMaterial(std::shared_ptr<MaterialProperties> p) {
+    return std::make_shared<Material>(std::move(p));
+}
+
+// ---- device context --------------------------------------------------------
+
+// One library context: one HIP stream on one device.
+class Device {
+   public:
+    explicit Device(int device = 0, int flags = 0) {
+        const aphip_config cfg{device, flags};
+        Check(aphip_create(&m_ctx, &cfg));
+    }
+    ~Device() { aphip_destroy(m_ctx); }
+    Device(const Device &) = delete;
+    Device &operator=(const Device &) = delete;
+    aphip_ctx *get() const { return m_ctx; }
+    static int Count() {
+        int32_t n = 0;
+        Check(aphip_device_count(&n));
+        return n;
+    }
+
+   private:
+    aphip_ctx *m_ctx = nullptr;
+};
+
+// ---- elements --------------------------------------------------------------
+
+enum class ElementTopology { Tetrahedron4, Hexahedron8 };       // include/Constants.h:23-25
+enum class ApproximationSpaceType { FiniteElement, ReproducingKernel };
+enum class IntegrationSchemeType { GaussQuadrature, StrainSmoothing };
+
+// An element block in the caller's numbering: coordinates n x 3, connectivity
+// n_elem x nodes_per_element (0-based).
+struct MeshBlock {
+    std::vector<double> coordinates;
+    std::vector<int32_t> connectivity;
+    size_t NumNodes() const { return coordinates.size() / 3; }
+};
+
+// include/ElementBase.h:27-117.  Fields are exchanged in the caller's numbering and
+// the reference's layout (node vectors n x 3, element tensors n_elem x 9 row-major).
+class ElementBase {
+   public:
+    ElementBase(size_t num_nodes, std::shared_ptr<Device> device, std::shared_ptr<Material> material)
+        : m_num_nodes(num_nodes), m_device(std::move(device)), m_material(std::move(material)) {}
+    virtual ~ElementBase() = default;
+    virtual bool CheckIfUpdateIsNeeded() const { return false; }
+    virtual void UpdateShapeFunctions() {}
+    size_t GetNumNodes() const { return m_num_nodes; }
+    virtual void ComputeInternalForceAllElements(double time_increment, double time) = 0;
+    virtual void PopulateElementOutputs() {}
+    virtual void FinishPreprocessing() {}
+    std::shared_ptr<Material> GetMaterial() const { return m_material; }
+    aphip_ctx *Context() const { return m_device->get(); }
+    size_t NumMeshNodes() const { return m_mesh_nodes; }
+    size_t NumElements() const { return m_elements; }
+
+    void SetField(int field, const std::vector<double> &values) {
+        Check(FieldSize(field) == values.size() ? APHIP_OK : APHIP_ERR_INVALID, "field size mismatch");
+        Check(aphip_set_field(Context(), field, values.data()));
+    }
+    std::vector<double> GetField(int field) const {
+        std::vector<double> out(FieldSize(field));
+        Check(aphip_get_field(Context(), field, out.data()));
+        return out;
+    }
+    // Velocity and acceleration held at zero on the DOFs marked 1 (n x 3 bytes).
+    void SetFixedDofs(const std::vector<uint8_t> &mask) {
+        Check(mask.size() == 3 * m_mesh_nodes ? APHIP_OK : APHIP_ERR_INVALID, "fixed-DOF mask size mismatch");
+        Check(aphip_set_fixed(Context(), mask.data()));
+    }
+    double ComputeMass() {
+        double total = 0.0;
+        Check(aphip_compute_mass(Context(), &total));
+        return total;
+    }
+
+   protected:
+    static void Check(int rc, const char *msg = nullptr) {
+        if (rc == APHIP_OK) return;
+        throw std::runtime_error(msg ? std::string(msg) : std::string(aphip_last_error()));
+    }
+    size_t FieldSize(int field) const {
+        switch (field) {
+            case APHIP_FIELD_U: case APHIP_FIELD_V: case APHIP_FIELD_A: case APHIP_FIELD_F: return 3 * m_mesh_nodes;
+            case APHIP_FIELD_MASS: return m_mesh_nodes;
+            case APHIP_FIELD_H: case APHIP_FIELD_P: return 9 * m_elements;
+            case APHIP_FIELD_VOLUME: return m_elements;
+            case APHIP_FIELD_STATE: return m_elements * m_material->NumberOfStateVariables();
+            default: throw std::runtime_error("unknown field");
+        }
+    }
+    void BindMesh(int etype, const MeshBlock &mesh) {
+        const size_t npe = etype == APHIP_TET4 ? 4 : 8;
+        if (mesh.coordinates.size() % 3 || mesh.connectivity.size() % npe)
+            throw std::runtime_error("mesh arrays do not match the element topology");
+        m_mesh_nodes = mesh.NumNodes();
+        m_elements = mesh.connectivity.size() / npe;
+        Check(aphip_set_mesh(Context(), etype, (int64_t)m_mesh_nodes, mesh.coordinates.data(), (int64_t)m_elements,
+                             mesh.connectivity.data()));
+        const aphip_material_params p = m_material->Params();
+        Check(aphip_set_material(Context(), m_material->Kind(), &p));
+    }
+
+    size_t m_num_nodes;
+    std::shared_ptr<Device> m_device;
+    std::shared_ptr<Material> m_material;
+    size_t m_mesh_nodes = 0, m_elements = 0;
+};
+
+// ElementTetrahedron4 / ElementHexahedron8 with Gauss quadrature
+// (include/ElementTetrahedron4.h:33-204, include/ElementHexahedron8.h:31-192).
+class ElementGaussianHip : public ElementBase {
+   public:
+    ElementGaussianHip(ElementTopology topology, const MeshBlock &mesh, std::shared_ptr<Device> device,
+                       std::shared_ptr<Material> material)
+        : ElementBase(topology == ElementTopology::Tetrahedron4 ? 4 : 8, std::move(device), std::move(material)) {
+        BindMesh(topology == ElementTopology::Tetrahedron4 ? APHIP_TET4 : APHIP_HEX8, mesh);
+    }
+    void ComputeInternalForceAllElements(double time_increment, double time) override {
+        Check(aphip_internal_force(Context(), time, time_increment));
+    }
+};
+
+// ElementReproducingKernelTet4, one-pass element-as-cell strain smoothing
+// (include/ElementReproducingKernel.h:67-80, :214-224).
+class ElementReproducingKernelHip : public ElementBase {
+   public:
+    ElementReproducingKernelHip(const MeshBlock &mesh, double kernel_radius_scale_factor,
+                                std::shared_ptr<Device> device, std::shared_ptr<Material> material)
+        : ElementBase(4, std::move(device), std::move(material)) {
+        BindMesh(APHIP_TET4, mesh);
+        Check(aphip_rk_setup(Context(), kernel_radius_scale_factor));
+    }
+    void ComputeInternalForceAllElements(double time_increment, double time) override {
+        Check(aphip_internal_force(Context(), time, time_increment));
+    }
+    // Neighbor lists in caller numbering: counts per node, ids n x 40 (-1 padded);
+    // returns how many nodes had more than 40 candidates.
+    int64_t GetNeighbors(std::vector<int32_t> &counts, std::vector<int32_t> &neighbors) const {
+        counts.assign(m_mesh_nodes, 0);
+        neighbors.assign(m_mesh_nodes * 40, -1);
+        int64_t truncated = 0;
+        Check(aphip_rk_get_neighbors(Context(), counts.data(), neighbors.data(), &truncated));
+        return truncated;
+    }
+};
+
+// include/Element.h:32-86: the supported (topology, approximation, integration)
+// combinations and the reference's messages for the unsupported ones.
+inline std::shared_ptr<ElementBase> CreateElement(ElementTopology topology, ApproximationSpaceType space,
+                                                  IntegrationSchemeType scheme, const MeshBlock &mesh,
+                                                  std::shared_ptr<Device> device, std::shared_ptr<Material> material,
+                                                  double kernel_radius_scale_factor = 0.0) {
+    if (space == ApproximationSpaceType::ReproducingKernel) {
+        if (scheme != IntegrationSchemeType::StrainSmoothing)
+            throw std::runtime_error("Gauss Quadrature is not supported for Reproducing Kernel");
+        if (topology != ElementTopology::Tetrahedron4)
+            throw std::runtime_error("Reproducing kernel cells on Hexahedron8 are not supported by the MI355X force step");
+        return std::make_shared<ElementReproducingKernelHip>(mesh, kernel_radius_scale_factor, std::move(device),
+                                                             std::move(material));
+    }
+    if (scheme == IntegrationSchemeType::StrainSmoothing) {
+        if (topology == ElementTopology::Hexahedron8)
+            throw std::runtime_error("Strain Smoothing is not supported for finite elements and Hexahedron8");
+        throw std::runtime_error("Finite-element strain smoothing on Tetrahedron4 is not supported by the MI355X force step");
+    }
+    return std::make_shared<ElementGaussianHip>(topology, mesh, std::move(device), std::move(material));
+}
+
+// include/InternalForceContribution.h:26-146: the force contribution of one block.
+class InternalForceContribution {
+   public:
+    explicit InternalForceContribution(std::shared_ptr<ElementBase> element) : m_element(std::move(element)) {
+        if (!m_element) throw std::runtime_error("InternalForceContribution needs an element");
+    }
+    void Preprocess() {}
+    void FinishPreprocessing() { m_element->FinishPreprocessing(); }
+    bool CheckIfUpdateIsNeeded() const { return m_element->CheckIfUpdateIsNeeded(); }
+    void UpdateShapeFunctions() { m_element->UpdateShapeFunctions(); }
+    void ComputeForce(double time, double time_increment) {
+        m_element->ComputeInternalForceAllElements(time_increment, time);
+    }
+    std::shared_ptr<ElementBase> GetElement() const { return m_element; }
+
+   private:
+    std::shared_ptr<ElementBase> m_element;
+};
+
+// The explicit loop of src/Solvers/Explicit.cpp:308-498 with a fixed increment
+// (DirectTimeStepper, include/TimeStepper.h:97-128), run on the device: the
+// initial force and acceleration, then central-difference steps.
+class ExplicitSolver {
+   public:
+    explicit ExplicitSolver(std::shared_ptr<InternalForceContribution> contribution)
+        : m_contribution(std::move(contribution)) {}
+    // Mass, the initial force and a = f/m (Explicit.cpp:362-376).
+    double Initialize() {
+        const double mass = m_contribution->GetElement()->ComputeMass();
+        Check(aphip_initialize(Context()));
+        m_initialized = true;
+        return mass;
+    }
+    // n steps of the loop body (Explicit.cpp:383-452); returns the time reached.
+    double Solve(double time_increment, int n_steps) {
+        if (!m_initialized) Initialize();
+        Check(aphip_step(Context(), time_increment, n_steps));
+        m_time += time_increment * n_steps;
+        return m_time;
+    }
+    double GetTime() const { return m_time; }
+
+   private:
+    aphip_ctx *Context() const { return m_contribution->GetElement()->Context(); }
+    std::shared_ptr<InternalForceContribution> m_contribution;
+    bool m_initialized = false;
+    double m_time = 0.0;
+};
+
+}  // namespace aperi_hip

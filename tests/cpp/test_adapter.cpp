@@ -1,0 +1,377 @@
+// Tests of the C++ adapter (aperi-mech_amd/cpp/aperi_hip.hpp), written like the
+// reference's own unit tests (Suite.Name cases, EXPECT-style checks):
+//   LinearElasticPropertiesTest   test/unit_tests/LinearElasticPropertiesTest.cpp:16-179
+//   CreateElement errors          include/Element.h:57-84
+//   ForceTest shear               test/unit_tests/ForceTest.cpp:20-171 (uniform shear: H, P,
+//                                 zero interior force and net force, and the exact force
+//                                 moment and virtual-work identities of a uniform stress)
+//   SolverTest                    test/unit_tests/SolverTest.cpp:12-33
+//   parity with the CPU oracle    oracle/oracle.h (the C restatement; test infrastructure)
+// Usage: test_adapter cpu|gpu   (gpu cases need a visible device)
+#include <aperi_hip.hpp>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <functional>
+#include <string>
+#include <vector>
+
+extern "C" {
+#include "oracle.h"
+}
+
+using namespace aperi_hip;
+
+namespace {
+
+int g_failures = 0;
+std::string g_case;
+
+#define EXPECT_TRUE(c)                                                                   \
+    do {                                                                                 \
+        if (!(c)) {                                                                      \
+            std::printf("  FAILED %s: %s (line %d)\n", g_case.c_str(), #c, __LINE__);   \
+            ++g_failures;                                                                \
+        }                                                                                \
+    } while (0)
+
+#define EXPECT_NEAR_REL(a, b, tol) EXPECT_TRUE(std::fabs((a) - (b)) <= (tol) * std::max(std::fabs(b), 1e-300))
+
+template <typename F>
+bool Throws(F f, const std::string &needle = "") {
+    try {
+        f();
+    } catch (const std::runtime_error &e) {
+        return needle.empty() || std::string(e.what()).find(needle) != std::string::npos;
+    }
+    return false;
+}
+
+void Run(const char *name, const std::function<void()> &body) {
+    g_case = name;
+    const int before = g_failures;
+    try {
+        body();
+    } catch (const std::exception &e) {
+        std::printf("  FAILED %s: exception %s\n", name, e.what());
+        ++g_failures;
+    }
+    std::printf("[%s] %s\n", g_failures == before ? "  OK  " : " FAIL ", name);
+    std::fflush(stdout);
+}
+
+// ---- meshes ------------------------------------------------------------------
+
+uint64_t Mix(uint64_t x) {  // splitmix64 finalizer
+    x += 0x9e3779b97f4a7c15ull;
+    x = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ull;
+    x = (x ^ (x >> 27)) * 0x94d049bb133111ebull;
+    return x ^ (x >> 31);
+}
+double Uniform(uint64_t seed, uint64_t i) { return (double)(Mix(seed * 0x100000001b3ull + i) >> 11) * 0x1.0p-53; }
+
+// n^3 hexes on [0,1]^3; interior nodes jittered by up to `jitter` of the spacing.
+MeshBlock HexLattice(int n, double jitter, uint64_t seed, std::vector<int32_t> *hexes) {
+    MeshBlock m;
+    const int p = n + 1;
+    for (int k = 0; k < p; ++k)
+        for (int j = 0; j < p; ++j)
+            for (int i = 0; i < p; ++i) {
+                const int idx[3] = {i, j, k};
+                for (int d = 0; d < 3; ++d) {
+                    double x = (double)idx[d] / n;
+                    const bool interior = i > 0 && i < n && j > 0 && j < n && k > 0 && k < n;
+                    if (interior) x += jitter / n * (2.0 * Uniform(seed, 3 * (size_t)(i + p * (j + p * k)) + d) - 1.0);
+                    m.coordinates.push_back(x);
+                }
+            }
+    auto id = [p](int i, int j, int k) { return (int32_t)(i + p * (j + p * k)); };
+    for (int k = 0; k < n; ++k)
+        for (int j = 0; j < n; ++j)
+            for (int i = 0; i < n; ++i) {
+                const int32_t h[8] = {id(i, j, k),         id(i + 1, j, k),     id(i + 1, j + 1, k),
+                                      id(i, j + 1, k),     id(i, j, k + 1),     id(i + 1, j, k + 1),
+                                      id(i + 1, j + 1, k + 1), id(i, j + 1, k + 1)};
+                hexes->insert(hexes->end(), h, h + 8);
+            }
+    return m;
+}
+
+double SignedVolume(const std::vector<double> &X, const int32_t *t) {
+    double a[3], b[3], c[3];
+    for (int d = 0; d < 3; ++d) {
+        a[d] = X[3 * t[1] + d] - X[3 * t[0] + d];
+        b[d] = X[3 * t[2] + d] - X[3 * t[0] + d];
+        c[d] = X[3 * t[3] + d] - X[3 * t[0] + d];
+    }
+    return (a[0] * (b[1] * c[2] - b[2] * c[1]) - a[1] * (b[0] * c[2] - b[2] * c[0]) + a[2] * (b[0] * c[1] - b[1] * c[0])) / 6.0;
+}
+
+// Six tetrahedra around the 0-6 diagonal of each hex, positively oriented.
+std::vector<int32_t> SplitTets(const MeshBlock &m, const std::vector<int32_t> &hexes) {
+    static const int kTets[6][4] = {{0, 1, 2, 6}, {0, 2, 3, 6}, {0, 3, 7, 6}, {0, 7, 4, 6}, {0, 4, 5, 6}, {0, 5, 1, 6}};
+    std::vector<int32_t> tets;
+    for (size_t h = 0; h < hexes.size() / 8; ++h)
+        for (const auto &tt : kTets) {
+            int32_t t[4];
+            for (int k = 0; k < 4; ++k) t[k] = hexes[8 * h + tt[k]];
+            if (SignedVolume(m.coordinates, t) < 0) std::swap(t[2], t[3]);
+            tets.insert(tets.end(), t, t + 4);
+        }
+    return tets;
+}
+
+std::shared_ptr<Material> MakeMaterial(MaterialType type, double E = 200e9, double nu = 0.3) {
+    auto p = std::make_shared<MaterialProperties>();
+    p->material_type = type;
+    p->density = 7850.0;
+    p->linear_elastic_properties.youngs_modulus = E;
+    p->linear_elastic_properties.poisson_ratio = nu;
+    if (type == PLASTIC) {
+        p->properties["yield_stress"] = 300e6;
+        p->properties["hardening_modulus"] = 1e9;
+    }
+    return CreateMaterial(p);
+}
+
+// ---- CPU cases -------------------------------------------------------------------
+
+void CpuCases() {
+    Run("LinearElasticPropertiesTest.AllValidPairs", [] {
+        const double E = 200e9, nu = 0.3, G = E / (2 * (1 + nu)), L = E * nu / ((1 + nu) * (1 - 2 * nu)),
+                     K = E / (3 * (1 - 2 * nu));
+        const double ref[5] = {G, L, E, nu, K};
+        for (int a = 0; a < 5; ++a)
+            for (int b = a + 1; b < 5; ++b) {
+                LinearElasticProperties p;
+                double *f[5] = {&p.shear_modulus, &p.lambda, &p.youngs_modulus, &p.poisson_ratio, &p.bulk_modulus};
+                *f[a] = ref[a];
+                *f[b] = ref[b];
+                p.CompleteProperties();
+                for (int c = 0; c < 5; ++c) EXPECT_NEAR_REL(*f[c], ref[c], 1e-12);
+            }
+    });
+    Run("LinearElasticPropertiesTest.InvalidNumberOfProperties", [] {
+        LinearElasticProperties none;
+        EXPECT_TRUE(Throws([&] { none.CompleteProperties(); }, "Exactly two properties must be defined"));
+        LinearElasticProperties three;
+        three.youngs_modulus = 1.0;
+        three.poisson_ratio = 0.2;
+        three.bulk_modulus = 1.0;
+        EXPECT_TRUE(Throws([&] { three.CompleteProperties(); }, "Defined: 3"));
+    });
+    Run("LinearElasticPropertiesTest.PhysicalBounds", [] {
+        LinearElasticProperties p;
+        p.youngs_modulus = 1.0;
+        p.poisson_ratio = 0.5;
+        EXPECT_TRUE(Throws([&] { p.CompleteProperties(); }, "Poisson's ratio violation"));
+        LinearElasticProperties q;
+        q.youngs_modulus = -1.0;
+        q.poisson_ratio = 0.2;
+        EXPECT_TRUE(Throws([&] { q.CompleteProperties(); }, "must be positive"));
+    });
+    Run("MaterialTest.KindsAndErrors", [] {
+        EXPECT_TRUE(MakeMaterial(NEO_HOOKEAN)->Kind() == APHIP_NEO_HOOKEAN);
+        EXPECT_TRUE(MakeMaterial(PLASTIC)->HasState());
+        EXPECT_TRUE(MakeMaterial(PLASTIC)->NumberOfStateVariables() == 1);
+        EXPECT_TRUE(!MakeMaterial(LINEAR_ELASTIC)->HasState());
+        EXPECT_TRUE(Throws([] { MakeMaterial(DRUCKER_PRAGER); }, "not supported"));
+        auto p = std::make_shared<MaterialProperties>();
+        p->material_type = PLASTIC;
+        p->linear_elastic_properties.youngs_modulus = 1.0;
+        p->linear_elastic_properties.poisson_ratio = 0.2;
+        EXPECT_TRUE(Throws([&] { CreateMaterial(p); }, "Missing material property"));
+    });
+    Run("CreateElementTest.UnsupportedCombinations", [] {
+        MeshBlock empty;
+        auto mat = MakeMaterial(LINEAR_ELASTIC);
+        // the combination is checked before the device is touched
+        EXPECT_TRUE(Throws(
+            [&] {
+                CreateElement(ElementTopology::Hexahedron8, ApproximationSpaceType::FiniteElement,
+                              IntegrationSchemeType::StrainSmoothing, empty, nullptr, mat);
+            },
+            "Strain Smoothing is not supported for finite elements and Hexahedron8"));
+        EXPECT_TRUE(Throws(
+            [&] {
+                CreateElement(ElementTopology::Tetrahedron4, ApproximationSpaceType::ReproducingKernel,
+                              IntegrationSchemeType::GaussQuadrature, empty, nullptr, mat);
+            },
+            "Gauss Quadrature is not supported for Reproducing Kernel"));
+    });
+}
+
+// ---- GPU cases -------------------------------------------------------------------
+
+// Uniform shear u = gamma * X_1 e_0: H, P uniform; interior force zero; the internal
+// forces on the nodes of the face x_a = 1 sum to -P e_a A and balance overall.
+void ShearCase(ElementTopology topo, MaterialType type) {
+    std::vector<int32_t> hexes;
+    MeshBlock mesh = HexLattice(4, 0.2, 7, &hexes);
+    mesh.connectivity = topo == ElementTopology::Hexahedron8 ? hexes : SplitTets(mesh, hexes);
+    auto dev = std::make_shared<Device>();
+    auto mat = MakeMaterial(type);
+    auto elem = CreateElement(topo, ApproximationSpaceType::FiniteElement, IntegrationSchemeType::GaussQuadrature,
+                              mesh, dev, mat);
+    InternalForceContribution force(elem);
+    const size_t n = mesh.NumNodes();
+    const double gamma = 1e-3;
+    std::vector<double> u(3 * n, 0.0);
+    for (size_t i = 0; i < n; ++i) u[3 * i] = gamma * mesh.coordinates[3 * i + 1];
+    elem->SetField(APHIP_FIELD_U, u);
+    force.ComputeForce(0.0, 1e-7);
+    const std::vector<double> H = elem->GetField(APHIP_FIELD_H), P = elem->GetField(APHIP_FIELD_P),
+                              f = elem->GetField(APHIP_FIELD_F);
+    double Hx[9] = {0, gamma, 0, 0, 0, 0, 0, 0, 0}, Px[9];
+    const LinearElasticProperties &le = mat->GetMaterialProperties()->linear_elastic_properties;
+    or_stress(mat->Kind(), le.lambda, le.shear_modulus, 0.0, 0.0, Hx, nullptr, nullptr, Px);
+    double hmax = 0.0, pmax = 0.0, pscale = 0.0;
+    for (size_t e = 0; e < elem->NumElements(); ++e)
+        for (int c = 0; c < 9; ++c) {
+            hmax = std::max(hmax, std::fabs(H[9 * e + c] - Hx[c]));
+            pmax = std::max(pmax, std::fabs(P[9 * e + c] - Px[c]));
+            pscale = std::max(pscale, std::fabs(Px[c]));
+        }
+    EXPECT_TRUE(hmax < 1e-15);
+    EXPECT_TRUE(pmax < 1e-12 * pscale);
+    // with uniform P and linearly complete shape functions (sum_I X_I (x) grad N_I = I):
+    //   sum_I f_I = 0,  sum_I f_I (x) X_I = -P V,  sum_I f_I . u_I = -P : H V   (V = 1)
+    const double fscale = pscale / 16.0;  // stress x the face area of one lattice cell
+    double sum[3] = {0, 0, 0}, moment[9] = {0}, work = 0.0, interior = 0.0;
+    for (size_t i = 0; i < n; ++i) {
+        bool inside = true;
+        for (int a = 0; a < 3; ++a) {
+            const double x = mesh.coordinates[3 * i + a];
+            if (x < 1e-12 || x > 1 - 1e-12) inside = false;
+            sum[a] += f[3 * i + a];
+            work += f[3 * i + a] * u[3 * i + a];
+            for (int b = 0; b < 3; ++b) moment[3 * a + b] += f[3 * i + a] * mesh.coordinates[3 * i + b];
+        }
+        if (inside)
+            for (int c = 0; c < 3; ++c) interior = std::max(interior, std::fabs(f[3 * i + c]));
+    }
+    EXPECT_TRUE(interior < 1e-9 * fscale);
+    for (int c = 0; c < 3; ++c) EXPECT_TRUE(std::fabs(sum[c]) < 1e-9 * fscale);
+    for (int c = 0; c < 9; ++c) EXPECT_TRUE(std::fabs(moment[c] + Px[c]) < 1e-11 * pscale);
+    double PH = 0.0;
+    for (int c = 0; c < 9; ++c) PH += Px[c] * Hx[c];
+    EXPECT_NEAR_REL(work, -PH, 1e-9);
+}
+
+void GpuCases() {
+    Run("ForceTest.ExplicitShearForceTet4LinearElastic", [] { ShearCase(ElementTopology::Tetrahedron4, LINEAR_ELASTIC); });
+    Run("ForceTest.ExplicitShearForceTet4NeoHookean", [] { ShearCase(ElementTopology::Tetrahedron4, NEO_HOOKEAN); });
+    Run("ForceTest.ExplicitShearForceHex8LinearElastic", [] { ShearCase(ElementTopology::Hexahedron8, LINEAR_ELASTIC); });
+    Run("ForceTest.ExplicitShearForceHex8NeoHookean", [] { ShearCase(ElementTopology::Hexahedron8, NEO_HOOKEAN); });
+
+    Run("SolverTest.ExplicitFreeFlight", [] {
+        // uniform velocity, no strain: u = v t exactly and the force stays zero
+        std::vector<int32_t> hexes;
+        MeshBlock mesh = HexLattice(3, 0.0, 1, &hexes);
+        mesh.connectivity = SplitTets(mesh, hexes);
+        auto elem = CreateElement(ElementTopology::Tetrahedron4, ApproximationSpaceType::FiniteElement,
+                                  IntegrationSchemeType::GaussQuadrature, mesh, std::make_shared<Device>(),
+                                  MakeMaterial(NEO_HOOKEAN));
+        const size_t n = mesh.NumNodes();
+        std::vector<double> v(3 * n);
+        for (size_t i = 0; i < n; ++i) {
+            v[3 * i] = 0.25;
+            v[3 * i + 1] = -0.5;
+            v[3 * i + 2] = 1.0;
+        }
+        elem->SetField(APHIP_FIELD_V, v);
+        ExplicitSolver solver(std::make_shared<InternalForceContribution>(elem));
+        const double mass = solver.Initialize();
+        EXPECT_NEAR_REL(mass, 7850.0, 1e-12);
+        const double t = solver.Solve(1e-6, 50);
+        const std::vector<double> u = elem->GetField(APHIP_FIELD_U), vv = elem->GetField(APHIP_FIELD_V);
+        for (size_t i = 0; i < 3 * n; ++i) {
+            EXPECT_TRUE(std::fabs(u[i] - v[i] * t) < 1e-15);
+            EXPECT_TRUE(vv[i] == v[i]);
+        }
+    });
+
+    Run("ElementTest.GaussianForceMatchesOracle", [] {
+        // jittered Tet4 and Hex8 blocks, random displacement: f, H, P against the C oracle
+        for (int hex = 0; hex < 2; ++hex) {
+            std::vector<int32_t> hexes;
+            MeshBlock mesh = HexLattice(5, 0.25, 3, &hexes);
+            mesh.connectivity = hex ? hexes : SplitTets(mesh, hexes);
+            auto mat = MakeMaterial(NEO_HOOKEAN);
+            auto elem = CreateElement(hex ? ElementTopology::Hexahedron8 : ElementTopology::Tetrahedron4,
+                                      ApproximationSpaceType::FiniteElement, IntegrationSchemeType::GaussQuadrature,
+                                      mesh, std::make_shared<Device>(), mat);
+            const size_t n = mesh.NumNodes(), ne = elem->NumElements();
+            std::vector<double> u(3 * n);
+            for (size_t i = 0; i < 3 * n; ++i) u[i] = 2e-2 * (2.0 * Uniform(11, i) - 1.0);
+            elem->SetField(APHIP_FIELD_U, u);
+            InternalForceContribution(elem).ComputeForce(0.0, 1e-7);
+            const LinearElasticProperties &le = mat->GetMaterialProperties()->linear_elastic_properties;
+            or_material om{OR_NEO_HOOKEAN, 0, le.lambda, le.shear_modulus, 0.0, 0.0, nullptr, nullptr};
+            std::vector<double> f(3 * n), H(9 * ne), P(9 * ne);
+            or_internal_force(hex ? OR_HEX8 : OR_TET4, (int64_t)n, mesh.coordinates.data(), (int64_t)ne,
+                              mesh.connectivity.data(), &om, u.data(), nullptr, nullptr, H.data(), P.data(), f.data(), 1);
+            auto rel = [](const std::vector<double> &a, const std::vector<double> &b) {
+                double d = 0.0, s = 0.0;
+                for (size_t i = 0; i < a.size(); ++i) {
+                    d = std::max(d, std::fabs(a[i] - b[i]));
+                    s = std::max(s, std::fabs(b[i]));
+                }
+                return d / s;
+            };
+            EXPECT_TRUE(rel(elem->GetField(APHIP_FIELD_F), f) < 1e-12);
+            EXPECT_TRUE(rel(elem->GetField(APHIP_FIELD_H), H) < 1e-12);
+            EXPECT_TRUE(rel(elem->GetField(APHIP_FIELD_P), P) < 1e-12);
+        }
+    });
+
+    Run("ReproducingKernelTest.NeighborListsMatchOracle", [] {
+        std::vector<int32_t> hexes;
+        MeshBlock mesh = HexLattice(4, 0.2, 5, &hexes);
+        mesh.connectivity = SplitTets(mesh, hexes);
+        const double scale = 1.5;
+        auto elem = std::dynamic_pointer_cast<ElementReproducingKernelHip>(
+            CreateElement(ElementTopology::Tetrahedron4, ApproximationSpaceType::ReproducingKernel,
+                          IntegrationSchemeType::StrainSmoothing, mesh, std::make_shared<Device>(),
+                          MakeMaterial(PLASTIC), scale));
+        EXPECT_TRUE(elem != nullptr);
+        std::vector<int32_t> cnt, nbr;
+        elem->GetNeighbors(cnt, nbr);
+        const size_t n = mesh.NumNodes();
+        std::vector<double> edge(n), radius(n);
+        or_max_edge_length((int64_t)n, mesh.coordinates.data(), (int64_t)(mesh.connectivity.size() / 4),
+                           mesh.connectivity.data(), 4, nullptr, edge.data());
+        for (size_t i = 0; i < n; ++i) radius[i] = scale * edge[i];
+        std::vector<int32_t> rc(n), rn(n * 40);
+        int64_t trunc = 0;
+        or_neighbor_search((int64_t)n, mesh.coordinates.data(), radius.data(), 40, 0, rc.data(), rn.data(), &trunc);
+        EXPECT_TRUE(cnt == rc);
+        bool same = true;
+        for (size_t i = 0; i < n; ++i)
+            for (int k = 0; k < rc[i]; ++k) same = same && nbr[40 * i + k] == rn[40 * i + k];
+        EXPECT_TRUE(same);
+    });
+}
+
+}  // namespace
+
+int main(int argc, char **argv) {
+    const std::string mode = argc > 1 ? argv[1] : "cpu";
+    if (mode == "cpu") {
+        CpuCases();
+    } else if (mode == "gpu") {
+        if (Device::Count() < 1) {
+            std::printf("no HIP device visible\n");
+            return 2;
+        }
+        GpuCases();
+    } else {
+        std::printf("usage: %s cpu|gpu\n", argv[0]);
+        return 2;
+    }
+    std::printf("%d failure(s)\n", g_failures);
+    return g_failures == 0 ? 0 : 1;
+}
