@@ -178,3 +178,25 @@ def ensemble(n_specimens=256, seed=DEFAULT_SEED, nmin=8, nmax=48):
         v0 = 2.0 * uniform01(seed + 7919 * (s + 1), 2, X.size).reshape(X.shape) - 1.0
         specs.append(dict(X=X, conn=conn, etype=1, v0=v0, E=float(Es[s]), nu=float(nus[s]), rho=7850.0, n=n))
     return specs
+
+
+def fourier_field(points, seed, stream, n_modes=64, length=0.2):
+    """Smooth seeded random field of about unit variance: sqrt(2/M) sum_m cos(k_m . x + phi_m),
+    k_m ~ N(0, I / length^2), phi_m ~ U[0, 2 pi)  (a random-Fourier-feature Gaussian field
+    with squared-exponential correlation of length `length`)."""
+    k = normal(seed, stream, 3 * n_modes).reshape(n_modes, 3) / length
+    phi = 2.0 * math.pi * uniform01(seed, stream + 1, n_modes)
+    return math.sqrt(2.0 / n_modes) * np.cos(points @ k.T + phi).sum(axis=1)
+
+
+def adjoint_block(n=64, seed=DEFAULT_SEED):
+    """n^3 Hex8 on [0,1]^3 with per-element E, nu from smooth random fields around 200 GPa
+    (+-20 % at two sigma) and 0.3 (+-0.05); v0 ~ U[-1,1]^3 m/s; the target displacement
+    comes from the same forward run with E scaled by (1 + 0.05 g), g a third field."""
+    X, conn = hex_lattice(n, n, n)
+    cen = X[conn].mean(axis=1)
+    E = 200e9 * (1.0 + 0.1 * fourier_field(cen, seed, 30))
+    nu = 0.3 + 0.025 * fourier_field(cen, seed, 32)
+    E_target = E * (1.0 + 0.05 * fourier_field(cen, seed, 34))
+    v0 = 2.0 * uniform01(seed, 36, X.size).reshape(X.shape) - 1.0
+    return dict(X=X, conn=conn, etype=1, E=E, nu=nu, E_target=E_target, v0=v0, rho=7850.0)

@@ -88,6 +88,155 @@ def run_gpu(w, kind, steps, warmup):
                 n_step=n_step, finite=ok, dt=dt)
 
 
+def bench_batched(n_specimens=256, steps=50):
+    """Uneven Hex8 ensemble (edges log-uniform in [8, 48], E ~ U[50, 250] GPa, nu ~ U[0.2, 0.45],
+    own dt per specimen): one batched context stepping all specimens (two launches per step,
+    replayed from a HIP graph) against the plain loop, one solver per specimen run to
+    completion in turn.  Every field of every specimen must be bitwise identical."""
+    import aperi_hip as A
+    from aperi_hip import meshgen as M
+    t0 = time.time()
+    specs = M.ensemble(n_specimens)
+    dts = [0.5 * M.stable_dt_estimate(sp["X"], sp["conn"], sp["E"], sp["nu"], sp["rho"]) for sp in specs]
+    ne = sum(len(sp["conn"]) for sp in specs)
+    g = A.Solver()
+    g.batch_begin(A.HEX8, A.NEO_HOOKEAN)
+    for sp, dt in zip(specs, dts):
+        g.batch_add(sp["X"], sp["conn"], sp["rho"], sp["E"], sp["nu"], dt)
+    g.batch_end()
+    g.compute_mass()
+    g.set_field(A.FIELD_V, np.concatenate([sp["v0"] for sp in specs]))
+    g.initialize()
+    plain = []
+    for sp in specs:
+        s = A.Solver()
+        s.set_mesh(A.HEX8, sp["X"], sp["conn"])
+        s.set_material(A.NEO_HOOKEAN, sp["rho"], sp["E"], sp["nu"])
+        s.compute_mass()
+        s.set_field(A.FIELD_V, sp["v0"])
+        s.initialize()
+        plain.append(s)
+    for s in plain:
+        s.synchronize()
+    log(f"[bench] ensemble set-up {time.time() - t0:.1f}s ({n_specimens} specimens, {ne} Hex8)")
+    g.set_graph(True)
+    g.synchronize()
+    t0 = time.perf_counter()
+    g.step(0.0, steps)  # the increment of each node is its specimen's dt
+    g.synchronize()
+    t_batch = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    for s, dt in zip(plain, dts):
+        s.step(dt, steps)
+        s.synchronize()
+    t_loop = time.perf_counter() - t0
+    identical = True
+    fields = (A.FIELD_U, A.FIELD_V, A.FIELD_A, A.FIELD_F, A.FIELD_H, A.FIELD_P)
+    got = {f: g.get(f) for f in fields}
+    for k, s in enumerate(plain):
+        n0, nn, e0, nel = g.batch_ranges(k)
+        for f in fields:
+            lo, hi = (e0, e0 + nel) if f in (A.FIELD_H, A.FIELD_P) else (n0, n0 + nn)
+            if not np.array_equal(got[f][lo:hi], s.get(f)):
+                identical = False
+        s.close()
+    g.close()
+    return {"workload": f"{n_specimens} Hex8 cubes, edges log-uniform in [8,48] ({ne} Hex8), neo-Hookean, "
+                        f"per-specimen E, nu and dt, {steps} steps",
+            "elements": ne, "steps": steps,
+            "batched_s": t_batch, "plain_loop_s": t_loop, "speedup": t_loop / t_batch,
+            "batched_element_updates_per_s": ne * steps / t_batch,
+            "plain_loop_element_updates_per_s": ne * steps / t_loop,
+            "identical_results": identical, "launch": "batched: HIP graph, two launches per step for all "
+                                                       "specimens; plain: eager, one solver after another"}
+
+
+def bench_adjoint(n=64, steps=200, interval=20, n_check=32, h=1e-6):
+    """Gradient of L = 1/2 |u_T - u*|^2 with respect to every element's E and nu and the
+    initial velocity (64^3 Hex8, smooth E/nu fields), 200 steps with checkpoints every 20,
+    against the plain forward; 32 random parameters checked by central differences of the
+    same GPU forward (h relative), with L+ - L- formed from the two displacement fields."""
+    import aperi_hip as A
+    from aperi_hip import meshgen as M
+    w = M.adjoint_block(n)
+    X, conn, rho = w["X"], w["conn"], w["rho"]
+    dt = 0.5 * M.stable_dt_estimate(X, conn, float(w["E"].max()), float(w["nu"].min()), rho)
+    s = A.Solver()
+    s.set_mesh(A.HEX8, X, conn)
+    s.set_material(A.NEO_HOOKEAN, rho, 200e9, 0.3)
+    s.compute_mass()
+
+    def forward(E, nu, v0):
+        s.set_material_fields(E, nu)
+        s.set_field(A.FIELD_U, np.zeros_like(X))
+        s.set_field(A.FIELD_V, v0)
+        s.initialize()
+        s.step(dt, steps)
+        return s.get(A.FIELD_U)
+
+    u_star = forward(w["E_target"], w["nu"], w["v0"])
+    # plain forward timing
+    s.set_material_fields(w["E"], w["nu"])
+    s.set_field(A.FIELD_U, np.zeros_like(X))
+    s.set_field(A.FIELD_V, w["v0"])
+    s.initialize()
+    s.synchronize()
+    t0 = time.perf_counter()
+    s.step(dt, steps)
+    s.synchronize()
+    t_plain = time.perf_counter() - t0
+    # forward with checkpoints + reverse sweep
+    s.set_field(A.FIELD_U, np.zeros_like(X))
+    s.set_field(A.FIELD_V, w["v0"])
+    s.initialize()
+    s.synchronize()
+    t0 = time.perf_counter()
+    s.adjoint_forward(dt, steps, interval)
+    s.synchronize()
+    t_fwd = time.perf_counter() - t0
+    u_T = s.get(A.FIELD_U)
+    t0 = time.perf_counter()
+    Eb, nub, v0b = s.adjoint_backward(u_T - u_star)
+    t_bwd = time.perf_counter() - t0
+    # central differences on random parameters: 12 E, 10 nu, 10 initial-velocity components
+    rng = np.random.default_rng(20261202)
+    picks = ([("E", int(i)) for i in rng.choice(len(conn), 12, replace=False)] +
+             [("nu", int(i)) for i in rng.choice(len(conn), 10, replace=False)] +
+             [("v0", int(i)) for i in rng.choice(X.size, n_check - 22, replace=False)])
+    def central_difference(kind, i, hrel):
+        E, nu, v0 = w["E"].copy(), w["nu"].copy(), w["v0"].copy()
+        arr = {"E": E, "nu": nu, "v0": v0.reshape(-1)}[kind]
+        base = arr[i]
+        step = hrel * abs(base)
+        arr[i] = base + step
+        up = forward(E, nu, v0)
+        arr[i] = base - step
+        um = forward(E, nu, v0)
+        return np.sum((up - um) * (0.5 * (up + um) - u_star)) / (2.0 * step)
+
+    errs, errs10, per_kind = [], [], {}
+    for kind, i in picks:
+        ad = {"E": Eb, "nu": nub, "v0": v0b.reshape(-1)}[kind][i]
+        fd = central_difference(kind, i, h)
+        fd10 = central_difference(kind, i, 10 * h)
+        errs.append(abs(ad - fd) / max(abs(fd), 1e-300))
+        errs10.append(abs(ad - fd10) / max(abs(fd10), 1e-300))
+        per_kind.setdefault(kind, []).append(errs[-1])
+    s.close()
+    errs, errs10 = np.array(errs), np.array(errs10)
+    return {"workload": f"{n}^3 Hex8 ({len(conn)} el), neo-Hookean with per-element E/nu fields, "
+                        f"{steps} steps, checkpoints every {interval}",
+            "plain_forward_s": t_plain, "checkpointed_forward_s": t_fwd, "backward_s": t_bwd,
+            "gradient_over_forward": (t_fwd + t_bwd) / t_plain,
+            "element_updates_per_s_forward": len(conn) * steps / t_plain,
+            "fd_check": {"params": len(picks), "h_rel": h, "max_rel_err": float(errs.max()),
+                         "median_rel_err": float(np.median(errs)), "within_1e-6": int((errs < 1e-6).sum()),
+                         "max_rel_err_by_kind": {k: float(max(v)) for k, v in per_kind.items()},
+                         "max_rel_err_h10": float(errs10.max()), "within_1e-6_h10": int((errs10 < 1e-6).sum()),
+                         "reference": "central differences of the same GPU forward (oracle parity of that "
+                                      "forward is tested at 1e-12 per step)"}}
+
+
 def cpu_baseline(w, kind, budget_s=12.0):
     """Oracle (C restatement of the reference data flow) on all host cores."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -122,6 +271,7 @@ def main():
     ap.add_argument("--n", type=int, default=100, help="hexes per edge (x6 tets)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--material", default="nh", choices=["nh", "le"])
+    ap.add_argument("--no-extras", action="store_true", help="skip the ensemble and adjoint measurements")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only time a device copy of BYTES (HBM yardstick) and print it")
     args = ap.parse_args()
@@ -177,6 +327,15 @@ def main():
     if not args.no_cpu:
         log("[bench] cpu baseline ...")
         line["cpu_baseline"] = cpu_baseline(w, kind)
+    del w
+    if not args.no_extras:
+        log("[bench] batched ensemble ...")
+        line["batched"] = bench_batched()
+        log(f"[bench] batched: {line['batched']['speedup']:.2f}x, identical={line['batched']['identical_results']}")
+        log("[bench] adjoint ...")
+        line["adjoint"] = bench_adjoint()
+        log(f"[bench] adjoint: {line['adjoint']['gradient_over_forward']:.2f}x forward, "
+            f"fd max rel err {line['adjoint']['fd_check']['max_rel_err']:.2e}")
     print(json.dumps(line), flush=True)
 
 
