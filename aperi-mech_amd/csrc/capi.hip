@@ -203,7 +203,7 @@ static bool use_tiles(const Ctx &c) { return !c.cells.ready && c.tiles.ready && 
 static void default_tiling(Ctx &c) {
     if (c.ne == 0 || (c.flags & APHIP_FLAG_TWO_PASS)) return;
     if (c.etype == APHIP_TET4) tile_build(c, 512, 512);
-    else tile_build(c, 32, 256);
+    else tile_build(c, 128, 128);
 }
 
 
