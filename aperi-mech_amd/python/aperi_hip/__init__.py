@@ -295,6 +295,12 @@ class Solver:
         _check(self.L.aphip_rk_get_function_values(self.ctx, _p(phi)))
         return phi
 
+    def rk_nnz(self):
+        """Entries of the smoothed-cell CSR (sum of the cell node-list lengths)."""
+        nnz = ctypes.c_int64()
+        _check(self.L.aphip_rk_get_cells(self.ctx, ctypes.byref(nnz), None, None, None, None))
+        return nnz.value
+
     def rk_cells(self):
         nnz = ctypes.c_int64()
         _check(self.L.aphip_rk_get_cells(self.ctx, ctypes.byref(nnz), None, None, None, None))

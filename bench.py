@@ -237,6 +237,56 @@ def bench_adjoint(n=64, steps=200, interval=20, n_check=32, h=1e-6):
                                       "forward is tested at 1e-12 per step)"}}
 
 
+def bench_rkpm(n_nodes_edge=64, steps=100, scale=1.5):
+    """Strain-smoothed RKPM: 64^3-node jittered lattice, Kuhn Tet4 background, element-as-cell
+    smoothing, kernel support 1.5 x the max edge length, J2 with linear hardening, a velocity
+    ramp about the centre reaching a nominal strain of 2 sigma_y / E at step 100."""
+    import aperi_hip as A
+    from aperi_hip import meshgen as M
+    w = M.rkpm_lattice(n_nodes_edge)
+    X, conn = w["X"], w["conn"]
+    s = A.Solver()
+    s.set_mesh(A.TET4, X, conn)
+    s.set_material(A.J2, w["rho"], w["E"], w["nu"], w["yield_stress"], w["hardening"])
+    s.synchronize()
+    t0 = time.perf_counter()
+    s.rk_setup(scale)
+    s.synchronize()
+    t_setup = time.perf_counter() - t0
+    num, _, truncated = s.rk_neighbors()
+    nnz = s.rk_nnz()
+    ne, nn = len(conn), len(X)
+    s.compute_mass()
+    dt = 0.5 * M.stable_dt_estimate(X, conn, w["E"], w["nu"], w["rho"])
+    vstar = 2.0 * w["yield_stress"] / w["E"] / (steps * dt)
+    s.set_field(A.FIELD_V, M.rkpm_velocity(X, vstar, w["ramp_scale"]))
+    s.initialize()
+    s.set_profiling(True)
+    s.reset_profile()
+    t0 = time.perf_counter()
+    s.step(dt, steps)
+    s.synchronize()
+    wall = time.perf_counter() - t0
+    n0, m0 = s.profile(0)
+    n1, m1 = s.profile(1)
+    n2, m2 = s.profile(2)
+    yielded = int((s.get(A.FIELD_STATE) > 0).sum())
+    s.close()
+    lc = nnz / ne
+    bytes_per_update = 28.0 * lc + 168.0 + 200.0 * nn / ne
+    step_ms = m0 / max(n0, 1)
+    achieved = bytes_per_update * ne / (step_ms * 1e-3) / 1e9
+    return {"workload": f"{n_nodes_edge}^3-node RKPM lattice ({nn} nodes, {ne} Tet4 cells), J2, kernel scale {scale}, "
+                        f"{steps} steps",
+            "setup_s": t_setup, "mean_neighbors": float(num.mean()), "max_neighbors": int(num.max()),
+            "truncated_nodes": int(truncated), "cell_list_mean": lc,
+            "cell_updates_per_s": ne * steps / wall, "ms_per_step": 1e3 * wall / steps,
+            "kernels_ms": {"step": step_ms, "cell_force": m1 / max(n1, 1), "node_gather": m2 / max(n2, 1)},
+            "roofline": {"bound": "hbm", "bytes_per_update": bytes_per_update, "achieved": achieved,
+                         "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS},
+            "yielded_cells": yielded}
+
+
 def cpu_baseline(w, kind, budget_s=12.0):
     """Oracle (C restatement of the reference data flow) on all host cores."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -332,6 +382,10 @@ def main():
         log("[bench] batched ensemble ...")
         line["batched"] = bench_batched()
         log(f"[bench] batched: {line['batched']['speedup']:.2f}x, identical={line['batched']['identical_results']}")
+        log("[bench] rkpm ...")
+        line["rkpm"] = bench_rkpm()
+        log(f"[bench] rkpm: {line['rkpm']['cell_updates_per_s']:.3e} cell-updates/s, "
+            f"setup {line['rkpm']['setup_s']:.2f}s, roofline {line['rkpm']['roofline']['frac']:.3f}")
         log("[bench] adjoint ...")
         line["adjoint"] = bench_adjoint()
         log(f"[bench] adjoint: {line['adjoint']['gradient_over_forward']:.2f}x forward, "
