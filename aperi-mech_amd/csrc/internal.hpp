@@ -113,13 +113,13 @@ struct CellData {
     DevBuf<int64_t> node_start; // [nn + 1] node -> entries naming it
     DevBuf<int32_t> node_inc;   // unused (kept for layout stability)
     DevBuf<int32_t> inc_entry;  // [nnz] entries sorted by node, ascending entry
-    DevBuf<double> st;          // [ne * 9] -V P^T of the last force evaluation
+    DevBuf<double> g;           // [nnz * 3] per-entry force contributions of the last evaluation
     void reset() {
         ready = false;
         nnz = n_truncated = 0;
         num_nbr.release(); nbr.release(); phi.release(); radius.release(); start.release();
         nodes.release(); dphi.release(); volume.release(); inc_cell.release(); node_start.release();
-        node_inc.release(); inc_entry.release(); st.release();
+        node_inc.release(); inc_entry.release(); g.release();
     }
 };
 

@@ -45,6 +45,12 @@ __device__ __forceinline__ void ld3(const double *p, int64_t i, double *o) {
     o[2] = p[3 * i + 2];
 }
 
+__device__ __forceinline__ void st3(double *p, int64_t i, const double *o) {
+    p[3 * i] = o[0];
+    p[3 * i + 1] = o[1];
+    p[3 * i + 2] = o[2];
+}
+
 // |a - b|^2 with a fixed fused order (the oracle evaluates the same expression)
 __device__ __forceinline__ double dist2(const double *a, const double *b) {
     const double dx = a[0] - b[0], dy = a[1] - b[1], dz = a[2] - b[2];
@@ -459,6 +465,19 @@ __global__ __launch_bounds__(256) void k_count_keys(int64_t n, const int32_t *ke
     if (i < n) atomicAdd(&count[keys[i]], 1);
 }
 
+// ---- smoothed-cell force step --------------------------------------------
+// Cells: a group of 16 lanes per cell.  The lanes stride over the cell's CSR entries
+// (entry t goes to lane t mod 16, ascending within a lane), accumulate their share
+// of H = sum_t u_node(t) (x) grad phi_t with fused multiply-adds and combine the 16
+// partial sums with a xor butterfly; addition is commutative, so every lane ends
+// with the same bits and evaluates the stress update itself (no broadcast).  Each
+// entry then gets its force contribution g_t = -V P grad phi_t (3 doubles), the
+// deterministic replacement of the reference's per-entry atomic scatter
+// (include/ComputeInternalForceSmoothedCell.h:312-351).
+// Nodes: a group of 16 lanes per node sums g over the node's incidences (ascending
+// entry, strided over the lanes, then the same butterfly) and finishes the node.
+constexpr int kCellLanes = 16;
+
 struct CellForceArgs {
     int64_t ne;
     const int64_t *start;
@@ -467,28 +486,39 @@ struct CellForceArgs {
     const double *volume;
     const double *u;
     double *H, *P;
-    double *st;  // -V P^T per cell (AoS 9)
+    double *g;  // [nnz * 3] per-entry force contributions
     const double *state_n;
     double *state_np1;
     MatParams mat;
     const double *elem_lambda, *elem_mu;
 };
 
+__device__ __forceinline__ double group_sum16(double x) {
+#pragma unroll
+    for (int s = 8; s >= 1; s >>= 1) x += __shfl_xor(x, s, kCellLanes);
+    return x;
+}
+
 template <int KIND>
-__global__ __launch_bounds__(128) void k_rk_cells(CellForceArgs A) {
-    const int64_t c = (int64_t)blockIdx.x * 128 + threadIdx.x;
-    if (c >= A.ne) return;
+__global__ __launch_bounds__(256) void k_rk_cells(CellForceArgs A) {
+#pragma clang fp contract(off)
+    const int l = threadIdx.x & (kCellLanes - 1);
+    const int64_t c_raw = (int64_t)blockIdx.x * (256 / kCellLanes) + (threadIdx.x / kCellLanes);
+    const bool live = c_raw < A.ne;
+    const int64_t c = live ? c_raw : A.ne - 1;  // idle groups recompute the last cell and store nothing
+    const int64_t t0 = A.start[c], t1 = A.start[c + 1];
     double H[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, P[9];
-    const int64_t t1 = A.start[c + 1];
-    for (int64_t t = A.start[c]; t < t1; ++t) {
+    for (int64_t t = t0 + l; t < t1; t += kCellLanes) {
         double uj[3], g[3];
         ld3(A.u, A.nodes[t], uj);
         ld3(A.dphi, t, g);
 #pragma unroll
         for (int a = 0; a < 3; ++a)
 #pragma unroll
-            for (int b = 0; b < 3; ++b) H[3 * a + b] += uj[a] * g[b];
+            for (int b = 0; b < 3; ++b) H[3 * a + b] = __fma_rn(uj[a], g[b], H[3 * a + b]);
     }
+#pragma unroll
+    for (int i = 0; i < 9; ++i) H[i] = group_sum16(H[i]);
     MatParams m = A.mat;
     if (A.elem_lambda) {
         m.lambda = A.elem_lambda[c];
@@ -497,26 +527,31 @@ __global__ __launch_bounds__(128) void k_rk_cells(CellForceArgs A) {
     double sn = 0.0, snp1 = 0.0;
     if constexpr (KIND == APHIP_J2) sn = A.state_n[c];
     stress<KIND>(H, m, sn, snp1, P);
-    if constexpr (KIND == APHIP_J2) A.state_np1[c] = snp1;
-    const double V = A.volume[c];
+    if (live && l == 0) {
+        if constexpr (KIND == APHIP_J2) A.state_np1[c] = snp1;
 #pragma unroll
-    for (int i = 0; i < 9; ++i) {
-        store_once(A.H + i * A.ne + c, H[i]);
-        store_once(A.P + i * A.ne + c, P[i]);
+        for (int i = 0; i < 9; ++i) {
+            store_once(A.H + i * A.ne + c, H[i]);
+            store_once(A.P + i * A.ne + c, P[i]);
+        }
     }
+    const double mv = -A.volume[c];
+    if (live) {
+        for (int64_t t = t0 + l; t < t1; t += kCellLanes) {
+            double g[3], f[3];
+            ld3(A.dphi, t, g);
 #pragma unroll
-    for (int a = 0; a < 3; ++a)
-#pragma unroll
-        for (int b = 0; b < 3; ++b) A.st[9 * c + 3 * a + b] = P[3 * b + a] * -V;
+            for (int a = 0; a < 3; ++a) f[a] = mv * dot3(P[3 * a], g[0], P[3 * a + 1], g[1], P[3 * a + 2], g[2]);
+            st3(A.g, t, f);
+        }
+    }
 }
 
 struct RkNodeArgs {
     int64_t nn;
     const int64_t *inc_start;
     const int32_t *inc_entry;
-    const int32_t *entry_cell;
-    const double *dphi;
-    const double *st;
+    const double *g;
     const double *mass;
     const uint8_t *fixed;
     double *f, *a, *v;
@@ -525,18 +560,22 @@ struct RkNodeArgs {
 
 template <int MODE>
 __global__ __launch_bounds__(256) void k_rk_nodes(RkNodeArgs A) {
-    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (i >= A.nn) return;
+    const int l = threadIdx.x & (kCellLanes - 1);
+    const int64_t i_raw = (int64_t)blockIdx.x * (256 / kCellLanes) + (threadIdx.x / kCellLanes);
+    const bool live = i_raw < A.nn;
+    const int64_t i = live ? i_raw : A.nn - 1;
     double f[3] = {0.0, 0.0, 0.0};
     const int64_t t1 = A.inc_start[i + 1];
-    for (int64_t t = A.inc_start[i]; t < t1; ++t) {
-        const int64_t en = A.inc_entry[t];
-        const double *s = A.st + 9 * (int64_t)A.entry_cell[en];
+    for (int64_t t = A.inc_start[i] + l; t < t1; t += kCellLanes) {
         double g[3];
-        ld3(A.dphi, en, g);
-#pragma unroll
-        for (int b = 0; b < 3; ++b) f[b] += g[0] * s[b] + g[1] * s[3 + b] + g[2] * s[6 + b];
+        ld3(A.g, A.inc_entry[t], g);
+        f[0] += g[0];
+        f[1] += g[1];
+        f[2] += g[2];
     }
+#pragma unroll
+    for (int d = 0; d < 3; ++d) f[d] = group_sum16(f[d]);
+    if (!live || l != 0) return;
     A.f[3 * i] = f[0];
     A.f[3 * i + 1] = f[1];
     A.f[3 * i + 2] = f[2];
@@ -726,7 +765,7 @@ void rk_setup(Ctx &c, double scale) {
         APHIP_HIP(hipMemcpyAsync(D.inc_entry.p, inc.p, sizeof(int32_t) * std::max<int64_t>(D.nnz, 1),
                                  hipMemcpyDeviceToDevice, c.stream));
     }
-    D.st.alloc(9 * std::max<int64_t>(ne, 1));
+    D.g.alloc(3 * std::max<int64_t>(D.nnz, 1));
     APHIP_HIP(hipStreamSynchronize(c.stream));
     D.ready = true;
 }
@@ -734,16 +773,16 @@ void rk_setup(Ctx &c, double scale) {
 void rk_cell_force(Ctx &c) {
     CellData &D = c.cells;
     if (c.ne == 0) return;
-    CellForceArgs A{c.ne, D.start.p, D.nodes.p, D.dphi.p, D.volume.p, c.u.p, c.H.p, c.P.p, D.st.p,
+    CellForceArgs A{c.ne, D.start.p, D.nodes.p, D.dphi.p, D.volume.p, c.u.p, c.H.p, c.P.p, D.g.p,
                     c.state[1 - c.state_np1].p, c.state[c.state_np1].p,
                     MatParams{c.mat.lambda, c.mat.mu, c.mat.yield_stress, c.mat.hardening},
                     c.has_fields ? c.elem_lambda.p : nullptr, c.has_fields ? c.elem_mu.p : nullptr};
-    const dim3 g = grid1(c.ne, 128);
+    const dim3 g = grid1(c.ne, 256 / kCellLanes);
     switch (c.mat.kind) {
-        case APHIP_LINEAR_ELASTIC: hipLaunchKernelGGL(k_rk_cells<APHIP_LINEAR_ELASTIC>, g, dim3(128), 0, c.stream, A); break;
-        case APHIP_NEO_HOOKEAN: hipLaunchKernelGGL(k_rk_cells<APHIP_NEO_HOOKEAN>, g, dim3(128), 0, c.stream, A); break;
-        case APHIP_J2: hipLaunchKernelGGL(k_rk_cells<APHIP_J2>, g, dim3(128), 0, c.stream, A); break;
-        case APHIP_SVK: hipLaunchKernelGGL(k_rk_cells<APHIP_SVK>, g, dim3(128), 0, c.stream, A); break;
+        case APHIP_LINEAR_ELASTIC: hipLaunchKernelGGL(k_rk_cells<APHIP_LINEAR_ELASTIC>, g, dim3(256), 0, c.stream, A); break;
+        case APHIP_NEO_HOOKEAN: hipLaunchKernelGGL(k_rk_cells<APHIP_NEO_HOOKEAN>, g, dim3(256), 0, c.stream, A); break;
+        case APHIP_J2: hipLaunchKernelGGL(k_rk_cells<APHIP_J2>, g, dim3(256), 0, c.stream, A); break;
+        case APHIP_SVK: hipLaunchKernelGGL(k_rk_cells<APHIP_SVK>, g, dim3(256), 0, c.stream, A); break;
         default: throw Error(APHIP_ERR_INVALID, "material not set");
     }
     APHIP_HIP(hipGetLastError());
@@ -751,9 +790,9 @@ void rk_cell_force(Ctx &c) {
 
 void rk_node_update(Ctx &c, double dt, int mode) {
     CellData &D = c.cells;
-    RkNodeArgs A{c.nn, D.node_start.p, D.inc_entry.p, D.inc_cell.p, D.dphi.p, D.st.p, c.mass.p,
+    RkNodeArgs A{c.nn, D.node_start.p, D.inc_entry.p, D.g.p, c.mass.p,
                  c.has_fixed ? c.fixed.p : nullptr, c.f.p, c.a.p, c.v.p, dt};
-    const dim3 g = grid1(c.nn, 256);
+    const dim3 g = grid1(c.nn, 256 / kCellLanes);
     if (mode == 0) hipLaunchKernelGGL(k_rk_nodes<0>, g, dim3(256), 0, c.stream, A);
     else if (mode == 1) hipLaunchKernelGGL(k_rk_nodes<1>, g, dim3(256), 0, c.stream, A);
     else hipLaunchKernelGGL(k_rk_nodes<2>, g, dim3(256), 0, c.stream, A);
