@@ -386,6 +386,15 @@ class ExplicitSolver {
         return m_time;
     }
     double GetTime() const { return m_time; }
+    // PowerMethodProcessor::ComputeStableTimeIncrement (include/PowerMethodProcessor.h:331-470)
+    // with the reference's defaults; the simulation state is left as it was.
+    double ComputeStableTimeIncrement(int num_iterations = 50, double epsilon = 1e-5, double tolerance = 1e-3,
+                                      double variation_tolerance = 1e-7, uint64_t seed = 20261202) {
+        double dt = 0.0;
+        Check(aphip_power_method_time_increment(Context(), num_iterations, epsilon, tolerance, variation_tolerance,
+                                                seed, &dt, nullptr, nullptr));
+        return dt;
+    }
 
    private:
     aphip_ctx *Context() const { return m_contribution->GetElement()->Context(); }

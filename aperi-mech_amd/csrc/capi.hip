@@ -320,6 +320,122 @@ static void step_once(Ctx &c, double dt) {
 
 void step_once_tiles(Ctx &c, double dt) { step_once(c, dt); }
 
+// f, H, P and the NP1 state from the current u (ComputeInternalForceAllElements).
+static void internal_force_eval(Ctx &c) {
+    if (use_tiles(c)) {
+        tile_run(c, 0.0, 0);
+    } else {
+        force_pass(c);
+        node_pass(c, 0.0, 0);
+    }
+}
+
+struct PowerMethodResult {
+    double dt;
+    int iterations;
+    bool converged;
+};
+
+// Stable time increment 2 / sqrt(lambda_max(M^-1 K)) by the reference's power
+// method (include/PowerMethodProcessor.h:331-470, ComputeStableTimeIncrement): the
+// force is linearised about the current displacement by a perturbation of eps
+// times the node's max edge length; convergence when successive increments agree
+// to `tolerance`, or when the variance of the last five is below
+// `variation_tolerance` x their mean; otherwise the smallest of the last five.
+// The reference's oscillation test (:424-437) takes its else branch at i = 0 and so
+// never reports oscillation; that outcome is kept.  u, f, H, P and the NP1 state are
+// restored afterwards, so the call leaves the simulation state untouched.
+static PowerMethodResult power_method(Ctx &c, int n_iter, double eps, double tol, double vtol, uint64_t seed) {
+    const int64_t n3 = 3 * c.nn;
+    DevBuf<double> u_in, f_in, f0, v, l, H, P, st, partial;
+    DevBuf<int32_t> caller;
+    u_in.alloc(n3);
+    f_in.alloc(n3);
+    f0.alloc(n3);
+    v.alloc(n3);
+    l.alloc(c.nn);
+    caller.upload(c.node_iperm.data(), c.nn, c.stream);
+    const int64_t ns = c.state_count();
+    APHIP_HIP(hipMemcpyAsync(u_in.p, c.u.p, sizeof(double) * n3, hipMemcpyDeviceToDevice, c.stream));
+    APHIP_HIP(hipMemcpyAsync(f_in.p, c.f.p, sizeof(double) * n3, hipMemcpyDeviceToDevice, c.stream));
+    if (c.ne) {
+        H.alloc(9 * c.ne);
+        P.alloc(9 * c.ne);
+        APHIP_HIP(hipMemcpyAsync(H.p, c.H.p, sizeof(double) * 9 * c.ne, hipMemcpyDeviceToDevice, c.stream));
+        APHIP_HIP(hipMemcpyAsync(P.p, c.P.p, sizeof(double) * 9 * c.ne, hipMemcpyDeviceToDevice, c.stream));
+    }
+    if (ns) {
+        st.alloc(ns);
+        APHIP_HIP(hipMemcpyAsync(st.p, c.state[c.state_np1].p, sizeof(double) * ns, hipMemcpyDeviceToDevice,
+                                 c.stream));
+    }
+    c.graph_reset();
+    pm_max_edge(c, l.p);
+    pm_start(c, caller.p, seed, eps, l.p, v.p);
+    internal_force_eval(c);  // F(u)
+    APHIP_HIP(hipMemcpyAsync(f0.p, c.f.p, sizeof(double) * n3, hipMemcpyDeviceToDevice, c.stream));
+
+    const int window = 5;
+    std::vector<double> recent(window, 0.0);
+    double dt_np1 = 1.0e-16, dt_n = 0.0;
+    bool converged = false, oscillating = false;
+    int used = n_iter;
+    for (int k = 0; k < n_iter; ++k) {
+        dt_n = dt_np1;
+        pm_perturb(c, u_in.p, v.p, l.p, eps);
+        internal_force_eval(c);
+        const double lambda = pm_next(c, f0.p, l.p, eps, v.p, partial);
+        pm_scale(c, lambda, v.p);
+        dt_np1 = 2.0 / std::sqrt(lambda);
+        recent[k % window] = dt_np1;
+        if (std::fabs(dt_np1 - dt_n) < tol * dt_np1) {
+            converged = true;
+            used = k + 1;
+            break;
+        }
+        if (k >= window) {
+            double mean = 0.0;
+            for (double x : recent) mean += x;
+            mean /= window;
+            double var = 0.0;
+            for (double x : recent) var += (x - mean) * (x - mean);
+            var /= (window - 1);
+            if (k > window + 1) {
+                double alt = 0.0;
+                for (int i = 0; i < window - 1; ++i) {
+                    const double diff = recent[i + 1] - recent[i];
+                    if (i > 0 && diff * alt < 0) {
+                        oscillating = true;
+                    } else {
+                        oscillating = false;
+                        break;
+                    }
+                    alt = diff;
+                }
+            }
+            if (var < vtol * mean || (oscillating && var < vtol * mean * 2.0)) {
+                converged = true;
+                used = k + 1;
+                break;
+            }
+        }
+    }
+    const double wmin = *std::min_element(recent.begin(), recent.end());
+    const double dt = converged ? (oscillating ? wmin : dt_np1) : wmin;
+
+    APHIP_HIP(hipMemcpyAsync(c.u.p, u_in.p, sizeof(double) * n3, hipMemcpyDeviceToDevice, c.stream));
+    APHIP_HIP(hipMemcpyAsync(c.f.p, f_in.p, sizeof(double) * n3, hipMemcpyDeviceToDevice, c.stream));
+    if (c.ne) {
+        APHIP_HIP(hipMemcpyAsync(c.H.p, H.p, sizeof(double) * 9 * c.ne, hipMemcpyDeviceToDevice, c.stream));
+        APHIP_HIP(hipMemcpyAsync(c.P.p, P.p, sizeof(double) * 9 * c.ne, hipMemcpyDeviceToDevice, c.stream));
+    }
+    if (ns)
+        APHIP_HIP(hipMemcpyAsync(c.state[c.state_np1].p, st.p, sizeof(double) * ns, hipMemcpyDeviceToDevice,
+                                 c.stream));
+    APHIP_HIP(hipStreamSynchronize(c.stream));
+    return PowerMethodResult{dt, used, converged};
+}
+
 static void get_node_field(Ctx &c, const DevBuf<double> &b, double *out) {
     std::vector<double> h(3 * c.nn);
     b.download(h.data(), h.size(), c.stream);
@@ -619,13 +735,24 @@ int aphip_internal_force(aphip_ctx *c, double time, double dt) {
     (void)time;
     (void)dt;
     APHIP_HIP(hipSetDevice(c->device));
-    if (use_tiles(*c)) {
-        tile_run(*c, 0.0, 0);
-    } else {
-        force_pass(*c);
-        node_pass(*c, 0.0, 0);
-    }
+    internal_force_eval(*c);
     APHIP_HIP(hipStreamSynchronize(c->stream));
+    APHIP_API_END
+}
+
+int aphip_power_method_time_increment(aphip_ctx *c, int32_t num_iterations, double epsilon, double tolerance,
+                                      double variation_tolerance, uint64_t seed, double *time_increment,
+                                      int32_t *iterations, int32_t *converged) {
+    APHIP_API_BEGIN
+    APHIP_REQUIRE(c != nullptr && time_increment != nullptr, "NULL argument");
+    require_ready(*c, true);
+    APHIP_REQUIRE(num_iterations > 0 && epsilon > 0.0 && tolerance > 0.0 && variation_tolerance >= 0.0,
+                  "power method needs positive iterations, epsilon and tolerance");
+    APHIP_HIP(hipSetDevice(c->device));
+    const PowerMethodResult r = power_method(*c, num_iterations, epsilon, tolerance, variation_tolerance, seed);
+    *time_increment = r.dt;
+    if (iterations) *iterations = r.iterations;
+    if (converged) *converged = r.converged ? 1 : 0;
     APHIP_API_END
 }
 

@@ -279,6 +279,14 @@ void rk_mass(Ctx &c, double *total);
 void prof_begin(Ctx &c, int which);
 void prof_end(Ctx &c, int which);
 
+// ---- power-method stable increment (power.hip) ----
+void pm_max_edge(Ctx &c, double *l);
+void pm_start(Ctx &c, const int32_t *caller_id, uint64_t seed, double eps, const double *l, double *v);
+void pm_perturb(Ctx &c, const double *u_in, const double *v, const double *l, double eps);
+// v = (f - f0) / (l eps m), returns |v|_2
+double pm_next(Ctx &c, const double *f0, const double *l, double eps, double *v, DevBuf<double> &partial);
+void pm_scale(Ctx &c, double norm, double *v);  // v /= norm
+
 // ---- copy yardstick (probe.hip): device milliseconds per copy of nbytes ----
 double bandwidth_probe(int device, int64_t nbytes, int reps);
 

@@ -39,7 +39,7 @@ EXPORTED = [
     "aphip_batch_add", "aphip_batch_end", "aphip_batch_ranges", "aphip_internal_force_vjp",
     "aphip_adjoint_forward", "aphip_adjoint_backward", "aphip_rk_setup", "aphip_rk_get_neighbors", "aphip_rk_get_function_values",
     "aphip_rk_get_cells", "aphip_set_profiling", "aphip_get_profile", "aphip_reset_profile",
-    "aphip_bandwidth_probe",
+    "aphip_bandwidth_probe", "aphip_power_method_time_increment",
 ]
 
 
@@ -197,6 +197,17 @@ class Solver:
         out = np.zeros(self._shape(field))
         _check(self.L.aphip_get_field(self.ctx, field, _p(out)))
         return out
+
+    def power_method_time_increment(self, num_iterations=50, epsilon=1e-5, tolerance=1e-3,
+                                    variation_tolerance=1e-7, seed=20261202):
+        """(dt, iterations, converged) of the power-method stable increment."""
+        dt, it, cv = ctypes.c_double(), ctypes.c_int32(), ctypes.c_int32()
+        _check(self.L.aphip_power_method_time_increment(self.ctx, int(num_iterations), ctypes.c_double(epsilon),
+                                                        ctypes.c_double(tolerance),
+                                                        ctypes.c_double(variation_tolerance),
+                                                        ctypes.c_uint64(seed), ctypes.byref(dt), ctypes.byref(it),
+                                                        ctypes.byref(cv)))
+        return dt.value, it.value, bool(cv.value)
 
     # ---- hot path ----
     def internal_force(self, time=0.0, dt=0.0):

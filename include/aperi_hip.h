@@ -141,6 +141,17 @@ int aphip_initialize(aphip_ctx *ctx);
  * velocity BC, u += dt v, internal force, a = f/m, acceleration BC, v += dt/2 a. */
 int aphip_step(aphip_ctx *ctx, double time_increment, int32_t n_steps);
 
+/* Stable time increment 2 / sqrt(lambda_max(M^-1 K)) by the power method about the
+ * current displacement: replaces PowerMethodProcessor::ComputeStableTimeIncrement
+ * (include/PowerMethodProcessor.h:331-470) of the power-method time stepper
+ * (include/TimeStepper.h:138-229); the reference's defaults are 50 iterations,
+ * epsilon 1e-5, tolerance 1e-3, variation tolerance 1e-7.  The start vector is
+ * uniform in [-1, 1] per DOF from a counter-based generator keyed by `seed` and the
+ * caller's DOF index.  u, f, H, P and the state are restored on return. */
+int aphip_power_method_time_increment(aphip_ctx *ctx, int32_t num_iterations, double epsilon, double tolerance,
+                                      double variation_tolerance, uint64_t seed, double *time_increment,
+                                      int32_t *iterations, int32_t *converged);
+
 /* Element-tile shape of the fused step: elements per tile and workgroup size.
  * Tet4: one element per thread (64, 128, 256 or 512).  Hex8: one element per
  * thread (64 or 128), or eight lanes per element, one per Gauss point

@@ -151,3 +151,103 @@ done:
     free(vec); free(up); free(f0); free(f1); free(st); free(st2);
     return lambda > 0.0 ? 2.0 / sqrt(lambda) : 0.0;
 }
+
+/*
+ * The reference's power-method stable increment, restated step by step
+ * (include/PowerMethodProcessor.h:331-470 ComputeStableTimeIncrement,
+ *  InitializeEigenvector :195-221, PerturbDisplacementCoefficients :230-260,
+ *  ComputeNextEigenvector :262-330; NormalizeField = L2 norm,
+ *  include/EntityProcessor.h:676-697):
+ *   v_i = U[-1,1] * eps * l_i                      (start; l = max edge length per node)
+ *   repeat: u_eps = u + v l eps (fixed DOFs: u); F1 = F(u_eps)
+ *           v = (F1 - F(u)) / (l eps m) (fixed DOFs: 0); lambda = |v|_2; v /= lambda
+ *           dt = 2 / sqrt(lambda); stop when |dt - dt_prev| < tol dt, or when the variance
+ *           of the last five dt is below vtol x their mean
+ *   result: the last dt if converged, else the smallest of the last five.
+ * The start vector is splitmix64(seed ^ dof) (the reference draws Kokkos XorShift64
+ * numbers keyed by node id: the start differs, the converged eigenvalue does not;
+ * parity of the start vector itself is unpinned).  The reference's oscillation test
+ * (:424-437) takes its else branch at i = 0 and never reports oscillation; restated.
+ */
+int or_power_method(const or_problem *p, const double *u_in, const double *state_n, const double *max_edge,
+                    int n_iter, double epsilon, double tolerance, double variation_tolerance, uint64_t seed,
+                    int nthreads, double *dt_out, int *iterations_out, int *converged_out) {
+    const int64_t N = p->n_nodes, M = 3 * N;
+    const int64_t ns = p->mat.kind == OR_J2 ? (p->etype == OR_HEX8 && p->mat.state_per_qp ? 8 : 1) * p->n_elem : 1;
+    double *vec = (double *)calloc((size_t)M, sizeof(double));
+    double *up = (double *)calloc((size_t)M, sizeof(double));
+    double *f0 = (double *)calloc((size_t)M, sizeof(double));
+    double *f1 = (double *)calloc((size_t)M, sizeof(double));
+    double *sn = (double *)calloc((size_t)ns, sizeof(double));
+    double *snp1 = (double *)calloc((size_t)ns, sizeof(double));
+    int rc = -1;
+    if (!vec || !up || !f0 || !f1 || !sn || !snp1) goto done;
+    if (state_n && p->mat.kind == OR_J2) memcpy(sn, state_n, sizeof(double) * (size_t)ns);
+    for (int64_t i = 0; i < M; ++i) {
+        const double r = (double)(splitmix64(seed ^ (uint64_t)i) >> 11) * (1.0 / 9007199254740992.0);
+        vec[i] = (2.0 * r - 1.0) * (epsilon * max_edge[i / 3]);
+    }
+    if ((rc = compute_force(p, u_in, sn, snp1, NULL, NULL, f0, nthreads)) != 0) goto done;
+    {
+        enum { W = 5 };
+        double recent[W] = {0, 0, 0, 0, 0};
+        double dt_np1 = 1.0e-16, dt_n;
+        int converged = 0, oscillating = 0, used = n_iter;
+        for (int k = 0; k < n_iter; ++k) {
+            dt_n = dt_np1;
+            for (int64_t i = 0; i < M; ++i)
+                up[i] = (p->fixed && p->fixed[i]) ? u_in[i] : fma(vec[i] * max_edge[i / 3], epsilon, u_in[i]);
+            if ((rc = compute_force(p, up, sn, snp1, NULL, NULL, f1, nthreads)) != 0) goto done;
+            double s = 0.0;
+            for (int64_t i = 0; i < M; ++i) {
+                const double w = (p->fixed && p->fixed[i]) ? 0.0
+                                                             : (f1[i] - f0[i]) / ((max_edge[i / 3] * epsilon) * p->mass[i / 3]);
+                vec[i] = w;
+                s = fma(w, w, s);
+            }
+            const double lambda = sqrt(s);
+            for (int64_t i = 0; i < M; ++i) vec[i] /= lambda;
+            dt_np1 = 2.0 / sqrt(lambda);
+            recent[k % W] = dt_np1;
+            if (fabs(dt_np1 - dt_n) < tolerance * dt_np1) {
+                converged = 1;
+                used = k + 1;
+                break;
+            }
+            if (k >= W) {
+                double mean = 0.0, var = 0.0;
+                for (int i = 0; i < W; ++i) mean += recent[i];
+                mean /= W;
+                for (int i = 0; i < W; ++i) var += (recent[i] - mean) * (recent[i] - mean);
+                var /= (W - 1);
+                if (k > W + 1) {
+                    double alt = 0.0;
+                    for (int i = 0; i < W - 1; ++i) {
+                        const double diff = recent[i + 1] - recent[i];
+                        if (i > 0 && diff * alt < 0) {
+                            oscillating = 1;
+                        } else {
+                            oscillating = 0;
+                            break;
+                        }
+                        alt = diff;
+                    }
+                }
+                if (var < variation_tolerance * mean || (oscillating && var < variation_tolerance * mean * 2.0)) {
+                    converged = 1;
+                    used = k + 1;
+                    break;
+                }
+            }
+        }
+        double wmin = recent[0];
+        for (int i = 1; i < W; ++i) wmin = recent[i] < wmin ? recent[i] : wmin;
+        *dt_out = converged ? (oscillating ? wmin : dt_np1) : wmin;
+        if (iterations_out) *iterations_out = used;
+        if (converged_out) *converged_out = converged;
+        rc = 0;
+    }
+done:
+    free(vec); free(up); free(f0); free(f1); free(sn); free(snp1);
+    return rc;
+}

@@ -261,6 +261,22 @@ class Solver:
         return lib().or_power_method_dt(ctypes.byref(self.p), n_iter, ctypes.c_double(epsilon),
                                         ctypes.c_uint64(seed), nthreads)
 
+    def power_method(self, max_edge, u_in=None, n_iter=50, epsilon=1e-5, tolerance=1e-3,
+                     variation_tolerance=1e-7, seed=20261202, nthreads=1):
+        """The reference's power-method increment about u_in (default: the solver's u):
+        returns (dt, iterations, converged)."""
+        u = _f64(self.u if u_in is None else u_in)
+        l = _f64(max_edge)
+        dt, it, cv = ctypes.c_double(), ctypes.c_int(), ctypes.c_int()
+        st = None if self.state.size == 0 else _f64(self.state)
+        rc = lib().or_power_method(ctypes.byref(self.p), _p(u), _p(st), _p(l), n_iter, ctypes.c_double(epsilon),
+                                   ctypes.c_double(tolerance), ctypes.c_double(variation_tolerance),
+                                   ctypes.c_uint64(seed), nthreads, ctypes.byref(dt), ctypes.byref(it),
+                                   ctypes.byref(cv))
+        if rc != 0:
+            raise RuntimeError("or_power_method failed")
+        return dt.value, it.value, bool(cv.value)
+
 
 # ---------------- RKPM ----------------
 

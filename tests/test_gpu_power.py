@@ -1,0 +1,80 @@
+"""GPU power-method stable increment (aphip_power_method_time_increment) against the
+oracle's restatement of the reference's power method (include/PowerMethodProcessor.h:
+331-470): same seeded start vector, same iteration count and convergence flag, the
+increment to rounding, and the simulation state untouched by the call."""
+import numpy as np
+import pytest
+
+import oracle as O
+import aperi_hip as A
+from aperi_hip import meshgen as M
+from conftest import rel_err
+
+pytestmark = pytest.mark.gpu
+
+MATS = {"le": A.LINEAR_ELASTIC, "nh": A.NEO_HOOKEAN}
+E, NU, RHO = 200e9, 0.3, 7850.0
+
+
+def _mesh(etype, n, seed):
+    X, hc = M.hex_lattice(n, n, n)
+    X = M.jitter(X, 1.0 / n, 0.15, seed, 1)
+    return X, (hc if etype == A.HEX8 else M.split_tets(X, hc))
+
+
+@pytest.mark.parametrize("etype", [A.TET4, A.HEX8], ids=["tet4", "hex8"])
+@pytest.mark.parametrize("mat", list(MATS))
+def test_power_method_matches_oracle(gpu_available, etype, mat):
+    kind = MATS[mat]
+    X, conn = _mesh(etype, 5, 17)
+    lam, mu = O.lame(E, NU)
+    fixed = np.zeros(X.shape, dtype=np.uint8)
+    fixed[X[:, 2] < 1e-12] = 1
+    u = 1e-3 * (2.0 * M.uniform01(8, 1, X.size).reshape(X.shape) - 1.0)
+    u[fixed.astype(bool)] = 0.0
+    mass = O.lumped_mass(etype, X, conn, RHO)
+    ref = O.Solver(etype, X, conn, kind, lam, mu, mass, fixed)
+    l = O.max_edge_length(X, conn)
+    print(f"power method etype={etype} mat={mat}", flush=True)
+    with A.Solver() as s:
+        s.set_mesh(etype, X, conn)
+        s.set_material(kind, RHO, E, NU)
+        s.compute_mass()
+        s.set_fixed(fixed)
+        s.set_field(A.FIELD_U, u)
+        s.internal_force()
+        before = {f: s.get(f) for f in (A.FIELD_U, A.FIELD_F, A.FIELD_H, A.FIELD_P)}
+        for kw in (dict(), dict(num_iterations=400, tolerance=1e-12, variation_tolerance=0.0)):
+            dt_ref, it_ref, cv_ref = ref.power_method(l, u_in=u, n_iter=kw.get("num_iterations", 50),
+                                                      tolerance=kw.get("tolerance", 1e-3),
+                                                      variation_tolerance=kw.get("variation_tolerance", 1e-7))
+            dt, it, cv = s.power_method_time_increment(**kw)
+            if not kw:  # the reference's tolerances: the same stopping iteration
+                assert (it, cv) == (it_ref, cv_ref)
+                assert dt == pytest.approx(dt_ref, rel=1e-9)
+            else:  # stopping at 1e-12 sits at the rounding noise of the force difference
+                assert dt == pytest.approx(dt_ref, rel=1e-8)
+        for f, b in before.items():
+            assert np.array_equal(s.get(f), b), f
+
+
+def test_power_method_rkpm_matches_oracle(gpu_available):
+    X, conn = _mesh(A.TET4, 4, 5)
+    scale = 1.5
+    lam, mu = O.lame(70e9, 0.33)
+    with A.Solver() as s:
+        s.set_mesh(A.TET4, X, conn)
+        s.set_material(A.J2, 2700.0, 70e9, 0.33, 300e6, 1e9)
+        s.rk_setup(scale)
+        s.compute_mass()
+        num, nbr, _ = s.rk_neighbors()
+        phi = s.rk_function_values()
+        cells = O.rk_cells(X, conn, num, nbr, phi)
+        mass = O.rk_mass(conn, cells[3], 2700.0, num, nbr, phi)
+        ref = O.Solver(-1, X, None, O.J2, lam, mu, mass, None, 300e6, 1e9, cells=cells)
+        l = O.max_edge_length(X, conn)
+        dt_ref, it_ref, cv_ref = ref.power_method(l)
+        dt, it, cv = s.power_method_time_increment()
+        assert (it, cv) == (it_ref, cv_ref)
+        assert dt == pytest.approx(dt_ref, rel=1e-9)
+        assert rel_err(s.get(A.FIELD_MASS), mass) < 1e-12

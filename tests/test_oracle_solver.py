@@ -133,3 +133,38 @@ def test_power_method_dt_bounds_the_estimate(etype, jit):
     lmax = np.linalg.eigvalsh(K / np.repeat(mt, 3)[:, None]).max()
     st = O.Solver(0, Xt, ct, O.LINEAR_ELASTIC, lam, mu, mt)
     assert st.power_method_dt(n_iter=500) == pytest.approx(2 / np.sqrt(lmax), rel=1e-6)
+
+
+@pytest.mark.parametrize("etype", [0, 1], ids=["tet4", "hex8"])
+def test_reference_power_method_against_dense_eigenproblem(etype):
+    """The restated reference power method (per-node edge-length scaling, convergence
+    tests, result rule) reproduces 2/sqrt(lambda_max(M^-1 K)) of the dense matrix:
+    tightly with a tight tolerance; at the reference's tolerance (successive increments
+    within 1e-3) the iteration stops early and lands within a few percent."""
+    X, hc = M.hex_lattice(3, 3, 3)
+    X = M.jitter(X, 1.0 / 3, 0.15, 4, 1)
+    conn = hc if etype == 1 else M.split_tets(X, hc)
+    E, nu, rho = 200e9, 0.3, 7850.0
+    lam, mu = O.lame(E, nu)
+    m = O.lumped_mass(etype, X, conn, rho)
+    s = O.Solver(etype, X, conn, O.LINEAR_ELASTIC, lam, mu, m)
+    n = X.size
+    K = np.zeros((n, n))
+    for j in range(n):
+        u = np.zeros(n)
+        u[j] = 1e-7
+        f, *_ = O.internal_force(etype, X, conn, O.LINEAR_ELASTIC, lam, mu, u.reshape(-1, 3))
+        K[:, j] = -f.ravel() / 1e-7
+    Minv = 1.0 / np.repeat(m, 3)
+    lmax = np.linalg.eigvals(Minv[:, None] * K).real.max()
+    dt_exact = 2.0 / np.sqrt(lmax)
+    l = O.max_edge_length(X, conn)
+    dt, it, conv = s.power_method(l)
+    assert conv and it < 50
+    assert abs(dt - dt_exact) < 0.1 * dt_exact
+    dt_tight, it_tight, _ = s.power_method(l, n_iter=3000, tolerance=1e-13, variation_tolerance=0.0)
+    assert dt_tight == pytest.approx(dt_exact, rel=1e-6)
+    # fewer than five iterations without convergence: the smallest of the window of five,
+    # whose unfilled slots are zero (the reference's result rule)
+    dt3, it3, conv3 = s.power_method(l, n_iter=3, tolerance=1e-15, variation_tolerance=0.0)
+    assert not conv3 and it3 == 3 and dt3 == 0.0
