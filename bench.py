@@ -198,11 +198,12 @@ def bench_adjoint(n=64, steps=200, interval=20, n_check=32, h=1e-6):
     t0 = time.perf_counter()
     Eb, nub, v0b = s.adjoint_backward(u_T - u_star)
     t_bwd = time.perf_counter() - t0
-    # central differences on random parameters: 12 E, 10 nu, 10 initial-velocity components
+    # central differences on random parameters (of 32: 12 E, 10 nu, 10 initial-velocity components)
     rng = np.random.default_rng(20261202)
-    picks = ([("E", int(i)) for i in rng.choice(len(conn), 12, replace=False)] +
-             [("nu", int(i)) for i in rng.choice(len(conn), 10, replace=False)] +
-             [("v0", int(i)) for i in rng.choice(X.size, n_check - 22, replace=False)])
+    n_e, n_nu = (12 * n_check) // 32, (10 * n_check) // 32
+    picks = ([("E", int(i)) for i in rng.choice(len(conn), n_e, replace=False)] +
+             [("nu", int(i)) for i in rng.choice(len(conn), n_nu, replace=False)] +
+             [("v0", int(i)) for i in rng.choice(X.size, n_check - n_e - n_nu, replace=False)])
     def central_difference(kind, i, hrel):
         E, nu, v0 = w["E"].copy(), w["nu"].copy(), w["v0"].copy()
         arr = {"E": E, "nu": nu, "v0": v0.reshape(-1)}[kind]
