@@ -90,42 +90,112 @@ APHIP_DEV void hex8_sign(int k, int &s0, int &s1, int &s2) {
     s2 = (k >= 4) ? 1 : -1;
 }
 
-// Hex8 2x2x2; state_n / state_np1 point at the element's state slot(s) (per_qp: 8 slots).
+// Hex8 2x2x2 in modal form.  A trilinear field is a combination of the eight
+// corner modes 1, s0, s1, s2, s0 s1, s0 s2, s1 s2, s0 s1 s2 (s_d = the corner's sign
+// along parametric axis d); its Walsh-Hadamard transform over the corners gives the
+// mode coefficients, and at a Gauss point p = (p0, p1, p2)
+//   8 dX/dxi0 = M[s0] + p1 M[s0s1] + p2 M[s0s2] + p1 p2 M[s0s1s2]   (and cyclically),
+// which replaces the 8-corner sums of J and grad u by 3 multiply-adds per entry.
+// The nodal force f_k = -sum_q dN_k(q) Q_q is accumulated in the same seven modes
+// and returned to the corners by one inverse transform.  The 1/8 of the shape
+// function derivatives is carried as exact powers of two (J and grad u scaled by 8,
+// J^-1 by 1/8, w = det / 512), which leaves every rounding unchanged.
+// Gauss points in the order of include/ElementHexahedron8.h:107-115, w = detJ;
+// stored H / P are those of the last point (ComputeInternalForceGaussian.h:89, :99-107).
+
+// corner k (hex8_sign order) -> binary index b0 + 2 b1 + 4 b2 with b_d = (s_d + 1) / 2
+APHIP_DEV constexpr int hex8_bin(int k) { return k == 2 ? 3 : k == 3 ? 2 : k == 6 ? 7 : k == 7 ? 6 : k; }
+
+// w[S] = sum over corners of prod_{d in S} s_d * v; S a bitmask of axes (w[0] unused)
+APHIP_DEV void hex8_modes(const double (&v)[8], double (&w)[8]) {
+#pragma clang fp contract(off)
+#pragma unroll
+    for (int k = 0; k < 8; ++k) w[hex8_bin(k)] = v[k];
+#pragma unroll
+    for (int h = 1; h < 8; h <<= 1)
+#pragma unroll
+        for (int i = 0; i < 8; ++i)
+            if (!(i & h)) {
+                const double a = w[i], b = w[i | h];
+                w[i] = b + a;
+                w[i | h] = b - a;
+            }
+}
+
+// v_k = sum_S prod_{d in S} s_d(k) r[S], with r[0] = 0
+APHIP_DEV void hex8_corners(const double (&r)[8], double (&v)[8]) {
+#pragma clang fp contract(off)
+    double w[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) w[i] = r[i];
+#pragma unroll
+    for (int h = 1; h < 8; h <<= 1)
+#pragma unroll
+        for (int i = 0; i < 8; ++i)
+            if (!(i & h)) {
+                const double a = w[i], b = w[i | h];
+                w[i] = a - b;
+                w[i | h] = a + b;
+            }
+#pragma unroll
+    for (int k = 0; k < 8; ++k) v[k] = w[hex8_bin(k)];
+}
+
+// 8 d(field_i)/dxi at Gauss point (p0, p1, p2); p01 = p0 p1 etc.
+APHIP_DEV void hex8_grad(const double (&w)[8], double p0, double p1, double p2, double p01, double p02, double p12,
+                         double (&g)[3]) {
+#pragma clang fp contract(off)
+    g[0] = __fma_rn(p12, w[7], __fma_rn(p2, w[5], __fma_rn(p1, w[3], w[1])));
+    g[1] = __fma_rn(p02, w[7], __fma_rn(p2, w[6], __fma_rn(p0, w[3], w[2])));
+    g[2] = __fma_rn(p01, w[7], __fma_rn(p1, w[6], __fma_rn(p0, w[5], w[4])));
+}
+
+// state_n / state_np1 point at the element's state slot(s) (per_qp: 8 slots).
 template <int KIND, bool PER_QP>
 APHIP_DEV void hex8_element(const double (&X)[8][3], const double (&U)[8][3], const MatParams &m, double ga,
                             const double *state_n, double *state_np1, double *H, double *P, double (&fe)[8][3]) {
 #pragma clang fp contract(off)
+    double XM[3][8], UM[3][8];
 #pragma unroll
-    for (int k = 0; k < 8; ++k) fe[k][0] = fe[k][1] = fe[k][2] = 0.0;
+    for (int i = 0; i < 3; ++i) {
+        double vx[8], vu[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            vx[k] = X[k][i];
+            vu[k] = U[k][i];
+        }
+        hex8_modes(vx, XM[i]);
+        hex8_modes(vu, UM[i]);
+    }
+    // force modes r[j][S] (S = 1..7)
+    double R[3][8];
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+#pragma unroll
+        for (int S = 0; S < 8; ++S) R[j][S] = 0.0;
+    const double ga2 = ga * ga;
     double sn = 0.0;
     if constexpr (KIND == APHIP_J2 && !PER_QP) sn = state_n[0];
-#pragma unroll
+#pragma unroll 1
     for (int q = 0; q < 8; ++q) {
         int sq0, sq1, sq2;
         hex8_sign(q, sq0, sq1, sq2);
         const double p0 = sq0 * ga, p1 = sq1 * ga, p2 = sq2 * ga;
-        double J[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, G[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-        double dN[8][3];
+        const double p01 = (sq0 * sq1) * ga2, p02 = (sq0 * sq2) * ga2, p12 = (sq1 * sq2) * ga2;
+        double J[9], G[9];  // 8 J, 8 grad u (row i = component, column = parametric axis)
 #pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            int s0, s1, s2;
-            hex8_sign(k, s0, s1, s2);
-            const double f0 = s0 < 0 ? (1.0 - p0) : (1.0 + p0);
-            const double f1 = s1 < 0 ? (1.0 - p1) : (1.0 + p1);
-            const double f2 = s2 < 0 ? (1.0 - p2) : (1.0 + p2);
-            dN[k][0] = (s0 * 0.125) * f1 * f2;
-            dN[k][1] = (s1 * 0.125) * f0 * f2;
-            dN[k][2] = (s2 * 0.125) * f0 * f1;
+        for (int i = 0; i < 3; ++i) {
+            double gx[3], gu[3];
+            hex8_grad(XM[i], p0, p1, p2, p01, p02, p12, gx);
+            hex8_grad(UM[i], p0, p1, p2, p01, p02, p12, gu);
 #pragma unroll
-            for (int i = 0; i < 3; ++i)
-#pragma unroll
-                for (int d = 0; d < 3; ++d) {
-                    J[3 * i + d] = __fma_rn(X[k][i], dN[k][d], J[3 * i + d]);
-                    G[3 * i + d] = __fma_rn(U[k][i], dN[k][d], G[3 * i + d]);
-                }
+            for (int d = 0; d < 3; ++d) {
+                J[3 * i + d] = gx[d];
+                G[3 * i + d] = gu[d];
+            }
         }
-        double Ji[9];
-        const double w = inverse3(J, Ji);
+        double Ji[9];  // (8 J)^-1 = J^-1 / 8
+        const double w = inverse3(J, Ji) * (-1.0 / 512.0);  // -detJ
 #pragma unroll
         for (int i = 0; i < 3; ++i)
 #pragma unroll
@@ -138,16 +208,27 @@ APHIP_DEV void hex8_element(const double (&X)[8][3], const double (&U)[8][3], co
             if constexpr (PER_QP) state_np1[q] = snp1;
             else if (q == 7) state_np1[0] = snp1;
         }
-        double Q[9];
 #pragma unroll
-        for (int n = 0; n < 3; ++n)
+        for (int j = 0; j < 3; ++j) {
+            // Q[n] = -(w / 8) (J^-1 P^T)[n][j]: the force of axis n's derivative
+            double Q[3];
 #pragma unroll
-            for (int j = 0; j < 3; ++j)
-                Q[3 * n + j] = w * dot3(Ji[3 * n], P[3 * j], Ji[3 * n + 1], P[3 * j + 1], Ji[3 * n + 2], P[3 * j + 2]);
+            for (int n = 0; n < 3; ++n) Q[n] = w * dot3(Ji[3 * n], P[3 * j], Ji[3 * n + 1], P[3 * j + 1], Ji[3 * n + 2], P[3 * j + 2]);
+            R[j][1] += Q[0];
+            R[j][2] += Q[1];
+            R[j][4] += Q[2];
+            R[j][3] = __fma_rn(p1, Q[0], __fma_rn(p0, Q[1], R[j][3]));
+            R[j][5] = __fma_rn(p2, Q[0], __fma_rn(p0, Q[2], R[j][5]));
+            R[j][6] = __fma_rn(p2, Q[1], __fma_rn(p1, Q[2], R[j][6]));
+            R[j][7] = __fma_rn(p12, Q[0], __fma_rn(p02, Q[1], __fma_rn(p01, Q[2], R[j][7])));
+        }
+    }
 #pragma unroll
-        for (int k = 0; k < 8; ++k)
+    for (int j = 0; j < 3; ++j) {
+        double v[8];
+        hex8_corners(R[j], v);
 #pragma unroll
-            for (int j = 0; j < 3; ++j) fe[k][j] -= dot3(dN[k][0], Q[j], dN[k][1], Q[3 + j], dN[k][2], Q[6 + j]);
+        for (int k = 0; k < 8; ++k) fe[k][j] = v[k];
     }
 }
 

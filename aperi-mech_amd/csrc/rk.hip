@@ -479,7 +479,7 @@ __global__ __launch_bounds__(256) void k_count_keys(int64_t n, const int32_t *ke
 constexpr int kCellLanes = 16;
 
 struct CellForceArgs {
-    int64_t ne;
+    int64_t ne, nblocks;
     const int64_t *start;
     const int32_t *nodes;
     const double *dphi;
@@ -493,6 +493,19 @@ struct CellForceArgs {
     const double *elem_lambda, *elem_mu;
 };
 
+constexpr int kCellCache = 6;  // entries per lane whose grad phi stays in registers
+constexpr int kNodeBatch = 4;  // incidences per lane whose loads are issued together
+constexpr int kNumXcd = 8;
+
+// Consecutive logical blocks go to the same XCD (the dispatcher places block b on
+// XCD b % 8), so neighbouring cells / nodes share that XCD's L2 for their gathers.
+// The grid is padded to a multiple of 8 so the map is a bijection.
+__device__ __forceinline__ int64_t xcd_block(int64_t nblocks_padded) {
+    const int64_t b = blockIdx.x;
+    const int64_t per = nblocks_padded / kNumXcd;
+    return (b % kNumXcd) * per + b / kNumXcd;
+}
+
 __device__ __forceinline__ double group_sum16(double x) {
 #pragma unroll
     for (int s = 8; s >= 1; s >>= 1) x += __shfl_xor(x, s, kCellLanes);
@@ -503,12 +516,39 @@ template <int KIND>
 __global__ __launch_bounds__(256) void k_rk_cells(CellForceArgs A) {
 #pragma clang fp contract(off)
     const int l = threadIdx.x & (kCellLanes - 1);
-    const int64_t c_raw = (int64_t)blockIdx.x * (256 / kCellLanes) + (threadIdx.x / kCellLanes);
+    const int64_t c_raw = xcd_block(A.nblocks) * (256 / kCellLanes) + (threadIdx.x / kCellLanes);
     const bool live = c_raw < A.ne;
     const int64_t c = live ? c_raw : A.ne - 1;  // idle groups recompute the last cell and store nothing
     const int64_t t0 = A.start[c], t1 = A.start[c + 1];
+    const double mv = -A.volume[c];
     double H[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, P[9];
-    for (int64_t t = t0 + l; t < t1; t += kCellLanes) {
+    // The first kCellCache entries of the lane: every load issued before the first
+    // use (clamped indices, absent entries weighted by zero), grad phi kept for the
+    // force pass.  Entries are accumulated in ascending order either way.
+    double gc[kCellCache][3] = {};
+    if (t1 > t0) {  // (an empty cell has no entry to clamp to)
+        int32_t nd[kCellCache];
+#pragma unroll
+        for (int r = 0; r < kCellCache; ++r) nd[r] = A.nodes[min(t0 + l + r * kCellLanes, t1 - 1)];
+        double uj[kCellCache][3];
+#pragma unroll
+        for (int r = 0; r < kCellCache; ++r) {
+            const int64_t t = min(t0 + l + r * kCellLanes, t1 - 1);
+            ld3(A.u, nd[r], uj[r]);
+            ld3(A.dphi, t, gc[r]);
+        }
+#pragma unroll
+        for (int r = 0; r < kCellCache; ++r) {
+            const bool in = t0 + l + r * kCellLanes < t1;
+#pragma unroll
+            for (int b = 0; b < 3; ++b) gc[r][b] = in ? gc[r][b] : 0.0;
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+#pragma unroll
+                for (int b = 0; b < 3; ++b) H[3 * a + b] = __fma_rn(uj[r][a], gc[r][b], H[3 * a + b]);
+        }
+    }
+    for (int64_t t = t0 + l + kCellCache * kCellLanes; t < t1; t += kCellLanes) {
         double uj[3], g[3];
         ld3(A.u, A.nodes[t], uj);
         ld3(A.dphi, t, g);
@@ -535,9 +575,19 @@ __global__ __launch_bounds__(256) void k_rk_cells(CellForceArgs A) {
             store_once(A.P + i * A.ne + c, P[i]);
         }
     }
-    const double mv = -A.volume[c];
     if (live) {
-        for (int64_t t = t0 + l; t < t1; t += kCellLanes) {
+#pragma unroll
+        for (int r = 0; r < kCellCache; ++r) {
+            const int64_t t = t0 + l + r * kCellLanes;
+            if (t < t1) {
+                double f[3];
+#pragma unroll
+                for (int a = 0; a < 3; ++a)
+                    f[a] = mv * dot3(P[3 * a], gc[r][0], P[3 * a + 1], gc[r][1], P[3 * a + 2], gc[r][2]);
+                st3(A.g, t, f);
+            }
+        }
+        for (int64_t t = t0 + l + kCellCache * kCellLanes; t < t1; t += kCellLanes) {
             double g[3], f[3];
             ld3(A.dphi, t, g);
 #pragma unroll
@@ -548,7 +598,7 @@ __global__ __launch_bounds__(256) void k_rk_cells(CellForceArgs A) {
 }
 
 struct RkNodeArgs {
-    int64_t nn;
+    int64_t nn, nblocks;
     const int64_t *inc_start;
     const int32_t *inc_entry;
     const double *g;
@@ -561,17 +611,26 @@ struct RkNodeArgs {
 template <int MODE>
 __global__ __launch_bounds__(256) void k_rk_nodes(RkNodeArgs A) {
     const int l = threadIdx.x & (kCellLanes - 1);
-    const int64_t i_raw = (int64_t)blockIdx.x * (256 / kCellLanes) + (threadIdx.x / kCellLanes);
+    const int64_t i_raw = xcd_block(A.nblocks) * (256 / kCellLanes) + (threadIdx.x / kCellLanes);
     const bool live = i_raw < A.nn;
     const int64_t i = live ? i_raw : A.nn - 1;
     double f[3] = {0.0, 0.0, 0.0};
-    const int64_t t1 = A.inc_start[i + 1];
-    for (int64_t t = A.inc_start[i] + l; t < t1; t += kCellLanes) {
-        double g[3];
-        ld3(A.g, A.inc_entry[t], g);
-        f[0] += g[0];
-        f[1] += g[1];
-        f[2] += g[2];
+    const int64_t t0 = A.inc_start[i], t1 = A.inc_start[i + 1];
+    // kNodeBatch incidences at a time: indices, then the gathers, all issued before
+    // the first sum (clamped indices, absent incidences add zero); ascending order.
+    for (int64_t tb = t0 + l; tb < t1; tb += kNodeBatch * kCellLanes) {
+        int32_t en[kNodeBatch];
+#pragma unroll
+        for (int r = 0; r < kNodeBatch; ++r) en[r] = A.inc_entry[min(tb + r * kCellLanes, t1 - 1)];
+        double g[kNodeBatch][3];
+#pragma unroll
+        for (int r = 0; r < kNodeBatch; ++r) ld3(A.g, en[r], g[r]);
+#pragma unroll
+        for (int r = 0; r < kNodeBatch; ++r) {
+            const bool in = tb + r * kCellLanes < t1;
+#pragma unroll
+            for (int d = 0; d < 3; ++d) f[d] += in ? g[r][d] : 0.0;
+        }
     }
 #pragma unroll
     for (int d = 0; d < 3; ++d) f[d] = group_sum16(f[d]);
@@ -625,6 +684,12 @@ __global__ __launch_bounds__(256) void k_rk_mass(int64_t nn, const int32_t *num,
 }
 
 inline dim3 grid1(int64_t n, int block) { return dim3((unsigned)((n + block - 1) / block)); }
+
+// blocks of 256 / kCellLanes groups for n groups, padded to a multiple of the XCD count
+inline int64_t padded_groups(int64_t n) {
+    const int64_t per = 256 / kCellLanes, nb = (n + per - 1) / per;
+    return ((nb + kNumXcd - 1) / kNumXcd) * kNumXcd;
+}
 
 // counts -> exclusive prefix (host) -> device start array of n+1 entries
 template <typename T>
@@ -773,11 +838,12 @@ void rk_setup(Ctx &c, double scale) {
 void rk_cell_force(Ctx &c) {
     CellData &D = c.cells;
     if (c.ne == 0) return;
-    CellForceArgs A{c.ne, D.start.p, D.nodes.p, D.dphi.p, D.volume.p, c.u.p, c.H.p, c.P.p, D.g.p,
+    const int64_t nbc = padded_groups(c.ne);
+    CellForceArgs A{c.ne, nbc, D.start.p, D.nodes.p, D.dphi.p, D.volume.p, c.u.p, c.H.p, c.P.p, D.g.p,
                     c.state[1 - c.state_np1].p, c.state[c.state_np1].p,
                     MatParams{c.mat.lambda, c.mat.mu, c.mat.yield_stress, c.mat.hardening},
                     c.has_fields ? c.elem_lambda.p : nullptr, c.has_fields ? c.elem_mu.p : nullptr};
-    const dim3 g = grid1(c.ne, 256 / kCellLanes);
+    const dim3 g((unsigned)nbc);
     switch (c.mat.kind) {
         case APHIP_LINEAR_ELASTIC: hipLaunchKernelGGL(k_rk_cells<APHIP_LINEAR_ELASTIC>, g, dim3(256), 0, c.stream, A); break;
         case APHIP_NEO_HOOKEAN: hipLaunchKernelGGL(k_rk_cells<APHIP_NEO_HOOKEAN>, g, dim3(256), 0, c.stream, A); break;
@@ -790,9 +856,10 @@ void rk_cell_force(Ctx &c) {
 
 void rk_node_update(Ctx &c, double dt, int mode) {
     CellData &D = c.cells;
-    RkNodeArgs A{c.nn, D.node_start.p, D.inc_entry.p, D.g.p, c.mass.p,
+    const int64_t nbn = padded_groups(c.nn);
+    RkNodeArgs A{c.nn, nbn, D.node_start.p, D.inc_entry.p, D.g.p, c.mass.p,
                  c.has_fixed ? c.fixed.p : nullptr, c.f.p, c.a.p, c.v.p, dt};
-    const dim3 g = grid1(c.nn, 256 / kCellLanes);
+    const dim3 g((unsigned)nbn);
     if (mode == 0) hipLaunchKernelGGL(k_rk_nodes<0>, g, dim3(256), 0, c.stream, A);
     else if (mode == 1) hipLaunchKernelGGL(k_rk_nodes<1>, g, dim3(256), 0, c.stream, A);
     else hipLaunchKernelGGL(k_rk_nodes<2>, g, dim3(256), 0, c.stream, A);

@@ -127,6 +127,10 @@ __device__ __forceinline__ void load_elem_lame(const TileArgs &A, int64_t e, Mat
     m.mu = A.elem_lambda ? mu : m.mu;
 }
 
+// s_waitcnt vmcnt(0) (expcnt and lgkmcnt at their maxima: not waited on).  At a
+// point where no store is outstanding this costs nothing beyond the loads' latency.
+__device__ __forceinline__ void wait_vmem_loads() { __builtin_amdgcn_s_waitcnt(0x0f70); }
+
 // MODE 0: force only; 1: force + a = f/m (initial); 2: full central-difference step.
 // QPAR (Hex8 only): eight lanes per element, one per Gauss point.
 template <int NPE, int KIND, bool PER_QP, int MODE, bool QPAR>
@@ -207,6 +211,12 @@ __global__ __launch_bounds__(NPE == 4 ? 512 : (QPAR ? 512 : 128)) void k_tile(Ti
             st3(L.U, ln, u);
         }
     }
+    // Every load of the wave completes here, on every path (a wave whose lanes all
+    // skip the staging loop still has its element prefetch in flight).  Otherwise the
+    // join after the element phase merges a path with pending loads into the path
+    // with the H / P stores, and the first register reuse in the gather waits for
+    // vmcnt(0): for every store of the wave.
+    wait_vmem_loads();
     __syncthreads();
 
     // B: one element per thread, or one Gauss point per lane
