@@ -112,14 +112,14 @@ struct CellData {
     DevBuf<int32_t> inc_cell;   // [nnz] cell owning each entry
     DevBuf<int64_t> node_start; // [nn + 1] node -> entries naming it
     DevBuf<int32_t> node_inc;   // unused (kept for layout stability)
-    DevBuf<int32_t> inc_entry;  // [nnz] entries sorted by node, ascending entry
-    DevBuf<double> g;           // [nnz * 3] per-entry force contributions of the last evaluation
+    DevBuf<int32_t> entry_pos;  // [nnz] position of each entry in node order (by node, then ascending entry)
+    DevBuf<double> g;           // [nnz * 3] per-entry force contributions of the last evaluation, node order
     void reset() {
         ready = false;
         nnz = n_truncated = 0;
         num_nbr.release(); nbr.release(); phi.release(); radius.release(); start.release();
         nodes.release(); dphi.release(); volume.release(); inc_cell.release(); node_start.release();
-        node_inc.release(); inc_entry.release(); g.release();
+        node_inc.release(); entry_pos.release(); g.release();
     }
 };
 

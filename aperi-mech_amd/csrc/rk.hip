@@ -460,6 +460,12 @@ __global__ __launch_bounds__(256) void k_iota(int64_t n, int32_t *v) {
     if (i < n) v[i] = (int32_t)i;
 }
 
+// pos[perm[k]] = k
+__global__ __launch_bounds__(256) void k_invert_perm(int64_t n, const int32_t *perm, int32_t *pos) {
+    const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (k < n) pos[perm[k]] = (int32_t)k;
+}
+
 __global__ __launch_bounds__(256) void k_count_keys(int64_t n, const int32_t *keys, int32_t *count) {
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (i < n) atomicAdd(&count[keys[i]], 1);
@@ -473,9 +479,10 @@ __global__ __launch_bounds__(256) void k_count_keys(int64_t n, const int32_t *ke
 // with the same bits and evaluates the stress update itself (no broadcast).  Each
 // entry then gets its force contribution g_t = -V P grad phi_t (3 doubles), the
 // deterministic replacement of the reference's per-entry atomic scatter
-// (include/ComputeInternalForceSmoothedCell.h:312-351).
-// Nodes: a group of 16 lanes per node sums g over the node's incidences (ascending
-// entry, strided over the lanes, then the same butterfly) and finishes the node.
+// (include/ComputeInternalForceSmoothedCell.h:312-351), stored at the entry's
+// position in node order (all entries naming node 0, ascending entry, then node 1 ...).
+// Nodes: a group of 16 lanes per node sums its contiguous run of g (ascending entry,
+// strided over the lanes, then the same butterfly) and finishes the node.
 constexpr int kCellLanes = 16;
 
 struct CellForceArgs {
@@ -486,7 +493,8 @@ struct CellForceArgs {
     const double *volume;
     const double *u;
     double *H, *P;
-    double *g;  // [nnz * 3] per-entry force contributions
+    double *g;  // [nnz * 3] per-entry force contributions, node order
+    const int32_t *pos;  // [nnz] node-order position of each entry
     const double *state_n;
     double *state_np1;
     MatParams mat;
@@ -526,10 +534,15 @@ __global__ __launch_bounds__(256) void k_rk_cells(CellForceArgs A) {
     // use (clamped indices, absent entries weighted by zero), grad phi kept for the
     // force pass.  Entries are accumulated in ascending order either way.
     double gc[kCellCache][3] = {};
+    int32_t ps[kCellCache] = {};
     if (t1 > t0) {  // (an empty cell has no entry to clamp to)
         int32_t nd[kCellCache];
 #pragma unroll
-        for (int r = 0; r < kCellCache; ++r) nd[r] = A.nodes[min(t0 + l + r * kCellLanes, t1 - 1)];
+        for (int r = 0; r < kCellCache; ++r) {
+            const int64_t t = min(t0 + l + r * kCellLanes, t1 - 1);
+            nd[r] = A.nodes[t];
+            ps[r] = A.pos[t];
+        }
         double uj[kCellCache][3];
 #pragma unroll
         for (int r = 0; r < kCellCache; ++r) {
@@ -584,7 +597,7 @@ __global__ __launch_bounds__(256) void k_rk_cells(CellForceArgs A) {
 #pragma unroll
                 for (int a = 0; a < 3; ++a)
                     f[a] = mv * dot3(P[3 * a], gc[r][0], P[3 * a + 1], gc[r][1], P[3 * a + 2], gc[r][2]);
-                st3(A.g, t, f);
+                st3(A.g, ps[r], f);
             }
         }
         for (int64_t t = t0 + l + kCellCache * kCellLanes; t < t1; t += kCellLanes) {
@@ -592,7 +605,7 @@ __global__ __launch_bounds__(256) void k_rk_cells(CellForceArgs A) {
             ld3(A.dphi, t, g);
 #pragma unroll
             for (int a = 0; a < 3; ++a) f[a] = mv * dot3(P[3 * a], g[0], P[3 * a + 1], g[1], P[3 * a + 2], g[2]);
-            st3(A.g, t, f);
+            st3(A.g, A.pos[t], f);
         }
     }
 }
@@ -600,8 +613,7 @@ __global__ __launch_bounds__(256) void k_rk_cells(CellForceArgs A) {
 struct RkNodeArgs {
     int64_t nn, nblocks;
     const int64_t *inc_start;
-    const int32_t *inc_entry;
-    const double *g;
+    const double *g;  // node order
     const double *mass;
     const uint8_t *fixed;
     double *f, *a, *v;
@@ -616,15 +628,12 @@ __global__ __launch_bounds__(256) void k_rk_nodes(RkNodeArgs A) {
     const int64_t i = live ? i_raw : A.nn - 1;
     double f[3] = {0.0, 0.0, 0.0};
     const int64_t t0 = A.inc_start[i], t1 = A.inc_start[i + 1];
-    // kNodeBatch incidences at a time: indices, then the gathers, all issued before
-    // the first sum (clamped indices, absent incidences add zero); ascending order.
+    // kNodeBatch incidences at a time, all loads issued before the first sum (clamped
+    // indices, absent incidences add zero); ascending order.
     for (int64_t tb = t0 + l; tb < t1; tb += kNodeBatch * kCellLanes) {
-        int32_t en[kNodeBatch];
-#pragma unroll
-        for (int r = 0; r < kNodeBatch; ++r) en[r] = A.inc_entry[min(tb + r * kCellLanes, t1 - 1)];
         double g[kNodeBatch][3];
 #pragma unroll
-        for (int r = 0; r < kNodeBatch; ++r) ld3(A.g, en[r], g[r]);
+        for (int r = 0; r < kNodeBatch; ++r) ld3(A.g, min(tb + r * kCellLanes, t1 - 1), g[r]);
 #pragma unroll
         for (int r = 0; r < kNodeBatch; ++r) {
             const bool in = tb + r * kCellLanes < t1;
@@ -826,9 +835,9 @@ void rk_setup(Ctx &c, double scale) {
         ccount.zero(c.stream);
         hipLaunchKernelGGL(k_count_keys, grid1(D.nnz, 256), dim3(256), 0, c.stream, D.nnz, D.nodes.p, ccount.p);
         scan_counts(c, ccount, nn, D.node_start, nullptr);
-        D.inc_entry.alloc(std::max<int64_t>(D.nnz, 1));
-        APHIP_HIP(hipMemcpyAsync(D.inc_entry.p, inc.p, sizeof(int32_t) * std::max<int64_t>(D.nnz, 1),
-                                 hipMemcpyDeviceToDevice, c.stream));
+        D.entry_pos.alloc(std::max<int64_t>(D.nnz, 1));
+        hipLaunchKernelGGL(k_invert_perm, grid1(D.nnz, 256), dim3(256), 0, c.stream, D.nnz, inc.p, D.entry_pos.p);
+        APHIP_HIP(hipGetLastError());
     }
     D.g.alloc(3 * std::max<int64_t>(D.nnz, 1));
     APHIP_HIP(hipStreamSynchronize(c.stream));
@@ -839,7 +848,7 @@ void rk_cell_force(Ctx &c) {
     CellData &D = c.cells;
     if (c.ne == 0) return;
     const int64_t nbc = padded_groups(c.ne);
-    CellForceArgs A{c.ne, nbc, D.start.p, D.nodes.p, D.dphi.p, D.volume.p, c.u.p, c.H.p, c.P.p, D.g.p,
+    CellForceArgs A{c.ne, nbc, D.start.p, D.nodes.p, D.dphi.p, D.volume.p, c.u.p, c.H.p, c.P.p, D.g.p, D.entry_pos.p,
                     c.state[1 - c.state_np1].p, c.state[c.state_np1].p,
                     MatParams{c.mat.lambda, c.mat.mu, c.mat.yield_stress, c.mat.hardening},
                     c.has_fields ? c.elem_lambda.p : nullptr, c.has_fields ? c.elem_mu.p : nullptr};
@@ -857,7 +866,7 @@ void rk_cell_force(Ctx &c) {
 void rk_node_update(Ctx &c, double dt, int mode) {
     CellData &D = c.cells;
     const int64_t nbn = padded_groups(c.nn);
-    RkNodeArgs A{c.nn, nbn, D.node_start.p, D.inc_entry.p, D.g.p, c.mass.p,
+    RkNodeArgs A{c.nn, nbn, D.node_start.p, D.g.p, c.mass.p,
                  c.has_fixed ? c.fixed.p : nullptr, c.f.p, c.a.p, c.v.p, dt};
     const dim3 g((unsigned)nbn);
     if (mode == 0) hipLaunchKernelGGL(k_rk_nodes<0>, g, dim3(256), 0, c.stream, A);

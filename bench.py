@@ -314,6 +314,35 @@ def cpu_baseline(w, kind, budget_s=12.0):
                        f"({el:.1f}s, OpenMP atomic scatter, oracle/liboracle.so)")
 
 
+def bench_power(n=100):
+    """Power-method stable time increment (the reference's PowerMethodProcessor algorithm)
+    on the large Tet4 block at the reference's default settings: each iteration is one
+    perturbed internal-force evaluation plus a norm; reports the wall time, the iteration
+    count, the increment and its ratio to the element-size estimate used by the step."""
+    import aperi_hip as A
+    from aperi_hip import meshgen as M
+    w = workload(n)
+    X, conn = w["X"], w["conn"]
+    s = A.Solver(device=0)
+    s.set_mesh(A.TET4, X, conn)
+    s.set_material(A.NEO_HOOKEAN, w["rho"], w["E"], w["nu"])
+    s.compute_mass()
+    s.set_field(A.FIELD_V, w["v0"])
+    s.initialize()
+    est = M.stable_dt_estimate(X, conn, w["E"], w["nu"], w["rho"])
+    s.step(0.5 * est, 2)  # a deformed state: the increment is evaluated about the current displacement
+    s.synchronize()
+    s.power_method_time_increment(num_iterations=3)  # warm-up (allocations)
+    t0 = time.perf_counter()
+    dt, it, cv = s.power_method_time_increment()
+    s.synchronize()
+    el = time.perf_counter() - t0
+    s.close()
+    return {"wall_s": el, "iterations": it, "converged": cv, "ms_per_iteration": 1e3 * el / max(it, 1),
+            "time_increment": dt, "over_element_estimate": dt / est,
+            "workload": f"{len(conn)} Tet4, {len(X)} nodes, neo-Hookean, 50 iterations max, tolerance 1e-3"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -322,7 +351,7 @@ def main():
     ap.add_argument("--n", type=int, default=100, help="hexes per edge (x6 tets)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--material", default="nh", choices=["nh", "le"])
-    ap.add_argument("--no-extras", action="store_true", help="skip the ensemble and adjoint measurements")
+    ap.add_argument("--no-extras", action="store_true", help="skip the ensemble, RKPM, power-method and adjoint measurements")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only time a device copy of BYTES (HBM yardstick) and print it")
     args = ap.parse_args()
@@ -387,6 +416,10 @@ def main():
         line["rkpm"] = bench_rkpm()
         log(f"[bench] rkpm: {line['rkpm']['cell_updates_per_s']:.3e} cell-updates/s, "
             f"setup {line['rkpm']['setup_s']:.2f}s, roofline {line['rkpm']['roofline']['frac']:.3f}")
+        log("[bench] power method ...")
+        line["power_method"] = bench_power(args.n)
+        log(f"[bench] power method: {line['power_method']['iterations']} iterations, "
+            f"{line['power_method']['ms_per_iteration']:.2f} ms each")
         log("[bench] adjoint ...")
         line["adjoint"] = bench_adjoint()
         log(f"[bench] adjoint: {line['adjoint']['gradient_over_forward']:.2f}x forward, "
