@@ -8,7 +8,10 @@
 //   ElementBase                                                include/ElementBase.h:27-117
 //   CreateElement (supported combinations, error messages)     include/Element.h:32-86
 //   InternalForceContribution::ComputeForce                    include/InternalForceContribution.h:26-146
-//   ExplicitSolver loop body with a fixed increment            src/Solvers/Explicit.cpp:308-498
+//   TimeIncrementScheduler, StepScheduler                      include/Scheduler.h:10-121
+//   TimeStepper, DirectTimeStepper, PowerMethodTimeStepper     include/TimeStepper.h:24-229
+//   ExplicitSolver::Solve (time loop, output schedule)         src/Solvers/Explicit.cpp:308-498
+//   PowerMethodProcessor::ComputeStableTimeIncrement           include/PowerMethodProcessor.h:331-470
 // Errors are reported the way the reference reports them: std::runtime_error.
 // Header-only; link against libaperi_hip.so.
 #pragma once
@@ -17,6 +20,7 @@
 
 #include <cmath>
 #include <cstdint>
+#include <functional>
 #include <limits>
 #include <map>
 #include <memory>
@@ -367,6 +371,96 @@ class InternalForceContribution {
 // The explicit loop of src/Solvers/Explicit.cpp:308-498 with a fixed increment
 // (DirectTimeStepper, include/TimeStepper.h:97-128), run on the device: the
 // initial force and acceleration, then central-difference steps.
+// ---- schedulers and time steppers --------------------------------------------
+
+// An event every time_increment from time_start; several events that one call has
+// passed count once (include/Scheduler.h:37-63).
+class TimeIncrementScheduler {
+   public:
+    TimeIncrementScheduler(double time_start, double time_end, double time_increment)
+        : m_time_end(time_end), m_next(time_start), m_increment(time_increment) {}
+    bool AtNextEvent(double time) {
+        const double tolerance = 1.0e-15;
+        const bool at_event = time + tolerance >= m_next;
+        if (at_event) {
+            m_next += m_increment;
+            AtNextEvent(time);
+            if (m_next > m_time_end) m_next = std::numeric_limits<double>::max();
+        }
+        return at_event;
+    }
+
+   private:
+    double m_time_end, m_next, m_increment;
+};
+
+// An event at steps start, start + increment, ... (include/Scheduler.h:100-121).
+class StepScheduler {
+   public:
+    StepScheduler(size_t start, size_t increment) : m_next(start), m_increment(increment) {}
+    bool AtNextEvent(size_t step) {
+        const bool at_event = step == m_next;
+        if (at_event) m_next += m_increment;
+        return at_event;
+    }
+
+   private:
+    size_t m_next, m_increment;
+};
+
+class ExplicitSolver;
+
+// include/TimeStepper.h:24-90
+class TimeStepper {
+   public:
+    explicit TimeStepper(double time_end) : m_time_end(time_end) {}
+    virtual ~TimeStepper() = default;
+    virtual void Initialize(ExplicitSolver & /*solver*/) {}
+    double GetTimeEnd() const { return m_time_end; }
+    bool AtEnd(double time) const { return time >= GetTimeEnd(); }
+    virtual double GetTimeIncrement(double time, size_t increment) = 0;
+
+   protected:
+    // the step that ends at time_end exactly (TimeStepper.h:113-121, :207-214)
+    double Clamp(double time, double time_increment) const {
+        if (time + time_increment <= GetTimeEnd()) return time_increment;
+        if (time > GetTimeEnd()) return 0.0;
+        return GetTimeEnd() - time;
+    }
+
+   private:
+    double m_time_end;
+};
+
+// A fixed increment (include/TimeStepper.h:97-128).
+class DirectTimeStepper : public TimeStepper {
+   public:
+    DirectTimeStepper(double time_increment, double time_end) : TimeStepper(time_end), m_time_increment(time_increment) {}
+    double GetTimeIncrement(double time, size_t /*increment*/) override { return Clamp(time, m_time_increment); }
+
+   private:
+    double m_time_increment;
+};
+
+// scale_factor x the power-method increment, recomputed every time_update_interval
+// steps from step 0 (include/TimeStepper.h:138-216).
+class PowerMethodTimeStepper : public TimeStepper {
+   public:
+    PowerMethodTimeStepper(double scale_factor, size_t time_update_interval, double time_end)
+        : TimeStepper(time_end), m_scale_factor(scale_factor), m_scheduler(0, time_update_interval) {}
+    void Initialize(ExplicitSolver &solver) override { m_solver = &solver; }
+    double GetTimeIncrement(double time, size_t increment) override;
+    // (increment, full increment) of every update so far
+    const std::vector<std::pair<size_t, double>> &Updates() const { return m_updates; }
+
+   private:
+    double m_scale_factor;
+    double m_time_increment = 0.0;
+    StepScheduler m_scheduler;
+    ExplicitSolver *m_solver = nullptr;
+    std::vector<std::pair<size_t, double>> m_updates;
+};
+
 class ExplicitSolver {
    public:
     explicit ExplicitSolver(std::shared_ptr<InternalForceContribution> contribution)
@@ -378,29 +472,68 @@ class ExplicitSolver {
         m_initialized = true;
         return mass;
     }
-    // n steps of the loop body (Explicit.cpp:383-452); returns the time reached.
+    // n steps of the loop body (Explicit.cpp:383-452) with a fixed increment; returns the
+    // time reached.
     double Solve(double time_increment, int n_steps) {
         if (!m_initialized) Initialize();
         Check(aphip_step(Context(), time_increment, n_steps));
         m_time += time_increment * n_steps;
         return m_time;
     }
+    // The time loop of ExplicitSolver::Solve (src/Solvers/Explicit.cpp:308-452): the first
+    // increment is requested before the initial force; then, until the stepper's end time,
+    // the increment for step n, one step, time += increment; write_output(time, n) runs at
+    // t = 0 and after every step at which the output scheduler fires.  Returns the final time.
+    double Solve(TimeStepper &time_stepper, TimeIncrementScheduler &output_scheduler,
+                 const std::function<void(double, size_t)> &write_output) {
+        double time = 0.0;
+        size_t n = 0;
+        if (!m_initialized) m_contribution->GetElement()->ComputeMass();  // BuildMassMatrix (:313)
+        time_stepper.Initialize(*this);
+        double time_increment = time_stepper.GetTimeIncrement(time, n);
+        if (!m_initialized) {
+            Check(aphip_initialize(Context()));
+            m_initialized = true;
+        }
+        if (output_scheduler.AtNextEvent(time)) write_output(time, n);
+        while (!time_stepper.AtEnd(time)) {
+            time_increment = time_stepper.GetTimeIncrement(time, n);
+            Check(aphip_step(Context(), time_increment, 1));
+            time += time_increment;
+            ++n;
+            if (output_scheduler.AtNextEvent(time)) write_output(time, n);
+        }
+        m_time = time;
+        return time;
+    }
     double GetTime() const { return m_time; }
     // PowerMethodProcessor::ComputeStableTimeIncrement (include/PowerMethodProcessor.h:331-470)
-    // with the reference's defaults; the simulation state is left as it was.
-    double ComputeStableTimeIncrement(int num_iterations = 50, double epsilon = 1e-5, double tolerance = 1e-3,
-                                      double variation_tolerance = 1e-7, uint64_t seed = 20261202) {
+    // with the reference's defaults about the current displacement; time_increment is the
+    // last full increment (0 on the first call); the simulation state is left as it was.
+    double ComputeStableTimeIncrement(double time_increment = 0.0, int num_iterations = 50, double epsilon = 1e-5,
+                                      double tolerance = 1e-3, double variation_tolerance = 1e-7,
+                                      uint64_t seed = 20261202) {
         double dt = 0.0;
-        Check(aphip_power_method_time_increment(Context(), num_iterations, epsilon, tolerance, variation_tolerance,
-                                                seed, &dt, nullptr, nullptr));
+        Check(aphip_power_method_time_increment(Context(), time_increment, num_iterations, epsilon, tolerance,
+                                                variation_tolerance, seed, &dt, nullptr, nullptr));
         return dt;
     }
+    aphip_ctx *Context() const { return m_contribution->GetElement()->Context(); }
 
    private:
-    aphip_ctx *Context() const { return m_contribution->GetElement()->Context(); }
     std::shared_ptr<InternalForceContribution> m_contribution;
     bool m_initialized = false;
     double m_time = 0.0;
 };
+
+inline double PowerMethodTimeStepper::GetTimeIncrement(double time, size_t increment) {
+    if (m_solver == nullptr) throw std::runtime_error("PowerMethodTimeStepper used before Initialize");
+    if (m_scheduler.AtNextEvent(increment)) {
+        const double full = m_solver->ComputeStableTimeIncrement(m_time_increment / m_scale_factor);
+        m_time_increment = full * m_scale_factor;
+        m_updates.emplace_back(increment, full);
+    }
+    return Clamp(time, m_time_increment);
+}
 
 }  // namespace aperi_hip

@@ -196,6 +196,7 @@ void renumber_nodes(Ctx &c, const std::vector<int32_t> &old_of_new) {
     permute_device(c, c.f, 3, old_of_new);
     if (c.has_mass) permute_device(c, c.mass, 1, old_of_new);
     if (c.has_fixed) permute_device(c, c.fixed, 1, old_of_new);
+    if (c.pm_eig_valid) permute_device(c, c.pm_eig, 3, old_of_new);
 }
 
 static bool use_tiles(const Ctx &c) { return !c.cells.ready && c.tiles.ready && !(c.flags & APHIP_FLAG_TWO_PASS); }
@@ -221,6 +222,7 @@ static void setup_mesh(Ctx &c, int32_t elem_type, int64_t n_nodes, const double 
     for (int64_t i = 0; i < n_elem * npe; ++i)
         APHIP_REQUIRE(conn[i] >= 0 && conn[i] < n_nodes, "connectivity references a node out of range");
     c.cells.reset();
+    c.pm_eig_valid = false;
     c.tiles.reset();
     c.graph_reset();
     c.etype = elem_type;
@@ -343,9 +345,13 @@ struct PowerMethodResult {
 // to `tolerance`, or when the variance of the last five is below
 // `variation_tolerance` x their mean; otherwise the smallest of the last five.
 // The reference's oscillation test (:424-437) takes its else branch at i = 0 and so
-// never reports oscillation; that outcome is kept.  u, f, H, P and the NP1 state are
-// restored afterwards, so the call leaves the simulation state untouched.
-static PowerMethodResult power_method(Ctx &c, int n_iter, double eps, double tol, double vtol, uint64_t seed) {
+// never reports oscillation; that outcome is kept.  The first comparison is against
+// prev_dt (1e-16 when 0, :363-364).  Like the reference's processor, the context keeps
+// the final eigenvector and starts the next call from it (:350-361, :468-469); only the
+// first call after the mesh is set draws the random start.  u, f, H, P and the NP1
+// state are restored afterwards, so the call leaves the simulation state untouched.
+static PowerMethodResult power_method(Ctx &c, double prev_dt, int n_iter, double eps, double tol, double vtol,
+                                      uint64_t seed) {
     const int64_t n3 = 3 * c.nn;
     DevBuf<double> u_in, f_in, f0, v, l, H, P, st, partial;
     DevBuf<int32_t> caller;
@@ -371,13 +377,16 @@ static PowerMethodResult power_method(Ctx &c, int n_iter, double eps, double tol
     }
     c.graph_reset();
     pm_max_edge(c, l.p);
-    pm_start(c, caller.p, seed, eps, l.p, v.p);
+    if (c.pm_eig_valid)
+        APHIP_HIP(hipMemcpyAsync(v.p, c.pm_eig.p, sizeof(double) * n3, hipMemcpyDeviceToDevice, c.stream));
+    else
+        pm_start(c, caller.p, seed, eps, l.p, v.p);
     internal_force_eval(c);  // F(u)
     APHIP_HIP(hipMemcpyAsync(f0.p, c.f.p, sizeof(double) * n3, hipMemcpyDeviceToDevice, c.stream));
 
     const int window = 5;
     std::vector<double> recent(window, 0.0);
-    double dt_np1 = 1.0e-16, dt_n = 0.0;
+    double dt_np1 = prev_dt == 0.0 ? 1.0e-16 : prev_dt, dt_n = 0.0;
     bool converged = false, oscillating = false;
     int used = n_iter;
     for (int k = 0; k < n_iter; ++k) {
@@ -422,6 +431,9 @@ static PowerMethodResult power_method(Ctx &c, int n_iter, double eps, double tol
     }
     const double wmin = *std::min_element(recent.begin(), recent.end());
     const double dt = converged ? (oscillating ? wmin : dt_np1) : wmin;
+    c.pm_eig.alloc(n3);
+    APHIP_HIP(hipMemcpyAsync(c.pm_eig.p, v.p, sizeof(double) * n3, hipMemcpyDeviceToDevice, c.stream));
+    c.pm_eig_valid = true;
 
     APHIP_HIP(hipMemcpyAsync(c.u.p, u_in.p, sizeof(double) * n3, hipMemcpyDeviceToDevice, c.stream));
     APHIP_HIP(hipMemcpyAsync(c.f.p, f_in.p, sizeof(double) * n3, hipMemcpyDeviceToDevice, c.stream));
@@ -740,16 +752,18 @@ int aphip_internal_force(aphip_ctx *c, double time, double dt) {
     APHIP_API_END
 }
 
-int aphip_power_method_time_increment(aphip_ctx *c, int32_t num_iterations, double epsilon, double tolerance,
-                                      double variation_tolerance, uint64_t seed, double *time_increment,
-                                      int32_t *iterations, int32_t *converged) {
+int aphip_power_method_time_increment(aphip_ctx *c, double previous_time_increment, int32_t num_iterations,
+                                      double epsilon, double tolerance, double variation_tolerance, uint64_t seed,
+                                      double *time_increment, int32_t *iterations, int32_t *converged) {
     APHIP_API_BEGIN
     APHIP_REQUIRE(c != nullptr && time_increment != nullptr, "NULL argument");
     require_ready(*c, true);
     APHIP_REQUIRE(num_iterations > 0 && epsilon > 0.0 && tolerance > 0.0 && variation_tolerance >= 0.0,
                   "power method needs positive iterations, epsilon and tolerance");
+    APHIP_REQUIRE(previous_time_increment >= 0.0, "previous time increment must be >= 0");
     APHIP_HIP(hipSetDevice(c->device));
-    const PowerMethodResult r = power_method(*c, num_iterations, epsilon, tolerance, variation_tolerance, seed);
+    const PowerMethodResult r =
+        power_method(*c, previous_time_increment, num_iterations, epsilon, tolerance, variation_tolerance, seed);
     *time_increment = r.dt;
     if (iterations) *iterations = r.iterations;
     if (converged) *converged = r.converged ? 1 : 0;

@@ -234,6 +234,10 @@ struct Ctx {
     bool has_mass = false;
     bool has_fixed = false;
 
+    // power method: the eigenvector of the last call, the next call's start vector
+    DevBuf<double> pm_eig;               // 3 nn, internal numbering
+    bool pm_eig_valid = false;
+
     CellData cells;
     TilePlan tiles;
     AdjointData adj;

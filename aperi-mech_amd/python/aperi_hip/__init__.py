@@ -199,10 +199,12 @@ class Solver:
         return out
 
     def power_method_time_increment(self, num_iterations=50, epsilon=1e-5, tolerance=1e-3,
-                                    variation_tolerance=1e-7, seed=20261202):
-        """(dt, iterations, converged) of the power-method stable increment."""
+                                    variation_tolerance=1e-7, seed=20261202, previous_time_increment=0.0):
+        """(dt, iterations, converged) of the power-method stable increment; the start
+        vector is the eigenvector of the previous call (random on the first call)."""
         dt, it, cv = ctypes.c_double(), ctypes.c_int32(), ctypes.c_int32()
-        _check(self.L.aphip_power_method_time_increment(self.ctx, int(num_iterations), ctypes.c_double(epsilon),
+        _check(self.L.aphip_power_method_time_increment(self.ctx, ctypes.c_double(previous_time_increment),
+                                                        int(num_iterations), ctypes.c_double(epsilon),
                                                         ctypes.c_double(tolerance),
                                                         ctypes.c_double(variation_tolerance),
                                                         ctypes.c_uint64(seed), ctypes.byref(dt), ctypes.byref(it),

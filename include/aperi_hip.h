@@ -145,10 +145,15 @@ int aphip_step(aphip_ctx *ctx, double time_increment, int32_t n_steps);
  * current displacement: replaces PowerMethodProcessor::ComputeStableTimeIncrement
  * (include/PowerMethodProcessor.h:331-470) of the power-method time stepper
  * (include/TimeStepper.h:138-229); the reference's defaults are 50 iterations,
- * epsilon 1e-5, tolerance 1e-3, variation tolerance 1e-7.  The start vector is
- * uniform in [-1, 1] per DOF from a counter-based generator keyed by `seed` and the
- * caller's DOF index.  u, f, H, P and the state are restored on return. */
-int aphip_power_method_time_increment(aphip_ctx *ctx, int32_t num_iterations, double epsilon, double tolerance,
+ * epsilon 1e-5, tolerance 1e-3, variation tolerance 1e-7.  previous_time_increment is
+ * the stepper's last full (unscaled) increment, 0 on the first call: the first
+ * iteration's convergence test compares against it.  The first call after the mesh is
+ * set starts from a vector uniform in [-1, 1] per DOF (a counter-based generator keyed
+ * by `seed` and the caller's DOF index, times epsilon and the node's max edge length);
+ * every later call starts from the eigenvector the previous call ended with, as the
+ * reference's processor does.  u, f, H, P and the state are restored on return. */
+int aphip_power_method_time_increment(aphip_ctx *ctx, double previous_time_increment, int32_t num_iterations,
+                                      double epsilon, double tolerance,
                                       double variation_tolerance, uint64_t seed, double *time_increment,
                                       int32_t *iterations, int32_t *converged);
 

@@ -164,14 +164,20 @@ done:
  *           dt = 2 / sqrt(lambda); stop when |dt - dt_prev| < tol dt, or when the variance
  *           of the last five dt is below vtol x their mean
  *   result: the last dt if converged, else the smallest of the last five.
- * The start vector is splitmix64(seed ^ dof) (the reference draws Kokkos XorShift64
+ * The first comparison is against previous_dt (the caller's last full increment, or
+ * 1e-16 when it is 0: :363-364).  The eigenvector persists between calls of one
+ * processor (:350-361, :468-469): with eigenvector_valid the iteration starts from
+ * `eigenvector` (the normalised vector the previous call ended with), and the final
+ * vector is written back to it when it is given.
+ * The random start is splitmix64(seed ^ dof) (the reference draws Kokkos XorShift64
  * numbers keyed by node id: the start differs, the converged eigenvalue does not;
  * parity of the start vector itself is unpinned).  The reference's oscillation test
  * (:424-437) takes its else branch at i = 0 and never reports oscillation; restated.
  */
 int or_power_method(const or_problem *p, const double *u_in, const double *state_n, const double *max_edge,
                     int n_iter, double epsilon, double tolerance, double variation_tolerance, uint64_t seed,
-                    int nthreads, double *dt_out, int *iterations_out, int *converged_out) {
+                    double previous_dt, double *eigenvector, int eigenvector_valid, int nthreads, double *dt_out,
+                    int *iterations_out, int *converged_out) {
     const int64_t N = p->n_nodes, M = 3 * N;
     const int64_t ns = p->mat.kind == OR_J2 ? (p->etype == OR_HEX8 && p->mat.state_per_qp ? 8 : 1) * p->n_elem : 1;
     double *vec = (double *)calloc((size_t)M, sizeof(double));
@@ -183,15 +189,19 @@ int or_power_method(const or_problem *p, const double *u_in, const double *state
     int rc = -1;
     if (!vec || !up || !f0 || !f1 || !sn || !snp1) goto done;
     if (state_n && p->mat.kind == OR_J2) memcpy(sn, state_n, sizeof(double) * (size_t)ns);
-    for (int64_t i = 0; i < M; ++i) {
-        const double r = (double)(splitmix64(seed ^ (uint64_t)i) >> 11) * (1.0 / 9007199254740992.0);
-        vec[i] = (2.0 * r - 1.0) * (epsilon * max_edge[i / 3]);
+    if (eigenvector && eigenvector_valid) {
+        memcpy(vec, eigenvector, sizeof(double) * (size_t)M);
+    } else {
+        for (int64_t i = 0; i < M; ++i) {
+            const double r = (double)(splitmix64(seed ^ (uint64_t)i) >> 11) * (1.0 / 9007199254740992.0);
+            vec[i] = (2.0 * r - 1.0) * (epsilon * max_edge[i / 3]);
+        }
     }
     if ((rc = compute_force(p, u_in, sn, snp1, NULL, NULL, f0, nthreads)) != 0) goto done;
     {
         enum { W = 5 };
         double recent[W] = {0, 0, 0, 0, 0};
-        double dt_np1 = 1.0e-16, dt_n;
+        double dt_np1 = previous_dt == 0.0 ? 1.0e-16 : previous_dt, dt_n;
         int converged = 0, oscillating = 0, used = n_iter;
         for (int k = 0; k < n_iter; ++k) {
             dt_n = dt_np1;
@@ -243,6 +253,7 @@ int or_power_method(const or_problem *p, const double *u_in, const double *state
         double wmin = recent[0];
         for (int i = 1; i < W; ++i) wmin = recent[i] < wmin ? recent[i] : wmin;
         *dt_out = converged ? (oscillating ? wmin : dt_np1) : wmin;
+        if (eigenvector) memcpy(eigenvector, vec, sizeof(double) * (size_t)M);
         if (iterations_out) *iterations_out = used;
         if (converged_out) *converged_out = converged;
         rc = 0;

@@ -99,10 +99,13 @@ int or_solve(const or_problem *p, double dt, int n_steps, int do_initial,
  * iteration count, perturbation eps v about u = 0. */
 double or_power_method_dt(const or_problem *p, int n_iter, double epsilon, uint64_t seed, int nthreads);
 /* The reference's power method (include/PowerMethodProcessor.h:331-470) about u_in with
- * per-node max edge lengths, its convergence tests and its result rule; returns 0 on success. */
+ * per-node max edge lengths, its convergence tests and its result rule; previous_dt seeds
+ * the first convergence test, `eigenvector` (3 n_nodes, optional) carries the vector from
+ * one call to the next (used as the start when eigenvector_valid); returns 0 on success. */
 int or_power_method(const or_problem *p, const double *u_in, const double *state_n, const double *max_edge,
                     int n_iter, double epsilon, double tolerance, double variation_tolerance, uint64_t seed,
-                    int nthreads, double *dt_out, int *iterations_out, int *converged_out);
+                    double previous_dt, double *eigenvector, int eigenvector_valid, int nthreads, double *dt_out,
+                    int *iterations_out, int *converged_out);
 
 /* ---- RKPM (or_rk.c) ---- */
 int or_max_edge_length(int64_t n_nodes, const double *X, int64_t n_elem, const int32_t *conn, int npe,

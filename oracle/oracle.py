@@ -262,17 +262,22 @@ class Solver:
                                         ctypes.c_uint64(seed), nthreads)
 
     def power_method(self, max_edge, u_in=None, n_iter=50, epsilon=1e-5, tolerance=1e-3,
-                     variation_tolerance=1e-7, seed=20261202, nthreads=1):
+                     variation_tolerance=1e-7, seed=20261202, nthreads=1, previous_dt=0.0, eigenvector=None):
         """The reference's power-method increment about u_in (default: the solver's u):
-        returns (dt, iterations, converged)."""
+        returns (dt, iterations, converged).  `eigenvector`, a dict, carries the vector
+        between calls like the reference's processor does (empty dict: random start)."""
         u = _f64(self.u if u_in is None else u_in)
         l = _f64(max_edge)
         dt, it, cv = ctypes.c_double(), ctypes.c_int(), ctypes.c_int()
         st = None if self.state.size == 0 else _f64(self.state)
+        eig, valid = None, 0
+        if eigenvector is not None:
+            valid = 1 if "v" in eigenvector else 0
+            eig = eigenvector.setdefault("v", np.zeros(u.size))
         rc = lib().or_power_method(ctypes.byref(self.p), _p(u), _p(st), _p(l), n_iter, ctypes.c_double(epsilon),
                                    ctypes.c_double(tolerance), ctypes.c_double(variation_tolerance),
-                                   ctypes.c_uint64(seed), nthreads, ctypes.byref(dt), ctypes.byref(it),
-                                   ctypes.byref(cv))
+                                   ctypes.c_uint64(seed), ctypes.c_double(previous_dt), _p(eig), valid, nthreads,
+                                   ctypes.byref(dt), ctypes.byref(it), ctypes.byref(cv))
         if rc != 0:
             raise RuntimeError("or_power_method failed")
         return dt.value, it.value, bool(cv.value)

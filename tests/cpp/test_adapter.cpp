@@ -124,10 +124,10 @@ std::vector<int32_t> SplitTets(const MeshBlock &m, const std::vector<int32_t> &h
     return tets;
 }
 
-std::shared_ptr<Material> MakeMaterial(MaterialType type, double E = 200e9, double nu = 0.3) {
+std::shared_ptr<Material> MakeMaterial(MaterialType type, double E = 200e9, double nu = 0.3, double rho = 7850.0) {
     auto p = std::make_shared<MaterialProperties>();
     p->material_type = type;
-    p->density = 7850.0;
+    p->density = rho;
     p->linear_elastic_properties.youngs_modulus = E;
     p->linear_elastic_properties.poisson_ratio = nu;
     if (type == PLASTIC) {
@@ -261,7 +261,79 @@ void ShearCase(ElementTopology topo, MaterialType type) {
     EXPECT_NEAR_REL(work, -PH, 1e-9);
 }
 
+// The reference's single-tet regression (test/regression_tests/single_tet/fem): mesh of
+// mesh/tet.jou (base on the unit circle with node 2 on the x axis, apex 1 above the base
+// centre), base fixed, apex free in z, neo-Hookean rho = E = 1, nu = 0.3, v_z = -0.001.
+// Gold maxima: compare.exodiff:26-52 (also tests/golden/single_tet_fem_maxima.json).
+std::shared_ptr<ElementBase> SingleTet() {
+    MeshBlock mesh;
+    const double s3 = std::sqrt(3.0) / 2.0;
+    mesh.coordinates = {0.0, 0.0, 1.0, 1.0, 0.0, 0.0, -0.5, s3, 0.0, -0.5, -s3, 0.0};
+    mesh.connectivity = {0, 1, 3, 2};
+    auto elem = CreateElement(ElementTopology::Tetrahedron4, ApproximationSpaceType::FiniteElement,
+                              IntegrationSchemeType::GaussQuadrature, mesh, std::make_shared<Device>(),
+                              MakeMaterial(NEO_HOOKEAN, 1.0, 0.3, 1.0));
+    elem->SetFixedDofs({1, 1, 0, 1, 1, 1, 1, 1, 1, 1, 1, 1});
+    elem->SetField(APHIP_FIELD_V, {0, 0, -0.001, 0, 0, -0.001, 0, 0, -0.001, 0, 0, -0.001});
+    return elem;
+}
+
 void GpuCases() {
+    Run("RegressionTest.SingleTetMatchesReferenceGold", [] {
+        // ExplicitSolver::Solve with a DirectTimeStepper of 0.086222 to 8.6222 and output
+        // every 0.086222: the maxima of |u_z|, |a_z|, |f_x|, |f_y|, |f_z| over all outputs
+        // and nodes with their 1-offset output index and node, against the gold summary.
+        auto elem = SingleTet();
+        ExplicitSolver solver(std::make_shared<InternalForceContribution>(elem));
+        DirectTimeStepper stepper(0.086222, 8.6222);
+        TimeIncrementScheduler output(0.0, 1000.0, 0.086222);
+        struct Max {
+            double v = 0.0;
+            size_t t = 0, n = 0;
+        } mx[5];
+        const int field[5] = {APHIP_FIELD_U, APHIP_FIELD_A, APHIP_FIELD_F, APHIP_FIELD_F, APHIP_FIELD_F};
+        const int comp[5] = {2, 2, 0, 1, 2};
+        size_t n_out = 0;
+        const double t_end = solver.Solve(stepper, output, [&](double, size_t) {
+            ++n_out;
+            for (int q = 0; q < 5; ++q) {
+                const std::vector<double> x = elem->GetField(field[q]);
+                for (size_t i = 0; i < 4; ++i)
+                    if (std::fabs(x[3 * i + comp[q]]) > mx[q].v) mx[q] = {std::fabs(x[3 * i + comp[q]]), n_out, i + 1};
+            }
+        });
+        EXPECT_TRUE(n_out == 101);
+        EXPECT_NEAR_REL(t_end, 8.6222, 1e-12);
+        const double gold[5] = {0.00043301454, 0.0023315511, 7.2100279e-05, 6.2440674e-05, 0.00025239781};
+        const size_t gold_t[5] = {56, 9, 9, 9, 9}, gold_n[5] = {1, 1, 2, 3, 1};
+        for (int q = 0; q < 5; ++q) {
+            std::printf("    max %d: %.10e at t%zu,n%zu (gold %.8e at t%zu,n%zu)\n", q, mx[q].v, mx[q].t, mx[q].n, gold[q],
+                        gold_t[q], gold_n[q]);
+            EXPECT_NEAR_REL(mx[q].v, gold[q], 5e-8);
+            EXPECT_TRUE(mx[q].t == gold_t[q] && mx[q].n == gold_n[q]);
+        }
+    });
+
+    Run("TimeStepperTest.PowerMethodStepperSingleTet", [] {
+        // The input's own stepper (scale 0.1, update every 10 steps): updates at steps
+        // 0, 10, ..., the first at the closed-form 2 / sqrt(V (lambda + 2 mu) / m) to the
+        // finite-difference accuracy, the run ending exactly at time_end.
+        auto elem = SingleTet();
+        ExplicitSolver solver(std::make_shared<InternalForceContribution>(elem));
+        PowerMethodTimeStepper stepper(0.1, 10, 8.6222);
+        TimeIncrementScheduler output(0.0, 1000.0, 0.086222);
+        size_t last = 0;
+        const double t_end = solver.Solve(stepper, output, [&](double, size_t n) { last = n; });
+        EXPECT_TRUE(t_end == 8.6222);
+        const auto &up = stepper.Updates();
+        EXPECT_TRUE(last >= 100 && up.size() >= 10 && 10 * (up.size() - 1) < last);
+        for (size_t k = 0; k < up.size(); ++k) EXPECT_TRUE(up[k].first == 10 * k);
+        const double lam = 0.3 / (1.3 * 0.4), mu = 1.0 / 2.6, V = std::sqrt(3.0) / 4.0, m = V / 4.0;
+        EXPECT_NEAR_REL(up[0].second, 2.0 / std::sqrt(V * (lam + 2.0 * mu) / m), 2e-5);
+        for (const auto &u : up) EXPECT_NEAR_REL(u.second, up[0].second, 2e-3);  // stiffness varies with u
+    });
+
+
     Run("ForceTest.ExplicitShearForceTet4LinearElastic", [] { ShearCase(ElementTopology::Tetrahedron4, LINEAR_ELASTIC); });
     Run("ForceTest.ExplicitShearForceTet4NeoHookean", [] { ShearCase(ElementTopology::Tetrahedron4, NEO_HOOKEAN); });
     Run("ForceTest.ExplicitShearForceHex8LinearElastic", [] { ShearCase(ElementTopology::Hexahedron8, LINEAR_ELASTIC); });
