@@ -131,6 +131,8 @@ struct TilePlan {
     int threads = 0;                  // workgroup size
     int64_t ntiles = 0;
     int maxtn = 0;                    // most nodes in one tile
+    int max_nint = 0;                 // most interior nodes in one tile
+    int f_rounds = 1;                 // LDS rounds of the element forces (tile kernel)
     int64_t nb = 0;                   // shared nodes (finished by the second launch)
     int64_t n_interior = 0;           // first shared node id
     int64_t total_tile_nodes = 0;
@@ -153,7 +155,8 @@ struct TilePlan {
     DevBuf<int32_t> meta;             // [ntiles * 8] packed per-tile scalars of the pipelined step
     void reset() {
         ready = false;
-        te = threads = maxtn = 0;
+        te = threads = maxtn = max_nint = 0;
+        f_rounds = 1;
         ntiles = nb = n_interior = total_tile_nodes = nslot = 0;
         lds_bytes = 0;
         for (int &g : stream_grid) g = 0;

@@ -76,10 +76,15 @@ struct TileArgs {
     int64_t ne, ntiles, nblocks;
     int64_t nn, nslot;      // nodes; shared-node slots (bpart holds 3 doubles each)
     int te, maxtn;
+    int max_nint;  // most interior nodes in one tile (Hex8: LDS rows of the corrector data)
     double dt;
 };
 
-// LDS carve-up (bytes), sized on the host from maxtn and te.
+// LDS carve-up (bytes), sized on the host (tile_lds_bytes) from maxtn, max_nint, te
+// and the number of force rounds.  X and U (the predicted displacement) are kept for
+// every tile node; v_half, m, dt and the fixed mask for max_nint rows (Hex8: the
+// interior nodes, the only ones the kernel finishes; Tet4: every tile node); F holds
+// the element forces of one round.
 struct TileLds {
     double *X, *U, *V, *F, *M, *DT;
     uint32_t *csr;
@@ -88,19 +93,25 @@ struct TileLds {
     uint8_t *fx;
 };
 
-__device__ __forceinline__ TileLds carve(char *base, int maxtn, int te, int npe) {
+__host__ __device__ __forceinline__ TileLds carve(char *base, int maxtn, int max_nint, int te, int f_rounds,
+                                                  int npe) {
     TileLds L;
     L.X = reinterpret_cast<double *>(base);
     L.U = L.X + 3 * maxtn;
     L.V = L.U + 3 * maxtn;
-    L.M = L.V + 3 * maxtn;
-    L.DT = L.M + maxtn;
-    L.F = L.DT + maxtn;
-    L.csr = reinterpret_cast<uint32_t *>(L.F + 3 * te * npe);
+    L.M = L.V + 3 * max_nint;
+    L.DT = L.M + max_nint;
+    L.F = L.DT + max_nint;
+    L.csr = reinterpret_cast<uint32_t *>(L.F + 3 * (te / f_rounds) * npe);
     L.bdst = reinterpret_cast<int32_t *>(L.csr + maxtn);
     L.ent = reinterpret_cast<uint16_t *>(L.bdst + maxtn);
     L.fx = reinterpret_cast<uint8_t *>(L.ent + te * npe);
     return L;
+}
+
+__host__ inline size_t tile_lds_bytes(int maxtn, int max_nint, int te, int f_rounds, int npe) {
+    return sizeof(double) * (6 * (size_t)maxtn + 5 * (size_t)max_nint + 3 * (size_t)(te / f_rounds) * npe) +
+           8 * (size_t)maxtn + 2 * (size_t)te * npe + (size_t)max_nint + 16;
 }
 
 // Optional arrays are read without branches: an absent array is replaced by the
@@ -131,6 +142,33 @@ __device__ __forceinline__ void load_elem_lame(const TileArgs &A, int64_t e, Mat
 // point where no store is outstanding this costs nothing beyond the loads' latency.
 __device__ __forceinline__ void wait_vmem_loads() { __builtin_amdgcn_s_waitcnt(0x0f70); }
 
+// Node ln's force f is complete: an interior node is finished and written (MODE 0: f;
+// 1: f and a = f/m; 2: the corrector); a shared node parks f in its slot for the
+// shared-node pass.
+template <int MODE>
+__device__ __forceinline__ void finish_node(const TileArgs &A, const TileLds &L, int ln, int nint, int64_t ibase,
+                                            const double (&f)[3]) {
+    if (ln < nint) {
+        const int64_t g = ibase + ln;
+        st3(A.f, g, f);
+        if constexpr (MODE == 1) {
+            const double mm = L.M[ln];
+            const double acc[3] = {f[0] / mm, f[1] / mm, f[2] / mm};
+            st3(A.a, g, acc);
+        } else if constexpr (MODE == 2) {
+            double vh[3], u1[3], a1[3], v1[3];
+            ld3(L.V, ln, vh);
+            ld3(L.U, ln, u1);
+            cd_correct(f, L.M[ln], L.fx[ln], L.DT[ln], vh, a1, v1);
+            st3(A.u, g, u1);
+            st3(A.v, g, v1);
+            st3(A.a, g, a1);
+        }
+    } else {
+        st3(A.bpart, L.bdst[ln], f);
+    }
+}
+
 // MODE 0: force only; 1: force + a = f/m (initial); 2: full central-difference step.
 // QPAR (Hex8 only): eight lanes per element, one per Gauss point.
 template <int NPE, int KIND, bool PER_QP, int MODE, bool QPAR>
@@ -138,7 +176,11 @@ __global__ __launch_bounds__(NPE == 4 ? 512 : (QPAR ? 512 : 128)) void k_tile(Ti
     extern __shared__ __align__(16) char lds_raw[];
     const int64_t t = xcd_tile(A.nblocks);
     if (t >= A.ntiles) return;
-    const TileLds L = carve(lds_raw, A.maxtn, A.te, NPE);
+    // One-element-per-thread Hex8 tiles pass their forces through LDS in two rounds
+    // (24 doubles per element otherwise keep a 128-element tile at one wave per SIMD);
+    // Tet4 and QPAR tiles in one.  The host sizes the LDS with the same counts.
+    constexpr int ROUNDS = (NPE == 8 && !QPAR) ? 2 : 1;
+    const TileLds L = carve(lds_raw, A.maxtn, NPE == 4 ? A.maxtn : A.max_nint, A.te, ROUNDS, NPE);
     const int tid = threadIdx.x;
     const int tb = A.tn_start[t];
     const int tn = A.tn_start[t + 1] - tb;
@@ -199,14 +241,19 @@ __global__ __launch_bounds__(NPE == 4 ? 512 : (QPAR ? 512 : 128)) void k_tile(Ti
         L.csr[ln] = csr;
         L.bdst[ln] = bdst;
         st3(L.X, ln, x);
-        L.fx[ln] = (uint8_t)fx;
-        L.M[ln] = mm;
-        L.DT[ln] = nd;
+        // Hex8 keeps the corrector data of interior nodes only (the only ones it finishes);
+        // Tet4 writes it for every node (measured faster with its 512-element tiles).
+        const bool interior = NPE == 4 || ln < nint;
+        if (interior) {
+            L.fx[ln] = (uint8_t)fx;
+            L.M[ln] = mm;
+            L.DT[ln] = nd;
+        }
         if constexpr (MODE == 2) {
             double vh[3], u1[3];
             cd_predict(u, v, a, fx, nd, vh, u1);
             st3(L.U, ln, u1);
-            st3(L.V, ln, vh);
+            if (interior) st3(L.V, ln, vh);
         } else {
             st3(L.U, ln, u);
         }
@@ -219,14 +266,19 @@ __global__ __launch_bounds__(NPE == 4 ? 512 : (QPAR ? 512 : 128)) void k_tile(Ti
     wait_vmem_loads();
     __syncthreads();
 
-    // B: one element per thread, or one Gauss point per lane
+    // B: one element per thread, or one Gauss point per lane.  The element's nodal
+    // forces (QPAR: this lane's node) stay in registers until their round's LDS pass.
+    constexpr int KF = (QPAR && NPE == 8) ? 1 : NPE;
+    double fe[KF][3];
+    const int te_round = ROUNDS == 1 ? A.te : A.te / ROUNDS;
+    const int my_round = ROUNDS == 1 ? 0 : le / te_round;
     if constexpr (QPAR && NPE == 8) {
         if (has_elem) {
             const int q = tid & 7;
             const int64_t so = PER_QP ? e * 8 : e;
-            double H[9], P[9], fk[3];
+            double H[9], P[9];
             hex8_qp_lane<KIND, PER_QP>(q, lc, L.X, L.U, m, A.gauss_a, KIND == APHIP_J2 ? A.state_n + so : nullptr,
-                                       KIND == APHIP_J2 ? A.state_np1 + so : nullptr, H, P, fk);
+                                       KIND == APHIP_J2 ? A.state_np1 + so : nullptr, H, P, fe[0]);
             if (q == 7) {
 #pragma unroll
                 for (int i = 0; i < 9; ++i) {
@@ -234,10 +286,10 @@ __global__ __launch_bounds__(NPE == 4 ? 512 : (QPAR ? 512 : 128)) void k_tile(Ti
                     store_once(A.P + i * A.ne + e, P[i]);
                 }
             }
-            st3(L.F, le * NPE + q, fk);
+            if (my_round == 0) st3(L.F, le * NPE + q, fe[0]);
         }
     } else if (has_elem) {
-        double X[NPE][3], U[NPE][3], fe[NPE][3];
+        double X[NPE][3], U[NPE][3];
 #pragma unroll
         for (int k = 0; k < NPE; ++k) {
             ld3(L.X, lc[k], X[k]);
@@ -258,41 +310,77 @@ __global__ __launch_bounds__(NPE == 4 ? 512 : (QPAR ? 512 : 128)) void k_tile(Ti
             store_once(A.H + i * A.ne + e, H[i]);
             store_once(A.P + i * A.ne + e, P[i]);
         }
+        if (my_round == 0) {
 #pragma unroll
-        for (int k = 0; k < NPE; ++k) st3(L.F, tid * NPE + k, fe[k]);
+            for (int k = 0; k < NPE; ++k) st3(L.F, le * NPE + k, fe[k]);
+        }
     }
     __syncthreads();
 
-    // C: deterministic per-node gather from LDS; finish interior nodes, park shared-node sums
-    for (int ln = tid; ln < tn; ln += blockDim.x) {
-        const uint32_t pk = L.csr[ln];
-        const int st = (int)(pk >> 16), cnt = (int)(pk & 0xffffu);
-        double f[3] = {0.0, 0.0, 0.0};
-        for (int j = 0; j < cnt; ++j) {
-            const int s = L.ent[st + j];
-            f[0] += L.F[3 * s];
-            f[1] += L.F[3 * s + 1];
-            f[2] += L.F[3 * s + 2];
-        }
-        if (ln < nint) {
-            const int64_t g = ibase + ln;
-            st3(A.f, g, f);
-            if constexpr (MODE == 1) {
-                const double mm = L.M[ln];
-                const double acc[3] = {f[0] / mm, f[1] / mm, f[2] / mm};
-                st3(A.a, g, acc);
-            } else if constexpr (MODE == 2) {
-                double vh[3], u1[3], a1[3], v1[3];
-                ld3(L.V, ln, vh);
-                ld3(L.U, ln, u1);
-                cd_correct(f, L.M[ln], L.fx[ln], L.DT[ln], vh, a1, v1);
-                st3(A.u, g, u1);
-                st3(A.v, g, v1);
-                st3(A.a, g, a1);
+    // C: deterministic per-node gather from LDS (each node's entries in ascending order:
+    // round r adds the entries of elements [r, r + 1) x te / ROUNDS, a contiguous run of
+    // the node's sorted list, so the sum is the same sequence for every round count);
+    // then finish interior nodes and park shared-node sums.
+    if constexpr (ROUNDS == 1) {
+        for (int ln = tid; ln < tn; ln += blockDim.x) {
+            const uint32_t pk = L.csr[ln];
+            const int st = (int)(pk >> 16), cnt = (int)(pk & 0xffffu);
+            double f[3] = {0.0, 0.0, 0.0};
+            for (int j = 0; j < cnt; ++j) {
+                const int s = L.ent[st + j];
+                f[0] += L.F[3 * s];
+                f[1] += L.F[3 * s + 1];
+                f[2] += L.F[3 * s + 2];
             }
-        } else {
-            st3(A.bpart, L.bdst[ln], f);
+            finish_node<MODE>(A, L, ln, nint, ibase, f);
         }
+        return;
+    }
+    // Several rounds: a thread owns the tile nodes tid + q * blockDim (maxtn <= NPE x te
+    // <= NPE x blockDim) and keeps their partial sums in registers between rounds.
+    double acc[NPE][3];
+    int jpos[NPE];
+#pragma unroll
+    for (int q = 0; q < NPE; ++q) {
+        acc[q][0] = acc[q][1] = acc[q][2] = 0.0;
+        jpos[q] = 0;
+    }
+    for (int r = 0; r < ROUNDS; ++r) {
+        if (r > 0) {
+            __syncthreads();  // the previous round's gather is done with F
+            if (has_elem && my_round == r) {
+                if constexpr (QPAR && NPE == 8) {
+                    st3(L.F, (le - r * te_round) * NPE + (tid & 7), fe[0]);
+                } else {
+#pragma unroll
+                    for (int k = 0; k < NPE; ++k) st3(L.F, (le - r * te_round) * NPE + k, fe[k]);
+                }
+            }
+            __syncthreads();
+        }
+        const int off = r * te_round * NPE, lim = off + te_round * NPE;
+#pragma unroll
+        for (int q = 0; q < NPE; ++q) {
+            const int ln = tid + q * (int)blockDim.x;
+            if (ln < tn) {
+                const uint32_t pk = L.csr[ln];
+                const int st = (int)(pk >> 16), cnt = (int)(pk & 0xffffu);
+                int j = jpos[q];
+                for (; j < cnt; ++j) {
+                    const int s = L.ent[st + j];
+                    if (s >= lim) break;
+                    acc[q][0] += L.F[3 * (s - off)];
+                    acc[q][1] += L.F[3 * (s - off) + 1];
+                    acc[q][2] += L.F[3 * (s - off) + 2];
+                }
+                jpos[q] = j;
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < NPE; ++q) {
+        const int ln = tid + q * (int)blockDim.x;
+        if (ln < tn) finish_node<MODE>(A, L, ln, nint, ibase, acc[q]);
     }
 }
 
@@ -857,9 +945,21 @@ void tile_build(Ctx &c, int te, int threads) {
         P.meta.upload(meta.data(), meta.size(), c.stream);
     }
     // LDS: X, U, V (3 doubles each) + mass + increment per node, partials, csr/bdst per node, entries, flags
-    P.lds_bytes = sizeof(double) * (11 * (size_t)maxtn + 3 * (size_t)te * npe) + 8 * (size_t)maxtn +
-                  2 * (size_t)te * npe + (size_t)maxtn + 16;
+    int max_nint = 0;
+    for (int64_t t = 0; t < nt; ++t) max_nint = std::max(max_nint, ninterior[t]);
+    P.max_nint = npe == 4 ? maxtn : max_nint;  // LDS rows of the corrector data (the kernel's choice)
+    // One-element-per-thread Hex8 tiles pass their forces through LDS in two rounds of
+    // te / 2 elements: half the F rows, so twice the workgroups fit per CU.
+    P.f_rounds = (npe == 8 && te == threads) ? 2 : 1;  // the kernel's ROUNDS
+    P.lds_bytes = tile_lds_bytes(P.maxtn, P.max_nint, te, P.f_rounds, npe);  // the kernel carves with these
     APHIP_REQUIRE(P.lds_bytes <= 160 * 1024, "tile needs more LDS than a CU has; use a smaller tile");
+    {  // the kernel's carve-up must end inside the allocation
+        const TileLds L = carve(nullptr, P.maxtn, P.max_nint, te, P.f_rounds, npe);
+        APHIP_REQUIRE((size_t)(reinterpret_cast<const char *>(L.fx + P.max_nint) - static_cast<const char *>(nullptr)) <=
+                          P.lds_bytes,
+                      "tile LDS layout exceeds its allocation");
+    }
+    APHIP_REQUIRE(maxtn <= npe * threads, "tile has more nodes than its gather covers");
     P.use_stream = (c.flags & APHIP_FLAG_PIPELINED) != 0;
     for (int &g : P.stream_grid) g = 0;  // sized (and LDS attribute set) at the first launch
     P.ready = true;
@@ -906,6 +1006,7 @@ void tile_run(Ctx &c, double dt, int mode) {
     A.nblocks = padded(P.ntiles);
     A.te = P.te;
     A.maxtn = P.maxtn;
+    A.max_nint = P.max_nint;
     A.dt = dt;
     prof_begin(c, 1);
     if (c.npe == 4) {

@@ -32,6 +32,12 @@ NODE_BYTES = 200
 # over the step's two launches.  profiles/tile_step_tet4_6M_neohookean_hbm_traffic.txt
 # holds the commands and per-kernel figures.  Keyed by (material, hexes per edge).
 MEASURED_TRAFFIC = {("nh", 100): 1451.0e6}
+# FP64 flops per step of the same workload from rocprofv3 --pmc SQ_INSTS_VALU_{FMA,MUL,ADD,
+# TRANS}_F64 on the tile kernel (64 lanes x (2 FMA + MUL + ADD + TRANS) per wave
+# instruction); profiles/tile_step_tet4_6M_neohookean_fp64_valu.txt.  The shared-node
+# kernel adds 0.4 %.  FP64 vector peak of the MI355X: 78.6 TFLOP/s.
+MEASURED_FP64_FLOPS = {("nh", 100): 3.33e9}
+FP64_PEAK_TFLOPS = 78.6
 
 
 def log(msg):
@@ -309,9 +315,23 @@ def cpu_baseline(w, kind, budget_s=12.0):
     ref.run(dt, n, nthreads=threads)
     el = time.perf_counter() - t0
     ne = conn.shape[0]
+    t0 = time.perf_counter()
+    ref.run(dt, 1, nthreads=1)
+    one_thread = time.perf_counter() - t0
     return dict(value=ne * n / el, unit="element-updates/s", cores=threads, kind="port",
                 sample=f"{n} steps of the same 6M-tet neo-Hookean workload after 1 warm step "
-                       f"({el:.1f}s, OpenMP atomic scatter, oracle/liboracle.so)")
+                       f"({el:.1f}s, OpenMP atomic scatter, oracle/liboracle.so)",
+                single_thread_value=ne / one_thread)
+
+
+def fp64_line(flops_per_step, step_ms):
+    """FP64 VALU use beside the HBM roofline: counter-measured flops per step over the
+    event-timed step."""
+    if flops_per_step is None:
+        return None
+    achieved = flops_per_step / (step_ms * 1e-3) / 1e12
+    return {"flops_per_step": flops_per_step, "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": achieved / FP64_PEAK_TFLOPS}
 
 
 def bench_power(n=100):
@@ -399,6 +419,7 @@ def main():
                      "bytes_per_update": bytes_per_update,
                      "algorithmic_bytes_per_step": bytes_per_update * ne},
         "kernels_ms": {"step": r["step_ms"], "tile_kernel": r["force_ms"], "shared_node_kernel": r["node_ms"]},
+        "fp64_valu": fp64_line(MEASURED_FP64_FLOPS.get((args.material, args.n)), r["step_ms"]),
         "launch": {"mode": r["mode"], "ms_per_step": {k: 1e3 * v / args.steps for k, v in r["walls"].items()},
                    "tiling": "512 Tet4 per workgroup, one per thread (default plan)"},
     }
