@@ -874,7 +874,8 @@ int aphip_autotune(aphip_ctx *c, double dt, int32_t trial_steps, int32_t *chosen
         cand = {{64, 64, false},  {128, 128, false}, {256, 256, false}, {512, 512, false},
                 {128, 128, true}, {256, 256, true}};
     else
-        cand = {{64, 64, false}, {128, 128, false}, {16, 128, false}, {32, 256, false}, {64, 512, false}};
+        cand = {{64, 64, false}, {128, 128, false}, {256, 256, false}, {16, 128, false}, {32, 256, false},
+                {64, 512, false}};
     const bool was_prof = c->prof.enabled;
     Cand best{prev_te, c->tiles.threads, c->tiles.use_stream};
     double best_ms = 1e300;

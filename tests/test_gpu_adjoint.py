@@ -163,3 +163,26 @@ def test_torch_autograd_function(gpu_available):
     assert np.allclose(uT.detach().numpy(), u, rtol=0, atol=0)
     assert np.array_equal(Et.grad.numpy(), Eb) and np.array_equal(nt.grad.numpy(), nb)
     assert np.array_equal(vt.grad.numpy(), v0b)
+
+
+def test_vjp_independent_of_tile_shape(gpu_available):
+    """The VJP evaluates one Hex8 per thread in workgroups of at most 128: a context
+    planned with 256-element tiles is re-planned for it, and the result matches a
+    context with the default plan to rounding.  The mesh (9^3 = 729 Hex8) spans both
+    force rounds of every tile shape."""
+    X, conn = _mesh(A.HEX8, 9, 12)
+    E, nu = _fields(len(conn), 3)
+    u = 2e-2 * (2 * M.uniform01(16, 7, X.size).reshape(X.shape) - 1)
+    fb = M.normal(18, 1, X.size).reshape(X.shape)
+    out = []
+    for tiling in (None, (256, 256)):
+        with A.Solver() as s:
+            s.set_mesh(A.HEX8, X, conn)
+            s.set_material(A.NEO_HOOKEAN, RHO, 200e9, 0.3)
+            s.set_material_fields(E, nu)
+            if tiling:
+                s.set_tiling(*tiling)
+            s.set_field(A.FIELD_U, u)
+            out.append(s.internal_force_vjp(fb))
+    for a, b in zip(*out):
+        assert np.abs(a - b).max() <= 1e-12 * np.abs(b).max()
