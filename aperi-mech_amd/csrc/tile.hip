@@ -686,35 +686,55 @@ struct BoundArgs {
     const double *mass;
     const uint8_t *fixed;
     const double *node_dt;
+    const double *zero;  // 8 zero doubles: stands in for an absent optional array
     double dt;
 };
 
 template <int MODE>
 __global__ __launch_bounds__(256) void k_boundary(BoundArgs A) {
-    const int64_t b = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (b >= A.nb) return;
+    const int64_t b_raw = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const bool live = b_raw < A.nb;
+    const int64_t b = live ? b_raw : A.nb - 1;  // idle lanes redo the last node and store nothing
     const int64_t g = A.first + b;
-    double f[3] = {0.0, 0.0, 0.0};
-    const int32_t s1 = A.bstart[b + 1];
-    for (int32_t s = A.bstart[b]; s < s1; ++s) {
-        f[0] += A.bpart[3 * (int64_t)s];
-        f[1] += A.bpart[3 * (int64_t)s + 1];
-        f[2] += A.bpart[3 * (int64_t)s + 2];
-    }
-    st3(A.f, g, f);
-    if constexpr (MODE == 1) {
-        const double m = A.mass[g];
-        const double acc[3] = {f[0] / m, f[1] / m, f[2] / m};
-        st3(A.a, g, acc);
-    } else if constexpr (MODE == 2) {
-        double u[3], v[3], a[3], vh[3], u1[3], a1[3], v1[3];
+    const int32_t s0 = A.bstart[b], s1 = A.bstart[b + 1];
+    // node data first: independent of the slot sums, so its loads overlap them
+    double u[3], v[3], a[3], m = 1.0, dt = A.dt;
+    unsigned fx = 0u;
+    if constexpr (MODE >= 1) m = A.mass[g];
+    if constexpr (MODE == 2) {
         ld3(A.u, g, u);
         ld3(A.v, g, v);
         ld3(A.a, g, a);
-        const unsigned fx = A.fixed ? A.fixed[g] : 0u;
-        const double dt = A.node_dt ? A.node_dt[g] : A.dt;
+        const uint8_t *pf = A.fixed ? A.fixed : reinterpret_cast<const uint8_t *>(A.zero);
+        const double *pd = A.node_dt ? A.node_dt : A.zero;
+        const unsigned f8 = pf[A.fixed ? g : 0];
+        const double d8 = pd[A.node_dt ? g : 0];
+        fx = A.fixed ? f8 : 0u;
+        dt = A.node_dt ? d8 : A.dt;
+    }
+    // slot sums in ascending tile order, four slots' loads at a time (clamped; an
+    // absent slot adds zero)
+    double f[3] = {0.0, 0.0, 0.0};
+    for (int32_t sb = s0; sb < s1; sb += 4) {
+        double p[4][3];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) ld3(A.bpart, min(sb + r, s1 - 1), p[r]);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const bool in = sb + r < s1;
+#pragma unroll
+            for (int d = 0; d < 3; ++d) f[d] += in ? p[r][d] : 0.0;
+        }
+    }
+    if (!live) return;
+    st3(A.f, g, f);
+    if constexpr (MODE == 1) {
+        const double acc[3] = {f[0] / m, f[1] / m, f[2] / m};
+        st3(A.a, g, acc);
+    } else if constexpr (MODE == 2) {
+        double vh[3], u1[3], a1[3], v1[3];
         cd_predict(u, v, a, fx, dt, vh, u1);
-        cd_correct(f, A.mass[g], fx, dt, vh, a1, v1);
+        cd_correct(f, m, fx, dt, vh, a1, v1);
         st3(A.u, g, u1);
         st3(A.v, g, v1);
         st3(A.a, g, a1);
@@ -1022,7 +1042,7 @@ void tile_run(Ctx &c, double dt, int mode) {
     prof_end(c, 1);
     if (P.nb > 0) {
         BoundArgs B{P.nb, P.n_interior, P.bstart.p, P.bpart.p, c.u.p, c.v.p, c.a.p, c.f.p, c.mass.p,
-                    c.has_fixed ? c.fixed.p : nullptr, c.node_dt.p, dt};
+                    c.has_fixed ? c.fixed.p : nullptr, c.node_dt.p, P.zero.p, dt};
         const dim3 grid((unsigned)((P.nb + 255) / 256));
         prof_begin(c, 2);
         if (mode == 0) hipLaunchKernelGGL(k_boundary<0>, grid, dim3(256), 0, c.stream, B);
