@@ -485,8 +485,9 @@ void require_adjoint_ready(Ctx &c) {
     APHIP_REQUIRE(c.mat.kind == APHIP_LINEAR_ELASTIC || c.mat.kind == APHIP_NEO_HOOKEAN,
                   "adjoints are provided for linear elastic and neo-Hookean materials");
     APHIP_REQUIRE(c.tiles.ready && !c.cells.ready, "adjoints run on the tiled Gaussian element path");
-    // the adjoint kernels evaluate one element per thread, at most 128 Hex8 per workgroup
-    if (c.tiles.threads != c.tiles.te || (c.npe == 8 && c.tiles.te > 128)) tile_build(c, 64, 64);
+    // the adjoint kernels evaluate one element per thread, at most 512 Tet4 or 128 Hex8
+    // per workgroup
+    if (c.tiles.threads != c.tiles.te || c.tiles.te > (c.npe == 8 ? 128 : 512)) tile_build(c, 64, 64);
 }
 
 void alloc_adjoint(Ctx &c) {

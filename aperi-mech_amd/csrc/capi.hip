@@ -203,8 +203,16 @@ static bool use_tiles(const Ctx &c) { return !c.cells.ready && c.tiles.ready && 
 
 static void default_tiling(Ctx &c) {
     if (c.ne == 0 || (c.flags & APHIP_FLAG_TWO_PASS)) return;
-    if (c.etype == APHIP_TET4) tile_build(c, 512, 512);
-    else tile_build(c, 128, 128);
+    if (c.etype == APHIP_TET4) {
+        // 1024-element tiles (fewer shared nodes) when the mesh's tiles fit a CU's LDS
+        try {
+            tile_build(c, 1024, 1024);
+        } catch (const Error &) {
+            tile_build(c, 512, 512);
+        }
+    } else {
+        tile_build(c, 128, 128);
+    }
 }
 
 
@@ -872,7 +880,7 @@ int aphip_autotune(aphip_ctx *c, double dt, int32_t trial_steps, int32_t *chosen
     std::vector<Cand> cand;
     if (c->etype == APHIP_TET4)
         cand = {{64, 64, false},  {128, 128, false}, {256, 256, false}, {512, 512, false},
-                {128, 128, true}, {256, 256, true}};
+                {1024, 1024, false}, {128, 128, true}, {256, 256, true}};
     else
         cand = {{64, 64, false}, {128, 128, false}, {256, 256, false}, {16, 128, false}, {32, 256, false},
                 {64, 512, false}};
@@ -884,7 +892,11 @@ int aphip_autotune(aphip_ctx *c, double dt, int32_t trial_steps, int32_t *chosen
     APHIP_HIP(hipEventCreate(&e1));
     c->prof.enabled = false;
     for (const auto &tt : cand) {
-        tile_build(*c, tt.te, tt.threads);
+        try {
+            tile_build(*c, tt.te, tt.threads);
+        } catch (const Error &) {
+            continue;  // this shape does not fit the mesh (e.g. a tile needs more LDS than a CU has)
+        }
         c->tiles.use_stream = tt.pipelined;
         step_once(*c, dt);  // warm-up
         APHIP_HIP(hipEventRecord(e0, c->stream));

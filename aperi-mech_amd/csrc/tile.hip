@@ -172,7 +172,7 @@ __device__ __forceinline__ void finish_node(const TileArgs &A, const TileLds &L,
 // MODE 0: force only; 1: force + a = f/m (initial); 2: full central-difference step.
 // QPAR (Hex8 only): eight lanes per element, one per Gauss point.
 template <int NPE, int KIND, bool PER_QP, int MODE, bool QPAR>
-__global__ __launch_bounds__(NPE == 4 ? 512 : (QPAR ? 512 : 256), NPE == 8 && !QPAR ? 2 : 1) void k_tile(TileArgs A) {
+__global__ __launch_bounds__(NPE == 4 ? 1024 : (QPAR ? 512 : 256), NPE == 8 && !QPAR ? 2 : 1) void k_tile(TileArgs A) {
     extern __shared__ __align__(16) char lds_raw[];
     const int64_t t = xcd_tile(A.nblocks);
     if (t >= A.ntiles) return;
@@ -824,8 +824,9 @@ void tile_build(Ctx &c, int te, int threads) {
     APHIP_REQUIRE(!c.cells.ready, "tiling applies to the Gaussian element path");
     c.graph_reset();
     if (npe == 4) {
-        APHIP_REQUIRE(te == threads && (threads == 64 || threads == 128 || threads == 256 || threads == 512),
-                      "Tet4 tiles: one element per thread, 64, 128, 256 or 512 threads");
+        APHIP_REQUIRE(te == threads && (threads == 64 || threads == 128 || threads == 256 || threads == 512 ||
+                                        threads == 1024),
+                      "Tet4 tiles: one element per thread, 64, 128, 256, 512 or 1024 threads");
     } else {
         const bool per_elem = te == threads && (threads == 64 || threads == 128 || threads == 256);
         const bool per_qp = threads == 8 * te && (te == 8 || te == 16 || te == 32 || te == 64);

@@ -421,7 +421,7 @@ def main():
         "kernels_ms": {"step": r["step_ms"], "tile_kernel": r["force_ms"], "shared_node_kernel": r["node_ms"]},
         "fp64_valu": fp64_line(MEASURED_FP64_FLOPS.get((args.material, args.n)), r["step_ms"]),
         "launch": {"mode": r["mode"], "ms_per_step": {k: 1e3 * v / args.steps for k, v in r["walls"].items()},
-                   "tiling": "512 Tet4 per workgroup, one per thread (default plan)"},
+                   "tiling": "1024 Tet4 per workgroup, one per thread (default plan)"},
     }
     if not r["finite"]:
         line["error"] = "non-finite displacement"

@@ -158,7 +158,7 @@ int aphip_power_method_time_increment(aphip_ctx *ctx, double previous_time_incre
                                       int32_t *iterations, int32_t *converged);
 
 /* Element-tile shape of the fused step: elements per tile and workgroup size.
- * Tet4: one element per thread (64, 128, 256 or 512).  Hex8: one element per
+ * Tet4: one element per thread (64, 128, 256, 512 or 1024).  Hex8: one element per
  * thread (64, 128 or 256), or eight lanes per element, one per Gauss point
  * (threads = 8 x elements, 8..64 elements).  Rebuilds the tile plan; results do
  * not depend on it beyond the rounding of the summation grouping.  No reference

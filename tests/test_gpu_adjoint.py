@@ -165,19 +165,20 @@ def test_torch_autograd_function(gpu_available):
     assert np.array_equal(vt.grad.numpy(), v0b)
 
 
-def test_vjp_independent_of_tile_shape(gpu_available):
-    """The VJP evaluates one Hex8 per thread in workgroups of at most 128: a context
-    planned with 256-element tiles is re-planned for it, and the result matches a
-    context with the default plan to rounding.  The mesh (9^3 = 729 Hex8) spans both
-    force rounds of every tile shape."""
-    X, conn = _mesh(A.HEX8, 9, 12)
+@pytest.mark.parametrize("etype,big", [(A.TET4, (1024, 1024)), (A.HEX8, (256, 256))], ids=["tet4", "hex8"])
+def test_vjp_independent_of_tile_shape(gpu_available, etype, big):
+    """The VJP evaluates one element per thread in workgroups of at most 512 Tet4 or 128
+    Hex8: a context planned with larger tiles is re-planned for it, and the result
+    matches a context with a small plan to rounding.  The meshes (9^3 Hex8 = 729 Hex8,
+    4374 Tet4) span several tiles of every shape."""
+    X, conn = _mesh(etype, 9, 12)
     E, nu = _fields(len(conn), 3)
     u = 2e-2 * (2 * M.uniform01(16, 7, X.size).reshape(X.shape) - 1)
     fb = M.normal(18, 1, X.size).reshape(X.shape)
     out = []
-    for tiling in (None, (256, 256)):
+    for tiling in ((64, 64), big):
         with A.Solver() as s:
-            s.set_mesh(A.HEX8, X, conn)
+            s.set_mesh(etype, X, conn)
             s.set_material(A.NEO_HOOKEAN, RHO, 200e9, 0.3)
             s.set_material_fields(E, nu)
             if tiling:

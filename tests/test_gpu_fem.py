@@ -170,7 +170,7 @@ def test_errors_are_reported_not_crashes(gpu_available):
     s.close()
 
 
-TILINGS = [(A.TET4, (64, 64)), (A.TET4, (128, 128)), (A.TET4, (256, 256)), (A.TET4, (512, 512)),
+TILINGS = [(A.TET4, (64, 64)), (A.TET4, (128, 128)), (A.TET4, (256, 256)), (A.TET4, (512, 512)), (A.TET4, (1024, 1024)),
            (A.TET4, "two_pass"), (A.HEX8, (64, 64)), (A.HEX8, (128, 128)), (A.HEX8, (256, 256)), (A.HEX8, (8, 64)),
            (A.HEX8, (16, 128)), (A.HEX8, (32, 256)), (A.HEX8, (64, 512)), (A.HEX8, "two_pass")]
 
