@@ -12,6 +12,7 @@
 //   TimeStepper, DirectTimeStepper, PowerMethodTimeStepper     include/TimeStepper.h:24-229
 //   ExplicitSolver::Solve (time loop, output schedule)         src/Solvers/Explicit.cpp:308-498
 //   PowerMethodProcessor::ComputeStableTimeIncrement           include/PowerMethodProcessor.h:331-470
+//   IoMesh results output (Exodus II)                          src/IoMesh.cpp:190-240
 // Errors are reported the way the reference reports them: std::runtime_error.
 // Header-only; link against libaperi_hip.so.
 #pragma once
@@ -371,6 +372,33 @@ class InternalForceContribution {
 // The explicit loop of src/Solvers/Explicit.cpp:308-498 with a fixed increment
 // (DirectTimeStepper, include/TimeStepper.h:97-128), run on the device: the
 // initial force and acceleration, then central-difference steps.
+// ---- results output ------------------------------------------------------------
+
+// The results half of the reference's IoMesh (src/IoMesh.cpp:190-240): an Exodus II
+// file of the element's mesh with the displacement, velocity, acceleration, force, mass
+// and essential_boundary node variables and the displacement_gradient, pk1_stress and
+// volume element variables (written by the library, netCDF 64-bit-offset format).
+class IoMesh {
+   public:
+    explicit IoMesh(std::shared_ptr<ElementBase> element) : m_element(std::move(element)) {}
+    ~IoMesh() {
+        if (m_open) (void)aphip_results_close(m_element->Context());
+    }
+    void CreateFieldResultsFile(const std::string &filename, const std::string &title = "") {
+        Check(aphip_results_open(m_element->Context(), filename.c_str(), title.c_str()));
+        m_open = true;
+    }
+    void WriteFieldResults(double time) const { Check(aphip_results_write(m_element->Context(), time)); }
+    void CloseFieldResultsFile() {
+        Check(aphip_results_close(m_element->Context()));
+        m_open = false;
+    }
+
+   private:
+    std::shared_ptr<ElementBase> m_element;
+    bool m_open = false;
+};
+
 // ---- schedulers and time steppers --------------------------------------------
 
 // An event every time_increment from time_start; several events that one call has

@@ -1176,4 +1176,107 @@ int aphip_bandwidth_probe(int32_t device, int64_t nbytes, int32_t reps, double *
     APHIP_API_END
 }
 
+// ---- Exodus II results (the reference's field output) --------------------------
+
+static const char *const kTensorSuffix[9] = {"xx", "xy", "xz", "yx", "yy", "yz", "zx", "zy", "zz"};  // row-major
+
+int aphip_results_open(aphip_ctx *c, const char *path, const char *title) {
+    APHIP_API_BEGIN
+    APHIP_REQUIRE(c != nullptr && path != nullptr, "NULL argument");
+    APHIP_REQUIRE(c->etype >= 0, "mesh not set");
+    APHIP_HIP(hipSetDevice(c->device));
+    c->results.reset();
+    // coordinates and connectivity in the caller's numbering
+    std::vector<double> Xi(3 * c->nn), x(c->nn), y(c->nn), z(c->nn);
+    c->X.download(Xi.data(), Xi.size(), c->stream);
+    APHIP_HIP(hipStreamSynchronize(c->stream));
+    for (int64_t i = 0; i < c->nn; ++i) {
+        const int64_t o = c->node_iperm[i];
+        x[o] = Xi[3 * i];
+        y[o] = Xi[3 * i + 1];
+        z[o] = Xi[3 * i + 2];
+    }
+    std::vector<int32_t> conn((size_t)c->ne * c->npe);
+    for (int64_t e = 0; e < c->ne; ++e)
+        for (int k = 0; k < c->npe; ++k)
+            conn[(size_t)c->elem_iperm[e] * c->npe + k] = c->node_iperm[c->conn_h[(size_t)e * c->npe + k]];
+    exo::Mesh m;
+    m.title = title ? title : "";
+    m.n_nodes = c->nn;
+    m.x = x.data();
+    m.y = y.data();
+    m.z = z.data();
+    m.n_elem = c->ne;
+    m.npe = c->npe;
+    m.conn = conn.data();
+    m.elem_type = c->npe == 4 ? "TETRA4" : "HEX8";
+    std::vector<std::string> nv, ev;
+    for (const char *f : {"displacement", "velocity", "acceleration", "force", "mass", "essential_boundary"})
+        for (const char *d : {"_x", "_y", "_z"}) nv.push_back(std::string(f) + d);
+    for (const char *f : {"displacement_gradient", "pk1_stress"})
+        for (const char *t : kTensorSuffix) ev.push_back(std::string(f) + "_" + t);
+    ev.push_back("volume");
+    if (c->state_count() == c->ne) ev.push_back("equivalent_plastic_strain");
+    c->results.reset(new exo::Writer(path, m, nv, ev));
+    APHIP_API_END
+}
+
+int aphip_results_write(aphip_ctx *c, double time) {
+    APHIP_API_BEGIN
+    APHIP_REQUIRE(c != nullptr, "NULL argument");
+    APHIP_REQUIRE(c->results != nullptr, "no results file open");
+    const int64_t nn = c->nn, ne = c->ne;
+    std::vector<double> nodal(18 * nn, 0.0), elem(19 * ne + ne, 0.0), tmp(9 * std::max(nn, ne));
+    const int32_t vec_fields[4] = {APHIP_FIELD_U, APHIP_FIELD_V, APHIP_FIELD_A, APHIP_FIELD_F};
+    for (int q = 0; q < 4; ++q) {
+        const int rc = aphip_get_field(c, vec_fields[q], tmp.data());
+        if (rc != APHIP_OK) return rc;
+        for (int64_t i = 0; i < nn; ++i)
+            for (int d = 0; d < 3; ++d) nodal[(3 * q + d) * nn + i] = tmp[3 * i + d];
+    }
+    if (c->has_mass) {  // one scalar per node, written as the reference's three equal components
+        const int rc = aphip_get_field(c, APHIP_FIELD_MASS, tmp.data());
+        if (rc != APHIP_OK) return rc;
+        for (int64_t i = 0; i < nn; ++i)
+            for (int d = 0; d < 3; ++d) nodal[(12 + d) * nn + i] = tmp[i];
+    }
+    if (c->has_fixed) {
+        std::vector<uint8_t> fx(nn);
+        c->fixed.download(fx.data(), nn, c->stream);
+        APHIP_HIP(hipStreamSynchronize(c->stream));
+        for (int64_t i = 0; i < nn; ++i)
+            for (int d = 0; d < 3; ++d) nodal[(15 + d) * nn + c->node_iperm[i]] = (fx[i] >> d) & 1u ? 1.0 : 0.0;
+    }
+    const int32_t ten_fields[2] = {APHIP_FIELD_H, APHIP_FIELD_P};
+    for (int q = 0; q < 2; ++q) {
+        const int rc = aphip_get_field(c, ten_fields[q], tmp.data());
+        if (rc != APHIP_OK) return rc;
+        for (int64_t e = 0; e < ne; ++e)
+            for (int k = 0; k < 9; ++k) elem[(9 * q + k) * ne + e] = tmp[9 * e + k];
+    }
+    {
+        const int rc = aphip_get_field(c, APHIP_FIELD_VOLUME, tmp.data());
+        if (rc != APHIP_OK) return rc;
+        std::copy(tmp.begin(), tmp.begin() + ne, elem.begin() + 18 * ne);
+    }
+    const bool state = c->state_count() == ne;
+    if (state) {
+        const int rc = aphip_get_field(c, APHIP_FIELD_STATE, tmp.data());
+        if (rc != APHIP_OK) return rc;
+        std::copy(tmp.begin(), tmp.begin() + ne, elem.begin() + 19 * ne);
+    }
+    std::vector<const double *> nvals, evals;
+    for (int q = 0; q < 18; ++q) nvals.push_back(nodal.data() + q * nn);
+    for (int q = 0; q < (state ? 20 : 19); ++q) evals.push_back(elem.data() + q * ne);
+    c->results->write_step(time, nvals, evals);
+    APHIP_API_END
+}
+
+int aphip_results_close(aphip_ctx *c) {
+    APHIP_API_BEGIN
+    APHIP_REQUIRE(c != nullptr, "NULL argument");
+    c->results.reset();
+    APHIP_API_END
+}
+
 }  // extern "C"

@@ -4,11 +4,13 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <vector>
 
 #include "aperi_hip.h"
+#include "exodus_format.hpp"
 
 namespace aphip {
 
@@ -236,6 +238,8 @@ struct Ctx {
     }
     bool has_mass = false;
     bool has_fixed = false;
+
+    std::unique_ptr<exo::Writer> results;  // open Exodus II results file, if any
 
     // power method: the eigenvector of the last call, the next call's start vector
     DevBuf<double> pm_eig;               // 3 nn, internal numbering

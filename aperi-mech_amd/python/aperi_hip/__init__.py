@@ -40,6 +40,7 @@ EXPORTED = [
     "aphip_adjoint_forward", "aphip_adjoint_backward", "aphip_rk_setup", "aphip_rk_get_neighbors", "aphip_rk_get_function_values",
     "aphip_rk_get_cells", "aphip_set_profiling", "aphip_get_profile", "aphip_reset_profile",
     "aphip_bandwidth_probe", "aphip_power_method_time_increment",
+    "aphip_results_open", "aphip_results_write", "aphip_results_close",
 ]
 
 
@@ -210,6 +211,21 @@ class Solver:
                                                         ctypes.c_uint64(seed), ctypes.byref(dt), ctypes.byref(it),
                                                         ctypes.byref(cv)))
         return dt.value, it.value, bool(cv.value)
+
+    # ---- Exodus II results file ----
+    def results_open(self, path, title=""):
+        """Create an Exodus II results file of this mesh (caller numbering)."""
+        _check(self.L.aphip_results_open(self.ctx, os.fsencode(os.fspath(path)), title.encode()))
+        return self
+
+    def results_write(self, time):
+        """Append the current fields as one time step."""
+        _check(self.L.aphip_results_write(self.ctx, ctypes.c_double(time)))
+        return self
+
+    def results_close(self):
+        _check(self.L.aphip_results_close(self.ctx))
+        return self
 
     # ---- hot path ----
     def internal_force(self, time=0.0, dt=0.0):

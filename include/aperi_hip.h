@@ -157,6 +157,22 @@ int aphip_power_method_time_increment(aphip_ctx *ctx, double previous_time_incre
                                       double variation_tolerance, uint64_t seed, double *time_increment,
                                       int32_t *iterations, int32_t *converged);
 
+/* Exodus II results file (netCDF 64-bit-offset format, written by this library with
+ * no netCDF dependency): replaces the reference's results output, IoMesh::
+ * CreateFieldResultsFile / AddFieldResultsOutput / WriteFieldResults / CloseFieldResults
+ * File (src/IoMesh.cpp:190-240) as driven by Solver::WriteOutput (src/Solvers/Base.cpp:
+ * 114-127).  One element block with the caller's node and element numbering
+ * (connectivity 1-based).  Node variables, with the reference's output names
+ * (src/Solvers/Explicit.cpp:47-53, src/Solvers/Base.cpp:79-80): displacement, velocity,
+ * acceleration, force, mass and essential_boundary, each as _x, _y, _z.  Element
+ * variables: displacement_gradient and pk1_stress as _xx, _xy, _xz, _yx, ..., _zz (the
+ * row-major components), volume, and, for J2 with one state per element,
+ * equivalent_plastic_strain.  aphip_results_write appends one time step (the current
+ * fields); a later aphip_results_open or aphip_destroy closes the file. */
+int aphip_results_open(aphip_ctx *ctx, const char *path, const char *title);
+int aphip_results_write(aphip_ctx *ctx, double time);
+int aphip_results_close(aphip_ctx *ctx);
+
 /* Element-tile shape of the fused step: elements per tile and workgroup size.
  * Tet4: one element per thread (64, 128, 256, 512 or 1024).  Hex8: one element per
  * thread (64, 128 or 256), or eight lanes per element, one per Gauss point

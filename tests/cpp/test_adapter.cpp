@@ -19,6 +19,8 @@
 #include <string>
 #include <vector>
 
+#include "exodus_format.hpp"
+
 extern "C" {
 #include "oracle.h"
 }
@@ -294,7 +296,10 @@ void GpuCases() {
         const int field[5] = {APHIP_FIELD_U, APHIP_FIELD_A, APHIP_FIELD_F, APHIP_FIELD_F, APHIP_FIELD_F};
         const int comp[5] = {2, 2, 0, 1, 2};
         size_t n_out = 0;
-        const double t_end = solver.Solve(stepper, output, [&](double, size_t) {
+        IoMesh io(elem);
+        io.CreateFieldResultsFile("/tmp/aphip_single_tet_results.exo", "single tet");
+        const double t_end = solver.Solve(stepper, output, [&](double time, size_t) {
+            io.WriteFieldResults(time);
             ++n_out;
             for (int q = 0; q < 5; ++q) {
                 const std::vector<double> x = elem->GetField(field[q]);
@@ -430,8 +435,43 @@ void GpuCases() {
 
 }  // namespace
 
+// A results file of two Hex8 (12 nodes) with three time steps of known values, for the
+// format test (tests/test_results_file.py reads it back with an independent netCDF reader).
+void WriteExodusSample(const char *path) {
+    std::vector<double> x(12), y(12), z(12);
+    for (int i = 0; i < 12; ++i) {
+        x[i] = i % 3 * 0.5;
+        y[i] = (i / 3) % 2 * 0.5;
+        z[i] = i / 6 * 0.5;
+    }
+    const std::vector<int32_t> conn = {0, 1, 4, 3, 6, 7, 10, 9, 1, 2, 5, 4, 7, 8, 11, 10};
+    aphip::exo::Mesh m;
+    m.title = "two hexes";
+    m.n_nodes = 12;
+    m.x = x.data();
+    m.y = y.data();
+    m.z = z.data();
+    m.n_elem = 2;
+    m.npe = 8;
+    m.conn = conn.data();
+    m.elem_type = "HEX8";
+    aphip::exo::Writer w(path, m, {"displacement_x", "force_z"}, {"pk1_stress_xy", "volume"});
+    for (int s = 0; s < 3; ++s) {
+        std::vector<double> a(12), b(12), p = {1.0 + s, -2.0 - s}, v = {0.125, 0.125};
+        for (int i = 0; i < 12; ++i) {
+            a[i] = 1e-3 * s * i;
+            b[i] = -0.5 * s + i;
+        }
+        w.write_step(0.25 * s, {a.data(), b.data()}, {p.data(), v.data()});
+    }
+}
+
 int main(int argc, char **argv) {
     const std::string mode = argc > 1 ? argv[1] : "cpu";
+    if (mode == "exodus" && argc > 2) {
+        WriteExodusSample(argv[2]);
+        return 0;
+    }
     if (mode == "cpu") {
         CpuCases();
     } else if (mode == "gpu") {
