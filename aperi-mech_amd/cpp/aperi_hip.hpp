@@ -417,6 +417,8 @@ class TimeIncrementScheduler {
         }
         return at_event;
     }
+    double GetTimeEnd() const { return m_time_end; }
+    double GetNextEventTime() const { return m_next; }
 
    private:
     double m_time_end, m_next, m_increment;
@@ -431,6 +433,7 @@ class StepScheduler {
         if (at_event) m_next += m_increment;
         return at_event;
     }
+    size_t GetNextEventTime() const { return m_next; }
 
    private:
     size_t m_next, m_increment;

@@ -361,6 +361,16 @@ struct PowerMethodResult {
 static PowerMethodResult power_method(Ctx &c, double prev_dt, int n_iter, double eps, double tol, double vtol,
                                       uint64_t seed) {
     const int64_t n3 = 3 * c.nn;
+    if (c.has_fixed) {  // PowerMethodProcessor::CheckEssentialBoundaries (:159-173)
+        std::vector<uint8_t> fx(c.nn);
+        c.fixed.download(fx.data(), fx.size(), c.stream);
+        APHIP_HIP(hipStreamSynchronize(c.stream));
+        bool all = true;
+        for (uint8_t m : fx) all = all && (m & 7u) == 7u;
+        APHIP_REQUIRE(!all,
+                      "All nodes are in the essential boundary set. Cannot run the power method. Use the "
+                      "direct_time_stepper instead.");
+    }
     DevBuf<double> u_in, f_in, f0, v, l, H, P, st, partial;
     DevBuf<int32_t> caller;
     u_in.alloc(n3);

@@ -16,6 +16,7 @@
 #include <cstdio>
 #include <cstring>
 #include <functional>
+#include <limits>
 #include <string>
 #include <vector>
 
@@ -141,7 +142,76 @@ std::shared_ptr<Material> MakeMaterial(MaterialType type, double E = 200e9, doub
 
 // ---- CPU cases -------------------------------------------------------------------
 
+// The reference's own scheduler and time-stepper tests, case for case
+// (test/unit_tests/SchedulerTest.cpp:33-92, test/unit_tests/TimeStepperTest.cpp:25-36).
+void SchedulerCases() {
+    Run("SchedulerTest.TimeIncrementSchedulerAtNextEvent", [] {
+        TimeIncrementScheduler s(0.0, 1.0, 0.1);
+        const double tol = 1.0e-15;
+        EXPECT_TRUE(s.GetTimeEnd() == 1.0);
+        EXPECT_TRUE(s.GetNextEventTime() == 0.0);
+        EXPECT_TRUE(!s.AtNextEvent(-0.1));
+        EXPECT_TRUE(std::fabs(s.GetNextEventTime() - 0.0) <= tol);
+        EXPECT_TRUE(s.AtNextEvent(0.0));
+        EXPECT_TRUE(std::fabs(s.GetNextEventTime() - 0.1) <= tol);
+        EXPECT_TRUE(!s.AtNextEvent(0.05));
+        EXPECT_TRUE(std::fabs(s.GetNextEventTime() - 0.1) <= tol);
+        EXPECT_TRUE(!s.AtNextEvent(0.09));
+        EXPECT_TRUE(std::fabs(s.GetNextEventTime() - 0.1) <= tol);
+        EXPECT_TRUE(s.AtNextEvent(0.1));
+        EXPECT_TRUE(std::fabs(s.GetNextEventTime() - 0.2) <= tol);
+        EXPECT_TRUE(!s.AtNextEvent(0.19));
+        EXPECT_TRUE(std::fabs(s.GetNextEventTime() - 0.2) <= tol);
+        EXPECT_TRUE(s.AtNextEvent(0.21));
+        EXPECT_TRUE(std::fabs(s.GetNextEventTime() - 0.3) <= tol);
+        EXPECT_TRUE(s.AtNextEvent(0.3));
+        EXPECT_TRUE(std::fabs(s.GetNextEventTime() - 0.4) <= tol);
+        EXPECT_TRUE(s.AtNextEvent(0.75));
+        EXPECT_TRUE(std::fabs(s.GetNextEventTime() - 0.8) <= tol);
+        EXPECT_TRUE(!s.AtNextEvent(0.76));
+        EXPECT_TRUE(std::fabs(s.GetNextEventTime() - 0.8) <= tol);
+        EXPECT_TRUE(s.AtNextEvent(0.99));
+        EXPECT_TRUE(std::fabs(s.GetNextEventTime() - 1.0) <= tol);
+        EXPECT_TRUE(s.AtNextEvent(1.0));
+        EXPECT_TRUE(std::fabs(s.GetNextEventTime() - std::numeric_limits<double>::max()) <= 1.0);
+        EXPECT_TRUE(!s.AtNextEvent(1.1));
+        EXPECT_TRUE(!s.AtNextEvent(10.1));
+    });
+    Run("SchedulerTest.StepSchedulerAtNextEvent", [] {
+        StepScheduler s(0, 5);
+        EXPECT_TRUE(s.GetNextEventTime() == 0);
+        EXPECT_TRUE(s.AtNextEvent(0));
+        EXPECT_TRUE(s.GetNextEventTime() == 5);
+        EXPECT_TRUE(!s.AtNextEvent(1));
+        EXPECT_TRUE(s.GetNextEventTime() == 5);
+        EXPECT_TRUE(!s.AtNextEvent(4));
+        EXPECT_TRUE(s.GetNextEventTime() == 5);
+        EXPECT_TRUE(s.AtNextEvent(5));
+        EXPECT_TRUE(s.GetNextEventTime() == 10);
+        EXPECT_TRUE(!s.AtNextEvent(6));
+        EXPECT_TRUE(s.GetNextEventTime() == 10);
+        EXPECT_TRUE(!s.AtNextEvent(9));
+        EXPECT_TRUE(s.GetNextEventTime() == 10);
+        EXPECT_TRUE(s.AtNextEvent(10));
+        EXPECT_TRUE(s.GetNextEventTime() == 15);
+    });
+    Run("TimeStepperTest.DirectTimeStepperTimeIncrement", [] {
+        DirectTimeStepper t(0.1, 1.0);
+        const double tol = 1.0e-15;
+        EXPECT_TRUE(t.GetTimeEnd() == 1.0);
+        EXPECT_TRUE(std::fabs(t.GetTimeIncrement(0.0, 0) - 0.1) <= tol);
+        EXPECT_TRUE(std::fabs(t.GetTimeIncrement(0.1, 0) - 0.1) <= tol);
+        EXPECT_TRUE(std::fabs(t.GetTimeIncrement(0.9, 0) - 0.1) <= tol);
+        EXPECT_TRUE(std::fabs(t.GetTimeIncrement(0.95, 0) - 0.05) <= tol);
+        EXPECT_TRUE(std::fabs(t.GetTimeIncrement(1.0, 0) - 0.0) <= tol);
+        EXPECT_TRUE(std::fabs(t.GetTimeIncrement(1.1, 0) - 0.0) <= tol);
+        PowerMethodTimeStepper p(0.1, 10, 1.0);
+        EXPECT_TRUE(p.GetTimeEnd() == 1.0);
+    });
+}
+
 void CpuCases() {
+    SchedulerCases();
     Run("LinearElasticPropertiesTest.AllValidPairs", [] {
         const double E = 200e9, nu = 0.3, G = E / (2 * (1 + nu)), L = E * nu / ((1 + nu) * (1 - 2 * nu)),
                      K = E / (3 * (1 - 2 * nu));
