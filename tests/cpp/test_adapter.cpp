@@ -389,6 +389,39 @@ void GpuCases() {
         }
     });
 
+    Run("SolverTest.ExplicitPowerMethod", [] {
+        // The reference's case (test/unit_tests/SolverTest.cpp:35-66): a 1x1x1|tets block
+        // of the fixture's elastic (SVK) material, density 1.21 x 7.15555e9, in free flight
+        // at v = 1.23 (1.4, -2.5, 2.9), power-method stepper (scale 0.9, update every 3
+        // steps) to t = 1: u = v t, v unchanged, a = 0.
+        std::vector<int32_t> hexes;
+        MeshBlock mesh = HexLattice(1, 0.0, 1, &hexes);
+        mesh.connectivity = SplitTets(mesh, hexes);
+        auto elem = CreateElement(ElementTopology::Tetrahedron4, ApproximationSpaceType::FiniteElement,
+                                  IntegrationSchemeType::GaussQuadrature, mesh, std::make_shared<Device>(),
+                                  MakeMaterial(ELASTIC, 2.1e11, 0.3, 1.21 * 7.15555e9));
+        const size_t n = mesh.NumNodes();
+        const double v0[3] = {1.23 * 1.4, 1.23 * -2.5, 1.23 * 2.9};
+        std::vector<double> v(3 * n);
+        for (size_t i = 0; i < n; ++i)
+            for (int d = 0; d < 3; ++d) v[3 * i + d] = v0[d];
+        elem->SetField(APHIP_FIELD_V, v);
+        ExplicitSolver solver(std::make_shared<InternalForceContribution>(elem));
+        PowerMethodTimeStepper stepper(0.9, 3, 1.0);
+        TimeIncrementScheduler output(0.0, 1.0, 1.0);
+        const double t = solver.Solve(stepper, output, [](double, size_t) {});
+        EXPECT_TRUE(t == 1.0);
+        EXPECT_TRUE(stepper.Updates().size() >= 2);
+        const std::vector<double> u = elem->GetField(APHIP_FIELD_U), vv = elem->GetField(APHIP_FIELD_V),
+                                  a = elem->GetField(APHIP_FIELD_A);
+        for (size_t i = 0; i < n; ++i)
+            for (int d = 0; d < 3; ++d) {
+                EXPECT_NEAR_REL(u[3 * i + d], v0[d] * 1.0, 1e-12);
+                EXPECT_NEAR_REL(vv[3 * i + d], v0[d], 1e-12);
+                EXPECT_TRUE(std::fabs(a[3 * i + d]) < 1e-9);
+            }
+    });
+
     Run("TimeStepperTest.PowerMethodStepperSingleTet", [] {
         // The input's own stepper (scale 0.1, update every 10 steps): updates at steps
         // 0, 10, ..., the first at the closed-form 2 / sqrt(V (lambda + 2 mu) / m) to the
