@@ -264,7 +264,8 @@ class ElementBase {
     }
     size_t FieldSize(int field) const {
         switch (field) {
-            case APHIP_FIELD_U: case APHIP_FIELD_V: case APHIP_FIELD_A: case APHIP_FIELD_F: return 3 * m_mesh_nodes;
+            case APHIP_FIELD_U: case APHIP_FIELD_V: case APHIP_FIELD_A: case APHIP_FIELD_F:
+            case APHIP_FIELD_RK_DISPLACEMENT: case APHIP_FIELD_RK_FORCE_LOCAL: return 3 * m_mesh_nodes;
             case APHIP_FIELD_MASS: return m_mesh_nodes;
             case APHIP_FIELD_H: case APHIP_FIELD_P: return 9 * m_elements;
             case APHIP_FIELD_VOLUME: return m_elements;
@@ -304,15 +305,18 @@ class ElementGaussianHip : public ElementBase {
     }
 };
 
-// ElementReproducingKernelTet4, one-pass element-as-cell strain smoothing
-// (include/ElementReproducingKernel.h:67-80, :214-224).
+// ElementReproducingKernelTet4, element-as-cell strain smoothing
+// (include/ElementReproducingKernel.h:67-80, :214-224), one-pass by default
+// (IntegrationSchemeParameters::uses_one_pass_method); false selects the two-pass
+// force (gather, cell force on the physical displacement, scatter).
 class ElementReproducingKernelHip : public ElementBase {
    public:
     ElementReproducingKernelHip(const MeshBlock &mesh, double kernel_radius_scale_factor,
-                                std::shared_ptr<Device> device, std::shared_ptr<Material> material)
+                                std::shared_ptr<Device> device, std::shared_ptr<Material> material,
+                                bool use_one_pass_method = true)
         : ElementBase(4, std::move(device), std::move(material)) {
         BindMesh(APHIP_TET4, mesh);
-        Check(aphip_rk_setup(Context(), kernel_radius_scale_factor));
+        Check(aphip_rk_setup_method(Context(), kernel_radius_scale_factor, use_one_pass_method ? 1 : 0));
     }
     void ComputeInternalForceAllElements(double time_increment, double time) override {
         Check(aphip_internal_force(Context(), time, time_increment));
@@ -333,14 +337,15 @@ class ElementReproducingKernelHip : public ElementBase {
 inline std::shared_ptr<ElementBase> CreateElement(ElementTopology topology, ApproximationSpaceType space,
                                                   IntegrationSchemeType scheme, const MeshBlock &mesh,
                                                   std::shared_ptr<Device> device, std::shared_ptr<Material> material,
-                                                  double kernel_radius_scale_factor = 0.0) {
+                                                  double kernel_radius_scale_factor = 0.0,
+                                                  bool use_one_pass_method = true) {
     if (space == ApproximationSpaceType::ReproducingKernel) {
         if (scheme != IntegrationSchemeType::StrainSmoothing)
             throw std::runtime_error("Gauss Quadrature is not supported for Reproducing Kernel");
         if (topology != ElementTopology::Tetrahedron4)
             throw std::runtime_error("Reproducing kernel cells on Hexahedron8 are not supported by the MI355X force step");
         return std::make_shared<ElementReproducingKernelHip>(mesh, kernel_radius_scale_factor, std::move(device),
-                                                             std::move(material));
+                                                             std::move(material), use_one_pass_method);
     }
     if (scheme == IntegrationSchemeType::StrainSmoothing) {
         if (topology == ElementTopology::Hexahedron8)

@@ -746,6 +746,14 @@ int aphip_get_field(aphip_ctx *c, int32_t field, double *out) {
         APHIP_HIP(hipStreamSynchronize(c->stream));
         for (int64_t e = 0; e < c->ne; ++e)
             for (int q = 0; q < per; ++q) out[(int64_t)c->elem_iperm[e] * per + q] = h[e * per + q];
+    } else if (field == APHIP_FIELD_RK_DISPLACEMENT || field == APHIP_FIELD_RK_FORCE_LOCAL) {
+        APHIP_REQUIRE(c->cells.ready && c->cells.two_pass, "field exists with the two-pass RK method only");
+        const DevBuf<double> &b = field == APHIP_FIELD_RK_DISPLACEMENT ? c->cells.u_phys : c->cells.f_local;
+        std::vector<double> h(3 * c->nn);
+        b.download(h.data(), h.size(), c->stream);
+        APHIP_HIP(hipStreamSynchronize(c->stream));
+        for (int64_t i = 0; i < c->nn; ++i)
+            for (int d = 0; d < 3; ++d) out[3 * (int64_t)c->node_iperm[i] + d] = h[3 * i + d];
     } else if (field == APHIP_FIELD_VOLUME) {
         const DevBuf<double> &b = c->cells.ready ? c->cells.volume : c->volume;
         std::vector<double> h(c->ne);
@@ -1039,7 +1047,9 @@ int aphip_synchronize(aphip_ctx *c) {
 }
 
 
-int aphip_rk_setup(aphip_ctx *c, double scale) {
+int aphip_rk_setup(aphip_ctx *c, double scale) { return aphip_rk_setup_method(c, scale, 1); }
+
+int aphip_rk_setup_method(aphip_ctx *c, double scale, int32_t use_one_pass_method) {
     APHIP_API_BEGIN
     APHIP_REQUIRE(c != nullptr, "ctx is NULL");
     require_ready(*c, false);
@@ -1048,7 +1058,7 @@ int aphip_rk_setup(aphip_ctx *c, double scale) {
     APHIP_HIP(hipSetDevice(c->device));
     APHIP_HIP(hipStreamSynchronize(c->stream));
     c->graph_reset();
-    rk_setup(*c, scale);
+    rk_setup(*c, scale, use_one_pass_method == 0);
     c->has_mass = false;
     APHIP_HIP(hipStreamSynchronize(c->stream));
     APHIP_API_END

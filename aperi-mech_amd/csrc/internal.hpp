@@ -116,8 +116,18 @@ struct CellData {
     DevBuf<int32_t> node_inc;   // unused (kept for layout stability)
     DevBuf<int32_t> entry_pos;  // [nnz] position of each entry in node order (by node, then ascending entry)
     DevBuf<double> g;           // [nnz * 3] per-entry force contributions of the last evaluation, node order
+    DevBuf<int64_t> rstart;     // [nn + 1] reverse neighbor lists: points i that list node J
+    DevBuf<int32_t> rsrc;       // [nrev] i, ascending per node
+    DevBuf<double> rphi;        // [nrev] phi_iJ
+    // two-pass method: cells list the element's own nodes; the cell force acts on
+    // u_phys = Phi d and f = Phi^T f_local
+    bool two_pass = false;
+    DevBuf<double> u_phys;      // [nn * 3] physical displacement of the last evaluation
+    DevBuf<double> f_local;     // [nn * 3] cell force before the scatter
     void reset() {
         ready = false;
+        two_pass = false;
+        rstart.release(); rsrc.release(); rphi.release(); u_phys.release(); f_local.release();
         nnz = n_truncated = 0;
         num_nbr.release(); nbr.release(); phi.release(); radius.release(); start.release();
         nodes.release(); dphi.release(); volume.release(); inc_cell.release(); node_start.release();
@@ -281,7 +291,7 @@ void force_vjp(Ctx &c, const double *f_bar, double *u_bar, double *E_bar, double
 void step_once_tiles(Ctx &c, double dt);
 
 // ---- RK (rk.hip) ----
-void rk_setup(Ctx &c, double scale);
+void rk_setup(Ctx &c, double scale, bool two_pass);
 void rk_cell_force(Ctx &c);
 void rk_node_update(Ctx &c, double dt, int mode);
 void rk_mass(Ctx &c, double *total);

@@ -14,11 +14,18 @@
 //   cell node lists     include/SmoothedCellDataProcessor.h:407-539 (first-seen union)
 //   cell derivatives    include/SmoothedCellDataProcessor.h:542-675, include/QuadratureSmoothed.h:32-62
 //   RK mass             src/MassUtils.cpp:79-112 via FunctionEvaluationProcessor::ScatterValues
+//   two-pass cells      include/SmoothedCellDataProcessor.h:474-486, :637-648 (the element's own
+//                       nodes, derivative (B_k V)/V)
 // Per step:
 //   cell kernel         include/ComputeInternalForceSmoothedCell.h:86-117, :186-224 (H = sum u (x) grad phi,
 //                       stress; the stress term -V P^T of :325-326 is kept per cell)
 //   node kernel         :328-349 as a node gather over the cell entries that name the node, in ascending
 //                       (cell, entry) order, fused with the central-difference node update
+// Per step, two-pass method (src/Solvers/Explicit.cpp:148-186):
+//   gather kernel       FunctionEvaluationProcessor::ComputeValues (include/FunctionEvaluationProcessor.h:
+//                       70-117): physical displacement = Phi d before the cell kernel
+//   scatter kernel      ScatterValues (:130-181) as a gather over reverse neighbor lists (points that
+//                       list the node, ascending), fused with the central-difference node update
 #include <algorithm>
 #include <cmath>
 #include <numeric>
@@ -455,6 +462,27 @@ __global__ __launch_bounds__(kCellBlock) void k_cell_fill(CellFillArgs A) {
     A.volume[e] = V;
 }
 
+// Two-pass cells: entry 4e + k is node k of element e with derivative (B_k V) / V.
+__global__ __launch_bounds__(256) void k_cell_two_pass(int64_t ne, const int32_t *conn, const double *X,
+                                                       int32_t *nodes, double *dphi, double *volume,
+                                                       int32_t *entry_cell) {
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= ne) return;
+    const int32_t en[4] = {conn[4 * e], conn[4 * e + 1], conn[4 * e + 2], conn[4 * e + 3]};
+    double X4[4][3], B[4][3], V;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) ld3(X, en[k], X4[k]);
+    smoothed_tet4(X4, B, V);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        nodes[4 * e + k] = en[k];
+        entry_cell[4 * e + k] = (int32_t)e;
+#pragma unroll
+        for (int m = 0; m < 3; ++m) dphi[3 * (4 * e + k) + m] = (B[k][m] * V) / V;
+    }
+    volume[e] = V;
+}
+
 __global__ __launch_bounds__(256) void k_iota(int64_t n, int32_t *v) {
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (i < n) v[i] = (int32_t)i;
@@ -514,45 +542,51 @@ __device__ __forceinline__ int64_t xcd_block(int64_t nblocks_padded) {
     return (b % kNumXcd) * per + b / kNumXcd;
 }
 
-__device__ __forceinline__ double group_sum16(double x) {
+template <int LANES>
+__device__ __forceinline__ double group_sum(double x) {
 #pragma unroll
-    for (int s = 8; s >= 1; s >>= 1) x += __shfl_xor(x, s, kCellLanes);
+    for (int s = LANES / 2; s >= 1; s >>= 1) x += __shfl_xor(x, s, LANES);
     return x;
 }
 
-template <int KIND>
+__device__ __forceinline__ double group_sum16(double x) { return group_sum<kCellLanes>(x); }
+
+// LANES lanes per cell: 16 for the one-pass cells (tens of entries), 4 for the
+// two-pass cells (the element's four nodes, one entry per lane).
+template <int KIND, int LANES>
 __global__ __launch_bounds__(256) void k_rk_cells(CellForceArgs A) {
 #pragma clang fp contract(off)
-    const int l = threadIdx.x & (kCellLanes - 1);
-    const int64_t c_raw = xcd_block(A.nblocks) * (256 / kCellLanes) + (threadIdx.x / kCellLanes);
+    constexpr int kCache = LANES == kCellLanes ? kCellCache : 1;
+    const int l = threadIdx.x & (LANES - 1);
+    const int64_t c_raw = xcd_block(A.nblocks) * (256 / LANES) + (threadIdx.x / LANES);
     const bool live = c_raw < A.ne;
     const int64_t c = live ? c_raw : A.ne - 1;  // idle groups recompute the last cell and store nothing
     const int64_t t0 = A.start[c], t1 = A.start[c + 1];
     const double mv = -A.volume[c];
     double H[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, P[9];
-    // The first kCellCache entries of the lane: every load issued before the first
+    // The first kCache entries of the lane: every load issued before the first
     // use (clamped indices, absent entries weighted by zero), grad phi kept for the
     // force pass.  Entries are accumulated in ascending order either way.
-    double gc[kCellCache][3] = {};
-    int32_t ps[kCellCache] = {};
+    double gc[kCache][3] = {};
+    int32_t ps[kCache] = {};
     if (t1 > t0) {  // (an empty cell has no entry to clamp to)
-        int32_t nd[kCellCache];
+        int32_t nd[kCache];
 #pragma unroll
-        for (int r = 0; r < kCellCache; ++r) {
-            const int64_t t = min(t0 + l + r * kCellLanes, t1 - 1);
+        for (int r = 0; r < kCache; ++r) {
+            const int64_t t = min(t0 + l + r * LANES, t1 - 1);
             nd[r] = A.nodes[t];
             ps[r] = A.pos[t];
         }
-        double uj[kCellCache][3];
+        double uj[kCache][3];
 #pragma unroll
-        for (int r = 0; r < kCellCache; ++r) {
-            const int64_t t = min(t0 + l + r * kCellLanes, t1 - 1);
+        for (int r = 0; r < kCache; ++r) {
+            const int64_t t = min(t0 + l + r * LANES, t1 - 1);
             ld3(A.u, nd[r], uj[r]);
             ld3(A.dphi, t, gc[r]);
         }
 #pragma unroll
-        for (int r = 0; r < kCellCache; ++r) {
-            const bool in = t0 + l + r * kCellLanes < t1;
+        for (int r = 0; r < kCache; ++r) {
+            const bool in = t0 + l + r * LANES < t1;
 #pragma unroll
             for (int b = 0; b < 3; ++b) gc[r][b] = in ? gc[r][b] : 0.0;
 #pragma unroll
@@ -561,7 +595,7 @@ __global__ __launch_bounds__(256) void k_rk_cells(CellForceArgs A) {
                 for (int b = 0; b < 3; ++b) H[3 * a + b] = __fma_rn(uj[r][a], gc[r][b], H[3 * a + b]);
         }
     }
-    for (int64_t t = t0 + l + kCellCache * kCellLanes; t < t1; t += kCellLanes) {
+    for (int64_t t = t0 + l + kCache * LANES; t < t1; t += LANES) {
         double uj[3], g[3];
         ld3(A.u, A.nodes[t], uj);
         ld3(A.dphi, t, g);
@@ -571,7 +605,7 @@ __global__ __launch_bounds__(256) void k_rk_cells(CellForceArgs A) {
             for (int b = 0; b < 3; ++b) H[3 * a + b] = __fma_rn(uj[a], g[b], H[3 * a + b]);
     }
 #pragma unroll
-    for (int i = 0; i < 9; ++i) H[i] = group_sum16(H[i]);
+    for (int i = 0; i < 9; ++i) H[i] = group_sum<LANES>(H[i]);
     MatParams m = A.mat;
     if (A.elem_lambda) {
         m.lambda = A.elem_lambda[c];
@@ -590,8 +624,8 @@ __global__ __launch_bounds__(256) void k_rk_cells(CellForceArgs A) {
     }
     if (live) {
 #pragma unroll
-        for (int r = 0; r < kCellCache; ++r) {
-            const int64_t t = t0 + l + r * kCellLanes;
+        for (int r = 0; r < kCache; ++r) {
+            const int64_t t = t0 + l + r * LANES;
             if (t < t1) {
                 double f[3];
 #pragma unroll
@@ -600,7 +634,7 @@ __global__ __launch_bounds__(256) void k_rk_cells(CellForceArgs A) {
                 st3(A.g, ps[r], f);
             }
         }
-        for (int64_t t = t0 + l + kCellCache * kCellLanes; t < t1; t += kCellLanes) {
+        for (int64_t t = t0 + l + kCache * LANES; t < t1; t += LANES) {
             double g[3], f[3];
             ld3(A.dphi, t, g);
 #pragma unroll
@@ -619,6 +653,30 @@ struct RkNodeArgs {
     double *f, *a, *v;
     double dt;
 };
+
+// Node i's force f is complete: store it and finish the node (MODE 0: force only,
+// 1: a = f/m, 2: central-difference corrector).
+template <int MODE>
+__device__ __forceinline__ void rk_finish(const RkNodeArgs &A, int64_t i, const double (&f)[3]) {
+    A.f[3 * i] = f[0];
+    A.f[3 * i + 1] = f[1];
+    A.f[3 * i + 2] = f[2];
+    if constexpr (MODE == 1) {
+        const double m = A.mass[i];
+        A.a[3 * i] = f[0] / m;
+        A.a[3 * i + 1] = f[1] / m;
+        A.a[3 * i + 2] = f[2] / m;
+    } else if constexpr (MODE == 2) {
+        double vh[3], a1[3], v1[3];
+        ld3(A.v, i, vh);
+        cd_correct(f, A.mass[i], A.fixed ? A.fixed[i] : 0u, A.dt, vh, a1, v1);
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            A.a[3 * i + d] = a1[d];
+            A.v[3 * i + d] = v1[d];
+        }
+    }
+}
 
 template <int MODE>
 __global__ __launch_bounds__(256) void k_rk_nodes(RkNodeArgs A) {
@@ -644,24 +702,98 @@ __global__ __launch_bounds__(256) void k_rk_nodes(RkNodeArgs A) {
 #pragma unroll
     for (int d = 0; d < 3; ++d) f[d] = group_sum16(f[d]);
     if (!live || l != 0) return;
-    A.f[3 * i] = f[0];
-    A.f[3 * i + 1] = f[1];
-    A.f[3 * i + 2] = f[2];
-    if constexpr (MODE == 1) {
-        const double m = A.mass[i];
-        A.a[3 * i] = f[0] / m;
-        A.a[3 * i + 1] = f[1] / m;
-        A.a[3 * i + 2] = f[2] / m;
-    } else if constexpr (MODE == 2) {
-        double vh[3], a1[3], v1[3];
-        ld3(A.v, i, vh);
-        cd_correct(f, A.mass[i], A.fixed ? A.fixed[i] : 0u, A.dt, vh, a1, v1);
+    rk_finish<MODE>(A, i, f);
+}
+
+// ---- two-pass method: gather before, scatter after the cell force -----------
+struct RkGatherArgs {
+    int64_t nn, nblocks;
+    const int32_t *num, *nbr;
+    const double *phi;
+    const double *src;  // coefficients [nn * 3]
+    double *dst;        // physical values [nn * 3]
+};
+
+// dst_i = sum_k phi_ik src_(nbr_ik): 16 lanes per node, lane l takes entries
+// l, l + 16, l + 32 of the node's row (contiguous across the lanes), the partial
+// sums are combined by the xor butterfly.  A node without neighbors copies src_i.
+__global__ __launch_bounds__(256) void k_rk_gather(RkGatherArgs A) {
+    constexpr int R = (kMaxNbr + kCellLanes - 1) / kCellLanes;
+    const int l = threadIdx.x & (kCellLanes - 1);
+    const int64_t i_raw = xcd_block(A.nblocks) * (256 / kCellLanes) + (threadIdx.x / kCellLanes);
+    const bool live = i_raw < A.nn;
+    const int64_t i = live ? i_raw : A.nn - 1;
+    const int n = A.num[i];
+    const int64_t row = i * kMaxNbr;
+    int32_t j[R];
+    double w[R];
 #pragma unroll
-        for (int d = 0; d < 3; ++d) {
-            A.a[3 * i + d] = a1[d];
-            A.v[3 * i + d] = v1[d];
-        }
+    for (int r = 0; r < R; ++r) {
+        const int k = l + r * kCellLanes;
+        const int kk = max(min(k, n - 1), 0);  // inside the row; entries past n are weighted by zero
+        const int32_t jl = A.nbr[row + kk];
+        const double wl = A.phi[row + kk];
+        j[r] = n > 0 ? jl : (int32_t)i;
+        w[r] = k < n ? wl : 0.0;
     }
+    double s[3] = {0.0, 0.0, 0.0};
+    double x[R][3];
+#pragma unroll
+    for (int r = 0; r < R; ++r) ld3(A.src, j[r], x[r]);
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) s[d] += x[r][d] * w[r];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) s[d] = group_sum16(s[d]);
+    if (!live || l != 0) return;
+    if (n == 0) ld3(A.src, i, s);
+    A.dst[3 * i] = s[0];
+    A.dst[3 * i + 1] = s[1];
+    A.dst[3 * i + 2] = s[2];
+}
+
+struct RkScatterArgs {
+    RkNodeArgs out;          // force and node update of the coefficients
+    const int32_t *num;
+    const int64_t *rstart;   // [nn + 1] reverse neighbor lists
+    const int32_t *rsrc;     // points i listing the node, ascending
+    const double *rphi;      // phi_iJ of each reverse entry
+    const double *src;       // local (physical) force [nn * 3]
+};
+
+// f_J = sum over points i that list J of phi_iJ src_i (the transpose of the
+// gather), 16 lanes per node over its contiguous reverse entries, kNodeBatch at a
+// time, then the node update.  A node without neighbors takes src_J.
+template <int MODE>
+__global__ __launch_bounds__(256) void k_rk_scatter(RkScatterArgs A) {
+    const int l = threadIdx.x & (kCellLanes - 1);
+    const int64_t i_raw = xcd_block(A.out.nblocks) * (256 / kCellLanes) + (threadIdx.x / kCellLanes);
+    const bool live = i_raw < A.out.nn;
+    const int64_t i = live ? i_raw : A.out.nn - 1;
+    double f[3] = {0.0, 0.0, 0.0};
+    const int64_t t0 = A.rstart[i], t1 = A.rstart[i + 1];
+    for (int64_t tb = t0 + l; tb < t1; tb += kNodeBatch * kCellLanes) {
+        int32_t src[kNodeBatch];
+        double w[kNodeBatch], x[kNodeBatch][3];
+#pragma unroll
+        for (int r = 0; r < kNodeBatch; ++r) {
+            const int64_t t = min(tb + r * kCellLanes, t1 - 1);
+            src[r] = A.rsrc[t];
+            w[r] = tb + r * kCellLanes < t1 ? A.rphi[t] : 0.0;
+        }
+#pragma unroll
+        for (int r = 0; r < kNodeBatch; ++r) ld3(A.src, src[r], x[r]);
+#pragma unroll
+        for (int r = 0; r < kNodeBatch; ++r)
+#pragma unroll
+            for (int d = 0; d < 3; ++d) f[d] += x[r][d] * w[r];
+    }
+#pragma unroll
+    for (int d = 0; d < 3; ++d) f[d] = group_sum16(f[d]);
+    if (!live || l != 0) return;
+    if (A.num[i] == 0) ld3(A.src, i, f);
+    rk_finish<MODE>(A.out, i, f);
 }
 
 // mass_from_elements: rho V / 4 from every cell touching the node, ascending cell
@@ -676,7 +808,7 @@ __global__ __launch_bounds__(256) void k_rk_mfe(int64_t nn, const int32_t *node_
 
 // m_J = sum over points i that list J of phi_iJ * mfe_i, ascending i (ScatterValues)
 __global__ __launch_bounds__(256) void k_rk_mass(int64_t nn, const int32_t *num, const int64_t *rstart,
-                                                 const int32_t *rval, const double *phi, const double *mfe,
+                                                 const int32_t *rsrc, const double *rphi, const double *mfe,
                                                  double *mass) {
     const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (j >= nn) return;
@@ -685,18 +817,15 @@ __global__ __launch_bounds__(256) void k_rk_mass(int64_t nn, const int32_t *num,
         return;
     }
     double m = 0.0;
-    for (int64_t t = rstart[j]; t < rstart[j + 1]; ++t) {
-        const int64_t slot = rval[t];
-        m += mfe[slot / kMaxNbr] * phi[slot];
-    }
+    for (int64_t t = rstart[j]; t < rstart[j + 1]; ++t) m += mfe[rsrc[t]] * rphi[t];
     mass[j] = m;
 }
 
 inline dim3 grid1(int64_t n, int block) { return dim3((unsigned)((n + block - 1) / block)); }
 
-// blocks of 256 / kCellLanes groups for n groups, padded to a multiple of the XCD count
-inline int64_t padded_groups(int64_t n) {
-    const int64_t per = 256 / kCellLanes, nb = (n + per - 1) / per;
+// blocks of 256 / lanes groups for n groups, padded to a multiple of the XCD count
+inline int64_t padded_groups(int64_t n, int lanes = kCellLanes) {
+    const int64_t per = 256 / lanes, nb = (n + per - 1) / per;
     return ((nb + kNumXcd - 1) / kNumXcd) * kNumXcd;
 }
 
@@ -715,7 +844,7 @@ void scan_counts(Ctx &c, const DevBuf<int32_t> &counts, int64_t n, DevBuf<T> &st
 
 }  // namespace
 
-void rk_setup(Ctx &c, double scale) {
+void rk_setup(Ctx &c, double scale, bool two_pass) {
     CellData &D = c.cells;
     D.reset();
     const int64_t nn = c.nn, ne = c.ne;
@@ -799,24 +928,76 @@ void rk_setup(Ctx &c, double scale) {
     APHIP_HIP(hipGetLastError());
 
     // smoothed cells: counts, offsets, lists and derivatives
-    DevBuf<int32_t> clen;
-    clen.alloc(std::max<int64_t>(ne, 1));
-    if (ne) {
-        hipLaunchKernelGGL(k_cell_count, grid1(ne, kCellBlock), dim3(kCellBlock), 0, c.stream, ne, c.conn.p,
-                           D.num_nbr.p, D.nbr.p, clen.p);
-        APHIP_HIP(hipGetLastError());
+    D.two_pass = two_pass;
+    if (two_pass) {
+        // the element's own nodes (SmoothedCellDataProcessor.h:474-486, :637-648)
+        D.nnz = 4 * ne;
+        APHIP_REQUIRE(D.nnz < (int64_t)INT32_MAX, "too many cell entries for int32 incidences");
+        std::vector<int64_t> hs(ne + 1);
+        for (int64_t e = 0; e <= ne; ++e) hs[e] = 4 * e;
+        D.start.upload(hs.data(), hs.size(), c.stream);
+        D.nodes.alloc(std::max<int64_t>(D.nnz, 1));
+        D.dphi.alloc(3 * std::max<int64_t>(D.nnz, 1));
+        D.volume.alloc(std::max<int64_t>(ne, 1));
+        D.inc_cell.alloc(std::max<int64_t>(D.nnz, 1));
+        if (ne) {
+            hipLaunchKernelGGL(k_cell_two_pass, grid1(ne, 256), dim3(256), 0, c.stream, ne, c.conn.p, c.X.p,
+                               D.nodes.p, D.dphi.p, D.volume.p, D.inc_cell.p);
+            APHIP_HIP(hipGetLastError());
+        }
+        D.u_phys.alloc(3 * nn);
+        D.u_phys.zero(c.stream);
+        D.f_local.alloc(3 * nn);
+        D.f_local.zero(c.stream);
+    } else {
+        DevBuf<int32_t> clen;
+        clen.alloc(std::max<int64_t>(ne, 1));
+        if (ne) {
+            hipLaunchKernelGGL(k_cell_count, grid1(ne, kCellBlock), dim3(kCellBlock), 0, c.stream, ne, c.conn.p,
+                               D.num_nbr.p, D.nbr.p, clen.p);
+            APHIP_HIP(hipGetLastError());
+        }
+        scan_counts(c, clen, ne, D.start, &D.nnz);
+        APHIP_REQUIRE(D.nnz < (int64_t)INT32_MAX, "too many cell entries for int32 incidences");
+        D.nodes.alloc(std::max<int64_t>(D.nnz, 1));
+        D.dphi.alloc(3 * std::max<int64_t>(D.nnz, 1));
+        D.volume.alloc(std::max<int64_t>(ne, 1));
+        D.inc_cell.alloc(std::max<int64_t>(D.nnz, 1));
+        if (ne) {
+            CellFillArgs F{ne, c.conn.p, c.X.p, D.num_nbr.p, D.nbr.p, D.phi.p, D.start.p, D.nodes.p, D.dphi.p,
+                           D.volume.p, D.inc_cell.p};
+            hipLaunchKernelGGL(k_cell_fill, grid1(ne, kCellBlock), dim3(kCellBlock), 0, c.stream, F);
+            APHIP_HIP(hipGetLastError());
+        }
     }
-    scan_counts(c, clen, ne, D.start, &D.nnz);
-    APHIP_REQUIRE(D.nnz < (int64_t)INT32_MAX, "too many cell entries for int32 incidences");
-    D.nodes.alloc(std::max<int64_t>(D.nnz, 1));
-    D.dphi.alloc(3 * std::max<int64_t>(D.nnz, 1));
-    D.volume.alloc(std::max<int64_t>(ne, 1));
-    D.inc_cell.alloc(std::max<int64_t>(D.nnz, 1));
-    if (ne) {
-        CellFillArgs F{ne, c.conn.p, c.X.p, D.num_nbr.p, D.nbr.p, D.phi.p, D.start.p, D.nodes.p, D.dphi.p,
-                       D.volume.p, D.inc_cell.p};
-        hipLaunchKernelGGL(k_cell_fill, grid1(ne, kCellBlock), dim3(kCellBlock), 0, c.stream, F);
-        APHIP_HIP(hipGetLastError());
+
+    // reverse neighbor lists: for node J the points i that list it, ascending i,
+    // with phi_iJ (the mass scatter and the two-pass force scatter)
+    {
+        std::vector<int32_t> hn(nn), hb(nn * kMaxNbr);
+        std::vector<double> hp(nn * kMaxNbr);
+        D.num_nbr.download(hn.data(), nn, c.stream);
+        D.nbr.download(hb.data(), hb.size(), c.stream);
+        D.phi.download(hp.data(), hp.size(), c.stream);
+        APHIP_HIP(hipStreamSynchronize(c.stream));
+        std::vector<int64_t> rstart(nn + 1, 0);
+        for (int64_t i = 0; i < nn; ++i)
+            for (int l = 0; l < hn[i]; ++l) rstart[hb[i * kMaxNbr + l] + 1]++;
+        for (int64_t j = 0; j < nn; ++j) rstart[j + 1] += rstart[j];
+        const int64_t nrev = std::max<int64_t>(rstart[nn], 1);
+        std::vector<int32_t> rsrc(nrev, 0);
+        std::vector<double> rphi(nrev, 0.0);
+        std::vector<int64_t> fill(rstart.begin(), rstart.end() - 1);
+        for (int64_t i = 0; i < nn; ++i)
+            for (int l = 0; l < hn[i]; ++l) {
+                const int64_t t = fill[hb[i * kMaxNbr + l]]++;
+                rsrc[t] = (int32_t)i;
+                rphi[t] = hp[i * kMaxNbr + l];
+            }
+        D.rstart.upload(rstart.data(), rstart.size(), c.stream);
+        D.rsrc.upload(rsrc.data(), rsrc.size(), c.stream);
+        D.rphi.upload(rphi.data(), rphi.size(), c.stream);
+        APHIP_HIP(hipStreamSynchronize(c.stream));
     }
 
     // node -> entries naming it, ascending entry (stable radix sort by node)
@@ -846,29 +1027,58 @@ void rk_setup(Ctx &c, double scale) {
 
 void rk_cell_force(Ctx &c) {
     CellData &D = c.cells;
+    if (D.two_pass) {  // physical displacement = Phi d (ExplicitSolver::ComputeForce, Explicit.cpp:148-152)
+        const int64_t nbg = padded_groups(c.nn);
+        RkGatherArgs G{c.nn, nbg, D.num_nbr.p, D.nbr.p, D.phi.p, c.u.p, D.u_phys.p};
+        hipLaunchKernelGGL(k_rk_gather, dim3((unsigned)nbg), dim3(256), 0, c.stream, G);
+        APHIP_HIP(hipGetLastError());
+    }
     if (c.ne == 0) return;
-    const int64_t nbc = padded_groups(c.ne);
-    CellForceArgs A{c.ne, nbc, D.start.p, D.nodes.p, D.dphi.p, D.volume.p, c.u.p, c.H.p, c.P.p, D.g.p, D.entry_pos.p,
+    const int lanes = D.two_pass ? 4 : kCellLanes;
+    const int64_t nbc = padded_groups(c.ne, lanes);
+    CellForceArgs A{c.ne, nbc, D.start.p, D.nodes.p, D.dphi.p, D.volume.p, D.two_pass ? D.u_phys.p : c.u.p,
+                    c.H.p, c.P.p, D.g.p, D.entry_pos.p,
                     c.state[1 - c.state_np1].p, c.state[c.state_np1].p,
                     MatParams{c.mat.lambda, c.mat.mu, c.mat.yield_stress, c.mat.hardening},
                     c.has_fields ? c.elem_lambda.p : nullptr, c.has_fields ? c.elem_mu.p : nullptr};
     const dim3 g((unsigned)nbc);
-    switch (c.mat.kind) {
-        case APHIP_LINEAR_ELASTIC: hipLaunchKernelGGL(k_rk_cells<APHIP_LINEAR_ELASTIC>, g, dim3(256), 0, c.stream, A); break;
-        case APHIP_NEO_HOOKEAN: hipLaunchKernelGGL(k_rk_cells<APHIP_NEO_HOOKEAN>, g, dim3(256), 0, c.stream, A); break;
-        case APHIP_J2: hipLaunchKernelGGL(k_rk_cells<APHIP_J2>, g, dim3(256), 0, c.stream, A); break;
-        case APHIP_SVK: hipLaunchKernelGGL(k_rk_cells<APHIP_SVK>, g, dim3(256), 0, c.stream, A); break;
-        default: throw Error(APHIP_ERR_INVALID, "material not set");
+#define APHIP_RK_CELLS(L)                                                                                      \
+    switch (c.mat.kind) {                                                                                      \
+        case APHIP_LINEAR_ELASTIC: hipLaunchKernelGGL((k_rk_cells<APHIP_LINEAR_ELASTIC, L>), g, dim3(256), 0, c.stream, A); break; \
+        case APHIP_NEO_HOOKEAN: hipLaunchKernelGGL((k_rk_cells<APHIP_NEO_HOOKEAN, L>), g, dim3(256), 0, c.stream, A); break;       \
+        case APHIP_J2: hipLaunchKernelGGL((k_rk_cells<APHIP_J2, L>), g, dim3(256), 0, c.stream, A); break;                         \
+        case APHIP_SVK: hipLaunchKernelGGL((k_rk_cells<APHIP_SVK, L>), g, dim3(256), 0, c.stream, A); break;                       \
+        default: throw Error(APHIP_ERR_INVALID, "material not set");                                           \
     }
+    if (D.two_pass) {
+        APHIP_RK_CELLS(4)
+    } else {
+        APHIP_RK_CELLS(kCellLanes)
+    }
+#undef APHIP_RK_CELLS
     APHIP_HIP(hipGetLastError());
 }
 
 void rk_node_update(Ctx &c, double dt, int mode) {
     CellData &D = c.cells;
     const int64_t nbn = padded_groups(c.nn);
+    const dim3 g((unsigned)nbn);
     RkNodeArgs A{c.nn, nbn, D.node_start.p, D.g.p, c.mass.p,
                  c.has_fixed ? c.fixed.p : nullptr, c.f.p, c.a.p, c.v.p, dt};
-    const dim3 g((unsigned)nbn);
+    if (D.two_pass) {
+        // cell entries -> local force, then f = Phi^T f_local with the node update
+        // (ScatterValues, Explicit.cpp:181-186)
+        RkNodeArgs L = A;
+        L.f = D.f_local.p;
+        hipLaunchKernelGGL(k_rk_nodes<0>, g, dim3(256), 0, c.stream, L);
+        APHIP_HIP(hipGetLastError());
+        RkScatterArgs S{A, D.num_nbr.p, D.rstart.p, D.rsrc.p, D.rphi.p, D.f_local.p};
+        if (mode == 0) hipLaunchKernelGGL(k_rk_scatter<0>, g, dim3(256), 0, c.stream, S);
+        else if (mode == 1) hipLaunchKernelGGL(k_rk_scatter<1>, g, dim3(256), 0, c.stream, S);
+        else hipLaunchKernelGGL(k_rk_scatter<2>, g, dim3(256), 0, c.stream, S);
+        APHIP_HIP(hipGetLastError());
+        return;
+    }
     if (mode == 0) hipLaunchKernelGGL(k_rk_nodes<0>, g, dim3(256), 0, c.stream, A);
     else if (mode == 1) hipLaunchKernelGGL(k_rk_nodes<1>, g, dim3(256), 0, c.stream, A);
     else hipLaunchKernelGGL(k_rk_nodes<2>, g, dim3(256), 0, c.stream, A);
@@ -883,26 +1093,9 @@ void rk_mass(Ctx &c, double *total) {
     hipLaunchKernelGGL(k_rk_mfe, grid1(nn, 256), dim3(256), 0, c.stream, nn, c.node_start.p, c.node_inc.p,
                        D.volume.p, c.mat.density, mfe.p);
     APHIP_HIP(hipGetLastError());
-    // reverse neighbor lists: (neighbor J, slot i*40+l) sorted stably by J
-    std::vector<int32_t> hn(nn), hb(nn * kMaxNbr);
-    D.num_nbr.download(hn.data(), nn, c.stream);
-    D.nbr.download(hb.data(), hb.size(), c.stream);
-    APHIP_HIP(hipStreamSynchronize(c.stream));
-    std::vector<int64_t> rstart(nn + 1, 0);
-    for (int64_t i = 0; i < nn; ++i)
-        for (int l = 0; l < hn[i]; ++l) rstart[hb[i * kMaxNbr + l] + 1]++;
-    for (int64_t j = 0; j < nn; ++j) rstart[j + 1] += rstart[j];
-    std::vector<int32_t> rval(std::max<int64_t>(rstart[nn], 1));
-    std::vector<int64_t> fill(rstart.begin(), rstart.end() - 1);
-    for (int64_t i = 0; i < nn; ++i)
-        for (int l = 0; l < hn[i]; ++l) rval[fill[hb[i * kMaxNbr + l]]++] = (int32_t)(i * kMaxNbr + l);
-    DevBuf<int64_t> drs;
-    DevBuf<int32_t> drv;
-    drs.upload(rstart.data(), rstart.size(), c.stream);
-    drv.upload(rval.data(), rval.size(), c.stream);
     c.mass.alloc(nn);
-    hipLaunchKernelGGL(k_rk_mass, grid1(nn, 256), dim3(256), 0, c.stream, nn, D.num_nbr.p, drs.p, drv.p, D.phi.p,
-                       mfe.p, c.mass.p);
+    hipLaunchKernelGGL(k_rk_mass, grid1(nn, 256), dim3(256), 0, c.stream, nn, D.num_nbr.p, D.rstart.p, D.rsrc.p,
+                       D.rphi.p, mfe.p, c.mass.p);
     APHIP_HIP(hipGetLastError());
     std::vector<double> hm(nn);
     c.mass.download(hm.data(), nn, c.stream);

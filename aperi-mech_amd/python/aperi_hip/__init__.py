@@ -22,6 +22,7 @@ LIB_PATH = os.path.normpath(os.path.join(_PKG, "..", "..", "lib", "libaperi_hip.
 TET4, HEX8 = 0, 1
 LINEAR_ELASTIC, NEO_HOOKEAN, J2, SVK = 0, 1, 2, 3
 FIELD_U, FIELD_V, FIELD_A, FIELD_F, FIELD_MASS, FIELD_H, FIELD_P, FIELD_STATE, FIELD_VOLUME = range(9)
+FIELD_RK_DISPLACEMENT, FIELD_RK_FORCE_LOCAL = 9, 10
 MAX_NODE_NUM_NEIGHBORS = 40
 FLAG_TWO_PASS = 1
 FLAG_PIPELINED = 2
@@ -37,7 +38,7 @@ EXPORTED = [
     "aphip_set_field", "aphip_get_field", "aphip_internal_force", "aphip_initialize", "aphip_step",
     "aphip_synchronize", "aphip_set_tiling", "aphip_set_graph", "aphip_autotune", "aphip_batch_begin",
     "aphip_batch_add", "aphip_batch_end", "aphip_batch_ranges", "aphip_internal_force_vjp",
-    "aphip_adjoint_forward", "aphip_adjoint_backward", "aphip_rk_setup", "aphip_rk_get_neighbors", "aphip_rk_get_function_values",
+    "aphip_adjoint_forward", "aphip_adjoint_backward", "aphip_rk_setup", "aphip_rk_setup_method", "aphip_rk_get_neighbors", "aphip_rk_get_function_values",
     "aphip_rk_get_cells", "aphip_set_profiling", "aphip_get_profile", "aphip_reset_profile",
     "aphip_bandwidth_probe", "aphip_power_method_time_increment",
     "aphip_results_open", "aphip_results_write", "aphip_results_close",
@@ -184,7 +185,7 @@ class Solver:
         return self
 
     def _shape(self, field):
-        if field in (FIELD_U, FIELD_V, FIELD_A, FIELD_F):
+        if field in (FIELD_U, FIELD_V, FIELD_A, FIELD_F, FIELD_RK_DISPLACEMENT, FIELD_RK_FORCE_LOCAL):
             return (self.nn, 3)
         if field == FIELD_MASS:
             return (self.nn,)
@@ -308,8 +309,10 @@ class Solver:
         return a.value, b.value, c.value, d.value
 
     # ---- RKPM ----
-    def rk_setup(self, kernel_radius_scale):
-        _check(self.L.aphip_rk_setup(self.ctx, ctypes.c_double(kernel_radius_scale)))
+    def rk_setup(self, kernel_radius_scale, one_pass=True):
+        """RK functions and smoothed cells; one_pass=False selects the two-pass force
+        (physical displacement gathered before, force scattered after the cells)."""
+        _check(self.L.aphip_rk_setup_method(self.ctx, ctypes.c_double(kernel_radius_scale), 1 if one_pass else 0))
         return self
 
     def rk_neighbors(self):

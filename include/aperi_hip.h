@@ -56,7 +56,12 @@ enum {
     APHIP_FIELD_H = 5,
     APHIP_FIELD_P = 6,
     APHIP_FIELD_STATE = 7,
-    APHIP_FIELD_VOLUME = 8
+    APHIP_FIELD_VOLUME = 8,
+    /* two-pass RK method only, read-only: the physical displacement Phi d of the last
+     * force evaluation ("displacement", src/Solvers/Base.cpp:58-63) and the cell force
+     * before its scatter ("force", include/Solvers/Explicit.h:63) */
+    APHIP_FIELD_RK_DISPLACEMENT = 9,
+    APHIP_FIELD_RK_FORCE_LOCAL = 10
 };
 
 /* aphip_config.flags */
@@ -208,6 +213,17 @@ int aphip_synchronize(aphip_ctx *ctx);
  * SmoothedCellDataProcessor::SetSubcellNodeIndices / ComputeFunctionDerivatives
  * (include/SmoothedCellDataProcessor.h:407-675).  Requires a Tet4 mesh. */
 int aphip_rk_setup(aphip_ctx *ctx, double kernel_radius_scale);
+/* aphip_rk_setup with the reference's force-method switch
+ * (IntegrationSchemeStrainSmoothingParameters, include/InternalForceContributionParameters.h:
+ * 115-117, "force_two_pass_method"): use_one_pass_method = 1 is aphip_rk_setup;
+ * 0 selects the two-pass method of ExplicitSolver::ComputeForce (src/Solvers/Explicit.cpp:
+ * 148-186): each cell lists its element's own nodes with derivative (B_k V)/V
+ * (include/SmoothedCellDataProcessor.h:474-486, :637-648), every force evaluation
+ * gathers the physical displacement Phi d (FunctionEvaluationProcessor::ComputeValues,
+ * include/FunctionEvaluationProcessor.h:70-117), computes the cell force on it and
+ * scatters it back, f = Phi^T f_local (ScatterValues, :130-181).  Both methods give
+ * the same force up to rounding. */
+int aphip_rk_setup_method(aphip_ctx *ctx, double kernel_radius_scale, int32_t use_one_pass_method);
 /* Neighbor lists in caller numbering: num[n_nodes], nbr[n_nodes x 40] (-1 padded),
  * the 40-cap truncation count (NeighborSearchProcessor.h:307-316). */
 int aphip_rk_get_neighbors(aphip_ctx *ctx, int32_t *num, int32_t *nbr, int64_t *n_truncated);

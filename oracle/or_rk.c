@@ -19,6 +19,12 @@
  *   cell force          include/ComputeInternalForceSmoothedCell.h:86-117, :186-224, :312-351, :416-442
  *   RK mass             src/MassUtils.cpp:79-112 with FunctionEvaluationProcessor::ScatterValues
  *                       (include/FunctionEvaluationProcessor.h:130-181)
+ *   two-pass method     include/SmoothedCellDataProcessor.h:474-486, :637-648 (cells list the
+ *                       element's own nodes, derivative (B_k V)/V), FunctionEvaluationProcessor::
+ *                       ComputeValues (include/FunctionEvaluationProcessor.h:70-117, neighbors
+ *                       taken in descending order) and ScatterValues (:130-181), called around
+ *                       the internal force by ExplicitSolver::ComputeForce (src/Solvers/
+ *                       Explicit.cpp:148-186)
  *
  * Pinning (tests/test_oracle_rk.py): neighbor counts of NeighborSearchProcessorTest.cpp:68-207
  * (radii 0.5, 1.01, 1.01*diagonal, variable radius 1.1 and 1.01 incl. the sqrt2/sqrt3 radius
@@ -26,6 +32,10 @@
  * unity and linear reproduction sum phi_j x_j = x_i (ShapeFunctionsFunctorsTest.cpp:182-272),
  * tet-only neighbors reproduce the FE values; cell derivatives: partition of nullity and a
  * linear field u = (F-I)X giving H = F-I exactly (KinematicsTestFixture.h:338-357).
+ * Two-pass: gather of a linear coefficient field d = G X + c returns G X + c at every node
+ * (linear reproduction) and the cells then give H = G exactly; the two-pass force Phi^T f(Phi d)
+ * equals the one-pass force of the same d (the one-pass cell derivative is the same sum taken
+ * in the other order), which fails for a transposed Phi in either the gather or the scatter.
  * Inclusion ties on the sphere boundary follow the third-party STK intersection test and
  * are parity unpinned (jittered inputs make them measure-zero).
  */
@@ -317,5 +327,59 @@ int or_rk_mass(int64_t n_nodes, int64_t n_elem, const int32_t *conn, const doubl
         for (int k = 0; k < n; ++k) mass[neighbors[i * max_neighbors + k]] += mfe[i] * phi[i * max_neighbors + k];
     }
     free(mfe);
+    return 0;
+}
+
+int or_rk_cells_two_pass(int64_t n_elem, const double *X, const int32_t *conn, int64_t *cell_start,
+                         int32_t *cell_nodes, double *cell_dphi, double *cell_volume) {
+    cell_start[0] = 0;
+    for (int64_t e = 0; e < n_elem; ++e) {
+        double Xe[12], B[12], V;
+        for (int k = 0; k < 4; ++k)
+            for (int d = 0; d < 3; ++d) Xe[3 * k + d] = X[3 * (int64_t)conn[4 * e + k] + d];
+        or_smoothed_tet4_bmatrix(Xe, B, &V);
+        for (int k = 0; k < 4; ++k) {
+            cell_nodes[4 * e + k] = conn[4 * e + k];
+            for (int m = 0; m < 3; ++m) cell_dphi[12 * e + 3 * k + m] = (B[3 * k + m] * V) / V;
+        }
+        cell_volume[e] = V;
+        cell_start[e + 1] = 4 * (e + 1);
+    }
+    return 0;
+}
+
+int or_rk_gather(int64_t n_nodes, int max_neighbors, const int32_t *num_neighbors, const int32_t *neighbors,
+                 const double *phi, const double *src, double *dst) {
+    for (int64_t i = 0; i < n_nodes; ++i) {
+        const int n = num_neighbors[i];
+        if (n == 0) {
+            for (int d = 0; d < 3; ++d) dst[3 * i + d] = src[3 * i + d];
+            continue;
+        }
+        double s[3] = {0.0, 0.0, 0.0};
+        for (int k = n - 1; k >= 0; --k) {
+            const int64_t j = neighbors[i * max_neighbors + k];
+            const double w = phi[i * max_neighbors + k];
+            for (int d = 0; d < 3; ++d) s[d] += src[3 * j + d] * w;
+        }
+        for (int d = 0; d < 3; ++d) dst[3 * i + d] = s[d];
+    }
+    return 0;
+}
+
+int or_rk_scatter(int64_t n_nodes, int max_neighbors, const int32_t *num_neighbors, const int32_t *neighbors,
+                  const double *phi, const double *src, double *dst) {
+    memset(dst, 0, sizeof(double) * 3 * n_nodes);
+    for (int64_t i = 0; i < n_nodes; ++i) {
+        const int n = num_neighbors[i];
+        for (int k = 0; k < n; ++k) {
+            const int64_t j = neighbors[i * max_neighbors + k];
+            const double w = phi[i * max_neighbors + k];
+            for (int d = 0; d < 3; ++d) dst[3 * j + d] += src[3 * i + d] * w;
+        }
+    }
+    for (int64_t i = 0; i < n_nodes; ++i)
+        if (num_neighbors[i] == 0)
+            for (int d = 0; d < 3; ++d) dst[3 * i + d] = src[3 * i + d];
     return 0;
 }

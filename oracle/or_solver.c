@@ -40,6 +40,19 @@ int or_lumped_mass(int etype, int64_t n_nodes, const double *X, int64_t n_elem, 
 
 static int compute_force(const or_problem *p, const double *u, const double *sn, double *snp1, double *H,
                          double *P, double *f, int nthreads) {
+    if (p->etype == -1 && p->rk_two_pass) {  /* ExplicitSolver::ComputeForce (Explicit.cpp:148-186) */
+        double *up = (double *)malloc(sizeof(double) * 3 * (size_t)(p->n_nodes > 0 ? p->n_nodes : 1));
+        double *fl = (double *)malloc(sizeof(double) * 3 * (size_t)(p->n_nodes > 0 ? p->n_nodes : 1));
+        int rc = (up && fl) ? 0 : -1;
+        if (!rc) rc = or_rk_gather(p->n_nodes, p->max_neighbors, p->num_neighbors, p->neighbors, p->phi, u, up);
+        if (!rc)
+            rc = or_cell_internal_force(p->n_nodes, p->n_elem, p->cell_start, p->cell_nodes, p->cell_dphi,
+                                        p->cell_volume, &p->mat, up, sn, snp1, H, P, fl, nthreads);
+        if (!rc) rc = or_rk_scatter(p->n_nodes, p->max_neighbors, p->num_neighbors, p->neighbors, p->phi, fl, f);
+        free(up);
+        free(fl);
+        return rc;
+    }
     if (p->etype == -1)
         return or_cell_internal_force(p->n_nodes, p->n_elem, p->cell_start, p->cell_nodes, p->cell_dphi,
                                       p->cell_volume, &p->mat, u, sn, snp1, H, P, f, nthreads);

@@ -84,6 +84,13 @@ typedef struct {
     const int32_t *cell_nodes;     /* [nnz] */
     const double *cell_dphi;       /* [nnz*3] */
     const double *cell_volume;     /* [n_elem] */
+    /* two-pass RK force (etype == -1, rk_two_pass = 1): the cells list the element's
+     * own nodes; u_phys = Phi d before the cell force, f = Phi^T f_local after it */
+    int32_t rk_two_pass;
+    int32_t max_neighbors;
+    const int32_t *num_neighbors;  /* [n_nodes] */
+    const int32_t *neighbors;      /* [n_nodes*max_neighbors] */
+    const double *phi;             /* [n_nodes*max_neighbors] */
 } or_problem;
 
 /* Initial force and acceleration (Explicit.cpp:363-376), then n_steps of the
@@ -126,6 +133,16 @@ int or_cell_internal_force(int64_t n_nodes, int64_t n_cells, const int64_t *cell
 int or_rk_mass(int64_t n_nodes, int64_t n_elem, const int32_t *conn, const double *elem_volume, double density,
                int max_neighbors, const int32_t *num_neighbors, const int32_t *neighbors, const double *phi,
                double *mass);
+/* Two-pass method.  Cells of the element's own nodes: start[e] = 4e, grad N_k = (B_k V) / V. */
+int or_rk_cells_two_pass(int64_t n_elem, const double *X, const int32_t *conn, int64_t *cell_start,
+                         int32_t *cell_nodes, double *cell_dphi, double *cell_volume);
+/* dst_i = sum_k phi_ik src_(nbr_ik), k descending; src_i for a node without neighbors (3 components). */
+int or_rk_gather(int64_t n_nodes, int max_neighbors, const int32_t *num_neighbors, const int32_t *neighbors,
+                 const double *phi, const double *src, double *dst);
+/* dst (zeroed here) gets phi_ik src_i added at every neighbor nbr_ik, ascending i then k; a node
+ * without neighbors takes src_i alone (3 components). */
+int or_rk_scatter(int64_t n_nodes, int max_neighbors, const int32_t *num_neighbors, const int32_t *neighbors,
+                  const double *phi, const double *src, double *dst);
 
 #ifdef __cplusplus
 }

@@ -35,7 +35,9 @@ class Problem(ctypes.Structure):
     _fields_ = [("etype", ctypes.c_int32), ("n_nodes", ctypes.c_int64), ("X", _d),
                 ("n_elem", ctypes.c_int64), ("conn", _i32), ("mat", Material),
                 ("mass", _d), ("fixed", _u8), ("node_dt", _d),
-                ("cell_start", _i64), ("cell_nodes", _i32), ("cell_dphi", _d), ("cell_volume", _d)]
+                ("cell_start", _i64), ("cell_nodes", _i32), ("cell_dphi", _d), ("cell_volume", _d),
+                ("rk_two_pass", ctypes.c_int32), ("max_neighbors", ctypes.c_int32), ("num_neighbors", _i32),
+                ("neighbors", _i32), ("phi", _d)]
 
 
 def build():
@@ -203,7 +205,9 @@ class Solver:
     """The explicit loop of src/Solvers/Explicit.cpp:308-452 on the CPU oracle."""
 
     def __init__(self, etype, X, conn, kind, lam, mu, mass, fixed=None, yield_stress=0.0, hardening=0.0,
-                 elem_lambda=None, elem_mu=None, node_dt=None, state_per_qp=0, cells=None):
+                 elem_lambda=None, elem_mu=None, node_dt=None, state_per_qp=0, cells=None, two_pass=None):
+        """cells: RK smoothed cells (start, nodes, dphi, volume); two_pass: (num, nbr, phi)
+        of the RK functions, with `cells` from rk_cells_two_pass, for the two-pass force."""
         self.X = _f64(X)
         self.conn = None if conn is None else _i32a(conn)
         self.mass = _f64(mass)
@@ -224,6 +228,11 @@ class Solver:
             p.cell_start, p.cell_nodes = _p(self.cs, _i64), _p(self.cn, _i32)
             p.cell_dphi, p.cell_volume = _p(self.cd), _p(self.cv)
             p.conn = None
+            if two_pass is not None:
+                self.tp = (_i32a(two_pass[0]), _i32a(two_pass[1]), _f64(two_pass[2]))
+                p.rk_two_pass = 1
+                p.max_neighbors = self.tp[1].shape[1]
+                p.num_neighbors, p.neighbors, p.phi = _p(self.tp[0], _i32), _p(self.tp[1], _i32), _p(self.tp[2])
         else:
             p.n_elem = self.conn.shape[0]
             p.conn = _p(self.conn, _i32)
@@ -350,6 +359,42 @@ def cell_internal_force(n_nodes, cells, kind, lam, mu, u, state_n=None, yield_st
                                  _p(_i32a(nodes), _i32), _p(_f64(dphi)), _p(_f64(vol)), ctypes.byref(mat),
                                  _p(_f64(u)), _p(sn), _p(snp1), _p(H), _p(P), _p(f), nthreads)
     return f, H, P, snp1
+
+
+def rk_cells_two_pass(X, conn):
+    """Cells of the two-pass method: each element's own nodes and (B_k V)/V."""
+    X, conn = _f64(X), _i32a(conn)
+    ne = conn.shape[0]
+    start = np.zeros(ne + 1, dtype=np.int64)
+    nodes = np.zeros(4 * ne, dtype=np.int32)
+    dphi = np.zeros((4 * ne, 3))
+    vol = np.zeros(ne)
+    lib().or_rk_cells_two_pass(ne, _p(X), _p(conn, _i32), _p(start, _i64), _p(nodes, _i32), _p(dphi), _p(vol))
+    return start, nodes, dphi, vol
+
+
+def rk_gather(num, nbr, phi, src):
+    """Physical values from coefficients: sum_k phi_ik src_(nbr_ik)."""
+    num, nbr, phi, src = _i32a(num), _i32a(nbr), _f64(phi), _f64(src)
+    out = np.zeros_like(src)
+    lib().or_rk_gather(nbr.shape[0], nbr.shape[1], _p(num, _i32), _p(nbr, _i32), _p(phi), _p(src), _p(out))
+    return out
+
+
+def rk_scatter(num, nbr, phi, src):
+    """Coefficient values from physical ones: the transpose of rk_gather."""
+    num, nbr, phi, src = _i32a(num), _i32a(nbr), _f64(phi), _f64(src)
+    out = np.zeros_like(src)
+    lib().or_rk_scatter(nbr.shape[0], nbr.shape[1], _p(num, _i32), _p(nbr, _i32), _p(phi), _p(src), _p(out))
+    return out
+
+
+def rk_two_pass_force(n_nodes, cells2, num, nbr, phi, kind, lam, mu, d, state_n=None, yield_stress=0.0,
+                      hardening=0.0):
+    """Two-pass internal force of coefficient displacements d: (f, H, P, state, u_phys, f_local)."""
+    up = rk_gather(num, nbr, phi, d)
+    fl, H, P, s = cell_internal_force(n_nodes, cells2, kind, lam, mu, up, state_n, yield_stress, hardening)
+    return rk_scatter(num, nbr, phi, fl), H, P, s, up, fl
 
 
 def rk_mass(conn, elem_volume, density, num, nbr, phi):

@@ -534,6 +534,41 @@ void GpuCases() {
             for (int k = 0; k < rc[i]; ++k) same = same && nbr[40 * i + k] == rn[40 * i + k];
         EXPECT_TRUE(same);
     });
+
+    Run("CreateElementStrainSmoothedTest.ReproducingKernelOnTet4TwoPass", [] {
+        // ElementTest.cpp:154-162: kernel radius scale 0.03 ("small so it is like a tet4"),
+        // the two-pass method; every node is its own only neighbor, so the force, H and P
+        // are those of the Tet4 element (C oracle) and the physical displacement is u
+        std::vector<int32_t> hexes;
+        MeshBlock mesh = HexLattice(4, 0.2, 9, &hexes);
+        mesh.connectivity = SplitTets(mesh, hexes);
+        auto mat = MakeMaterial(NEO_HOOKEAN);
+        auto elem = CreateElement(ElementTopology::Tetrahedron4, ApproximationSpaceType::ReproducingKernel,
+                                  IntegrationSchemeType::StrainSmoothing, mesh, std::make_shared<Device>(), mat, 0.03,
+                                  /*use_one_pass_method=*/false);
+        const size_t n = mesh.NumNodes(), ne = elem->NumElements();
+        std::vector<double> u(3 * n);
+        for (size_t i = 0; i < 3 * n; ++i) u[i] = 1e-2 * (2.0 * Uniform(13, i) - 1.0);
+        elem->SetField(APHIP_FIELD_U, u);
+        InternalForceContribution(elem).ComputeForce(0.0, 1e-7);
+        const LinearElasticProperties &le = mat->GetMaterialProperties()->linear_elastic_properties;
+        or_material om{OR_NEO_HOOKEAN, 0, le.lambda, le.shear_modulus, 0.0, 0.0, nullptr, nullptr};
+        std::vector<double> f(3 * n), H(9 * ne), P(9 * ne);
+        or_internal_force(OR_TET4, (int64_t)n, mesh.coordinates.data(), (int64_t)ne, mesh.connectivity.data(), &om,
+                          u.data(), nullptr, nullptr, H.data(), P.data(), f.data(), 1);
+        auto rel = [](const std::vector<double> &a, const std::vector<double> &b) {
+            double d = 0.0, s = 0.0;
+            for (size_t i = 0; i < a.size(); ++i) {
+                d = std::max(d, std::fabs(a[i] - b[i]));
+                s = std::max(s, std::fabs(b[i]));
+            }
+            return d / s;
+        };
+        EXPECT_TRUE(elem->GetField(APHIP_FIELD_RK_DISPLACEMENT) == u);
+        EXPECT_TRUE(rel(elem->GetField(APHIP_FIELD_F), f) < 1e-12);
+        EXPECT_TRUE(rel(elem->GetField(APHIP_FIELD_H), H) < 1e-12);
+        EXPECT_TRUE(rel(elem->GetField(APHIP_FIELD_P), P) < 1e-12);
+    });
 }
 
 }  // namespace

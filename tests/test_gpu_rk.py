@@ -124,3 +124,129 @@ def test_rk_j2_100_steps(gpu_available):
                         (A.FIELD_H, ref.H), (A.FIELD_P, ref.P)):
             assert rel_err(s.get(fld), rr) < 1e-10, fld
         assert rel_err(s.get(A.FIELD_STATE), ref.state) < 1e-10
+
+
+# ---- two-pass method ---------------------------------------------------------
+
+NH = dict(E=1e9, nu=0.3, rho=1000.0)
+
+
+def _gpu2(X, t, scale, kind=A.J2, p=J2):
+    s = A.Solver()
+    s.set_mesh(A.TET4, X, t)
+    s.set_material(kind, p["rho"], p["E"], p["nu"], p.get("yield_stress", 0.0), p.get("hardening", 0.0))
+    s.rk_setup(scale, one_pass=False)
+    return s
+
+
+@pytest.mark.parametrize("kind,p", [(A.J2, J2), (A.NEO_HOOKEAN, NH)])
+def test_two_pass_cells_mass_and_force(gpu_available, kind, p):
+    X, t = _mesh(6, seed=5)
+    scale = 1.5
+    r, num, nbr, tr, phi = _oracle_rk(X, t, scale)
+    cells2 = O.rk_cells_two_pass(X, t)
+    mass = O.rk_mass(t, cells2[3], p["rho"], num, nbr, phi)
+    lam, mu = O.lame(p["E"], p["nu"])
+    u = 4e-3 * (2 * M.uniform01(8, 7, X.size).reshape(X.shape) - 1)
+    kw = dict(yield_stress=p.get("yield_stress", 0.0), hardening=p.get("hardening", 0.0))
+    f_ref, H_ref, P_ref, s_ref, up_ref, fl_ref = O.rk_two_pass_force(len(X), cells2, num, nbr, phi, kind, lam, mu,
+                                                                      u, **kw)
+    print(f"rk two-pass kind={kind} nodes={len(X)} cells={len(t)}", flush=True)
+    with _gpu2(X, t, scale, kind, p) as s:
+        start, nodes, dphi, vol = s.rk_cells()
+        assert np.array_equal(start, cells2[0]) and np.array_equal(nodes, cells2[1])
+        assert rel_err(dphi, cells2[2]) < 1e-12 and rel_err(vol, cells2[3]) < 1e-14
+        s.compute_mass()
+        assert rel_err(s.get(A.FIELD_MASS), mass) < 1e-13
+        s.set_field(A.FIELD_U, u)
+        s.internal_force()
+        assert rel_err(s.get(A.FIELD_RK_DISPLACEMENT), up_ref) < 1e-12
+        assert rel_err(s.get(A.FIELD_RK_FORCE_LOCAL), fl_ref) < 1e-12
+        assert rel_err(s.get(A.FIELD_F), f_ref) < 1e-12
+        assert rel_err(s.get(A.FIELD_H), H_ref) < 1e-12
+        assert rel_err(s.get(A.FIELD_P), P_ref) < 1e-12
+        if kind == A.J2:
+            assert s_ref.max() > 0
+            assert rel_err(s.get(A.FIELD_STATE), s_ref) < 1e-12
+        f2 = s.get(A.FIELD_F)
+    # the one-pass method on the GPU gives the same force up to rounding
+    with _gpu(X, t, scale, kind, p) as s:
+        s.set_field(A.FIELD_U, u)
+        s.internal_force()
+        assert rel_err(f2, s.get(A.FIELD_F)) < 1e-12
+
+
+def test_two_pass_small_radius_is_the_tet4_element(gpu_available):
+    # ElementTest.cpp:154-162 (ReproducingKernelOnTet4TwoPass): a kernel radius scale of
+    # 0.03 leaves every node its own only neighbor, so the force is the smoothed Tet4's
+    X, t = _mesh(5, seed=2)
+    lam, mu = O.lame(NH["E"], NH["nu"])
+    u = 2e-3 * (2 * M.uniform01(3, 1, X.size).reshape(X.shape) - 1)
+    f_fe = O.internal_force(O.TET4, X, t, O.NEO_HOOKEAN, lam, mu, u)[0]
+    with _gpu2(X, t, 0.03, A.NEO_HOOKEAN, NH) as s:
+        num, _, _ = s.rk_neighbors()
+        assert (num == 1).all() and (s.rk_function_values()[:, 0] == 1.0).all()
+        s.set_field(A.FIELD_U, u)
+        s.internal_force()
+        assert np.array_equal(s.get(A.FIELD_RK_DISPLACEMENT), u)
+        assert rel_err(s.get(A.FIELD_F), f_fe) < 1e-12
+
+
+def test_two_pass_j2_100_steps(gpu_available):
+    X, t = _mesh(6, seed=9)
+    scale = 1.5
+    r, num, nbr, tr, phi = _oracle_rk(X, t, scale)
+    cells2 = O.rk_cells_two_pass(X, t)
+    mass = O.rk_mass(t, cells2[3], J2["rho"], num, nbr, phi)
+    lam, mu = O.lame(J2["E"], J2["nu"])
+    c = X - 0.5
+    v0 = 60.0 * np.stack([-0.5 * c[:, 0], -0.5 * c[:, 1], c[:, 2]], axis=1)
+    dt = 0.5 * M.stable_dt_estimate(X, t, J2["E"], J2["nu"], J2["rho"])
+    ref = O.Solver(-1, X, None, O.J2, lam, mu, mass, None, J2["yield_stress"], J2["hardening"], cells=cells2,
+                   two_pass=(num, nbr, phi))
+    ref.v[:] = v0
+    ref.run(dt, 1)
+    with _gpu2(X, t, scale) as s:
+        s.compute_mass()
+        s.set_field(A.FIELD_V, v0)
+        s.initialize()
+        s.step(dt, 1)
+        for fld, rr in ((A.FIELD_U, ref.u), (A.FIELD_V, ref.v), (A.FIELD_F, ref.f), (A.FIELD_P, ref.P)):
+            assert rel_err(s.get(fld), rr) < 1e-12, fld
+        s.step(dt, 99)
+        ref.run(dt, 99)
+        assert ref.state.max() > 0.0
+        for fld, rr in ((A.FIELD_U, ref.u), (A.FIELD_V, ref.v), (A.FIELD_A, ref.a), (A.FIELD_F, ref.f),
+                        (A.FIELD_H, ref.H), (A.FIELD_P, ref.P)):
+            assert rel_err(s.get(fld), rr) < 1e-10, fld
+        assert rel_err(s.get(A.FIELD_STATE), ref.state) < 1e-10
+
+
+def test_two_pass_graph_replay_and_power_method(gpu_available):
+    X, t = _mesh(5, seed=4)
+    scale = 1.5
+    lam, mu = O.lame(NH["E"], NH["nu"])
+    dt = 0.3 * M.stable_dt_estimate(X, t, NH["E"], NH["nu"], NH["rho"])
+    v0 = 0.5 * (2 * M.uniform01(6, 2, X.size).reshape(X.shape) - 1)
+    out = []
+    for graph in (False, True):
+        with _gpu2(X, t, scale, A.NEO_HOOKEAN, NH) as s:
+            s.compute_mass()
+            s.set_graph(graph)
+            s.set_field(A.FIELD_V, v0)
+            s.initialize()
+            for _ in range(3):
+                s.step(dt, 10)
+                out.append(s.get(A.FIELD_U))
+    for k in range(3):
+        assert np.array_equal(out[k], out[3 + k])
+    # the power-method increment of the two-pass operator equals the one-pass one
+    res = []
+    for one in (True, False):
+        with A.Solver() as s:
+            s.set_mesh(A.TET4, X, t)
+            s.set_material(A.NEO_HOOKEAN, NH["rho"], NH["E"], NH["nu"])
+            s.rk_setup(scale, one_pass=one)
+            s.compute_mass()
+            res.append(s.power_method_time_increment(num_iterations=50))
+    assert res[0][1] == res[1][1] and abs(res[0][0] - res[1][0]) < 1e-9 * res[0][0]

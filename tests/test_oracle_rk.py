@@ -163,3 +163,93 @@ def test_rk_mass_sums_to_rho_v():
     start, nodes, dphi, vol = O.rk_cells(X, t, num, nbr, phi)
     m = O.rk_mass(t, vol, 2700.0, num, nbr, phi)
     assert m.sum() == pytest.approx(2700.0 * vol.sum(), rel=1e-11)
+
+
+# ---- two-pass method (ExplicitSolver::ComputeForce with force_two_pass_method) ----
+
+def test_two_pass_cells_are_element_b_matrices():
+    # a cell of the two-pass method lists its element's nodes; the face-smoothed
+    # derivative of a linear tet is its exact (constant) gradient
+    X, t, r, num, nbr, phi = _rk(n=4)
+    start, nodes, dphi, vol = O.rk_cells_two_pass(X, t)
+    assert np.array_equal(start, 4 * np.arange(len(t) + 1)) and np.array_equal(nodes, t.ravel())
+    assert np.allclose(vol, O.element_volumes(0, X, t), rtol=1e-13, atol=0)
+    for e in range(0, len(t), 7):
+        B, V = O.tet4_bmatrix(X[t[e]])
+        assert np.allclose(dphi[4 * e:4 * e + 4], B, rtol=0, atol=1e-11 * np.abs(B).max())
+
+
+def test_gather_reproduces_linear_fields_and_scatter_is_its_transpose():
+    X, t, r, num, nbr, phi = _rk()
+    G = np.array([[1e-3, 2e-3, -1e-3], [4e-4, -2e-3, 3e-3], [1e-3, 0.0, 5e-4]])
+    d = X @ G.T + np.array([0.1, -0.2, 0.3])
+    assert np.abs(O.rk_gather(num, nbr, phi, d) - d).max() < 1e-14
+    # the transpose applied as a gather does not reproduce (a planted Phi^T is caught)
+    assert np.abs(O.rk_scatter(num, nbr, phi, d) - d).max() > 1e-3
+    rng = np.random.default_rng(4)
+    a, b = rng.standard_normal(d.shape), rng.standard_normal(d.shape)
+    lhs = (O.rk_gather(num, nbr, phi, a) * b).sum()
+    rhs = (a * O.rk_scatter(num, nbr, phi, b)).sum()
+    assert lhs == pytest.approx(rhs, rel=1e-13)
+    # dense check: gather is Phi, scatter is Phi^T
+    n = len(X)
+    Phi = np.zeros((n, n))
+    for i in range(n):
+        Phi[i, nbr[i, :num[i]]] = phi[i, :num[i]]
+    assert np.abs(O.rk_gather(num, nbr, phi, a) - Phi @ a).max() < 1e-13 * np.abs(a).max()
+    assert np.abs(O.rk_scatter(num, nbr, phi, b) - Phi.T @ b).max() < 1e-13 * np.abs(b).max()
+
+
+def test_two_pass_linear_patch():
+    X, t, r, num, nbr, phi = _rk()
+    cells2 = O.rk_cells_two_pass(X, t)
+    G = np.array([[0.01, 0.002, 0.0], [0.0, -0.004, 0.003], [0.001, 0.0, 0.02]])
+    d = X @ G.T + 0.05
+    lam, mu = O.lame(70e9, 0.33)
+    f, H, P, _, up, fl = O.rk_two_pass_force(len(X), cells2, num, nbr, phi, O.LINEAR_ELASTIC, lam, mu, d)
+    assert np.allclose(up, d, atol=1e-14)
+    assert np.allclose(H, G.ravel(), atol=1e-12)
+
+
+@pytest.mark.parametrize("kind", [O.NEO_HOOKEAN, O.J2])
+def test_two_pass_force_equals_one_pass(kind):
+    # one-pass: grad phi_J = sum_k grad N_k phi_kJ per cell; two-pass: Phi^T f(Phi d).
+    # The same bilinear form in another order, so H, P and f agree to rounding;
+    # a transposed Phi on either side breaks the agreement.
+    X, t, r, num, nbr, phi = _rk()
+    cells1 = O.rk_cells(X, t, num, nbr, phi)
+    cells2 = O.rk_cells_two_pass(X, t)
+    lam, mu = O.lame(70e9, 0.33)
+    d = 4e-3 * (2 * M.uniform01(8, 7, X.size).reshape(X.shape) - 1)
+    kw = dict(yield_stress=100e6, hardening=1e9)
+    f1, H1, P1, s1 = O.cell_internal_force(len(X), cells1, kind, lam, mu, d, **kw)
+    f2, H2, P2, s2, up, fl = O.rk_two_pass_force(len(X), cells2, num, nbr, phi, kind, lam, mu, d, **kw)
+    if kind == O.J2:
+        assert s1.max() > 0
+    for a, b in ((H2, H1), (P2, P1), (f2, f1)):
+        assert np.abs(a - b).max() < 1e-12 * np.abs(b).max()
+    assert np.abs(s2 - s1).max() < 1e-12 * s1.max() if kind == O.J2 else True
+    wrong = O.rk_scatter(num, nbr, phi, d)  # Phi^T in the gather
+    fw = O.cell_internal_force(len(X), cells2, kind, lam, mu, wrong, **kw)[0]
+    assert np.abs(O.rk_gather(num, nbr, phi, fw) - f1).max() > 1e-3 * np.abs(f1).max()
+
+
+def test_two_pass_solver_matches_one_pass_over_steps():
+    X, t, r, num, nbr, phi = _rk(n=4)
+    cells1 = O.rk_cells(X, t, num, nbr, phi)
+    cells2 = O.rk_cells_two_pass(X, t)
+    lam, mu = O.lame(70e9, 0.33)
+    mass = O.rk_mass(t, cells1[3], 2700.0, num, nbr, phi)
+    assert np.array_equal(O.rk_mass(t, cells2[3], 2700.0, num, nbr, phi), mass)
+    c = X - 0.5
+    v0 = 60.0 * np.stack([-0.5 * c[:, 0], -0.5 * c[:, 1], c[:, 2]], axis=1)
+    dt = 0.5 * M.stable_dt_estimate(X, t, 70e9, 0.33, 2700.0)
+    a = O.Solver(-1, X, None, O.J2, lam, mu, mass, None, 300e6, 1e9, cells=cells1)
+    b = O.Solver(-1, X, None, O.J2, lam, mu, mass, None, 300e6, 1e9, cells=cells2, two_pass=(num, nbr, phi))
+    a.v[:] = v0
+    b.v[:] = v0
+    a.run(dt, 50)
+    b.run(dt, 50)
+    assert a.state.max() > 0
+    for x, y in ((b.u, a.u), (b.v, a.v), (b.f, a.f), (b.P, a.P)):
+        assert np.abs(x - y).max() < 1e-10 * np.abs(y).max()
