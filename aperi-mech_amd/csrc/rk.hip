@@ -551,12 +551,10 @@ __device__ __forceinline__ double group_sum(double x) {
 
 __device__ __forceinline__ double group_sum16(double x) { return group_sum<kCellLanes>(x); }
 
-// LANES lanes per cell: 16 for the one-pass cells (tens of entries), 4 for the
-// two-pass cells (the element's four nodes, one entry per lane).
 template <int KIND, int LANES>
 __global__ __launch_bounds__(256) void k_rk_cells(CellForceArgs A) {
 #pragma clang fp contract(off)
-    constexpr int kCache = LANES == kCellLanes ? kCellCache : 1;
+    constexpr int kCache = kCellCache;
     const int l = threadIdx.x & (LANES - 1);
     const int64_t c_raw = xcd_block(A.nblocks) * (256 / LANES) + (threadIdx.x / LANES);
     const bool live = c_raw < A.ne;
@@ -641,6 +639,67 @@ __global__ __launch_bounds__(256) void k_rk_cells(CellForceArgs A) {
             for (int a = 0; a < 3; ++a) f[a] = mv * dot3(P[3 * a], g[0], P[3 * a + 1], g[1], P[3 * a + 2], g[2]);
             st3(A.g, A.pos[t], f);
         }
+    }
+}
+
+// Two-pass cells, one thread per cell: the element's four nodes, their derivative
+// (B_k V) / V recomputed from the coordinates by the set-up's formula (instead of
+// streaming the stored 96 B per cell), H, stress and the four entry forces
+// -V P grad N_k at their node-order positions.
+template <int KIND>
+__global__ __launch_bounds__(256) void k_rk_cells_two_pass(CellForceArgs A, const double *X) {
+#pragma clang fp contract(off)
+    const int64_t c_raw = xcd_block(A.nblocks) * 256 + threadIdx.x;
+    const bool live = c_raw < A.ne;
+    const int64_t c = live ? c_raw : A.ne - 1;
+    int32_t en[4], ps[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        en[k] = A.nodes[4 * c + k];
+        ps[k] = A.pos[4 * c + k];
+    }
+    double X4[4][3], uj[4][3];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        ld3(X, en[k], X4[k]);
+        ld3(A.u, en[k], uj[k]);
+    }
+    double sn = 0.0, snp1 = 0.0;
+    if constexpr (KIND == APHIP_J2) sn = A.state_n[c];
+    MatParams m = A.mat;
+    if (A.elem_lambda) {
+        m.lambda = A.elem_lambda[c];
+        m.mu = A.elem_mu[c];
+    }
+    double B[4][3], V;
+    smoothed_tet4(X4, B, V);
+    double gc[4][3];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+#pragma unroll
+        for (int b = 0; b < 3; ++b) gc[k][b] = (B[k][b] * V) / V;
+    double H[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, P[9];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int b = 0; b < 3; ++b) H[3 * a + b] = __fma_rn(uj[k][a], gc[k][b], H[3 * a + b]);
+    stress<KIND>(H, m, sn, snp1, P);
+    if (!live) return;
+    if constexpr (KIND == APHIP_J2) A.state_np1[c] = snp1;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        store_once(A.H + i * A.ne + c, H[i]);
+        store_once(A.P + i * A.ne + c, P[i]);
+    }
+    const double mv = -V;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        double f[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) f[a] = mv * dot3(P[3 * a], gc[k][0], P[3 * a + 1], gc[k][1], P[3 * a + 2], gc[k][2]);
+        st3(A.g, ps[k], f);
     }
 }
 
@@ -1034,28 +1093,31 @@ void rk_cell_force(Ctx &c) {
         APHIP_HIP(hipGetLastError());
     }
     if (c.ne == 0) return;
-    const int lanes = D.two_pass ? 4 : kCellLanes;
-    const int64_t nbc = padded_groups(c.ne, lanes);
+    const int64_t nbc = padded_groups(c.ne, D.two_pass ? 1 : kCellLanes);
     CellForceArgs A{c.ne, nbc, D.start.p, D.nodes.p, D.dphi.p, D.volume.p, D.two_pass ? D.u_phys.p : c.u.p,
                     c.H.p, c.P.p, D.g.p, D.entry_pos.p,
                     c.state[1 - c.state_np1].p, c.state[c.state_np1].p,
                     MatParams{c.mat.lambda, c.mat.mu, c.mat.yield_stress, c.mat.hardening},
                     c.has_fields ? c.elem_lambda.p : nullptr, c.has_fields ? c.elem_mu.p : nullptr};
     const dim3 g((unsigned)nbc);
-#define APHIP_RK_CELLS(L)                                                                                      \
-    switch (c.mat.kind) {                                                                                      \
-        case APHIP_LINEAR_ELASTIC: hipLaunchKernelGGL((k_rk_cells<APHIP_LINEAR_ELASTIC, L>), g, dim3(256), 0, c.stream, A); break; \
-        case APHIP_NEO_HOOKEAN: hipLaunchKernelGGL((k_rk_cells<APHIP_NEO_HOOKEAN, L>), g, dim3(256), 0, c.stream, A); break;       \
-        case APHIP_J2: hipLaunchKernelGGL((k_rk_cells<APHIP_J2, L>), g, dim3(256), 0, c.stream, A); break;                         \
-        case APHIP_SVK: hipLaunchKernelGGL((k_rk_cells<APHIP_SVK, L>), g, dim3(256), 0, c.stream, A); break;                       \
-        default: throw Error(APHIP_ERR_INVALID, "material not set");                                           \
-    }
     if (D.two_pass) {
-        APHIP_RK_CELLS(4)
+        const double *X = c.X.p;
+        switch (c.mat.kind) {
+            case APHIP_LINEAR_ELASTIC: hipLaunchKernelGGL(k_rk_cells_two_pass<APHIP_LINEAR_ELASTIC>, g, dim3(256), 0, c.stream, A, X); break;
+            case APHIP_NEO_HOOKEAN: hipLaunchKernelGGL(k_rk_cells_two_pass<APHIP_NEO_HOOKEAN>, g, dim3(256), 0, c.stream, A, X); break;
+            case APHIP_J2: hipLaunchKernelGGL(k_rk_cells_two_pass<APHIP_J2>, g, dim3(256), 0, c.stream, A, X); break;
+            case APHIP_SVK: hipLaunchKernelGGL(k_rk_cells_two_pass<APHIP_SVK>, g, dim3(256), 0, c.stream, A, X); break;
+            default: throw Error(APHIP_ERR_INVALID, "material not set");
+        }
     } else {
-        APHIP_RK_CELLS(kCellLanes)
+        switch (c.mat.kind) {
+            case APHIP_LINEAR_ELASTIC: hipLaunchKernelGGL((k_rk_cells<APHIP_LINEAR_ELASTIC, kCellLanes>), g, dim3(256), 0, c.stream, A); break;
+            case APHIP_NEO_HOOKEAN: hipLaunchKernelGGL((k_rk_cells<APHIP_NEO_HOOKEAN, kCellLanes>), g, dim3(256), 0, c.stream, A); break;
+            case APHIP_J2: hipLaunchKernelGGL((k_rk_cells<APHIP_J2, kCellLanes>), g, dim3(256), 0, c.stream, A); break;
+            case APHIP_SVK: hipLaunchKernelGGL((k_rk_cells<APHIP_SVK, kCellLanes>), g, dim3(256), 0, c.stream, A); break;
+            default: throw Error(APHIP_ERR_INVALID, "material not set");
+        }
     }
-#undef APHIP_RK_CELLS
     APHIP_HIP(hipGetLastError());
 }
 

@@ -272,6 +272,11 @@ __global__ __launch_bounds__(NPE == 4 ? 1024 : (QPAR ? 512 : 256), NPE == 8 && !
     double fe[KF][3];
     const int te_round = ROUNDS == 1 ? A.te : A.te / ROUNDS;
     const int my_round = ROUNDS == 1 ? 0 : le / te_round;
+    // LDS row of entry (element j of the round, slot k).  One element per thread:
+    // slot-major, so a wave's writes of one slot are consecutive 24-byte rows (no bank
+    // conflicts; element-major rows 96 bytes apart hit 16 of the 64 banks).  One
+    // Gauss point per lane: element-major, which is consecutive for those lanes.
+    auto frow = [te_round](int j, int k) { return QPAR ? j * NPE + k : k * te_round + j; };
     if constexpr (QPAR && NPE == 8) {
         if (has_elem) {
             const int q = tid & 7;
@@ -286,7 +291,7 @@ __global__ __launch_bounds__(NPE == 4 ? 1024 : (QPAR ? 512 : 256), NPE == 8 && !
                     store_once(A.P + i * A.ne + e, P[i]);
                 }
             }
-            if (my_round == 0) st3(L.F, le * NPE + q, fe[0]);
+            if (my_round == 0) st3(L.F, frow(le, q), fe[0]);
         }
     } else if (has_elem) {
         double X[NPE][3], U[NPE][3];
@@ -312,7 +317,7 @@ __global__ __launch_bounds__(NPE == 4 ? 1024 : (QPAR ? 512 : 256), NPE == 8 && !
         }
         if (my_round == 0) {
 #pragma unroll
-            for (int k = 0; k < NPE; ++k) st3(L.F, le * NPE + k, fe[k]);
+            for (int k = 0; k < NPE; ++k) st3(L.F, frow(le, k), fe[k]);
         }
     }
     __syncthreads();
@@ -328,9 +333,10 @@ __global__ __launch_bounds__(NPE == 4 ? 1024 : (QPAR ? 512 : 256), NPE == 8 && !
             double f[3] = {0.0, 0.0, 0.0};
             for (int j = 0; j < cnt; ++j) {
                 const int s = L.ent[st + j];
-                f[0] += L.F[3 * s];
-                f[1] += L.F[3 * s + 1];
-                f[2] += L.F[3 * s + 2];
+                const int rw = 3 * frow(s / NPE, s % NPE);
+                f[0] += L.F[rw];
+                f[1] += L.F[rw + 1];
+                f[2] += L.F[rw + 2];
             }
             finish_node<MODE>(A, L, ln, nint, ibase, f);
         }
@@ -350,10 +356,10 @@ __global__ __launch_bounds__(NPE == 4 ? 1024 : (QPAR ? 512 : 256), NPE == 8 && !
             __syncthreads();  // the previous round's gather is done with F
             if (has_elem && my_round == r) {
                 if constexpr (QPAR && NPE == 8) {
-                    st3(L.F, (le - r * te_round) * NPE + (tid & 7), fe[0]);
+                    st3(L.F, frow(le - r * te_round, tid & 7), fe[0]);
                 } else {
 #pragma unroll
-                    for (int k = 0; k < NPE; ++k) st3(L.F, (le - r * te_round) * NPE + k, fe[k]);
+                    for (int k = 0; k < NPE; ++k) st3(L.F, frow(le - r * te_round, k), fe[k]);
                 }
             }
             __syncthreads();
@@ -369,9 +375,10 @@ __global__ __launch_bounds__(NPE == 4 ? 1024 : (QPAR ? 512 : 256), NPE == 8 && !
                 for (; j < cnt; ++j) {
                     const int s = L.ent[st + j];
                     if (s >= lim) break;
-                    acc[q][0] += L.F[3 * (s - off)];
-                    acc[q][1] += L.F[3 * (s - off) + 1];
-                    acc[q][2] += L.F[3 * (s - off) + 2];
+                    const int rw = 3 * frow((s - off) / NPE, (s - off) % NPE);
+                    acc[q][0] += L.F[rw];
+                    acc[q][1] += L.F[rw + 1];
+                    acc[q][2] += L.F[rw + 2];
                 }
                 jpos[q] = j;
             }

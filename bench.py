@@ -244,10 +244,11 @@ def bench_adjoint(n=64, steps=200, interval=20, n_check=32, h=1e-6):
                                       "forward is tested at 1e-12 per step)"}}
 
 
-def bench_rkpm(n_nodes_edge=64, steps=100, scale=1.5):
+def bench_rkpm(n_nodes_edge=64, steps=100, scale=1.5, one_pass=True):
     """Strain-smoothed RKPM: 64^3-node jittered lattice, Kuhn Tet4 background, element-as-cell
     smoothing, kernel support 1.5 x the max edge length, J2 with linear hardening, a velocity
-    ramp about the centre reaching a nominal strain of 2 sigma_y / E at step 100."""
+    ramp about the centre reaching a nominal strain of 2 sigma_y / E at step 100.
+    one_pass=False: the two-pass force (gather Phi d, element-node cells, scatter Phi^T f)."""
     import aperi_hip as A
     from aperi_hip import meshgen as M
     w = M.rkpm_lattice(n_nodes_edge)
@@ -257,7 +258,7 @@ def bench_rkpm(n_nodes_edge=64, steps=100, scale=1.5):
     s.set_material(A.J2, w["rho"], w["E"], w["nu"], w["yield_stress"], w["hardening"])
     s.synchronize()
     t0 = time.perf_counter()
-    s.rk_setup(scale)
+    s.rk_setup(scale, one_pass=one_pass)
     s.synchronize()
     t_setup = time.perf_counter() - t0
     num, _, truncated = s.rk_neighbors()
@@ -280,15 +281,23 @@ def bench_rkpm(n_nodes_edge=64, steps=100, scale=1.5):
     yielded = int((s.get(A.FIELD_STATE) > 0).sum())
     s.close()
     lc = nnz / ne
+    # per cell: entries (4 B id + 24 B grad) + H, P, volume, J2 state N/NP1; per node: the
+    # node streams; two-pass adds per node the neighbor rows of the gather and of the
+    # scatter (4 B id + 8 B phi per neighbor each) and the physical displacement and
+    # local force written once and read once
     bytes_per_update = 28.0 * lc + 168.0 + 200.0 * nn / ne
+    if not one_pass:
+        bytes_per_update += (24.0 * float(num.mean()) + 96.0) * nn / ne
     step_ms = m0 / max(n0, 1)
     achieved = bytes_per_update * ne / (step_ms * 1e-3) / 1e9
+    method = "one-pass" if one_pass else "two-pass"
     return {"workload": f"{n_nodes_edge}^3-node RKPM lattice ({nn} nodes, {ne} Tet4 cells), J2, kernel scale {scale}, "
-                        f"{steps} steps",
+                        f"{method} force, {steps} steps",
             "setup_s": t_setup, "mean_neighbors": float(num.mean()), "max_neighbors": int(num.max()),
             "truncated_nodes": int(truncated), "cell_list_mean": lc,
             "cell_updates_per_s": ne * steps / wall, "ms_per_step": 1e3 * wall / steps,
-            "kernels_ms": {"step": step_ms, "cell_force": m1 / max(n1, 1), "node_gather": m2 / max(n2, 1)},
+            "kernels_ms": {"step": step_ms, "cell_force" if one_pass else "gather_and_cell_force": m1 / max(n1, 1),
+                           "node_gather" if one_pass else "node_sum_and_scatter": m2 / max(n2, 1)},
             "roofline": {"bound": "hbm", "bytes_per_update": bytes_per_update, "achieved": achieved,
                          "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS},
             "yielded_cells": yielded}
@@ -437,6 +446,10 @@ def main():
         line["rkpm"] = bench_rkpm()
         log(f"[bench] rkpm: {line['rkpm']['cell_updates_per_s']:.3e} cell-updates/s, "
             f"setup {line['rkpm']['setup_s']:.2f}s, roofline {line['rkpm']['roofline']['frac']:.3f}")
+        log("[bench] rkpm two-pass ...")
+        line["rkpm_two_pass"] = bench_rkpm(one_pass=False)
+        log(f"[bench] rkpm two-pass: {line['rkpm_two_pass']['cell_updates_per_s']:.3e} cell-updates/s, "
+            f"roofline {line['rkpm_two_pass']['roofline']['frac']:.3f}")
         log("[bench] power method ...")
         line["power_method"] = bench_power(args.n)
         log(f"[bench] power method: {line['power_method']['iterations']} iterations, "
