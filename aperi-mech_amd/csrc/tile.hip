@@ -93,6 +93,12 @@ struct TileLds {
     uint8_t *fx;
 };
 
+// Rows between the slot blocks of the element-force rows: te_round + pad with
+// stride = 4 (mod 32), so that the 24-byte rows of one element's slots start 24 banks
+// apart (mod 64): the 4 or 8 slots of an element, which the nodes of that element
+// read at the same time in the gather, never share a bank.
+__host__ __device__ __forceinline__ int f_stride(int te_round) { return te_round + ((4 - te_round) & 31); }
+
 __host__ __device__ __forceinline__ TileLds carve(char *base, int maxtn, int max_nint, int te, int f_rounds,
                                                   int npe) {
     TileLds L;
@@ -102,7 +108,7 @@ __host__ __device__ __forceinline__ TileLds carve(char *base, int maxtn, int max
     L.M = L.V + 3 * max_nint;
     L.DT = L.M + max_nint;
     L.F = L.DT + max_nint;
-    L.csr = reinterpret_cast<uint32_t *>(L.F + 3 * (te / f_rounds) * npe);
+    L.csr = reinterpret_cast<uint32_t *>(L.F + 3 * (size_t)f_stride(te / f_rounds) * npe);
     L.bdst = reinterpret_cast<int32_t *>(L.csr + maxtn);
     L.ent = reinterpret_cast<uint16_t *>(L.bdst + maxtn);
     L.fx = reinterpret_cast<uint8_t *>(L.ent + te * npe);
@@ -110,7 +116,7 @@ __host__ __device__ __forceinline__ TileLds carve(char *base, int maxtn, int max
 }
 
 __host__ inline size_t tile_lds_bytes(int maxtn, int max_nint, int te, int f_rounds, int npe) {
-    return sizeof(double) * (6 * (size_t)maxtn + 5 * (size_t)max_nint + 3 * (size_t)(te / f_rounds) * npe) +
+    return sizeof(double) * (6 * (size_t)maxtn + 5 * (size_t)max_nint + 3 * (size_t)f_stride(te / f_rounds) * npe) +
            8 * (size_t)maxtn + 2 * (size_t)te * npe + (size_t)max_nint + 16;
 }
 
@@ -273,10 +279,12 @@ __global__ __launch_bounds__(NPE == 4 ? 1024 : (QPAR ? 512 : 256), NPE == 8 && !
     const int te_round = ROUNDS == 1 ? A.te : A.te / ROUNDS;
     const int my_round = ROUNDS == 1 ? 0 : le / te_round;
     // LDS row of entry (element j of the round, slot k).  One element per thread:
-    // slot-major, so a wave's writes of one slot are consecutive 24-byte rows (no bank
-    // conflicts; element-major rows 96 bytes apart hit 16 of the 64 banks).  One
-    // Gauss point per lane: element-major, which is consecutive for those lanes.
-    auto frow = [te_round](int j, int k) { return QPAR ? j * NPE + k : k * te_round + j; };
+    // slot-major with padded slot blocks, so a wave's writes of one slot are
+    // consecutive 24-byte rows (no bank conflicts; element-major rows 96 bytes apart
+    // hit 16 of the 64 banks) and the slots of one element sit in distinct banks for
+    // the gather.  One Gauss point per lane: element-major, consecutive for those lanes.
+    const int fs = f_stride(te_round);
+    auto frow = [fs](int j, int k) { return QPAR ? j * NPE + k : k * fs + j; };
     if constexpr (QPAR && NPE == 8) {
         if (has_elem) {
             const int q = tid & 7;
