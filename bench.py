@@ -281,13 +281,16 @@ def bench_rkpm(n_nodes_edge=64, steps=100, scale=1.5, one_pass=True):
     yielded = int((s.get(A.FIELD_STATE) > 0).sum())
     s.close()
     lc = nnz / ne
-    # per cell: entries (4 B id + 24 B grad) + H, P, volume, J2 state N/NP1; per node: the
-    # node streams; two-pass adds per node the neighbor rows of the gather and of the
-    # scatter (4 B id + 8 B phi per neighbor each) and the physical displacement and
-    # local force written once and read once
-    bytes_per_update = 28.0 * lc + 168.0 + 200.0 * nn / ne
-    if not one_pass:
-        bytes_per_update += (24.0 * float(num.mean()) + 96.0) * nn / ne
+    # one-pass, per cell: entries (4 B id + 24 B grad) + H, P, volume, J2 state N/NP1;
+    # per node: the node streams.  Two-pass, per cell: the four node ids + H, P, J2 state
+    # (the derivatives and volume follow from X, part of the node streams); per node: the
+    # node streams, the neighbor rows of the gather and of the scatter (4 B id + 8 B phi
+    # per neighbor each) and the physical displacement and local force, each written
+    # once and read once
+    if one_pass:
+        bytes_per_update = 28.0 * lc + 168.0 + 200.0 * nn / ne
+    else:
+        bytes_per_update = 16.0 + 160.0 + (200.0 + 96.0 + 24.0 * float(num.mean())) * nn / ne
     step_ms = m0 / max(n0, 1)
     achieved = bytes_per_update * ne / (step_ms * 1e-3) / 1e9
     method = "one-pass" if one_pass else "two-pass"
