@@ -134,7 +134,10 @@ struct MaterialProperties {
     MaterialType material_type = NONE;
     double density = std::numeric_limits<double>::quiet_NaN();
     LinearElasticProperties linear_elastic_properties;
-    std::map<std::string, double> properties;  // "yield_stress", "hardening_modulus" for PLASTIC
+    // "yield_stress", "hardening_modulus" for PLASTIC; "I1_critical", "I1_failure", "alpha" and
+    // the optional "maximum_damage", "maximum_damage_increment", "stop_failing_at_time",
+    // "fail_all_at_time" for NEO_HOOKEAN_WITH_DAMAGE (include/Materials/Factory.h:70-101)
+    std::map<std::string, double> properties;
 };
 
 // Host side of a material: its kind on the device and its completed constants.
@@ -148,23 +151,37 @@ class Material {
             case NEO_HOOKEAN: m_kind = APHIP_NEO_HOOKEAN; break;
             case PLASTIC: m_kind = APHIP_J2; break;
             case ELASTIC: m_kind = APHIP_SVK; break;
+            case NEO_HOOKEAN_WITH_DAMAGE: m_kind = APHIP_NEO_HOOKEAN_DAMAGE; break;
             default: throw std::runtime_error("Material type not supported by the MI355X force step");
         }
         if (m_kind == APHIP_J2) {
             m_yield = Required("yield_stress");
             m_hardening = Required("hardening_modulus");
         }
+        if (m_kind == APHIP_NEO_HOOKEAN_DAMAGE) {
+            m_damage[0] = Required("I1_critical");
+            m_damage[1] = Required("I1_failure");
+            m_damage[2] = Required("alpha");
+            m_damage[3] = Optional("maximum_damage", 0.99);
+            m_damage[4] = Optional("maximum_damage_increment", 1.0);
+            m_damage[5] = Optional("stop_failing_at_time", -1.0);
+            m_damage[6] = Optional("fail_all_at_time", -1.0);
+        }
     }
     MaterialType GetMaterialType() const { return m_props->material_type; }
     std::shared_ptr<MaterialProperties> GetMaterialProperties() const { return m_props; }
     double GetDensity() const { return m_props->density; }
-    bool HasState() const { return m_kind == APHIP_J2; }
+    bool HasState() const { return m_kind == APHIP_J2 || m_kind == APHIP_NEO_HOOKEAN_DAMAGE; }
+    // the reference's count (NeoHookeWithDamage.h:279-282 reports 1, the damage)
     size_t NumberOfStateVariables() const { return HasState() ? 1 : 0; }
+    // values per element of the state field (the damage model also stores I1)
+    size_t StateFieldWidth() const { return m_kind == APHIP_NEO_HOOKEAN_DAMAGE ? 2 : NumberOfStateVariables(); }
     int Kind() const { return m_kind; }
     aphip_material_params Params(int state_per_qp = 0) const {
         const LinearElasticProperties &le = m_props->linear_elastic_properties;
         return aphip_material_params{m_props->density, le.youngs_modulus, le.poisson_ratio, m_yield, m_hardening,
-                                     state_per_qp, 0};
+                                     state_per_qp, 0, m_damage[0], m_damage[1], m_damage[2], m_damage[3],
+                                     m_damage[4], m_damage[5], m_damage[6]};
     }
 
    private:
@@ -173,9 +190,14 @@ class Material {
         if (it == m_props->properties.end()) throw std::runtime_error(std::string("Missing material property: ") + name);
         return it->second;
     }
+    double Optional(const char *name, double fallback) const {
+        const auto it = m_props->properties.find(name);
+        return it == m_props->properties.end() ? fallback : it->second;
+    }
     std::shared_ptr<MaterialProperties> m_props;
     int m_kind = -1;
     double m_yield = 0.0, m_hardening = 0.0;
+    double m_damage[7] = {0, 0, 0, 0, 0, 0, 0};
 };
 
 inline std::shared_ptr<Material> CreateMaterial(std::shared_ptr<MaterialProperties> p) {
@@ -269,7 +291,7 @@ class ElementBase {
             case APHIP_FIELD_MASS: return m_mesh_nodes;
             case APHIP_FIELD_H: case APHIP_FIELD_P: return 9 * m_elements;
             case APHIP_FIELD_VOLUME: return m_elements;
-            case APHIP_FIELD_STATE: return m_elements * m_material->NumberOfStateVariables();
+            case APHIP_FIELD_STATE: return m_elements * m_material->StateFieldWidth();
             default: throw std::runtime_error("unknown field");
         }
     }

@@ -448,7 +448,7 @@ void adj_run(Ctx &c, const double *u1, const double *fbar, int mode, double dt) 
     A.bpart = P.bpart.p;
     A.lam_bar = D.lam_bar.p;
     A.mu_bar = D.mu_bar.p;
-    A.mat = MatParams{c.mat.lambda, c.mat.mu, c.mat.yield_stress, c.mat.hardening};
+    A.mat = c.mat.params();
     A.elem_lambda = c.has_fields ? c.elem_lambda.p : nullptr;
     A.elem_mu = c.has_fields ? c.elem_mu.p : nullptr;
     A.gauss_a = 1.0 / std::sqrt(3.0);

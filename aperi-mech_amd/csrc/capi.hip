@@ -114,7 +114,7 @@ static std::vector<int32_t> morton_order(const std::vector<double> &pts, int64_t
 }
 
 static void resolve_material(int kind, const aphip_material_params *p, MaterialConst &m) {
-    APHIP_REQUIRE(kind >= APHIP_LINEAR_ELASTIC && kind <= APHIP_SVK, "unknown material kind");
+    APHIP_REQUIRE(kind >= APHIP_LINEAR_ELASTIC && kind <= APHIP_NEO_HOOKEAN_DAMAGE, "unknown material kind");
     APHIP_REQUIRE(p != nullptr, "material params are NULL");
     const double E = p->youngs_modulus, nu = p->poissons_ratio;
     // LinearElasticProperties::CompleteProperties, E/nu branch and bound checks
@@ -131,6 +131,22 @@ static void resolve_material(int kind, const aphip_material_params *p, MaterialC
     m.E = E;
     m.nu = nu;
     m.state_per_qp = p->state_per_qp ? 1 : 0;
+    if (kind == APHIP_NEO_HOOKEAN_DAMAGE) {
+        // NeoHookeanWithDamageGetStressFunctor's checks and clamps (NeoHookeWithDamage.h:126-168)
+        APHIP_REQUIRE(m.lambda >= 0.0, "NeoHookeanWithDamage: lambda must be >= 0");
+        APHIP_REQUIRE(m.mu >= 0.0, "NeoHookeanWithDamage: mu must be >= 0");
+        APHIP_REQUIRE(p->I1_critical >= 0.0, "NeoHookeanWithDamage: I1_critical must be >= 0");
+        APHIP_REQUIRE(p->I1_failure >= 0.0, "NeoHookeanWithDamage: I1_failure must be >= 0");
+        APHIP_REQUIRE(p->I1_critical < p->I1_failure, "NeoHookeanWithDamage: I1_critical must be < I1_failure");
+        APHIP_REQUIRE(p->alpha >= 0.0, "NeoHookeanWithDamage: alpha must be >= 0");
+        APHIP_REQUIRE(p->maximum_damage >= 0.0 && p->maximum_damage <= 1.0,
+                      "NeoHookeanWithDamage: max_effective_damage must be in [0, 1]");
+        m.I1_critical = p->I1_critical;
+        m.I1_failure = p->I1_failure;
+        m.alpha = p->alpha;
+        m.max_damage = p->maximum_damage;
+        m.max_damage_increment = std::min(std::max(p->maximum_damage_increment, 0.0), 1.0);
+    }
 }
 
 static void alloc_state(Ctx &c) {
@@ -953,7 +969,8 @@ int aphip_batch_begin(aphip_ctx *c, int32_t elem_type, int32_t kind) {
     APHIP_API_BEGIN
     APHIP_REQUIRE(c != nullptr, "ctx is NULL");
     APHIP_REQUIRE(elem_type == APHIP_TET4 || elem_type == APHIP_HEX8, "element type must be TET4 or HEX8");
-    APHIP_REQUIRE(kind >= APHIP_LINEAR_ELASTIC && kind <= APHIP_SVK, "unknown material kind");
+    APHIP_REQUIRE(kind >= APHIP_LINEAR_ELASTIC && kind <= APHIP_SVK,
+                  "ensembles take the linear-elastic, neo-Hookean, J2 and SVK materials");
     c->batch = BatchBuild();
     c->batch.open = true;
     c->batch.etype = elem_type;

@@ -6,6 +6,7 @@
 //                   F^-1 by cofactors, as the reference's CUDA build does, MathUtils.h:314-316)
 //   J2              include/Materials/Plastic.h:88-140
 //   SVK             include/Materials/Elastic.h:61-79
+//   NH with damage  include/Materials/NeoHookeWithDamage.h:180-243
 // The material is a compile-time template parameter; the reference's device
 // virtual call (ComputeInternalForceGaussian.h:107) is resolved on the host.
 #pragma once
@@ -79,11 +80,40 @@ APHIP_DEV double inverse3(const double *A, double *Ai) {
 
 struct MatParams {
     double lambda, mu, yield_stress, hardening;
+    // neo-Hookean with damage; max_damage and max_damage_increment lie in [0, 1]
+    double I1_critical, I1_failure, alpha, max_damage, max_damage_increment;
 };
 
-// P = stress(H); sn / snp1: J2 equivalent plastic strain in / out.
+// State values per evaluation point (element or Gauss point): J2 the equivalent
+// plastic strain; neo-Hookean with damage the damage and I1 (in that order).
 template <int KIND>
-APHIP_DEV void stress(const double *H, const MatParams &m, double sn, double &snp1, double *P) {
+constexpr int state_width() {
+    return KIND == APHIP_J2 ? 1 : KIND == APHIP_NEO_HOOKEAN_DAMAGE ? 2 : 0;
+}
+
+// The first state value of evaluation point `slot` (the only one a material reads).
+template <int KIND>
+APHIP_DEV double load_state(const double *state, int64_t slot) {
+    if constexpr (state_width<KIND>() == 0) return 0.0;
+    else return state[state_width<KIND>() * slot];
+}
+
+// New state of evaluation point `slot`: snp1 and, for the damage model, aux (I1).
+template <int KIND>
+APHIP_DEV void store_state(double *state, int64_t slot, double snp1, double aux) {
+    if constexpr (state_width<KIND>() == 1) {
+        state[slot] = snp1;
+    } else if constexpr (state_width<KIND>() == 2) {
+        state[2 * slot] = snp1;
+        state[2 * slot + 1] = aux;
+    }
+}
+
+// P = stress(H).  sn / snp1: J2 equivalent plastic strain, or damage, in / out;
+// aux: the damage model's I1.  The smoothed-cell path turns a new failure's marker
+// 2.0 into the maximum damage afterwards (cell_damage_marker).
+template <int KIND>
+APHIP_DEV void stress(const double *H, const MatParams &m, double sn, double &snp1, double &aux, double *P) {
 #pragma clang fp contract(off)
     if constexpr (KIND == APHIP_LINEAR_ELASTIC) {
         const double two_mu = m.mu * 2.0;
@@ -155,6 +185,53 @@ APHIP_DEV void stress(const double *H, const MatParams &m, double sn, double &sn
         for (int i = 0; i < 9; ++i) P[i] = dev[i];
 #pragma unroll
         for (int i = 0; i < 3; ++i) P[4 * i] += p;
+    } else if constexpr (KIND == APHIP_NEO_HOOKEAN_DAMAGE) {
+        // tau = lambda ln J I + mu (B - I), P = tau F^-T as the neo-Hookean branch, then
+        // damage from I1 = tr B (NeoHookeWithDamage.h:187-243)
+        double BmI[9], tau[9], F[9], Fi[9];
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                const double hh = dot3(H[3 * i], H[3 * j], H[3 * i + 1], H[3 * j + 1], H[3 * i + 2], H[3 * j + 2]);
+                BmI[3 * i + j] = hh + H[3 * j + i] + H[3 * i + j];
+                tau[3 * i + j] = m.mu * BmI[3 * i + j];
+            }
+        const double I1 = BmI[0] + BmI[4] + BmI[8] + 3.0;
+        const double lnJ = log1p(det_ap_im1(H));
+#pragma unroll
+        for (int i = 0; i < 3; ++i) tau[4 * i] = __fma_rn(m.lambda, lnJ, tau[4 * i]);
+#pragma unroll
+        for (int i = 0; i < 9; ++i) F[i] = H[i];
+        F[0] += 1.0;
+        F[4] += 1.0;
+        F[8] += 1.0;
+        inverse3(F, Fi);
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+                P[3 * i + j] = dot3(tau[3 * i], Fi[3 * j], tau[3 * i + 1], Fi[3 * j + 1], tau[3 * i + 2], Fi[3 * j + 2]);
+        const double Dn = fmin(fmax(sn, 0.0), m.max_damage);
+        double D = Dn;
+        if (I1 > m.I1_failure) {
+            D = 1.0;
+        } else if (I1 > m.I1_critical) {
+            D = pow((I1 - m.I1_critical) / (m.I1_failure - m.I1_critical), m.alpha);
+            D = fmin(fmax(D, 0.0), 1.0);
+            D = fmax(D, Dn);  // no healing
+        }
+        if (D != 0.0) {
+            D = fmin(D, Dn + m.max_damage_increment);
+            const bool failed = D >= m.max_damage;
+            if (failed) D = m.max_damage;
+            const double keep = 1.0 - D;
+#pragma unroll
+            for (int i = 0; i < 9; ++i) P[i] = P[i] * keep;
+            if (failed && Dn < m.max_damage) D = 2.0;  // marks a new failure
+        }
+        snp1 = D;
+        aux = I1;
     } else {  // SVK
         const double two_mu = m.mu * 2.0;
         double E[9], S[9];
@@ -178,6 +255,14 @@ APHIP_DEV void stress(const double *H, const MatParams &m, double sn, double &sn
                                     S[6 + j], (i == 2 ? 1.0 : 0.0) + H[3 * i + 2]);
         snp1 = sn;
     }
+}
+
+// The smoothed-cell path's handling of a new failure (ComputeInternalForceSmoothedCell.h:
+// 219-223 with NeoHookeWithDamage.h:247-252): the marker becomes the maximum damage.
+template <int KIND>
+APHIP_DEV double cell_damage_marker(double snp1, const MatParams &m) {
+    if constexpr (KIND == APHIP_NEO_HOOKEAN_DAMAGE) return snp1 == 2.0 ? m.max_damage : snp1;
+    else return snp1;
 }
 
 }  // namespace aphip

@@ -49,10 +49,10 @@ APHIP_DEV void cd_correct(const double *f, double m, unsigned fixed, double dt, 
 // data that neighbouring tiles re-read from L2.
 APHIP_DEV void store_once(double *p, double v) { __builtin_nontemporal_store(v, p); }
 
-// Tet4: fe[k][j] = -(B_k . P_j) w
+// Tet4: fe[k][j] = -(B_k . P_j) w; sn, snp1, aux as for stress<KIND>
 template <int KIND>
 APHIP_DEV void tet4_element(const double (&X)[4][3], const double (&U)[4][3], const MatParams &m, double sn,
-                            double &snp1, double *H, double *P, double (&fe)[4][3]) {
+                            double &snp1, double &aux, double *H, double *P, double (&fe)[4][3]) {
 #pragma clang fp contract(off)
     double J[9], Ji[9];
 #pragma unroll
@@ -76,7 +76,7 @@ APHIP_DEV void tet4_element(const double (&X)[4][3], const double (&U)[4][3], co
 #pragma unroll
         for (int j = 0; j < 3; ++j)
             H[3 * i + j] = dot4(U[0][i], B[0][j], U[1][i], B[1][j], U[2][i], B[2][j], U[3][i], B[3][j]);
-    stress<KIND>(H, m, sn, snp1, P);
+    stress<KIND>(H, m, sn, snp1, aux, P);
 #pragma unroll
     for (int k = 0; k < 4; ++k)
 #pragma unroll
@@ -150,7 +150,8 @@ APHIP_DEV void hex8_grad(const double (&w)[8], double p0, double p1, double p2, 
     g[2] = __fma_rn(p01, w[7], __fma_rn(p1, w[6], __fma_rn(p0, w[5], w[4])));
 }
 
-// state_n / state_np1 point at the element's state slot(s) (per_qp: 8 slots).
+// state_n / state_np1 point at the element's state slot(s) (per_qp: 8 slots), each
+// state_width<KIND>() values wide.
 template <int KIND, bool PER_QP>
 APHIP_DEV void hex8_element(const double (&X)[8][3], const double (&U)[8][3], const MatParams &m, double ga,
                             const double *state_n, double *state_np1, double *H, double *P, double (&fe)[8][3]) {
@@ -175,7 +176,7 @@ APHIP_DEV void hex8_element(const double (&X)[8][3], const double (&U)[8][3], co
         for (int S = 0; S < 8; ++S) R[j][S] = 0.0;
     const double ga2 = ga * ga;
     double sn = 0.0;
-    if constexpr (KIND == APHIP_J2 && !PER_QP) sn = state_n[0];
+    if constexpr (!PER_QP) sn = load_state<KIND>(state_n, 0);
 #pragma unroll 1
     for (int q = 0; q < 8; ++q) {
         int sq0, sq1, sq2;
@@ -201,13 +202,11 @@ APHIP_DEV void hex8_element(const double (&X)[8][3], const double (&U)[8][3], co
 #pragma unroll
             for (int j = 0; j < 3; ++j)
                 H[3 * i + j] = dot3(G[3 * i], Ji[j], G[3 * i + 1], Ji[3 + j], G[3 * i + 2], Ji[6 + j]);
-        double snp1 = 0.0;
-        if constexpr (KIND == APHIP_J2 && PER_QP) sn = state_n[q];
-        stress<KIND>(H, m, sn, snp1, P);
-        if constexpr (KIND == APHIP_J2) {
-            if constexpr (PER_QP) state_np1[q] = snp1;
-            else if (q == 7) state_np1[0] = snp1;
-        }
+        double snp1 = 0.0, aux = 0.0;
+        if constexpr (PER_QP) sn = load_state<KIND>(state_n, q);
+        stress<KIND>(H, m, sn, snp1, aux, P);
+        if constexpr (PER_QP) store_state<KIND>(state_np1, q, snp1, aux);
+        else if (q == 7) store_state<KIND>(state_np1, 0, snp1, aux);
 #pragma unroll
         for (int j = 0; j < 3; ++j) {
             // Q[n] = -(w / 8) (J^-1 P^T)[n][j]: the force of axis n's derivative
@@ -284,13 +283,11 @@ APHIP_DEV void hex8_qp_lane(int q, const int (&lc)[8], const double *sX, const d
 #pragma unroll
         for (int j = 0; j < 3; ++j)
             H[3 * i + j] = dot3(G[3 * i], Ji[j], G[3 * i + 1], Ji[3 + j], G[3 * i + 2], Ji[6 + j]);
-    double sn = 0.0, snp1 = 0.0;
-    if constexpr (KIND == APHIP_J2) sn = PER_QP ? state_n[q] : state_n[0];
-    stress<KIND>(H, m, sn, snp1, P);
-    if constexpr (KIND == APHIP_J2) {
-        if constexpr (PER_QP) state_np1[q] = snp1;
-        else if (q == 7) state_np1[0] = snp1;
-    }
+    double sn = 0.0, snp1 = 0.0, aux = 0.0;
+    sn = load_state<KIND>(state_n, PER_QP ? q : 0);
+    stress<KIND>(H, m, sn, snp1, aux, P);
+    if constexpr (PER_QP) store_state<KIND>(state_np1, q, snp1, aux);
+    else if (q == 7) store_state<KIND>(state_np1, 0, snp1, aux);
     double Q[9];
 #pragma unroll
     for (int n = 0; n < 3; ++n)

@@ -81,10 +81,10 @@ __global__ __launch_bounds__(kElemBlock) void k_tet4_force(ElemArgs A) {
         m.lambda = A.elem_lambda[e];
         m.mu = A.elem_mu[e];
     }
-    double sn = 0.0, snp1 = 0.0, H[9], P[9];
-    if constexpr (KIND == APHIP_J2) sn = A.state_n[e];
-    tet4_element<KIND>(X, U, m, sn, snp1, H, P, fe);
-    if constexpr (KIND == APHIP_J2) A.state_np1[e] = snp1;
+    double snp1 = 0.0, aux = 0.0, H[9], P[9];
+    const double sn = load_state<KIND>(A.state_n, e);
+    tet4_element<KIND>(X, U, m, sn, snp1, aux, H, P, fe);
+    store_state<KIND>(A.state_np1, e, snp1, aux);
 #pragma unroll
     for (int i = 0; i < 9; ++i) {
         store_once(A.H + i * A.ne + e, H[i]);
@@ -115,10 +115,10 @@ __global__ __launch_bounds__(128) void k_hex8_force(ElemArgs A) {
         m.lambda = A.elem_lambda[e];
         m.mu = A.elem_mu[e];
     }
-    const int64_t so = PER_QP ? e * 8 : e;
+    const int64_t so = state_width<KIND>() * (PER_QP ? e * 8 : e);
     double H[9], P[9];
-    hex8_element<KIND, PER_QP>(X, U, m, A.gauss_a, KIND == APHIP_J2 ? A.state_n + so : nullptr,
-                               KIND == APHIP_J2 ? A.state_np1 + so : nullptr, H, P, fe);
+    hex8_element<KIND, PER_QP>(X, U, m, A.gauss_a, state_width<KIND>() ? A.state_n + so : nullptr,
+                               state_width<KIND>() ? A.state_np1 + so : nullptr, H, P, fe);
 #pragma unroll
     for (int i = 0; i < 9; ++i) {
         store_once(A.H + i * A.ne + e, H[i]);
@@ -266,7 +266,7 @@ template <int KIND>
 void launch_elem_kind(Ctx &c, const ElemArgs &A) {
     if (c.etype == APHIP_TET4) {
         hipLaunchKernelGGL(k_tet4_force<KIND>, dim3((unsigned)A.nblocks), dim3(kElemBlock), 0, c.stream, A);
-    } else if (KIND == APHIP_J2 && c.mat.state_per_qp) {
+    } else if (state_width<KIND>() && c.mat.state_per_qp) {
         hipLaunchKernelGGL((k_hex8_force<KIND, true>), dim3((unsigned)A.nblocks), dim3(128), 0, c.stream, A);
     } else {
         hipLaunchKernelGGL((k_hex8_force<KIND, false>), dim3((unsigned)A.nblocks), dim3(128), 0, c.stream, A);
@@ -287,7 +287,7 @@ void fem_element_force(Ctx &c) {
     A.state_n = c.state[1 - c.state_np1].p;
     A.state_np1 = c.state[c.state_np1].p;
     A.partial = c.partial.p;
-    A.mat = MatParams{c.mat.lambda, c.mat.mu, c.mat.yield_stress, c.mat.hardening};
+    A.mat = c.mat.params();
     A.elem_lambda = c.has_fields ? c.elem_lambda.p : nullptr;
     A.elem_mu = c.has_fields ? c.elem_mu.p : nullptr;
     A.gauss_a = 1.0 / std::sqrt(3.0);
@@ -297,6 +297,7 @@ void fem_element_force(Ctx &c) {
         case APHIP_NEO_HOOKEAN: launch_elem_kind<APHIP_NEO_HOOKEAN>(c, A); break;
         case APHIP_J2: launch_elem_kind<APHIP_J2>(c, A); break;
         case APHIP_SVK: launch_elem_kind<APHIP_SVK>(c, A); break;
+        case APHIP_NEO_HOOKEAN_DAMAGE: launch_elem_kind<APHIP_NEO_HOOKEAN_DAMAGE>(c, A); break;
         default: throw Error(APHIP_ERR_INVALID, "material not set");
     }
     APHIP_HIP(hipGetLastError());

@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "aperi_hip.h"
+#include "device_math.hpp"
 #include "exodus_format.hpp"
 
 namespace aphip {
@@ -75,6 +76,14 @@ struct MaterialConst {
     double yield_stress = 0.0, hardening = 0.0;
     double E = 0.0, nu = 0.0;
     int state_per_qp = 0;
+    // neo-Hookean with damage (increment and maximum already clamped to [0, 1])
+    double I1_critical = 0.0, I1_failure = 0.0, alpha = 0.0, max_damage = 0.0, max_damage_increment = 0.0;
+    MatParams params() const {
+        return MatParams{lambda, mu, yield_stress, hardening, I1_critical, I1_failure, alpha, max_damage,
+                         max_damage_increment};
+    }
+    // state values per evaluation point
+    int state_width() const { return kind == APHIP_J2 ? 1 : kind == APHIP_NEO_HOOKEAN_DAMAGE ? 2 : 0; }
 };
 
 // Checkpointed forward run for the adjoint (adjoint.hip).
@@ -261,8 +270,7 @@ struct Ctx {
     Profile prof;
 
     int64_t state_count() const {
-        if (mat.kind != APHIP_J2) return 0;
-        return ne * ((etype == APHIP_HEX8 && mat.state_per_qp) ? 8 : 1);
+        return mat.state_width() * ne * ((etype == APHIP_HEX8 && mat.state_per_qp) ? 8 : 1);
     }
 };
 

@@ -609,11 +609,12 @@ __global__ __launch_bounds__(256) void k_rk_cells(CellForceArgs A) {
         m.lambda = A.elem_lambda[c];
         m.mu = A.elem_mu[c];
     }
-    double sn = 0.0, snp1 = 0.0;
-    if constexpr (KIND == APHIP_J2) sn = A.state_n[c];
-    stress<KIND>(H, m, sn, snp1, P);
+    double snp1 = 0.0, aux = 0.0;
+    const double sn = load_state<KIND>(A.state_n, c);
+    stress<KIND>(H, m, sn, snp1, aux, P);
+    snp1 = cell_damage_marker<KIND>(snp1, m);
     if (live && l == 0) {
-        if constexpr (KIND == APHIP_J2) A.state_np1[c] = snp1;
+        store_state<KIND>(A.state_np1, c, snp1, aux);
 #pragma unroll
         for (int i = 0; i < 9; ++i) {
             store_once(A.H + i * A.ne + c, H[i]);
@@ -664,8 +665,8 @@ __global__ __launch_bounds__(256) void k_rk_cells_two_pass(CellForceArgs A, cons
         ld3(X, en[k], X4[k]);
         ld3(A.u, en[k], uj[k]);
     }
-    double sn = 0.0, snp1 = 0.0;
-    if constexpr (KIND == APHIP_J2) sn = A.state_n[c];
+    double snp1 = 0.0, aux = 0.0;
+    const double sn = load_state<KIND>(A.state_n, c);
     MatParams m = A.mat;
     if (A.elem_lambda) {
         m.lambda = A.elem_lambda[c];
@@ -685,9 +686,10 @@ __global__ __launch_bounds__(256) void k_rk_cells_two_pass(CellForceArgs A, cons
         for (int a = 0; a < 3; ++a)
 #pragma unroll
             for (int b = 0; b < 3; ++b) H[3 * a + b] = __fma_rn(uj[k][a], gc[k][b], H[3 * a + b]);
-    stress<KIND>(H, m, sn, snp1, P);
+    stress<KIND>(H, m, sn, snp1, aux, P);
+    snp1 = cell_damage_marker<KIND>(snp1, m);
     if (!live) return;
-    if constexpr (KIND == APHIP_J2) A.state_np1[c] = snp1;
+    store_state<KIND>(A.state_np1, c, snp1, aux);
 #pragma unroll
     for (int i = 0; i < 9; ++i) {
         store_once(A.H + i * A.ne + c, H[i]);
@@ -1097,7 +1099,7 @@ void rk_cell_force(Ctx &c) {
     CellForceArgs A{c.ne, nbc, D.start.p, D.nodes.p, D.dphi.p, D.volume.p, D.two_pass ? D.u_phys.p : c.u.p,
                     c.H.p, c.P.p, D.g.p, D.entry_pos.p,
                     c.state[1 - c.state_np1].p, c.state[c.state_np1].p,
-                    MatParams{c.mat.lambda, c.mat.mu, c.mat.yield_stress, c.mat.hardening},
+                    c.mat.params(),
                     c.has_fields ? c.elem_lambda.p : nullptr, c.has_fields ? c.elem_mu.p : nullptr};
     const dim3 g((unsigned)nbc);
     if (D.two_pass) {
@@ -1107,6 +1109,9 @@ void rk_cell_force(Ctx &c) {
             case APHIP_NEO_HOOKEAN: hipLaunchKernelGGL(k_rk_cells_two_pass<APHIP_NEO_HOOKEAN>, g, dim3(256), 0, c.stream, A, X); break;
             case APHIP_J2: hipLaunchKernelGGL(k_rk_cells_two_pass<APHIP_J2>, g, dim3(256), 0, c.stream, A, X); break;
             case APHIP_SVK: hipLaunchKernelGGL(k_rk_cells_two_pass<APHIP_SVK>, g, dim3(256), 0, c.stream, A, X); break;
+            case APHIP_NEO_HOOKEAN_DAMAGE:
+                hipLaunchKernelGGL(k_rk_cells_two_pass<APHIP_NEO_HOOKEAN_DAMAGE>, g, dim3(256), 0, c.stream, A, X);
+                break;
             default: throw Error(APHIP_ERR_INVALID, "material not set");
         }
     } else {
@@ -1115,6 +1120,9 @@ void rk_cell_force(Ctx &c) {
             case APHIP_NEO_HOOKEAN: hipLaunchKernelGGL((k_rk_cells<APHIP_NEO_HOOKEAN, kCellLanes>), g, dim3(256), 0, c.stream, A); break;
             case APHIP_J2: hipLaunchKernelGGL((k_rk_cells<APHIP_J2, kCellLanes>), g, dim3(256), 0, c.stream, A); break;
             case APHIP_SVK: hipLaunchKernelGGL((k_rk_cells<APHIP_SVK, kCellLanes>), g, dim3(256), 0, c.stream, A); break;
+            case APHIP_NEO_HOOKEAN_DAMAGE:
+                hipLaunchKernelGGL((k_rk_cells<APHIP_NEO_HOOKEAN_DAMAGE, kCellLanes>), g, dim3(256), 0, c.stream, A);
+                break;
             default: throw Error(APHIP_ERR_INVALID, "material not set");
         }
     }

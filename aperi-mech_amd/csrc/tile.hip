@@ -213,7 +213,7 @@ __global__ __launch_bounds__(NPE == 4 ? 1024 : (QPAR ? 512 : 256), NPE == 8 && !
         lc[4] = c1.x; lc[5] = c1.y; lc[6] = c1.z; lc[7] = c1.w;
     }
     load_elem_lame(A, ec, m);
-    if constexpr (KIND == APHIP_J2 && !PER_QP && !QPAR) sn = A.state_n[ec];
+    if constexpr (NPE == 4) sn = load_state<KIND>(A.state_n, ec);
 
     // A: tile CSR entries and node data into LDS
     {
@@ -288,10 +288,10 @@ __global__ __launch_bounds__(NPE == 4 ? 1024 : (QPAR ? 512 : 256), NPE == 8 && !
     if constexpr (QPAR && NPE == 8) {
         if (has_elem) {
             const int q = tid & 7;
-            const int64_t so = PER_QP ? e * 8 : e;
+            const int64_t so = state_width<KIND>() * (PER_QP ? e * 8 : e);
             double H[9], P[9];
-            hex8_qp_lane<KIND, PER_QP>(q, lc, L.X, L.U, m, A.gauss_a, KIND == APHIP_J2 ? A.state_n + so : nullptr,
-                                       KIND == APHIP_J2 ? A.state_np1 + so : nullptr, H, P, fe[0]);
+            hex8_qp_lane<KIND, PER_QP>(q, lc, L.X, L.U, m, A.gauss_a, state_width<KIND>() ? A.state_n + so : nullptr,
+                                       state_width<KIND>() ? A.state_np1 + so : nullptr, H, P, fe[0]);
             if (q == 7) {
 #pragma unroll
                 for (int i = 0; i < 9; ++i) {
@@ -310,13 +310,13 @@ __global__ __launch_bounds__(NPE == 4 ? 1024 : (QPAR ? 512 : 256), NPE == 8 && !
         }
         double H[9], P[9];
         if constexpr (NPE == 4) {
-            double snp1 = 0.0;
-            tet4_element<KIND>(X, U, m, sn, snp1, H, P, fe);
-            if constexpr (KIND == APHIP_J2) A.state_np1[e] = snp1;
+            double snp1 = 0.0, aux = 0.0;
+            tet4_element<KIND>(X, U, m, sn, snp1, aux, H, P, fe);
+            store_state<KIND>(A.state_np1, e, snp1, aux);
         } else {
-            const int64_t so = PER_QP ? e * 8 : e;
-            hex8_element<KIND, PER_QP>(X, U, m, A.gauss_a, KIND == APHIP_J2 ? A.state_n + so : nullptr,
-                                       KIND == APHIP_J2 ? A.state_np1 + so : nullptr, H, P, fe);
+            const int64_t so = state_width<KIND>() * (PER_QP ? e * 8 : e);
+            hex8_element<KIND, PER_QP>(X, U, m, A.gauss_a, state_width<KIND>() ? A.state_n + so : nullptr,
+                                       state_width<KIND>() ? A.state_np1 + so : nullptr, H, P, fe);
         }
 #pragma unroll
         for (int i = 0; i < 9; ++i) {
@@ -634,8 +634,9 @@ __global__ __launch_bounds__(256) void k_tile_stream(TileArgs A, int nblk) {
                 ld3(S.X, lc[k], X[k]);
                 ld3(S.U, lc[k], U[k]);
             }
-            double H[9], P[9], snp1 = 0.0;
-            tet4_element<KIND>(X, U, m, S.SN[tid], snp1, H, P, fe);
+            static_assert(state_width<KIND>() <= 1, "the pipelined kernel keeps one state value per element");
+            double H[9], P[9], snp1 = 0.0, aux = 0.0;
+            tet4_element<KIND>(X, U, m, S.SN[tid], snp1, aux, H, P, fe);
 #pragma unroll
             for (int k = 0; k < 4; ++k) st3(F, tid * 4 + k, fe[k]);
             // commit the prefetched tile before this tile's stores are issued
@@ -825,6 +826,10 @@ void dispatch_kind(Ctx &c, const TileArgs &A, size_t lds) {
             else launch_tile<NPE, APHIP_J2, false, MODE>(c, A, lds);
             break;
         case APHIP_SVK: launch_tile<NPE, APHIP_SVK, false, MODE>(c, A, lds); break;
+        case APHIP_NEO_HOOKEAN_DAMAGE:
+            if (pq) launch_tile<NPE, APHIP_NEO_HOOKEAN_DAMAGE, true, MODE>(c, A, lds);
+            else launch_tile<NPE, APHIP_NEO_HOOKEAN_DAMAGE, false, MODE>(c, A, lds);
+            break;
         default: throw Error(APHIP_ERR_INVALID, "material not set");
     }
 }
@@ -1028,7 +1033,7 @@ void tile_run(Ctx &c, double dt, int mode) {
     A.P = c.P.p;
     A.state_n = c.state[1 - c.state_np1].p;
     A.state_np1 = c.state[c.state_np1].p;
-    A.mat = MatParams{c.mat.lambda, c.mat.mu, c.mat.yield_stress, c.mat.hardening};
+    A.mat = c.mat.params();
     A.elem_lambda = c.has_fields ? c.elem_lambda.p : nullptr;
     A.elem_mu = c.has_fields ? c.elem_mu.p : nullptr;
     A.node_dt = c.node_dt.p;

@@ -20,7 +20,10 @@ _PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.normpath(os.path.join(_PKG, "..", "..", "lib", "libaperi_hip.so"))
 
 TET4, HEX8 = 0, 1
-LINEAR_ELASTIC, NEO_HOOKEAN, J2, SVK = 0, 1, 2, 3
+LINEAR_ELASTIC, NEO_HOOKEAN, J2, SVK, NEO_HOOKEAN_DAMAGE = 0, 1, 2, 3, 4
+# optional neo-Hookean-with-damage keys and the reference's defaults (Materials/Factory.h:78-95)
+DAMAGE_DEFAULTS = dict(maximum_damage=0.99, maximum_damage_increment=1.0, stop_failing_at_time=-1.0,
+                       fail_all_at_time=-1.0)
 FIELD_U, FIELD_V, FIELD_A, FIELD_F, FIELD_MASS, FIELD_H, FIELD_P, FIELD_STATE, FIELD_VOLUME = range(9)
 FIELD_RK_DISPLACEMENT, FIELD_RK_FORCE_LOCAL = 9, 10
 MAX_NODE_NUM_NEIGHBORS = 40
@@ -53,7 +56,20 @@ class MaterialParams(ctypes.Structure):
     _fields_ = [("density", ctypes.c_double), ("youngs_modulus", ctypes.c_double),
                 ("poissons_ratio", ctypes.c_double), ("yield_stress", ctypes.c_double),
                 ("hardening_modulus", ctypes.c_double), ("state_per_qp", ctypes.c_int32),
-                ("reserved", ctypes.c_int32)]
+                ("reserved", ctypes.c_int32),
+                ("I1_critical", ctypes.c_double), ("I1_failure", ctypes.c_double), ("alpha", ctypes.c_double),
+                ("maximum_damage", ctypes.c_double), ("maximum_damage_increment", ctypes.c_double),
+                ("stop_failing_at_time", ctypes.c_double), ("fail_all_at_time", ctypes.c_double)]
+
+
+def material_params(density, E, nu, yield_stress=0.0, hardening=0.0, state_per_qp=0, damage=None):
+    """aphip_material_params; damage: dict with I1_critical, I1_failure, alpha and optionally
+    maximum_damage, maximum_damage_increment, stop_failing_at_time, fail_all_at_time."""
+    p = MaterialParams(density, E, nu, yield_stress, hardening, state_per_qp, 0)
+    if damage is not None:
+        for k, v in dict(DAMAGE_DEFAULTS, **damage).items():
+            setattr(p, k, v)
+    return p
 
 
 class AphipError(RuntimeError):
@@ -151,8 +167,8 @@ class Solver:
         _check(self.L.aphip_get_node_permutation(self.ctx, _p(out, _i32)))
         return out
 
-    def set_material(self, kind, density, E, nu, yield_stress=0.0, hardening=0.0, state_per_qp=0):
-        p = MaterialParams(density, E, nu, yield_stress, hardening, state_per_qp, 0)
+    def set_material(self, kind, density, E, nu, yield_stress=0.0, hardening=0.0, state_per_qp=0, damage=None):
+        p = material_params(density, E, nu, yield_stress, hardening, state_per_qp, damage)
         _check(self.L.aphip_set_material(self.ctx, kind, ctypes.byref(p)))
         self.kind, self.state_per_qp = kind, state_per_qp
         return self
@@ -192,7 +208,8 @@ class Solver:
         if field in (FIELD_H, FIELD_P):
             return (self.ne, 9)
         if field == FIELD_STATE:
-            return (self.ne * (8 if (self.etype == HEX8 and self.state_per_qp) else 1),)
+            width = 2 if self.kind == NEO_HOOKEAN_DAMAGE else 1
+            return (width * self.ne * (8 if (self.etype == HEX8 and self.state_per_qp) else 1),)
         return (self.ne,)
 
     def get(self, field):

@@ -39,13 +39,14 @@ typedef struct aphip_ctx aphip_ctx;
 /* Element topology: include/Constants.h:23-25 (ElementTopology). */
 enum { APHIP_TET4 = 0, APHIP_HEX8 = 1 };
 
-/* Material kinds: include/Materials/{LinearElastic,NeoHooke,Plastic,Elastic}.h */
-enum { APHIP_LINEAR_ELASTIC = 0, APHIP_NEO_HOOKEAN = 1, APHIP_J2 = 2, APHIP_SVK = 3 };
+/* Material kinds: include/Materials/{LinearElastic,NeoHooke,Plastic,Elastic,NeoHookeWithDamage}.h */
+enum { APHIP_LINEAR_ELASTIC = 0, APHIP_NEO_HOOKEAN = 1, APHIP_J2 = 2, APHIP_SVK = 3, APHIP_NEO_HOOKEAN_DAMAGE = 4 };
 
 /* Fields for aphip_get_field / aphip_set_field.
  * Node fields (n_nodes x 3): U, V, A, F; MASS is n_nodes x 1.
  * Element fields (n_elem x 9): H (displacement_gradient), P (pk1_stress);
- * STATE is n_elem x 1 (J2 equivalent plastic strain; n_elem x 8 with state_per_qp);
+ * STATE is n_elem x 1 (J2 equivalent plastic strain; n_elem x 8 with state_per_qp) or
+ * n_elem x 2 (neo-Hookean with damage: damage, I1; n_elem x 16 with state_per_qp);
  * VOLUME is n_elem x 1. */
 enum {
     APHIP_FIELD_U = 0,
@@ -83,8 +84,20 @@ typedef struct {
     double poissons_ratio;
     double yield_stress;      /* J2 only */
     double hardening_modulus; /* J2 only */
-    int32_t state_per_qp;     /* J2 + Hex8: one state per Gauss point (extension; 0 = reference) */
+    int32_t state_per_qp;     /* J2 or damage + Hex8: one state per Gauss point (extension; 0 = reference) */
     int32_t reserved;
+    /* neo-Hookean with damage (include/Materials/NeoHookeWithDamage.h; YAML keys of
+     * include/Materials/Factory.h:70-101, whose defaults for the optional ones are
+     * maximum_damage 0.99, maximum_damage_increment 1, the two times -1).  The two times
+     * only label separations for the reference's optional material separator and do
+     * not change any field here. */
+    double I1_critical;
+    double I1_failure;
+    double alpha;
+    double maximum_damage;
+    double maximum_damage_increment;
+    double stop_failing_at_time;
+    double fail_all_at_time;
 } aphip_material_params;
 
 /* ---- lifetime ---------------------------------------------------------- */

@@ -166,7 +166,8 @@ int or_internal_force(int etype, int64_t n_nodes, const double *X, int64_t n_ele
     if (etype != OR_TET4 && etype != OR_HEX8) return -1;
     const int npe = etype == OR_TET4 ? 4 : 8;
     const int nq = etype == OR_TET4 ? 1 : 8;
-    const int has_state = mat->kind == OR_J2;
+    const int sw = or_state_width(mat->kind);
+    const int has_state = sw > 0;
     const int per_qp = has_state && mat->state_per_qp && etype == OR_HEX8;
     memset(f, 0, sizeof(double) * 3 * n_nodes);
 #pragma omp parallel for schedule(static) num_threads(nthreads > 0 ? nthreads : 1)
@@ -191,9 +192,9 @@ int or_internal_force(int etype, int64_t n_nodes, const double *X, int64_t n_ele
                     for (int k = 0; k < npe; ++k) s += ue[3 * k + i] * B[3 * k + j];
                     He[3 * i + j] = s;
                 }
-            const int64_t sidx = per_qp ? e * 8 + q : e;
-            or_stress(mat->kind, lam, mu, mat->yield_stress, mat->hardening_modulus, He,
-                      has_state ? &state_n[sidx] : NULL, has_state ? &state_np1[sidx] : NULL, Pe);
+            const int64_t sidx = (per_qp ? e * 8 + q : e) * sw;
+            or_stress_mat(mat, lam, mu, He, has_state ? &state_n[sidx] : NULL, has_state ? &state_np1[sidx] : NULL,
+                          Pe);
             for (int k = 0; k < npe; ++k)
                 for (int j = 0; j < 3; ++j) {
                     double s = 0.0;

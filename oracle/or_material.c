@@ -6,6 +6,11 @@
  *   neo-Hookean     include/Materials/NeoHooke.h:67-92 (InvertMatrix = FullPivLU, MathUtils.h:307-319)
  *   J2 plasticity   include/Materials/Plastic.h:88-140 (small strain, Simo-Hughes box 7.5)
  *   SVK "elastic"   include/Materials/Elastic.h:61-79
+ *   NH with damage  include/Materials/NeoHookeWithDamage.h:180-243 (state: damage, I1; damage
+ *                   from I1 between I1_critical and I1_failure with exponent alpha, never
+ *                   decreasing, at most maximum_damage_increment per evaluation, capped at
+ *                   maximum_damage, where a new failure is marked 2.0; the constructor's
+ *                   clamping of the increment to [0, 1], :153-168)
  *
  * Pinning (tests/test_oracle_materials.py):
  *   LE   textbook Hooke's law on random symmetric strains; P independent of skew(H)
@@ -14,6 +19,10 @@
  *        rotation invariance P(RF) = R P(F); small-strain limit -> LE
  *   J2   1-element uniaxial radial return with hardening, analytic plastic strain
  *   SVK  PatchTestFixture.h:138-169 closed form P = F S with E = 1, nu = 0
+ *   NH with damage (no reference test exists): below I1_critical identical to NH with
+ *        state (0, I1); uniaxial stretch F = diag(l, 1, 1): I1 = l^2 + 2 and
+ *        P = (1 - ((I1 - Ic) / (If - Ic))^alpha) P_NH; the increment cap, no decrease on
+ *        unloading, the failure marker then the clamped damage on the next evaluation
  */
 #include <math.h>
 
@@ -97,4 +106,63 @@ void or_stress(int kind, double lambda, double mu, double yield_stress, double h
                 P[3 * i + j] = s;
             }
     }
+}
+
+int or_state_width(int kind) { return kind == OR_J2 ? 1 : kind == OR_NH_DAMAGE ? 2 : 0; }
+
+static double clampd(double x, double lo, double hi) { return x < lo ? lo : (x > hi ? hi : x); }
+
+static void nh_damage(const or_material *mat, double lambda, double mu, const double H[9], const double *state_n,
+                      double *state_np1, double P[9]) {
+    const double dmax = clampd(mat->maximum_damage, 0.0, 1.0);
+    const double dinc = clampd(mat->maximum_damage_increment, 0.0, 1.0);
+    const double Dn = clampd(state_n ? state_n[0] : 0.0, 0.0, dmax);
+    double BmI[9], F[9], Finv[9], tau[9];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            double hh = 0.0;
+            for (int k = 0; k < 3; ++k) hh += H[3 * i + k] * H[3 * j + k];
+            BmI[3 * i + j] = hh + H[3 * j + i] + H[3 * i + j];
+        }
+    const double I1 = BmI[0] + BmI[4] + BmI[8] + 3.0;
+    if (state_np1) state_np1[1] = I1;
+    const double lnJ = log1p(or_det_ap_im1(H));
+    for (int i = 0; i < 9; ++i) tau[i] = mu * BmI[i];
+    for (int i = 0; i < 3; ++i) tau[4 * i] += lambda * lnJ;
+    for (int i = 0; i < 9; ++i) F[i] = H[i];
+    F[0] += 1.0; F[4] += 1.0; F[8] += 1.0;
+    or_fullpivlu_inverse(3, F, Finv);
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            double s = 0.0;
+            for (int k = 0; k < 3; ++k) s += tau[3 * i + k] * Finv[3 * j + k];
+            P[3 * i + j] = s;
+        }
+    double D = Dn;
+    if (I1 > mat->I1_failure) {
+        D = 1.0;
+    } else if (I1 > mat->I1_critical) {
+        D = pow((I1 - mat->I1_critical) / (mat->I1_failure - mat->I1_critical), mat->alpha);
+        D = clampd(D, 0.0, 1.0);
+        D = D > Dn ? D : Dn;
+    }
+    if (D == 0.0) {
+        if (state_np1) state_np1[0] = 0.0;
+        return;
+    }
+    D = D < Dn + dinc ? D : Dn + dinc;
+    int failed = 0;
+    if (D >= dmax) {
+        failed = 1;
+        D = dmax;
+    }
+    for (int i = 0; i < 9; ++i) P[i] *= 1.0 - D;
+    if (failed && Dn < dmax) D = 2.0;
+    if (state_np1) state_np1[0] = D;
+}
+
+void or_stress_mat(const or_material *mat, double lambda, double mu, const double H[9], const double *state_n,
+                   double *state_np1, double P[9]) {
+    if (mat->kind == OR_NH_DAMAGE) nh_damage(mat, lambda, mu, H, state_n, state_np1, P);
+    else or_stress(mat->kind, lambda, mu, mat->yield_stress, mat->hardening_modulus, H, state_n, state_np1, P);
 }

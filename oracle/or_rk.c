@@ -278,7 +278,8 @@ int or_cell_internal_force(int64_t n_nodes, int64_t n_cells, const int64_t *cell
                            const double *cell_dphi, const double *cell_volume, const or_material *mat,
                            const double *u, const double *state_n, double *state_np1, double *H, double *P,
                            double *f, int nthreads) {
-    const int has_state = mat->kind == OR_J2;
+    const int sw = or_state_width(mat->kind);
+    const int has_state = sw > 0;
     memset(f, 0, sizeof(double) * 3 * n_nodes);
 #pragma omp parallel for schedule(static) num_threads(nthreads > 0 ? nthreads : 1)
     for (int64_t c = 0; c < n_cells; ++c) {
@@ -290,8 +291,12 @@ int or_cell_internal_force(int64_t n_nodes, int64_t n_cells, const int64_t *cell
             for (int a = 0; a < 3; ++a)
                 for (int b = 0; b < 3; ++b) He[3 * a + b] += u[3 * j + a] * cell_dphi[3 * t + b];
         }
-        or_stress(mat->kind, lam, mu, mat->yield_stress, mat->hardening_modulus, He,
-                  has_state ? &state_n[c] : NULL, has_state ? &state_np1[c] : NULL, Pe);
+        or_stress_mat(mat, lam, mu, He, has_state ? &state_n[c * sw] : NULL, has_state ? &state_np1[c * sw] : NULL,
+                      Pe);
+        /* a new failure's marker becomes the maximum damage in the smoothed-cell path
+         * (ComputeInternalForceSmoothedCell.h:219-223, NeoHookeWithDamage.h:247-252) */
+        if (mat->kind == OR_NH_DAMAGE && state_np1[c * sw] == 2.0)
+            state_np1[c * sw] = mat->maximum_damage < 0.0 ? 0.0 : (mat->maximum_damage > 1.0 ? 1.0 : mat->maximum_damage);
         if (H) memcpy(&H[9 * c], He, sizeof(He));
         if (P) memcpy(&P[9 * c], Pe, sizeof(Pe));
         const double V = cell_volume[c];

@@ -130,7 +130,9 @@ double or_power_method_dt(const or_problem *p, int n_iter, double epsilon, uint6
     double *up = (double *)calloc((size_t)M, sizeof(double));
     double *f0 = (double *)calloc((size_t)M, sizeof(double));
     double *f1 = (double *)calloc((size_t)M, sizeof(double));
-    const int64_t ns = p->mat.kind == OR_J2 ? (p->etype == OR_HEX8 && p->mat.state_per_qp ? 8 : 1) * p->n_elem : 1;
+    const int64_t ns = or_state_width(p->mat.kind) > 0
+                           ? or_state_width(p->mat.kind) * (p->etype == OR_HEX8 && p->mat.state_per_qp ? 8 : 1) * p->n_elem
+                           : 1;
     double *st = (double *)calloc((size_t)ns, sizeof(double));
     double *st2 = (double *)calloc((size_t)ns, sizeof(double));
     double lambda = 0.0;
@@ -192,7 +194,9 @@ int or_power_method(const or_problem *p, const double *u_in, const double *state
                     double previous_dt, double *eigenvector, int eigenvector_valid, int nthreads, double *dt_out,
                     int *iterations_out, int *converged_out) {
     const int64_t N = p->n_nodes, M = 3 * N;
-    const int64_t ns = p->mat.kind == OR_J2 ? (p->etype == OR_HEX8 && p->mat.state_per_qp ? 8 : 1) * p->n_elem : 1;
+    const int64_t ns = or_state_width(p->mat.kind) > 0
+                           ? or_state_width(p->mat.kind) * (p->etype == OR_HEX8 && p->mat.state_per_qp ? 8 : 1) * p->n_elem
+                           : 1;
     double *vec = (double *)calloc((size_t)M, sizeof(double));
     double *up = (double *)calloc((size_t)M, sizeof(double));
     double *f0 = (double *)calloc((size_t)M, sizeof(double));
@@ -201,7 +205,7 @@ int or_power_method(const or_problem *p, const double *u_in, const double *state
     double *snp1 = (double *)calloc((size_t)ns, sizeof(double));
     int rc = -1;
     if (!vec || !up || !f0 || !f1 || !sn || !snp1) goto done;
-    if (state_n && p->mat.kind == OR_J2) memcpy(sn, state_n, sizeof(double) * (size_t)ns);
+    if (state_n && or_state_width(p->mat.kind) > 0) memcpy(sn, state_n, sizeof(double) * (size_t)ns);
     if (eigenvector && eigenvector_valid) {
         memcpy(vec, eigenvector, sizeof(double) * (size_t)M);
     } else {

@@ -26,7 +26,7 @@ extern "C" {
 #endif
 
 enum { OR_TET4 = 0, OR_HEX8 = 1 };
-enum { OR_LINEAR_ELASTIC = 0, OR_NEO_HOOKEAN = 1, OR_J2 = 2, OR_SVK = 3 };
+enum { OR_LINEAR_ELASTIC = 0, OR_NEO_HOOKEAN = 1, OR_J2 = 2, OR_SVK = 3, OR_NH_DAMAGE = 4 };
 
 /* Material: uniform parameters, optionally overridden per element. */
 typedef struct {
@@ -36,7 +36,13 @@ typedef struct {
     double yield_stress, hardening_modulus;
     const double *elem_lambda; /* optional [n_elem] */
     const double *elem_mu;     /* optional [n_elem] */
+    /* neo-Hookean with damage (include/Materials/NeoHookeWithDamage.h) */
+    double I1_critical, I1_failure, alpha, maximum_damage, maximum_damage_increment;
 } or_material;
+
+/* State values per evaluation point: J2 1 (equivalent plastic strain), neo-Hookean with
+ * damage 2 (damage, I1), others 0. */
+int or_state_width(int kind);
 
 /* ---- math (or_math.c) ---- */
 int or_lame_from_E_nu(double E, double nu, double *lambda, double *mu, double *K);
@@ -47,6 +53,10 @@ int or_fullpivlu_inverse(int n, const double *A, double *Ainv);
 int or_elastic_properties(int given_a, double a, int given_b, double b, double out[5]);
 
 /* ---- materials (or_material.c) ---- */
+/* Stress of the material `mat` with Lame parameters lambda, mu (per element);
+ * state_n / state_np1 hold or_state_width(kind) values (NULL when 0). */
+void or_stress_mat(const or_material *mat, double lambda, double mu, const double H[9], const double *state_n,
+                   double *state_np1, double P[9]);
 void or_stress(int kind, double lambda, double mu, double yield_stress, double hardening,
                const double H[9], const double *state_n, double *state_np1, double P[9]);
 

@@ -15,7 +15,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = None
 
 TET4, HEX8 = 0, 1
-LINEAR_ELASTIC, NEO_HOOKEAN, J2, SVK = 0, 1, 2, 3
+LINEAR_ELASTIC, NEO_HOOKEAN, J2, SVK, NH_DAMAGE = 0, 1, 2, 3, 4
 MAX_NODE_NUM_NEIGHBORS = 40  # include/Constants.h:15
 
 _d = ctypes.POINTER(ctypes.c_double)
@@ -28,7 +28,9 @@ class Material(ctypes.Structure):
     _fields_ = [("kind", ctypes.c_int32), ("state_per_qp", ctypes.c_int32),
                 ("lam", ctypes.c_double), ("mu", ctypes.c_double),
                 ("yield_stress", ctypes.c_double), ("hardening_modulus", ctypes.c_double),
-                ("elem_lambda", _d), ("elem_mu", _d)]
+                ("elem_lambda", _d), ("elem_mu", _d),
+                ("I1_critical", ctypes.c_double), ("I1_failure", ctypes.c_double), ("alpha", ctypes.c_double),
+                ("maximum_damage", ctypes.c_double), ("maximum_damage_increment", ctypes.c_double)]
 
 
 class Problem(ctypes.Structure):
@@ -149,29 +151,55 @@ def smoothed_hex8_bmatrix(X):
     return B.reshape(8, 3), V.value
 
 
-def _material(kind, lam, mu, yield_stress=0.0, hardening=0.0, elem_lambda=None, elem_mu=None, state_per_qp=0):
+DAMAGE_DEFAULTS = dict(maximum_damage=0.99, maximum_damage_increment=1.0)  # Materials/Factory.h:78-81
+
+
+def _material(kind, lam, mu, yield_stress=0.0, hardening=0.0, elem_lambda=None, elem_mu=None, state_per_qp=0,
+              damage=None):
+    """damage: dict of I1_critical, I1_failure, alpha[, maximum_damage, maximum_damage_increment]
+    for NH_DAMAGE."""
     m = Material()
     m.kind, m.state_per_qp = kind, state_per_qp
     m.lam, m.mu, m.yield_stress, m.hardening_modulus = lam, mu, yield_stress, hardening
     m.elem_lambda = _p(elem_lambda)
     m.elem_mu = _p(elem_mu)
+    if damage is not None:
+        d = dict(DAMAGE_DEFAULTS, **damage)
+        for k in ("I1_critical", "I1_failure", "alpha", "maximum_damage", "maximum_damage_increment"):
+            setattr(m, k, d[k])
     return m
 
 
+def state_width(kind):
+    return {J2: 1, NH_DAMAGE: 2}.get(kind, 0)
+
+
 def state_size(etype, kind, n_elem, state_per_qp=0):
-    if kind != J2:
+    w = state_width(kind)
+    if w == 0:
         return 1
-    return n_elem * (8 if (etype == HEX8 and state_per_qp) else 1)
+    return w * n_elem * (8 if (etype == HEX8 and state_per_qp) else 1)
+
+
+def stress_damage(lam, mu, H, damage, state_n=(0.0, 0.0)):
+    """neo-Hookean with damage: (P, (damage, I1)) from H and the previous damage."""
+    mat = _material(NH_DAMAGE, lam, mu, damage=damage)
+    H = _f64(H).reshape(9)
+    P = np.zeros(9)
+    sn = _f64(np.asarray(state_n, dtype=np.float64))
+    snp1 = np.zeros(2)
+    lib().or_stress_mat(ctypes.byref(mat), ctypes.c_double(lam), ctypes.c_double(mu), _p(H), _p(sn), _p(snp1), _p(P))
+    return P.reshape(3, 3), snp1
 
 
 def internal_force(etype, X, conn, kind, lam, mu, u, state_n=None, yield_stress=0.0, hardening=0.0,
-                   elem_lambda=None, elem_mu=None, state_per_qp=0, nthreads=1):
+                   elem_lambda=None, elem_mu=None, state_per_qp=0, nthreads=1, damage=None):
     """f = -sum_e int B^T P dV, with H, P (last Gauss point) and the new state."""
     X, conn, u = _f64(X), _i32a(conn), _f64(u)
     ne, nn = conn.shape[0], X.shape[0]
     el = None if elem_lambda is None else _f64(elem_lambda)
     em = None if elem_mu is None else _f64(elem_mu)
-    mat = _material(kind, lam, mu, yield_stress, hardening, el, em, state_per_qp)
+    mat = _material(kind, lam, mu, yield_stress, hardening, el, em, state_per_qp, damage)
     ns = state_size(etype, kind, ne, state_per_qp)
     sn = np.zeros(ns) if state_n is None else _f64(state_n).copy()
     snp1 = np.zeros(ns)
@@ -205,7 +233,8 @@ class Solver:
     """The explicit loop of src/Solvers/Explicit.cpp:308-452 on the CPU oracle."""
 
     def __init__(self, etype, X, conn, kind, lam, mu, mass, fixed=None, yield_stress=0.0, hardening=0.0,
-                 elem_lambda=None, elem_mu=None, node_dt=None, state_per_qp=0, cells=None, two_pass=None):
+                 elem_lambda=None, elem_mu=None, node_dt=None, state_per_qp=0, cells=None, two_pass=None,
+                 damage=None):
         """cells: RK smoothed cells (start, nodes, dphi, volume); two_pass: (num, nbr, phi)
         of the RK functions, with `cells` from rk_cells_two_pass, for the two-pass force."""
         self.X = _f64(X)
@@ -236,7 +265,7 @@ class Solver:
         else:
             p.n_elem = self.conn.shape[0]
             p.conn = _p(self.conn, _i32)
-        p.mat = _material(kind, lam, mu, yield_stress, hardening, self.el, self.em, state_per_qp)
+        p.mat = _material(kind, lam, mu, yield_stress, hardening, self.el, self.em, state_per_qp, damage)
         p.mass = _p(self.mass)
         p.fixed = _p(self.fixed, _u8)
         p.node_dt = _p(self.node_dt)
@@ -348,12 +377,13 @@ def rk_cells(X, conn, num, nbr, phi):
 
 
 def cell_internal_force(n_nodes, cells, kind, lam, mu, u, state_n=None, yield_stress=0.0, hardening=0.0,
-                        nthreads=1):
+                        nthreads=1, damage=None):
     start, nodes, dphi, vol = cells
     nc = vol.shape[0]
-    mat = _material(kind, lam, mu, yield_stress, hardening)
-    sn = np.zeros(nc) if state_n is None else _f64(state_n).copy()
-    snp1 = np.zeros(nc)
+    mat = _material(kind, lam, mu, yield_stress, hardening, damage=damage)
+    w = max(state_width(kind), 1)
+    sn = np.zeros(nc * w) if state_n is None else _f64(state_n).copy()
+    snp1 = np.zeros(nc * w)
     H, P, f = np.zeros((nc, 9)), np.zeros((nc, 9)), np.zeros((n_nodes, 3))
     lib().or_cell_internal_force(n_nodes, nc, _p(np.ascontiguousarray(start, dtype=np.int64), _i64),
                                  _p(_i32a(nodes), _i32), _p(_f64(dphi)), _p(_f64(vol)), ctypes.byref(mat),
@@ -390,10 +420,11 @@ def rk_scatter(num, nbr, phi, src):
 
 
 def rk_two_pass_force(n_nodes, cells2, num, nbr, phi, kind, lam, mu, d, state_n=None, yield_stress=0.0,
-                      hardening=0.0):
+                      hardening=0.0, damage=None):
     """Two-pass internal force of coefficient displacements d: (f, H, P, state, u_phys, f_local)."""
     up = rk_gather(num, nbr, phi, d)
-    fl, H, P, s = cell_internal_force(n_nodes, cells2, kind, lam, mu, up, state_n, yield_stress, hardening)
+    fl, H, P, s = cell_internal_force(n_nodes, cells2, kind, lam, mu, up, state_n, yield_stress, hardening,
+                                      damage=damage)
     return rk_scatter(num, nbr, phi, fl), H, P, s, up, fl
 
 

@@ -107,3 +107,88 @@ def test_svk_patch_closed_form():
     F = np.eye(3) + H
     Egl = 0.5 * (F.T @ F - np.eye(3))
     assert np.allclose(P, F @ (lam * np.trace(Egl) * np.eye(3) + 2 * mu * Egl), rtol=1e-14, atol=1e-15)
+
+
+# ---- neo-Hookean with damage (include/Materials/NeoHookeWithDamage.h:180-243) ----
+# The reference has no test of this material; the closed forms below pin it.
+
+DMG = dict(I1_critical=3.1, I1_failure=4.0, alpha=1.5)
+
+
+def _uniaxial_nh(lam, mu, l):
+    # F = diag(l, 1, 1): tau = lam ln(l) I + mu (B - I), P = tau F^-T
+    P = np.zeros((3, 3))
+    P[0, 0] = (lam * np.log(l) + mu * (l * l - 1.0)) / l
+    P[1, 1] = P[2, 2] = lam * np.log(l)
+    return P
+
+
+def test_damage_free_is_neo_hookean():
+    lam, mu = O.lame(1e9, 0.3)
+    rng = np.random.default_rng(7)
+    for _ in range(20):
+        H = 0.01 * rng.standard_normal((3, 3))
+        P, st = O.stress_damage(lam, mu, H, DMG)
+        Pn, _ = O.stress(O.NEO_HOOKEAN, lam, mu, H)
+        assert np.array_equal(P, Pn)
+        F = np.eye(3) + H
+        assert st[0] == 0.0 and st[1] == pytest.approx(np.trace(F @ F.T), rel=1e-14)
+
+
+@pytest.mark.parametrize("l", [1.05, 1.1, 1.2])
+def test_damage_uniaxial_closed_form(l):
+    lam, mu = O.lame(2e9, 0.25)
+    H = np.diag([l - 1.0, 0.0, 0.0])
+    P, st = O.stress_damage(lam, mu, H, DMG)
+    I1 = l * l + 2.0
+    D = ((I1 - DMG["I1_critical"]) / (DMG["I1_failure"] - DMG["I1_critical"])) ** DMG["alpha"] if I1 > 3.1 else 0.0
+    assert st[1] == pytest.approx(I1, rel=1e-15)
+    assert st[0] == pytest.approx(D, rel=1e-14)
+    assert np.allclose(P, (1.0 - D) * _uniaxial_nh(lam, mu, l), rtol=1e-13, atol=1e-6)
+
+
+def test_damage_increment_cap_and_no_healing():
+    lam, mu = O.lame(2e9, 0.25)
+    H = np.diag([0.2, 0.0, 0.0])  # I1 = 3.44: D = (0.34 / 0.9)^1.5 = 0.2322
+    full = ((1.2 ** 2 + 2 - 3.1) / 0.9) ** 1.5
+    P, st = O.stress_damage(lam, mu, H, dict(DMG, maximum_damage_increment=0.05), state_n=(0.1, 0.0))
+    assert st[0] == pytest.approx(0.15, rel=1e-15)  # min(D, Dn + 0.05)
+    assert np.allclose(P, 0.85 * _uniaxial_nh(lam, mu, 1.2), rtol=1e-13)
+    P, st = O.stress_damage(lam, mu, H, DMG, state_n=(0.1, 0.0))
+    assert st[0] == pytest.approx(full, rel=1e-14)
+    # unloading below the critical I1 keeps the damage and its softening
+    Hs = np.diag([0.01, 0.0, 0.0])
+    P, st = O.stress_damage(lam, mu, Hs, DMG, state_n=(0.4, 0.0))
+    assert st[0] == 0.4 and np.allclose(P, 0.6 * _uniaxial_nh(lam, mu, 1.01), rtol=1e-13)
+    # a larger stored damage is never reduced by a smaller evolution value
+    P, st = O.stress_damage(lam, mu, H, DMG, state_n=(0.5, 0.0))
+    assert st[0] == 0.5
+
+
+def test_damage_failure_marker_and_cap():
+    lam, mu = O.lame(2e9, 0.25)
+    H = np.diag([0.5, 0.0, 0.0])  # I1 = 4.25 > I1_failure
+    P, st = O.stress_damage(lam, mu, H, DMG)
+    assert st[0] == 2.0  # new failure (NeoHookeWithDamage.h:238-240)
+    assert np.allclose(P, 0.01 * _uniaxial_nh(lam, mu, 1.5), rtol=1e-13)
+    P, st = O.stress_damage(lam, mu, H, DMG, state_n=(2.0, 0.0))
+    assert st[0] == 0.99  # clamped on input, no longer new
+    assert np.allclose(P, 0.01 * _uniaxial_nh(lam, mu, 1.5), rtol=1e-13)
+    P, st = O.stress_damage(lam, mu, H, dict(DMG, maximum_damage=0.9))
+    assert st[0] == 2.0 and np.allclose(P, 0.1 * _uniaxial_nh(lam, mu, 1.5), rtol=1e-13)
+
+
+def test_damage_in_elements_and_cells():
+    # the element loop stores (damage, I1) per element; the smoothed-cell path turns a
+    # new failure's marker into the maximum damage (ComputeInternalForceSmoothedCell.h:219-223)
+    X = np.array([[0.0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]])
+    t = np.array([[0, 1, 2, 3]])
+    lam, mu = O.lame(2e9, 0.25)
+    u = X @ np.diag([0.5, 0.0, 0.0]).T
+    f, H, P, st = O.internal_force(O.TET4, X, t, O.NH_DAMAGE, lam, mu, u, damage=DMG)
+    assert st.shape == (2,) and st[0] == 2.0 and st[1] == pytest.approx(4.25, rel=1e-14)
+    B, V = O.smoothed_tet4_bmatrix(X)
+    cells = (np.array([0, 4]), np.arange(4, dtype=np.int32), B, np.array([V]))
+    fc, Hc, Pc, sc = O.cell_internal_force(4, cells, O.NH_DAMAGE, lam, mu, u, damage=DMG)
+    assert sc[0] == 0.99 and sc[1] == pytest.approx(4.25, rel=1e-14)
+    assert np.allclose(fc, f, rtol=1e-12, atol=1e-9 * np.abs(f).max())
