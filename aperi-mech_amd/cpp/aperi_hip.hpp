@@ -152,6 +152,7 @@ class Material {
             case PLASTIC: m_kind = APHIP_J2; break;
             case ELASTIC: m_kind = APHIP_SVK; break;
             case NEO_HOOKEAN_WITH_DAMAGE: m_kind = APHIP_NEO_HOOKEAN_DAMAGE; break;
+            case POWER_LAW_CREEP: m_kind = APHIP_POWER_LAW_CREEP; break;
             default: throw std::runtime_error("Material type not supported by the MI355X force step");
         }
         if (m_kind == APHIP_J2) {
@@ -167,13 +168,27 @@ class Material {
             m_damage[5] = Optional("stop_failing_at_time", -1.0);
             m_damage[6] = Optional("fail_all_at_time", -1.0);
         }
+        if (m_kind == APHIP_POWER_LAW_CREEP) {  // Materials/Factory.h:134-153
+            m_creep[0] = Required("A");
+            m_creep[1] = Required("n");
+            m_creep[2] = Required("m");
+            m_creep[3] = Optional("constant_temperature", 0.0);
+            m_creep_const_t = Optional("use_constant_temperature", 0.0) != 0.0 ? 1 : 0;
+        }
     }
     MaterialType GetMaterialType() const { return m_props->material_type; }
     std::shared_ptr<MaterialProperties> GetMaterialProperties() const { return m_props; }
     double GetDensity() const { return m_props->density; }
-    bool HasState() const { return m_kind == APHIP_J2 || m_kind == APHIP_NEO_HOOKEAN_DAMAGE; }
-    // the reference's count (NeoHookeWithDamage.h:279-282 reports 1, the damage)
-    size_t NumberOfStateVariables() const { return HasState() ? 1 : 0; }
+    bool HasState() const {
+        return m_kind == APHIP_J2 || m_kind == APHIP_NEO_HOOKEAN_DAMAGE || m_kind == APHIP_POWER_LAW_CREEP;
+    }
+    // the reference's counts (NeoHookeWithDamage.h:279-282 reports 1, the damage;
+    // PowerLawCreep.h:186-189 reports 2)
+    size_t NumberOfStateVariables() const {
+        return m_kind == APHIP_POWER_LAW_CREEP ? 2 : HasState() ? 1 : 0;
+    }
+    bool NeedsVelocityGradient() const { return m_kind == APHIP_POWER_LAW_CREEP; }
+    bool NeedsOldStress() const { return m_kind == APHIP_POWER_LAW_CREEP; }
     // values per element of the state field (the damage model also stores I1)
     size_t StateFieldWidth() const { return m_kind == APHIP_NEO_HOOKEAN_DAMAGE ? 2 : NumberOfStateVariables(); }
     int Kind() const { return m_kind; }
@@ -181,7 +196,8 @@ class Material {
         const LinearElasticProperties &le = m_props->linear_elastic_properties;
         return aphip_material_params{m_props->density, le.youngs_modulus, le.poisson_ratio, m_yield, m_hardening,
                                      state_per_qp, 0, m_damage[0], m_damage[1], m_damage[2], m_damage[3],
-                                     m_damage[4], m_damage[5], m_damage[6]};
+                                     m_damage[4], m_damage[5], m_damage[6], m_creep[0], m_creep[1], m_creep[2],
+                                     m_creep[3], m_creep_const_t, 0};
     }
 
    private:
@@ -198,6 +214,8 @@ class Material {
     int m_kind = -1;
     double m_yield = 0.0, m_hardening = 0.0;
     double m_damage[7] = {0, 0, 0, 0, 0, 0, 0};
+    double m_creep[4] = {0, 0, 0, 0};
+    int32_t m_creep_const_t = 0;
 };
 
 inline std::shared_ptr<Material> CreateMaterial(std::shared_ptr<MaterialProperties> p) {

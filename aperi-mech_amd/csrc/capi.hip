@@ -114,7 +114,7 @@ static std::vector<int32_t> morton_order(const std::vector<double> &pts, int64_t
 }
 
 static void resolve_material(int kind, const aphip_material_params *p, MaterialConst &m) {
-    APHIP_REQUIRE(kind >= APHIP_LINEAR_ELASTIC && kind <= APHIP_NEO_HOOKEAN_DAMAGE, "unknown material kind");
+    APHIP_REQUIRE(kind >= APHIP_LINEAR_ELASTIC && kind <= APHIP_POWER_LAW_CREEP, "unknown material kind");
     APHIP_REQUIRE(p != nullptr, "material params are NULL");
     const double E = p->youngs_modulus, nu = p->poissons_ratio;
     // LinearElasticProperties::CompleteProperties, E/nu branch and bound checks
@@ -147,6 +147,20 @@ static void resolve_material(int kind, const aphip_material_params *p, MaterialC
         m.max_damage = p->maximum_damage;
         m.max_damage_increment = std::min(std::max(p->maximum_damage_increment, 0.0), 1.0);
     }
+    if (kind == APHIP_POWER_LAW_CREEP) {
+        // PowerLawCreepGetStressFunctor's checks (PowerLawCreep.h:80-90); bulk modulus of the
+        // E / nu pair (LinearElasticProperties::CompleteProperties)
+        APHIP_REQUIRE(p->creep_A >= 0.0, "PowerLawCreep: A must be > 0");
+        APHIP_REQUIRE(p->creep_n >= 0.0, "PowerLawCreep: n must be > 0");
+        APHIP_REQUIRE(p->creep_m >= 0.0, "PowerLawCreep: m must be > 0");
+        APHIP_REQUIRE(p->use_constant_temperature != 0,
+                      "PowerLawCreep: currently have to use constant temperature, but use_constant_temperature is "
+                      "false");
+        const double T = p->constant_temperature;
+        m.bulk = E / (3 * (1 - 2 * nu));
+        m.creep_rate = T == 0.0 ? p->creep_A * std::exp(-p->creep_m) : p->creep_A * std::exp(-p->creep_m / T);
+        m.creep_n = p->creep_n;
+    }
 }
 
 static void alloc_state(Ctx &c) {
@@ -158,6 +172,15 @@ static void alloc_state(Ctx &c) {
         } else {
             c.state[s].release();
         }
+    }
+    if (c.mat.rate_form()) {  // the N halves of the H / P pairs start at zero
+        c.H1.alloc(9 * std::max<int64_t>(c.ne, 1));
+        c.P1.alloc(9 * std::max<int64_t>(c.ne, 1));
+        c.H1.zero(c.stream);
+        c.P1.zero(c.stream);
+    } else {
+        c.H1.release();
+        c.P1.release();
     }
     c.state_np1 = 1;
 }
@@ -335,6 +358,7 @@ void step_once_tiles(Ctx &c, double dt);
 // one step of the explicit loop body (src/Solvers/Explicit.cpp:391-452)
 static void step_once(Ctx &c, double dt) {
     c.state_np1 = 1 - c.state_np1;  // UpdateFieldStates (Explicit.cpp:393)
+    c.rate_dt = dt;                 // SetTimeIncrement (ElementTetrahedron4.h:187)
     if (use_tiles(c)) {
         tile_run(c, dt, 2);
     } else {
@@ -401,8 +425,8 @@ static PowerMethodResult power_method(Ctx &c, double prev_dt, int n_iter, double
     if (c.ne) {
         H.alloc(9 * c.ne);
         P.alloc(9 * c.ne);
-        APHIP_HIP(hipMemcpyAsync(H.p, c.H.p, sizeof(double) * 9 * c.ne, hipMemcpyDeviceToDevice, c.stream));
-        APHIP_HIP(hipMemcpyAsync(P.p, c.P.p, sizeof(double) * 9 * c.ne, hipMemcpyDeviceToDevice, c.stream));
+        APHIP_HIP(hipMemcpyAsync(H.p, c.Hnp1().p, sizeof(double) * 9 * c.ne, hipMemcpyDeviceToDevice, c.stream));
+        APHIP_HIP(hipMemcpyAsync(P.p, c.Pnp1().p, sizeof(double) * 9 * c.ne, hipMemcpyDeviceToDevice, c.stream));
     }
     if (ns) {
         st.alloc(ns);
@@ -472,8 +496,8 @@ static PowerMethodResult power_method(Ctx &c, double prev_dt, int n_iter, double
     APHIP_HIP(hipMemcpyAsync(c.u.p, u_in.p, sizeof(double) * n3, hipMemcpyDeviceToDevice, c.stream));
     APHIP_HIP(hipMemcpyAsync(c.f.p, f_in.p, sizeof(double) * n3, hipMemcpyDeviceToDevice, c.stream));
     if (c.ne) {
-        APHIP_HIP(hipMemcpyAsync(c.H.p, H.p, sizeof(double) * 9 * c.ne, hipMemcpyDeviceToDevice, c.stream));
-        APHIP_HIP(hipMemcpyAsync(c.P.p, P.p, sizeof(double) * 9 * c.ne, hipMemcpyDeviceToDevice, c.stream));
+        APHIP_HIP(hipMemcpyAsync(c.Hnp1().p, H.p, sizeof(double) * 9 * c.ne, hipMemcpyDeviceToDevice, c.stream));
+        APHIP_HIP(hipMemcpyAsync(c.Pnp1().p, P.p, sizeof(double) * 9 * c.ne, hipMemcpyDeviceToDevice, c.stream));
     }
     if (ns)
         APHIP_HIP(hipMemcpyAsync(c.state[c.state_np1].p, st.p, sizeof(double) * ns, hipMemcpyDeviceToDevice,
@@ -747,7 +771,7 @@ int aphip_get_field(aphip_ctx *c, int32_t field, double *out) {
         APHIP_HIP(hipStreamSynchronize(c->stream));
         for (int64_t i = 0; i < c->nn; ++i) out[c->node_iperm[i]] = h[i];
     } else if (field == APHIP_FIELD_H || field == APHIP_FIELD_P) {
-        const DevBuf<double> &b = field == APHIP_FIELD_H ? c->H : c->P;
+        const DevBuf<double> &b = field == APHIP_FIELD_H ? c->Hnp1() : c->Pnp1();
         std::vector<double> h(9 * c->ne);
         b.download(h.data(), h.size(), c->stream);
         APHIP_HIP(hipStreamSynchronize(c->stream));
@@ -787,7 +811,8 @@ int aphip_internal_force(aphip_ctx *c, double time, double dt) {
     APHIP_REQUIRE(c != nullptr, "ctx is NULL");
     require_ready(*c, false);
     (void)time;
-    (void)dt;
+    APHIP_REQUIRE(std::isfinite(dt) && dt >= 0.0, "time increment must be finite and non-negative");
+    c->rate_dt = dt;
     APHIP_HIP(hipSetDevice(c->device));
     internal_force_eval(*c);
     APHIP_HIP(hipStreamSynchronize(c->stream));
@@ -899,8 +924,8 @@ int aphip_autotune(aphip_ctx *c, double dt, int32_t trial_steps, int32_t *chosen
     std::vector<std::vector<double>> ssave;
     {
         std::vector<double> h(9 * c->ne);
-        c->H.download(sH.data(), sH.size(), c->stream);
-        c->P.download(sP.data(), sP.size(), c->stream);
+        c->Hnp1().download(sH.data(), sH.size(), c->stream);
+        c->Pnp1().download(sP.data(), sP.size(), c->stream);
         if (ns) {
             c->state[0].download(s0.data(), ns, c->stream);
             c->state[1].download(s1.data(), ns, c->stream);
@@ -953,8 +978,8 @@ int aphip_autotune(aphip_ctx *c, double dt, int32_t trial_steps, int32_t *chosen
     set_node_field(*c, c->v, sv.data());
     set_node_field(*c, c->a, sa.data());
     set_node_field(*c, c->f, sf.data());
-    c->H.upload(sH.data(), sH.size(), c->stream);
-    c->P.upload(sP.data(), sP.size(), c->stream);
+    c->Hnp1().upload(sH.data(), sH.size(), c->stream);
+    c->Pnp1().upload(sP.data(), sP.size(), c->stream);
     if (ns) {
         c->state[0].upload(s0.data(), ns, c->stream);
         c->state[1].upload(s1.data(), ns, c->stream);

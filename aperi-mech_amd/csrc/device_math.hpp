@@ -7,6 +7,7 @@
 //   J2              include/Materials/Plastic.h:88-140
 //   SVK             include/Materials/Elastic.h:61-79
 //   NH with damage  include/Materials/NeoHookeWithDamage.h:180-243
+//   power-law creep include/Materials/PowerLawCreep.h:101-173 (hypoelastic, rate form)
 // The material is a compile-time template parameter; the reference's device
 // virtual call (ComputeInternalForceGaussian.h:107) is resolved on the host.
 #pragma once
@@ -82,13 +83,41 @@ struct MatParams {
     double lambda, mu, yield_stress, hardening;
     // neo-Hookean with damage; max_damage and max_damage_increment lie in [0, 1]
     double I1_critical, I1_failure, alpha, max_damage, max_damage_increment;
+    // power-law creep: bulk modulus and the creep rate factor at the constant
+    // temperature, a = A exp(-m / T) (A exp(-m) at T = 0), and the exponent n
+    double bulk, creep_rate, creep_n;
 };
+
+// Rate-form materials read the previous step's displacement gradient and stress.
+template <int KIND>
+constexpr bool rate_form() {
+    return KIND == APHIP_POWER_LAW_CREEP;
+}
+
+// Inputs of a rate-form update: H_n, P_n (row-major) and the time increment.
+struct RateIn {
+    double Hn[9], Pn[9];
+    double dt;
+};
+
+// H_n, P_n of element e from the SoA (9 x ne) N buffers; nothing for other materials.
+template <int KIND>
+APHIP_DEV void load_rate(RateIn &r, const double *Hn, const double *Pn, int64_t ne, int64_t e, double dt) {
+    if constexpr (rate_form<KIND>()) {
+#pragma unroll
+        for (int i = 0; i < 9; ++i) {
+            r.Hn[i] = Hn[i * ne + e];
+            r.Pn[i] = Pn[i * ne + e];
+        }
+        r.dt = dt;
+    }
+}
 
 // State values per evaluation point (element or Gauss point): J2 the equivalent
 // plastic strain; neo-Hookean with damage the damage and I1 (in that order).
 template <int KIND>
 constexpr int state_width() {
-    return KIND == APHIP_J2 ? 1 : KIND == APHIP_NEO_HOOKEAN_DAMAGE ? 2 : 0;
+    return KIND == APHIP_J2 ? 1 : (KIND == APHIP_NEO_HOOKEAN_DAMAGE || KIND == APHIP_POWER_LAW_CREEP) ? 2 : 0;
 }
 
 // The first state value of evaluation point `slot` (the only one a material reads).
@@ -109,11 +138,119 @@ APHIP_DEV void store_state(double *state, int64_t slot, double snp1, double aux)
     }
 }
 
-// P = stress(H).  sn / snp1: J2 equivalent plastic strain, or damage, in / out;
-// aux: the damage model's I1.  The smoothed-cell path turns a new failure's marker
-// 2.0 into the maximum damage afterwards (cell_damage_marker).
+APHIP_DEV double tr3(const double *A) { return A[0] + A[4] + A[8]; }
+
+// sqrt(A : A), summed in row-major order
+APHIP_DEV double magnitude9(const double *A) {
+#pragma clang fp contract(off)
+    double s = 0.0;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) s = __fma_rn(A[i], A[i], s);
+    return sqrt(s);
+}
+
+// Power-law creep (PowerLawCreep.h:101-173).  Cauchy stress from the old PK1 stress,
+// sigma = P_n F^T / J; rate of deformation from L = (H - H_n) / dt (I + H_n)^-1
+// (ComputeInternalForceBase.h:182-187; L = 0 when dt = 0); one explicit sub-step of
+// the deviator over min(critt, dt) (the reference's loop leaves after its first pass:
+// tm1 = min(critt, dt) <= dt), pressure from the bulk modulus; P = J sigma F^-T.
+// State: effective creep strain (accumulated), effective stress rate.
+APHIP_DEV void creep_stress(const double *H, const MatParams &m, const RateIn &r, double sn, double &snp1,
+                            double &rate, double *P) {
+#pragma clang fp contract(off)
+    double F[9], Fi[9], Fn[9], Fni[9], L[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        F[i] = H[i];
+        Fn[i] = r.Hn[i];
+    }
+    F[0] += 1.0;
+    F[4] += 1.0;
+    F[8] += 1.0;
+    Fn[0] += 1.0;
+    Fn[4] += 1.0;
+    Fn[8] += 1.0;
+    const double J = inverse3(F, Fi);
+    inverse3(Fn, Fni);
+    const double dt = r.dt;
+    auto per_dt = [dt](double x) { return dt != 0.0 ? x / dt : 0.0; };
+    double dH[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) dH[i] = per_dt(H[i] - r.Hn[i]);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) L[3 * i + j] = dot3(dH[3 * i], Fni[j], dH[3 * i + 1], Fni[3 + j], dH[3 * i + 2], Fni[6 + j]);
+    // sigma = (1/J) P_n F^T
+    const double iJ = 1.0 / J;
+    double sig[9];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            sig[3 * i + j] = dot3(iJ * r.Pn[3 * i], F[3 * j], iJ * r.Pn[3 * i + 1], F[3 * j + 1], iJ * r.Pn[3 * i + 2],
+                                  F[3 * j + 2]);
+    const double G = m.mu;
+    double sd[9], de[9], ded[9], cs[9];
+    const double sm = tr3(sig) / 3.0;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) de[3 * i + j] = (0.5 * (L[3 * i + j] + L[3 * j + i])) * dt;
+    const double dm = tr3(de) / 3.0;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        sd[i] = sig[i];
+        ded[i] = de[i];
+    }
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        sd[4 * i] -= sm;
+        ded[4 * i] -= dm;
+    }
+#pragma unroll
+    for (int i = 0; i < 9; ++i) cs[i] = ded[i] + sd[i] / 2.0 / G;
+    const double s32 = sqrt(3.0 / 2.0);
+    double eff = magnitude9(sd) * s32;
+    double edot = m.creep_rate * pow(eff, m.creep_n - 1.0);
+    double critt = edot == 0.0 ? 1.1 * dt : 4.0 * 0.98 / (6.0 * G * m.creep_n * edot);
+    const double delt = fmin(critt, dt);
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        const double dp = 1.5 * sd[i] * edot;
+        const double sdot = 2.0 * G * (per_dt(ded[i]) - dp);
+        sd[i] = __fma_rn(sdot, delt, sd[i]);
+    }
+    edot = m.creep_rate * pow(magnitude9(sd) * s32, m.creep_n - 1.0);
+    const double mean = __fma_rn(m.bulk, tr3(de), tr3(sig) / 3.0);
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        sig[i] = sd[i];
+        cs[i] -= sd[i] / 2.0 / G;
+    }
+#pragma unroll
+    for (int i = 0; i < 3; ++i) sig[4 * i] += mean;
+    snp1 = __fma_rn(sqrt(2.0 / 3.0), magnitude9(cs), sn);
+    double ds[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) ds[i] = 2.0 * G * (per_dt(ded[i]) - 3.0 * edot * sd[i] / 2.0);
+    rate = s32 * magnitude9(ds);
+    // P = (J sigma) F^-T
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            P[3 * i + j] = dot3(J * sig[3 * i], Fi[3 * j], J * sig[3 * i + 1], Fi[3 * j + 1], J * sig[3 * i + 2],
+                                Fi[3 * j + 2]);
+}
+
+// P = stress(H).  sn / snp1: J2 equivalent plastic strain, damage, or effective creep
+// strain, in / out; aux: the damage model's I1 or the creep model's effective stress
+// rate.  The smoothed-cell path turns a new failure's marker 2.0 into the maximum
+// damage afterwards (cell_damage_marker).  r: the rate-form inputs (creep only).
 template <int KIND>
-APHIP_DEV void stress(const double *H, const MatParams &m, double sn, double &snp1, double &aux, double *P) {
+APHIP_DEV void stress(const double *H, const MatParams &m, double sn, double &snp1, double &aux, double *P,
+                      const RateIn *r = nullptr) {
 #pragma clang fp contract(off)
     if constexpr (KIND == APHIP_LINEAR_ELASTIC) {
         const double two_mu = m.mu * 2.0;
@@ -232,6 +369,8 @@ APHIP_DEV void stress(const double *H, const MatParams &m, double sn, double &sn
         }
         snp1 = D;
         aux = I1;
+    } else if constexpr (KIND == APHIP_POWER_LAW_CREEP) {
+        creep_stress(H, m, *r, sn, snp1, aux, P);
     } else {  // SVK
         const double two_mu = m.mu * 2.0;
         double E[9], S[9];

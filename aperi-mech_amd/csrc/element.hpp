@@ -49,10 +49,11 @@ APHIP_DEV void cd_correct(const double *f, double m, unsigned fixed, double dt, 
 // data that neighbouring tiles re-read from L2.
 APHIP_DEV void store_once(double *p, double v) { __builtin_nontemporal_store(v, p); }
 
-// Tet4: fe[k][j] = -(B_k . P_j) w; sn, snp1, aux as for stress<KIND>
+// Tet4: fe[k][j] = -(B_k . P_j) w; sn, snp1, aux, r as for stress<KIND>
 template <int KIND>
 APHIP_DEV void tet4_element(const double (&X)[4][3], const double (&U)[4][3], const MatParams &m, double sn,
-                            double &snp1, double &aux, double *H, double *P, double (&fe)[4][3]) {
+                            double &snp1, double &aux, double *H, double *P, double (&fe)[4][3],
+                            const RateIn *r = nullptr) {
 #pragma clang fp contract(off)
     double J[9], Ji[9];
 #pragma unroll
@@ -76,7 +77,7 @@ APHIP_DEV void tet4_element(const double (&X)[4][3], const double (&U)[4][3], co
 #pragma unroll
         for (int j = 0; j < 3; ++j)
             H[3 * i + j] = dot4(U[0][i], B[0][j], U[1][i], B[1][j], U[2][i], B[2][j], U[3][i], B[3][j]);
-    stress<KIND>(H, m, sn, snp1, aux, P);
+    stress<KIND>(H, m, sn, snp1, aux, P, r);
 #pragma unroll
     for (int k = 0; k < 4; ++k)
 #pragma unroll
@@ -151,10 +152,12 @@ APHIP_DEV void hex8_grad(const double (&w)[8], double p0, double p1, double p2, 
 }
 
 // state_n / state_np1 point at the element's state slot(s) (per_qp: 8 slots), each
-// state_width<KIND>() values wide.
+// state_width<KIND>() values wide.  r: the element's H_n, P_n (those of its last Gauss
+// point in the previous step, like the reference's element fields) for every point.
 template <int KIND, bool PER_QP>
 APHIP_DEV void hex8_element(const double (&X)[8][3], const double (&U)[8][3], const MatParams &m, double ga,
-                            const double *state_n, double *state_np1, double *H, double *P, double (&fe)[8][3]) {
+                            const double *state_n, double *state_np1, double *H, double *P, double (&fe)[8][3],
+                            const RateIn *r = nullptr) {
 #pragma clang fp contract(off)
     double XM[3][8], UM[3][8];
 #pragma unroll
@@ -204,7 +207,7 @@ APHIP_DEV void hex8_element(const double (&X)[8][3], const double (&U)[8][3], co
                 H[3 * i + j] = dot3(G[3 * i], Ji[j], G[3 * i + 1], Ji[3 + j], G[3 * i + 2], Ji[6 + j]);
         double snp1 = 0.0, aux = 0.0;
         if constexpr (PER_QP) sn = load_state<KIND>(state_n, q);
-        stress<KIND>(H, m, sn, snp1, aux, P);
+        stress<KIND>(H, m, sn, snp1, aux, P, r);
         if constexpr (PER_QP) store_state<KIND>(state_np1, q, snp1, aux);
         else if (q == 7) store_state<KIND>(state_np1, 0, snp1, aux);
 #pragma unroll
@@ -243,7 +246,7 @@ APHIP_DEV void hex8_element(const double (&X)[8][3], const double (&U)[8][3], co
 template <int KIND, bool PER_QP>
 APHIP_DEV void hex8_qp_lane(int q, const int (&lc)[8], const double *sX, const double *sU, const MatParams &m,
                             double ga, const double *state_n, double *state_np1, double *H, double *P,
-                            double (&fk)[3]) {
+                            double (&fk)[3], const RateIn *r = nullptr) {
 #pragma clang fp contract(off)
     int sq0, sq1, sq2;
     hex8_sign(q, sq0, sq1, sq2);
@@ -285,7 +288,7 @@ APHIP_DEV void hex8_qp_lane(int q, const int (&lc)[8], const double *sX, const d
             H[3 * i + j] = dot3(G[3 * i], Ji[j], G[3 * i + 1], Ji[3 + j], G[3 * i + 2], Ji[6 + j]);
     double sn = 0.0, snp1 = 0.0, aux = 0.0;
     sn = load_state<KIND>(state_n, PER_QP ? q : 0);
-    stress<KIND>(H, m, sn, snp1, aux, P);
+    stress<KIND>(H, m, sn, snp1, aux, P, r);
     if constexpr (PER_QP) store_state<KIND>(state_np1, q, snp1, aux);
     else if (q == 7) store_state<KIND>(state_np1, 0, snp1, aux);
     double Q[9];

@@ -49,6 +49,8 @@ struct ElemArgs {
     int64_t nblocks;
     double *H;
     double *P;
+    const double *Hn, *Pn;  // rate-form materials: H and P of the previous step
+    double rate_dt;
     const double *state_n;
     double *state_np1;
     double *partial;
@@ -83,7 +85,9 @@ __global__ __launch_bounds__(kElemBlock) void k_tet4_force(ElemArgs A) {
     }
     double snp1 = 0.0, aux = 0.0, H[9], P[9];
     const double sn = load_state<KIND>(A.state_n, e);
-    tet4_element<KIND>(X, U, m, sn, snp1, aux, H, P, fe);
+    RateIn rin;
+    load_rate<KIND>(rin, A.Hn, A.Pn, A.ne, e, A.rate_dt);
+    tet4_element<KIND>(X, U, m, sn, snp1, aux, H, P, fe, &rin);
     store_state<KIND>(A.state_np1, e, snp1, aux);
 #pragma unroll
     for (int i = 0; i < 9; ++i) {
@@ -117,8 +121,10 @@ __global__ __launch_bounds__(128) void k_hex8_force(ElemArgs A) {
     }
     const int64_t so = state_width<KIND>() * (PER_QP ? e * 8 : e);
     double H[9], P[9];
+    RateIn rin;
+    load_rate<KIND>(rin, A.Hn, A.Pn, A.ne, e, A.rate_dt);
     hex8_element<KIND, PER_QP>(X, U, m, A.gauss_a, state_width<KIND>() ? A.state_n + so : nullptr,
-                               state_width<KIND>() ? A.state_np1 + so : nullptr, H, P, fe);
+                               state_width<KIND>() ? A.state_np1 + so : nullptr, H, P, fe, &rin);
 #pragma unroll
     for (int i = 0; i < 9; ++i) {
         store_once(A.H + i * A.ne + e, H[i]);
@@ -282,8 +288,11 @@ void fem_element_force(Ctx &c) {
     A.conn = c.conn.p;
     A.ne = c.ne;
     A.nblocks = padded_blocks(c.ne, c.etype == APHIP_TET4 ? kElemBlock : 128);
-    A.H = c.H.p;
-    A.P = c.P.p;
+    A.H = c.Hnp1().p;
+    A.P = c.Pnp1().p;
+    A.Hn = c.mat.rate_form() ? c.Hn().p : nullptr;
+    A.Pn = c.mat.rate_form() ? c.Pn().p : nullptr;
+    A.rate_dt = c.rate_dt;
     A.state_n = c.state[1 - c.state_np1].p;
     A.state_np1 = c.state[c.state_np1].p;
     A.partial = c.partial.p;
@@ -298,6 +307,7 @@ void fem_element_force(Ctx &c) {
         case APHIP_J2: launch_elem_kind<APHIP_J2>(c, A); break;
         case APHIP_SVK: launch_elem_kind<APHIP_SVK>(c, A); break;
         case APHIP_NEO_HOOKEAN_DAMAGE: launch_elem_kind<APHIP_NEO_HOOKEAN_DAMAGE>(c, A); break;
+        case APHIP_POWER_LAW_CREEP: launch_elem_kind<APHIP_POWER_LAW_CREEP>(c, A); break;
         default: throw Error(APHIP_ERR_INVALID, "material not set");
     }
     APHIP_HIP(hipGetLastError());

@@ -78,12 +78,18 @@ struct MaterialConst {
     int state_per_qp = 0;
     // neo-Hookean with damage (increment and maximum already clamped to [0, 1])
     double I1_critical = 0.0, I1_failure = 0.0, alpha = 0.0, max_damage = 0.0, max_damage_increment = 0.0;
+    // power-law creep: bulk modulus, rate factor A exp(-m / T), exponent
+    double bulk = 0.0, creep_rate = 0.0, creep_n = 0.0;
     MatParams params() const {
         return MatParams{lambda, mu, yield_stress, hardening, I1_critical, I1_failure, alpha, max_damage,
-                         max_damage_increment};
+                         max_damage_increment, bulk, creep_rate, creep_n};
     }
     // state values per evaluation point
-    int state_width() const { return kind == APHIP_J2 ? 1 : kind == APHIP_NEO_HOOKEAN_DAMAGE ? 2 : 0; }
+    int state_width() const {
+        return kind == APHIP_J2 ? 1 : (kind == APHIP_NEO_HOOKEAN_DAMAGE || kind == APHIP_POWER_LAW_CREEP) ? 2 : 0;
+    }
+    // reads the previous displacement gradient and stress (velocity gradient, old stress)
+    bool rate_form() const { return kind == APHIP_POWER_LAW_CREEP; }
 };
 
 // Checkpointed forward run for the adjoint (adjoint.hip).
@@ -230,8 +236,16 @@ struct Ctx {
     DevBuf<uint8_t> fixed;               // bit d set = DOF d held at zero
     DevBuf<int32_t> conn;                // AoS ne x npe, internal node ids
     DevBuf<double> H, P;                 // SoA 9 x ne
-    DevBuf<double> state[2];             // J2 equivalent plastic strain (N / NP1)
+    DevBuf<double> H1, P1;               // rate-form materials: the second of the N / NP1 pairs
+    DevBuf<double> state[2];             // material state (N / NP1)
     int state_np1 = 1;                   // index of the NP1 state buffer
+    double rate_dt = 0.0;                // increment for force-only evaluations of rate-form materials
+    // H and P of the current evaluation (NP1) and of the previous step (N, rate form
+    // only); the pairs swap with the state buffers
+    DevBuf<double> &Hnp1() { return mat.rate_form() && state_np1 == 0 ? H1 : H; }
+    DevBuf<double> &Pnp1() { return mat.rate_form() && state_np1 == 0 ? P1 : P; }
+    DevBuf<double> &Hn() { return state_np1 == 0 ? H : H1; }
+    DevBuf<double> &Pn() { return state_np1 == 0 ? P : P1; }
     DevBuf<double> volume;               // ne
     DevBuf<int32_t> node_start;          // nn + 1
     DevBuf<int32_t> node_inc;            // ne * npe: element * npe + slot, ascending element

@@ -527,6 +527,8 @@ struct CellForceArgs {
     double *state_np1;
     MatParams mat;
     const double *elem_lambda, *elem_mu;
+    const double *Hn, *Pn;  // rate-form materials: H and P of the previous step
+    double rate_dt;
 };
 
 constexpr int kCellCache = 6;  // entries per lane whose grad phi stays in registers
@@ -611,7 +613,9 @@ __global__ __launch_bounds__(256) void k_rk_cells(CellForceArgs A) {
     }
     double snp1 = 0.0, aux = 0.0;
     const double sn = load_state<KIND>(A.state_n, c);
-    stress<KIND>(H, m, sn, snp1, aux, P);
+    RateIn rin;
+    load_rate<KIND>(rin, A.Hn, A.Pn, A.ne, c, A.rate_dt);
+    stress<KIND>(H, m, sn, snp1, aux, P, &rin);
     snp1 = cell_damage_marker<KIND>(snp1, m);
     if (live && l == 0) {
         store_state<KIND>(A.state_np1, c, snp1, aux);
@@ -686,7 +690,9 @@ __global__ __launch_bounds__(256) void k_rk_cells_two_pass(CellForceArgs A, cons
         for (int a = 0; a < 3; ++a)
 #pragma unroll
             for (int b = 0; b < 3; ++b) H[3 * a + b] = __fma_rn(uj[k][a], gc[k][b], H[3 * a + b]);
-    stress<KIND>(H, m, sn, snp1, aux, P);
+    RateIn rin;
+    load_rate<KIND>(rin, A.Hn, A.Pn, A.ne, c, A.rate_dt);
+    stress<KIND>(H, m, sn, snp1, aux, P, &rin);
     snp1 = cell_damage_marker<KIND>(snp1, m);
     if (!live) return;
     store_state<KIND>(A.state_np1, c, snp1, aux);
@@ -1097,10 +1103,11 @@ void rk_cell_force(Ctx &c) {
     if (c.ne == 0) return;
     const int64_t nbc = padded_groups(c.ne, D.two_pass ? 1 : kCellLanes);
     CellForceArgs A{c.ne, nbc, D.start.p, D.nodes.p, D.dphi.p, D.volume.p, D.two_pass ? D.u_phys.p : c.u.p,
-                    c.H.p, c.P.p, D.g.p, D.entry_pos.p,
+                    c.Hnp1().p, c.Pnp1().p, D.g.p, D.entry_pos.p,
                     c.state[1 - c.state_np1].p, c.state[c.state_np1].p,
                     c.mat.params(),
-                    c.has_fields ? c.elem_lambda.p : nullptr, c.has_fields ? c.elem_mu.p : nullptr};
+                    c.has_fields ? c.elem_lambda.p : nullptr, c.has_fields ? c.elem_mu.p : nullptr,
+                    c.mat.rate_form() ? c.Hn().p : nullptr, c.mat.rate_form() ? c.Pn().p : nullptr, c.rate_dt};
     const dim3 g((unsigned)nbc);
     if (D.two_pass) {
         const double *X = c.X.p;
@@ -1112,6 +1119,9 @@ void rk_cell_force(Ctx &c) {
             case APHIP_NEO_HOOKEAN_DAMAGE:
                 hipLaunchKernelGGL(k_rk_cells_two_pass<APHIP_NEO_HOOKEAN_DAMAGE>, g, dim3(256), 0, c.stream, A, X);
                 break;
+            case APHIP_POWER_LAW_CREEP:
+                hipLaunchKernelGGL(k_rk_cells_two_pass<APHIP_POWER_LAW_CREEP>, g, dim3(256), 0, c.stream, A, X);
+                break;
             default: throw Error(APHIP_ERR_INVALID, "material not set");
         }
     } else {
@@ -1122,6 +1132,9 @@ void rk_cell_force(Ctx &c) {
             case APHIP_SVK: hipLaunchKernelGGL((k_rk_cells<APHIP_SVK, kCellLanes>), g, dim3(256), 0, c.stream, A); break;
             case APHIP_NEO_HOOKEAN_DAMAGE:
                 hipLaunchKernelGGL((k_rk_cells<APHIP_NEO_HOOKEAN_DAMAGE, kCellLanes>), g, dim3(256), 0, c.stream, A);
+                break;
+            case APHIP_POWER_LAW_CREEP:
+                hipLaunchKernelGGL((k_rk_cells<APHIP_POWER_LAW_CREEP, kCellLanes>), g, dim3(256), 0, c.stream, A);
                 break;
             default: throw Error(APHIP_ERR_INVALID, "material not set");
         }

@@ -65,6 +65,8 @@ struct TileArgs {
     const uint16_t *lconn;
     double *bpart;
     double *H, *P;
+    const double *Hn, *Pn;  // rate-form materials: H and P of the previous step
+    double rate_dt;         // and the increment of this evaluation
     const double *state_n;
     double *state_np1;
     MatParams mat;
@@ -214,6 +216,8 @@ __global__ __launch_bounds__(NPE == 4 ? 1024 : (QPAR ? 512 : 256), NPE == 8 && !
     }
     load_elem_lame(A, ec, m);
     if constexpr (NPE == 4) sn = load_state<KIND>(A.state_n, ec);
+    RateIn rin;
+    load_rate<KIND>(rin, A.Hn, A.Pn, A.ne, ec, A.rate_dt);
 
     // A: tile CSR entries and node data into LDS
     {
@@ -291,7 +295,7 @@ __global__ __launch_bounds__(NPE == 4 ? 1024 : (QPAR ? 512 : 256), NPE == 8 && !
             const int64_t so = state_width<KIND>() * (PER_QP ? e * 8 : e);
             double H[9], P[9];
             hex8_qp_lane<KIND, PER_QP>(q, lc, L.X, L.U, m, A.gauss_a, state_width<KIND>() ? A.state_n + so : nullptr,
-                                       state_width<KIND>() ? A.state_np1 + so : nullptr, H, P, fe[0]);
+                                       state_width<KIND>() ? A.state_np1 + so : nullptr, H, P, fe[0], &rin);
             if (q == 7) {
 #pragma unroll
                 for (int i = 0; i < 9; ++i) {
@@ -311,12 +315,12 @@ __global__ __launch_bounds__(NPE == 4 ? 1024 : (QPAR ? 512 : 256), NPE == 8 && !
         double H[9], P[9];
         if constexpr (NPE == 4) {
             double snp1 = 0.0, aux = 0.0;
-            tet4_element<KIND>(X, U, m, sn, snp1, aux, H, P, fe);
+            tet4_element<KIND>(X, U, m, sn, snp1, aux, H, P, fe, &rin);
             store_state<KIND>(A.state_np1, e, snp1, aux);
         } else {
             const int64_t so = state_width<KIND>() * (PER_QP ? e * 8 : e);
             hex8_element<KIND, PER_QP>(X, U, m, A.gauss_a, state_width<KIND>() ? A.state_n + so : nullptr,
-                                       state_width<KIND>() ? A.state_np1 + so : nullptr, H, P, fe);
+                                       state_width<KIND>() ? A.state_np1 + so : nullptr, H, P, fe, &rin);
         }
 #pragma unroll
         for (int i = 0; i < 9; ++i) {
@@ -830,6 +834,10 @@ void dispatch_kind(Ctx &c, const TileArgs &A, size_t lds) {
             if (pq) launch_tile<NPE, APHIP_NEO_HOOKEAN_DAMAGE, true, MODE>(c, A, lds);
             else launch_tile<NPE, APHIP_NEO_HOOKEAN_DAMAGE, false, MODE>(c, A, lds);
             break;
+        case APHIP_POWER_LAW_CREEP:
+            if (pq) launch_tile<NPE, APHIP_POWER_LAW_CREEP, true, MODE>(c, A, lds);
+            else launch_tile<NPE, APHIP_POWER_LAW_CREEP, false, MODE>(c, A, lds);
+            break;
         default: throw Error(APHIP_ERR_INVALID, "material not set");
     }
 }
@@ -1029,8 +1037,11 @@ void tile_run(Ctx &c, double dt, int mode) {
     A.lcsr = P.lcsr.p;
     A.lconn = P.lconn.p;
     A.bpart = P.bpart.p;
-    A.H = c.H.p;
-    A.P = c.P.p;
+    A.H = c.Hnp1().p;
+    A.P = c.Pnp1().p;
+    A.Hn = c.mat.rate_form() ? c.Hn().p : nullptr;
+    A.Pn = c.mat.rate_form() ? c.Pn().p : nullptr;
+    A.rate_dt = mode == 2 ? dt : c.rate_dt;
     A.state_n = c.state[1 - c.state_np1].p;
     A.state_np1 = c.state[c.state_np1].p;
     A.mat = c.mat.params();

@@ -20,7 +20,7 @@ _PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.normpath(os.path.join(_PKG, "..", "..", "lib", "libaperi_hip.so"))
 
 TET4, HEX8 = 0, 1
-LINEAR_ELASTIC, NEO_HOOKEAN, J2, SVK, NEO_HOOKEAN_DAMAGE = 0, 1, 2, 3, 4
+LINEAR_ELASTIC, NEO_HOOKEAN, J2, SVK, NEO_HOOKEAN_DAMAGE, POWER_LAW_CREEP = 0, 1, 2, 3, 4, 5
 # optional neo-Hookean-with-damage keys and the reference's defaults (Materials/Factory.h:78-95)
 DAMAGE_DEFAULTS = dict(maximum_damage=0.99, maximum_damage_increment=1.0, stop_failing_at_time=-1.0,
                        fail_all_at_time=-1.0)
@@ -59,16 +59,24 @@ class MaterialParams(ctypes.Structure):
                 ("reserved", ctypes.c_int32),
                 ("I1_critical", ctypes.c_double), ("I1_failure", ctypes.c_double), ("alpha", ctypes.c_double),
                 ("maximum_damage", ctypes.c_double), ("maximum_damage_increment", ctypes.c_double),
-                ("stop_failing_at_time", ctypes.c_double), ("fail_all_at_time", ctypes.c_double)]
+                ("stop_failing_at_time", ctypes.c_double), ("fail_all_at_time", ctypes.c_double),
+                ("creep_A", ctypes.c_double), ("creep_n", ctypes.c_double), ("creep_m", ctypes.c_double),
+                ("constant_temperature", ctypes.c_double), ("use_constant_temperature", ctypes.c_int32),
+                ("reserved2", ctypes.c_int32)]
 
 
-def material_params(density, E, nu, yield_stress=0.0, hardening=0.0, state_per_qp=0, damage=None):
+def material_params(density, E, nu, yield_stress=0.0, hardening=0.0, state_per_qp=0, damage=None, creep=None):
     """aphip_material_params; damage: dict with I1_critical, I1_failure, alpha and optionally
-    maximum_damage, maximum_damage_increment, stop_failing_at_time, fail_all_at_time."""
+    maximum_damage, maximum_damage_increment, stop_failing_at_time, fail_all_at_time;
+    creep: dict with A, n, m and constant_temperature (the reference requires one)."""
     p = MaterialParams(density, E, nu, yield_stress, hardening, state_per_qp, 0)
     if damage is not None:
         for k, v in dict(DAMAGE_DEFAULTS, **damage).items():
             setattr(p, k, v)
+    if creep is not None:
+        p.creep_A, p.creep_n, p.creep_m = creep["A"], creep["n"], creep["m"]
+        if "constant_temperature" in creep:
+            p.constant_temperature, p.use_constant_temperature = creep["constant_temperature"], 1
     return p
 
 
@@ -167,8 +175,9 @@ class Solver:
         _check(self.L.aphip_get_node_permutation(self.ctx, _p(out, _i32)))
         return out
 
-    def set_material(self, kind, density, E, nu, yield_stress=0.0, hardening=0.0, state_per_qp=0, damage=None):
-        p = material_params(density, E, nu, yield_stress, hardening, state_per_qp, damage)
+    def set_material(self, kind, density, E, nu, yield_stress=0.0, hardening=0.0, state_per_qp=0, damage=None,
+                     creep=None):
+        p = material_params(density, E, nu, yield_stress, hardening, state_per_qp, damage, creep)
         _check(self.L.aphip_set_material(self.ctx, kind, ctypes.byref(p)))
         self.kind, self.state_per_qp = kind, state_per_qp
         return self
@@ -208,7 +217,7 @@ class Solver:
         if field in (FIELD_H, FIELD_P):
             return (self.ne, 9)
         if field == FIELD_STATE:
-            width = 2 if self.kind == NEO_HOOKEAN_DAMAGE else 1
+            width = 2 if self.kind in (NEO_HOOKEAN_DAMAGE, POWER_LAW_CREEP) else 1
             return (width * self.ne * (8 if (self.etype == HEX8 and self.state_per_qp) else 1),)
         return (self.ne,)
 

@@ -39,14 +39,23 @@ typedef struct aphip_ctx aphip_ctx;
 /* Element topology: include/Constants.h:23-25 (ElementTopology). */
 enum { APHIP_TET4 = 0, APHIP_HEX8 = 1 };
 
-/* Material kinds: include/Materials/{LinearElastic,NeoHooke,Plastic,Elastic,NeoHookeWithDamage}.h */
-enum { APHIP_LINEAR_ELASTIC = 0, APHIP_NEO_HOOKEAN = 1, APHIP_J2 = 2, APHIP_SVK = 3, APHIP_NEO_HOOKEAN_DAMAGE = 4 };
+/* Material kinds: include/Materials/{LinearElastic,NeoHooke,Plastic,Elastic,NeoHookeWithDamage,
+ * PowerLawCreep}.h */
+enum {
+    APHIP_LINEAR_ELASTIC = 0,
+    APHIP_NEO_HOOKEAN = 1,
+    APHIP_J2 = 2,
+    APHIP_SVK = 3,
+    APHIP_NEO_HOOKEAN_DAMAGE = 4,
+    APHIP_POWER_LAW_CREEP = 5
+};
 
 /* Fields for aphip_get_field / aphip_set_field.
  * Node fields (n_nodes x 3): U, V, A, F; MASS is n_nodes x 1.
  * Element fields (n_elem x 9): H (displacement_gradient), P (pk1_stress);
  * STATE is n_elem x 1 (J2 equivalent plastic strain; n_elem x 8 with state_per_qp) or
- * n_elem x 2 (neo-Hookean with damage: damage, I1; n_elem x 16 with state_per_qp);
+ * n_elem x 2 (neo-Hookean with damage: damage, I1; power-law creep: effective creep strain,
+ * effective stress rate; n_elem x 16 with state_per_qp);
  * VOLUME is n_elem x 1. */
 enum {
     APHIP_FIELD_U = 0,
@@ -98,6 +107,15 @@ typedef struct {
     double maximum_damage_increment;
     double stop_failing_at_time;
     double fail_all_at_time;
+    /* power-law creep (include/Materials/PowerLawCreep.h, YAML keys of Factory.h:134-153):
+     * creep constant A, exponent n, thermal constant m; the reference requires a constant
+     * temperature (use_constant_temperature = 1 with constant_temperature). */
+    double creep_A;
+    double creep_n;
+    double creep_m;
+    double constant_temperature;
+    int32_t use_constant_temperature;
+    int32_t reserved2;
 } aphip_material_params;
 
 /* ---- lifetime ---------------------------------------------------------- */

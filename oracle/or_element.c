@@ -193,8 +193,12 @@ int or_internal_force(int etype, int64_t n_nodes, const double *X, int64_t n_ele
                     He[3 * i + j] = s;
                 }
             const int64_t sidx = (per_qp ? e * 8 + q : e) * sw;
-            or_stress_mat(mat, lam, mu, He, has_state ? &state_n[sidx] : NULL, has_state ? &state_np1[sidx] : NULL,
-                          Pe);
+            if (or_rate_form(mat->kind))  /* the element's H_n, P_n for every Gauss point */
+                or_stress_rate(mat, lam, mu, He, &mat->H_n[9 * e], &mat->P_n[9 * e], mat->dt, &state_n[sidx],
+                               &state_np1[sidx], Pe);
+            else
+                or_stress_mat(mat, lam, mu, He, has_state ? &state_n[sidx] : NULL,
+                              has_state ? &state_np1[sidx] : NULL, Pe);
             for (int k = 0; k < npe; ++k)
                 for (int j = 0; j < 3; ++j) {
                     double s = 0.0;

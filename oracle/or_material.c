@@ -6,6 +6,10 @@
  *   neo-Hookean     include/Materials/NeoHooke.h:67-92 (InvertMatrix = FullPivLU, MathUtils.h:307-319)
  *   J2 plasticity   include/Materials/Plastic.h:88-140 (small strain, Simo-Hughes box 7.5)
  *   SVK "elastic"   include/Materials/Elastic.h:61-79
+ *   power-law creep include/Materials/PowerLawCreep.h:101-173 with the velocity gradient of
+ *                   include/ComputeInternalForceBase.h:182-187 (L = (H - H_n) / dt (I + H_n)^-1);
+ *                   the sub-stepping loop of :122-144 leaves after its first pass (tm1 =
+ *                   min(critt, dt) <= dt), so one explicit sub-step of min(critt, dt)
  *   NH with damage  include/Materials/NeoHookeWithDamage.h:180-243 (state: damage, I1; damage
  *                   from I1 between I1_critical and I1_failure with exponent alpha, never
  *                   decreasing, at most maximum_damage_increment per evaluation, capped at
@@ -108,7 +112,88 @@ void or_stress(int kind, double lambda, double mu, double yield_stress, double h
     }
 }
 
-int or_state_width(int kind) { return kind == OR_J2 ? 1 : kind == OR_NH_DAMAGE ? 2 : 0; }
+int or_state_width(int kind) { return kind == OR_J2 ? 1 : (kind == OR_NH_DAMAGE || kind == OR_CREEP) ? 2 : 0; }
+
+int or_rate_form(int kind) { return kind == OR_CREEP; }
+
+static double mag9(const double *A) {
+    double s = 0.0;
+    for (int i = 0; i < 9; ++i) s += A[i] * A[i];
+    return sqrt(s);
+}
+
+void or_stress_rate(const or_material *mat, double lambda, double mu, const double H[9], const double Hn[9],
+                    const double Pn[9], double dt, const double *state_n, double *state_np1, double P[9]) {
+    (void)lambda;
+    const double G = mu, K = mat->bulk;
+    double F[9], Fi[9], Fn[9], Fni[9], L[9], J;
+    for (int i = 0; i < 9; ++i) {
+        F[i] = H[i];
+        Fn[i] = Hn[i];
+    }
+    F[0] += 1.0; F[4] += 1.0; F[8] += 1.0;
+    Fn[0] += 1.0; Fn[4] += 1.0; Fn[8] += 1.0;
+    or_inverse3_cofactor(F, Fi, &J);
+    or_fullpivlu_inverse(3, Fn, Fni);
+    /* L = (H - H_n) / dt (I + H_n)^-1; a zero increment gives L = 0 */
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            double s = 0.0;
+            for (int k = 0; k < 3; ++k) s += (dt != 0.0 ? (H[3 * i + k] - Hn[3 * i + k]) / dt : 0.0) * Fni[3 * k + j];
+            L[3 * i + j] = s;
+        }
+    /* sigma = (1/J) P_n F^T */
+    double sig[9], sd[9], de[9], ded[9], cs[9], ds[9];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            double s = 0.0;
+            for (int k = 0; k < 3; ++k) s += (1.0 / J * Pn[3 * i + k]) * F[3 * j + k];
+            sig[3 * i + j] = s;
+        }
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) de[3 * i + j] = 0.5 * (L[3 * i + j] + L[3 * j + i]) * dt;
+    const double sm = (sig[0] + sig[4] + sig[8]) / 3.0, dm = (de[0] + de[4] + de[8]) / 3.0;
+    for (int i = 0; i < 9; ++i) {
+        sd[i] = sig[i];
+        ded[i] = de[i];
+    }
+    for (int i = 0; i < 3; ++i) {
+        sd[4 * i] -= sm;
+        ded[4 * i] -= dm;
+    }
+    for (int i = 0; i < 9; ++i) cs[i] = ded[i] + sd[i] / 2.0 / G;
+    const double T = mat->temperature;
+    const double a = T == 0.0 ? mat->creep_A * exp(-mat->creep_m) : mat->creep_A * exp(-mat->creep_m / T);
+    const double n = mat->creep_n, s32 = sqrt(3.0 / 2.0);
+    double edot = a * pow(mag9(sd) * s32, n - 1.0);
+    const double critt = edot == 0.0 ? 1.1 * dt : 4.0 * 0.98 / (6.0 * G * n * edot);
+    const double delt = critt < dt ? critt : dt;
+    for (int i = 0; i < 9; ++i) {
+        const double rate = dt != 0.0 ? ded[i] / dt : 0.0;
+        const double sdot = 2.0 * G * (rate - 1.5 * sd[i] * edot);
+        sd[i] += sdot * delt;
+    }
+    edot = a * pow(mag9(sd) * s32, n - 1.0);
+    const double mean = (sig[0] + sig[4] + sig[8]) / 3.0 + K * (de[0] + de[4] + de[8]);
+    for (int i = 0; i < 9; ++i) {
+        sig[i] = sd[i];
+        cs[i] -= sd[i] / 2.0 / G;
+    }
+    for (int i = 0; i < 3; ++i) sig[4 * i] += mean;
+    if (state_np1) state_np1[0] = (state_n ? state_n[0] : 0.0) + sqrt(2.0 / 3.0) * mag9(cs);
+    for (int i = 0; i < 9; ++i) {
+        const double rate = dt != 0.0 ? ded[i] / dt : 0.0;
+        ds[i] = 2.0 * G * (rate - 3.0 * edot * sd[i] / 2.0);
+    }
+    if (state_np1) state_np1[1] = s32 * mag9(ds);
+    /* P = (J sigma) F^-T */
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            double s = 0.0;
+            for (int k = 0; k < 3; ++k) s += (J * sig[3 * i + k]) * Fi[3 * j + k];
+            P[3 * i + j] = s;
+        }
+}
 
 static double clampd(double x, double lo, double hi) { return x < lo ? lo : (x > hi ? hi : x); }
 

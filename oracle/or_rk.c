@@ -291,8 +291,12 @@ int or_cell_internal_force(int64_t n_nodes, int64_t n_cells, const int64_t *cell
             for (int a = 0; a < 3; ++a)
                 for (int b = 0; b < 3; ++b) He[3 * a + b] += u[3 * j + a] * cell_dphi[3 * t + b];
         }
-        or_stress_mat(mat, lam, mu, He, has_state ? &state_n[c * sw] : NULL, has_state ? &state_np1[c * sw] : NULL,
-                      Pe);
+        if (or_rate_form(mat->kind))
+            or_stress_rate(mat, lam, mu, He, &mat->H_n[9 * c], &mat->P_n[9 * c], mat->dt, &state_n[c * sw],
+                           &state_np1[c * sw], Pe);
+        else
+            or_stress_mat(mat, lam, mu, He, has_state ? &state_n[c * sw] : NULL,
+                          has_state ? &state_np1[c * sw] : NULL, Pe);
         /* a new failure's marker becomes the maximum damage in the smoothed-cell path
          * (ComputeInternalForceSmoothedCell.h:219-223, NeoHookeWithDamage.h:247-252) */
         if (mat->kind == OR_NH_DAMAGE && state_np1[c * sw] == 2.0)

@@ -67,6 +67,12 @@ int or_solve(const or_problem *p, double dt, int n_steps, int do_initial, double
     const int nt = nthreads > 0 ? nthreads : 1;
     double *sn = state_a, *snp1 = state_b;
     int rc;
+    /* rate-form materials: this call's increment; H_n, P_n are the caller's buffers */
+    or_problem q = *p;
+    q.mat.dt = dt;
+    const int rate = or_rate_form(p->mat.kind);
+    if (rate && (!q.mat.H_n || !q.mat.P_n)) return -1;
+    p = &q;
     if (do_initial) {
         /* Explicit.cpp:363-376: force and acceleration at the initial state (written to NP1) */
         rc = compute_force(p, u, snp1, snp1, H, P, f, nt);
@@ -78,6 +84,10 @@ int or_solve(const or_problem *p, double dt, int n_steps, int do_initial, double
     for (int s = 0; s < n_steps; ++s) {
         /* UpdateFieldStates: N <- NP1 */
         double *t = sn; sn = snp1; snp1 = t;
+        if (rate) {
+            memcpy(q.mat.H_n, H, sizeof(double) * 9 * (size_t)p->n_elem);
+            memcpy(q.mat.P_n, P, sizeof(double) * 9 * (size_t)p->n_elem);
+        }
 #pragma omp parallel for schedule(static) num_threads(nt)
         for (int64_t i = 0; i < N; ++i) {
             const double dti = p->node_dt ? p->node_dt[i] : dt;

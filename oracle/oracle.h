@@ -26,7 +26,7 @@ extern "C" {
 #endif
 
 enum { OR_TET4 = 0, OR_HEX8 = 1 };
-enum { OR_LINEAR_ELASTIC = 0, OR_NEO_HOOKEAN = 1, OR_J2 = 2, OR_SVK = 3, OR_NH_DAMAGE = 4 };
+enum { OR_LINEAR_ELASTIC = 0, OR_NEO_HOOKEAN = 1, OR_J2 = 2, OR_SVK = 3, OR_NH_DAMAGE = 4, OR_CREEP = 5 };
 
 /* Material: uniform parameters, optionally overridden per element. */
 typedef struct {
@@ -38,7 +38,20 @@ typedef struct {
     const double *elem_mu;     /* optional [n_elem] */
     /* neo-Hookean with damage (include/Materials/NeoHookeWithDamage.h) */
     double I1_critical, I1_failure, alpha, maximum_damage, maximum_damage_increment;
+    /* power-law creep (include/Materials/PowerLawCreep.h): bulk modulus, A, n, m and the
+     * constant temperature */
+    double bulk, creep_A, creep_n, creep_m, temperature;
+    /* rate-form materials: H and P of the previous step ([n_elem][9], row-major; owned
+     * by the caller, or_solve refreshes them each step) and the increment */
+    double *H_n, *P_n;
+    double dt;
 } or_material;
+
+/* 1 for materials that read the previous step's H and P (velocity gradient, old stress). */
+int or_rate_form(int kind);
+/* The rate-form stress of `mat` from H, the previous step's Hn, Pn and the increment dt. */
+void or_stress_rate(const or_material *mat, double lambda, double mu, const double H[9], const double Hn[9],
+                    const double Pn[9], double dt, const double *state_n, double *state_np1, double P[9]);
 
 /* State values per evaluation point: J2 1 (equivalent plastic strain), neo-Hookean with
  * damage 2 (damage, I1), others 0. */

@@ -15,7 +15,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = None
 
 TET4, HEX8 = 0, 1
-LINEAR_ELASTIC, NEO_HOOKEAN, J2, SVK, NH_DAMAGE = 0, 1, 2, 3, 4
+LINEAR_ELASTIC, NEO_HOOKEAN, J2, SVK, NH_DAMAGE, CREEP = 0, 1, 2, 3, 4, 5
 MAX_NODE_NUM_NEIGHBORS = 40  # include/Constants.h:15
 
 _d = ctypes.POINTER(ctypes.c_double)
@@ -30,7 +30,10 @@ class Material(ctypes.Structure):
                 ("yield_stress", ctypes.c_double), ("hardening_modulus", ctypes.c_double),
                 ("elem_lambda", _d), ("elem_mu", _d),
                 ("I1_critical", ctypes.c_double), ("I1_failure", ctypes.c_double), ("alpha", ctypes.c_double),
-                ("maximum_damage", ctypes.c_double), ("maximum_damage_increment", ctypes.c_double)]
+                ("maximum_damage", ctypes.c_double), ("maximum_damage_increment", ctypes.c_double),
+                ("bulk", ctypes.c_double), ("creep_A", ctypes.c_double), ("creep_n", ctypes.c_double),
+                ("creep_m", ctypes.c_double), ("temperature", ctypes.c_double),
+                ("H_n", _d), ("P_n", _d), ("dt", ctypes.c_double)]
 
 
 class Problem(ctypes.Structure):
@@ -155,10 +158,17 @@ DAMAGE_DEFAULTS = dict(maximum_damage=0.99, maximum_damage_increment=1.0)  # Mat
 
 
 def _material(kind, lam, mu, yield_stress=0.0, hardening=0.0, elem_lambda=None, elem_mu=None, state_per_qp=0,
-              damage=None):
+              damage=None, creep=None, rate=None):
     """damage: dict of I1_critical, I1_failure, alpha[, maximum_damage, maximum_damage_increment]
-    for NH_DAMAGE."""
+    for NH_DAMAGE; creep: dict of A, n, m, constant_temperature and bulk (the bulk modulus)
+    for CREEP; rate: (H_n, P_n, dt) with H_n, P_n float64 [n_elem, 9] arrays (kept alive by
+    the caller)."""
     m = Material()
+    if creep is not None:
+        m.bulk, m.creep_A, m.creep_n, m.creep_m = creep["bulk"], creep["A"], creep["n"], creep["m"]
+        m.temperature = creep.get("constant_temperature", 0.0)
+    if rate is not None:
+        m.H_n, m.P_n, m.dt = _p(rate[0]), _p(rate[1]), rate[2]
     m.kind, m.state_per_qp = kind, state_per_qp
     m.lam, m.mu, m.yield_stress, m.hardening_modulus = lam, mu, yield_stress, hardening
     m.elem_lambda = _p(elem_lambda)
@@ -171,7 +181,7 @@ def _material(kind, lam, mu, yield_stress=0.0, hardening=0.0, elem_lambda=None, 
 
 
 def state_width(kind):
-    return {J2: 1, NH_DAMAGE: 2}.get(kind, 0)
+    return {J2: 1, NH_DAMAGE: 2, CREEP: 2}.get(kind, 0)
 
 
 def state_size(etype, kind, n_elem, state_per_qp=0):
@@ -179,6 +189,17 @@ def state_size(etype, kind, n_elem, state_per_qp=0):
     if w == 0:
         return 1
     return w * n_elem * (8 if (etype == HEX8 and state_per_qp) else 1)
+
+
+def stress_creep(lam, mu, H, Hn, Pn, dt, creep, state_n=(0.0, 0.0)):
+    """power-law creep: (P, (effective creep strain, effective stress rate))."""
+    mat = _material(CREEP, lam, mu, creep=creep)
+    P = np.zeros(9)
+    snp1 = np.zeros(2)
+    sn = _f64(np.asarray(state_n, dtype=np.float64))
+    lib().or_stress_rate(ctypes.byref(mat), ctypes.c_double(lam), ctypes.c_double(mu), _p(_f64(H).reshape(9)),
+                         _p(_f64(Hn).reshape(9)), _p(_f64(Pn).reshape(9)), ctypes.c_double(dt), _p(sn), _p(snp1), _p(P))
+    return P.reshape(3, 3), snp1
 
 
 def stress_damage(lam, mu, H, damage, state_n=(0.0, 0.0)):
@@ -193,13 +214,17 @@ def stress_damage(lam, mu, H, damage, state_n=(0.0, 0.0)):
 
 
 def internal_force(etype, X, conn, kind, lam, mu, u, state_n=None, yield_stress=0.0, hardening=0.0,
-                   elem_lambda=None, elem_mu=None, state_per_qp=0, nthreads=1, damage=None):
+                   elem_lambda=None, elem_mu=None, state_per_qp=0, nthreads=1, damage=None, creep=None, rate=None):
     """f = -sum_e int B^T P dV, with H, P (last Gauss point) and the new state."""
     X, conn, u = _f64(X), _i32a(conn), _f64(u)
     ne, nn = conn.shape[0], X.shape[0]
     el = None if elem_lambda is None else _f64(elem_lambda)
     em = None if elem_mu is None else _f64(elem_mu)
-    mat = _material(kind, lam, mu, yield_stress, hardening, el, em, state_per_qp, damage)
+    if kind == CREEP and rate is None:
+        rate = (np.zeros((ne, 9)), np.zeros((ne, 9)), 0.0)
+    if rate is not None:
+        rate = (_f64(rate[0]), _f64(rate[1]), rate[2])
+    mat = _material(kind, lam, mu, yield_stress, hardening, el, em, state_per_qp, damage, creep, rate)
     ns = state_size(etype, kind, ne, state_per_qp)
     sn = np.zeros(ns) if state_n is None else _f64(state_n).copy()
     snp1 = np.zeros(ns)
@@ -234,7 +259,7 @@ class Solver:
 
     def __init__(self, etype, X, conn, kind, lam, mu, mass, fixed=None, yield_stress=0.0, hardening=0.0,
                  elem_lambda=None, elem_mu=None, node_dt=None, state_per_qp=0, cells=None, two_pass=None,
-                 damage=None):
+                 damage=None, creep=None):
         """cells: RK smoothed cells (start, nodes, dphi, volume); two_pass: (num, nbr, phi)
         of the RK functions, with `cells` from rk_cells_two_pass, for the two-pass force."""
         self.X = _f64(X)
@@ -265,7 +290,10 @@ class Solver:
         else:
             p.n_elem = self.conn.shape[0]
             p.conn = _p(self.conn, _i32)
-        p.mat = _material(kind, lam, mu, yield_stress, hardening, self.el, self.em, state_per_qp, damage)
+        ne_ = self.cv.shape[0] if cells is not None else self.conn.shape[0]
+        self.Hn, self.Pn = np.zeros((ne_, 9)), np.zeros((ne_, 9))  # N halves of the H, P pairs
+        p.mat = _material(kind, lam, mu, yield_stress, hardening, self.el, self.em, state_per_qp, damage, creep,
+                          (self.Hn, self.Pn, 0.0))
         p.mass = _p(self.mass)
         p.fixed = _p(self.fixed, _u8)
         p.node_dt = _p(self.node_dt)
@@ -377,10 +405,14 @@ def rk_cells(X, conn, num, nbr, phi):
 
 
 def cell_internal_force(n_nodes, cells, kind, lam, mu, u, state_n=None, yield_stress=0.0, hardening=0.0,
-                        nthreads=1, damage=None):
+                        nthreads=1, damage=None, creep=None, rate=None):
     start, nodes, dphi, vol = cells
     nc = vol.shape[0]
-    mat = _material(kind, lam, mu, yield_stress, hardening, damage=damage)
+    if kind == CREEP and rate is None:
+        rate = (np.zeros((nc, 9)), np.zeros((nc, 9)), 0.0)
+    if rate is not None:
+        rate = (_f64(rate[0]), _f64(rate[1]), rate[2])
+    mat = _material(kind, lam, mu, yield_stress, hardening, damage=damage, creep=creep, rate=rate)
     w = max(state_width(kind), 1)
     sn = np.zeros(nc * w) if state_n is None else _f64(state_n).copy()
     snp1 = np.zeros(nc * w)

@@ -192,3 +192,69 @@ def test_damage_in_elements_and_cells():
     fc, Hc, Pc, sc = O.cell_internal_force(4, cells, O.NH_DAMAGE, lam, mu, u, damage=DMG)
     assert sc[0] == 0.99 and sc[1] == pytest.approx(4.25, rel=1e-14)
     assert np.allclose(fc, f, rtol=1e-12, atol=1e-9 * np.abs(f).max())
+
+
+# ---- power-law creep (include/Materials/PowerLawCreep.h:101-173) ----
+# The reference has no test of this material; the closed forms below pin it.
+
+def _creep(E, nu, A, n, m=0.0, T=0.0):
+    return dict(A=A, n=n, m=m, constant_temperature=T, bulk=E / (3 * (1 - 2 * nu)))
+
+
+def test_creep_without_creep_is_a_hooke_increment():
+    # A = 0: from a stress-free state, sigma = lambda tr(eps) I + 2 mu eps with eps = sym(H),
+    # P = J sigma F^-T; no creep strain; stress rate sqrt(3/2) |2 mu dev(eps)| / dt
+    E, nu = 1e9, 0.3
+    lam, mu = O.lame(E, nu)
+    rng = np.random.default_rng(3)
+    for _ in range(10):
+        H = 1e-3 * rng.standard_normal((3, 3))
+        dt = 1e-5
+        P, st = O.stress_creep(lam, mu, H, np.zeros(9), np.zeros(9), dt, _creep(E, nu, 0.0, 3.0))
+        eps = 0.5 * (H + H.T)
+        sig = lam * np.trace(eps) * np.eye(3) + 2 * mu * eps
+        F = np.eye(3) + H
+        Pc = np.linalg.det(F) * sig @ np.linalg.inv(F).T
+        assert np.allclose(P, Pc, rtol=0, atol=1e-12 * np.abs(Pc).max())
+        dev = eps - np.trace(eps) / 3 * np.eye(3)
+        assert abs(st[0]) < 1e-18
+        assert st[1] == pytest.approx(np.sqrt(1.5) * np.linalg.norm(2 * mu * dev / dt), rel=1e-12)
+
+
+@pytest.mark.parametrize("n", [1.0, 3.0])
+def test_creep_relaxation_is_one_norton_euler_step(n):
+    # no deformation (H = H_n = 0, F = I): the deviator relaxes by one explicit step of
+    # s' = -3 G a seq^(n-1) s / ... over min(critt, dt); pressure is unchanged; the creep
+    # strain is the relaxed elastic strain
+    E, nu = 1e9, 0.3
+    lam, mu = O.lame(E, nu)
+    s0 = np.array([[2e6, 5e5, 0.0], [5e5, -1e6, 3e5], [0.0, 3e5, -1e6]]) + 4e6 * np.eye(3)
+    sd0 = s0 - np.trace(s0) / 3 * np.eye(3)
+    seq = np.sqrt(1.5) * np.linalg.norm(sd0)
+    a = 1e-3 / seq ** (n - 1) / (3 * mu)  # edot such that 3 G edot = 1e-3 per unit time
+    for dt in (1.0, 1e3):  # below and above the sub-step limit critt
+        P, st = O.stress_creep(lam, mu, np.zeros((3, 3)), np.zeros(9), s0.reshape(9), dt,
+                               _creep(E, nu, a, n), state_n=(0.25, 0.0))
+        edot = a * seq ** (n - 1)
+        critt = 4 * 0.98 / (6 * mu * n * edot)
+        delt = min(critt, dt)
+        sd1 = sd0 * (1 - 3 * mu * edot * delt)
+        assert np.allclose(P - np.trace(P) / 3 * np.eye(3), sd1, rtol=0, atol=1e-9 * np.abs(sd0).max())
+        assert np.trace(P) / 3 == pytest.approx(np.trace(s0) / 3, rel=1e-14)
+        dcreep = np.sqrt(2 / 3) * np.linalg.norm((sd0 - sd1) / (2 * mu))
+        assert st[0] == pytest.approx(0.25 + dcreep, rel=1e-12)
+
+
+def test_creep_pressure_does_not_creep():
+    E, nu = 1e9, 0.3
+    lam, mu = O.lame(E, nu)
+    rng = np.random.default_rng(5)
+    H, Hn = 2e-3 * rng.standard_normal((3, 3)), 1e-3 * rng.standard_normal((3, 3))
+    Pn = 1e6 * rng.standard_normal(9)
+    p0 = O.stress_creep(lam, mu, H, Hn, Pn, 1e-4, _creep(E, nu, 0.0, 2.0))[0]
+    p1 = O.stress_creep(lam, mu, H, Hn, Pn, 1e-4, _creep(E, nu, 1e-12, 2.0))[0]
+    F = np.eye(3) + H
+    J = np.linalg.det(F)
+    s0, s1 = p0 @ F.T / J, p1 @ F.T / J
+    assert np.trace(s0) == pytest.approx(np.trace(s1), rel=1e-12)
+    assert np.abs(s0 - s1).max() > 1e-6 * np.abs(s0).max()  # the deviator did creep
