@@ -153,6 +153,7 @@ class Material {
             case ELASTIC: m_kind = APHIP_SVK; break;
             case NEO_HOOKEAN_WITH_DAMAGE: m_kind = APHIP_NEO_HOOKEAN_DAMAGE; break;
             case POWER_LAW_CREEP: m_kind = APHIP_POWER_LAW_CREEP; break;
+            case DRUCKER_PRAGER: m_kind = APHIP_DRUCKER_PRAGER; break;
             default: throw std::runtime_error("Material type not supported by the MI355X force step");
         }
         if (m_kind == APHIP_J2) {
@@ -175,20 +176,26 @@ class Material {
             m_creep[3] = Optional("constant_temperature", 0.0);
             m_creep_const_t = Optional("use_constant_temperature", 0.0) != 0.0 ? 1 : 0;
         }
+        if (m_kind == APHIP_DRUCKER_PRAGER) {  // Materials/Factory.h:118-132
+            m_dp[0] = Required("A1");
+            m_dp[1] = Required("A2");
+            m_dp[2] = Required("A2G");
+        }
     }
     MaterialType GetMaterialType() const { return m_props->material_type; }
     std::shared_ptr<MaterialProperties> GetMaterialProperties() const { return m_props; }
     double GetDensity() const { return m_props->density; }
     bool HasState() const {
-        return m_kind == APHIP_J2 || m_kind == APHIP_NEO_HOOKEAN_DAMAGE || m_kind == APHIP_POWER_LAW_CREEP;
+        return m_kind == APHIP_J2 || m_kind == APHIP_NEO_HOOKEAN_DAMAGE || m_kind == APHIP_POWER_LAW_CREEP ||
+               m_kind == APHIP_DRUCKER_PRAGER;
     }
     // the reference's counts (NeoHookeWithDamage.h:279-282 reports 1, the damage;
-    // PowerLawCreep.h:186-189 reports 2)
+    // PowerLawCreep.h:186-189 and DruckerPrager.h:224-227 report 2)
     size_t NumberOfStateVariables() const {
-        return m_kind == APHIP_POWER_LAW_CREEP ? 2 : HasState() ? 1 : 0;
+        return (m_kind == APHIP_POWER_LAW_CREEP || m_kind == APHIP_DRUCKER_PRAGER) ? 2 : HasState() ? 1 : 0;
     }
-    bool NeedsVelocityGradient() const { return m_kind == APHIP_POWER_LAW_CREEP; }
-    bool NeedsOldStress() const { return m_kind == APHIP_POWER_LAW_CREEP; }
+    bool NeedsVelocityGradient() const { return m_kind == APHIP_POWER_LAW_CREEP || m_kind == APHIP_DRUCKER_PRAGER; }
+    bool NeedsOldStress() const { return NeedsVelocityGradient(); }
     // values per element of the state field (the damage model also stores I1)
     size_t StateFieldWidth() const { return m_kind == APHIP_NEO_HOOKEAN_DAMAGE ? 2 : NumberOfStateVariables(); }
     int Kind() const { return m_kind; }
@@ -197,7 +204,7 @@ class Material {
         return aphip_material_params{m_props->density, le.youngs_modulus, le.poisson_ratio, m_yield, m_hardening,
                                      state_per_qp, 0, m_damage[0], m_damage[1], m_damage[2], m_damage[3],
                                      m_damage[4], m_damage[5], m_damage[6], m_creep[0], m_creep[1], m_creep[2],
-                                     m_creep[3], m_creep_const_t, 0};
+                                     m_creep[3], m_creep_const_t, 0, m_dp[0], m_dp[1], m_dp[2]};
     }
 
    private:
@@ -215,6 +222,7 @@ class Material {
     double m_yield = 0.0, m_hardening = 0.0;
     double m_damage[7] = {0, 0, 0, 0, 0, 0, 0};
     double m_creep[4] = {0, 0, 0, 0};
+    double m_dp[3] = {0, 0, 0};
     int32_t m_creep_const_t = 0;
 };
 

@@ -114,7 +114,7 @@ static std::vector<int32_t> morton_order(const std::vector<double> &pts, int64_t
 }
 
 static void resolve_material(int kind, const aphip_material_params *p, MaterialConst &m) {
-    APHIP_REQUIRE(kind >= APHIP_LINEAR_ELASTIC && kind <= APHIP_POWER_LAW_CREEP, "unknown material kind");
+    APHIP_REQUIRE(kind >= APHIP_LINEAR_ELASTIC && kind <= APHIP_DRUCKER_PRAGER, "unknown material kind");
     APHIP_REQUIRE(p != nullptr, "material params are NULL");
     const double E = p->youngs_modulus, nu = p->poissons_ratio;
     // LinearElasticProperties::CompleteProperties, E/nu branch and bound checks
@@ -160,6 +160,12 @@ static void resolve_material(int kind, const aphip_material_params *p, MaterialC
         m.bulk = E / (3 * (1 - 2 * nu));
         m.creep_rate = T == 0.0 ? p->creep_A * std::exp(-p->creep_m) : p->creep_A * std::exp(-p->creep_m / T);
         m.creep_n = p->creep_n;
+    }
+    if (kind == APHIP_DRUCKER_PRAGER) {  // DruckerPragerMaterial::CreateStressFunctor (DruckerPrager.h:52-68)
+        m.bulk = E / (3 * (1 - 2 * nu));
+        m.dp_A1 = p->dp_A1;
+        m.dp_A2 = p->dp_A2;
+        m.dp_A2G = p->dp_A2G;
     }
 }
 

@@ -8,6 +8,7 @@
 //   SVK             include/Materials/Elastic.h:61-79
 //   NH with damage  include/Materials/NeoHookeWithDamage.h:180-243
 //   power-law creep include/Materials/PowerLawCreep.h:101-173 (hypoelastic, rate form)
+//   Drucker-Prager  include/Materials/DruckerPrager.h:97-216 (rate form, oblique return)
 // The material is a compile-time template parameter; the reference's device
 // virtual call (ComputeInternalForceGaussian.h:107) is resolved on the host.
 #pragma once
@@ -86,12 +87,14 @@ struct MatParams {
     // power-law creep: bulk modulus and the creep rate factor at the constant
     // temperature, a = A exp(-m / T) (A exp(-m) at T = 0), and the exponent n
     double bulk, creep_rate, creep_n;
+    // Drucker-Prager: yield s / sqrt(2) = A1 - A2 z, flow direction A2G
+    double dp_A1, dp_A2, dp_A2G;
 };
 
 // Rate-form materials read the previous step's displacement gradient and stress.
 template <int KIND>
 constexpr bool rate_form() {
-    return KIND == APHIP_POWER_LAW_CREEP;
+    return KIND == APHIP_POWER_LAW_CREEP || KIND == APHIP_DRUCKER_PRAGER;
 }
 
 // Inputs of a rate-form update: H_n, P_n (row-major) and the time increment.
@@ -117,7 +120,18 @@ APHIP_DEV void load_rate(RateIn &r, const double *Hn, const double *Pn, int64_t 
 // plastic strain; neo-Hookean with damage the damage and I1 (in that order).
 template <int KIND>
 constexpr int state_width() {
-    return KIND == APHIP_J2 ? 1 : (KIND == APHIP_NEO_HOOKEAN_DAMAGE || KIND == APHIP_POWER_LAW_CREEP) ? 2 : 0;
+    return KIND == APHIP_J2 ? 1
+           : (KIND == APHIP_NEO_HOOKEAN_DAMAGE || KIND == APHIP_POWER_LAW_CREEP || KIND == APHIP_DRUCKER_PRAGER)
+               ? 2
+               : 0;
+}
+
+// The second state value of evaluation point `slot`, for the material that reads it
+// (Drucker-Prager's volumetric plastic strain).
+template <int KIND>
+APHIP_DEV double load_state2(const double *state, int64_t slot) {
+    if constexpr (KIND == APHIP_DRUCKER_PRAGER) return state[2 * slot + 1];
+    else return 0.0;
 }
 
 // The first state value of evaluation point `slot` (the only one a material reads).
@@ -244,13 +258,164 @@ APHIP_DEV void creep_stress(const double *H, const MatParams &m, const RateIn &r
                                 Fi[3 * j + 2]);
 }
 
-// P = stress(H).  sn / snp1: J2 equivalent plastic strain, damage, or effective creep
-// strain, in / out; aux: the damage model's I1 or the creep model's effective stress
-// rate.  The smoothed-cell path turns a new failure's marker 2.0 into the maximum
-// damage afterwards (cell_damage_marker).  r: the rate-form inputs (creep only).
+// Lode components of A: z = tr A / sqrt 3, s = |dev A|, E = dev A / s (0 when s <= 1e-10)
+// (include/Tensor.h:34-57)
+APHIP_DEV void lode(const double *A, double &z, double &s, double *E) {
+#pragma clang fp contract(off)
+    const double t = tr3(A);
+    z = t / sqrt(3.0);
+    double d[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) d[i] = A[i];
+    const double iso = t / 3.0;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) d[4 * i] -= iso;
+    s = magnitude9(d);
+    const bool nz = fabs(s) > 1.0e-10;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) E[i] = nz ? d[i] / s : 0.0;
+}
+
+// L = (H - H_n) / dt (I + H_n)^-1 (ComputeInternalForceBase.h:182-187); 0 when dt = 0
+APHIP_DEV void velocity_gradient(const double *H, const RateIn &r, double *L) {
+#pragma clang fp contract(off)
+    double Fn[9], Fni[9], dH[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) Fn[i] = r.Hn[i];
+    Fn[0] += 1.0;
+    Fn[4] += 1.0;
+    Fn[8] += 1.0;
+    inverse3(Fn, Fni);
+#pragma unroll
+    for (int i = 0; i < 9; ++i) dH[i] = r.dt != 0.0 ? (H[i] - r.Hn[i]) / r.dt : 0.0;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) L[3 * i + j] = dot3(dH[3 * i], Fni[j], dH[3 * i + 1], Fni[3 + j], dH[3 * i + 2], Fni[6 + j]);
+}
+
+// Drucker-Prager (DruckerPrager.h:97-216): Kirchhoff stress tau = P_n F^T plus the
+// hypoelastic increment of de = dt/2 (L + L^T); beyond g = s / sqrt 2 - (A1 - A2 z) >
+// 1e-4 an oblique return in the Lode plane (Newton, at most 25 iterations; the deviatoric
+// direction E_s of the trial stress is kept), plastic strain increment from the
+// elastic part of dstrain = dt/2 (H + H^T) (the reference's choice); P = tau F^-T.
+// State: plastic strain, volumetric plastic strain (both accumulated).  Where the
+// reference aborts on a failed return, P and the state become NaN.
+APHIP_DEV void dp_stress(const double *H, const MatParams &m, const RateIn &r, double sn, double sn2, double &snp1,
+                         double &snp1b, double *P) {
+#pragma clang fp contract(off)
+    const double K = m.bulk, G = m.mu, dt = r.dt;
+    const double ez = 1.0 / sqrt(3.0);  // Ez = I / sqrt 3
+    double F[9], Fi[9], L[9], tau[9], de[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) F[i] = H[i];
+    F[0] += 1.0;
+    F[4] += 1.0;
+    F[8] += 1.0;
+    inverse3(F, Fi);
+    velocity_gradient(H, r, L);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            tau[3 * i + j] = dot3(r.Pn[3 * i], F[3 * j], r.Pn[3 * i + 1], F[3 * j + 1], r.Pn[3 * i + 2], F[3 * j + 2]);
+            de[3 * i + j] = (0.5 * dt) * (L[3 * i + j] + L[3 * j + i]);
+        }
+    const double trde = tr3(de);
+    {
+        const double iso = trde / 3.0, kt = K * trde, g2 = 2 * G;
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                const double dd = i == j ? de[3 * i + j] - iso : de[3 * i + j];
+                tau[3 * i + j] += (i == j ? kt : 0.0) + g2 * dd;
+            }
+    }
+    double zt, st, Es[9];
+    lode(tau, zt, st, Es);
+    double dstr[9], dep[9];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            dstr[3 * i + j] = (0.5 * (H[3 * i + j] + H[3 * j + i])) * dt;
+            dep[3 * i + j] = 0.0;
+        }
+    const double rs2 = sqrt(2.0);
+    double f = m.dp_A1 - m.dp_A2 * zt;
+    double g = st / rs2 - f;
+    bool failed = false;
+    if (g > 1.0e-4) {
+        bool converged = false;
+        const double Nz = m.dp_A2, Ns = 1.0 / rs2, Mz = m.dp_A2G, Ms = Ns;
+        const double Az = K * Mz, As = 2.0 * G * Ms;
+        for (int it = 0; it < 25; ++it) {
+            const double fac = sqrt((zt * zt + st * st) / (Mz * Mz + Ms * Ms)) / K;
+            const double Pz = fac * Az, Ps = fac * As;
+            const double beta = -g / (Nz * Pz + Ns * Ps);
+            if (fabs(beta) < 1.0e-10) {
+                converged = true;
+                break;
+            }
+            zt += beta * Pz;
+            st += beta * Ps;
+            f = m.dp_A1 - m.dp_A2 * zt;
+            g = st / rs2 - f;
+        }
+        failed = !converged;
+        double dtau[9];
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                const int k = 3 * i + j;
+                dtau[k] = zt * (i == j ? ez : 0.0) + st * Es[k] - tau[k];
+                tau[k] += dtau[k];
+            }
+        double dz, ds, Ed[9];
+        lode(dtau, dz, ds, Ed);
+        const double cz = 1.0 / 3.0 / K * dz, cs = 1.0 / 2.0 / G * ds;
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                const int k = 3 * i + j;
+                const double dee = cz * (i == j ? ez : 0.0) + cs * Ed[k];
+                dep[k] = dstr[k] - dee;
+            }
+    }
+    const double tp = tr3(dep);
+    double dd[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) dd[i] = dep[i];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) dd[4 * i] -= tp / 3.0;
+    double ddot = 0.0;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) ddot = __fma_rn(dd[i], dd[i], ddot);
+    const double deqps = sqrt(ddot + (tp * tp) / 3.0);
+    const double nan = __builtin_nan("");
+    snp1 = failed ? nan : sn + deqps;
+    snp1b = failed ? nan : sn2 + tp;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            const double v = dot3(tau[3 * i], Fi[3 * j], tau[3 * i + 1], Fi[3 * j + 1], tau[3 * i + 2], Fi[3 * j + 2]);
+            P[3 * i + j] = failed ? nan : v;
+        }
+}
+
+// P = stress(H).  sn / snp1: J2 equivalent plastic strain, damage, effective creep
+// strain or the Drucker-Prager plastic strain, in / out; aux: the damage model's I1,
+// the creep model's effective stress rate or the Drucker-Prager volumetric plastic
+// strain (whose previous value is sn2).  The smoothed-cell path turns a new failure's
+// marker 2.0 into the maximum damage afterwards (cell_damage_marker).  r: the
+// rate-form inputs (creep, Drucker-Prager).
 template <int KIND>
 APHIP_DEV void stress(const double *H, const MatParams &m, double sn, double &snp1, double &aux, double *P,
-                      const RateIn *r = nullptr) {
+                      const RateIn *r = nullptr, double sn2 = 0.0) {
 #pragma clang fp contract(off)
     if constexpr (KIND == APHIP_LINEAR_ELASTIC) {
         const double two_mu = m.mu * 2.0;
@@ -371,6 +536,8 @@ APHIP_DEV void stress(const double *H, const MatParams &m, double sn, double &sn
         aux = I1;
     } else if constexpr (KIND == APHIP_POWER_LAW_CREEP) {
         creep_stress(H, m, *r, sn, snp1, aux, P);
+    } else if constexpr (KIND == APHIP_DRUCKER_PRAGER) {
+        dp_stress(H, m, *r, sn, sn2, snp1, aux, P);
     } else {  // SVK
         const double two_mu = m.mu * 2.0;
         double E[9], S[9];

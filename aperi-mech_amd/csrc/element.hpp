@@ -53,7 +53,7 @@ APHIP_DEV void store_once(double *p, double v) { __builtin_nontemporal_store(v, 
 template <int KIND>
 APHIP_DEV void tet4_element(const double (&X)[4][3], const double (&U)[4][3], const MatParams &m, double sn,
                             double &snp1, double &aux, double *H, double *P, double (&fe)[4][3],
-                            const RateIn *r = nullptr) {
+                            const RateIn *r = nullptr, double sn2 = 0.0) {
 #pragma clang fp contract(off)
     double J[9], Ji[9];
 #pragma unroll
@@ -77,7 +77,7 @@ APHIP_DEV void tet4_element(const double (&X)[4][3], const double (&U)[4][3], co
 #pragma unroll
         for (int j = 0; j < 3; ++j)
             H[3 * i + j] = dot4(U[0][i], B[0][j], U[1][i], B[1][j], U[2][i], B[2][j], U[3][i], B[3][j]);
-    stress<KIND>(H, m, sn, snp1, aux, P, r);
+    stress<KIND>(H, m, sn, snp1, aux, P, r, sn2);
 #pragma unroll
     for (int k = 0; k < 4; ++k)
 #pragma unroll
@@ -178,8 +178,11 @@ APHIP_DEV void hex8_element(const double (&X)[8][3], const double (&U)[8][3], co
 #pragma unroll
         for (int S = 0; S < 8; ++S) R[j][S] = 0.0;
     const double ga2 = ga * ga;
-    double sn = 0.0;
-    if constexpr (!PER_QP) sn = load_state<KIND>(state_n, 0);
+    double sn = 0.0, sn2 = 0.0;
+    if constexpr (!PER_QP) {
+        sn = load_state<KIND>(state_n, 0);
+        sn2 = load_state2<KIND>(state_n, 0);
+    }
 #pragma unroll 1
     for (int q = 0; q < 8; ++q) {
         int sq0, sq1, sq2;
@@ -206,8 +209,11 @@ APHIP_DEV void hex8_element(const double (&X)[8][3], const double (&U)[8][3], co
             for (int j = 0; j < 3; ++j)
                 H[3 * i + j] = dot3(G[3 * i], Ji[j], G[3 * i + 1], Ji[3 + j], G[3 * i + 2], Ji[6 + j]);
         double snp1 = 0.0, aux = 0.0;
-        if constexpr (PER_QP) sn = load_state<KIND>(state_n, q);
-        stress<KIND>(H, m, sn, snp1, aux, P, r);
+        if constexpr (PER_QP) {
+            sn = load_state<KIND>(state_n, q);
+            sn2 = load_state2<KIND>(state_n, q);
+        }
+        stress<KIND>(H, m, sn, snp1, aux, P, r, sn2);
         if constexpr (PER_QP) store_state<KIND>(state_np1, q, snp1, aux);
         else if (q == 7) store_state<KIND>(state_np1, 0, snp1, aux);
 #pragma unroll
@@ -288,7 +294,8 @@ APHIP_DEV void hex8_qp_lane(int q, const int (&lc)[8], const double *sX, const d
             H[3 * i + j] = dot3(G[3 * i], Ji[j], G[3 * i + 1], Ji[3 + j], G[3 * i + 2], Ji[6 + j]);
     double sn = 0.0, snp1 = 0.0, aux = 0.0;
     sn = load_state<KIND>(state_n, PER_QP ? q : 0);
-    stress<KIND>(H, m, sn, snp1, aux, P, r);
+    const double sn2 = load_state2<KIND>(state_n, PER_QP ? q : 0);
+    stress<KIND>(H, m, sn, snp1, aux, P, r, sn2);
     if constexpr (PER_QP) store_state<KIND>(state_np1, q, snp1, aux);
     else if (q == 7) store_state<KIND>(state_np1, 0, snp1, aux);
     double Q[9];

@@ -84,10 +84,10 @@ __global__ __launch_bounds__(kElemBlock) void k_tet4_force(ElemArgs A) {
         m.mu = A.elem_mu[e];
     }
     double snp1 = 0.0, aux = 0.0, H[9], P[9];
-    const double sn = load_state<KIND>(A.state_n, e);
+    const double sn = load_state<KIND>(A.state_n, e), sn2 = load_state2<KIND>(A.state_n, e);
     RateIn rin;
     load_rate<KIND>(rin, A.Hn, A.Pn, A.ne, e, A.rate_dt);
-    tet4_element<KIND>(X, U, m, sn, snp1, aux, H, P, fe, &rin);
+    tet4_element<KIND>(X, U, m, sn, snp1, aux, H, P, fe, &rin, sn2);
     store_state<KIND>(A.state_np1, e, snp1, aux);
 #pragma unroll
     for (int i = 0; i < 9; ++i) {
@@ -308,6 +308,7 @@ void fem_element_force(Ctx &c) {
         case APHIP_SVK: launch_elem_kind<APHIP_SVK>(c, A); break;
         case APHIP_NEO_HOOKEAN_DAMAGE: launch_elem_kind<APHIP_NEO_HOOKEAN_DAMAGE>(c, A); break;
         case APHIP_POWER_LAW_CREEP: launch_elem_kind<APHIP_POWER_LAW_CREEP>(c, A); break;
+        case APHIP_DRUCKER_PRAGER: launch_elem_kind<APHIP_DRUCKER_PRAGER>(c, A); break;
         default: throw Error(APHIP_ERR_INVALID, "material not set");
     }
     APHIP_HIP(hipGetLastError());

@@ -78,18 +78,19 @@ struct MaterialConst {
     int state_per_qp = 0;
     // neo-Hookean with damage (increment and maximum already clamped to [0, 1])
     double I1_critical = 0.0, I1_failure = 0.0, alpha = 0.0, max_damage = 0.0, max_damage_increment = 0.0;
-    // power-law creep: bulk modulus, rate factor A exp(-m / T), exponent
+    // power-law creep and Drucker-Prager: bulk modulus; creep rate factor A exp(-m / T), exponent
     double bulk = 0.0, creep_rate = 0.0, creep_n = 0.0;
+    double dp_A1 = 0.0, dp_A2 = 0.0, dp_A2G = 0.0;
     MatParams params() const {
-        return MatParams{lambda, mu, yield_stress, hardening, I1_critical, I1_failure, alpha, max_damage,
-                         max_damage_increment, bulk, creep_rate, creep_n};
-    }
-    // state values per evaluation point
-    int state_width() const {
-        return kind == APHIP_J2 ? 1 : (kind == APHIP_NEO_HOOKEAN_DAMAGE || kind == APHIP_POWER_LAW_CREEP) ? 2 : 0;
+        return MatParams{lambda, mu,     yield_stress, hardening, I1_critical, I1_failure, alpha, max_damage,
+                         max_damage_increment, bulk, creep_rate, creep_n, dp_A1, dp_A2, dp_A2G};
     }
     // reads the previous displacement gradient and stress (velocity gradient, old stress)
-    bool rate_form() const { return kind == APHIP_POWER_LAW_CREEP; }
+    bool rate_form() const { return kind == APHIP_POWER_LAW_CREEP || kind == APHIP_DRUCKER_PRAGER; }
+    // state values per evaluation point
+    int state_width() const {
+        return kind == APHIP_J2 ? 1 : (kind == APHIP_NEO_HOOKEAN_DAMAGE || rate_form()) ? 2 : 0;
+    }
 };
 
 // Checkpointed forward run for the adjoint (adjoint.hip).

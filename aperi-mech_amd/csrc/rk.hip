@@ -612,10 +612,10 @@ __global__ __launch_bounds__(256) void k_rk_cells(CellForceArgs A) {
         m.mu = A.elem_mu[c];
     }
     double snp1 = 0.0, aux = 0.0;
-    const double sn = load_state<KIND>(A.state_n, c);
+    const double sn = load_state<KIND>(A.state_n, c), sn2 = load_state2<KIND>(A.state_n, c);
     RateIn rin;
     load_rate<KIND>(rin, A.Hn, A.Pn, A.ne, c, A.rate_dt);
-    stress<KIND>(H, m, sn, snp1, aux, P, &rin);
+    stress<KIND>(H, m, sn, snp1, aux, P, &rin, sn2);
     snp1 = cell_damage_marker<KIND>(snp1, m);
     if (live && l == 0) {
         store_state<KIND>(A.state_np1, c, snp1, aux);
@@ -670,7 +670,7 @@ __global__ __launch_bounds__(256) void k_rk_cells_two_pass(CellForceArgs A, cons
         ld3(A.u, en[k], uj[k]);
     }
     double snp1 = 0.0, aux = 0.0;
-    const double sn = load_state<KIND>(A.state_n, c);
+    const double sn = load_state<KIND>(A.state_n, c), sn2 = load_state2<KIND>(A.state_n, c);
     MatParams m = A.mat;
     if (A.elem_lambda) {
         m.lambda = A.elem_lambda[c];
@@ -692,7 +692,7 @@ __global__ __launch_bounds__(256) void k_rk_cells_two_pass(CellForceArgs A, cons
             for (int b = 0; b < 3; ++b) H[3 * a + b] = __fma_rn(uj[k][a], gc[k][b], H[3 * a + b]);
     RateIn rin;
     load_rate<KIND>(rin, A.Hn, A.Pn, A.ne, c, A.rate_dt);
-    stress<KIND>(H, m, sn, snp1, aux, P, &rin);
+    stress<KIND>(H, m, sn, snp1, aux, P, &rin, sn2);
     snp1 = cell_damage_marker<KIND>(snp1, m);
     if (!live) return;
     store_state<KIND>(A.state_np1, c, snp1, aux);
@@ -1122,6 +1122,9 @@ void rk_cell_force(Ctx &c) {
             case APHIP_POWER_LAW_CREEP:
                 hipLaunchKernelGGL(k_rk_cells_two_pass<APHIP_POWER_LAW_CREEP>, g, dim3(256), 0, c.stream, A, X);
                 break;
+            case APHIP_DRUCKER_PRAGER:
+                hipLaunchKernelGGL(k_rk_cells_two_pass<APHIP_DRUCKER_PRAGER>, g, dim3(256), 0, c.stream, A, X);
+                break;
             default: throw Error(APHIP_ERR_INVALID, "material not set");
         }
     } else {
@@ -1135,6 +1138,9 @@ void rk_cell_force(Ctx &c) {
                 break;
             case APHIP_POWER_LAW_CREEP:
                 hipLaunchKernelGGL((k_rk_cells<APHIP_POWER_LAW_CREEP, kCellLanes>), g, dim3(256), 0, c.stream, A);
+                break;
+            case APHIP_DRUCKER_PRAGER:
+                hipLaunchKernelGGL((k_rk_cells<APHIP_DRUCKER_PRAGER, kCellLanes>), g, dim3(256), 0, c.stream, A);
                 break;
             default: throw Error(APHIP_ERR_INVALID, "material not set");
         }

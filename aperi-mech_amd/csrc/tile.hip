@@ -215,7 +215,11 @@ __global__ __launch_bounds__(NPE == 4 ? 1024 : (QPAR ? 512 : 256), NPE == 8 && !
         lc[4] = c1.x; lc[5] = c1.y; lc[6] = c1.z; lc[7] = c1.w;
     }
     load_elem_lame(A, ec, m);
-    if constexpr (NPE == 4) sn = load_state<KIND>(A.state_n, ec);
+    double sn2 = 0.0;
+    if constexpr (NPE == 4) {
+        sn = load_state<KIND>(A.state_n, ec);
+        sn2 = load_state2<KIND>(A.state_n, ec);
+    }
     RateIn rin;
     load_rate<KIND>(rin, A.Hn, A.Pn, A.ne, ec, A.rate_dt);
 
@@ -315,7 +319,7 @@ __global__ __launch_bounds__(NPE == 4 ? 1024 : (QPAR ? 512 : 256), NPE == 8 && !
         double H[9], P[9];
         if constexpr (NPE == 4) {
             double snp1 = 0.0, aux = 0.0;
-            tet4_element<KIND>(X, U, m, sn, snp1, aux, H, P, fe, &rin);
+            tet4_element<KIND>(X, U, m, sn, snp1, aux, H, P, fe, &rin, sn2);
             store_state<KIND>(A.state_np1, e, snp1, aux);
         } else {
             const int64_t so = state_width<KIND>() * (PER_QP ? e * 8 : e);
@@ -837,6 +841,10 @@ void dispatch_kind(Ctx &c, const TileArgs &A, size_t lds) {
         case APHIP_POWER_LAW_CREEP:
             if (pq) launch_tile<NPE, APHIP_POWER_LAW_CREEP, true, MODE>(c, A, lds);
             else launch_tile<NPE, APHIP_POWER_LAW_CREEP, false, MODE>(c, A, lds);
+            break;
+        case APHIP_DRUCKER_PRAGER:
+            if (pq) launch_tile<NPE, APHIP_DRUCKER_PRAGER, true, MODE>(c, A, lds);
+            else launch_tile<NPE, APHIP_DRUCKER_PRAGER, false, MODE>(c, A, lds);
             break;
         default: throw Error(APHIP_ERR_INVALID, "material not set");
     }

@@ -20,7 +20,7 @@ _PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.normpath(os.path.join(_PKG, "..", "..", "lib", "libaperi_hip.so"))
 
 TET4, HEX8 = 0, 1
-LINEAR_ELASTIC, NEO_HOOKEAN, J2, SVK, NEO_HOOKEAN_DAMAGE, POWER_LAW_CREEP = 0, 1, 2, 3, 4, 5
+LINEAR_ELASTIC, NEO_HOOKEAN, J2, SVK, NEO_HOOKEAN_DAMAGE, POWER_LAW_CREEP, DRUCKER_PRAGER = 0, 1, 2, 3, 4, 5, 6
 # optional neo-Hookean-with-damage keys and the reference's defaults (Materials/Factory.h:78-95)
 DAMAGE_DEFAULTS = dict(maximum_damage=0.99, maximum_damage_increment=1.0, stop_failing_at_time=-1.0,
                        fail_all_at_time=-1.0)
@@ -62,13 +62,16 @@ class MaterialParams(ctypes.Structure):
                 ("stop_failing_at_time", ctypes.c_double), ("fail_all_at_time", ctypes.c_double),
                 ("creep_A", ctypes.c_double), ("creep_n", ctypes.c_double), ("creep_m", ctypes.c_double),
                 ("constant_temperature", ctypes.c_double), ("use_constant_temperature", ctypes.c_int32),
-                ("reserved2", ctypes.c_int32)]
+                ("reserved2", ctypes.c_int32),
+                ("dp_A1", ctypes.c_double), ("dp_A2", ctypes.c_double), ("dp_A2G", ctypes.c_double)]
 
 
-def material_params(density, E, nu, yield_stress=0.0, hardening=0.0, state_per_qp=0, damage=None, creep=None):
+def material_params(density, E, nu, yield_stress=0.0, hardening=0.0, state_per_qp=0, damage=None, creep=None,
+                    drucker_prager=None):
     """aphip_material_params; damage: dict with I1_critical, I1_failure, alpha and optionally
     maximum_damage, maximum_damage_increment, stop_failing_at_time, fail_all_at_time;
-    creep: dict with A, n, m and constant_temperature (the reference requires one)."""
+    creep: dict with A, n, m and constant_temperature (the reference requires one);
+    drucker_prager: dict with A1, A2, A2G."""
     p = MaterialParams(density, E, nu, yield_stress, hardening, state_per_qp, 0)
     if damage is not None:
         for k, v in dict(DAMAGE_DEFAULTS, **damage).items():
@@ -77,6 +80,8 @@ def material_params(density, E, nu, yield_stress=0.0, hardening=0.0, state_per_q
         p.creep_A, p.creep_n, p.creep_m = creep["A"], creep["n"], creep["m"]
         if "constant_temperature" in creep:
             p.constant_temperature, p.use_constant_temperature = creep["constant_temperature"], 1
+    if drucker_prager is not None:
+        p.dp_A1, p.dp_A2, p.dp_A2G = drucker_prager["A1"], drucker_prager["A2"], drucker_prager["A2G"]
     return p
 
 
@@ -176,8 +181,8 @@ class Solver:
         return out
 
     def set_material(self, kind, density, E, nu, yield_stress=0.0, hardening=0.0, state_per_qp=0, damage=None,
-                     creep=None):
-        p = material_params(density, E, nu, yield_stress, hardening, state_per_qp, damage, creep)
+                     creep=None, drucker_prager=None):
+        p = material_params(density, E, nu, yield_stress, hardening, state_per_qp, damage, creep, drucker_prager)
         _check(self.L.aphip_set_material(self.ctx, kind, ctypes.byref(p)))
         self.kind, self.state_per_qp = kind, state_per_qp
         return self
@@ -217,7 +222,7 @@ class Solver:
         if field in (FIELD_H, FIELD_P):
             return (self.ne, 9)
         if field == FIELD_STATE:
-            width = 2 if self.kind in (NEO_HOOKEAN_DAMAGE, POWER_LAW_CREEP) else 1
+            width = 2 if self.kind in (NEO_HOOKEAN_DAMAGE, POWER_LAW_CREEP, DRUCKER_PRAGER) else 1
             return (width * self.ne * (8 if (self.etype == HEX8 and self.state_per_qp) else 1),)
         return (self.ne,)
 

@@ -40,14 +40,15 @@ typedef struct aphip_ctx aphip_ctx;
 enum { APHIP_TET4 = 0, APHIP_HEX8 = 1 };
 
 /* Material kinds: include/Materials/{LinearElastic,NeoHooke,Plastic,Elastic,NeoHookeWithDamage,
- * PowerLawCreep}.h */
+ * PowerLawCreep,DruckerPrager}.h */
 enum {
     APHIP_LINEAR_ELASTIC = 0,
     APHIP_NEO_HOOKEAN = 1,
     APHIP_J2 = 2,
     APHIP_SVK = 3,
     APHIP_NEO_HOOKEAN_DAMAGE = 4,
-    APHIP_POWER_LAW_CREEP = 5
+    APHIP_POWER_LAW_CREEP = 5,
+    APHIP_DRUCKER_PRAGER = 6
 };
 
 /* Fields for aphip_get_field / aphip_set_field.
@@ -55,7 +56,8 @@ enum {
  * Element fields (n_elem x 9): H (displacement_gradient), P (pk1_stress);
  * STATE is n_elem x 1 (J2 equivalent plastic strain; n_elem x 8 with state_per_qp) or
  * n_elem x 2 (neo-Hookean with damage: damage, I1; power-law creep: effective creep strain,
- * effective stress rate; n_elem x 16 with state_per_qp);
+ * effective stress rate; Drucker-Prager: plastic strain, volumetric plastic strain;
+ * n_elem x 16 with state_per_qp);
  * VOLUME is n_elem x 1. */
 enum {
     APHIP_FIELD_U = 0,
@@ -116,6 +118,11 @@ typedef struct {
     double constant_temperature;
     int32_t use_constant_temperature;
     int32_t reserved2;
+    /* Drucker-Prager (include/Materials/DruckerPrager.h, Factory.h:118-132): yield function
+     * s / sqrt(2) = A1 - A2 z of the Lode components, flow direction A2G */
+    double dp_A1;
+    double dp_A2;
+    double dp_A2G;
 } aphip_material_params;
 
 /* ---- lifetime ---------------------------------------------------------- */

@@ -10,6 +10,10 @@
  *                   include/ComputeInternalForceBase.h:182-187 (L = (H - H_n) / dt (I + H_n)^-1);
  *                   the sub-stepping loop of :122-144 leaves after its first pass (tm1 =
  *                   min(critt, dt) <= dt), so one explicit sub-step of min(critt, dt)
+ *   Drucker-Prager  include/Materials/DruckerPrager.h:97-216 (Kirchhoff stress P_n F^T plus the
+ *                   hypoelastic increment, oblique return in the Lode plane by Newton with the
+ *                   trial deviatoric direction, Lode components of include/Tensor.h:34-57; the
+ *                   reference aborts on 25 failed iterations, the oracle returns NaN)
  *   NH with damage  include/Materials/NeoHookeWithDamage.h:180-243 (state: damage, I1; damage
  *                   from I1 between I1_critical and I1_failure with exponent alpha, never
  *                   decreasing, at most maximum_damage_increment per evaluation, capped at
@@ -112,9 +116,11 @@ void or_stress(int kind, double lambda, double mu, double yield_stress, double h
     }
 }
 
-int or_state_width(int kind) { return kind == OR_J2 ? 1 : (kind == OR_NH_DAMAGE || kind == OR_CREEP) ? 2 : 0; }
+int or_state_width(int kind) {
+    return kind == OR_J2 ? 1 : (kind == OR_NH_DAMAGE || kind == OR_CREEP || kind == OR_DP) ? 2 : 0;
+}
 
-int or_rate_form(int kind) { return kind == OR_CREEP; }
+int or_rate_form(int kind) { return kind == OR_CREEP || kind == OR_DP; }
 
 static double mag9(const double *A) {
     double s = 0.0;
@@ -122,9 +128,108 @@ static double mag9(const double *A) {
     return sqrt(s);
 }
 
+static void lode(const double *A, double *z, double *s, double *E) {
+    const double t = A[0] + A[4] + A[8];
+    double d[9];
+    *z = t / sqrt(3.0);
+    for (int i = 0; i < 9; ++i) d[i] = A[i];
+    for (int i = 0; i < 3; ++i) d[4 * i] -= t / 3.0;
+    *s = mag9(d);
+    for (int i = 0; i < 9; ++i) E[i] = fabs(*s) > 1.0e-10 ? d[i] / *s : 0.0;
+}
+
+static void drucker_prager(const or_material *mat, double mu, const double H[9], const double L[9], double dt,
+                           const double Pn[9], const double *state_n, double *state_np1, double P[9]) {
+    const double K = mat->bulk, G = mu, ez = 1.0 / sqrt(3.0);
+    double F[9], Fi[9], J, tau[9], de[9], dstr[9], dep[9] = {0}, Es[9], zt, st;
+    for (int i = 0; i < 9; ++i) F[i] = H[i];
+    F[0] += 1.0; F[4] += 1.0; F[8] += 1.0;
+    or_inverse3_cofactor(F, Fi, &J);
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            double s = 0.0;
+            for (int k = 0; k < 3; ++k) s += Pn[3 * i + k] * F[3 * j + k];
+            tau[3 * i + j] = s;
+            de[3 * i + j] = (0.5 * dt) * (L[3 * i + j] + L[3 * j + i]);
+            dstr[3 * i + j] = 0.5 * (H[3 * i + j] + H[3 * j + i]) * dt;
+        }
+    const double trde = de[0] + de[4] + de[8];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            const double dd = de[3 * i + j] - (i == j ? trde / 3.0 : 0.0);
+            tau[3 * i + j] += (i == j ? K * trde : 0.0) + 2 * G * dd;
+        }
+    lode(tau, &zt, &st, Es);
+    double f = mat->dp_A1 - mat->dp_A2 * zt, g = st / sqrt(2.0) - f;
+    int failed = 0;
+    if (g > 1.0e-4) {
+        int converged = 0;
+        const double Nz = mat->dp_A2, Ns = 1.0 / sqrt(2.0), Mz = mat->dp_A2G, Ms = Ns;
+        for (int it = 0; it < 25; ++it) {
+            const double Az = K * Mz, As = 2.0 * G * Ms;
+            const double fac = sqrt((zt * zt + st * st) / (Mz * Mz + Ms * Ms)) / K;
+            const double Pz = fac * Az, Ps = fac * As;
+            const double beta = -g / (Nz * Pz + Ns * Ps);
+            if (fabs(beta) < 1.0e-10) { converged = 1; break; }
+            zt += beta * Pz;
+            st += beta * Ps;
+            f = mat->dp_A1 - mat->dp_A2 * zt;
+            g = st / sqrt(2.0) - f;
+        }
+        failed = !converged;
+        double dtau[9], dz, ds, Ed[9];
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) {
+                const int k = 3 * i + j;
+                dtau[k] = zt * (i == j ? ez : 0.0) + st * Es[k] - tau[k];
+                tau[k] += dtau[k];
+            }
+        lode(dtau, &dz, &ds, Ed);
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) {
+                const int k = 3 * i + j;
+                const double dee = 1.0 / 3.0 / K * dz * (i == j ? ez : 0.0) + 1.0 / 2.0 / G * ds * Ed[k];
+                dep[k] = dstr[k] - dee;
+            }
+    }
+    const double tp = dep[0] + dep[4] + dep[8];
+    double dd2 = 0.0;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            const double x = dep[3 * i + j] - (i == j ? tp / 3.0 : 0.0);
+            dd2 += x * x;
+        }
+    const double deqps = sqrt(dd2 + (tp * tp) / 3.0);
+    if (state_np1) {
+        state_np1[0] = failed ? NAN : (state_n ? state_n[0] : 0.0) + deqps;
+        state_np1[1] = failed ? NAN : (state_n ? state_n[1] : 0.0) + tp;
+    }
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            double s = 0.0;
+            for (int k = 0; k < 3; ++k) s += tau[3 * i + k] * Fi[3 * j + k];
+            P[3 * i + j] = failed ? NAN : s;
+        }
+}
+
 void or_stress_rate(const or_material *mat, double lambda, double mu, const double H[9], const double Hn[9],
                     const double Pn[9], double dt, const double *state_n, double *state_np1, double P[9]) {
     (void)lambda;
+    if (mat->kind == OR_DP) {
+        double Fn[9], Fni[9], L[9];
+        for (int i = 0; i < 9; ++i) Fn[i] = Hn[i];
+        Fn[0] += 1.0; Fn[4] += 1.0; Fn[8] += 1.0;
+        or_fullpivlu_inverse(3, Fn, Fni);
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) {
+                double s = 0.0;
+                for (int k = 0; k < 3; ++k)
+                    s += (dt != 0.0 ? (H[3 * i + k] - Hn[3 * i + k]) / dt : 0.0) * Fni[3 * k + j];
+                L[3 * i + j] = s;
+            }
+        drucker_prager(mat, mu, H, L, dt, Pn, state_n, state_np1, P);
+        return;
+    }
     const double G = mu, K = mat->bulk;
     double F[9], Fi[9], Fn[9], Fni[9], L[9], J;
     for (int i = 0; i < 9; ++i) {

@@ -26,7 +26,7 @@ extern "C" {
 #endif
 
 enum { OR_TET4 = 0, OR_HEX8 = 1 };
-enum { OR_LINEAR_ELASTIC = 0, OR_NEO_HOOKEAN = 1, OR_J2 = 2, OR_SVK = 3, OR_NH_DAMAGE = 4, OR_CREEP = 5 };
+enum { OR_LINEAR_ELASTIC = 0, OR_NEO_HOOKEAN = 1, OR_J2 = 2, OR_SVK = 3, OR_NH_DAMAGE = 4, OR_CREEP = 5, OR_DP = 6 };
 
 /* Material: uniform parameters, optionally overridden per element. */
 typedef struct {
@@ -41,6 +41,8 @@ typedef struct {
     /* power-law creep (include/Materials/PowerLawCreep.h): bulk modulus, A, n, m and the
      * constant temperature */
     double bulk, creep_A, creep_n, creep_m, temperature;
+    /* Drucker-Prager (include/Materials/DruckerPrager.h; bulk as above) */
+    double dp_A1, dp_A2, dp_A2G;
     /* rate-form materials: H and P of the previous step ([n_elem][9], row-major; owned
      * by the caller, or_solve refreshes them each step) and the increment */
     double *H_n, *P_n;

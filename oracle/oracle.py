@@ -15,7 +15,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = None
 
 TET4, HEX8 = 0, 1
-LINEAR_ELASTIC, NEO_HOOKEAN, J2, SVK, NH_DAMAGE, CREEP = 0, 1, 2, 3, 4, 5
+LINEAR_ELASTIC, NEO_HOOKEAN, J2, SVK, NH_DAMAGE, CREEP, DP = 0, 1, 2, 3, 4, 5, 6
 MAX_NODE_NUM_NEIGHBORS = 40  # include/Constants.h:15
 
 _d = ctypes.POINTER(ctypes.c_double)
@@ -33,6 +33,7 @@ class Material(ctypes.Structure):
                 ("maximum_damage", ctypes.c_double), ("maximum_damage_increment", ctypes.c_double),
                 ("bulk", ctypes.c_double), ("creep_A", ctypes.c_double), ("creep_n", ctypes.c_double),
                 ("creep_m", ctypes.c_double), ("temperature", ctypes.c_double),
+                ("dp_A1", ctypes.c_double), ("dp_A2", ctypes.c_double), ("dp_A2G", ctypes.c_double),
                 ("H_n", _d), ("P_n", _d), ("dt", ctypes.c_double)]
 
 
@@ -164,7 +165,9 @@ def _material(kind, lam, mu, yield_stress=0.0, hardening=0.0, elem_lambda=None, 
     for CREEP; rate: (H_n, P_n, dt) with H_n, P_n float64 [n_elem, 9] arrays (kept alive by
     the caller)."""
     m = Material()
-    if creep is not None:
+    if creep is not None and "A1" in creep:  # Drucker-Prager parameters (A1, A2, A2G, bulk)
+        m.bulk, m.dp_A1, m.dp_A2, m.dp_A2G = creep["bulk"], creep["A1"], creep["A2"], creep["A2G"]
+    elif creep is not None:
         m.bulk, m.creep_A, m.creep_n, m.creep_m = creep["bulk"], creep["A"], creep["n"], creep["m"]
         m.temperature = creep.get("constant_temperature", 0.0)
     if rate is not None:
@@ -181,7 +184,7 @@ def _material(kind, lam, mu, yield_stress=0.0, hardening=0.0, elem_lambda=None, 
 
 
 def state_width(kind):
-    return {J2: 1, NH_DAMAGE: 2, CREEP: 2}.get(kind, 0)
+    return {J2: 1, NH_DAMAGE: 2, CREEP: 2, DP: 2}.get(kind, 0)
 
 
 def state_size(etype, kind, n_elem, state_per_qp=0):
@@ -191,9 +194,11 @@ def state_size(etype, kind, n_elem, state_per_qp=0):
     return w * n_elem * (8 if (etype == HEX8 and state_per_qp) else 1)
 
 
-def stress_creep(lam, mu, H, Hn, Pn, dt, creep, state_n=(0.0, 0.0)):
-    """power-law creep: (P, (effective creep strain, effective stress rate))."""
-    mat = _material(CREEP, lam, mu, creep=creep)
+def stress_creep(lam, mu, H, Hn, Pn, dt, creep, state_n=(0.0, 0.0), kind=None):
+    """power-law creep: (P, (effective creep strain, effective stress rate)); with
+    kind=DP and creep = dict(A1, A2, A2G, bulk), Drucker-Prager: (P, (plastic strain,
+    volumetric plastic strain))."""
+    mat = _material(CREEP if kind is None else kind, lam, mu, creep=creep)
     P = np.zeros(9)
     snp1 = np.zeros(2)
     sn = _f64(np.asarray(state_n, dtype=np.float64))
@@ -220,7 +225,7 @@ def internal_force(etype, X, conn, kind, lam, mu, u, state_n=None, yield_stress=
     ne, nn = conn.shape[0], X.shape[0]
     el = None if elem_lambda is None else _f64(elem_lambda)
     em = None if elem_mu is None else _f64(elem_mu)
-    if kind == CREEP and rate is None:
+    if kind in (CREEP, DP) and rate is None:
         rate = (np.zeros((ne, 9)), np.zeros((ne, 9)), 0.0)
     if rate is not None:
         rate = (_f64(rate[0]), _f64(rate[1]), rate[2])
@@ -408,7 +413,7 @@ def cell_internal_force(n_nodes, cells, kind, lam, mu, u, state_n=None, yield_st
                         nthreads=1, damage=None, creep=None, rate=None):
     start, nodes, dphi, vol = cells
     nc = vol.shape[0]
-    if kind == CREEP and rate is None:
+    if kind in (CREEP, DP) and rate is None:
         rate = (np.zeros((nc, 9)), np.zeros((nc, 9)), 0.0)
     if rate is not None:
         rate = (_f64(rate[0]), _f64(rate[1]), rate[2])

@@ -250,7 +250,24 @@ void CpuCases() {
         EXPECT_TRUE(MakeMaterial(PLASTIC)->HasState());
         EXPECT_TRUE(MakeMaterial(PLASTIC)->NumberOfStateVariables() == 1);
         EXPECT_TRUE(!MakeMaterial(LINEAR_ELASTIC)->HasState());
-        EXPECT_TRUE(Throws([] { MakeMaterial(DRUCKER_PRAGER); }, "not supported"));
+        EXPECT_TRUE(Throws([] { MakeMaterial(NONE); }, "not supported"));
+        // the properties each model requires (include/Materials/Factory.h:70-153)
+        EXPECT_TRUE(Throws([] { MakeMaterial(DRUCKER_PRAGER); }, "Missing material property: A1"));
+        EXPECT_TRUE(Throws([] { MakeMaterial(POWER_LAW_CREEP); }, "Missing material property: A"));
+        EXPECT_TRUE(Throws([] { MakeMaterial(NEO_HOOKEAN_WITH_DAMAGE); }, "Missing material property: I1_critical"));
+        {
+            auto p = std::make_shared<MaterialProperties>();
+            p->material_type = DRUCKER_PRAGER;
+            p->density = 1000.0;
+            p->linear_elastic_properties.youngs_modulus = 1e9;
+            p->linear_elastic_properties.poisson_ratio = 0.3;
+            p->properties = {{"A1", 3e6}, {"A2", 0.2}, {"A2G", 0.1}};
+            auto m = CreateMaterial(p);
+            EXPECT_TRUE(m->Kind() == APHIP_DRUCKER_PRAGER && m->NumberOfStateVariables() == 2);
+            EXPECT_TRUE(m->NeedsVelocityGradient() && m->NeedsOldStress());
+            const aphip_material_params q = m->Params();
+            EXPECT_TRUE(q.dp_A1 == 3e6 && q.dp_A2 == 0.2 && q.dp_A2G == 0.1);
+        }
         auto p = std::make_shared<MaterialProperties>();
         p->material_type = PLASTIC;
         p->linear_elastic_properties.youngs_modulus = 1.0;

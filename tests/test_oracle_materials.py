@@ -258,3 +258,71 @@ def test_creep_pressure_does_not_creep():
     s0, s1 = p0 @ F.T / J, p1 @ F.T / J
     assert np.trace(s0) == pytest.approx(np.trace(s1), rel=1e-12)
     assert np.abs(s0 - s1).max() > 1e-6 * np.abs(s0).max()  # the deviator did creep
+
+
+# ---- Drucker-Prager (include/Materials/DruckerPrager.h:97-216) ----
+# The reference has no test of this material; the closed forms below pin it.
+
+def _dp(E, nu, A1, A2, A2G):
+    return dict(A1=A1, A2=A2, A2G=A2G, bulk=E / (3 * (1 - 2 * nu)))
+
+
+def _dp_stress(lam, mu, H, Pn, dt, prm, state_n=(0.0, 0.0)):
+    return O.stress_creep(lam, mu, H, np.zeros(9), Pn, dt, prm, state_n, kind=O.DP)
+
+
+def test_dp_elastic_is_a_kirchhoff_hooke_increment():
+    E, nu = 1e9, 0.3
+    lam, mu = O.lame(E, nu)
+    rng = np.random.default_rng(8)
+    for _ in range(10):
+        H = 1e-4 * rng.standard_normal((3, 3))
+        P, st = _dp_stress(lam, mu, H, np.zeros(9), 1e-5, _dp(E, nu, 1e9, 0.1, 0.1), state_n=(0.2, -0.1))
+        eps = 0.5 * (H + H.T)
+        tau = lam * np.trace(eps) * np.eye(3) + 2 * mu * eps
+        Pc = tau @ np.linalg.inv(np.eye(3) + H).T
+        assert np.allclose(P, Pc, rtol=0, atol=1e-12 * np.abs(Pc).max())
+        assert st[0] == 0.2 and st[1] == -0.1  # no plastic increment
+
+
+def _lode(A):
+    z = np.trace(A) / np.sqrt(3)
+    d = A - np.trace(A) / 3 * np.eye(3)
+    return z, np.linalg.norm(d), d / np.linalg.norm(d)
+
+
+def test_dp_cylinder_returns_radially():
+    # A2 = A2G = 0: the trial pressure is kept and s goes to sqrt(2) A1 along the trial
+    # deviator; the plastic strain has the trace of dstrain = dt sym(H)
+    E, nu = 1e9, 0.3
+    lam, mu = O.lame(E, nu)
+    H = np.array([[2e-3, 5e-4, 0.0], [1e-4, -1e-3, 3e-4], [0.0, 2e-4, 5e-4]])
+    dt = 1e-5
+    A1 = 1e6
+    P, st = _dp_stress(lam, mu, H, np.zeros(9), dt, _dp(E, nu, A1, 0.0, 0.0))
+    eps = 0.5 * (H + H.T)
+    trial = lam * np.trace(eps) * np.eye(3) + 2 * mu * eps
+    z0, s0, E0 = _lode(trial)
+    assert s0 / np.sqrt(2) > A1  # the trial state yields
+    tau = P @ (np.eye(3) + H).T
+    z1, s1, E1 = _lode(tau)
+    assert z1 == pytest.approx(z0, rel=1e-12)
+    assert s1 == pytest.approx(np.sqrt(2) * A1, rel=1e-12)
+    assert np.allclose(E1, E0, atol=1e-12)
+    assert st[1] == pytest.approx(np.trace(eps) * dt, rel=1e-10, abs=1e-22)
+
+
+@pytest.mark.parametrize("A2,A2G", [(0.2, 0.2), (0.3, 0.1)])
+def test_dp_return_lands_on_the_yield_surface(A2, A2G):
+    E, nu = 1e9, 0.3
+    lam, mu = O.lame(E, nu)
+    H = np.array([[3e-3, 1e-3, 0.0], [0.0, -2e-3, 5e-4], [1e-4, 0.0, 1e-3]])
+    A1 = 2e6
+    P, st = _dp_stress(lam, mu, H, np.zeros(9), 1e-5, _dp(E, nu, A1, A2, A2G))
+    eps = 0.5 * (H + H.T)
+    z0, s0, E0 = _lode(lam * np.trace(eps) * np.eye(3) + 2 * mu * eps)
+    assert s0 / np.sqrt(2) - (A1 - A2 * z0) > 1e-4
+    z1, s1, E1 = _lode(P @ (np.eye(3) + H).T)
+    assert abs(s1 / np.sqrt(2) - (A1 - A2 * z1)) < 1e-9 * A1  # consistency
+    assert np.allclose(E1, E0, atol=1e-10)  # the trial deviatoric direction
+    assert st[0] > 0.0
