@@ -258,6 +258,8 @@ class Device {
 enum class ElementTopology { Tetrahedron4, Hexahedron8 };       // include/Constants.h:23-25
 enum class ApproximationSpaceType { FiniteElement, ReproducingKernel };
 enum class IntegrationSchemeType { GaussQuadrature, StrainSmoothing };
+// include/Constants.h:33-36 (Incremental and Semi are not supported here)
+enum class LagrangianFormulationType { Total, Incremental, Updated, Semi };
 
 // An element block in the caller's numbering: coordinates n x 3, connectivity
 // n_elem x nodes_per_element (0-based).
@@ -344,9 +346,14 @@ class ElementBase {
 class ElementGaussianHip : public ElementBase {
    public:
     ElementGaussianHip(ElementTopology topology, const MeshBlock &mesh, std::shared_ptr<Device> device,
-                       std::shared_ptr<Material> material)
+                       std::shared_ptr<Material> material,
+                       LagrangianFormulationType formulation = LagrangianFormulationType::Total)
         : ElementBase(topology == ElementTopology::Tetrahedron4 ? 4 : 8, std::move(device), std::move(material)) {
         BindMesh(topology == ElementTopology::Tetrahedron4 ? APHIP_TET4 : APHIP_HEX8, mesh);
+        if (formulation == LagrangianFormulationType::Updated)
+            Check(aphip_set_formulation(Context(), APHIP_UPDATED_LAGRANGIAN));
+        else if (formulation != LagrangianFormulationType::Total)
+            throw std::runtime_error("Lagrangian formulation not supported by the MI355X force step");
     }
     void ComputeInternalForceAllElements(double time_increment, double time) override {
         Check(aphip_internal_force(Context(), time, time_increment));
@@ -386,7 +393,8 @@ inline std::shared_ptr<ElementBase> CreateElement(ElementTopology topology, Appr
                                                   IntegrationSchemeType scheme, const MeshBlock &mesh,
                                                   std::shared_ptr<Device> device, std::shared_ptr<Material> material,
                                                   double kernel_radius_scale_factor = 0.0,
-                                                  bool use_one_pass_method = true) {
+                                                  bool use_one_pass_method = true,
+                                                  LagrangianFormulationType formulation = LagrangianFormulationType::Total) {
     if (space == ApproximationSpaceType::ReproducingKernel) {
         if (scheme != IntegrationSchemeType::StrainSmoothing)
             throw std::runtime_error("Gauss Quadrature is not supported for Reproducing Kernel");
@@ -400,7 +408,7 @@ inline std::shared_ptr<ElementBase> CreateElement(ElementTopology topology, Appr
             throw std::runtime_error("Strain Smoothing is not supported for finite elements and Hexahedron8");
         throw std::runtime_error("Finite-element strain smoothing on Tetrahedron4 is not supported by the MI355X force step");
     }
-    return std::make_shared<ElementGaussianHip>(topology, mesh, std::move(device), std::move(material));
+    return std::make_shared<ElementGaussianHip>(topology, mesh, std::move(device), std::move(material), formulation);
 }
 
 // include/InternalForceContribution.h:26-146: the force contribution of one block.

@@ -179,13 +179,16 @@ static void alloc_state(Ctx &c) {
             c.state[s].release();
         }
     }
-    if (c.mat.rate_form()) {  // the N halves of the H / P pairs start at zero
+    if (c.h_pair()) {  // the N halves of the H / P pairs start at zero
         c.H1.alloc(9 * std::max<int64_t>(c.ne, 1));
-        c.P1.alloc(9 * std::max<int64_t>(c.ne, 1));
         c.H1.zero(c.stream);
-        c.P1.zero(c.stream);
     } else {
         c.H1.release();
+    }
+    if (c.mat.rate_form()) {
+        c.P1.alloc(9 * std::max<int64_t>(c.ne, 1));
+        c.P1.zero(c.stream);
+    } else {
         c.P1.release();
     }
     c.state_np1 = 1;
@@ -242,9 +245,17 @@ void renumber_nodes(Ctx &c, const std::vector<int32_t> &old_of_new) {
     if (c.has_mass) permute_device(c, c.mass, 1, old_of_new);
     if (c.has_fixed) permute_device(c, c.fixed, 1, old_of_new);
     if (c.pm_eig_valid) permute_device(c, c.pm_eig, 3, old_of_new);
+    if (c.formulation == APHIP_UPDATED_LAGRANGIAN) {
+        permute_device(c, c.xc[0], 3, old_of_new);
+        permute_device(c, c.xc[1], 3, old_of_new);
+        permute_device(c, c.dinc, 3, old_of_new);
+    }
 }
 
-static bool use_tiles(const Ctx &c) { return !c.cells.ready && c.tiles.ready && !(c.flags & APHIP_FLAG_TWO_PASS); }
+static bool use_tiles(const Ctx &c) {
+    return !c.cells.ready && c.tiles.ready && !(c.flags & APHIP_FLAG_TWO_PASS) &&
+           c.formulation == APHIP_TOTAL_LAGRANGIAN;
+}
 
 static void default_tiling(Ctx &c) {
     if (c.ne == 0 || (c.flags & APHIP_FLAG_TWO_PASS)) return;
@@ -275,6 +286,10 @@ static void setup_mesh(Ctx &c, int32_t elem_type, int64_t n_nodes, const double 
     for (int64_t i = 0; i < n_elem * npe; ++i)
         APHIP_REQUIRE(conn[i] >= 0 && conn[i] < n_nodes, "connectivity references a node out of range");
     c.cells.reset();
+    c.formulation = APHIP_TOTAL_LAGRANGIAN;
+    c.xc[0].release();
+    c.xc[1].release();
+    c.dinc.release();
     c.pm_eig_valid = false;
     c.tiles.reset();
     c.graph_reset();
@@ -407,6 +422,7 @@ struct PowerMethodResult {
 static PowerMethodResult power_method(Ctx &c, double prev_dt, int n_iter, double eps, double tol, double vtol,
                                       uint64_t seed) {
     const int64_t n3 = 3 * c.nn;
+    APHIP_REQUIRE(c.formulation == APHIP_TOTAL_LAGRANGIAN, "the power method runs with the total formulation");
     if (c.has_fixed) {  // PowerMethodProcessor::CheckEssentialBoundaries (:159-173)
         std::vector<uint8_t> fx(c.nn);
         c.fixed.download(fx.data(), fx.size(), c.stream);
@@ -635,6 +651,38 @@ int aphip_set_material(aphip_ctx *c, int32_t kind, const aphip_material_params *
     c->graph_reset();
     c->mat = m;
     if (c->etype >= 0) alloc_state(*c);
+    APHIP_HIP(hipStreamSynchronize(c->stream));
+    APHIP_API_END
+}
+
+int aphip_set_formulation(aphip_ctx *c, int32_t formulation) {
+    APHIP_API_BEGIN
+    APHIP_REQUIRE(c != nullptr, "ctx is NULL");
+    APHIP_REQUIRE(c->etype >= 0 && c->mat.kind >= 0, "mesh and material must be set");
+    APHIP_REQUIRE(formulation == APHIP_TOTAL_LAGRANGIAN || formulation == APHIP_UPDATED_LAGRANGIAN,
+                  "formulation must be total or updated Lagrangian");
+    APHIP_REQUIRE(formulation == APHIP_TOTAL_LAGRANGIAN || !c->cells.ready,
+                  "the updated formulation applies to the Gaussian elements, not to RKPM cells");
+    APHIP_REQUIRE(formulation == APHIP_TOTAL_LAGRANGIAN || c->n_specimens == 0,
+                  "ensembles use the total Lagrangian formulation");
+    APHIP_HIP(hipSetDevice(c->device));
+    APHIP_HIP(hipStreamSynchronize(c->stream));
+    c->graph_reset();
+    c->formulation = formulation;
+    if (formulation == APHIP_UPDATED_LAGRANGIAN) {
+        for (int s = 0; s < 2; ++s) {
+            c->xc[s].alloc(3 * c->nn);
+            APHIP_HIP(hipMemcpyAsync(c->xc[s].p, c->X.p, sizeof(double) * 3 * c->nn, hipMemcpyDeviceToDevice,
+                                     c->stream));
+        }
+        c->dinc.alloc(3 * c->nn);
+        c->dinc.zero(c->stream);
+    } else {
+        c->xc[0].release();
+        c->xc[1].release();
+        c->dinc.release();
+    }
+    alloc_state(*c);
     APHIP_HIP(hipStreamSynchronize(c->stream));
     APHIP_API_END
 }

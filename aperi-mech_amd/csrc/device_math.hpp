@@ -103,15 +103,17 @@ struct RateIn {
     double dt;
 };
 
-// H_n, P_n of element e from the SoA (9 x ne) N buffers; nothing for other materials.
-template <int KIND>
+// H_n, P_n of element e from the SoA (9 x ne) N buffers: both for the rate-form
+// materials, H_n alone for the updated Lagrangian formulation (UL), nothing otherwise.
+template <int KIND, bool UL = false>
 APHIP_DEV void load_rate(RateIn &r, const double *Hn, const double *Pn, int64_t ne, int64_t e, double dt) {
+    if constexpr (rate_form<KIND>() || UL) {
+#pragma unroll
+        for (int i = 0; i < 9; ++i) r.Hn[i] = Hn[i * ne + e];
+    }
     if constexpr (rate_form<KIND>()) {
 #pragma unroll
-        for (int i = 0; i < 9; ++i) {
-            r.Hn[i] = Hn[i * ne + e];
-            r.Pn[i] = Pn[i * ne + e];
-        }
+        for (int i = 0; i < 9; ++i) r.Pn[i] = Pn[i * ne + e];
         r.dt = dt;
     }
 }
@@ -561,6 +563,41 @@ APHIP_DEV void stress(const double *H, const MatParams &m, double sn, double &sn
                                     S[6 + j], (i == 2 ? 1.0 : 0.0) + H[3 * i + 2]);
         snp1 = sn;
     }
+}
+
+// Updated Lagrangian (ComputeInternalForceBase.h:158-166): with dH the gradient of the
+// step's displacement increment over the start-of-step configuration, the total
+// displacement gradient is H = dH + H_n + dH H_n.
+APHIP_DEV void ul_gradient(double *H, const double *Hn) {
+#pragma clang fp contract(off)
+    double dH[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) dH[i] = H[i];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            H[3 * i + j] = (dH[3 * i + j] + Hn[3 * i + j]) +
+                           dot3(dH[3 * i], Hn[j], dH[3 * i + 1], Hn[3 + j], dH[3 * i + 2], Hn[6 + j]);
+}
+
+// The reference integrates the force with B F_r and w / det F_r, F_r = I + H_n
+// (ComputeInternalForceGaussian.h:109-123).  Since (B F_r) P^T = B (P F_r^T)^T, the
+// same force comes from the start-of-step B and w with Pt = P F_r^T / det F_r.
+APHIP_DEV void ul_force_stress(const double *P, const double *Hn, double *Pt) {
+#pragma clang fp contract(off)
+    double Fr[9], Fri[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) Fr[i] = Hn[i];
+    Fr[0] += 1.0;
+    Fr[4] += 1.0;
+    Fr[8] += 1.0;
+    const double idet = 1.0 / inverse3(Fr, Fri);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            Pt[3 * i + j] = dot3(P[3 * i], Fr[3 * j], P[3 * i + 1], Fr[3 * j + 1], P[3 * i + 2], Fr[3 * j + 2]) * idet;
 }
 
 // The smoothed-cell path's handling of a new failure (ComputeInternalForceSmoothedCell.h:

@@ -49,8 +49,10 @@ APHIP_DEV void cd_correct(const double *f, double m, unsigned fixed, double dt, 
 // data that neighbouring tiles re-read from L2.
 APHIP_DEV void store_once(double *p, double v) { __builtin_nontemporal_store(v, p); }
 
-// Tet4: fe[k][j] = -(B_k . P_j) w; sn, snp1, aux, r as for stress<KIND>
-template <int KIND>
+// Tet4: fe[k][j] = -(B_k . P_j) w; sn, snp1, aux, r as for stress<KIND>.  UL: X are the
+// start-of-step coordinates, U the step's displacement increments, r->Hn the element's
+// previous displacement gradient (updated Lagrangian).
+template <int KIND, bool UL = false>
 APHIP_DEV void tet4_element(const double (&X)[4][3], const double (&U)[4][3], const MatParams &m, double sn,
                             double &snp1, double &aux, double *H, double *P, double (&fe)[4][3],
                             const RateIn *r = nullptr, double sn2 = 0.0) {
@@ -77,12 +79,20 @@ APHIP_DEV void tet4_element(const double (&X)[4][3], const double (&U)[4][3], co
 #pragma unroll
         for (int j = 0; j < 3; ++j)
             H[3 * i + j] = dot4(U[0][i], B[0][j], U[1][i], B[1][j], U[2][i], B[2][j], U[3][i], B[3][j]);
+    if constexpr (UL) ul_gradient(H, r->Hn);
     stress<KIND>(H, m, sn, snp1, aux, P, r, sn2);
+    double Pf[9];
+    if constexpr (UL) {
+        ul_force_stress(P, r->Hn, Pf);
+    } else {
+#pragma unroll
+        for (int i = 0; i < 9; ++i) Pf[i] = P[i];
+    }
 #pragma unroll
     for (int k = 0; k < 4; ++k)
 #pragma unroll
         for (int j = 0; j < 3; ++j)
-            fe[k][j] = -(dot3(B[k][0], P[3 * j], B[k][1], P[3 * j + 1], B[k][2], P[3 * j + 2]) * w);
+            fe[k][j] = -(dot3(B[k][0], Pf[3 * j], B[k][1], Pf[3 * j + 1], B[k][2], Pf[3 * j + 2]) * w);
 }
 
 APHIP_DEV void hex8_sign(int k, int &s0, int &s1, int &s2) {
@@ -154,7 +164,7 @@ APHIP_DEV void hex8_grad(const double (&w)[8], double p0, double p1, double p2, 
 // state_n / state_np1 point at the element's state slot(s) (per_qp: 8 slots), each
 // state_width<KIND>() values wide.  r: the element's H_n, P_n (those of its last Gauss
 // point in the previous step, like the reference's element fields) for every point.
-template <int KIND, bool PER_QP>
+template <int KIND, bool PER_QP, bool UL = false>
 APHIP_DEV void hex8_element(const double (&X)[8][3], const double (&U)[8][3], const MatParams &m, double ga,
                             const double *state_n, double *state_np1, double *H, double *P, double (&fe)[8][3],
                             const RateIn *r = nullptr) {
@@ -213,15 +223,23 @@ APHIP_DEV void hex8_element(const double (&X)[8][3], const double (&U)[8][3], co
             sn = load_state<KIND>(state_n, q);
             sn2 = load_state2<KIND>(state_n, q);
         }
+        if constexpr (UL) ul_gradient(H, r->Hn);
         stress<KIND>(H, m, sn, snp1, aux, P, r, sn2);
         if constexpr (PER_QP) store_state<KIND>(state_np1, q, snp1, aux);
         else if (q == 7) store_state<KIND>(state_np1, 0, snp1, aux);
+        double Pf[9];
+        if constexpr (UL) {
+            ul_force_stress(P, r->Hn, Pf);
+        } else {
+#pragma unroll
+            for (int i = 0; i < 9; ++i) Pf[i] = P[i];
+        }
 #pragma unroll
         for (int j = 0; j < 3; ++j) {
             // Q[n] = -(w / 8) (J^-1 P^T)[n][j]: the force of axis n's derivative
             double Q[3];
 #pragma unroll
-            for (int n = 0; n < 3; ++n) Q[n] = w * dot3(Ji[3 * n], P[3 * j], Ji[3 * n + 1], P[3 * j + 1], Ji[3 * n + 2], P[3 * j + 2]);
+            for (int n = 0; n < 3; ++n) Q[n] = w * dot3(Ji[3 * n], Pf[3 * j], Ji[3 * n + 1], Pf[3 * j + 1], Ji[3 * n + 2], Pf[3 * j + 2]);
             R[j][1] += Q[0];
             R[j][2] += Q[1];
             R[j][4] += Q[2];

@@ -66,7 +66,8 @@ __device__ __forceinline__ void load3(const double *p, int64_t i, double *o) {
     o[2] = p[3 * i + 2];
 }
 
-template <int KIND>
+// UL: updated Lagrangian (A.X the start-of-step coordinates, A.u the increments)
+template <int KIND, bool UL>
 __global__ __launch_bounds__(kElemBlock) void k_tet4_force(ElemArgs A) {
     const int64_t e = xcd_block(A.nblocks) * kElemBlock + threadIdx.x;
     if (e >= A.ne) return;
@@ -86,8 +87,8 @@ __global__ __launch_bounds__(kElemBlock) void k_tet4_force(ElemArgs A) {
     double snp1 = 0.0, aux = 0.0, H[9], P[9];
     const double sn = load_state<KIND>(A.state_n, e), sn2 = load_state2<KIND>(A.state_n, e);
     RateIn rin;
-    load_rate<KIND>(rin, A.Hn, A.Pn, A.ne, e, A.rate_dt);
-    tet4_element<KIND>(X, U, m, sn, snp1, aux, H, P, fe, &rin, sn2);
+    load_rate<KIND, UL>(rin, A.Hn, A.Pn, A.ne, e, A.rate_dt);
+    tet4_element<KIND, UL>(X, U, m, sn, snp1, aux, H, P, fe, &rin, sn2);
     store_state<KIND>(A.state_np1, e, snp1, aux);
 #pragma unroll
     for (int i = 0; i < 9; ++i) {
@@ -101,7 +102,7 @@ __global__ __launch_bounds__(kElemBlock) void k_tet4_force(ElemArgs A) {
         for (int j = 0; j < 3; ++j) fp[3 * k + j] = fe[k][j];
 }
 
-template <int KIND, bool PER_QP>
+template <int KIND, bool PER_QP, bool UL>
 __global__ __launch_bounds__(128) void k_hex8_force(ElemArgs A) {
     const int64_t e = xcd_block(A.nblocks) * 128 + threadIdx.x;
     if (e >= A.ne) return;
@@ -122,9 +123,9 @@ __global__ __launch_bounds__(128) void k_hex8_force(ElemArgs A) {
     const int64_t so = state_width<KIND>() * (PER_QP ? e * 8 : e);
     double H[9], P[9];
     RateIn rin;
-    load_rate<KIND>(rin, A.Hn, A.Pn, A.ne, e, A.rate_dt);
-    hex8_element<KIND, PER_QP>(X, U, m, A.gauss_a, state_width<KIND>() ? A.state_n + so : nullptr,
-                               state_width<KIND>() ? A.state_np1 + so : nullptr, H, P, fe, &rin);
+    load_rate<KIND, UL>(rin, A.Hn, A.Pn, A.ne, e, A.rate_dt);
+    hex8_element<KIND, PER_QP, UL>(X, U, m, A.gauss_a, state_width<KIND>() ? A.state_n + so : nullptr,
+                                   state_width<KIND>() ? A.state_np1 + so : nullptr, H, P, fe, &rin);
 #pragma unroll
     for (int i = 0; i < 9; ++i) {
         store_once(A.H + i * A.ne + e, H[i]);
@@ -150,6 +151,10 @@ struct NodeArgs {
     double *u;
     const double *node_dt;
     double dt;
+    // updated Lagrangian: the step's increments and the current coordinates at n, n+1
+    double *dinc;
+    const double *xn;
+    double *xnp1;
 };
 
 // v1/2 = v + dt/2 a; velocity BC; u += dt v1/2  (ExplicitTimeIntegrator.h:116-126, :348-357)
@@ -165,6 +170,28 @@ __global__ __launch_bounds__(kNodeBlock) void k_predict(NodeArgs A) {
     for (int d = 0; d < 3; ++d) {
         A.v[3 * i + d] = vh[d];
         A.u[3 * i + d] = u1[d];
+    }
+}
+
+// Updated Lagrangian predictor (ExplicitTimeIntegrator.h:193-226): v1/2 as above, the
+// increment dt v1/2, u += increment, x_{n+1} = x_n + increment.
+__global__ __launch_bounds__(kNodeBlock) void k_predict_ul(NodeArgs A) {
+#pragma clang fp contract(off)
+    const int64_t i = (int64_t)blockIdx.x * kNodeBlock + threadIdx.x;
+    if (i >= A.nn) return;
+    double u[3], v[3], a[3], vh[3], u1[3], x[3];
+    load3(A.u, i, u);
+    load3(A.v, i, v);
+    load3(A.a, i, a);
+    load3(A.xn, i, x);
+    cd_predict(u, v, a, A.fixed ? A.fixed[i] : 0u, A.dt, vh, u1);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const double inc = A.dt * vh[d];
+        A.v[3 * i + d] = vh[d];
+        A.u[3 * i + d] = u[d] + inc;
+        A.dinc[3 * i + d] = inc;
+        A.xnp1[3 * i + d] = x[d] + inc;
     }
 }
 
@@ -268,29 +295,36 @@ __global__ __launch_bounds__(kNodeBlock) void k_lumped_mass(int64_t nn, const in
     mass[i] = m;
 }
 
+template <int KIND, bool UL>
+void launch_elem_kind_ul(Ctx &c, const ElemArgs &A) {
+    if (c.etype == APHIP_TET4) {
+        hipLaunchKernelGGL((k_tet4_force<KIND, UL>), dim3((unsigned)A.nblocks), dim3(kElemBlock), 0, c.stream, A);
+    } else if (state_width<KIND>() && c.mat.state_per_qp) {
+        hipLaunchKernelGGL((k_hex8_force<KIND, true, UL>), dim3((unsigned)A.nblocks), dim3(128), 0, c.stream, A);
+    } else {
+        hipLaunchKernelGGL((k_hex8_force<KIND, false, UL>), dim3((unsigned)A.nblocks), dim3(128), 0, c.stream, A);
+    }
+}
+
 template <int KIND>
 void launch_elem_kind(Ctx &c, const ElemArgs &A) {
-    if (c.etype == APHIP_TET4) {
-        hipLaunchKernelGGL(k_tet4_force<KIND>, dim3((unsigned)A.nblocks), dim3(kElemBlock), 0, c.stream, A);
-    } else if (state_width<KIND>() && c.mat.state_per_qp) {
-        hipLaunchKernelGGL((k_hex8_force<KIND, true>), dim3((unsigned)A.nblocks), dim3(128), 0, c.stream, A);
-    } else {
-        hipLaunchKernelGGL((k_hex8_force<KIND, false>), dim3((unsigned)A.nblocks), dim3(128), 0, c.stream, A);
-    }
+    if (c.formulation == APHIP_UPDATED_LAGRANGIAN) launch_elem_kind_ul<KIND, true>(c, A);
+    else launch_elem_kind_ul<KIND, false>(c, A);
 }
 
 }  // namespace
 
 void fem_element_force(Ctx &c) {
+    const bool ul = c.formulation == APHIP_UPDATED_LAGRANGIAN;
     ElemArgs A;
-    A.X = c.X.p;
-    A.u = c.u.p;
+    A.X = ul ? c.xcn().p : c.X.p;
+    A.u = ul ? c.dinc.p : c.u.p;
     A.conn = c.conn.p;
     A.ne = c.ne;
     A.nblocks = padded_blocks(c.ne, c.etype == APHIP_TET4 ? kElemBlock : 128);
     A.H = c.Hnp1().p;
     A.P = c.Pnp1().p;
-    A.Hn = c.mat.rate_form() ? c.Hn().p : nullptr;
+    A.Hn = c.h_pair() ? c.Hn().p : nullptr;
     A.Pn = c.mat.rate_form() ? c.Pn().p : nullptr;
     A.rate_dt = c.rate_dt;
     A.state_n = c.state[1 - c.state_np1].p;
@@ -328,14 +362,18 @@ static NodeArgs node_args(Ctx &c, double dt) {
     A.u = c.u.p;
     A.node_dt = c.node_dt.p;
     A.dt = dt;
+    A.dinc = c.dinc.p;
+    A.xn = c.formulation == APHIP_UPDATED_LAGRANGIAN ? c.xcn().p : nullptr;
+    A.xnp1 = c.formulation == APHIP_UPDATED_LAGRANGIAN ? c.xcnp1().p : nullptr;
     return A;
 }
 
 void fem_predict(Ctx &c, double dt) {
     if (c.nn == 0) return;
     NodeArgs A = node_args(c, dt);
-    hipLaunchKernelGGL(k_predict, dim3((unsigned)((c.nn + kNodeBlock - 1) / kNodeBlock)), dim3(kNodeBlock), 0,
-                       c.stream, A);
+    const dim3 grid((unsigned)((c.nn + kNodeBlock - 1) / kNodeBlock));
+    if (c.formulation == APHIP_UPDATED_LAGRANGIAN) hipLaunchKernelGGL(k_predict_ul, grid, dim3(kNodeBlock), 0, c.stream, A);
+    else hipLaunchKernelGGL(k_predict, grid, dim3(kNodeBlock), 0, c.stream, A);
     APHIP_HIP(hipGetLastError());
 }
 

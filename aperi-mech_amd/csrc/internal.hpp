@@ -241,9 +241,16 @@ struct Ctx {
     DevBuf<double> state[2];             // material state (N / NP1)
     int state_np1 = 1;                   // index of the NP1 state buffer
     double rate_dt = 0.0;                // increment for force-only evaluations of rate-form materials
-    // H and P of the current evaluation (NP1) and of the previous step (N, rate form
-    // only); the pairs swap with the state buffers
-    DevBuf<double> &Hnp1() { return mat.rate_form() && state_np1 == 0 ? H1 : H; }
+    int formulation = APHIP_TOTAL_LAGRANGIAN;
+    DevBuf<double> xc[2];                // updated Lagrangian: current coordinates (N / NP1 with the state)
+    DevBuf<double> dinc;                 // updated Lagrangian: the step's displacement increments
+    DevBuf<double> &xcn() { return xc[1 - state_np1]; }
+    DevBuf<double> &xcnp1() { return xc[state_np1]; }
+    // H of the previous step is read by the rate-form materials and the updated formulation
+    bool h_pair() const { return mat.rate_form() || formulation == APHIP_UPDATED_LAGRANGIAN; }
+    // H and P of the current evaluation (NP1) and of the previous step (N); the pairs
+    // swap with the state buffers
+    DevBuf<double> &Hnp1() { return h_pair() && state_np1 == 0 ? H1 : H; }
     DevBuf<double> &Pnp1() { return mat.rate_form() && state_np1 == 0 ? P1 : P; }
     DevBuf<double> &Hn() { return state_np1 == 0 ? H : H1; }
     DevBuf<double> &Pn() { return state_np1 == 0 ? P : P1; }

@@ -76,6 +76,10 @@ enum {
     APHIP_FIELD_RK_FORCE_LOCAL = 10
 };
 
+/* Kinematic formulation: the values of LagrangianFormulationType (include/Constants.h:33-36;
+ * Incremental = 1 and Semi = 3 are not supported). */
+enum { APHIP_TOTAL_LAGRANGIAN = 0, APHIP_UPDATED_LAGRANGIAN = 2 };
+
 /* aphip_config.flags */
 #define APHIP_FLAG_TWO_PASS 1 /* element partials through HBM + node CSR gather (no LDS tiles) */
 #define APHIP_FLAG_PIPELINED 2 /* Tet4 step as a persistent, software-pipelined kernel instead of
@@ -151,6 +155,18 @@ int aphip_set_material(aphip_ctx *ctx, int32_t kind, const aphip_material_params
 /* Per-element Young's modulus / Poisson ratio (extension; no reference counterpart:
  * the reference holds one material per part).  NULL, NULL clears them. */
 int aphip_set_material_fields(aphip_ctx *ctx, const double *E, const double *nu);
+
+/* Kinematic formulation of the Gaussian elements (the reference's part formulation,
+ * include/InternalForceContributionParameters.h lagrangian_formulation_type).
+ * APHIP_UPDATED_LAGRANGIAN: each step integrates over the start-of-step configuration
+ * x_n with the step's displacement increment (ExplicitTimeIntegratorUpdated,
+ * include/ExplicitTimeIntegrator.h:193-226, :466-486), H = dH + H_n + dH H_n
+ * (include/ComputeInternalForceBase.h:158-166) and the force with B F_r and w / det F_r
+ * (include/ComputeInternalForceGaussian.h:109-123).  Call after aphip_set_mesh and
+ * aphip_set_material; the current coordinates start at the mesh coordinates.  Runs on
+ * the element-partials kernels; not with RKPM cells, ensembles, the power method or
+ * the adjoint (errors). */
+int aphip_set_formulation(aphip_ctx *ctx, int32_t formulation);
 
 /* Lumped mass rho V_e / n_nodes per node: replaces ComputeMassMatrixForPart +
  * FinishComputingMassMatrix (src/MassUtils.cpp:57-112).  After aphip_rk_setup the

@@ -192,6 +192,17 @@ int or_internal_force(int etype, int64_t n_nodes, const double *X, int64_t n_ele
                     for (int k = 0; k < npe; ++k) s += ue[3 * k + i] * B[3 * k + j];
                     He[3 * i + j] = s;
                 }
+            if (mat->updated) { /* H = dH + H_n + dH H_n (ComputeInternalForceBase.h:158-166) */
+                const double *Hn = &mat->H_n[9 * e];
+                double Ht[9];
+                for (int i = 0; i < 3; ++i)
+                    for (int j = 0; j < 3; ++j) {
+                        double s = 0.0;
+                        for (int k = 0; k < 3; ++k) s += He[3 * i + k] * Hn[3 * k + j];
+                        Ht[3 * i + j] = He[3 * i + j] + Hn[3 * i + j] + s;
+                    }
+                memcpy(He, Ht, sizeof(Ht));
+            }
             const int64_t sidx = (per_qp ? e * 8 + q : e) * sw;
             if (or_rate_form(mat->kind))  /* the element's H_n, P_n for every Gauss point */
                 or_stress_rate(mat, lam, mu, He, &mat->H_n[9 * e], &mat->P_n[9 * e], mat->dt, &state_n[sidx],
@@ -199,6 +210,20 @@ int or_internal_force(int etype, int64_t n_nodes, const double *X, int64_t n_ele
             else
                 or_stress_mat(mat, lam, mu, He, has_state ? &state_n[sidx] : NULL,
                               has_state ? &state_np1[sidx] : NULL, Pe);
+            if (mat->updated) { /* B F_r and w / det F_r (ComputeInternalForceGaussian.h:109-123) */
+                double Fr[9], Fri[9], dFr, Bp[24];
+                for (int i = 0; i < 9; ++i) Fr[i] = mat->H_n[9 * e + i];
+                Fr[0] += 1.0; Fr[4] += 1.0; Fr[8] += 1.0;
+                or_inverse3_cofactor(Fr, Fri, &dFr);
+                for (int k = 0; k < npe; ++k)
+                    for (int j = 0; j < 3; ++j) {
+                        double s = 0.0;
+                        for (int m = 0; m < 3; ++m) s += B[3 * k + m] * Fr[3 * m + j];
+                        Bp[3 * k + j] = s;
+                    }
+                memcpy(B, Bp, sizeof(double) * 3 * npe);
+                w = w / dFr;
+            }
             for (int k = 0; k < npe; ++k)
                 for (int j = 0; j < 3; ++j) {
                     double s = 0.0;

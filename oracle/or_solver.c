@@ -9,6 +9,9 @@
  *                      acceleration BC; v = v1/2 + dt/2 a)
  *   node functors      include/ExplicitTimeIntegrator.h:116-126, :267-276, :308-318, :348-357
  *   N/NP1 state swap   Explicit.cpp:393 (UpdateFieldStates) for the element state field
+ *   updated Lagrangian ExplicitTimeIntegratorUpdated (include/ExplicitTimeIntegrator.h:193-226,
+ *                      :466-486): increment dt v1/2, d += increment, x_{n+1} = x_n + increment,
+ *                      the force over x_n with the increments and H_n
  *
  * Pinning (tests/test_oracle_solver.py): sum m = rho V (MassUtilsTest.cpp:202-265);
  * free flight (zero stiffness path: rigid translation gives f = 0 and u = v0 t exactly);
@@ -70,13 +73,20 @@ int or_solve(const or_problem *p, double dt, int n_steps, int do_initial, double
     /* rate-form materials: this call's increment; H_n, P_n are the caller's buffers */
     or_problem q = *p;
     q.mat.dt = dt;
-    const int rate = or_rate_form(p->mat.kind);
-    if (rate && (!q.mat.H_n || !q.mat.P_n)) return -1;
+    const int rate = or_rate_form(p->mat.kind), ul = p->mat.updated;
+    if ((rate || ul) && (!q.mat.H_n || !q.mat.P_n)) return -1;
+    if (ul && (!q.x_cur || !q.d_inc)) return -1;
+    double *x_np1 = NULL;
+    if (ul) {
+        x_np1 = (double *)malloc(sizeof(double) * 3 * (size_t)(N > 0 ? N : 1));
+        if (!x_np1) return -1;
+        q.X = q.x_cur; /* the force integrates over the start-of-step configuration */
+    }
     p = &q;
     if (do_initial) {
         /* Explicit.cpp:363-376: force and acceleration at the initial state (written to NP1) */
-        rc = compute_force(p, u, snp1, snp1, H, P, f, nt);
-        if (rc) return rc;
+        rc = compute_force(p, ul ? q.d_inc : u, snp1, snp1, H, P, f, nt);
+        if (rc) { free(x_np1); return rc; }
 #pragma omp parallel for schedule(static) num_threads(nt)
         for (int64_t i = 0; i < N; ++i)
             for (int d = 0; d < 3; ++d) a[3 * i + d] = f[3 * i + d] / p->mass[i];
@@ -84,7 +94,7 @@ int or_solve(const or_problem *p, double dt, int n_steps, int do_initial, double
     for (int s = 0; s < n_steps; ++s) {
         /* UpdateFieldStates: N <- NP1 */
         double *t = sn; sn = snp1; snp1 = t;
-        if (rate) {
+        if (rate || ul) {
             memcpy(q.mat.H_n, H, sizeof(double) * 9 * (size_t)p->n_elem);
             memcpy(q.mat.P_n, P, sizeof(double) * 9 * (size_t)p->n_elem);
         }
@@ -97,13 +107,23 @@ int or_solve(const or_problem *p, double dt, int n_steps, int do_initial, double
         if (p->fixed)
             for (int64_t i = 0; i < 3 * N; ++i)
                 if (p->fixed[i]) v[i] = 0.0;
+        if (ul) {
+            for (int64_t i = 0; i < 3 * N; ++i) {
+                const double inc = dt * v[i];
+                u[i] = u[i] + inc;
+                q.d_inc[i] = inc;
+                x_np1[i] = q.x_cur[i] + inc;
+            }
+        } else {
 #pragma omp parallel for schedule(static) num_threads(nt)
-        for (int64_t i = 0; i < N; ++i) {
-            const double dti = p->node_dt ? p->node_dt[i] : dt;
-            for (int d = 0; d < 3; ++d) u[3 * i + d] = u[3 * i + d] + dti * v[3 * i + d];
+            for (int64_t i = 0; i < N; ++i) {
+                const double dti = p->node_dt ? p->node_dt[i] : dt;
+                for (int d = 0; d < 3; ++d) u[3 * i + d] = u[3 * i + d] + dti * v[3 * i + d];
+            }
         }
-        rc = compute_force(p, u, sn, snp1, H, P, f, nt);
-        if (rc) return rc;
+        rc = compute_force(p, ul ? q.d_inc : u, sn, snp1, H, P, f, nt);
+        if (rc) { free(x_np1); return rc; }
+        if (ul) memcpy(q.x_cur, x_np1, sizeof(double) * 3 * (size_t)N);
 #pragma omp parallel for schedule(static) num_threads(nt)
         for (int64_t i = 0; i < N; ++i)
             for (int d = 0; d < 3; ++d) a[3 * i + d] = f[3 * i + d] / p->mass[i];
@@ -118,6 +138,7 @@ int or_solve(const or_problem *p, double dt, int n_steps, int do_initial, double
         }
     }
     if (state_out_is_b) *state_out_is_b = (snp1 == state_b);
+    free(x_np1);
     return 0;
 }
 

@@ -47,6 +47,10 @@ typedef struct {
      * by the caller, or_solve refreshes them each step) and the increment */
     double *H_n, *P_n;
     double dt;
+    /* updated Lagrangian: the coordinates passed are those at the start of the step, the
+     * displacements the step's increments, H_n the previous displacement gradient */
+    int32_t updated;
+    int32_t pad_;
 } or_material;
 
 /* 1 for materials that read the previous step's H and P (velocity gradient, old stress). */
@@ -116,6 +120,10 @@ typedef struct {
     const int32_t *num_neighbors;  /* [n_nodes] */
     const int32_t *neighbors;      /* [n_nodes*max_neighbors] */
     const double *phi;             /* [n_nodes*max_neighbors] */
+    /* updated Lagrangian (mat.updated = 1): current coordinates at the start of the step and
+     * the step's displacement increments, [n_nodes*3], owned by the caller */
+    double *x_cur;
+    double *d_inc;
 } or_problem;
 
 /* Initial force and acceleration (Explicit.cpp:363-376), then n_steps of the

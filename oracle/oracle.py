@@ -34,7 +34,8 @@ class Material(ctypes.Structure):
                 ("bulk", ctypes.c_double), ("creep_A", ctypes.c_double), ("creep_n", ctypes.c_double),
                 ("creep_m", ctypes.c_double), ("temperature", ctypes.c_double),
                 ("dp_A1", ctypes.c_double), ("dp_A2", ctypes.c_double), ("dp_A2G", ctypes.c_double),
-                ("H_n", _d), ("P_n", _d), ("dt", ctypes.c_double)]
+                ("H_n", _d), ("P_n", _d), ("dt", ctypes.c_double),
+                ("updated", ctypes.c_int32), ("pad_", ctypes.c_int32)]
 
 
 class Problem(ctypes.Structure):
@@ -43,7 +44,7 @@ class Problem(ctypes.Structure):
                 ("mass", _d), ("fixed", _u8), ("node_dt", _d),
                 ("cell_start", _i64), ("cell_nodes", _i32), ("cell_dphi", _d), ("cell_volume", _d),
                 ("rk_two_pass", ctypes.c_int32), ("max_neighbors", ctypes.c_int32), ("num_neighbors", _i32),
-                ("neighbors", _i32), ("phi", _d)]
+                ("neighbors", _i32), ("phi", _d), ("x_cur", _d), ("d_inc", _d)]
 
 
 def build():
@@ -264,13 +265,16 @@ class Solver:
 
     def __init__(self, etype, X, conn, kind, lam, mu, mass, fixed=None, yield_stress=0.0, hardening=0.0,
                  elem_lambda=None, elem_mu=None, node_dt=None, state_per_qp=0, cells=None, two_pass=None,
-                 damage=None, creep=None):
+                 damage=None, creep=None, updated=False):
+        """updated: the updated Lagrangian formulation (Gaussian elements)."""
         """cells: RK smoothed cells (start, nodes, dphi, volume); two_pass: (num, nbr, phi)
         of the RK functions, with `cells` from rk_cells_two_pass, for the two-pass force."""
         self.X = _f64(X)
         self.conn = None if conn is None else _i32a(conn)
         self.mass = _f64(mass)
         self.fixed = None if fixed is None else np.ascontiguousarray(fixed, dtype=np.uint8)
+        if self.fixed is not None and self.fixed.size != self.X.size:
+            raise ValueError("fixed must hold n_nodes x 3 flags")
         self.el = None if elem_lambda is None else _f64(elem_lambda)
         self.em = None if elem_mu is None else _f64(elem_mu)
         self.node_dt = None if node_dt is None else _f64(node_dt)
@@ -299,6 +303,10 @@ class Solver:
         self.Hn, self.Pn = np.zeros((ne_, 9)), np.zeros((ne_, 9))  # N halves of the H, P pairs
         p.mat = _material(kind, lam, mu, yield_stress, hardening, self.el, self.em, state_per_qp, damage, creep,
                           (self.Hn, self.Pn, 0.0))
+        if updated:
+            self.xc, self.dinc = self.X.copy(), np.zeros_like(self.X)
+            p.mat.updated = 1
+            p.x_cur, p.d_inc = _p(self.xc), _p(self.dinc)
         p.mass = _p(self.mass)
         p.fixed = _p(self.fixed, _u8)
         p.node_dt = _p(self.node_dt)

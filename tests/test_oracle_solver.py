@@ -168,3 +168,48 @@ def test_reference_power_method_against_dense_eigenproblem(etype):
     # whose unfilled slots are zero (the reference's result rule)
     dt3, it3, conv3 = s.power_method(l, n_iter=3, tolerance=1e-15, variation_tolerance=0.0)
     assert not conv3 and it3 == 3 and dt3 == 0.0
+
+
+# ---- updated Lagrangian (include/ExplicitTimeIntegrator.h:193-226, ComputeInternalForceGaussian.h:109-123) ----
+
+def _ul_block(etype, n=3, seed=4):
+    X, hc = M.hex_lattice(n, n, n)
+    X = M.jitter(X, 1.0 / n, 0.1, seed, 1)
+    return X, (hc if etype == O.HEX8 else M.split_tets(X, hc))
+
+
+@pytest.mark.parametrize("kind", [O.NEO_HOOKEAN, O.SVK, O.J2])
+def test_updated_equals_total_for_tet4(kind):
+    # Tet4 gradients are constant: F_{n+1} = (I + dH) F_n, B(x_n) F_n = B(X) and
+    # w_n / det F_n = w_X, so the updated formulation is the total one up to rounding;
+    # a slip in the H update or in B F_r, w / det F_r breaks the agreement
+    X, t = _ul_block(O.TET4)
+    E, nu, rho = 1e9, 0.3, 1000.0
+    lam, mu = O.lame(E, nu)
+    mass = O.lumped_mass(O.TET4, X, t, rho)
+    dt = 0.5 * M.stable_dt_estimate(X, t, E, nu, rho)
+    v0 = np.zeros_like(X)
+    v0[:, 0] = 0.2 / (100 * dt) * (X[:, 0] - 0.5)
+    v0[:, 1] = 0.05 / (100 * dt) * (X[:, 2] - 0.5)
+    kw = dict(yield_stress=2e6, hardening=1e8) if kind == O.J2 else {}
+    tl = O.Solver(O.TET4, X, t, kind, lam, mu, mass, **kw)
+    ul = O.Solver(O.TET4, X, t, kind, lam, mu, mass, updated=True, **kw)
+    for s in (tl, ul):
+        s.v[:] = v0
+        s.run(dt, 100)
+    assert np.abs(tl.u).max() > 1e-3
+    for a, b in ((ul.u, tl.u), (ul.v, tl.v), (ul.f, tl.f), (ul.H, tl.H), (ul.P, tl.P)):
+        assert np.abs(a - b).max() < 1e-9 * np.abs(b).max()
+    assert np.allclose(ul.xc, X + ul.u, rtol=0, atol=1e-14)
+
+
+@pytest.mark.parametrize("etype", [O.TET4, O.HEX8])
+def test_updated_rigid_translation_is_force_free(etype):
+    X, conn = _ul_block(etype)
+    lam, mu = O.lame(1e9, 0.3)
+    mass = O.lumped_mass(etype, X, conn, 1000.0)
+    ul = O.Solver(etype, X, conn, O.NEO_HOOKEAN, lam, mu, mass, updated=True)
+    ul.v[:] = np.array([3.0, -1.0, 2.0])
+    ul.run(1e-4, 50)
+    assert np.abs(ul.f).max() < 1e-6 and np.abs(ul.H).max() < 1e-13
+    assert np.allclose(ul.u, np.array([3.0, -1.0, 2.0]) * 50e-4, rtol=1e-12)
