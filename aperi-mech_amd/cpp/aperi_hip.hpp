@@ -258,7 +258,7 @@ class Device {
 enum class ElementTopology { Tetrahedron4, Hexahedron8 };       // include/Constants.h:23-25
 enum class ApproximationSpaceType { FiniteElement, ReproducingKernel };
 enum class IntegrationSchemeType { GaussQuadrature, StrainSmoothing };
-// include/Constants.h:33-36 (Incremental and Semi are not supported here)
+// include/Constants.h:33-36 (Incremental is not supported here)
 enum class LagrangianFormulationType { Total, Incremental, Updated, Semi };
 
 // An element block in the caller's numbering: coordinates n x 3, connectivity
@@ -347,11 +347,14 @@ class ElementGaussianHip : public ElementBase {
    public:
     ElementGaussianHip(ElementTopology topology, const MeshBlock &mesh, std::shared_ptr<Device> device,
                        std::shared_ptr<Material> material,
-                       LagrangianFormulationType formulation = LagrangianFormulationType::Total)
+                       LagrangianFormulationType formulation = LagrangianFormulationType::Total,
+                       int semi_update_interval = 1)
         : ElementBase(topology == ElementTopology::Tetrahedron4 ? 4 : 8, std::move(device), std::move(material)) {
         BindMesh(topology == ElementTopology::Tetrahedron4 ? APHIP_TET4 : APHIP_HEX8, mesh);
         if (formulation == LagrangianFormulationType::Updated)
             Check(aphip_set_formulation(Context(), APHIP_UPDATED_LAGRANGIAN));
+        else if (formulation == LagrangianFormulationType::Semi)
+            Check(aphip_set_formulation_semi(Context(), semi_update_interval));
         else if (formulation != LagrangianFormulationType::Total)
             throw std::runtime_error("Lagrangian formulation not supported by the MI355X force step");
     }

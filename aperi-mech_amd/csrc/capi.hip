@@ -245,7 +245,7 @@ void renumber_nodes(Ctx &c, const std::vector<int32_t> &old_of_new) {
     if (c.has_mass) permute_device(c, c.mass, 1, old_of_new);
     if (c.has_fixed) permute_device(c, c.fixed, 1, old_of_new);
     if (c.pm_eig_valid) permute_device(c, c.pm_eig, 3, old_of_new);
-    if (c.formulation == APHIP_UPDATED_LAGRANGIAN) {
+    if (c.formulation != APHIP_TOTAL_LAGRANGIAN) {
         permute_device(c, c.xc[0], 3, old_of_new);
         permute_device(c, c.xc[1], 3, old_of_new);
         permute_device(c, c.dinc, 3, old_of_new);
@@ -290,6 +290,8 @@ static void setup_mesh(Ctx &c, int32_t elem_type, int64_t n_nodes, const double 
     c.xc[0].release();
     c.xc[1].release();
     c.dinc.release();
+    c.Href.release();
+    c.steps_done = 0;
     c.pm_eig_valid = false;
     c.tiles.reset();
     c.graph_reset();
@@ -655,35 +657,56 @@ int aphip_set_material(aphip_ctx *c, int32_t kind, const aphip_material_params *
     APHIP_API_END
 }
 
+static void set_formulation(Ctx &c, int formulation, int64_t interval) {
+    APHIP_REQUIRE(c.etype >= 0 && c.mat.kind >= 0, "mesh and material must be set");
+    APHIP_REQUIRE(formulation == APHIP_TOTAL_LAGRANGIAN || formulation == APHIP_UPDATED_LAGRANGIAN ||
+                      formulation == APHIP_SEMI_LAGRANGIAN,
+                  "formulation must be total, updated or semi-Lagrangian");
+    APHIP_REQUIRE(formulation == APHIP_TOTAL_LAGRANGIAN || !c.cells.ready,
+                  "the updated formulation applies to the Gaussian elements, not to RKPM cells");
+    APHIP_REQUIRE(formulation == APHIP_TOTAL_LAGRANGIAN || c.n_specimens == 0,
+                  "ensembles use the total Lagrangian formulation");
+    APHIP_REQUIRE(formulation != APHIP_SEMI_LAGRANGIAN || !c.mat.rate_form(),
+                  "the semi-Lagrangian formulation does not take the rate-form materials");
+    APHIP_REQUIRE(interval >= 1, "the reference update interval must be at least 1");
+    APHIP_HIP(hipSetDevice(c.device));
+    APHIP_HIP(hipStreamSynchronize(c.stream));
+    c.graph_reset();
+    c.formulation = formulation;
+    c.semi_interval = interval;
+    c.steps_done = 0;
+    c.Href.release();
+    if (formulation != APHIP_TOTAL_LAGRANGIAN) {
+        for (int s = 0; s < 2; ++s) {
+            c.xc[s].alloc(3 * c.nn);
+            APHIP_HIP(hipMemcpyAsync(c.xc[s].p, c.X.p, sizeof(double) * 3 * c.nn, hipMemcpyDeviceToDevice, c.stream));
+        }
+        c.dinc.alloc(3 * c.nn);
+        c.dinc.zero(c.stream);
+        if (formulation == APHIP_SEMI_LAGRANGIAN) {
+            c.Href.alloc(9 * std::max<int64_t>(c.ne, 1));
+            c.Href.zero(c.stream);
+        }
+    } else {
+        c.xc[0].release();
+        c.xc[1].release();
+        c.dinc.release();
+    }
+    alloc_state(c);
+    APHIP_HIP(hipStreamSynchronize(c.stream));
+}
+
 int aphip_set_formulation(aphip_ctx *c, int32_t formulation) {
     APHIP_API_BEGIN
     APHIP_REQUIRE(c != nullptr, "ctx is NULL");
-    APHIP_REQUIRE(c->etype >= 0 && c->mat.kind >= 0, "mesh and material must be set");
-    APHIP_REQUIRE(formulation == APHIP_TOTAL_LAGRANGIAN || formulation == APHIP_UPDATED_LAGRANGIAN,
-                  "formulation must be total or updated Lagrangian");
-    APHIP_REQUIRE(formulation == APHIP_TOTAL_LAGRANGIAN || !c->cells.ready,
-                  "the updated formulation applies to the Gaussian elements, not to RKPM cells");
-    APHIP_REQUIRE(formulation == APHIP_TOTAL_LAGRANGIAN || c->n_specimens == 0,
-                  "ensembles use the total Lagrangian formulation");
-    APHIP_HIP(hipSetDevice(c->device));
-    APHIP_HIP(hipStreamSynchronize(c->stream));
-    c->graph_reset();
-    c->formulation = formulation;
-    if (formulation == APHIP_UPDATED_LAGRANGIAN) {
-        for (int s = 0; s < 2; ++s) {
-            c->xc[s].alloc(3 * c->nn);
-            APHIP_HIP(hipMemcpyAsync(c->xc[s].p, c->X.p, sizeof(double) * 3 * c->nn, hipMemcpyDeviceToDevice,
-                                     c->stream));
-        }
-        c->dinc.alloc(3 * c->nn);
-        c->dinc.zero(c->stream);
-    } else {
-        c->xc[0].release();
-        c->xc[1].release();
-        c->dinc.release();
-    }
-    alloc_state(*c);
-    APHIP_HIP(hipStreamSynchronize(c->stream));
+    set_formulation(*c, formulation, 1);
+    APHIP_API_END
+}
+
+int aphip_set_formulation_semi(aphip_ctx *c, int32_t update_interval) {
+    APHIP_API_BEGIN
+    APHIP_REQUIRE(c != nullptr, "ctx is NULL");
+    set_formulation(*c, APHIP_SEMI_LAGRANGIAN, update_interval);
     APHIP_API_END
 }
 
@@ -895,6 +918,7 @@ int aphip_initialize(aphip_ctx *c) {
     APHIP_API_BEGIN
     APHIP_REQUIRE(c != nullptr, "ctx is NULL");
     require_ready(*c, true);
+    c->steps_done = 0;
     APHIP_HIP(hipSetDevice(c->device));
     if (use_tiles(*c)) {
         tile_run(*c, 0.0, 1);
@@ -914,7 +938,7 @@ int aphip_step(aphip_ctx *c, double dt, int32_t n_steps) {
     APHIP_REQUIRE(std::isfinite(dt) && dt >= 0.0, "time increment must be finite and non-negative");
     APHIP_HIP(hipSetDevice(c->device));
     int32_t done = 0;
-    if (c->graph_enabled && !c->prof.enabled && n_steps >= 2) {
+    if (c->graph_enabled && !c->prof.enabled && n_steps >= 2 && c->formulation != APHIP_SEMI_LAGRANGIAN) {
         // two steps per graph: the N/NP1 state buffers swap twice and return to their roles
         if (!c->graph_exec || c->graph_dt != dt || c->graph_parity != c->state_np1) {
             c->graph_reset();
@@ -943,6 +967,8 @@ int aphip_step(aphip_ctx *c, double dt, int32_t n_steps) {
         prof_begin(*c, 0);
         step_once(*c, dt);
         prof_end(*c, 0);
+        // the reference's StepScheduler(interval, interval) after step n (Explicit.cpp:195-212)
+        if (c->formulation == APHIP_SEMI_LAGRANGIAN && ++c->steps_done % c->semi_interval == 0) fem_semi_update(*c);
     }
     APHIP_API_END
 }

@@ -151,10 +151,12 @@ struct NodeArgs {
     double *u;
     const double *node_dt;
     double dt;
-    // updated Lagrangian: the step's increments and the current coordinates at n, n+1
+    // updated Lagrangian: the step's increments and the current coordinates at n, n+1;
+    // semi-Lagrangian (semi = 1): the increments accumulate, no coordinates are moved
     double *dinc;
     const double *xn;
     double *xnp1;
+    int semi;
 };
 
 // v1/2 = v + dt/2 a; velocity BC; u += dt v1/2  (ExplicitTimeIntegrator.h:116-126, :348-357)
@@ -174,25 +176,44 @@ __global__ __launch_bounds__(kNodeBlock) void k_predict(NodeArgs A) {
 }
 
 // Updated Lagrangian predictor (ExplicitTimeIntegrator.h:193-226): v1/2 as above, the
-// increment dt v1/2, u += increment, x_{n+1} = x_n + increment.
+// increment dt v1/2, u += increment, x_{n+1} = x_n + increment; semi-Lagrangian: the
+// increment accumulates since the last reference update and no coordinates move.
 __global__ __launch_bounds__(kNodeBlock) void k_predict_ul(NodeArgs A) {
 #pragma clang fp contract(off)
     const int64_t i = (int64_t)blockIdx.x * kNodeBlock + threadIdx.x;
     if (i >= A.nn) return;
-    double u[3], v[3], a[3], vh[3], u1[3], x[3];
+    double u[3], v[3], a[3], vh[3], u1[3], x[3], di[3];
     load3(A.u, i, u);
     load3(A.v, i, v);
     load3(A.a, i, a);
     load3(A.xn, i, x);
+    load3(A.dinc, i, di);
     cd_predict(u, v, a, A.fixed ? A.fixed[i] : 0u, A.dt, vh, u1);
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
         const double inc = A.dt * vh[d];
         A.v[3 * i + d] = vh[d];
         A.u[3 * i + d] = u[d] + inc;
-        A.dinc[3 * i + d] = inc;
-        A.xnp1[3 * i + d] = x[d] + inc;
+        if (A.semi) {
+            A.dinc[3 * i + d] = di[d] + inc;
+        } else {
+            A.dinc[3 * i + d] = inc;
+            A.xnp1[3 * i + d] = x[d] + inc;
+        }
     }
+}
+
+// Semi-Lagrangian reference update (ExplicitTimeIntegratorSemi::UpdateReferenceConfiguration,
+// ExplicitTimeIntegrator.h:508-526): x_ref += increment, increment = 0 (nodes), and
+// H_ref = H (elements; i indexes 9 x ne doubles).
+__global__ __launch_bounds__(kNodeBlock) void k_semi_update(int64_t nn3, double *xref, double *dinc, int64_t nh,
+                                                            double *href, const double *H) {
+    const int64_t i = (int64_t)blockIdx.x * kNodeBlock + threadIdx.x;
+    if (i < nn3) {
+        xref[i] = xref[i] + dinc[i];
+        dinc[i] = 0.0;
+    }
+    if (i < nh) href[i] = H[i];
 }
 
 // f = sum of the node's element partials in ascending element order, then per mode:
@@ -308,14 +329,14 @@ void launch_elem_kind_ul(Ctx &c, const ElemArgs &A) {
 
 template <int KIND>
 void launch_elem_kind(Ctx &c, const ElemArgs &A) {
-    if (c.formulation == APHIP_UPDATED_LAGRANGIAN) launch_elem_kind_ul<KIND, true>(c, A);
+    if (c.formulation != APHIP_TOTAL_LAGRANGIAN) launch_elem_kind_ul<KIND, true>(c, A);
     else launch_elem_kind_ul<KIND, false>(c, A);
 }
 
 }  // namespace
 
 void fem_element_force(Ctx &c) {
-    const bool ul = c.formulation == APHIP_UPDATED_LAGRANGIAN;
+    const bool ul = c.formulation != APHIP_TOTAL_LAGRANGIAN;
     ElemArgs A;
     A.X = ul ? c.xcn().p : c.X.p;
     A.u = ul ? c.dinc.p : c.u.p;
@@ -324,7 +345,7 @@ void fem_element_force(Ctx &c) {
     A.nblocks = padded_blocks(c.ne, c.etype == APHIP_TET4 ? kElemBlock : 128);
     A.H = c.Hnp1().p;
     A.P = c.Pnp1().p;
-    A.Hn = c.h_pair() ? c.Hn().p : nullptr;
+    A.Hn = c.formulation == APHIP_SEMI_LAGRANGIAN ? c.Href.p : c.h_pair() ? c.Hn().p : nullptr;
     A.Pn = c.mat.rate_form() ? c.Pn().p : nullptr;
     A.rate_dt = c.rate_dt;
     A.state_n = c.state[1 - c.state_np1].p;
@@ -363,8 +384,9 @@ static NodeArgs node_args(Ctx &c, double dt) {
     A.node_dt = c.node_dt.p;
     A.dt = dt;
     A.dinc = c.dinc.p;
-    A.xn = c.formulation == APHIP_UPDATED_LAGRANGIAN ? c.xcn().p : nullptr;
+    A.xn = c.formulation != APHIP_TOTAL_LAGRANGIAN ? c.xcn().p : nullptr;
     A.xnp1 = c.formulation == APHIP_UPDATED_LAGRANGIAN ? c.xcnp1().p : nullptr;
+    A.semi = c.formulation == APHIP_SEMI_LAGRANGIAN ? 1 : 0;
     return A;
 }
 
@@ -372,8 +394,16 @@ void fem_predict(Ctx &c, double dt) {
     if (c.nn == 0) return;
     NodeArgs A = node_args(c, dt);
     const dim3 grid((unsigned)((c.nn + kNodeBlock - 1) / kNodeBlock));
-    if (c.formulation == APHIP_UPDATED_LAGRANGIAN) hipLaunchKernelGGL(k_predict_ul, grid, dim3(kNodeBlock), 0, c.stream, A);
+    if (c.formulation != APHIP_TOTAL_LAGRANGIAN) hipLaunchKernelGGL(k_predict_ul, grid, dim3(kNodeBlock), 0, c.stream, A);
     else hipLaunchKernelGGL(k_predict, grid, dim3(kNodeBlock), 0, c.stream, A);
+    APHIP_HIP(hipGetLastError());
+}
+
+void fem_semi_update(Ctx &c) {
+    const int64_t n3 = 3 * c.nn, nh = 9 * c.ne, n = std::max(n3, nh);
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_semi_update, dim3((unsigned)((n + kNodeBlock - 1) / kNodeBlock)), dim3(kNodeBlock), 0,
+                       c.stream, n3, c.xc[0].p, c.dinc.p, nh, c.Href.p, c.Hnp1().p);
     APHIP_HIP(hipGetLastError());
 }
 

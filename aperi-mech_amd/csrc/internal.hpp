@@ -242,9 +242,12 @@ struct Ctx {
     int state_np1 = 1;                   // index of the NP1 state buffer
     double rate_dt = 0.0;                // increment for force-only evaluations of rate-form materials
     int formulation = APHIP_TOTAL_LAGRANGIAN;
-    DevBuf<double> xc[2];                // updated Lagrangian: current coordinates (N / NP1 with the state)
-    DevBuf<double> dinc;                 // updated Lagrangian: the step's displacement increments
-    DevBuf<double> &xcn() { return xc[1 - state_np1]; }
+    DevBuf<double> xc[2];                // updated Lagrangian: current coordinates (N / NP1 with the state);
+                                         // semi-Lagrangian: xc[0] is the reference configuration
+    DevBuf<double> dinc;                 // the step's displacement increments (semi: accumulated)
+    DevBuf<double> Href;                 // semi-Lagrangian: reference displacement gradient (SoA 9 x ne)
+    int64_t semi_interval = 1, steps_done = 0;
+    DevBuf<double> &xcn() { return formulation == APHIP_SEMI_LAGRANGIAN ? xc[0] : xc[1 - state_np1]; }
     DevBuf<double> &xcnp1() { return xc[state_np1]; }
     // H of the previous step is read by the rate-form materials and the updated formulation
     bool h_pair() const { return mat.rate_form() || formulation == APHIP_UPDATED_LAGRANGIAN; }
@@ -301,6 +304,8 @@ void set_last_error(const std::string &msg);
 // ---- kernels (fem.hip) ----
 void fem_element_force(Ctx &c);
 void fem_predict(Ctx &c, double dt);
+// semi-Lagrangian reference update: x_ref += increment, increment = 0, H_ref = H
+void fem_semi_update(Ctx &c);
 // mode 0: f only; 1: f and a = f/m (initial); 2: f, a, BC, v += dt/2 a (step corrector)
 void fem_node_update(Ctx &c, double dt, int mode);
 void fem_volumes(Ctx &c);

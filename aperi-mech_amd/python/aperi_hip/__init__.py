@@ -26,7 +26,7 @@ DAMAGE_DEFAULTS = dict(maximum_damage=0.99, maximum_damage_increment=1.0, stop_f
                        fail_all_at_time=-1.0)
 FIELD_U, FIELD_V, FIELD_A, FIELD_F, FIELD_MASS, FIELD_H, FIELD_P, FIELD_STATE, FIELD_VOLUME = range(9)
 FIELD_RK_DISPLACEMENT, FIELD_RK_FORCE_LOCAL = 9, 10
-TOTAL_LAGRANGIAN, UPDATED_LAGRANGIAN = 0, 2
+TOTAL_LAGRANGIAN, UPDATED_LAGRANGIAN, SEMI_LAGRANGIAN = 0, 2, 3
 MAX_NODE_NUM_NEIGHBORS = 40
 FLAG_TWO_PASS = 1
 FLAG_PIPELINED = 2
@@ -38,7 +38,7 @@ _u8 = ctypes.POINTER(ctypes.c_uint8)
 
 EXPORTED = [
     "aphip_create", "aphip_destroy", "aphip_last_error", "aphip_device_count", "aphip_set_mesh", "aphip_get_node_permutation",
-    "aphip_set_material", "aphip_set_material_fields", "aphip_set_formulation", "aphip_compute_mass", "aphip_set_fixed",
+    "aphip_set_material", "aphip_set_material_fields", "aphip_set_formulation", "aphip_set_formulation_semi", "aphip_compute_mass", "aphip_set_fixed",
     "aphip_set_field", "aphip_get_field", "aphip_internal_force", "aphip_initialize", "aphip_step",
     "aphip_synchronize", "aphip_set_tiling", "aphip_set_graph", "aphip_autotune", "aphip_batch_begin",
     "aphip_batch_add", "aphip_batch_end", "aphip_batch_ranges", "aphip_internal_force_vjp",
@@ -191,6 +191,11 @@ class Solver:
     def set_formulation(self, formulation):
         """TOTAL_LAGRANGIAN or UPDATED_LAGRANGIAN (after set_mesh and set_material)."""
         _check(self.L.aphip_set_formulation(self.ctx, int(formulation)))
+        return self
+
+    def set_semi_lagrangian(self, update_interval):
+        """Semi-Lagrangian with a reference-configuration update every update_interval steps."""
+        _check(self.L.aphip_set_formulation_semi(self.ctx, int(update_interval)))
         return self
 
     def set_material_fields(self, E=None, nu=None):

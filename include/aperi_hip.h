@@ -77,8 +77,8 @@ enum {
 };
 
 /* Kinematic formulation: the values of LagrangianFormulationType (include/Constants.h:33-36;
- * Incremental = 1 and Semi = 3 are not supported). */
-enum { APHIP_TOTAL_LAGRANGIAN = 0, APHIP_UPDATED_LAGRANGIAN = 2 };
+ * Incremental = 1 is not supported). */
+enum { APHIP_TOTAL_LAGRANGIAN = 0, APHIP_UPDATED_LAGRANGIAN = 2, APHIP_SEMI_LAGRANGIAN = 3 };
 
 /* aphip_config.flags */
 #define APHIP_FLAG_TWO_PASS 1 /* element partials through HBM + node CSR gather (no LDS tiles) */
@@ -167,6 +167,17 @@ int aphip_set_material_fields(aphip_ctx *ctx, const double *E, const double *nu)
  * the element-partials kernels; not with RKPM cells, ensembles, the power method or
  * the adjoint (errors). */
 int aphip_set_formulation(aphip_ctx *ctx, int32_t formulation);
+/* APHIP_SEMI_LAGRANGIAN with its reference-configuration update interval (the reference's
+ * semi-Lagrangian configuration update interval, src/Application.cpp:301-307, 394-398):
+ * the force integrates over the reference configuration with the displacement increment
+ * accumulated since its last update and the reference displacement gradient
+ * (ExplicitTimeIntegratorSemi, include/ExplicitTimeIntegrator.h:488-546); after the
+ * steps n = interval, 2 interval, ... (counted from aphip_initialize) the reference
+ * configuration becomes the current one, the increment restarts at zero and the reference
+ * gradient takes H.  Graph replay is not used; rate-form materials are refused (their
+ * velocity gradient needs the previous step's H beside the reference one).
+ * aphip_set_formulation(APHIP_SEMI_LAGRANGIAN) uses interval 1. */
+int aphip_set_formulation_semi(aphip_ctx *ctx, int32_t update_interval);
 
 /* Lumped mass rho V_e / n_nodes per node: replaces ComputeMassMatrixForPart +
  * FinishComputingMassMatrix (src/MassUtils.cpp:57-112).  After aphip_rk_setup the

@@ -11,7 +11,11 @@
  *   N/NP1 state swap   Explicit.cpp:393 (UpdateFieldStates) for the element state field
  *   updated Lagrangian ExplicitTimeIntegratorUpdated (include/ExplicitTimeIntegrator.h:193-226,
  *                      :466-486): increment dt v1/2, d += increment, x_{n+1} = x_n + increment,
- *                      the force over x_n with the increments and H_n
+ *                      the force over x_n with the increments and H_n; semi-Lagrangian
+ *                      ExplicitTimeIntegratorSemi (:488-546): increments accumulate, the force
+ *                      over the reference configuration with the reference gradient, which
+ *                      StepScheduler(interval, interval) refreshes after steps n = k interval
+ *                      (src/Application.cpp:301-307, src/Solvers/Explicit.cpp:195-212)
  *
  * Pinning (tests/test_oracle_solver.py): sum m = rho V (MassUtilsTest.cpp:202-265);
  * free flight (zero stiffness path: rigid translation gives f = 0 and u = v0 t exactly);
@@ -73,9 +77,11 @@ int or_solve(const or_problem *p, double dt, int n_steps, int do_initial, double
     /* rate-form materials: this call's increment; H_n, P_n are the caller's buffers */
     or_problem q = *p;
     q.mat.dt = dt;
-    const int rate = or_rate_form(p->mat.kind), ul = p->mat.updated;
+    const int rate = or_rate_form(p->mat.kind), ul = p->mat.updated, semi = ul == 2;
     if ((rate || ul) && (!q.mat.H_n || !q.mat.P_n)) return -1;
     if (ul && (!q.x_cur || !q.d_inc)) return -1;
+    if (semi && (rate || !q.semi_steps || q.semi_interval < 1)) return -1;
+    if (semi && do_initial) *q.semi_steps = 0;
     double *x_np1 = NULL;
     if (ul) {
         x_np1 = (double *)malloc(sizeof(double) * 3 * (size_t)(N > 0 ? N : 1));
@@ -94,7 +100,7 @@ int or_solve(const or_problem *p, double dt, int n_steps, int do_initial, double
     for (int s = 0; s < n_steps; ++s) {
         /* UpdateFieldStates: N <- NP1 */
         double *t = sn; sn = snp1; snp1 = t;
-        if (rate || ul) {
+        if (rate || (ul && !semi)) {
             memcpy(q.mat.H_n, H, sizeof(double) * 9 * (size_t)p->n_elem);
             memcpy(q.mat.P_n, P, sizeof(double) * 9 * (size_t)p->n_elem);
         }
@@ -111,8 +117,12 @@ int or_solve(const or_problem *p, double dt, int n_steps, int do_initial, double
             for (int64_t i = 0; i < 3 * N; ++i) {
                 const double inc = dt * v[i];
                 u[i] = u[i] + inc;
-                q.d_inc[i] = inc;
-                x_np1[i] = q.x_cur[i] + inc;
+                if (semi) {
+                    q.d_inc[i] = q.d_inc[i] + inc;
+                } else {
+                    q.d_inc[i] = inc;
+                    x_np1[i] = q.x_cur[i] + inc;
+                }
             }
         } else {
 #pragma omp parallel for schedule(static) num_threads(nt)
@@ -123,7 +133,7 @@ int or_solve(const or_problem *p, double dt, int n_steps, int do_initial, double
         }
         rc = compute_force(p, ul ? q.d_inc : u, sn, snp1, H, P, f, nt);
         if (rc) { free(x_np1); return rc; }
-        if (ul) memcpy(q.x_cur, x_np1, sizeof(double) * 3 * (size_t)N);
+        if (ul && !semi) memcpy(q.x_cur, x_np1, sizeof(double) * 3 * (size_t)N);
 #pragma omp parallel for schedule(static) num_threads(nt)
         for (int64_t i = 0; i < N; ++i)
             for (int d = 0; d < 3; ++d) a[3 * i + d] = f[3 * i + d] / p->mass[i];
@@ -135,6 +145,13 @@ int or_solve(const or_problem *p, double dt, int n_steps, int do_initial, double
             const double dti = p->node_dt ? p->node_dt[i] : dt;
             const double h = 0.5 * dti;
             for (int d = 0; d < 3; ++d) v[3 * i + d] += h * a[3 * i + d];
+        }
+        if (semi && ++*q.semi_steps % q.semi_interval == 0) { /* UpdateReferenceConfiguration */
+            for (int64_t i = 0; i < 3 * N; ++i) {
+                q.x_cur[i] = q.x_cur[i] + q.d_inc[i];
+                q.d_inc[i] = 0.0;
+            }
+            memcpy(q.mat.H_n, H, sizeof(double) * 9 * (size_t)p->n_elem);
         }
     }
     if (state_out_is_b) *state_out_is_b = (snp1 == state_b);

@@ -47,8 +47,10 @@ typedef struct {
      * by the caller, or_solve refreshes them each step) and the increment */
     double *H_n, *P_n;
     double dt;
-    /* updated Lagrangian: the coordinates passed are those at the start of the step, the
-     * displacements the step's increments, H_n the previous displacement gradient */
+    /* updated Lagrangian (1; semi-Lagrangian 2): the coordinates passed are those at the
+     * start of the step (semi: the reference configuration), the displacements the step's
+     * increments (semi: accumulated since the reference update), H_n the previous (semi:
+     * the reference) displacement gradient */
     int32_t updated;
     int32_t pad_;
 } or_material;
@@ -124,6 +126,9 @@ typedef struct {
      * the step's displacement increments, [n_nodes*3], owned by the caller */
     double *x_cur;
     double *d_inc;
+    /* semi-Lagrangian: reference update interval and the caller's step counter */
+    int64_t semi_interval;
+    int64_t *semi_steps;
 } or_problem;
 
 /* Initial force and acceleration (Explicit.cpp:363-376), then n_steps of the

@@ -44,7 +44,8 @@ class Problem(ctypes.Structure):
                 ("mass", _d), ("fixed", _u8), ("node_dt", _d),
                 ("cell_start", _i64), ("cell_nodes", _i32), ("cell_dphi", _d), ("cell_volume", _d),
                 ("rk_two_pass", ctypes.c_int32), ("max_neighbors", ctypes.c_int32), ("num_neighbors", _i32),
-                ("neighbors", _i32), ("phi", _d), ("x_cur", _d), ("d_inc", _d)]
+                ("neighbors", _i32), ("phi", _d), ("x_cur", _d), ("d_inc", _d),
+                ("semi_interval", ctypes.c_int64), ("semi_steps", _i64)]
 
 
 def build():
@@ -265,8 +266,9 @@ class Solver:
 
     def __init__(self, etype, X, conn, kind, lam, mu, mass, fixed=None, yield_stress=0.0, hardening=0.0,
                  elem_lambda=None, elem_mu=None, node_dt=None, state_per_qp=0, cells=None, two_pass=None,
-                 damage=None, creep=None, updated=False):
-        """updated: the updated Lagrangian formulation (Gaussian elements)."""
+                 damage=None, creep=None, updated=False, semi_interval=0):
+        """updated: the updated Lagrangian formulation (Gaussian elements); semi_interval > 0:
+        the semi-Lagrangian one with that reference update interval."""
         """cells: RK smoothed cells (start, nodes, dphi, volume); two_pass: (num, nbr, phi)
         of the RK functions, with `cells` from rk_cells_two_pass, for the two-pass force."""
         self.X = _f64(X)
@@ -303,10 +305,12 @@ class Solver:
         self.Hn, self.Pn = np.zeros((ne_, 9)), np.zeros((ne_, 9))  # N halves of the H, P pairs
         p.mat = _material(kind, lam, mu, yield_stress, hardening, self.el, self.em, state_per_qp, damage, creep,
                           (self.Hn, self.Pn, 0.0))
-        if updated:
+        if updated or semi_interval:
             self.xc, self.dinc = self.X.copy(), np.zeros_like(self.X)
-            p.mat.updated = 1
+            p.mat.updated = 2 if semi_interval else 1
             p.x_cur, p.d_inc = _p(self.xc), _p(self.dinc)
+            self.semi_steps = np.zeros(1, dtype=np.int64)
+            p.semi_interval, p.semi_steps = int(semi_interval), _p(self.semi_steps, _i64)
         p.mass = _p(self.mass)
         p.fixed = _p(self.fixed, _u8)
         p.node_dt = _p(self.node_dt)

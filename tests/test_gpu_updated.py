@@ -103,3 +103,68 @@ def test_updated_rejections(gpu_available):
             s.power_method_time_increment()
         with pytest.raises(A.AphipError):
             s.set_formulation(1)  # incremental: not supported
+
+
+# ---- semi-Lagrangian (ExplicitTimeIntegratorSemi, include/ExplicitTimeIntegrator.h:488-546) ----
+
+@pytest.mark.parametrize("etype", [A.TET4, A.HEX8], ids=["tet4", "hex8"])
+@pytest.mark.parametrize("mat", ["nh", "j2"])
+def test_semi_lagrangian_100_steps(gpu_available, etype, mat):
+    kind, kw = KINDS[mat]
+    X, hc, conn = _block(etype)
+    lam, mu = O.lame(E, NU)
+    mass = O.lumped_mass(etype, X, conn, RHO)
+    dt = 0.5 * M.stable_dt_estimate(X, M.split_tets(X, hc), E, NU, RHO)
+    v0 = _v0(X, dt)
+    okind = {A.NEO_HOOKEAN: O.NEO_HOOKEAN, A.J2: O.J2}[kind]
+    ref = O.Solver(etype, X, conn, okind, lam, mu, mass, yield_stress=kw.get("yield_stress", 0.0),
+                   hardening=kw.get("hardening", 0.0), semi_interval=7)
+    ref.v[:] = v0
+    ref.run(dt, 100)
+    print(f"semi 100 steps etype={etype} mat={mat}", flush=True)
+    with A.Solver() as s:
+        s.set_mesh(etype, X, conn)
+        s.set_material(kind, RHO, E, NU, kw.get("yield_stress", 0.0), kw.get("hardening", 0.0))
+        s.set_semi_lagrangian(7)
+        s.compute_mass()
+        s.set_field(A.FIELD_V, v0)
+        s.initialize()
+        s.step(dt, 30)
+        s.step(dt, 70)
+        fields = [(A.FIELD_U, ref.u), (A.FIELD_V, ref.v), (A.FIELD_A, ref.a), (A.FIELD_F, ref.f),
+                  (A.FIELD_H, ref.H), (A.FIELD_P, ref.P)]
+        if kind == A.J2:
+            fields.append((A.FIELD_STATE, ref.state))
+        for fld, rr in fields:
+            assert rel_err(s.get(fld), rr) < 1e-10, fld
+
+
+def test_semi_interval_one_is_updated(gpu_available):
+    X, hc, conn = _block(A.HEX8)
+    dt = 0.5 * M.stable_dt_estimate(X, M.split_tets(X, hc), E, NU, RHO)
+    v0 = _v0(X, dt)
+    out = []
+    for semi in (False, True):
+        with A.Solver() as s:
+            s.set_mesh(A.HEX8, X, conn)
+            s.set_material(A.NEO_HOOKEAN, RHO, E, NU)
+            if semi:
+                s.set_semi_lagrangian(1)
+            else:
+                s.set_formulation(A.UPDATED_LAGRANGIAN)
+            s.compute_mass()
+            s.set_field(A.FIELD_V, v0)
+            s.initialize()
+            s.step(dt, 50)
+            out.append([s.get(f) for f in (A.FIELD_U, A.FIELD_F, A.FIELD_H, A.FIELD_P)])
+    for a, b in zip(*out):
+        assert np.array_equal(a, b)
+
+
+def test_semi_refuses_rate_form(gpu_available):
+    X, hc, conn = _block(A.TET4, 3)
+    with A.Solver() as s:
+        s.set_mesh(A.TET4, X, conn)
+        s.set_material(A.POWER_LAW_CREEP, RHO, E, NU, creep=CREEP)
+        with pytest.raises(A.AphipError, match="rate-form"):
+            s.set_semi_lagrangian(5)

@@ -213,3 +213,54 @@ def test_updated_rigid_translation_is_force_free(etype):
     ul.run(1e-4, 50)
     assert np.abs(ul.f).max() < 1e-6 and np.abs(ul.H).max() < 1e-13
     assert np.allclose(ul.u, np.array([3.0, -1.0, 2.0]) * 50e-4, rtol=1e-12)
+
+
+# ---- semi-Lagrangian (ExplicitTimeIntegratorSemi, include/ExplicitTimeIntegrator.h:488-546) ----
+
+@pytest.mark.parametrize("interval", [1, 7])
+def test_semi_lagrangian_tet4(interval):
+    # interval 1 is the updated formulation; for Tet4 every interval gives the total one
+    X, t = _ul_block(O.TET4)
+    E, nu, rho = 1e9, 0.3, 1000.0
+    lam, mu = O.lame(E, nu)
+    mass = O.lumped_mass(O.TET4, X, t, rho)
+    dt = 0.5 * M.stable_dt_estimate(X, t, E, nu, rho)
+    v0 = np.zeros_like(X)
+    v0[:, 0] = 0.2 / (100 * dt) * (X[:, 0] - 0.5)
+    tl = O.Solver(O.TET4, X, t, O.NEO_HOOKEAN, lam, mu, mass)
+    se = O.Solver(O.TET4, X, t, O.NEO_HOOKEAN, lam, mu, mass, semi_interval=interval)
+    ul = O.Solver(O.TET4, X, t, O.NEO_HOOKEAN, lam, mu, mass, updated=True)
+    for s in (tl, se, ul):
+        s.v[:] = v0
+        s.run(dt, 60)
+        s.run(dt, 40)
+    assert se.semi_steps[0] == 100
+    for a, b in ((se.u, tl.u), (se.f, tl.f), (se.H, tl.H), (se.P, tl.P)):
+        assert np.abs(a - b).max() < 1e-9 * np.abs(b).max()
+    if interval == 1:
+        for a, b in ((se.u, ul.u), (se.f, ul.f), (se.H, ul.H)):
+            assert np.abs(a - b).max() < 1e-13 * np.abs(b).max()
+    # the reference configuration lags the current one by the accumulated increment
+    assert np.allclose(se.xc + se.dinc, X + se.u, rtol=0, atol=1e-14)
+
+
+def test_semi_lagrangian_hex8_limits():
+    # Hex8 keeps one reference gradient per element (its last Gauss point), exact only for
+    # affine elements, so the formulations differ on a distorted block; the two limits are
+    # exact: interval 1 is the updated formulation, no update in the run is the total one
+    X, hc = _ul_block(O.HEX8)
+    lam, mu = O.lame(1e9, 0.3)
+    mass = O.lumped_mass(O.HEX8, X, hc, 1000.0)
+    dt = 0.5 * M.stable_dt_estimate(X, M.split_tets(X, hc), 1e9, 0.3, 1000.0)
+    v0 = np.zeros_like(X)
+    v0[:, 0] = 0.2 / (100 * dt) * (X[:, 0] - 0.5)
+
+    def run(**kw):
+        s = O.Solver(O.HEX8, X, hc, O.NEO_HOOKEAN, lam, mu, mass, **kw)
+        s.v[:] = v0
+        s.run(dt, 100)
+        return s
+
+    tl, ul, s1, s_none = run(), run(updated=True), run(semi_interval=1), run(semi_interval=1000)
+    for a, b in ((s1.u, ul.u), (s1.H, ul.H), (s1.f, ul.f), (s_none.u, tl.u), (s_none.H, tl.H), (s_none.P, tl.P)):
+        assert np.array_equal(a, b)
