@@ -131,7 +131,11 @@ struct CellData {
     DevBuf<int64_t> node_start; // [nn + 1] node -> entries naming it
     DevBuf<int32_t> node_inc;   // unused (kept for layout stability)
     DevBuf<int32_t> entry_pos;  // [nnz] position of each entry in node order (by node, then ascending entry)
-    DevBuf<double> g;           // [nnz * 3] per-entry force contributions of the last evaluation, node order
+    DevBuf<double> g;           // [nnz * 3] two-pass: per-entry force contributions of the last evaluation, node order
+    // one-pass: the node kernel recomputes each entry's force from its cell's P and volume
+    DevBuf<int32_t> ent_cell;   // [nnz] cell of the k-th entry in node order
+    DevBuf<double> dphi_node;   // [nnz * 3] grad phi of the k-th entry in node order
+    DevBuf<double> Paos;        // [ne * 10] P of the last evaluation (row-major) and -V, one row per cell
     DevBuf<int64_t> rstart;     // [nn + 1] reverse neighbor lists: points i that list node J
     DevBuf<int32_t> rsrc;       // [nrev] i, ascending per node
     DevBuf<double> rphi;        // [nrev] phi_iJ
@@ -144,6 +148,7 @@ struct CellData {
         ready = false;
         two_pass = false;
         rstart.release(); rsrc.release(); rphi.release(); u_phys.release(); f_local.release();
+        ent_cell.release(); dphi_node.release(); Paos.release();
         nnz = n_truncated = 0;
         num_nbr.release(); nbr.release(); phi.release(); radius.release(); start.release();
         nodes.release(); dphi.release(); volume.release(); inc_cell.release(); node_start.release();

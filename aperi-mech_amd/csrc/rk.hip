@@ -494,6 +494,18 @@ __global__ __launch_bounds__(256) void k_invert_perm(int64_t n, const int32_t *p
     if (k < n) pos[perm[k]] = (int32_t)k;
 }
 
+// node-order copies: dphi_node[k] = dphi[inc[k]], ent_cell[k] = cell of entry inc[k]
+__global__ __launch_bounds__(256) void k_node_order(int64_t n, const int32_t *inc, const double *dphi,
+                                                    const int32_t *entry_cell, double *dphi_node, int32_t *ent_cell) {
+    const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (k >= n) return;
+    const int64_t t = inc[k];
+    dphi_node[3 * k] = dphi[3 * t];
+    dphi_node[3 * k + 1] = dphi[3 * t + 1];
+    dphi_node[3 * k + 2] = dphi[3 * t + 2];
+    ent_cell[k] = entry_cell[t];
+}
+
 __global__ __launch_bounds__(256) void k_count_keys(int64_t n, const int32_t *keys, int32_t *count) {
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (i < n) atomicAdd(&count[keys[i]], 1);
@@ -504,13 +516,16 @@ __global__ __launch_bounds__(256) void k_count_keys(int64_t n, const int32_t *ke
 // (entry t goes to lane t mod 16, ascending within a lane), accumulate their share
 // of H = sum_t u_node(t) (x) grad phi_t with fused multiply-adds and combine the 16
 // partial sums with a xor butterfly; addition is commutative, so every lane ends
-// with the same bits and evaluates the stress update itself (no broadcast).  Each
-// entry then gets its force contribution g_t = -V P grad phi_t (3 doubles), the
-// deterministic replacement of the reference's per-entry atomic scatter
-// (include/ComputeInternalForceSmoothedCell.h:312-351), stored at the entry's
-// position in node order (all entries naming node 0, ascending entry, then node 1 ...).
-// Nodes: a group of 16 lanes per node sums its contiguous run of g (ascending entry,
-// strided over the lanes, then the same butterfly) and finishes the node.
+// with the same bits and evaluates the stress update itself (no broadcast).  The
+// cell stores H, P and P once more as one row per cell.
+// Nodes (one-pass): a group of 16 lanes per node walks its contiguous run of entries
+// in node order (all entries naming node 0, ascending entry, then node 1 ...), each
+// with its grad phi copied into that order and its cell, and sums the entry forces
+// -V P grad phi recomputed from the cell's row (ascending entry, strided over the
+// lanes, then the same butterfly): the deterministic replacement of the reference's
+// per-entry atomic scatter (include/ComputeInternalForceSmoothedCell.h:312-351)
+// without a per-entry force array in HBM.  The two-pass cells (four entries) keep
+// their entry forces in node order (g) for the plain node sum.
 constexpr int kCellLanes = 16;
 
 struct CellForceArgs {
@@ -529,8 +544,10 @@ struct CellForceArgs {
     const double *elem_lambda, *elem_mu;
     const double *Hn, *Pn;  // rate-form materials: H and P of the previous step
     double rate_dt;
+    double *Paos;           // one-pass: [ne * kRowLen] rows (P row-major, then -V)
 };
 
+constexpr int kRowLen = 10;  // doubles per cell row of the one-pass node kernel
 constexpr int kCellCache = 6;  // entries per lane whose grad phi stays in registers
 constexpr int kNodeBatch = 4;  // incidences per lane whose loads are issued together
 constexpr int kNumXcd = 8;
@@ -562,20 +579,17 @@ __global__ __launch_bounds__(256) void k_rk_cells(CellForceArgs A) {
     const bool live = c_raw < A.ne;
     const int64_t c = live ? c_raw : A.ne - 1;  // idle groups recompute the last cell and store nothing
     const int64_t t0 = A.start[c], t1 = A.start[c + 1];
-    const double mv = -A.volume[c];
     double H[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, P[9];
     // The first kCache entries of the lane: every load issued before the first
-    // use (clamped indices, absent entries weighted by zero), grad phi kept for the
-    // force pass.  Entries are accumulated in ascending order either way.
+    // use (clamped indices, absent entries weighted by zero).  Entries are
+    // accumulated in ascending order either way.
     double gc[kCache][3] = {};
-    int32_t ps[kCache] = {};
     if (t1 > t0) {  // (an empty cell has no entry to clamp to)
         int32_t nd[kCache];
 #pragma unroll
         for (int r = 0; r < kCache; ++r) {
             const int64_t t = min(t0 + l + r * LANES, t1 - 1);
             nd[r] = A.nodes[t];
-            ps[r] = A.pos[t];
         }
         double uj[kCache][3];
 #pragma unroll
@@ -625,25 +639,13 @@ __global__ __launch_bounds__(256) void k_rk_cells(CellForceArgs A) {
             store_once(A.P + i * A.ne + c, P[i]);
         }
     }
-    if (live) {
+    // P once more as one 80-byte row per cell, with -V in its tenth slot, for the node
+    // kernel's gathers (five aligned 16-byte loads per entry)
+    if (live && l < kRowLen) {
+        double pl = -A.volume[c];
 #pragma unroll
-        for (int r = 0; r < kCache; ++r) {
-            const int64_t t = t0 + l + r * LANES;
-            if (t < t1) {
-                double f[3];
-#pragma unroll
-                for (int a = 0; a < 3; ++a)
-                    f[a] = mv * dot3(P[3 * a], gc[r][0], P[3 * a + 1], gc[r][1], P[3 * a + 2], gc[r][2]);
-                st3(A.g, ps[r], f);
-            }
-        }
-        for (int64_t t = t0 + l + kCache * LANES; t < t1; t += LANES) {
-            double g[3], f[3];
-            ld3(A.dphi, t, g);
-#pragma unroll
-            for (int a = 0; a < 3; ++a) f[a] = mv * dot3(P[3 * a], g[0], P[3 * a + 1], g[1], P[3 * a + 2], g[2]);
-            st3(A.g, A.pos[t], f);
-        }
+        for (int i = 0; i < 9; ++i) pl = l == i ? P[i] : pl;
+        A.Paos[kRowLen * c + l] = pl;
     }
 }
 
@@ -770,6 +772,59 @@ __global__ __launch_bounds__(256) void k_rk_nodes(RkNodeArgs A) {
     for (int d = 0; d < 3; ++d) f[d] = group_sum16(f[d]);
     if (!live || l != 0) return;
     rk_finish<MODE>(A, i, f);
+}
+
+struct RkNodeCellArgs {
+    RkNodeArgs out;
+    const int32_t *ent_cell;   // [nnz] cell of each node-order entry
+    const double *dphi_node;   // [nnz * 3] its grad phi
+    const double *Paos;        // [ne * kRowLen] rows: P row-major, then -V
+};
+
+// One-pass node kernel: f_J = sum over J's node-order entries k of -V_c P_c grad phi_k
+// (c = the entry's cell), the cell kernel's former per-entry expression, in the same
+// order; then the node update.
+template <int MODE>
+__global__ __launch_bounds__(256) void k_rk_nodes_cells(RkNodeCellArgs A) {
+#pragma clang fp contract(off)
+    const int l = threadIdx.x & (kCellLanes - 1);
+    const int64_t i_raw = xcd_block(A.out.nblocks) * (256 / kCellLanes) + (threadIdx.x / kCellLanes);
+    const bool live = i_raw < A.out.nn;
+    const int64_t i = live ? i_raw : A.out.nn - 1;
+    double f[3] = {0.0, 0.0, 0.0};
+    const int64_t t0 = A.out.inc_start[i], t1 = A.out.inc_start[i + 1];
+    for (int64_t tb = t0 + l; tb < t1; tb += kNodeBatch * kCellLanes) {
+        int32_t cc[kNodeBatch];
+        double g[kNodeBatch][3];
+#pragma unroll
+        for (int r = 0; r < kNodeBatch; ++r) {
+            const int64_t t = min(tb + r * kCellLanes, t1 - 1);
+            cc[r] = A.ent_cell[t];
+            ld3(A.dphi_node, t, g[r]);
+        }
+#pragma unroll
+        for (int r = 0; r < kNodeBatch; ++r) {
+            const double2 *row = reinterpret_cast<const double2 *>(A.Paos + kRowLen * (int64_t)cc[r]);
+            double Pr[kRowLen];
+#pragma unroll
+            for (int q = 0; q < kRowLen / 2; ++q) {
+                const double2 w = row[q];
+                Pr[2 * q] = w.x;
+                Pr[2 * q + 1] = w.y;
+            }
+            const double mv = Pr[9];
+            const bool in = tb + r * kCellLanes < t1;
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                const double fa = mv * dot3(Pr[3 * a], g[r][0], Pr[3 * a + 1], g[r][1], Pr[3 * a + 2], g[r][2]);
+                f[a] += in ? fa : 0.0;
+            }
+        }
+    }
+#pragma unroll
+    for (int d = 0; d < 3; ++d) f[d] = group_sum16(f[d]);
+    if (!live || l != 0) return;
+    rk_finish<MODE>(A.out, i, f);
 }
 
 // ---- two-pass method: gather before, scatter after the cell force -----------
@@ -1086,8 +1141,18 @@ void rk_setup(Ctx &c, double scale, bool two_pass) {
         D.entry_pos.alloc(std::max<int64_t>(D.nnz, 1));
         hipLaunchKernelGGL(k_invert_perm, grid1(D.nnz, 256), dim3(256), 0, c.stream, D.nnz, inc.p, D.entry_pos.p);
         APHIP_HIP(hipGetLastError());
+        if (!two_pass) {
+            D.dphi_node.alloc(3 * std::max<int64_t>(D.nnz, 1));
+            D.ent_cell.alloc(std::max<int64_t>(D.nnz, 1));
+            if (D.nnz) {
+                hipLaunchKernelGGL(k_node_order, grid1(D.nnz, 256), dim3(256), 0, c.stream, D.nnz, inc.p, D.dphi.p,
+                                   D.inc_cell.p, D.dphi_node.p, D.ent_cell.p);
+                APHIP_HIP(hipGetLastError());
+            }
+        }
     }
-    D.g.alloc(3 * std::max<int64_t>(D.nnz, 1));
+    if (two_pass) D.g.alloc(3 * std::max<int64_t>(D.nnz, 1));
+    else D.Paos.alloc(kRowLen * std::max<int64_t>(ne, 1));
     APHIP_HIP(hipStreamSynchronize(c.stream));
     D.ready = true;
 }
@@ -1107,7 +1172,8 @@ void rk_cell_force(Ctx &c) {
                     c.state[1 - c.state_np1].p, c.state[c.state_np1].p,
                     c.mat.params(),
                     c.has_fields ? c.elem_lambda.p : nullptr, c.has_fields ? c.elem_mu.p : nullptr,
-                    c.mat.rate_form() ? c.Hn().p : nullptr, c.mat.rate_form() ? c.Pn().p : nullptr, c.rate_dt};
+                    c.mat.rate_form() ? c.Hn().p : nullptr, c.mat.rate_form() ? c.Pn().p : nullptr, c.rate_dt,
+                    D.Paos.p};
     const dim3 g((unsigned)nbc);
     if (D.two_pass) {
         const double *X = c.X.p;
@@ -1168,9 +1234,10 @@ void rk_node_update(Ctx &c, double dt, int mode) {
         APHIP_HIP(hipGetLastError());
         return;
     }
-    if (mode == 0) hipLaunchKernelGGL(k_rk_nodes<0>, g, dim3(256), 0, c.stream, A);
-    else if (mode == 1) hipLaunchKernelGGL(k_rk_nodes<1>, g, dim3(256), 0, c.stream, A);
-    else hipLaunchKernelGGL(k_rk_nodes<2>, g, dim3(256), 0, c.stream, A);
+    RkNodeCellArgs N{A, D.ent_cell.p, D.dphi_node.p, D.Paos.p};
+    if (mode == 0) hipLaunchKernelGGL(k_rk_nodes_cells<0>, g, dim3(256), 0, c.stream, N);
+    else if (mode == 1) hipLaunchKernelGGL(k_rk_nodes_cells<1>, g, dim3(256), 0, c.stream, N);
+    else hipLaunchKernelGGL(k_rk_nodes_cells<2>, g, dim3(256), 0, c.stream, N);
     APHIP_HIP(hipGetLastError());
 }
 
