@@ -363,17 +363,18 @@ class ElementGaussianHip : public ElementBase {
     }
 };
 
-// ElementReproducingKernelTet4, element-as-cell strain smoothing
-// (include/ElementReproducingKernel.h:67-80, :214-224), one-pass by default
-// (IntegrationSchemeParameters::uses_one_pass_method); false selects the two-pass
-// force (gather, cell force on the physical displacement, scatter).
+// ElementReproducingKernelTet4 / ElementReproducingKernelHex8, element-as-cell strain
+// smoothing (include/ElementReproducingKernel.h:67-80, :214-224, :317-378), one-pass by
+// default (IntegrationSchemeParameters::uses_one_pass_method); false selects the
+// two-pass force (gather, cell force on the physical displacement, scatter).
 class ElementReproducingKernelHip : public ElementBase {
    public:
     ElementReproducingKernelHip(const MeshBlock &mesh, double kernel_radius_scale_factor,
                                 std::shared_ptr<Device> device, std::shared_ptr<Material> material,
-                                bool use_one_pass_method = true)
-        : ElementBase(4, std::move(device), std::move(material)) {
-        BindMesh(APHIP_TET4, mesh);
+                                bool use_one_pass_method = true,
+                                ElementTopology topology = ElementTopology::Tetrahedron4)
+        : ElementBase(topology == ElementTopology::Tetrahedron4 ? 4 : 8, std::move(device), std::move(material)) {
+        BindMesh(topology == ElementTopology::Tetrahedron4 ? APHIP_TET4 : APHIP_HEX8, mesh);
         Check(aphip_rk_setup_method(Context(), kernel_radius_scale_factor, use_one_pass_method ? 1 : 0));
     }
     void ComputeInternalForceAllElements(double time_increment, double time) override {
@@ -401,10 +402,8 @@ inline std::shared_ptr<ElementBase> CreateElement(ElementTopology topology, Appr
     if (space == ApproximationSpaceType::ReproducingKernel) {
         if (scheme != IntegrationSchemeType::StrainSmoothing)
             throw std::runtime_error("Gauss Quadrature is not supported for Reproducing Kernel");
-        if (topology != ElementTopology::Tetrahedron4)
-            throw std::runtime_error("Reproducing kernel cells on Hexahedron8 are not supported by the MI355X force step");
         return std::make_shared<ElementReproducingKernelHip>(mesh, kernel_radius_scale_factor, std::move(device),
-                                                             std::move(material), use_one_pass_method);
+                                                             std::move(material), use_one_pass_method, topology);
     }
     if (scheme == IntegrationSchemeType::StrainSmoothing) {
         if (topology == ElementTopology::Hexahedron8)

@@ -1175,7 +1175,9 @@ int aphip_rk_setup_method(aphip_ctx *c, double scale, int32_t use_one_pass_metho
     APHIP_API_BEGIN
     APHIP_REQUIRE(c != nullptr, "ctx is NULL");
     require_ready(*c, false);
-    APHIP_REQUIRE(c->etype == APHIP_TET4, "RKPM element-as-cell smoothing needs a Tet4 background mesh");
+    APHIP_REQUIRE(c->etype == APHIP_TET4 || c->etype == APHIP_HEX8,
+                  "RKPM element-as-cell smoothing needs a Tet4 or Hex8 background mesh");
+    APHIP_REQUIRE(c->mat.state_per_qp == 0, "RKPM cells keep one material state per cell, not per Gauss point");
     APHIP_REQUIRE(scale >= 0.0, "Kernel radius scale factor must be non-negative.");
     APHIP_HIP(hipSetDevice(c->device));
     APHIP_HIP(hipStreamSynchronize(c->stream));

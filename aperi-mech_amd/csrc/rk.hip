@@ -44,7 +44,6 @@ namespace {
 constexpr int kMaxNbr = 40;  // include/Constants.h:15
 constexpr int kSearchBlock = 64;
 constexpr int kCellBlock = 64;
-constexpr int kMaxCellNodes = 4 * kMaxNbr;
 
 __device__ __forceinline__ void ld3(const double *p, int64_t i, double *o) {
     o[0] = p[3 * i];
@@ -64,6 +63,7 @@ __device__ __forceinline__ double dist2(const double *a, const double *b) {
     return __fma_rn(dz, dz, __fma_rn(dy, dy, dx * dx));
 }
 
+template <int NPE>
 __global__ __launch_bounds__(256) void k_max_edge(int64_t nn, const int32_t *node_start, const int32_t *node_inc,
                                                   const int32_t *conn, const double *X, double scale, double *radius) {
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
@@ -72,11 +72,11 @@ __global__ __launch_bounds__(256) void k_max_edge(int64_t nn, const int32_t *nod
     ld3(X, i, xi);
     double l = 0.0;
     for (int32_t t = node_start[i]; t < node_start[i + 1]; ++t) {
-        const int64_t e = node_inc[t] / 4;
+        const int64_t e = node_inc[t] / NPE;
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
+        for (int k = 0; k < NPE; ++k) {
             double xj[3];
-            ld3(X, conn[4 * e + k], xj);
+            ld3(X, conn[NPE * e + k], xj);
             l = fmax(l, sqrt(dist2(xi, xj)));
         }
     }
@@ -345,11 +345,12 @@ __global__ __launch_bounds__(128) void k_rk_phi(int64_t nn, const double *X, con
     }
 }
 
-// First-seen union of the neighbor lists of a cell's four nodes, in LDS.
+// First-seen union of the neighbor lists of a cell's NPE nodes, in LDS.
+template <int NPE>
 __device__ __forceinline__ int cell_union(const int32_t *en, const int32_t *num, const int32_t *nbr, int32_t *list,
                                           int stride) {
     int len = 0;
-    for (int k = 0; k < 4; ++k) {
+    for (int k = 0; k < NPE; ++k) {
         const int32_t a = en[k];
         const int na = num[a];
         for (int l = 0; l < na; ++l) {
@@ -366,13 +367,22 @@ __device__ __forceinline__ int cell_union(const int32_t *en, const int32_t *num,
     return len;
 }
 
-__global__ __launch_bounds__(kCellBlock) void k_cell_count(int64_t ne, const int32_t *conn, const int32_t *num,
-                                                           const int32_t *nbr, int32_t *len_out) {
-    __shared__ int32_t list[kMaxCellNodes * kCellBlock];
-    const int64_t e = (int64_t)blockIdx.x * kCellBlock + threadIdx.x;
+// cells per block of the set-up kernels: each thread keeps its union list of up to
+// NPE * kMaxNbr entries in LDS (40 KB per block either way)
+template <int NPE>
+constexpr int cell_block() { return NPE == 4 ? kCellBlock : kCellBlock / 2; }
+
+template <int NPE>
+__global__ __launch_bounds__(cell_block<NPE>()) void k_cell_count(int64_t ne, const int32_t *conn, const int32_t *num,
+                                                                  const int32_t *nbr, int32_t *len_out) {
+    constexpr int BLK = cell_block<NPE>();
+    __shared__ int32_t list[NPE * kMaxNbr * BLK];
+    const int64_t e = (int64_t)blockIdx.x * BLK + threadIdx.x;
     if (e >= ne) return;
-    const int32_t en[4] = {conn[4 * e], conn[4 * e + 1], conn[4 * e + 2], conn[4 * e + 3]};
-    len_out[e] = cell_union(en, num, nbr, list + threadIdx.x, kCellBlock);
+    int32_t en[NPE];
+#pragma unroll
+    for (int k = 0; k < NPE; ++k) en[k] = conn[NPE * e + k];
+    len_out[e] = cell_union<NPE>(en, num, nbr, list + threadIdx.x, BLK);
 }
 
 // Smoothed Tet4 B and volume from the faces (include/QuadratureSmoothed.h:32-62)
@@ -407,6 +417,57 @@ __device__ void smoothed_tet4(const double (&X)[4][3], double (&B)[4][3], double
     V = vol;
 }
 
+// Smoothed Hex8 B and volume from the faces, each split into four triangles about
+// its centroid (include/QuadratureSmoothed.h:96-150)
+__device__ void smoothed_hex8(const double (&X)[8][3], double (&B)[8][3], double &V) {
+    const int faces[6][4] = {{0, 1, 5, 4}, {1, 2, 6, 5}, {2, 3, 7, 6}, {0, 4, 7, 3}, {0, 3, 2, 1}, {4, 5, 6, 7}};
+    double vol = 0.0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) B[k][0] = B[k][1] = B[k][2] = 0.0;
+#pragma unroll
+    for (int f = 0; f < 6; ++f) {
+        double mid[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+#pragma unroll
+            for (int d = 0; d < 3; ++d) mid[d] += X[faces[f][k]][d];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) mid[d] /= 4.0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int km1 = (k + 3) & 3;
+            double e1[3], e2[3], n[3];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                e1[d] = X[faces[f][km1]][d] - mid[d];
+                e2[d] = X[faces[f][k]][d] - mid[d];
+            }
+            n[0] = (e1[1] * e2[2] - e1[2] * e2[1]) / 2.0;
+            n[1] = (e1[2] * e2[0] - e1[0] * e2[2]) / 2.0;
+            n[2] = (e1[0] * e2[1] - e1[1] * e2[0]) / 2.0;
+            vol += mid[0] * n[0] + mid[1] * n[1] + mid[2] * n[2];
+#pragma unroll
+            for (int l = 0; l < 4; ++l) {
+                const double cw = (l == k || l == km1) ? 5.0 / 12.0 : 1.0 / 12.0;
+#pragma unroll
+                for (int d = 0; d < 3; ++d) B[faces[f][l]][d] += n[d] * cw;
+            }
+        }
+    }
+    vol /= 3.0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) B[k][d] /= vol;
+    V = vol;
+}
+
+template <int NPE>
+__device__ __forceinline__ void smoothed_cell(const double (&X)[NPE][3], double (&B)[NPE][3], double &V) {
+    if constexpr (NPE == 4) smoothed_tet4(X, B, V);
+    else smoothed_hex8(X, B, V);
+}
+
 struct CellFillArgs {
     int64_t ne;
     const int32_t *conn;
@@ -421,35 +482,39 @@ struct CellFillArgs {
     int32_t *entry_cell;
 };
 
-__global__ __launch_bounds__(kCellBlock) void k_cell_fill(CellFillArgs A) {
-    __shared__ int32_t list[kMaxCellNodes * kCellBlock];
+template <int NPE>
+__global__ __launch_bounds__(cell_block<NPE>()) void k_cell_fill(CellFillArgs A) {
+    constexpr int BLK = cell_block<NPE>();
+    __shared__ int32_t list[NPE * kMaxNbr * BLK];
     const int tid = threadIdx.x;
-    const int64_t e = (int64_t)blockIdx.x * kCellBlock + tid;
+    const int64_t e = (int64_t)blockIdx.x * BLK + tid;
     if (e >= A.ne) return;
-    const int32_t en[4] = {A.conn[4 * e], A.conn[4 * e + 1], A.conn[4 * e + 2], A.conn[4 * e + 3]};
-    int32_t *my = list + tid;
-    const int len = cell_union(en, A.num, A.nbr, my, kCellBlock);
-    const int64_t s0 = A.start[e];
-    double X[4][3], B[4][3], V;
+    int32_t en[NPE];
 #pragma unroll
-    for (int k = 0; k < 4; ++k) ld3(A.X, en[k], X[k]);
-    smoothed_tet4(X, B, V);
+    for (int k = 0; k < NPE; ++k) en[k] = A.conn[NPE * e + k];
+    int32_t *my = list + tid;
+    const int len = cell_union<NPE>(en, A.num, A.nbr, my, BLK);
+    const int64_t s0 = A.start[e];
+    double X[NPE][3], B[NPE][3], V;
+#pragma unroll
+    for (int k = 0; k < NPE; ++k) ld3(A.X, en[k], X[k]);
+    smoothed_cell<NPE>(X, B, V);
     double *dp = A.dphi + 3 * s0;
     for (int t = 0; t < len; ++t) {
-        A.nodes[s0 + t] = my[t * kCellBlock];
+        A.nodes[s0 + t] = my[t * BLK];
         A.entry_cell[s0 + t] = (int32_t)e;
         dp[3 * t] = dp[3 * t + 1] = dp[3 * t + 2] = 0.0;
     }
     // grad phi_J = sum_k B_k V phi_{k,J} / V_sub (SmoothedCellDataProcessor.h:623-662)
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
+    for (int k = 0; k < NPE; ++k) {
         const int32_t a = en[k];
         const int na = A.num[a];
         for (int l = 0; l < na; ++l) {
             const int32_t j = A.nbr[(int64_t)a * kMaxNbr + l];
             const double fv = A.phi[(int64_t)a * kMaxNbr + l];
             int t = 0;
-            while (my[t * kCellBlock] != j) ++t;
+            while (my[t * BLK] != j) ++t;
 #pragma unroll
             for (int m = 0; m < 3; ++m) dp[3 * t + m] += B[k][m] * V * fv;
         }
@@ -463,22 +528,25 @@ __global__ __launch_bounds__(kCellBlock) void k_cell_fill(CellFillArgs A) {
 }
 
 // Two-pass cells: entry 4e + k is node k of element e with derivative (B_k V) / V.
+template <int NPE>
 __global__ __launch_bounds__(256) void k_cell_two_pass(int64_t ne, const int32_t *conn, const double *X,
                                                        int32_t *nodes, double *dphi, double *volume,
                                                        int32_t *entry_cell) {
     const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (e >= ne) return;
-    const int32_t en[4] = {conn[4 * e], conn[4 * e + 1], conn[4 * e + 2], conn[4 * e + 3]};
-    double X4[4][3], B[4][3], V;
+    int32_t en[NPE];
 #pragma unroll
-    for (int k = 0; k < 4; ++k) ld3(X, en[k], X4[k]);
-    smoothed_tet4(X4, B, V);
+    for (int k = 0; k < NPE; ++k) en[k] = conn[NPE * e + k];
+    double Xe[NPE][3], B[NPE][3], V;
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        nodes[4 * e + k] = en[k];
-        entry_cell[4 * e + k] = (int32_t)e;
+    for (int k = 0; k < NPE; ++k) ld3(X, en[k], Xe[k]);
+    smoothed_cell<NPE>(Xe, B, V);
 #pragma unroll
-        for (int m = 0; m < 3; ++m) dphi[3 * (4 * e + k) + m] = (B[k][m] * V) / V;
+    for (int k = 0; k < NPE; ++k) {
+        nodes[NPE * e + k] = en[k];
+        entry_cell[NPE * e + k] = (int32_t)e;
+#pragma unroll
+        for (int m = 0; m < 3; ++m) dphi[3 * (NPE * e + k) + m] = (B[k][m] * V) / V;
     }
     volume[e] = V;
 }
@@ -659,22 +727,22 @@ __global__ __launch_bounds__(256) void k_rk_cells(CellForceArgs A) {
 // (B_k V) / V recomputed from the coordinates by the set-up's formula (instead of
 // streaming the stored 96 B per cell), H, stress and the four entry forces
 // -V P grad N_k at their node-order positions.
-template <int KIND>
+template <int KIND, int NPE>
 __global__ __launch_bounds__(256) void k_rk_cells_two_pass(CellForceArgs A, const double *X) {
 #pragma clang fp contract(off)
     const int64_t c_raw = xcd_block(A.nblocks) * 256 + threadIdx.x;
     const bool live = c_raw < A.ne;
     const int64_t c = live ? c_raw : A.ne - 1;
-    int32_t en[4], ps[4];
+    int32_t en[NPE], ps[NPE];
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        en[k] = A.nodes[4 * c + k];
-        ps[k] = A.pos[4 * c + k];
+    for (int k = 0; k < NPE; ++k) {
+        en[k] = A.nodes[NPE * c + k];
+        ps[k] = A.pos[NPE * c + k];
     }
-    double X4[4][3], uj[4][3];
+    double Xe[NPE][3], uj[NPE][3];
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        ld3(X, en[k], X4[k]);
+    for (int k = 0; k < NPE; ++k) {
+        ld3(X, en[k], Xe[k]);
         ld3(A.u, en[k], uj[k]);
     }
     double snp1 = 0.0, aux = 0.0;
@@ -684,16 +752,16 @@ __global__ __launch_bounds__(256) void k_rk_cells_two_pass(CellForceArgs A, cons
         m.lambda = A.elem_lambda[c];
         m.mu = A.elem_mu[c];
     }
-    double B[4][3], V;
-    smoothed_tet4(X4, B, V);
-    double gc[4][3];
+    double B[NPE][3], V;
+    smoothed_cell<NPE>(Xe, B, V);
+    double gc[NPE][3];
 #pragma unroll
-    for (int k = 0; k < 4; ++k)
+    for (int k = 0; k < NPE; ++k)
 #pragma unroll
         for (int b = 0; b < 3; ++b) gc[k][b] = (B[k][b] * V) / V;
     double H[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, P[9];
 #pragma unroll
-    for (int k = 0; k < 4; ++k)
+    for (int k = 0; k < NPE; ++k)
 #pragma unroll
         for (int a = 0; a < 3; ++a)
 #pragma unroll
@@ -711,7 +779,7 @@ __global__ __launch_bounds__(256) void k_rk_cells_two_pass(CellForceArgs A, cons
     }
     const double mv = -V;
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
+    for (int k = 0; k < NPE; ++k) {
         double f[3];
 #pragma unroll
         for (int a = 0; a < 3; ++a) f[a] = mv * dot3(P[3 * a], gc[k][0], P[3 * a + 1], gc[k][1], P[3 * a + 2], gc[k][2]);
@@ -924,13 +992,14 @@ __global__ __launch_bounds__(256) void k_rk_scatter(RkScatterArgs A) {
     rk_finish<MODE>(A.out, i, f);
 }
 
-// mass_from_elements: rho V / 4 from every cell touching the node, ascending cell
+// mass_from_elements: rho V / NPE from every cell touching the node, ascending cell
+template <int NPE>
 __global__ __launch_bounds__(256) void k_rk_mfe(int64_t nn, const int32_t *node_start, const int32_t *node_inc,
                                                 const double *vol, double rho, double *mfe) {
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (i >= nn) return;
     double m = 0.0;
-    for (int32_t t = node_start[i]; t < node_start[i + 1]; ++t) m += (vol[node_inc[t] / 4] * rho) / 4;
+    for (int32_t t = node_start[i]; t < node_start[i + 1]; ++t) m += (vol[node_inc[t] / NPE] * rho) / NPE;
     mfe[i] = m;
 }
 
@@ -980,7 +1049,9 @@ void rk_setup(Ctx &c, double scale, bool two_pass) {
 
     // kernel radius from the max edge length
     D.radius.alloc(nn);
-    hipLaunchKernelGGL(k_max_edge, grid1(nn, 256), dim3(256), 0, c.stream, nn, c.node_start.p, c.node_inc.p, c.conn.p,
+    const bool hex = c.etype == APHIP_HEX8;  // Hex8 background: ElementReproducingKernelHex8
+    const int npe = hex ? 8 : 4;
+    hipLaunchKernelGGL(hex ? k_max_edge<8> : k_max_edge<4>, grid1(nn, 256), dim3(256), 0, c.stream, nn, c.node_start.p, c.node_inc.p, c.conn.p,
                        c.X.p, scale, D.radius.p);
     APHIP_HIP(hipGetLastError());
 
@@ -1059,17 +1130,17 @@ void rk_setup(Ctx &c, double scale, bool two_pass) {
     D.two_pass = two_pass;
     if (two_pass) {
         // the element's own nodes (SmoothedCellDataProcessor.h:474-486, :637-648)
-        D.nnz = 4 * ne;
+        D.nnz = npe * ne;
         APHIP_REQUIRE(D.nnz < (int64_t)INT32_MAX, "too many cell entries for int32 incidences");
         std::vector<int64_t> hs(ne + 1);
-        for (int64_t e = 0; e <= ne; ++e) hs[e] = 4 * e;
+        for (int64_t e = 0; e <= ne; ++e) hs[e] = npe * e;
         D.start.upload(hs.data(), hs.size(), c.stream);
         D.nodes.alloc(std::max<int64_t>(D.nnz, 1));
         D.dphi.alloc(3 * std::max<int64_t>(D.nnz, 1));
         D.volume.alloc(std::max<int64_t>(ne, 1));
         D.inc_cell.alloc(std::max<int64_t>(D.nnz, 1));
         if (ne) {
-            hipLaunchKernelGGL(k_cell_two_pass, grid1(ne, 256), dim3(256), 0, c.stream, ne, c.conn.p, c.X.p,
+            hipLaunchKernelGGL(hex ? k_cell_two_pass<8> : k_cell_two_pass<4>, grid1(ne, 256), dim3(256), 0, c.stream, ne, c.conn.p, c.X.p,
                                D.nodes.p, D.dphi.p, D.volume.p, D.inc_cell.p);
             APHIP_HIP(hipGetLastError());
         }
@@ -1081,7 +1152,8 @@ void rk_setup(Ctx &c, double scale, bool two_pass) {
         DevBuf<int32_t> clen;
         clen.alloc(std::max<int64_t>(ne, 1));
         if (ne) {
-            hipLaunchKernelGGL(k_cell_count, grid1(ne, kCellBlock), dim3(kCellBlock), 0, c.stream, ne, c.conn.p,
+            const int blk = hex ? cell_block<8>() : cell_block<4>();
+            hipLaunchKernelGGL(hex ? k_cell_count<8> : k_cell_count<4>, grid1(ne, blk), dim3(blk), 0, c.stream, ne, c.conn.p,
                                D.num_nbr.p, D.nbr.p, clen.p);
             APHIP_HIP(hipGetLastError());
         }
@@ -1094,7 +1166,8 @@ void rk_setup(Ctx &c, double scale, bool two_pass) {
         if (ne) {
             CellFillArgs F{ne, c.conn.p, c.X.p, D.num_nbr.p, D.nbr.p, D.phi.p, D.start.p, D.nodes.p, D.dphi.p,
                            D.volume.p, D.inc_cell.p};
-            hipLaunchKernelGGL(k_cell_fill, grid1(ne, kCellBlock), dim3(kCellBlock), 0, c.stream, F);
+            const int blk = hex ? cell_block<8>() : cell_block<4>();
+            hipLaunchKernelGGL(hex ? k_cell_fill<8> : k_cell_fill<4>, grid1(ne, blk), dim3(blk), 0, c.stream, F);
             APHIP_HIP(hipGetLastError());
         }
     }
@@ -1183,19 +1256,27 @@ void rk_cell_force(Ctx &c) {
     const dim3 g((unsigned)nbc);
     if (D.two_pass) {
         const double *X = c.X.p;
+        const bool hex = c.etype == APHIP_HEX8;
+        auto go = [&](auto k4, auto k8) {
+            hipLaunchKernelGGL(hex ? k8 : k4, g, dim3(256), 0, c.stream, A, X);
+        };
         switch (c.mat.kind) {
-            case APHIP_LINEAR_ELASTIC: hipLaunchKernelGGL(k_rk_cells_two_pass<APHIP_LINEAR_ELASTIC>, g, dim3(256), 0, c.stream, A, X); break;
-            case APHIP_NEO_HOOKEAN: hipLaunchKernelGGL(k_rk_cells_two_pass<APHIP_NEO_HOOKEAN>, g, dim3(256), 0, c.stream, A, X); break;
-            case APHIP_J2: hipLaunchKernelGGL(k_rk_cells_two_pass<APHIP_J2>, g, dim3(256), 0, c.stream, A, X); break;
-            case APHIP_SVK: hipLaunchKernelGGL(k_rk_cells_two_pass<APHIP_SVK>, g, dim3(256), 0, c.stream, A, X); break;
+            case APHIP_LINEAR_ELASTIC:
+                go(k_rk_cells_two_pass<APHIP_LINEAR_ELASTIC, 4>, k_rk_cells_two_pass<APHIP_LINEAR_ELASTIC, 8>);
+                break;
+            case APHIP_NEO_HOOKEAN:
+                go(k_rk_cells_two_pass<APHIP_NEO_HOOKEAN, 4>, k_rk_cells_two_pass<APHIP_NEO_HOOKEAN, 8>);
+                break;
+            case APHIP_J2: go(k_rk_cells_two_pass<APHIP_J2, 4>, k_rk_cells_two_pass<APHIP_J2, 8>); break;
+            case APHIP_SVK: go(k_rk_cells_two_pass<APHIP_SVK, 4>, k_rk_cells_two_pass<APHIP_SVK, 8>); break;
             case APHIP_NEO_HOOKEAN_DAMAGE:
-                hipLaunchKernelGGL(k_rk_cells_two_pass<APHIP_NEO_HOOKEAN_DAMAGE>, g, dim3(256), 0, c.stream, A, X);
+                go(k_rk_cells_two_pass<APHIP_NEO_HOOKEAN_DAMAGE, 4>, k_rk_cells_two_pass<APHIP_NEO_HOOKEAN_DAMAGE, 8>);
                 break;
             case APHIP_POWER_LAW_CREEP:
-                hipLaunchKernelGGL(k_rk_cells_two_pass<APHIP_POWER_LAW_CREEP>, g, dim3(256), 0, c.stream, A, X);
+                go(k_rk_cells_two_pass<APHIP_POWER_LAW_CREEP, 4>, k_rk_cells_two_pass<APHIP_POWER_LAW_CREEP, 8>);
                 break;
             case APHIP_DRUCKER_PRAGER:
-                hipLaunchKernelGGL(k_rk_cells_two_pass<APHIP_DRUCKER_PRAGER>, g, dim3(256), 0, c.stream, A, X);
+                go(k_rk_cells_two_pass<APHIP_DRUCKER_PRAGER, 4>, k_rk_cells_two_pass<APHIP_DRUCKER_PRAGER, 8>);
                 break;
             default: throw Error(APHIP_ERR_INVALID, "material not set");
         }
@@ -1255,7 +1336,7 @@ void rk_mass(Ctx &c, double *total) {
     const int64_t nn = c.nn;
     DevBuf<double> mfe;
     mfe.alloc(nn);
-    hipLaunchKernelGGL(k_rk_mfe, grid1(nn, 256), dim3(256), 0, c.stream, nn, c.node_start.p, c.node_inc.p,
+    hipLaunchKernelGGL(c.etype == APHIP_HEX8 ? k_rk_mfe<8> : k_rk_mfe<4>, grid1(nn, 256), dim3(256), 0, c.stream, nn, c.node_start.p, c.node_inc.p,
                        D.volume.p, c.mat.density, mfe.p);
     APHIP_HIP(hipGetLastError());
     c.mass.alloc(nn);

@@ -276,7 +276,9 @@ int aphip_synchronize(aphip_ctx *ctx);
  * FunctionCreationProcessor::compute_and_store_function_values
  * (include/FunctionCreationProcessor.h:73-134) and
  * SmoothedCellDataProcessor::SetSubcellNodeIndices / ComputeFunctionDerivatives
- * (include/SmoothedCellDataProcessor.h:407-675).  Requires a Tet4 mesh. */
+ * (include/SmoothedCellDataProcessor.h:407-675).  One smoothed cell per element of a Tet4
+ * mesh (ElementReproducingKernelTet4) or of a Hex8 mesh (ElementReproducingKernelHex8,
+ * include/ElementReproducingKernel.h:317-378; face-smoothed Hex8 B of QuadratureSmoothed.h:96-150). */
 int aphip_rk_setup(aphip_ctx *ctx, double kernel_radius_scale);
 /* aphip_rk_setup with the reference's force-method switch
  * (IntegrationSchemeStrainSmoothingParameters, include/InternalForceContributionParameters.h:

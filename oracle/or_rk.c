@@ -218,11 +218,11 @@ int or_rk_function_values(int64_t n_nodes, const double *X, const double *radius
     return 0;
 }
 
-/* First-seen union of the neighbors of the (4) element nodes; returns the list length. */
-static int cell_union(const int32_t *en, int max_neighbors, const int32_t *num_neighbors, const int32_t *neighbors,
-                      int32_t *out) {
+/* First-seen union of the neighbors of the npe element nodes; returns the list length. */
+static int cell_union(const int32_t *en, int npe, int max_neighbors, const int32_t *num_neighbors,
+                      const int32_t *neighbors, int32_t *out) {
     int len = 0;
-    for (int k = 0; k < 4; ++k) {
+    for (int k = 0; k < npe; ++k) {
         const int32_t a = en[k];
         for (int l = 0; l < num_neighbors[a]; ++l) {
             const int32_t j = neighbors[(int64_t)a * max_neighbors + l];
@@ -235,31 +235,36 @@ static int cell_union(const int32_t *en, int max_neighbors, const int32_t *num_n
     return len;
 }
 
-int64_t or_rk_cells_count(int64_t n_elem, const int32_t *conn, int max_neighbors, const int32_t *num_neighbors,
-                          const int32_t *neighbors, int64_t *cell_start) {
-    int32_t buf[4 * 64];
+int64_t or_rk_cells_count(int64_t n_elem, const int32_t *conn, int npe, int max_neighbors,
+                          const int32_t *num_neighbors, const int32_t *neighbors, int64_t *cell_start) {
+    int32_t buf[8 * 64];
+    if (npe * max_neighbors > 8 * 64) return -1;
     cell_start[0] = 0;
     for (int64_t e = 0; e < n_elem; ++e)
-        cell_start[e + 1] = cell_start[e] + cell_union(&conn[4 * e], max_neighbors, num_neighbors, neighbors, buf);
+        cell_start[e + 1] = cell_start[e] + cell_union(&conn[npe * e], npe, max_neighbors, num_neighbors, neighbors, buf);
     return cell_start[n_elem];
 }
 
-int or_rk_cells_build(int64_t n_nodes, const double *X, int64_t n_elem, const int32_t *conn, int max_neighbors,
-                      const int32_t *num_neighbors, const int32_t *neighbors, const double *phi,
+int or_rk_cells_build(int64_t n_nodes, const double *X, int64_t n_elem, const int32_t *conn, int npe,
+                      int max_neighbors, const int32_t *num_neighbors, const int32_t *neighbors, const double *phi,
                       const int64_t *cell_start, int32_t *cell_nodes, double *cell_dphi, double *cell_volume) {
     (void)n_nodes;
+    if (npe != 4 && npe != 8) return -1;
     for (int64_t e = 0; e < n_elem; ++e) {
         int32_t *list = &cell_nodes[cell_start[e]];
         double *dphi = &cell_dphi[3 * cell_start[e]];
-        const int len = cell_union(&conn[4 * e], max_neighbors, num_neighbors, neighbors, list);
+        const int len = cell_union(&conn[npe * e], npe, max_neighbors, num_neighbors, neighbors, list);
         if (len != cell_start[e + 1] - cell_start[e]) return -1;
-        double Xe[12], B[12], V;
-        for (int k = 0; k < 4; ++k)
-            for (int d = 0; d < 3; ++d) Xe[3 * k + d] = X[3 * (int64_t)conn[4 * e + k] + d];
-        or_smoothed_tet4_bmatrix(Xe, B, &V);
+        double Xe[24], B[24], V;
+        for (int k = 0; k < npe; ++k)
+            for (int d = 0; d < 3; ++d) Xe[3 * k + d] = X[3 * (int64_t)conn[npe * e + k] + d];
+        /* the element's smoothed B and volume (SmoothedCellDataProcessor.h:608 with
+         * QuadratureSmoothed.h:32-62 for Tet4, :96-150 for Hex8) */
+        if (npe == 4) or_smoothed_tet4_bmatrix(Xe, B, &V);
+        else or_smoothed_hex8_bmatrix(Xe, B, &V);
         for (int t = 0; t < 3 * len; ++t) dphi[t] = 0.0;
-        for (int k = 0; k < 4; ++k) {
-            const int32_t a = conn[4 * e + k];
+        for (int k = 0; k < npe; ++k) {
+            const int32_t a = conn[npe * e + k];
             for (int l = 0; l < num_neighbors[a]; ++l) {
                 const int32_t j = neighbors[(int64_t)a * max_neighbors + l];
                 const double fv = phi[(int64_t)a * max_neighbors + l];
@@ -320,14 +325,14 @@ int or_cell_internal_force(int64_t n_nodes, int64_t n_cells, const int64_t *cell
     return 0;
 }
 
-int or_rk_mass(int64_t n_nodes, int64_t n_elem, const int32_t *conn, const double *elem_volume, double density,
-               int max_neighbors, const int32_t *num_neighbors, const int32_t *neighbors, const double *phi,
+int or_rk_mass(int64_t n_nodes, int64_t n_elem, const int32_t *conn, int npe, const double *elem_volume,
+               double density, int max_neighbors, const int32_t *num_neighbors, const int32_t *neighbors, const double *phi,
                double *mass) {
     double *mfe = (double *)calloc((size_t)n_nodes, sizeof(double));
     if (!mfe) return -1;
     for (int64_t e = 0; e < n_elem; ++e) {
         const double m = elem_volume[e] * density;
-        for (int k = 0; k < 4; ++k) mfe[conn[4 * e + k]] += m / 4;
+        for (int k = 0; k < npe; ++k) mfe[conn[npe * e + k]] += m / npe;
     }
     memset(mass, 0, sizeof(double) * n_nodes);
     for (int64_t i = 0; i < n_nodes; ++i) {
@@ -339,20 +344,22 @@ int or_rk_mass(int64_t n_nodes, int64_t n_elem, const int32_t *conn, const doubl
     return 0;
 }
 
-int or_rk_cells_two_pass(int64_t n_elem, const double *X, const int32_t *conn, int64_t *cell_start,
+int or_rk_cells_two_pass(int64_t n_elem, const double *X, const int32_t *conn, int npe, int64_t *cell_start,
                          int32_t *cell_nodes, double *cell_dphi, double *cell_volume) {
+    if (npe != 4 && npe != 8) return -1;
     cell_start[0] = 0;
     for (int64_t e = 0; e < n_elem; ++e) {
-        double Xe[12], B[12], V;
-        for (int k = 0; k < 4; ++k)
-            for (int d = 0; d < 3; ++d) Xe[3 * k + d] = X[3 * (int64_t)conn[4 * e + k] + d];
-        or_smoothed_tet4_bmatrix(Xe, B, &V);
-        for (int k = 0; k < 4; ++k) {
-            cell_nodes[4 * e + k] = conn[4 * e + k];
-            for (int m = 0; m < 3; ++m) cell_dphi[12 * e + 3 * k + m] = (B[3 * k + m] * V) / V;
+        double Xe[24], B[24], V;
+        for (int k = 0; k < npe; ++k)
+            for (int d = 0; d < 3; ++d) Xe[3 * k + d] = X[3 * (int64_t)conn[npe * e + k] + d];
+        if (npe == 4) or_smoothed_tet4_bmatrix(Xe, B, &V);
+        else or_smoothed_hex8_bmatrix(Xe, B, &V);
+        for (int k = 0; k < npe; ++k) {
+            cell_nodes[npe * e + k] = conn[npe * e + k];
+            for (int m = 0; m < 3; ++m) cell_dphi[3 * (npe * e + k) + m] = (B[3 * k + m] * V) / V;
         }
         cell_volume[e] = V;
-        cell_start[e + 1] = 4 * (e + 1);
+        cell_start[e + 1] = npe * (e + 1);
     }
     return 0;
 }

@@ -159,20 +159,21 @@ int or_neighbor_search(int64_t n_nodes, const double *X, const double *radius, i
                        int brute_force, int32_t *num_neighbors, int32_t *neighbors, int64_t *n_truncated);
 int or_rk_function_values(int64_t n_nodes, const double *X, const double *radius, int max_neighbors,
                           const int32_t *num_neighbors, const int32_t *neighbors, double *phi);
-int64_t or_rk_cells_count(int64_t n_elem, const int32_t *conn, int max_neighbors,
+/* npe = 4 (Tet4 background) or 8 (Hex8 background): one cell per element */
+int64_t or_rk_cells_count(int64_t n_elem, const int32_t *conn, int npe, int max_neighbors,
                           const int32_t *num_neighbors, const int32_t *neighbors, int64_t *cell_start);
-int or_rk_cells_build(int64_t n_nodes, const double *X, int64_t n_elem, const int32_t *conn, int max_neighbors,
+int or_rk_cells_build(int64_t n_nodes, const double *X, int64_t n_elem, const int32_t *conn, int npe, int max_neighbors,
                       const int32_t *num_neighbors, const int32_t *neighbors, const double *phi,
                       const int64_t *cell_start, int32_t *cell_nodes, double *cell_dphi, double *cell_volume);
 int or_cell_internal_force(int64_t n_nodes, int64_t n_cells, const int64_t *cell_start, const int32_t *cell_nodes,
                            const double *cell_dphi, const double *cell_volume, const or_material *mat,
                            const double *u, const double *state_n, double *state_np1,
                            double *H, double *P, double *f, int nthreads);
-int or_rk_mass(int64_t n_nodes, int64_t n_elem, const int32_t *conn, const double *elem_volume, double density,
+int or_rk_mass(int64_t n_nodes, int64_t n_elem, const int32_t *conn, int npe, const double *elem_volume, double density,
                int max_neighbors, const int32_t *num_neighbors, const int32_t *neighbors, const double *phi,
                double *mass);
-/* Two-pass method.  Cells of the element's own nodes: start[e] = 4e, grad N_k = (B_k V) / V. */
-int or_rk_cells_two_pass(int64_t n_elem, const double *X, const int32_t *conn, int64_t *cell_start,
+/* Two-pass method.  Cells of the element's own nodes: start[e] = npe e, grad N_k = (B_k V) / V. */
+int or_rk_cells_two_pass(int64_t n_elem, const double *X, const int32_t *conn, int npe, int64_t *cell_start,
                          int32_t *cell_nodes, double *cell_dphi, double *cell_volume);
 /* dst_i = sum_k phi_ik src_(nbr_ik), k descending; src_i for a node without neighbors (3 components). */
 int or_rk_gather(int64_t n_nodes, int max_neighbors, const int32_t *num_neighbors, const int32_t *neighbors,

@@ -408,13 +408,15 @@ def rk_function_values(X, radius, num, nbr):
 
 def rk_cells(X, conn, num, nbr, phi):
     X, conn, num, nbr, phi = _f64(X), _i32a(conn), _i32a(num), _i32a(nbr), _f64(phi)
-    ne, mx = conn.shape[0], nbr.shape[1]
+    ne, npe, mx = conn.shape[0], conn.shape[1], nbr.shape[1]
     start = np.zeros(ne + 1, dtype=np.int64)
-    nnz = lib().or_rk_cells_count(ne, _p(conn, _i32), mx, _p(num, _i32), _p(nbr, _i32), _p(start, _i64))
+    nnz = lib().or_rk_cells_count(ne, _p(conn, _i32), npe, mx, _p(num, _i32), _p(nbr, _i32), _p(start, _i64))
+    if nnz < 0:
+        raise RuntimeError("or_rk_cells_count failed")
     nodes = np.zeros(nnz, dtype=np.int32)
     dphi = np.zeros((nnz, 3))
     vol = np.zeros(ne)
-    rc = lib().or_rk_cells_build(X.shape[0], _p(X), ne, _p(conn, _i32), mx, _p(num, _i32), _p(nbr, _i32), _p(phi),
+    rc = lib().or_rk_cells_build(X.shape[0], _p(X), ne, _p(conn, _i32), npe, mx, _p(num, _i32), _p(nbr, _i32), _p(phi),
                                  _p(start, _i64), _p(nodes, _i32), _p(dphi), _p(vol))
     if rc:
         raise RuntimeError("or_rk_cells_build failed")
@@ -443,12 +445,15 @@ def cell_internal_force(n_nodes, cells, kind, lam, mu, u, state_n=None, yield_st
 def rk_cells_two_pass(X, conn):
     """Cells of the two-pass method: each element's own nodes and (B_k V)/V."""
     X, conn = _f64(X), _i32a(conn)
-    ne = conn.shape[0]
+    ne, npe = conn.shape
     start = np.zeros(ne + 1, dtype=np.int64)
-    nodes = np.zeros(4 * ne, dtype=np.int32)
-    dphi = np.zeros((4 * ne, 3))
+    nodes = np.zeros(npe * ne, dtype=np.int32)
+    dphi = np.zeros((npe * ne, 3))
     vol = np.zeros(ne)
-    lib().or_rk_cells_two_pass(ne, _p(X), _p(conn, _i32), _p(start, _i64), _p(nodes, _i32), _p(dphi), _p(vol))
+    rc = lib().or_rk_cells_two_pass(ne, _p(X), _p(conn, _i32), npe, _p(start, _i64), _p(nodes, _i32), _p(dphi),
+                                    _p(vol))
+    if rc:
+        raise RuntimeError("or_rk_cells_two_pass failed")
     return start, nodes, dphi, vol
 
 
@@ -481,6 +486,6 @@ def rk_mass(conn, elem_volume, density, num, nbr, phi):
     conn, num, nbr, phi = _i32a(conn), _i32a(num), _i32a(nbr), _f64(phi)
     n, mx = nbr.shape
     m = np.zeros(n)
-    lib().or_rk_mass(n, conn.shape[0], _p(conn, _i32), _p(_f64(elem_volume)), ctypes.c_double(density), mx,
+    lib().or_rk_mass(n, conn.shape[0], _p(conn, _i32), conn.shape[1], _p(_f64(elem_volume)), ctypes.c_double(density), mx,
                      _p(num, _i32), _p(nbr, _i32), _p(phi), _p(m))
     return m

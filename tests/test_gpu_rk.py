@@ -250,3 +250,67 @@ def test_two_pass_graph_replay_and_power_method(gpu_available):
             s.compute_mass()
             res.append(s.power_method_time_increment(num_iterations=50))
     assert res[0][1] == res[1][1] and abs(res[0][0] - res[1][0]) < 1e-9 * res[0][0]
+
+
+# ---- Hex8 background mesh (ElementReproducingKernelHex8) ------------------------
+
+def _gpu_hex(X, hc, scale, one_pass):
+    s = A.Solver()
+    s.set_mesh(A.HEX8, X, hc)
+    s.set_material(A.J2, J2["rho"], J2["E"], J2["nu"], J2["yield_stress"], J2["hardening"])
+    s.rk_setup(scale, one_pass=one_pass)
+    return s
+
+
+@pytest.mark.parametrize("one_pass", [True, False], ids=["one_pass", "two_pass"])
+def test_hex8_background_setup_force_and_100_steps(gpu_available, one_pass):
+    """One smoothed cell per hex (element_smoothing_cell on a hex mesh): neighbors
+    bit-exact, RK functions, cell CSR, mass and the J2 force of a random field at
+    1e-12, then 100 yielding steps at 1e-10, one- and two-pass."""
+    n = 6
+    X, hc = M.hex_lattice(n, n, n)
+    X = M.jitter(X, 1.0 / n, 0.1, 13, 1)
+    scale = 1.5
+    r, num, nbr, tr, phi = _oracle_rk(X, hc, scale)
+    cells = O.rk_cells(X, hc, num, nbr, phi) if one_pass else O.rk_cells_two_pass(X, hc)
+    mass = O.rk_mass(hc, cells[3], J2["rho"], num, nbr, phi)
+    lam, mu = O.lame(J2["E"], J2["nu"])
+    kw = dict(yield_stress=J2["yield_stress"], hardening=J2["hardening"])
+    u = 4e-3 * (2 * M.uniform01(8, 7, X.size).reshape(X.shape) - 1)
+    if one_pass:
+        f_ref, H_ref, P_ref, s_ref = O.cell_internal_force(len(X), cells, O.J2, lam, mu, u, **kw)
+    else:
+        f_ref, H_ref, P_ref, s_ref, _, _ = O.rk_two_pass_force(len(X), cells, num, nbr, phi, O.J2, lam, mu, u, **kw)
+    print(f"rk hex8 background one_pass={one_pass} nodes={len(X)} cells={len(hc)} trunc={tr}", flush=True)
+    with _gpu_hex(X, hc, scale, one_pass) as s:
+        gnum, gnbr, gtr = s.rk_neighbors()
+        assert np.array_equal(gnum, num) and gtr == tr
+        assert np.array_equal(gnbr, np.where(np.arange(40)[None, :] < num[:, None], nbr, -1))
+        assert np.abs(s.rk_function_values() - phi).max() < 1e-12
+        start, nodes, dphi, vol = s.rk_cells()
+        assert np.array_equal(start, cells[0]) and np.array_equal(nodes, cells[1])
+        assert rel_err(dphi, cells[2]) < 1e-12 and rel_err(vol, cells[3]) < 1e-14
+        s.compute_mass()
+        assert rel_err(s.get(A.FIELD_MASS), mass) < 1e-13
+        s.set_field(A.FIELD_U, u)
+        s.internal_force()
+        assert s_ref.max() > 0
+        for fld, rr in ((A.FIELD_F, f_ref), (A.FIELD_H, H_ref), (A.FIELD_P, P_ref), (A.FIELD_STATE, s_ref)):
+            assert rel_err(s.get(fld), rr) < 1e-12, fld
+    c = X - 0.5
+    v0 = 60.0 * np.stack([-0.5 * c[:, 0], -0.5 * c[:, 1], c[:, 2]], axis=1)
+    dt = 0.5 * M.stable_dt_estimate(X, hc, J2["E"], J2["nu"], J2["rho"])
+    ref = O.Solver(-1, X, None, O.J2, lam, mu, mass, None, J2["yield_stress"], J2["hardening"], cells=cells,
+                   two_pass=None if one_pass else (num, nbr, phi))
+    ref.v[:] = v0
+    ref.run(dt, 100)
+    assert ref.state.max() > 0.0
+    with _gpu_hex(X, hc, scale, one_pass) as s:
+        s.compute_mass()
+        s.set_field(A.FIELD_V, v0)
+        s.initialize()
+        s.step(dt, 100)
+        for fld, rr in ((A.FIELD_U, ref.u), (A.FIELD_V, ref.v), (A.FIELD_A, ref.a), (A.FIELD_F, ref.f),
+                        (A.FIELD_H, ref.H), (A.FIELD_P, ref.P)):
+            assert rel_err(s.get(fld), rr) < 1e-10, fld
+        assert rel_err(s.get(A.FIELD_STATE), ref.state) < 1e-10

@@ -253,3 +253,60 @@ def test_two_pass_solver_matches_one_pass_over_steps():
     assert a.state.max() > 0
     for x, y in ((b.u, a.u), (b.v, a.v), (b.f, a.f), (b.P, a.P)):
         assert np.abs(x - y).max() < 1e-10 * np.abs(y).max()
+
+
+# ---- Hex8 background (ElementReproducingKernelHex8: one cell per hex) ----------
+
+def _rk_hex(n=4, scale=1.5, seed=11):
+    X, hc = M.hex_lattice(n, n, n)
+    X = M.jitter(X, 1.0 / n, 0.1, seed, 1)
+    r = O.kernel_radius(O.max_edge_length(X, hc), scale)
+    num, nbr, tr = O.neighbor_search(X, r)
+    phi = O.rk_function_values(X, r, num, nbr)
+    return X, hc, r, num, nbr, phi
+
+
+def test_hex8_background_cells_linear_patch_volume_and_mass():
+    # the face-smoothed Hex8 cell (QuadratureSmoothed.h:96-150) with RK functions:
+    # derivatives sum to zero, a linear field gives its exact gradient, cell volumes
+    # are the elements' smoothed volumes (which tile the unit cube when no node is
+    # jittered), cell lists are first-seen unions over the eight nodes, and the RK mass
+    # (rho V / 8 per element node, scattered by phi) sums to rho V
+    X0, hc0 = M.hex_lattice(4, 4, 4)
+    assert sum(O.smoothed_hex8_bmatrix(X0[h])[1] for h in hc0) == pytest.approx(1.0, rel=1e-13)
+    X, hc, r, num, nbr, phi = _rk_hex()
+    start, nodes, dphi, vol = O.rk_cells(X, hc, num, nbr, phi)
+    assert np.allclose(vol, [O.smoothed_hex8_bmatrix(X[h])[1] for h in hc], rtol=1e-14, atol=0)
+    for c in range(0, len(vol), 5):
+        assert np.allclose(dphi[start[c]:start[c + 1]].sum(0), 0.0, atol=1e-10)
+        seen = []
+        for a in hc[c]:
+            for j in nbr[a, :num[a]]:
+                if j not in seen:
+                    seen.append(j)
+        assert nodes[start[c]:start[c + 1]].tolist() == seen
+    Hu = np.array([[0.01, 0.002, 0.0], [0.0, -0.004, 0.003], [0.001, 0.0, 0.02]])
+    u = X @ Hu.T + np.array([0.1, -0.2, 0.05])
+    lam, mu = O.lame(70e9, 0.33)
+    f, H, P, _ = O.cell_internal_force(len(X), (start, nodes, dphi, vol), O.LINEAR_ELASTIC, lam, mu, u)
+    assert np.allclose(H, Hu.ravel(), atol=1e-12)
+    m = O.rk_mass(hc, vol, 2700.0, num, nbr, phi)
+    assert m.sum() == pytest.approx(2700.0 * vol.sum(), rel=1e-11)
+
+
+def test_hex8_background_small_support_is_the_smoothed_element():
+    # with supports smaller than any node spacing each node is its own only neighbor
+    # (phi = 1), so a one-pass cell lists its hex's nodes and their derivatives are the
+    # smoothed Hex8 B, the same as the two-pass cell
+    X, hc, r, num, nbr, phi = _rk_hex(scale=0.3)
+    assert (num == 1).all() and np.array_equal(nbr[:, 0], np.arange(len(X)))
+    assert np.allclose(phi[:, 0], 1.0, rtol=0, atol=1e-15)
+    start, nodes, dphi, vol = O.rk_cells(X, hc, num, nbr, phi)
+    s2, n2, d2, v2 = O.rk_cells_two_pass(X, hc)
+    assert np.array_equal(start, 8 * np.arange(len(hc) + 1)) and np.array_equal(nodes, hc.ravel())
+    assert np.array_equal(s2, start) and np.array_equal(n2, nodes)
+    for e in range(len(hc)):
+        B, V = O.smoothed_hex8_bmatrix(X[hc[e]])
+        assert vol[e] == pytest.approx(V, rel=1e-15) and v2[e] == pytest.approx(V, rel=1e-15)
+        assert np.allclose(dphi[8 * e:8 * e + 8], B, rtol=0, atol=1e-13 * np.abs(B).max())
+        assert np.allclose(d2[8 * e:8 * e + 8], B, rtol=0, atol=1e-13 * np.abs(B).max())
