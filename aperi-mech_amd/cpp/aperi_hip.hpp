@@ -391,6 +391,21 @@ class ElementReproducingKernelHip : public ElementBase {
     }
 };
 
+// ElementSmoothedTetrahedron4: finite-element strain smoothing, one cell per Tet4
+// (include/ElementSmoothedTetrahedron4.h; created by include/Element.h:44-47).
+class ElementSmoothedTetrahedron4Hip : public ElementBase {
+   public:
+    ElementSmoothedTetrahedron4Hip(const MeshBlock &mesh, std::shared_ptr<Device> device,
+                                   std::shared_ptr<Material> material)
+        : ElementBase(4, std::move(device), std::move(material)) {
+        BindMesh(APHIP_TET4, mesh);
+        Check(aphip_strain_smoothing_setup(Context()));
+    }
+    void ComputeInternalForceAllElements(double time_increment, double time) override {
+        Check(aphip_internal_force(Context(), time, time_increment));
+    }
+};
+
 // include/Element.h:32-86: the supported (topology, approximation, integration)
 // combinations and the reference's messages for the unsupported ones.
 inline std::shared_ptr<ElementBase> CreateElement(ElementTopology topology, ApproximationSpaceType space,
@@ -408,7 +423,7 @@ inline std::shared_ptr<ElementBase> CreateElement(ElementTopology topology, Appr
     if (scheme == IntegrationSchemeType::StrainSmoothing) {
         if (topology == ElementTopology::Hexahedron8)
             throw std::runtime_error("Strain Smoothing is not supported for finite elements and Hexahedron8");
-        throw std::runtime_error("Finite-element strain smoothing on Tetrahedron4 is not supported by the MI355X force step");
+        return std::make_shared<ElementSmoothedTetrahedron4Hip>(mesh, std::move(device), std::move(material));
     }
     return std::make_shared<ElementGaussianHip>(topology, mesh, std::move(device), std::move(material), formulation);
 }

@@ -1188,6 +1188,21 @@ int aphip_rk_setup_method(aphip_ctx *c, double scale, int32_t use_one_pass_metho
     APHIP_API_END
 }
 
+int aphip_strain_smoothing_setup(aphip_ctx *c) {
+    APHIP_API_BEGIN
+    APHIP_REQUIRE(c != nullptr, "ctx is NULL");
+    require_ready(*c, false);
+    APHIP_REQUIRE(c->etype != APHIP_HEX8, "Strain Smoothing is not supported for finite elements and Hexahedron8");
+    APHIP_REQUIRE(c->mat.state_per_qp == 0, "smoothed cells keep one material state per cell, not per Gauss point");
+    APHIP_HIP(hipSetDevice(c->device));
+    APHIP_HIP(hipStreamSynchronize(c->stream));
+    c->graph_reset();
+    rk_setup(*c, 0.0, false, true);
+    c->has_mass = false;
+    APHIP_HIP(hipStreamSynchronize(c->stream));
+    APHIP_API_END
+}
+
 int aphip_rk_get_neighbors(aphip_ctx *c, int32_t *num, int32_t *nbr, int64_t *n_truncated) {
     APHIP_API_BEGIN
     APHIP_REQUIRE(c != nullptr, "ctx is NULL");

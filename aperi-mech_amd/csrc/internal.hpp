@@ -331,7 +331,7 @@ void force_vjp(Ctx &c, const double *f_bar, double *u_bar, double *E_bar, double
 void step_once_tiles(Ctx &c, double dt);
 
 // ---- RK (rk.hip) ----
-void rk_setup(Ctx &c, double scale, bool two_pass);
+void rk_setup(Ctx &c, double scale, bool two_pass, bool fe_cells = false);
 void rk_cell_force(Ctx &c);
 void rk_node_update(Ctx &c, double dt, int mode);
 void rk_mass(Ctx &c, double *total);

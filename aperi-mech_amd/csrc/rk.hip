@@ -551,6 +551,14 @@ __global__ __launch_bounds__(256) void k_cell_two_pass(int64_t ne, const int32_t
     volume[e] = V;
 }
 
+__global__ __launch_bounds__(256) void k_identity_functions(int64_t nn, int32_t *num, int32_t *nbr, double *phi) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= nn) return;
+    num[i] = 1;
+    nbr[i * kMaxNbr] = (int32_t)i;
+    phi[i * kMaxNbr] = 1.0;
+}
+
 __global__ __launch_bounds__(256) void k_iota(int64_t n, int32_t *v) {
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (i < n) v[i] = (int32_t)i;
@@ -1041,16 +1049,22 @@ void scan_counts(Ctx &c, const DevBuf<int32_t> &counts, int64_t n, DevBuf<T> &st
 
 }  // namespace
 
-void rk_setup(Ctx &c, double scale, bool two_pass) {
+void fe_smoothing_setup(Ctx &c);
+void rk_cells_and_order(Ctx &c, bool two_pass, bool elem_cells);
+
+void rk_setup(Ctx &c, double scale, bool two_pass, bool fe_cells) {
     CellData &D = c.cells;
     D.reset();
-    const int64_t nn = c.nn, ne = c.ne;
+    const int64_t nn = c.nn;
+    if (fe_cells) {
+        fe_smoothing_setup(c);
+        return;
+    }
     if (scale == 1.0) scale += 1.0e-6;  // src/NeighborSearchProcessor.cpp:161-163
 
     // kernel radius from the max edge length
     D.radius.alloc(nn);
     const bool hex = c.etype == APHIP_HEX8;  // Hex8 background: ElementReproducingKernelHex8
-    const int npe = hex ? 8 : 4;
     hipLaunchKernelGGL(hex ? k_max_edge<8> : k_max_edge<4>, grid1(nn, 256), dim3(256), 0, c.stream, nn, c.node_start.p, c.node_inc.p, c.conn.p,
                        c.X.p, scale, D.radius.p);
     APHIP_HIP(hipGetLastError());
@@ -1126,9 +1140,39 @@ void rk_setup(Ctx &c, double scale, bool two_pass) {
                        D.phi.p);
     APHIP_HIP(hipGetLastError());
 
+    rk_cells_and_order(c, two_pass, two_pass);
+}
+
+// Finite-element strain smoothing with element cells (ElementSmoothedTetrahedron4,
+// include/ElementSmoothedTetrahedron4.h): the element's own shape functions, i.e. every
+// node its own only "neighbor" with value 1, and the cells of the element's nodes with
+// derivative (B_k V) / V (SmoothedCellDataProcessor.h:623-668 with unit function values).
+// The force runs on the one-pass kernels; the mass scatter is the identity.
+void fe_smoothing_setup(Ctx &c) {
+    CellData &D = c.cells;
+    const int64_t nn = c.nn;
+    D.radius.alloc(nn);
+    D.radius.zero(c.stream);
+    D.num_nbr.alloc(nn);
+    D.nbr.alloc(nn * kMaxNbr);
+    D.phi.alloc(nn * kMaxNbr);
+    hipLaunchKernelGGL(k_identity_functions, grid1(nn, 256), dim3(256), 0, c.stream, nn, D.num_nbr.p, D.nbr.p,
+                       D.phi.p);
+    APHIP_HIP(hipGetLastError());
+    D.n_truncated = 0;
+    rk_cells_and_order(c, false, true);
+}
+
+// Cells (element-node cells when elem_cells), reverse neighbor lists and the node
+// order of the cell entries; two_pass selects the force method.
+void rk_cells_and_order(Ctx &c, bool two_pass, bool elem_cells) {
+    CellData &D = c.cells;
+    const int64_t nn = c.nn, ne = c.ne;
+    const bool hex = c.etype == APHIP_HEX8;
+    const int npe = hex ? 8 : 4;
     // smoothed cells: counts, offsets, lists and derivatives
     D.two_pass = two_pass;
-    if (two_pass) {
+    if (elem_cells) {
         // the element's own nodes (SmoothedCellDataProcessor.h:474-486, :637-648)
         D.nnz = npe * ne;
         APHIP_REQUIRE(D.nnz < (int64_t)INT32_MAX, "too many cell entries for int32 incidences");
@@ -1144,10 +1188,12 @@ void rk_setup(Ctx &c, double scale, bool two_pass) {
                                D.nodes.p, D.dphi.p, D.volume.p, D.inc_cell.p);
             APHIP_HIP(hipGetLastError());
         }
-        D.u_phys.alloc(3 * nn);
-        D.u_phys.zero(c.stream);
-        D.f_local.alloc(3 * nn);
-        D.f_local.zero(c.stream);
+        if (two_pass) {
+            D.u_phys.alloc(3 * nn);
+            D.u_phys.zero(c.stream);
+            D.f_local.alloc(3 * nn);
+            D.f_local.zero(c.stream);
+        }
     } else {
         DevBuf<int32_t> clen;
         clen.alloc(std::max<int64_t>(ne, 1));

@@ -42,7 +42,7 @@ EXPORTED = [
     "aphip_set_field", "aphip_get_field", "aphip_internal_force", "aphip_initialize", "aphip_step",
     "aphip_synchronize", "aphip_set_tiling", "aphip_set_graph", "aphip_autotune", "aphip_batch_begin",
     "aphip_batch_add", "aphip_batch_end", "aphip_batch_ranges", "aphip_internal_force_vjp",
-    "aphip_adjoint_forward", "aphip_adjoint_backward", "aphip_rk_setup", "aphip_rk_setup_method", "aphip_rk_get_neighbors", "aphip_rk_get_function_values",
+    "aphip_adjoint_forward", "aphip_adjoint_backward", "aphip_rk_setup", "aphip_rk_setup_method", "aphip_strain_smoothing_setup", "aphip_rk_get_neighbors", "aphip_rk_get_function_values",
     "aphip_rk_get_cells", "aphip_set_profiling", "aphip_get_profile", "aphip_reset_profile",
     "aphip_bandwidth_probe", "aphip_power_method_time_increment",
     "aphip_results_open", "aphip_results_write", "aphip_results_close",
@@ -351,6 +351,11 @@ class Solver:
         return a.value, b.value, c.value, d.value
 
     # ---- RKPM ----
+    def strain_smoothing_setup(self):
+        """Finite-element strain smoothing, one cell per Tet4 (ElementSmoothedTetrahedron4)."""
+        _check(self.L.aphip_strain_smoothing_setup(self.ctx))
+        return self
+
     def rk_setup(self, kernel_radius_scale, one_pass=True):
         """RK functions and smoothed cells; one_pass=False selects the two-pass force
         (physical displacement gathered before, force scattered after the cells)."""

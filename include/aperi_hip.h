@@ -291,6 +291,15 @@ int aphip_rk_setup(aphip_ctx *ctx, double kernel_radius_scale);
  * scatters it back, f = Phi^T f_local (ScatterValues, :130-181).  Both methods give
  * the same force up to rounding. */
 int aphip_rk_setup_method(aphip_ctx *ctx, double kernel_radius_scale, int32_t use_one_pass_method);
+/* Finite-element strain smoothing with one cell per Tet4 (ElementSmoothedTetrahedron4,
+ * include/ElementSmoothedTetrahedron4.h, created by include/Element.h:44-47): each cell's
+ * derivative is the face-smoothed B of its own four nodes (QuadratureSmoothed.h:32-62,
+ * SmoothedCellDataProcessor.h:623-668 with the element's shape functions), the force
+ * f_k -= V P B_k, the lumped mass rho V / 4.  Replaces the element's construction and
+ * its ComputeInternalForceAllElements; a Hex8 mesh gets the reference's error
+ * (include/Element.h:66-69).  The functions read back through aphip_rk_get_* are then the
+ * identity (one neighbor, the node itself, value 1). */
+int aphip_strain_smoothing_setup(aphip_ctx *ctx);
 /* Neighbor lists in caller numbering: num[n_nodes], nbr[n_nodes x 40] (-1 padded),
  * the 40-cap truncation count (NeighborSearchProcessor.h:307-316). */
 int aphip_rk_get_neighbors(aphip_ctx *ctx, int32_t *num, int32_t *nbr, int64_t *n_truncated);

@@ -314,3 +314,61 @@ def test_hex8_background_setup_force_and_100_steps(gpu_available, one_pass):
                         (A.FIELD_H, ref.H), (A.FIELD_P, ref.P)):
             assert rel_err(s.get(fld), rr) < 1e-10, fld
         assert rel_err(s.get(A.FIELD_STATE), ref.state) < 1e-10
+
+
+# ---- finite-element strain smoothing (ElementSmoothedTetrahedron4) ---------------
+
+@pytest.mark.parametrize("kind,p", [(A.J2, J2), (A.NEO_HOOKEAN, NH)])
+def test_fe_strain_smoothing_tet4(gpu_available, kind, p):
+    """One smoothed cell per Tet4 with the element's own shape functions: the cells are
+    the element-node cells, the mass the lumped rho V / 4, the force at a random field
+    the oracle's cell force (1e-12); over 100 steps it follows the oracle (1e-10) and,
+    the smoothed B of a linear tet being its exact gradient, the Gauss Tet4 step (1e-9)."""
+    X, t = _mesh(6, seed=5)
+    cells = O.rk_cells_two_pass(X, t)
+    mass = O.lumped_mass(A.TET4, X, t, p["rho"])
+    lam, mu = O.lame(p["E"], p["nu"])
+    kw = dict(yield_stress=p.get("yield_stress", 0.0), hardening=p.get("hardening", 0.0))
+    u = 4e-3 * (2 * M.uniform01(8, 7, X.size).reshape(X.shape) - 1)
+    f_ref, H_ref, P_ref, s_ref = O.cell_internal_force(len(X), cells, kind, lam, mu, u, **kw)
+    print(f"fe strain smoothing kind={kind} nodes={len(X)} cells={len(t)}", flush=True)
+
+    def make():
+        s = A.Solver()
+        s.set_mesh(A.TET4, X, t)
+        s.set_material(kind, p["rho"], p["E"], p["nu"], kw["yield_stress"], kw["hardening"])
+        s.strain_smoothing_setup()
+        return s
+
+    with make() as s:
+        num, nbr, tr = s.rk_neighbors()
+        assert (num == 1).all() and np.array_equal(nbr[:, 0], np.arange(len(X))) and tr == 0
+        start, nodes, dphi, vol = s.rk_cells()
+        assert np.array_equal(start, cells[0]) and np.array_equal(nodes, cells[1])
+        assert rel_err(dphi, cells[2]) < 1e-12 and rel_err(vol, cells[3]) < 1e-14
+        s.compute_mass()
+        assert rel_err(s.get(A.FIELD_MASS), mass) < 1e-13
+        s.set_field(A.FIELD_U, u)
+        s.internal_force()
+        for fld, rr in ((A.FIELD_F, f_ref), (A.FIELD_H, H_ref), (A.FIELD_P, P_ref)):
+            assert rel_err(s.get(fld), rr) < 1e-12, fld
+    c = X - 0.5
+    v0 = 60.0 * np.stack([-0.5 * c[:, 0], -0.5 * c[:, 1], c[:, 2]], axis=1)
+    dt = 0.5 * M.stable_dt_estimate(X, t, p["E"], p["nu"], p["rho"])
+    ref = O.Solver(-1, X, None, kind, lam, mu, mass, None, kw["yield_stress"], kw["hardening"], cells=cells)
+    ref.v[:] = v0
+    ref.run(dt, 100)
+    with make() as s, A.Solver() as g:
+        s.compute_mass()
+        s.set_field(A.FIELD_V, v0)
+        s.initialize()
+        s.step(dt, 100)
+        for fld, rr in ((A.FIELD_U, ref.u), (A.FIELD_V, ref.v), (A.FIELD_F, ref.f), (A.FIELD_P, ref.P)):
+            assert rel_err(s.get(fld), rr) < 1e-10, fld
+        g.set_mesh(A.TET4, X, t)
+        g.set_material(kind, p["rho"], p["E"], p["nu"], kw["yield_stress"], kw["hardening"])
+        g.compute_mass()
+        g.set_field(A.FIELD_V, v0)
+        g.initialize()
+        g.step(dt, 100)
+        assert rel_err(s.get(A.FIELD_U), g.get(A.FIELD_U)) < 1e-9
