@@ -1,5 +1,6 @@
-// RKPM on gfx950: one-pass reproducing-kernel functions with element-as-cell
-// strain smoothing over a Tet4 background mesh.
+// RKPM on gfx950: reproducing-kernel functions with element-as-cell strain smoothing
+// over a Tet4 or Hex8 background mesh (one- and two-pass force), and finite-element
+// strain smoothing on Tet4 (the same cells with the element's own shape functions).
 //
 // Set-up (once, Total Lagrangian):
 //   max edge length     src/MaxEdgeLengthProcessor.cpp:36-60
