@@ -417,12 +417,16 @@ inline std::shared_ptr<ElementBase> CreateElement(ElementTopology topology, Appr
     if (space == ApproximationSpaceType::ReproducingKernel) {
         if (scheme != IntegrationSchemeType::StrainSmoothing)
             throw std::runtime_error("Gauss Quadrature is not supported for Reproducing Kernel");
+        if (formulation != LagrangianFormulationType::Total)
+            throw std::runtime_error("the updated formulation applies to the Gaussian elements, not to RKPM cells");
         return std::make_shared<ElementReproducingKernelHip>(mesh, kernel_radius_scale_factor, std::move(device),
                                                              std::move(material), use_one_pass_method, topology);
     }
     if (scheme == IntegrationSchemeType::StrainSmoothing) {
         if (topology == ElementTopology::Hexahedron8)
             throw std::runtime_error("Strain Smoothing is not supported for finite elements and Hexahedron8");
+        if (formulation != LagrangianFormulationType::Total)
+            throw std::runtime_error("the updated formulation applies to the Gaussian elements, not to smoothed cells");
         return std::make_shared<ElementSmoothedTetrahedron4Hip>(mesh, std::move(device), std::move(material));
     }
     return std::make_shared<ElementGaussianHip>(topology, mesh, std::move(device), std::move(material), formulation);

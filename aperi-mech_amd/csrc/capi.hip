@@ -648,6 +648,8 @@ int aphip_set_material(aphip_ctx *c, int32_t kind, const aphip_material_params *
     APHIP_REQUIRE(c != nullptr, "ctx is NULL");
     MaterialConst m;
     resolve_material(kind, p, m);
+    APHIP_REQUIRE(c->formulation != APHIP_SEMI_LAGRANGIAN || !m.rate_form(),
+                  "the semi-Lagrangian formulation does not take the rate-form materials");
     APHIP_HIP(hipSetDevice(c->device));
     APHIP_HIP(hipStreamSynchronize(c->stream));
     c->graph_reset();
@@ -1178,6 +1180,8 @@ int aphip_rk_setup_method(aphip_ctx *c, double scale, int32_t use_one_pass_metho
     APHIP_REQUIRE(c->etype == APHIP_TET4 || c->etype == APHIP_HEX8,
                   "RKPM element-as-cell smoothing needs a Tet4 or Hex8 background mesh");
     APHIP_REQUIRE(c->mat.state_per_qp == 0, "RKPM cells keep one material state per cell, not per Gauss point");
+    APHIP_REQUIRE(c->formulation == APHIP_TOTAL_LAGRANGIAN,
+                  "the updated formulation applies to the Gaussian elements, not to RKPM cells");
     APHIP_REQUIRE(scale >= 0.0, "Kernel radius scale factor must be non-negative.");
     APHIP_HIP(hipSetDevice(c->device));
     APHIP_HIP(hipStreamSynchronize(c->stream));
@@ -1194,6 +1198,8 @@ int aphip_strain_smoothing_setup(aphip_ctx *c) {
     require_ready(*c, false);
     APHIP_REQUIRE(c->etype != APHIP_HEX8, "Strain Smoothing is not supported for finite elements and Hexahedron8");
     APHIP_REQUIRE(c->mat.state_per_qp == 0, "smoothed cells keep one material state per cell, not per Gauss point");
+    APHIP_REQUIRE(c->formulation == APHIP_TOTAL_LAGRANGIAN,
+                  "the updated formulation applies to the Gaussian elements, not to smoothed cells");
     APHIP_HIP(hipSetDevice(c->device));
     APHIP_HIP(hipStreamSynchronize(c->stream));
     c->graph_reset();

@@ -556,8 +556,11 @@ __global__ __launch_bounds__(256) void k_identity_functions(int64_t nn, int32_t 
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (i >= nn) return;
     num[i] = 1;
-    nbr[i * kMaxNbr] = (int32_t)i;
-    phi[i * kMaxNbr] = 1.0;
+    // the whole row: the node itself with value 1, the unused entries -1 / 0
+    for (int k = 0; k < kMaxNbr; ++k) {
+        nbr[i * kMaxNbr + k] = k == 0 ? (int32_t)i : -1;
+        phi[i * kMaxNbr + k] = k == 0 ? 1.0 : 0.0;
+    }
 }
 
 __global__ __launch_bounds__(256) void k_iota(int64_t n, int32_t *v) {

@@ -290,6 +290,27 @@ void CpuCases() {
                               IntegrationSchemeType::GaussQuadrature, empty, nullptr, mat);
             },
             "Gauss Quadrature is not supported for Reproducing Kernel"));
+        // the cell elements take the total formulation only (the C ABI refuses the others)
+        for (auto form : {LagrangianFormulationType::Updated, LagrangianFormulationType::Semi}) {
+            EXPECT_TRUE(Throws(
+                [&] {
+                    CreateElement(ElementTopology::Tetrahedron4, ApproximationSpaceType::ReproducingKernel,
+                                  IntegrationSchemeType::StrainSmoothing, empty, nullptr, mat, 1.5, true, form);
+                },
+                "not to RKPM cells"));
+            EXPECT_TRUE(Throws(
+                [&] {
+                    CreateElement(ElementTopology::Hexahedron8, ApproximationSpaceType::ReproducingKernel,
+                                  IntegrationSchemeType::StrainSmoothing, empty, nullptr, mat, 1.5, false, form);
+                },
+                "not to RKPM cells"));
+            EXPECT_TRUE(Throws(
+                [&] {
+                    CreateElement(ElementTopology::Tetrahedron4, ApproximationSpaceType::FiniteElement,
+                                  IntegrationSchemeType::StrainSmoothing, empty, nullptr, mat, 0.0, true, form);
+                },
+                "not to smoothed cells"));
+        }
     });
 }
 
@@ -588,6 +609,96 @@ void GpuCases() {
     });
 }
 
+// Force of a random displacement through CreateElement against the oracle's cell
+// force on the oracle's own cells (the C restatement of the set-up and the force).
+double CellElementForceError(const std::shared_ptr<ElementBase> &elem, const MeshBlock &mesh, int npe,
+                             const std::shared_ptr<Material> &mat, const std::vector<int64_t> &start,
+                             const std::vector<int32_t> &nodes, const std::vector<double> &dphi,
+                             const std::vector<double> &vol, uint64_t seed, bool *yielded) {
+    const size_t n = mesh.NumNodes(), ne = mesh.connectivity.size() / npe;
+    std::vector<double> u(3 * n);
+    for (size_t i = 0; i < 3 * n; ++i) u[i] = 4e-3 * (2.0 * Uniform(seed, i) - 1.0);
+    elem->SetField(APHIP_FIELD_U, u);
+    InternalForceContribution(elem).ComputeForce(0.0, 1e-7);
+    const LinearElasticProperties &le = mat->GetMaterialProperties()->linear_elastic_properties;
+    or_material om{};
+    om.kind = mat->Kind() == APHIP_J2 ? OR_J2 : OR_NEO_HOOKEAN;
+    om.lambda = le.lambda;
+    om.mu = le.shear_modulus;
+    if (om.kind == OR_J2) {
+        om.yield_stress = mat->Params().yield_stress;
+        om.hardening_modulus = mat->Params().hardening_modulus;
+    }
+    std::vector<double> f(3 * n), H(9 * ne), P(9 * ne), s0(ne, 0.0), s1(ne, 0.0);
+    or_cell_internal_force((int64_t)n, (int64_t)ne, start.data(), nodes.data(), dphi.data(), vol.data(), &om,
+                           u.data(), s0.data(), s1.data(), H.data(), P.data(), f.data(), 1);
+    *yielded = *std::max_element(s1.begin(), s1.end()) > 0.0;
+    auto rel = [](const std::vector<double> &a, const std::vector<double> &b) {
+        double d = 0.0, s = 0.0;
+        for (size_t i = 0; i < a.size(); ++i) {
+            d = std::max(d, std::fabs(a[i] - b[i]));
+            s = std::max(s, std::fabs(b[i]));
+        }
+        return d / s;
+    };
+    double err = std::max(rel(elem->GetField(APHIP_FIELD_F), f), rel(elem->GetField(APHIP_FIELD_H), H));
+    err = std::max(err, rel(elem->GetField(APHIP_FIELD_P), P));
+    if (om.kind == OR_J2) err = std::max(err, rel(elem->GetField(APHIP_FIELD_STATE), s1));
+    return err;
+}
+
+void CellElementCases() {
+    Run("CreateElementTest.ReproducingKernelOnHex8MatchesOracle", [] {
+        // ElementReproducingKernelHex8 (include/ElementReproducingKernel.h:317-378): one smoothed
+        // cell per hex, kernel scale 1.5, the one-pass force, J2
+        std::vector<int32_t> hexes;
+        MeshBlock mesh = HexLattice(5, 0.2, 17, &hexes);
+        mesh.connectivity = hexes;
+        const double scale = 1.5;
+        auto mat = MakeMaterial(PLASTIC, 70e9, 0.33, 2700.0);
+        auto elem = CreateElement(ElementTopology::Hexahedron8, ApproximationSpaceType::ReproducingKernel,
+                                  IntegrationSchemeType::StrainSmoothing, mesh, std::make_shared<Device>(), mat, scale);
+        const size_t n = mesh.NumNodes(), ne = hexes.size() / 8;
+        std::vector<double> edge(n), radius(n), phi(40 * n);
+        or_max_edge_length((int64_t)n, mesh.coordinates.data(), (int64_t)ne, hexes.data(), 8, nullptr, edge.data());
+        for (size_t i = 0; i < n; ++i) radius[i] = scale * edge[i];
+        std::vector<int32_t> num(n), nbr(40 * n);
+        int64_t trunc = 0;
+        or_neighbor_search((int64_t)n, mesh.coordinates.data(), radius.data(), 40, 0, num.data(), nbr.data(), &trunc);
+        or_rk_function_values((int64_t)n, mesh.coordinates.data(), radius.data(), 40, num.data(), nbr.data(), phi.data());
+        std::vector<int64_t> start(ne + 1);
+        const int64_t nnz = or_rk_cells_count((int64_t)ne, hexes.data(), 8, 40, num.data(), nbr.data(), start.data());
+        std::vector<int32_t> nodes(nnz);
+        std::vector<double> dphi(3 * nnz), vol(ne);
+        or_rk_cells_build((int64_t)n, mesh.coordinates.data(), (int64_t)ne, hexes.data(), 8, 40, num.data(), nbr.data(),
+                          phi.data(), start.data(), nodes.data(), dphi.data(), vol.data());
+        bool yielded = false;
+        EXPECT_TRUE(CellElementForceError(elem, mesh, 8, mat, start, nodes, dphi, vol, 21, &yielded) < 1e-12);
+        EXPECT_TRUE(yielded);
+    });
+    Run("CreateElementTest.FiniteElementStrainSmoothingOnTet4MatchesOracle", [] {
+        // ElementSmoothedTetrahedron4 (include/Element.h:44-47): one cell per tet with its own
+        // shape functions, neo-Hookean and J2
+        std::vector<int32_t> hexes;
+        MeshBlock mesh = HexLattice(4, 0.2, 23, &hexes);
+        mesh.connectivity = SplitTets(mesh, hexes);
+        const size_t ne = mesh.connectivity.size() / 4;
+        std::vector<int64_t> start(ne + 1);
+        std::vector<int32_t> nodes(4 * ne);
+        std::vector<double> dphi(12 * ne), vol(ne);
+        or_rk_cells_two_pass((int64_t)ne, mesh.coordinates.data(), mesh.connectivity.data(), 4, start.data(),
+                             nodes.data(), dphi.data(), vol.data());
+        for (MaterialType type : {NEO_HOOKEAN, PLASTIC}) {
+            auto mat = MakeMaterial(type, 70e9, 0.33, 2700.0);
+            auto elem = CreateElement(ElementTopology::Tetrahedron4, ApproximationSpaceType::FiniteElement,
+                                      IntegrationSchemeType::StrainSmoothing, mesh, std::make_shared<Device>(), mat);
+            bool yielded = false;
+            EXPECT_TRUE(CellElementForceError(elem, mesh, 4, mat, start, nodes, dphi, vol, 29, &yielded) < 1e-12);
+            if (type == PLASTIC) EXPECT_TRUE(yielded);
+        }
+    });
+}
+
 }  // namespace
 
 // A results file of two Hex8 (12 nodes) with three time steps of known values, for the
@@ -635,6 +746,7 @@ int main(int argc, char **argv) {
             return 2;
         }
         GpuCases();
+        CellElementCases();
     } else {
         std::printf("usage: %s cpu|gpu\n", argv[0]);
         return 2;

@@ -340,9 +340,15 @@ def test_fe_strain_smoothing_tet4(gpu_available, kind, p):
         s.strain_smoothing_setup()
         return s
 
+    if kind == A.J2:
+        assert s_ref.max() > 0  # the random field yields some cells
     with make() as s:
         num, nbr, tr = s.rk_neighbors()
         assert (num == 1).all() and np.array_equal(nbr[:, 0], np.arange(len(X))) and tr == 0
+        assert (nbr[:, 1:] == -1).all()
+        ident = np.zeros((len(X), 40))
+        ident[:, 0] = 1.0
+        assert np.array_equal(s.rk_function_values(), ident)
         start, nodes, dphi, vol = s.rk_cells()
         assert np.array_equal(start, cells[0]) and np.array_equal(nodes, cells[1])
         assert rel_err(dphi, cells[2]) < 1e-12 and rel_err(vol, cells[3]) < 1e-14
@@ -350,7 +356,10 @@ def test_fe_strain_smoothing_tet4(gpu_available, kind, p):
         assert rel_err(s.get(A.FIELD_MASS), mass) < 1e-13
         s.set_field(A.FIELD_U, u)
         s.internal_force()
-        for fld, rr in ((A.FIELD_F, f_ref), (A.FIELD_H, H_ref), (A.FIELD_P, P_ref)):
+        fields = [(A.FIELD_F, f_ref), (A.FIELD_H, H_ref), (A.FIELD_P, P_ref)]
+        if kind == A.J2:
+            fields.append((A.FIELD_STATE, s_ref))
+        for fld, rr in fields:
             assert rel_err(s.get(fld), rr) < 1e-12, fld
     c = X - 0.5
     v0 = 60.0 * np.stack([-0.5 * c[:, 0], -0.5 * c[:, 1], c[:, 2]], axis=1)
@@ -363,8 +372,12 @@ def test_fe_strain_smoothing_tet4(gpu_available, kind, p):
         s.set_field(A.FIELD_V, v0)
         s.initialize()
         s.step(dt, 100)
-        for fld, rr in ((A.FIELD_U, ref.u), (A.FIELD_V, ref.v), (A.FIELD_F, ref.f), (A.FIELD_P, ref.P)):
+        for fld, rr in ((A.FIELD_U, ref.u), (A.FIELD_V, ref.v), (A.FIELD_F, ref.f), (A.FIELD_H, ref.H),
+                        (A.FIELD_P, ref.P)):
             assert rel_err(s.get(fld), rr) < 1e-10, fld
+        if kind == A.J2:
+            assert ref.state.max() > 0.0
+            assert rel_err(s.get(A.FIELD_STATE), ref.state) < 1e-10
         g.set_mesh(A.TET4, X, t)
         g.set_material(kind, p["rho"], p["E"], p["nu"], kw["yield_stress"], kw["hardening"])
         g.compute_mass()

@@ -103,6 +103,20 @@ def test_updated_rejections(gpu_available):
             s.power_method_time_increment()
         with pytest.raises(A.AphipError):
             s.set_formulation(1)  # incremental: not supported
+    # the reverse call orders: a formulation first, then the cell set-ups or a rate-form material
+    for form in (A.UPDATED_LAGRANGIAN, A.SEMI_LAGRANGIAN):
+        with _solver(A.TET4, X, conn, A.NEO_HOOKEAN, {}, form) as s:
+            with pytest.raises(A.AphipError, match="not to RKPM cells"):
+                s.rk_setup(1.5)
+            with pytest.raises(A.AphipError, match="not to RKPM cells"):
+                s.rk_setup(1.5, one_pass=False)
+            with pytest.raises(A.AphipError, match="not to smoothed cells"):
+                s.strain_smoothing_setup()
+    with _solver(A.TET4, X, conn, A.NEO_HOOKEAN, {}, A.SEMI_LAGRANGIAN) as s:
+        with pytest.raises(A.AphipError, match="rate-form"):
+            s.set_material(A.POWER_LAW_CREEP, RHO, E, NU, creep=CREEP)
+        with pytest.raises(A.AphipError, match="rate-form"):
+            s.set_material(A.DRUCKER_PRAGER, RHO, E, NU, drucker_prager=dict(A1=1e8, A2=0.1, A2G=0.1))
 
 
 # ---- semi-Lagrangian (ExplicitTimeIntegratorSemi, include/ExplicitTimeIntegrator.h:488-546) ----
