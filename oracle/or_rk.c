@@ -237,11 +237,15 @@ static int cell_union(const int32_t *en, int npe, int max_neighbors, const int32
 
 int64_t or_rk_cells_count(int64_t n_elem, const int32_t *conn, int npe, int max_neighbors,
                           const int32_t *num_neighbors, const int32_t *neighbors, int64_t *cell_start) {
-    int32_t buf[8 * 64];
     if (npe * max_neighbors > 8 * 64) return -1;
     cell_start[0] = 0;
-    for (int64_t e = 0; e < n_elem; ++e)
-        cell_start[e + 1] = cell_start[e] + cell_union(&conn[npe * e], npe, max_neighbors, num_neighbors, neighbors, buf);
+    /* lengths in parallel (cells are independent), then the running sum */
+#pragma omp parallel for schedule(dynamic, 4096)
+    for (int64_t e = 0; e < n_elem; ++e) {
+        int32_t buf[8 * 64];
+        cell_start[e + 1] = cell_union(&conn[npe * e], npe, max_neighbors, num_neighbors, neighbors, buf);
+    }
+    for (int64_t e = 0; e < n_elem; ++e) cell_start[e + 1] += cell_start[e];
     return cell_start[n_elem];
 }
 
@@ -250,11 +254,17 @@ int or_rk_cells_build(int64_t n_nodes, const double *X, int64_t n_elem, const in
                       const int64_t *cell_start, int32_t *cell_nodes, double *cell_dphi, double *cell_volume) {
     (void)n_nodes;
     if (npe != 4 && npe != 8) return -1;
+    int bad = 0;
+    /* cells are independent: each writes only its own CSR range */
+#pragma omp parallel for schedule(dynamic, 4096) reduction(| : bad)
     for (int64_t e = 0; e < n_elem; ++e) {
         int32_t *list = &cell_nodes[cell_start[e]];
         double *dphi = &cell_dphi[3 * cell_start[e]];
         const int len = cell_union(&conn[npe * e], npe, max_neighbors, num_neighbors, neighbors, list);
-        if (len != cell_start[e + 1] - cell_start[e]) return -1;
+        if (len != cell_start[e + 1] - cell_start[e]) {
+            bad = 1;
+            continue;
+        }
         double Xe[24], B[24], V;
         for (int k = 0; k < npe; ++k)
             for (int d = 0; d < 3; ++d) Xe[3 * k + d] = X[3 * (int64_t)conn[npe * e + k] + d];
@@ -276,7 +286,7 @@ int or_rk_cells_build(int64_t n_nodes, const double *X, int64_t n_elem, const in
         for (int t = 0; t < 3 * len; ++t) dphi[t] /= V;
         cell_volume[e] = V;
     }
-    return 0;
+    return bad ? -1 : 0;
 }
 
 int or_cell_internal_force(int64_t n_nodes, int64_t n_cells, const int64_t *cell_start, const int32_t *cell_nodes,
