@@ -226,6 +226,13 @@ static void permute_device(Ctx &c, DevBuf<T> &b, int comps, const std::vector<in
     APHIP_HIP(hipStreamSynchronize(c.stream));
 }
 
+// device copies of the internal -> caller permutations (device-side field transfer)
+static void upload_perms(Ctx &c) {
+    c.d_node_iperm.upload(c.node_iperm.data(), c.node_iperm.size(), c.stream);
+    c.d_elem_iperm.upload(c.elem_iperm.data(), c.elem_iperm.size(), c.stream);
+    c.d_count.alloc(1);
+}
+
 void renumber_nodes(Ctx &c, const std::vector<int32_t> &old_of_new) {
     const int64_t n = c.nn;
     std::vector<int32_t> new_of_old(n);
@@ -234,6 +241,7 @@ void renumber_nodes(Ctx &c, const std::vector<int32_t> &old_of_new) {
     for (int64_t i = 0; i < n; ++i) iperm[i] = c.node_iperm[old_of_new[i]];
     c.node_iperm.swap(iperm);
     for (int64_t i = 0; i < n; ++i) c.node_perm[c.node_iperm[i]] = (int32_t)i;
+    upload_perms(c);
     for (auto &x : c.conn_h) x = new_of_old[x];
     c.conn.upload(c.conn_h.data(), c.conn_h.size(), c.stream);
     upload_node_csr(c, c.conn_h);
@@ -334,6 +342,7 @@ static void setup_mesh(Ctx &c, int32_t elem_type, int64_t n_nodes, const double 
 
     c.X.upload(Xi.data(), Xi.size(), c.stream);
     c.conn.upload(ci.data(), ci.size(), c.stream);
+    upload_perms(c);
     upload_node_csr(c, ci);
     for (DevBuf<double> *b : {&c.u, &c.v, &c.a, &c.f}) {
         b->alloc(3 * n_nodes);
@@ -879,6 +888,98 @@ int aphip_get_field(aphip_ctx *c, int32_t field, double *out) {
         b.download(h.data(), h.size(), c->stream);
         APHIP_HIP(hipStreamSynchronize(c->stream));
         for (int64_t e = 0; e < c->ne; ++e) out[c->elem_iperm[e]] = h[e];
+    } else {
+        throw Error(APHIP_ERR_INVALID, "unknown field");
+    }
+    APHIP_API_END
+}
+
+}  // extern "C"
+
+// A caller array must be device memory of the context's device holding at least
+// `bytes` from p on (checked against the allocation that contains p), so that no
+// kernel of the transfer reads or writes outside the caller's allocation.
+static void require_device_array(const Ctx &c, const void *p, size_t bytes) {
+    hipPointerAttribute_t at{};
+    if (hipPointerGetAttributes(&at, p) != hipSuccess) {
+        (void)hipGetLastError();
+        throw Error(APHIP_ERR_INVALID, "array is not device memory known to HIP");
+    }
+    APHIP_REQUIRE(at.type == hipMemoryTypeDevice && at.device == c.device,
+                  "array is not device memory of the context's device");
+    hipDeviceptr_t base = nullptr;
+    size_t size = 0;
+    if (hipMemGetAddressRange(&base, &size, (hipDeviceptr_t)p) != hipSuccess) {
+        (void)hipGetLastError();
+        throw Error(APHIP_ERR_INVALID, "array is not inside a device allocation");
+    }
+    APHIP_REQUIRE((const char *)p + bytes <= (const char *)base + size,
+                  "device array is shorter than the field");
+}
+
+static DevBuf<double> &node_vector(Ctx &c, int32_t field) {
+    return field == APHIP_FIELD_U ? c.u : field == APHIP_FIELD_V ? c.v : field == APHIP_FIELD_A ? c.a : c.f;
+}
+
+extern "C" {
+
+int aphip_set_field_device(aphip_ctx *c, int32_t field, const double *data) {
+    APHIP_API_BEGIN
+    APHIP_REQUIRE(c != nullptr && data != nullptr, "NULL argument");
+    APHIP_REQUIRE(c->etype >= 0, "mesh not set");
+    APHIP_HIP(hipSetDevice(c->device));
+    if (c->graph_exec) {  // a captured step sequence is re-captured after a field changes
+        APHIP_HIP(hipStreamSynchronize(c->stream));
+        c->graph_reset();
+    }
+    if (field >= APHIP_FIELD_U && field <= APHIP_FIELD_F) {
+        require_device_array(*c, data, sizeof(double) * 3 * c->nn);
+        field_gather_rows(*c, c->nn, 3, c->d_node_iperm.p, data, node_vector(*c, field).p);
+    } else if (field == APHIP_FIELD_MASS) {
+        require_device_array(*c, data, sizeof(double) * c->nn);
+        c->mass.alloc(c->nn);
+        c->has_mass = false;
+        APHIP_REQUIRE(field_gather_mass(*c, data) == 0, "nodal mass must be positive");
+        c->has_mass = true;
+    } else if (field == APHIP_FIELD_STATE) {
+        const int64_t ns = c->state_count();
+        APHIP_REQUIRE(ns > 0, "material has no state");
+        require_device_array(*c, data, sizeof(double) * ns);
+        field_gather_rows(*c, c->ne, (int)(ns / c->ne), c->d_elem_iperm.p, data, c->state[c->state_np1].p);
+    } else {
+        throw Error(APHIP_ERR_INVALID, "field cannot be set");
+    }
+    APHIP_API_END
+}
+
+int aphip_get_field_device(aphip_ctx *c, int32_t field, double *out) {
+    APHIP_API_BEGIN
+    APHIP_REQUIRE(c != nullptr && out != nullptr, "NULL argument");
+    APHIP_REQUIRE(c->etype >= 0, "mesh not set");
+    APHIP_HIP(hipSetDevice(c->device));
+    if (field >= APHIP_FIELD_U && field <= APHIP_FIELD_F) {
+        require_device_array(*c, out, sizeof(double) * 3 * c->nn);
+        field_scatter_rows(*c, c->nn, 3, c->d_node_iperm.p, node_vector(*c, field).p, out);
+    } else if (field == APHIP_FIELD_MASS) {
+        APHIP_REQUIRE(c->has_mass, "mass not computed or set");
+        require_device_array(*c, out, sizeof(double) * c->nn);
+        field_scatter_rows(*c, c->nn, 1, c->d_node_iperm.p, c->mass.p, out);
+    } else if (field == APHIP_FIELD_H || field == APHIP_FIELD_P) {
+        require_device_array(*c, out, sizeof(double) * 9 * c->ne);
+        field_scatter_tensor(*c, (field == APHIP_FIELD_H ? c->Hnp1() : c->Pnp1()).p, out);
+    } else if (field == APHIP_FIELD_STATE) {
+        const int64_t ns = c->state_count();
+        APHIP_REQUIRE(ns > 0, "material has no state");
+        require_device_array(*c, out, sizeof(double) * ns);
+        field_scatter_rows(*c, c->ne, (int)(ns / c->ne), c->d_elem_iperm.p, c->state[c->state_np1].p, out);
+    } else if (field == APHIP_FIELD_RK_DISPLACEMENT || field == APHIP_FIELD_RK_FORCE_LOCAL) {
+        APHIP_REQUIRE(c->cells.ready && c->cells.two_pass, "field exists with the two-pass RK method only");
+        require_device_array(*c, out, sizeof(double) * 3 * c->nn);
+        field_scatter_rows(*c, c->nn, 3, c->d_node_iperm.p,
+                           (field == APHIP_FIELD_RK_DISPLACEMENT ? c->cells.u_phys : c->cells.f_local).p, out);
+    } else if (field == APHIP_FIELD_VOLUME) {
+        require_device_array(*c, out, sizeof(double) * c->ne);
+        field_scatter_rows(*c, c->ne, 1, c->d_elem_iperm.p, (c->cells.ready ? c->cells.volume : c->volume).p, out);
     } else {
         throw Error(APHIP_ERR_INVALID, "unknown field");
     }

@@ -238,6 +238,9 @@ struct Ctx {
     int64_t n_specimens = 0;
     std::vector<int64_t> spec_node_seg, spec_elem_seg;  // caller-numbering ranges per specimen
 
+    DevBuf<int32_t> d_node_iperm, d_elem_iperm;  // device copies of node_iperm / elem_iperm
+    DevBuf<int32_t> d_count;             // one counter (validation of device-side field input)
+
     DevBuf<double> X, u, v, a, f, mass;  // node vectors AoS n x 3 (mass n)
     DevBuf<uint8_t> fixed;               // bit d set = DOF d held at zero
     DevBuf<int32_t> conn;                // AoS ne x npe, internal node ids
@@ -347,6 +350,15 @@ void pm_perturb(Ctx &c, const double *u_in, const double *v, const double *l, do
 // v = (f - f0) / (l eps m), returns |v|_2
 double pm_next(Ctx &c, const double *f0, const double *l, double eps, double *v, DevBuf<double> &partial);
 void pm_scale(Ctx &c, double norm, double *v);  // v /= norm
+
+// ---- field transfer with caller device arrays (field.hip) ----
+// dst[i, :] = src[iperm[i], :] / dst[iperm[i], :] = src[i, :], w doubles per entity
+void field_gather_rows(Ctx &c, int64_t n, int w, const int32_t *iperm, const double *src, double *dst);
+void field_scatter_rows(Ctx &c, int64_t n, int w, const int32_t *iperm, const double *src, double *dst);
+// SoA 9 x ne tensor planes -> caller row-major n_elem x 9
+void field_scatter_tensor(Ctx &c, const double *planes, double *dst);
+// caller-numbered nodal mass into c.mass; returns how many values are not positive
+int32_t field_gather_mass(Ctx &c, const double *src);
 
 // ---- copy yardstick (probe.hip): device milliseconds per copy of nbytes ----
 double bandwidth_probe(int device, int64_t nbytes, int reps);

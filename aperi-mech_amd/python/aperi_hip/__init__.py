@@ -39,7 +39,8 @@ _u8 = ctypes.POINTER(ctypes.c_uint8)
 EXPORTED = [
     "aphip_create", "aphip_destroy", "aphip_last_error", "aphip_device_count", "aphip_set_mesh", "aphip_get_node_permutation",
     "aphip_set_material", "aphip_set_material_fields", "aphip_set_formulation", "aphip_set_formulation_semi", "aphip_compute_mass", "aphip_set_fixed",
-    "aphip_set_field", "aphip_get_field", "aphip_internal_force", "aphip_initialize", "aphip_step",
+    "aphip_set_field", "aphip_get_field", "aphip_set_field_device", "aphip_get_field_device",
+    "aphip_internal_force", "aphip_initialize", "aphip_step",
     "aphip_synchronize", "aphip_set_tiling", "aphip_set_graph", "aphip_autotune", "aphip_batch_begin",
     "aphip_batch_add", "aphip_batch_end", "aphip_batch_ranges", "aphip_internal_force_vjp",
     "aphip_adjoint_forward", "aphip_adjoint_backward", "aphip_rk_setup", "aphip_rk_setup_method", "aphip_strain_smoothing_setup", "aphip_rk_get_neighbors", "aphip_rk_get_function_values",
@@ -224,6 +225,21 @@ class Solver:
         d = np.ascontiguousarray(data, dtype=np.float64)
         _check(self.L.aphip_set_field(self.ctx, field, _p(d)))
         return self
+
+    def set_field_device(self, field, ptr):
+        """Field from a device array (integer device address, caller numbering), queued on
+        the context's stream."""
+        _check(self.L.aphip_set_field_device(self.ctx, field, ctypes.c_void_p(int(ptr))))
+        return self
+
+    def get_field_device(self, field, ptr):
+        """Field into a device array (integer device address, caller numbering), queued on
+        the context's stream; complete after synchronize()."""
+        _check(self.L.aphip_get_field_device(self.ctx, field, ctypes.c_void_p(int(ptr))))
+        return self
+
+    def field_shape(self, field):
+        return self._shape(field)
 
     def _shape(self, field):
         if field in (FIELD_U, FIELD_V, FIELD_A, FIELD_F, FIELD_RK_DISPLACEMENT, FIELD_RK_FORCE_LOCAL):

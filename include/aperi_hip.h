@@ -193,6 +193,23 @@ int aphip_set_fixed(aphip_ctx *ctx, const uint8_t *mask);
 int aphip_set_field(aphip_ctx *ctx, int32_t field, const double *data);
 int aphip_get_field(aphip_ctx *ctx, int32_t field, double *out);
 
+/* Device-resident field transfer, with no host memory touched: the same fields, caller
+ * numbering and layout as aphip_set_field / aphip_get_field, but `data` / `out` are
+ * device arrays on the context's device (hipMalloc'd; checked to be device memory of
+ * that device and long enough for the field, else APHIP_ERR_INVALID).  The renumbering
+ * runs on the GPU.  Both calls are queued on the context's stream and return without
+ * waiting: the caller's writes to `data` must be complete before the call (for example
+ * by synchronizing the stream that wrote them), `data` must stay unchanged until the
+ * context's stream has passed the call, and `out` holds the field once it has
+ * (aphip_synchronize, or any later call that waits for the stream, such as
+ * aphip_internal_force).  Setting MASS waits for the stream to validate the values.
+ * Replaces the field access of ComputeInternalForceBase::UpdateField and the NGP field
+ * views the reference's force functors read and write on the device
+ * (include/ComputeInternalForceBase.h:110-135, include/ElementTetrahedron4.h:184-192,
+ * which marks the fields modified on the device). */
+int aphip_set_field_device(aphip_ctx *ctx, int32_t field, const double *data);
+int aphip_get_field_device(aphip_ctx *ctx, int32_t field, double *out);
+
 /* ---- the hot path -------------------------------------------------------- */
 
 /* f = -sum_e int B^T P dV from the current displacement; writes H, P and the NP1
