@@ -2,8 +2,10 @@
 // renumbering, node-to-element CSR), material resolution, field transfer and
 // the step driver.  Host C++ around the kernels in fem.hip and rk.hip.
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <numeric>
 
 #include "internal.hpp"
@@ -348,6 +350,7 @@ static void setup_mesh(Ctx &c, int32_t elem_type, int64_t n_nodes, const double 
         b->alloc(3 * n_nodes);
         b->zero(c.stream);
     }
+    c.pm_eig.alloc(3 * n_nodes);  // the power method's eigenvector, kept between its calls
     c.H.alloc(9 * std::max<int64_t>(n_elem, 1));
     c.P.alloc(9 * std::max<int64_t>(n_elem, 1));
     c.H.zero(c.stream);
@@ -357,6 +360,7 @@ static void setup_mesh(Ctx &c, int32_t elem_type, int64_t n_nodes, const double 
     c.has_mass = false;
     c.fixed.release();
     c.has_fixed = false;
+    c.all_fixed = false;
     c.has_fields = false;
     c.elem_lambda.release();
     c.elem_mu.release();
@@ -430,62 +434,57 @@ struct PowerMethodResult {
 // the final eigenvector and starts the next call from it (:350-361, :468-469); only the
 // first call after the mesh is set draws the random start.  u, f, H, P and the NP1
 // state are restored afterwards, so the call leaves the simulation state untouched.
-static PowerMethodResult power_method(Ctx &c, double prev_dt, int n_iter, double eps, double tol, double vtol,
-                                      uint64_t seed) {
+// Device buffers of one power-method call (internal numbering): the displacement and
+// force on entry, F(u), the iterate, the max edge lengths, the H / P / state on entry and
+// the norm's partial sums.
+struct PowerMethodWork {
+    double *u_in, *f_in, *f0, *v, *l, *H, *P, *st, *partial;
+};
+
+// doubles each buffer of PowerMethodWork needs for the context's mesh and material
+static std::array<int64_t, 9> power_method_sizes(const Ctx &c) {
+    const int64_t n3 = 3 * c.nn, nt = c.ne ? 9 * c.ne : 0;
+    return {n3, n3, n3, n3, c.nn, nt, nt, c.state_count(), pm_partials(c)};
+}
+
+static PowerMethodResult power_method(Ctx &c, const PowerMethodWork &w, double prev_dt, int n_iter, double eps,
+                                      double tol, double vtol, uint64_t seed) {
     const int64_t n3 = 3 * c.nn;
     APHIP_REQUIRE(c.formulation == APHIP_TOTAL_LAGRANGIAN, "the power method runs with the total formulation");
-    if (c.has_fixed) {  // PowerMethodProcessor::CheckEssentialBoundaries (:159-173)
-        std::vector<uint8_t> fx(c.nn);
-        c.fixed.download(fx.data(), fx.size(), c.stream);
-        APHIP_HIP(hipStreamSynchronize(c.stream));
-        bool all = true;
-        for (uint8_t m : fx) all = all && (m & 7u) == 7u;
-        APHIP_REQUIRE(!all,
-                      "All nodes are in the essential boundary set. Cannot run the power method. Use the "
-                      "direct_time_stepper instead.");
-    }
-    DevBuf<double> u_in, f_in, f0, v, l, H, P, st, partial;
-    DevBuf<int32_t> caller;
-    u_in.alloc(n3);
-    f_in.alloc(n3);
-    f0.alloc(n3);
-    v.alloc(n3);
-    l.alloc(c.nn);
-    caller.upload(c.node_iperm.data(), c.nn, c.stream);
+    // PowerMethodProcessor::CheckEssentialBoundaries (:159-173)
+    APHIP_REQUIRE(!(c.has_fixed && c.all_fixed),
+                  "All nodes are in the essential boundary set. Cannot run the power method. Use the "
+                  "direct_time_stepper instead.");
     const int64_t ns = c.state_count();
-    APHIP_HIP(hipMemcpyAsync(u_in.p, c.u.p, sizeof(double) * n3, hipMemcpyDeviceToDevice, c.stream));
-    APHIP_HIP(hipMemcpyAsync(f_in.p, c.f.p, sizeof(double) * n3, hipMemcpyDeviceToDevice, c.stream));
+    APHIP_HIP(hipMemcpyAsync(w.u_in, c.u.p, sizeof(double) * n3, hipMemcpyDeviceToDevice, c.stream));
+    APHIP_HIP(hipMemcpyAsync(w.f_in, c.f.p, sizeof(double) * n3, hipMemcpyDeviceToDevice, c.stream));
     if (c.ne) {
-        H.alloc(9 * c.ne);
-        P.alloc(9 * c.ne);
-        APHIP_HIP(hipMemcpyAsync(H.p, c.Hnp1().p, sizeof(double) * 9 * c.ne, hipMemcpyDeviceToDevice, c.stream));
-        APHIP_HIP(hipMemcpyAsync(P.p, c.Pnp1().p, sizeof(double) * 9 * c.ne, hipMemcpyDeviceToDevice, c.stream));
+        APHIP_HIP(hipMemcpyAsync(w.H, c.Hnp1().p, sizeof(double) * 9 * c.ne, hipMemcpyDeviceToDevice, c.stream));
+        APHIP_HIP(hipMemcpyAsync(w.P, c.Pnp1().p, sizeof(double) * 9 * c.ne, hipMemcpyDeviceToDevice, c.stream));
     }
-    if (ns) {
-        st.alloc(ns);
-        APHIP_HIP(hipMemcpyAsync(st.p, c.state[c.state_np1].p, sizeof(double) * ns, hipMemcpyDeviceToDevice,
+    if (ns)
+        APHIP_HIP(hipMemcpyAsync(w.st, c.state[c.state_np1].p, sizeof(double) * ns, hipMemcpyDeviceToDevice,
                                  c.stream));
-    }
     c.graph_reset();
-    pm_max_edge(c, l.p);
+    pm_max_edge(c, w.l);
     if (c.pm_eig_valid)
-        APHIP_HIP(hipMemcpyAsync(v.p, c.pm_eig.p, sizeof(double) * n3, hipMemcpyDeviceToDevice, c.stream));
+        APHIP_HIP(hipMemcpyAsync(w.v, c.pm_eig.p, sizeof(double) * n3, hipMemcpyDeviceToDevice, c.stream));
     else
-        pm_start(c, caller.p, seed, eps, l.p, v.p);
+        pm_start(c, c.d_node_iperm.p, seed, eps, w.l, w.v);
     internal_force_eval(c);  // F(u)
-    APHIP_HIP(hipMemcpyAsync(f0.p, c.f.p, sizeof(double) * n3, hipMemcpyDeviceToDevice, c.stream));
+    APHIP_HIP(hipMemcpyAsync(w.f0, c.f.p, sizeof(double) * n3, hipMemcpyDeviceToDevice, c.stream));
 
-    const int window = 5;
-    std::vector<double> recent(window, 0.0);
+    constexpr int window = 5;
+    std::array<double, window> recent{};
     double dt_np1 = prev_dt == 0.0 ? 1.0e-16 : prev_dt, dt_n = 0.0;
     bool converged = false, oscillating = false;
     int used = n_iter;
     for (int k = 0; k < n_iter; ++k) {
         dt_n = dt_np1;
-        pm_perturb(c, u_in.p, v.p, l.p, eps);
+        pm_perturb(c, w.u_in, w.v, w.l, eps);
         internal_force_eval(c);
-        const double lambda = pm_next(c, f0.p, l.p, eps, v.p, partial);
-        pm_scale(c, lambda, v.p);
+        const double lambda = pm_next(c, w.f0, w.l, eps, w.v, w.partial);
+        pm_scale(c, lambda, w.v);
         dt_np1 = 2.0 / std::sqrt(lambda);
         recent[k % window] = dt_np1;
         if (std::fabs(dt_np1 - dt_n) < tol * dt_np1) {
@@ -522,18 +521,17 @@ static PowerMethodResult power_method(Ctx &c, double prev_dt, int n_iter, double
     }
     const double wmin = *std::min_element(recent.begin(), recent.end());
     const double dt = converged ? (oscillating ? wmin : dt_np1) : wmin;
-    c.pm_eig.alloc(n3);
-    APHIP_HIP(hipMemcpyAsync(c.pm_eig.p, v.p, sizeof(double) * n3, hipMemcpyDeviceToDevice, c.stream));
+    APHIP_HIP(hipMemcpyAsync(c.pm_eig.p, w.v, sizeof(double) * n3, hipMemcpyDeviceToDevice, c.stream));
     c.pm_eig_valid = true;
 
-    APHIP_HIP(hipMemcpyAsync(c.u.p, u_in.p, sizeof(double) * n3, hipMemcpyDeviceToDevice, c.stream));
-    APHIP_HIP(hipMemcpyAsync(c.f.p, f_in.p, sizeof(double) * n3, hipMemcpyDeviceToDevice, c.stream));
+    APHIP_HIP(hipMemcpyAsync(c.u.p, w.u_in, sizeof(double) * n3, hipMemcpyDeviceToDevice, c.stream));
+    APHIP_HIP(hipMemcpyAsync(c.f.p, w.f_in, sizeof(double) * n3, hipMemcpyDeviceToDevice, c.stream));
     if (c.ne) {
-        APHIP_HIP(hipMemcpyAsync(c.Hnp1().p, H.p, sizeof(double) * 9 * c.ne, hipMemcpyDeviceToDevice, c.stream));
-        APHIP_HIP(hipMemcpyAsync(c.Pnp1().p, P.p, sizeof(double) * 9 * c.ne, hipMemcpyDeviceToDevice, c.stream));
+        APHIP_HIP(hipMemcpyAsync(c.Hnp1().p, w.H, sizeof(double) * 9 * c.ne, hipMemcpyDeviceToDevice, c.stream));
+        APHIP_HIP(hipMemcpyAsync(c.Pnp1().p, w.P, sizeof(double) * 9 * c.ne, hipMemcpyDeviceToDevice, c.stream));
     }
     if (ns)
-        APHIP_HIP(hipMemcpyAsync(c.state[c.state_np1].p, st.p, sizeof(double) * ns, hipMemcpyDeviceToDevice,
+        APHIP_HIP(hipMemcpyAsync(c.state[c.state_np1].p, w.st, sizeof(double) * ns, hipMemcpyDeviceToDevice,
                                  c.stream));
     APHIP_HIP(hipStreamSynchronize(c.stream));
     return PowerMethodResult{dt, used, converged};
@@ -578,6 +576,37 @@ struct aphip_ctx : Ctx {};
         return APHIP_ERR_INVALID;                                       \
     }
 
+// Scratch memory of the allocation-free power method
+// (aphip_power_method_time_increment_noalloc): a fixed array in the code object, one copy
+// per GPU, present from module load to process end.  Host code never passes `scratch`
+// itself to a kernel or copy (on the host the name is only a stand-in for the symbol):
+// aphip_create looks up each device's copy once, with that device current, and every
+// later use goes through the address kept in g_scratch_ptr.
+constexpr size_t kScratchBytes = size_t(64) << 20;  // 64 MiB per GPU
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+static std::mutex g_scratch_mu;
+static void *g_scratch_ptr[kMaxDevices] = {};
+
+// Looks up this device's copy of `scratch` (the device must be current).
+static void scratch_resolve(int device) {
+    APHIP_REQUIRE(device >= 0 && device < kMaxDevices, "device ordinal beyond the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[device]) return;
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || p == nullptr) {
+        (void)hipGetLastError();
+        throw Error(APHIP_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    }
+    g_scratch_ptr[device] = p;
+}
+
+static void *scratch_address(int device) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    return (device >= 0 && device < kMaxDevices) ? g_scratch_ptr[device] : nullptr;
+}
+
 extern "C" {
 
 const char *aphip_last_error(void) { return g_last_error.c_str(); }
@@ -600,6 +629,7 @@ int aphip_create(aphip_ctx **out, const aphip_config *cfg) {
     const int dev = cfg ? cfg->device : 0;
     APHIP_REQUIRE(dev >= 0 && dev < ndev, "invalid HIP device ordinal");
     APHIP_HIP(hipSetDevice(dev));
+    scratch_resolve(dev);
     aphip_ctx *c = new aphip_ctx();
     c->device = dev;
     c->flags = cfg ? cfg->flags : 0;
@@ -788,16 +818,20 @@ int aphip_set_fixed(aphip_ctx *c, const uint8_t *mask) {
     if (!mask) {
         c->fixed.release();
         c->has_fixed = false;
+        c->all_fixed = false;
         return APHIP_OK;
     }
     std::vector<uint8_t> bits(c->nn, 0);
+    bool all = true;
     for (int64_t i = 0; i < c->nn; ++i) {
         const int64_t src = c->node_iperm[i];
         for (int d = 0; d < 3; ++d)
             if (mask[3 * src + d]) bits[i] |= (uint8_t)(1u << d);
+        all = all && bits[i] == 7u;
     }
     c->fixed.upload(bits.data(), bits.size(), c->stream);
     c->has_fixed = true;
+    c->all_fixed = all;
     APHIP_HIP(hipStreamSynchronize(c->stream));
     APHIP_API_END
 }
@@ -1009,8 +1043,44 @@ int aphip_power_method_time_increment(aphip_ctx *c, double previous_time_increme
                   "power method needs positive iterations, epsilon and tolerance");
     APHIP_REQUIRE(previous_time_increment >= 0.0, "previous time increment must be >= 0");
     APHIP_HIP(hipSetDevice(c->device));
-    const PowerMethodResult r =
-        power_method(*c, previous_time_increment, num_iterations, epsilon, tolerance, variation_tolerance, seed);
+    const auto n = power_method_sizes(*c);
+    DevBuf<double> b[9];
+    for (int k = 0; k < 9; ++k) b[k].alloc((size_t)n[k]);
+    const PowerMethodWork w{b[0].p, b[1].p, b[2].p, b[3].p, b[4].p, b[5].p, b[6].p, b[7].p, b[8].p};
+    const PowerMethodResult r = power_method(*c, w, previous_time_increment, num_iterations, epsilon, tolerance,
+                                             variation_tolerance, seed);
+    *time_increment = r.dt;
+    if (iterations) *iterations = r.iterations;
+    if (converged) *converged = r.converged ? 1 : 0;
+    APHIP_API_END
+}
+
+int aphip_power_method_time_increment_noalloc(aphip_ctx *c, double previous_time_increment, int32_t num_iterations,
+                                              double epsilon, double tolerance, double variation_tolerance,
+                                              uint64_t seed, double *time_increment, int32_t *iterations,
+                                              int32_t *converged) {
+    APHIP_API_BEGIN
+    APHIP_REQUIRE(c != nullptr && time_increment != nullptr, "NULL argument");
+    require_ready(*c, true);
+    APHIP_REQUIRE(num_iterations > 0 && epsilon > 0.0 && tolerance > 0.0 && variation_tolerance >= 0.0,
+                  "power method needs positive iterations, epsilon and tolerance");
+    APHIP_REQUIRE(previous_time_increment >= 0.0, "previous time increment must be >= 0");
+    // the work buffers, each on a 256-byte boundary inside this device's scratch
+    const auto n = power_method_sizes(*c);
+    size_t off[10] = {0};
+    for (int k = 0; k < 9; ++k) off[k + 1] = off[k] + (((size_t)n[k] * sizeof(double) + 255) & ~size_t(255));
+    if (off[9] > kScratchBytes)
+        throw Error(APHIP_ERR_INVALID, "the allocation-free power method holds its work buffers in " +
+                                           std::to_string(kScratchBytes >> 20) + " MiB of scratch per GPU; this mesh needs " +
+                                           std::to_string(off[9]) + " bytes (use aphip_power_method_time_increment)");
+    APHIP_HIP(hipSetDevice(c->device));
+    unsigned char *base = static_cast<unsigned char *>(scratch_address(c->device));
+    APHIP_REQUIRE(base != nullptr, "scratch of this device not resolved");
+    double *p[9];
+    for (int k = 0; k < 9; ++k) p[k] = reinterpret_cast<double *>(base + off[k]);
+    const PowerMethodWork w{p[0], p[1], p[2], p[3], p[4], p[5], p[6], p[7], p[8]};
+    const PowerMethodResult r = power_method(*c, w, previous_time_increment, num_iterations, epsilon, tolerance,
+                                             variation_tolerance, seed);
     *time_increment = r.dt;
     if (iterations) *iterations = r.iterations;
     if (converged) *converged = r.converged ? 1 : 0;

@@ -290,11 +290,12 @@ struct Ctx {
     }
     bool has_mass = false;
     bool has_fixed = false;
+    bool all_fixed = false;              // every DOF of every node is fixed
 
     std::unique_ptr<exo::Writer> results;  // open Exodus II results file, if any
 
     // power method: the eigenvector of the last call, the next call's start vector
-    DevBuf<double> pm_eig;               // 3 nn, internal numbering
+    DevBuf<double> pm_eig;               // 3 nn, internal numbering, allocated with the mesh
     bool pm_eig_valid = false;
 
     CellData cells;
@@ -347,8 +348,10 @@ void prof_end(Ctx &c, int which);
 void pm_max_edge(Ctx &c, double *l);
 void pm_start(Ctx &c, const int32_t *caller_id, uint64_t seed, double eps, const double *l, double *v);
 void pm_perturb(Ctx &c, const double *u_in, const double *v, const double *l, double eps);
-// v = (f - f0) / (l eps m), returns |v|_2
-double pm_next(Ctx &c, const double *f0, const double *l, double eps, double *v, DevBuf<double> &partial);
+// doubles of the partial-sum buffer pm_next needs (one per block of nodes, plus the total)
+int64_t pm_partials(const Ctx &c);
+// v = (f - f0) / (l eps m), returns |v|_2 (partial: pm_partials(c) doubles)
+double pm_next(Ctx &c, const double *f0, const double *l, double eps, double *v, double *partial);
 void pm_scale(Ctx &c, double norm, double *v);  // v /= norm
 
 // ---- field transfer with caller device arrays (field.hip) ----

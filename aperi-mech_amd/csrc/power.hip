@@ -2,7 +2,7 @@
 // PowerMethodProcessor, include/PowerMethodProcessor.h:150-470): per-node max edge
 // length, the seeded start vector, the perturbation u + v l eps, the next iterate
 // (F(u + v l eps) - F(u)) / (l eps m) and its L2 norm (deterministic block partial
-// sums, added on the host in block order).  The driver is in capi.hip.
+// sums, added in block order by one thread).  The driver is in capi.hip.
 #include <cmath>
 
 #include "internal.hpp"
@@ -91,6 +91,15 @@ __global__ __launch_bounds__(kBlock) void k_pm_next(int64_t nn, const double *f1
     if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
 }
 
+// partial[nb] = partial[0] + partial[1] + ... + partial[nb - 1], added in block order by
+// one thread (the order of a sequential host loop)
+__global__ __launch_bounds__(64) void k_pm_sum(int64_t nb, double *partial) {
+    if (threadIdx.x != 0) return;
+    double s = 0.0;
+    for (int64_t b = 0; b < nb; ++b) s += partial[b];
+    partial[nb] = s;
+}
+
 __global__ __launch_bounds__(kBlock) void k_pm_scale(int64_t n, double norm, double *v) {
     const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (i < n) v[i] /= norm;
@@ -117,17 +126,18 @@ void pm_perturb(Ctx &c, const double *u_in, const double *v, const double *l, do
     APHIP_HIP(hipGetLastError());
 }
 
-double pm_next(Ctx &c, const double *f0, const double *l, double eps, double *v, DevBuf<double> &partial) {
+int64_t pm_partials(const Ctx &c) { return (int64_t)blocks(c.nn).x + 1; }
+
+double pm_next(Ctx &c, const double *f0, const double *l, double eps, double *v, double *partial) {
     const dim3 g = blocks(c.nn);
-    partial.alloc(g.x);
     hipLaunchKernelGGL(k_pm_next, g, dim3(kBlock), 0, c.stream, c.nn, c.f.p, f0, c.mass.p, l,
-                       c.has_fixed ? c.fixed.p : nullptr, eps, v, partial.p);
+                       c.has_fixed ? c.fixed.p : nullptr, eps, v, partial);
     APHIP_HIP(hipGetLastError());
-    std::vector<double> h(g.x);
-    partial.download(h.data(), h.size(), c.stream);
-    APHIP_HIP(hipStreamSynchronize(c.stream));
+    hipLaunchKernelGGL(k_pm_sum, dim3(1), dim3(64), 0, c.stream, (int64_t)g.x, partial);
+    APHIP_HIP(hipGetLastError());
     double s = 0.0;
-    for (double x : h) s += x;
+    APHIP_HIP(hipMemcpyAsync(&s, partial + g.x, sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    APHIP_HIP(hipStreamSynchronize(c.stream));
     return sqrt(s);
 }
 

@@ -45,7 +45,7 @@ EXPORTED = [
     "aphip_batch_add", "aphip_batch_end", "aphip_batch_ranges", "aphip_internal_force_vjp",
     "aphip_adjoint_forward", "aphip_adjoint_backward", "aphip_rk_setup", "aphip_rk_setup_method", "aphip_strain_smoothing_setup", "aphip_rk_get_neighbors", "aphip_rk_get_function_values",
     "aphip_rk_get_cells", "aphip_set_profiling", "aphip_get_profile", "aphip_reset_profile",
-    "aphip_bandwidth_probe", "aphip_power_method_time_increment",
+    "aphip_bandwidth_probe", "aphip_power_method_time_increment", "aphip_power_method_time_increment_noalloc",
     "aphip_results_open", "aphip_results_write", "aphip_results_close",
 ]
 
@@ -259,11 +259,16 @@ class Solver:
         return out
 
     def power_method_time_increment(self, num_iterations=50, epsilon=1e-5, tolerance=1e-3,
-                                    variation_tolerance=1e-7, seed=20261202, previous_time_increment=0.0):
+                                    variation_tolerance=1e-7, seed=20261202, previous_time_increment=0.0,
+                                    no_alloc=False):
         """(dt, iterations, converged) of the power-method stable increment; the start
-        vector is the eigenvector of the previous call (random on the first call)."""
+        vector is the eigenvector of the previous call (random on the first call).
+        no_alloc: the entry point that allocates no device memory (meshes up to its
+        scratch size)."""
         dt, it, cv = ctypes.c_double(), ctypes.c_int32(), ctypes.c_int32()
-        _check(self.L.aphip_power_method_time_increment(self.ctx, ctypes.c_double(previous_time_increment),
+        fn = (self.L.aphip_power_method_time_increment_noalloc if no_alloc
+              else self.L.aphip_power_method_time_increment)
+        _check(fn(self.ctx, ctypes.c_double(previous_time_increment),
                                                         int(num_iterations), ctypes.c_double(epsilon),
                                                         ctypes.c_double(tolerance),
                                                         ctypes.c_double(variation_tolerance),

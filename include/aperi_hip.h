@@ -244,6 +244,22 @@ int aphip_power_method_time_increment(aphip_ctx *ctx, double previous_time_incre
                                       double variation_tolerance, uint64_t seed, double *time_increment,
                                       int32_t *iterations, int32_t *converged);
 
+/* aphip_power_method_time_increment without any device allocation: same arguments, same
+ * algorithm and the same results (it shares the kernels and the driver).  No counterpart
+ * in the reference: it is for callers that cannot allocate device memory, such as a time
+ * loop under HIP graph capture or a serving loop that forbids hipMalloc.  Its work buffers
+ * (the displacement, force, H, P and state on entry, F(u), the iterate, the edge lengths
+ * and the norm's partial sums: 8 (13 n_nodes + 18 n_elem + state values) bytes plus a
+ * few KiB) live in a fixed 64 MiB array per GPU, held from library load to process end;
+ * a mesh that needs more is refused with APHIP_ERR_INVALID.  Every call on a GPU shares
+ * that GPU's array: calls on contexts of the same GPU must not overlap in time (they
+ * would race on it).  The eigenvector kept between calls stays in the context, as for
+ * the allocating entry point. */
+int aphip_power_method_time_increment_noalloc(aphip_ctx *ctx, double previous_time_increment,
+                                              int32_t num_iterations, double epsilon, double tolerance,
+                                              double variation_tolerance, uint64_t seed, double *time_increment,
+                                              int32_t *iterations, int32_t *converged);
+
 /* Exodus II results file (netCDF 64-bit-offset format, written by this library with
  * no netCDF dependency): replaces the reference's results output, IoMesh::
  * CreateFieldResultsFile / AddFieldResultsOutput / WriteFieldResults / CloseFieldResults
