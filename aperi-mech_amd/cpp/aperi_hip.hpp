@@ -260,6 +260,13 @@ enum class ApproximationSpaceType { FiniteElement, ReproducingKernel };
 enum class IntegrationSchemeType { GaussQuadrature, StrainSmoothing };
 // include/Constants.h:33-36 (Incremental is not supported here)
 enum class LagrangianFormulationType { Total, Incremental, Updated, Semi };
+// include/MeshLabelerParameters.h:15-29: the smoothing cells of a strain-smoothed part and
+// their subdomains (nodal cells: 0 = every element a subcell, 1 = the cell one subcell)
+enum class SmoothingCellType { None, Nodal, Element };
+struct MeshLabelerParameters {
+    SmoothingCellType smoothing_cell_type = SmoothingCellType::None;
+    size_t num_subcells = 1;
+};
 
 // An element block in the caller's numbering: coordinates n x 3, connectivity
 // n_elem x nodes_per_element (0-based).
@@ -391,6 +398,32 @@ class ElementReproducingKernelHip : public ElementBase {
     }
 };
 
+// ElementReproducingKernelHex8 with nodal smoothing cells on a thex mesh (the reference's
+// nodal_smoothing_cell, include/MeshLabelerProcessor.h:227-330, :690-827) and, with several
+// subcells per cell, optionally F-bar (include/ComputeInternalForceSmoothedCell.h:120-413).
+class ElementNodalReproducingKernelHip : public ElementBase {
+   public:
+    ElementNodalReproducingKernelHip(const MeshBlock &mesh, double kernel_radius_scale_factor,
+                                     std::shared_ptr<Device> device, std::shared_ptr<Material> material,
+                                     size_t num_subcells, bool use_f_bar)
+        : ElementBase(8, std::move(device), std::move(material)) {
+        BindMesh(APHIP_HEX8, mesh);
+        Check(aphip_rk_nodal_setup(Context(), kernel_radius_scale_factor, (int32_t)num_subcells, use_f_bar ? 1 : 0));
+    }
+    void ComputeInternalForceAllElements(double time_increment, double time) override {
+        Check(aphip_internal_force(Context(), time, time_increment));
+    }
+    // Nodes carrying degrees of freedom (1) and whether F-bar is applied (only when the
+    // cells hold more than one subcell).
+    bool UsesFBar(std::vector<uint8_t> *active = nullptr) const {
+        int32_t fbar = 0;
+        if (active) active->assign(m_mesh_nodes, 0);
+        Check(aphip_rk_get_nodal_cells(Context(), nullptr, nullptr, nullptr, nullptr,
+                                       active ? active->data() : nullptr, &fbar));
+        return fbar != 0;
+    }
+};
+
 // ElementSmoothedTetrahedron4: finite-element strain smoothing, one cell per Tet4
 // (include/ElementSmoothedTetrahedron4.h; created by include/Element.h:44-47).
 class ElementSmoothedTetrahedron4Hip : public ElementBase {
@@ -413,12 +446,23 @@ inline std::shared_ptr<ElementBase> CreateElement(ElementTopology topology, Appr
                                                   std::shared_ptr<Device> device, std::shared_ptr<Material> material,
                                                   double kernel_radius_scale_factor = 0.0,
                                                   bool use_one_pass_method = true,
-                                                  LagrangianFormulationType formulation = LagrangianFormulationType::Total) {
+                                                  LagrangianFormulationType formulation = LagrangianFormulationType::Total,
+                                                  const MeshLabelerParameters &mesh_labeler = MeshLabelerParameters{},
+                                                  bool use_f_bar = false) {
     if (space == ApproximationSpaceType::ReproducingKernel) {
         if (scheme != IntegrationSchemeType::StrainSmoothing)
             throw std::runtime_error("Gauss Quadrature is not supported for Reproducing Kernel");
         if (formulation != LagrangianFormulationType::Total)
             throw std::runtime_error("the updated formulation applies to the Gaussian elements, not to RKPM cells");
+        if (mesh_labeler.smoothing_cell_type == SmoothingCellType::Nodal) {
+            if (topology != ElementTopology::Hexahedron8)
+                throw std::runtime_error(
+                    "Nodal integration will only work correctly when being built from hexahedral elements. Use "
+                    "'thex' to divide a tetrahedral mesh into hexahedral elements.");
+            return std::make_shared<ElementNodalReproducingKernelHip>(mesh, kernel_radius_scale_factor,
+                                                                      std::move(device), std::move(material),
+                                                                      mesh_labeler.num_subcells, use_f_bar);
+        }
         return std::make_shared<ElementReproducingKernelHip>(mesh, kernel_radius_scale_factor, std::move(device),
                                                              std::move(material), use_one_pass_method, topology);
     }

@@ -308,6 +308,7 @@ static void setup_mesh(Ctx &c, int32_t elem_type, int64_t n_nodes, const double 
     c.etype = elem_type;
     c.npe = npe;
     c.nn = n_nodes;
+    c.n_dof = n_nodes;
     c.ne = n_elem;
     c.elem_seg = elem_seg;
 
@@ -451,6 +452,7 @@ static PowerMethodResult power_method(Ctx &c, const PowerMethodWork &w, double p
                                       double tol, double vtol, uint64_t seed) {
     const int64_t n3 = 3 * c.nn;
     APHIP_REQUIRE(c.formulation == APHIP_TOTAL_LAGRANGIAN, "the power method runs with the total formulation");
+    APHIP_REQUIRE(!(c.cells.ready && c.cells.nodal), "the power method does not run on nodal cells");
     // PowerMethodProcessor::CheckEssentialBoundaries (:159-173)
     APHIP_REQUIRE(!(c.has_fixed && c.all_fixed),
                   "All nodes are in the essential boundary set. Cannot run the power method. Use the "
@@ -755,6 +757,7 @@ int aphip_set_material_fields(aphip_ctx *c, const double *E, const double *nu) {
     APHIP_API_BEGIN
     APHIP_REQUIRE(c != nullptr, "ctx is NULL");
     APHIP_REQUIRE(c->etype >= 0, "mesh not set");
+    APHIP_REQUIRE(!(c->cells.ready && c->cells.nodal), "nodal cells take one material per block");
     APHIP_HIP(hipSetDevice(c->device));
     APHIP_HIP(hipStreamSynchronize(c->stream));
     c->graph_reset();
@@ -862,9 +865,15 @@ int aphip_set_field(aphip_ctx *c, int32_t field, const double *data) {
         const int64_t ns = c->state_count();
         APHIP_REQUIRE(ns > 0, "material has no state");
         const int64_t per = ns / c->ne;
-        std::vector<double> h(ns);
-        for (int64_t e = 0; e < c->ne; ++e)
-            for (int q = 0; q < per; ++q) h[e * per + q] = data[(int64_t)c->elem_iperm[e] * per + q];
+        std::vector<double> h(ns, 0.0);
+        if (c->cells.ready && c->cells.nodal) {  // per subcell: the value of its first element
+            for (int64_t sc = 0; sc < c->cells.ncells; ++sc)
+                for (int q = 0; q < per; ++q)
+                    h[sc * per + q] = data[(int64_t)c->elem_iperm[c->cells.sub_first_elem[sc]] * per + q];
+        } else {
+            for (int64_t e = 0; e < c->ne; ++e)
+                for (int q = 0; q < per; ++q) h[e * per + q] = data[(int64_t)c->elem_iperm[e] * per + q];
+        }
         c->state[c->state_np1].upload(h.data(), h.size(), c->stream);
     } else {
         throw Error(APHIP_ERR_INVALID, "field cannot be set");
@@ -893,21 +902,30 @@ int aphip_get_field(aphip_ctx *c, int32_t field, double *out) {
         APHIP_HIP(hipStreamSynchronize(c->stream));
         for (int64_t i = 0; i < c->nn; ++i) out[c->node_iperm[i]] = h[i];
     } else if (field == APHIP_FIELD_H || field == APHIP_FIELD_P) {
+        // with nodal cells the values live per subcell: every element reads its subcell's
+        // (the reference's PopulateElementOutputs, SmoothedCellDataProcessor.h:676-720)
+        const bool nodal = c->cells.ready && c->cells.nodal;
+        const int64_t stride = nodal ? c->cells.ncells : c->ne;
         const DevBuf<double> &b = field == APHIP_FIELD_H ? c->Hnp1() : c->Pnp1();
         std::vector<double> h(9 * c->ne);
         b.download(h.data(), h.size(), c->stream);
         APHIP_HIP(hipStreamSynchronize(c->stream));
-        for (int64_t e = 0; e < c->ne; ++e)
-            for (int k = 0; k < 9; ++k) out[9 * (int64_t)c->elem_iperm[e] + k] = h[k * c->ne + e];
+        for (int64_t e = 0; e < c->ne; ++e) {
+            const int64_t src = nodal ? c->cells.sub_of_elem[e] : e;
+            for (int k = 0; k < 9; ++k) out[9 * (int64_t)c->elem_iperm[e] + k] = h[k * stride + src];
+        }
     } else if (field == APHIP_FIELD_STATE) {
         const int64_t ns = c->state_count();
         APHIP_REQUIRE(ns > 0, "material has no state");
+        const bool nodal = c->cells.ready && c->cells.nodal;
         const int64_t per = ns / c->ne;
         std::vector<double> h(ns);
         c->state[c->state_np1].download(h.data(), h.size(), c->stream);
         APHIP_HIP(hipStreamSynchronize(c->stream));
-        for (int64_t e = 0; e < c->ne; ++e)
-            for (int q = 0; q < per; ++q) out[(int64_t)c->elem_iperm[e] * per + q] = h[e * per + q];
+        for (int64_t e = 0; e < c->ne; ++e) {
+            const int64_t src = nodal ? c->cells.sub_of_elem[e] : e;
+            for (int q = 0; q < per; ++q) out[(int64_t)c->elem_iperm[e] * per + q] = h[src * per + q];
+        }
     } else if (field == APHIP_FIELD_RK_DISPLACEMENT || field == APHIP_FIELD_RK_FORCE_LOCAL) {
         APHIP_REQUIRE(c->cells.ready && c->cells.two_pass, "field exists with the two-pass RK method only");
         const DevBuf<double> &b = field == APHIP_FIELD_RK_DISPLACEMENT ? c->cells.u_phys : c->cells.f_local;
@@ -917,7 +935,7 @@ int aphip_get_field(aphip_ctx *c, int32_t field, double *out) {
         for (int64_t i = 0; i < c->nn; ++i)
             for (int d = 0; d < 3; ++d) out[3 * (int64_t)c->node_iperm[i] + d] = h[3 * i + d];
     } else if (field == APHIP_FIELD_VOLUME) {
-        const DevBuf<double> &b = c->cells.ready ? c->cells.volume : c->volume;
+        const DevBuf<double> &b = c->cells.ready ? (c->cells.nodal ? c->cells.evol : c->cells.volume) : c->volume;
         std::vector<double> h(c->ne);
         b.download(h.data(), h.size(), c->stream);
         APHIP_HIP(hipStreamSynchronize(c->stream));
@@ -978,6 +996,8 @@ int aphip_set_field_device(aphip_ctx *c, int32_t field, const double *data) {
     } else if (field == APHIP_FIELD_STATE) {
         const int64_t ns = c->state_count();
         APHIP_REQUIRE(ns > 0, "material has no state");
+        APHIP_REQUIRE(!(c->cells.ready && c->cells.nodal),
+                      "element fields of nodal cells are transferred through host arrays");
         require_device_array(*c, data, sizeof(double) * ns);
         field_gather_rows(*c, c->ne, (int)(ns / c->ne), c->d_elem_iperm.p, data, c->state[c->state_np1].p);
     } else {
@@ -990,6 +1010,9 @@ int aphip_get_field_device(aphip_ctx *c, int32_t field, double *out) {
     APHIP_API_BEGIN
     APHIP_REQUIRE(c != nullptr && out != nullptr, "NULL argument");
     APHIP_REQUIRE(c->etype >= 0, "mesh not set");
+    APHIP_REQUIRE(!(c->cells.ready && c->cells.nodal) || field <= APHIP_FIELD_MASS ||
+                      field == APHIP_FIELD_RK_DISPLACEMENT || field == APHIP_FIELD_RK_FORCE_LOCAL,
+                  "element fields of nodal cells are transferred through host arrays");
     APHIP_HIP(hipSetDevice(c->device));
     if (field >= APHIP_FIELD_U && field <= APHIP_FIELD_F) {
         require_device_array(*c, out, sizeof(double) * 3 * c->nn);
@@ -1380,6 +1403,44 @@ int aphip_strain_smoothing_setup(aphip_ctx *c) {
     APHIP_API_END
 }
 
+int aphip_rk_nodal_setup(aphip_ctx *c, double scale, int32_t subdomains, int32_t use_f_bar) {
+    APHIP_API_BEGIN
+    APHIP_REQUIRE(c != nullptr, "ctx is NULL");
+    require_ready(*c, false);
+    APHIP_REQUIRE(c->mat.state_per_qp == 0, "RKPM cells keep one material state per cell, not per Gauss point");
+    APHIP_REQUIRE(c->formulation == APHIP_TOTAL_LAGRANGIAN,
+                  "the updated formulation applies to the Gaussian elements, not to RKPM cells");
+    APHIP_REQUIRE(!c->has_fields, "nodal cells take one material per block");
+    APHIP_REQUIRE(scale >= 0.0, "Kernel radius scale factor must be non-negative.");
+    APHIP_REQUIRE(!use_f_bar || c->mat.kind == APHIP_LINEAR_ELASTIC || c->mat.kind == APHIP_NEO_HOOKEAN ||
+                      c->mat.kind == APHIP_J2 || c->mat.kind == APHIP_SVK,
+                  "F-bar takes the linear elastic, neo-Hookean, J2 and SVK materials");
+    APHIP_HIP(hipSetDevice(c->device));
+    APHIP_HIP(hipStreamSynchronize(c->stream));
+    c->graph_reset();
+    rk_setup(*c, scale, false, false, subdomains, use_f_bar != 0);
+    c->has_mass = false;
+    APHIP_HIP(hipStreamSynchronize(c->stream));
+    APHIP_API_END
+}
+
+int aphip_rk_get_nodal_cells(aphip_ctx *c, int64_t *n_cells, int64_t *n_subcells, int64_t *cell_subcells,
+                             int32_t *element_subcell, uint8_t *active, int32_t *f_bar) {
+    APHIP_API_BEGIN
+    APHIP_REQUIRE(c != nullptr, "ctx is NULL");
+    APHIP_REQUIRE(c->cells.ready && c->cells.nodal, "aphip_rk_nodal_setup has not run");
+    const CellData &D = c->cells;
+    if (n_cells) *n_cells = D.n_fcells;
+    if (n_subcells) *n_subcells = D.ncells;
+    if (f_bar) *f_bar = D.fbar ? 1 : 0;
+    if (cell_subcells) std::memcpy(cell_subcells, D.fcell_sub_h.data(), sizeof(int64_t) * (D.n_fcells + 1));
+    if (element_subcell)
+        for (int64_t e = 0; e < c->ne; ++e) element_subcell[c->elem_iperm[e]] = D.sub_of_elem[e];
+    if (active)
+        for (int64_t i = 0; i < c->nn; ++i) active[c->node_iperm[i]] = i < c->n_dof ? 1 : 0;
+    APHIP_API_END
+}
+
 int aphip_rk_get_neighbors(aphip_ctx *c, int32_t *num, int32_t *nbr, int64_t *n_truncated) {
     APHIP_API_BEGIN
     APHIP_REQUIRE(c != nullptr, "ctx is NULL");
@@ -1424,6 +1485,20 @@ int aphip_rk_get_cells(aphip_ctx *c, int64_t *nnz, int64_t *start, int32_t *node
     APHIP_HIP(hipSetDevice(c->device));
     if (nnz) *nnz = c->cells.nnz;
     if (!start && !nodes && !dphi && !volume) return APHIP_OK;
+    if (c->cells.nodal) {  // subcells, in their own order (aphip_rk_get_nodal_cells)
+        const int64_t ns = c->cells.ncells;
+        std::vector<int64_t> hs(ns + 1);
+        std::vector<int32_t> hn(c->cells.nnz);
+        c->cells.start.download(hs.data(), hs.size(), c->stream);
+        c->cells.nodes.download(hn.data(), hn.size(), c->stream);
+        if (dphi) c->cells.dphi.download(dphi, 3 * c->cells.nnz, c->stream);
+        if (volume) c->cells.volume.download(volume, ns, c->stream);
+        APHIP_HIP(hipStreamSynchronize(c->stream));
+        if (start) std::memcpy(start, hs.data(), sizeof(int64_t) * (ns + 1));
+        if (nodes)
+            for (int64_t t = 0; t < c->cells.nnz; ++t) nodes[t] = c->node_iperm[hn[t]];
+        return APHIP_OK;
+    }
     std::vector<int64_t> hs(c->ne + 1);
     std::vector<int32_t> hn(c->cells.nnz);
     std::vector<double> hd(3 * c->cells.nnz), hv(c->ne);

@@ -391,9 +391,10 @@ static NodeArgs node_args(Ctx &c, double dt) {
 }
 
 void fem_predict(Ctx &c, double dt) {
-    if (c.nn == 0) return;
+    if (c.n_dof == 0) return;
     NodeArgs A = node_args(c, dt);
-    const dim3 grid((unsigned)((c.nn + kNodeBlock - 1) / kNodeBlock));
+    A.nn = c.n_dof;  // nodes without degrees of freedom (last in the numbering) keep their values
+    const dim3 grid((unsigned)((c.n_dof + kNodeBlock - 1) / kNodeBlock));
     if (c.formulation != APHIP_TOTAL_LAGRANGIAN) hipLaunchKernelGGL(k_predict_ul, grid, dim3(kNodeBlock), 0, c.stream, A);
     else hipLaunchKernelGGL(k_predict, grid, dim3(kNodeBlock), 0, c.stream, A);
     APHIP_HIP(hipGetLastError());

@@ -139,6 +139,20 @@ struct CellData {
     DevBuf<int64_t> rstart;     // [nn + 1] reverse neighbor lists: points i that list node J
     DevBuf<int32_t> rsrc;       // [nrev] i, ascending per node
     DevBuf<double> rphi;        // [nrev] phi_iJ
+    int64_t ncells = 0;         // cells of the CSR above: the elements, or the subcells of nodal cells
+    // nodal smoothing cells on a thex mesh: the CSR cells above are subcells, grouped into
+    // one cell per active node; nodes with degrees of freedom come first (Ctx::n_dof)
+    bool nodal = false;
+    bool fbar = false;              // F-bar over the subcells of each cell
+    int64_t n_fcells = 0;           // nodal cells
+    DevBuf<int64_t> fcell_sub;      // [n_fcells + 1] subcell range of each nodal cell
+    DevBuf<double> fcell_volume;    // [n_fcells]
+    DevBuf<double> Jsub;            // [ncells] F-bar: J of each subcell
+    DevBuf<double> Pbar;            // [ncells * 9] F-bar: stress of H-bar, row-major
+    DevBuf<double> evol;            // [ne] smoothed volume of every element (nodal cells)
+    std::vector<int32_t> sub_of_elem;     // [ne] subcell of each internal element (nodal cells)
+    std::vector<int32_t> sub_first_elem;  // [ncells] first internal element of each subcell
+    std::vector<int64_t> fcell_sub_h;     // host copy of fcell_sub
     // two-pass method: cells list the element's own nodes; the cell force acts on
     // u_phys = Phi d and f = Phi^T f_local
     bool two_pass = false;
@@ -150,6 +164,10 @@ struct CellData {
         rstart.release(); rsrc.release(); rphi.release(); u_phys.release(); f_local.release();
         ent_cell.release(); dphi_node.release(); Paos.release();
         nnz = n_truncated = 0;
+        ncells = n_fcells = 0;
+        nodal = fbar = false;
+        fcell_sub.release(); fcell_volume.release(); Jsub.release(); Pbar.release(); evol.release();
+        sub_of_elem.clear(); sub_first_elem.clear(); fcell_sub_h.clear();
         num_nbr.release(); nbr.release(); phi.release(); radius.release(); start.release();
         nodes.release(); dphi.release(); volume.release(); inc_cell.release(); node_start.release();
         node_inc.release(); entry_pos.release(); g.release();
@@ -227,6 +245,7 @@ struct Ctx {
     int etype = -1;
     int npe = 0;
     int64_t nn = 0, ne = 0;
+    int64_t n_dof = 0;                // nodes with degrees of freedom: internal ids 0 .. n_dof - 1
     std::vector<int32_t> node_perm;   // caller -> internal
     std::vector<int32_t> node_iperm;  // internal -> caller
     std::vector<int32_t> elem_perm;   // caller -> internal
@@ -335,7 +354,10 @@ void force_vjp(Ctx &c, const double *f_bar, double *u_bar, double *E_bar, double
 void step_once_tiles(Ctx &c, double dt);
 
 // ---- RK (rk.hip) ----
-void rk_setup(Ctx &c, double scale, bool two_pass, bool fe_cells = false);
+// nodal_subdomains >= 0: nodal smoothing cells of a thex mesh (0: every hex a subcell,
+// 1: every cell one subcell), with F-bar over the subcells when fbar
+void rk_setup(Ctx &c, double scale, bool two_pass, bool fe_cells = false, int nodal_subdomains = -1,
+              bool fbar = false);
 void rk_cell_force(Ctx &c);
 void rk_node_update(Ctx &c, double dt, int mode);
 void rk_mass(Ctx &c, double *total);

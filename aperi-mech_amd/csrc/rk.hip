@@ -22,6 +22,18 @@
 //                       stress; the stress term -V P^T of :325-326 is kept per cell)
 //   node kernel         :328-349 as a node gather over the cell entries that name the node, in ascending
 //                       (cell, entry) order, fused with the central-difference node update
+// Nodal smoothing cells on a thex mesh (each tet split into four hexes):
+//   active nodes        include/MeshLabelerProcessor.h:227-330 (each hex's node of lowest
+//                       caller id; exactly one per hex), cells and subcells :690-827 (one cell
+//                       per active node: the hexes around it; subdomains 0: every hex a
+//                       subcell, 1: the cell one subcell); internal nodes are renumbered with
+//                       the active ones first, so nodes with degrees of freedom are a prefix
+//   max edge length     edges to inactive nodes doubled (src/MaxEdgeLengthProcessor.cpp:54-55)
+//   search              query points: every node; candidates: the active nodes
+//   subcell CSR         include/SmoothedCellDataProcessor.h:407-675 over the hexes of a subcell
+//   F-bar force         include/ComputeInternalForceSmoothedCell.h:120-413 (subcell J, cell
+//                       J-bar, stress of F-bar, dual pressure, un-barred stress), used when a
+//                       cell has more subcells than one (:458-459)
 // Per step, two-pass method (src/Solvers/Explicit.cpp:148-186):
 //   gather kernel       FunctionEvaluationProcessor::ComputeValues (include/FunctionEvaluationProcessor.h:
 //                       70-117): physical displacement = Phi d before the cell kernel
@@ -66,7 +78,8 @@ __device__ __forceinline__ double dist2(const double *a, const double *b) {
 
 template <int NPE>
 __global__ __launch_bounds__(256) void k_max_edge(int64_t nn, const int32_t *node_start, const int32_t *node_inc,
-                                                  const int32_t *conn, const double *X, double scale, double *radius) {
+                                                  const int32_t *conn, const double *X, double scale, int64_t n_active,
+                                                  double *radius) {
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (i >= nn) return;
     double xi[3];
@@ -77,8 +90,10 @@ __global__ __launch_bounds__(256) void k_max_edge(int64_t nn, const int32_t *nod
 #pragma unroll
         for (int k = 0; k < NPE; ++k) {
             double xj[3];
-            ld3(X, conn[NPE * e + k], xj);
-            l = fmax(l, sqrt(dist2(xi, xj)));
+            const int32_t j = conn[NPE * e + k];
+            ld3(X, j, xj);
+            // an edge to a node without degrees of freedom counts twice (nodes j >= n_active)
+            l = fmax(l, sqrt(dist2(xi, xj)) * (j >= n_active ? 2.0 : 1.0));
         }
     }
     radius[i] = l * scale;
@@ -387,7 +402,7 @@ __global__ __launch_bounds__(cell_block<NPE>()) void k_cell_count(int64_t ne, co
 }
 
 // Smoothed Tet4 B and volume from the faces (include/QuadratureSmoothed.h:32-62)
-__device__ void smoothed_tet4(const double (&X)[4][3], double (&B)[4][3], double &V) {
+__host__ __device__ void smoothed_tet4(const double (&X)[4][3], double (&B)[4][3], double &V) {
     const int faces[4][3] = {{0, 1, 3}, {1, 2, 3}, {0, 3, 2}, {0, 2, 1}};
     double vol = 0.0;
 #pragma unroll
@@ -420,7 +435,7 @@ __device__ void smoothed_tet4(const double (&X)[4][3], double (&B)[4][3], double
 
 // Smoothed Hex8 B and volume from the faces, each split into four triangles about
 // its centroid (include/QuadratureSmoothed.h:96-150)
-__device__ void smoothed_hex8(const double (&X)[8][3], double (&B)[8][3], double &V) {
+__host__ __device__ void smoothed_hex8(const double (&X)[8][3], double (&B)[8][3], double &V) {
     const int faces[6][4] = {{0, 1, 5, 4}, {1, 2, 6, 5}, {2, 3, 7, 6}, {0, 4, 7, 3}, {0, 3, 2, 1}, {4, 5, 6, 7}};
     double vol = 0.0;
 #pragma unroll
@@ -464,7 +479,7 @@ __device__ void smoothed_hex8(const double (&X)[8][3], double (&B)[8][3], double
 }
 
 template <int NPE>
-__device__ __forceinline__ void smoothed_cell(const double (&X)[NPE][3], double (&B)[NPE][3], double &V) {
+__host__ __device__ __forceinline__ void smoothed_cell(const double (&X)[NPE][3], double (&B)[NPE][3], double &V) {
     if constexpr (NPE == 4) smoothed_tet4(X, B, V);
     else smoothed_hex8(X, B, V);
 }
@@ -732,6 +747,132 @@ __global__ __launch_bounds__(256) void k_rk_cells(CellForceArgs A) {
             for (int i = 0; i < 9; ++i) pl = q == i ? P[i] : pl;
             A.Paos[kRowLen * c + q] = pl;
         }
+    }
+}
+
+// F-bar over nodal cells (ComputeInternalForceSmoothedCell.h:120-413), first pass: the
+// subcell displacement gradient (the one-pass cell accumulation, LANES lanes per subcell)
+// into the H planes and J = DetApIm1(H) + 1 (:186-224).
+template <int LANES>
+__global__ __launch_bounds__(256) void k_fbar_subcells(CellForceArgs A, double *J) {
+#pragma clang fp contract(off)
+    const int l = threadIdx.x & (LANES - 1);
+    const int64_t c_raw = xcd_block(A.nblocks) * (256 / LANES) + (threadIdx.x / LANES);
+    const bool live = c_raw < A.ne;
+    const int64_t c = live ? c_raw : A.ne - 1;
+    const int64_t t0 = A.start[c], t1 = A.start[c + 1];
+    double H[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int64_t t = t0 + l; t < t1; t += LANES) {
+        double uj[3], g[3];
+        ld3(A.u, A.nodes[t], uj);
+        ld3(A.dphi, t, g);
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int b = 0; b < 3; ++b) H[3 * a + b] = __fma_rn(uj[a], g[b], H[3 * a + b]);
+    }
+#pragma unroll
+    for (int i = 0; i < 9; ++i) H[i] = group_sum<LANES>(H[i]);
+    if (live && l == 0) {
+#pragma unroll
+        for (int i = 0; i < 9; ++i) A.H[i * A.ne + c] = H[i];
+        J[c] = det_ap_im1(H) + 1.0;
+    }
+}
+
+struct FbarArgs {
+    int64_t n_cells;               // nodal cells
+    int64_t ns;                    // subcells (the plane stride of H and P)
+    const int64_t *cell_sub;       // [n_cells + 1]
+    const double *cell_volume;     // [n_cells]
+    const double *sub_volume;      // [ns]
+    const double *J;               // [ns]
+    const double *H;               // [9 x ns] planes
+    double *P;                     // [9 x ns] planes: the un-barred stress
+    double *Pbar;                  // [ns x 9] row-major: the stress of H-bar
+    double *Paos;                  // [ns x kRowLen] rows of the node kernel (P, -V)
+    const double *state_n;
+    double *state_np1;
+    MatParams mat;
+};
+
+// Second pass, one thread per nodal cell: J-bar = sum J_s V_s / V_cell (ComputeFbar,
+// :120-175), H-bar = H cbrt(J-bar / J_s) + (cbrt(.) - 1) I unless the ratio is within
+// 3 eps of 1, the stress of H-bar (:177-213), the dual pressure and the un-barred
+// stress P = P-bar s + (p - s / (3 J_s) P-bar : F) J_s F^-T (UnBarStress, :215-309), and
+// the node kernel's row (P, -V_s) per subcell.
+template <int KIND>
+__global__ __launch_bounds__(256) void k_fbar_cells(FbarArgs A) {
+#pragma clang fp contract(off)
+    constexpr double kEps = 2.2204460492503131e-16;
+    const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (r >= A.n_cells) return;
+    const int64_t s0 = A.cell_sub[r], s1 = A.cell_sub[r + 1];
+    const double vc = A.cell_volume[r];
+    double jbar = 0.0;
+    for (int64_t s = s0; s < s1; ++s) jbar = jbar + A.J[s] * A.sub_volume[s];
+    jbar = jbar / vc;
+    for (int64_t s = s0; s < s1; ++s) {
+        double H[9], Hb[9], Pb[9];
+#pragma unroll
+        for (int i = 0; i < 9; ++i) H[i] = A.H[i * A.ns + s];
+        const double ratio = jbar / A.J[s];
+        if (fabs(ratio - 1.0) < 3.0 * kEps) {
+#pragma unroll
+            for (int i = 0; i < 9; ++i) Hb[i] = H[i];
+        } else {
+            const double js = cbrt(ratio);
+#pragma unroll
+            for (int i = 0; i < 9; ++i) Hb[i] = H[i] * js;
+            Hb[0] += js - 1.0;
+            Hb[4] += js - 1.0;
+            Hb[8] += js - 1.0;
+        }
+        double snp1 = 0.0, aux = 0.0;
+        const double sn = load_state<KIND>(A.state_n, s);
+        stress<KIND>(Hb, A.mat, sn, snp1, aux, Pb);
+        store_state<KIND>(A.state_np1, s, snp1, aux);
+#pragma unroll
+        for (int i = 0; i < 9; ++i) A.Pbar[9 * s + i] = Pb[i];
+    }
+    double dual = 0.0;
+    for (int64_t s = s0; s < s1; ++s) {
+        const double js = cbrt(jbar / A.J[s]);
+        const double djb = js / (3.0 * jbar);
+        double ph = 0.0;
+#pragma unroll
+        for (int i = 0; i < 9; ++i) ph += A.Pbar[9 * s + i] * A.H[i * A.ns + s];
+        const double tr = A.Pbar[9 * s] + A.Pbar[9 * s + 4] + A.Pbar[9 * s + 8];
+        dual += djb * A.sub_volume[s] * (ph + tr);
+    }
+    dual = dual / vc;
+    for (int64_t s = s0; s < s1; ++s) {
+        const double Js = A.J[s];
+        const double js = cbrt(jbar / Js);
+        double F[9], Fi[9], Pb[9], pf = 0.0;
+#pragma unroll
+        for (int i = 0; i < 9; ++i) {
+            F[i] = A.H[i * A.ns + s];
+            Pb[i] = A.Pbar[9 * s + i];
+        }
+        F[0] += 1.0;
+        F[4] += 1.0;
+        F[8] += 1.0;
+#pragma unroll
+        for (int i = 0; i < 9; ++i) pf += Pb[i] * F[i];
+        const double djj = -js / (3.0 * Js);
+        const double scalar = (dual + djj * pf) * Js;
+        inverse3(F, Fi);
+        double *row = A.Paos + kRowLen * s;
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                const double p = Pb[3 * i + j] * js + scalar * Fi[3 * j + i];
+                A.P[(3 * i + j) * A.ns + s] = p;
+                row[3 * i + j] = p;
+            }
+        row[9] = -A.sub_volume[s];
     }
 }
 
@@ -1055,25 +1196,195 @@ void scan_counts(Ctx &c, const DevBuf<int32_t> &counts, int64_t n, DevBuf<T> &st
 
 void fe_smoothing_setup(Ctx &c);
 void rk_cells_and_order(Ctx &c, bool two_pass, bool elem_cells);
+void rk_cells_order(Ctx &c, bool two_pass);
 
-void rk_setup(Ctx &c, double scale, bool two_pass, bool fe_cells) {
+// Nodal cells of a thex mesh (MeshLabelerProcessor.h:227-330, :690-827): marks each hex's
+// node of lowest caller id active, checks that every hex then holds exactly one active
+// node, renumbers the internal nodes with the active ones first (c.n_dof of them) and
+// records, per cell (active node, ascending caller id) its hexes in ascending caller
+// element id; with subdomains 0 every hex is a subcell, with 1 the whole cell.  Returns
+// the subcells as element lists (sub_start / sub_elems, internal element ids).
+static void nodal_label(Ctx &c, int subdomains, std::vector<int64_t> &sub_start, std::vector<int32_t> &sub_elems) {
+    CellData &D = c.cells;
+    const int64_t nn = c.nn, ne = c.ne;
+    APHIP_REQUIRE(c.etype == APHIP_HEX8,
+                  "Nodal integration will only work correctly when being built from hexahedral elements. Use 'thex' "
+                  "to divide a tetrahedral mesh into hexahedral elements.");
+    APHIP_REQUIRE(subdomains == 0 || subdomains == 1, "nodal smoothing cells take subdomains 0 or 1");
+    std::vector<uint8_t> act(nn, 0);
+    for (int64_t e = 0; e < ne; ++e) {
+        int best = 0;
+        for (int k = 1; k < 8; ++k)
+            if (c.node_iperm[c.conn_h[8 * e + k]] < c.node_iperm[c.conn_h[8 * e + best]]) best = k;
+        act[c.conn_h[8 * e + best]] = 1;
+    }
+    for (int64_t e = 0; e < ne; ++e) {
+        int n = 0;
+        for (int k = 0; k < 8; ++k) n += act[c.conn_h[8 * e + k]];
+        APHIP_REQUIRE(n == 1, "Nodal integration requires exactly one active node per element. Found " +
+                                  std::to_string(n) + " active nodes in element " +
+                                  std::to_string(c.elem_iperm[e]) + ".");
+    }
+    // active nodes first, each group in its current (Morton) order
+    std::vector<int32_t> old_of_new;
+    old_of_new.reserve(nn);
+    for (int64_t i = 0; i < nn; ++i)
+        if (act[i]) old_of_new.push_back((int32_t)i);
+    const int64_t n_act = (int64_t)old_of_new.size();
+    for (int64_t i = 0; i < nn; ++i)
+        if (!act[i]) old_of_new.push_back((int32_t)i);
+    renumber_nodes(c, old_of_new);
+    c.tiles.reset();  // the tile plan's node groups are void after the renumbering
+    c.n_dof = n_act;
+    // cells: active nodes by caller id; hexes by caller element id
+    std::vector<int32_t> act_nodes(n_act);
+    std::iota(act_nodes.begin(), act_nodes.end(), 0);
+    std::sort(act_nodes.begin(), act_nodes.end(),
+              [&](int32_t x, int32_t y) { return c.node_iperm[x] < c.node_iperm[y]; });
+    std::vector<int32_t> cell_of_node(n_act);
+    for (int64_t r = 0; r < n_act; ++r) cell_of_node[act_nodes[r]] = (int32_t)r;
+    std::vector<int32_t> hexes(ne);
+    std::iota(hexes.begin(), hexes.end(), 0);
+    std::vector<int32_t> hex_cell(ne);
+    for (int64_t e = 0; e < ne; ++e)
+        for (int k = 0; k < 8; ++k)
+            if (c.conn_h[8 * e + k] < n_act) hex_cell[e] = cell_of_node[c.conn_h[8 * e + k]];
+    std::sort(hexes.begin(), hexes.end(), [&](int32_t x, int32_t y) {
+        return hex_cell[x] != hex_cell[y] ? hex_cell[x] < hex_cell[y] : c.elem_iperm[x] < c.elem_iperm[y];
+    });
+    D.n_fcells = n_act;
+    std::vector<int64_t> cell_start(n_act + 1, 0);
+    for (int64_t e = 0; e < ne; ++e) cell_start[hex_cell[e] + 1]++;
+    for (int64_t r = 0; r < n_act; ++r) cell_start[r + 1] += cell_start[r];
+    sub_elems = hexes;
+    if (subdomains == 0) {
+        sub_start.resize(ne + 1);
+        std::iota(sub_start.begin(), sub_start.end(), 0);
+        D.fcell_sub_h = cell_start;
+    } else {
+        sub_start = cell_start;
+        D.fcell_sub_h.resize(n_act + 1);
+        std::iota(D.fcell_sub_h.begin(), D.fcell_sub_h.end(), 0);
+    }
+}
+
+// Subcell node lists (first-seen union of the neighbors of the nodes of the subcell's
+// hexes, in hex order) and derivatives sum_e sum_k B_k V_e phi / V_sub
+// (SmoothedCellDataProcessor.h:407-675), built on the host from the device's neighbor
+// lists and functions; cell volumes are the sums of their hexes' volumes.
+static void nodal_subcells(Ctx &c, const std::vector<int64_t> &sub_start, const std::vector<int32_t> &sub_elems) {
+    CellData &D = c.cells;
+    const int64_t nn = c.nn, ne = c.ne, ns = (int64_t)sub_start.size() - 1;
+    std::vector<int32_t> hn(nn), hb(nn * kMaxNbr);
+    std::vector<double> hp(nn * kMaxNbr), hX(3 * nn);
+    D.num_nbr.download(hn.data(), nn, c.stream);
+    D.nbr.download(hb.data(), hb.size(), c.stream);
+    D.phi.download(hp.data(), hp.size(), c.stream);
+    c.X.download(hX.data(), hX.size(), c.stream);
+    APHIP_HIP(hipStreamSynchronize(c.stream));
+    std::vector<int64_t> start(ns + 1, 0);
+    std::vector<int32_t> nodes, inc_cell;
+    std::vector<double> dphi, vsub(ns, 0.0), evol(ne, 0.0);
+    std::vector<int64_t> stamp(nn, -1);
+    std::vector<int32_t> pos(nn, 0);
+    D.sub_of_elem.assign(ne, -1);
+    D.sub_first_elem.assign(ns, -1);
+    for (int64_t sc = 0; sc < ns; ++sc) {
+        const int64_t t0 = (int64_t)nodes.size();
+        start[sc] = t0;
+        for (int64_t q = sub_start[sc]; q < sub_start[sc + 1]; ++q) {
+            const int32_t e = sub_elems[q];
+            D.sub_of_elem[e] = (int32_t)sc;
+            if (q == sub_start[sc]) D.sub_first_elem[sc] = e;
+            for (int k = 0; k < 8; ++k) {
+                const int32_t a = c.conn_h[8 * (int64_t)e + k];
+                for (int l = 0; l < hn[a]; ++l) {
+                    const int32_t j = hb[(int64_t)a * kMaxNbr + l];
+                    if (stamp[j] != sc) {
+                        stamp[j] = sc;
+                        pos[j] = (int32_t)((int64_t)nodes.size() - t0);
+                        nodes.push_back(j);
+                        inc_cell.push_back((int32_t)sc);
+                    }
+                }
+            }
+        }
+        dphi.resize(3 * nodes.size(), 0.0);
+        double vs = 0.0;
+        for (int64_t q = sub_start[sc]; q < sub_start[sc + 1]; ++q) {
+            const int32_t e = sub_elems[q];
+            double Xe[8][3], B[8][3], V;
+            for (int k = 0; k < 8; ++k)
+                for (int d = 0; d < 3; ++d) Xe[k][d] = hX[3 * (int64_t)c.conn_h[8 * (int64_t)e + k] + d];
+            smoothed_hex8(Xe, B, V);
+            evol[e] = V;
+            vs += V;
+            for (int k = 0; k < 8; ++k) {
+                const int32_t a = c.conn_h[8 * (int64_t)e + k];
+                for (int l = 0; l < hn[a]; ++l) {
+                    const int32_t j = hb[(int64_t)a * kMaxNbr + l];
+                    const double fv = hp[(int64_t)a * kMaxNbr + l];
+                    double *dp = &dphi[3 * (t0 + pos[j])];
+                    for (int m = 0; m < 3; ++m) dp[m] += B[k][m] * V * fv;
+                }
+            }
+        }
+        for (size_t t = 3 * t0; t < dphi.size(); ++t) dphi[t] /= vs;
+        vsub[sc] = vs;
+    }
+    start[ns] = (int64_t)nodes.size();
+    D.ncells = ns;
+    D.nnz = start[ns];
+    APHIP_REQUIRE(D.nnz < (int64_t)INT32_MAX, "too many cell entries for int32 incidences");
+    D.start.upload(start.data(), start.size(), c.stream);
+    D.nodes.upload(nodes.data(), std::max<size_t>(nodes.size(), 1), c.stream);
+    D.inc_cell.upload(inc_cell.data(), std::max<size_t>(inc_cell.size(), 1), c.stream);
+    D.dphi.upload(dphi.data(), std::max<size_t>(dphi.size(), 1), c.stream);
+    D.volume.upload(vsub.data(), std::max<int64_t>(ns, 1), c.stream);
+    D.evol.upload(evol.data(), std::max<int64_t>(ne, 1), c.stream);
+    // nodal cells: subcell ranges and volumes (sum of the cell's hex volumes, in order)
+    std::vector<double> vcell(D.n_fcells, 0.0);
+    for (int64_t r = 0; r < D.n_fcells; ++r)
+        for (int64_t sc = D.fcell_sub_h[r]; sc < D.fcell_sub_h[r + 1]; ++sc)
+            for (int64_t q = sub_start[sc]; q < sub_start[sc + 1]; ++q) vcell[r] += evol[sub_elems[q]];
+    D.fcell_sub.upload(D.fcell_sub_h.data(), D.fcell_sub_h.size(), c.stream);
+    D.fcell_volume.upload(vcell.data(), std::max<int64_t>(D.n_fcells, 1), c.stream);
+    if (D.fbar) {
+        D.Jsub.alloc(std::max<int64_t>(ns, 1));
+        D.Pbar.alloc(9 * std::max<int64_t>(ns, 1));
+    }
+    APHIP_HIP(hipStreamSynchronize(c.stream));
+}
+
+void rk_setup(Ctx &c, double scale, bool two_pass, bool fe_cells, int nodal_subdomains, bool fbar) {
     CellData &D = c.cells;
     D.reset();
+    c.n_dof = c.nn;
     const int64_t nn = c.nn;
     if (fe_cells) {
         fe_smoothing_setup(c);
         return;
     }
     if (scale == 1.0) scale += 1.0e-6;  // src/NeighborSearchProcessor.cpp:161-163
+    const bool nodal = nodal_subdomains >= 0;
+    std::vector<int64_t> sub_start;
+    std::vector<int32_t> sub_elems;
+    if (nodal) {
+        nodal_label(c, nodal_subdomains, sub_start, sub_elems);
+        D.nodal = true;
+        // F-bar only where a cell holds more than one subcell (ComputeInternalForceSmoothedCell.h:458-459)
+        D.fbar = fbar && (int64_t)sub_start.size() - 1 != D.n_fcells;
+    }
+    const int64_t n_act = c.n_dof;  // search candidates: the nodes with degrees of freedom
 
     // kernel radius from the max edge length
     D.radius.alloc(nn);
     const bool hex = c.etype == APHIP_HEX8;  // Hex8 background: ElementReproducingKernelHex8
-    hipLaunchKernelGGL(hex ? k_max_edge<8> : k_max_edge<4>, grid1(nn, 256), dim3(256), 0, c.stream, nn, c.node_start.p, c.node_inc.p, c.conn.p,
-                       c.X.p, scale, D.radius.p);
+    hipLaunchKernelGGL(hex ? k_max_edge<8> : k_max_edge<4>, grid1(nn, 256), dim3(256), 0, c.stream, nn, c.node_start.p,
+                       c.node_inc.p, c.conn.p, c.X.p, scale, n_act, D.radius.p);
     APHIP_HIP(hipGetLastError());
 
-    // uniform cell list with cell edge = largest radius
+    // uniform cell list of the candidates with cell edge = their largest radius
     std::vector<double> hX(3 * nn), hr(nn);
     c.X.download(hX.data(), hX.size(), c.stream);
     D.radius.download(hr.data(), hr.size(), c.stream);
@@ -1090,7 +1401,7 @@ void rk_setup(Ctx &c, double scale, bool two_pass, bool fe_cells) {
             G.lo[d] = std::min(G.lo[d], hX[3 * i + d]);
             hi[d] = std::max(hi[d], hX[3 * i + d]);
         }
-        rmax = std::max(rmax, hr[i]);
+        if (i < n_act) rmax = std::max(rmax, hr[i]);
     }
     double ext = 0.0;
     for (int d = 0; d < 3; ++d) ext = std::max(ext, hi[d] - G.lo[d]);
@@ -1107,25 +1418,25 @@ void rk_setup(Ctx &c, double scale, bool two_pass, bool fe_cells) {
     }
     const int64_t nbins = (int64_t)G.nb[0] * G.nb[1] * G.nb[2];
     DevBuf<int32_t> bin_of, bcount, bstart, bfill, items;
-    bin_of.alloc(nn);
+    bin_of.alloc(std::max<int64_t>(n_act, 1));
     bcount.alloc(nbins);
     bcount.zero(c.stream);
-    hipLaunchKernelGGL(k_bin_count, grid1(nn, 256), dim3(256), 0, c.stream, nn, c.X.p, G, bin_of.p, bcount.p);
+    hipLaunchKernelGGL(k_bin_count, grid1(n_act, 256), dim3(256), 0, c.stream, n_act, c.X.p, G, bin_of.p, bcount.p);
     APHIP_HIP(hipGetLastError());
     scan_counts(c, bcount, nbins, bstart, nullptr);
     bfill.alloc(nbins);
     bfill.zero(c.stream);
-    items.alloc(nn);
-    hipLaunchKernelGGL(k_bin_fill, grid1(nn, 256), dim3(256), 0, c.stream, nn, bin_of.p, bstart.p, bfill.p, items.p);
+    items.alloc(std::max<int64_t>(n_act, 1));
+    hipLaunchKernelGGL(k_bin_fill, grid1(n_act, 256), dim3(256), 0, c.stream, n_act, bin_of.p, bstart.p, bfill.p,
+                       items.p);
     APHIP_HIP(hipGetLastError());
 
-    // ball search, ranked by caller ids for ties
-    DevBuf<int32_t> caller, ncand;
-    caller.upload(c.node_iperm.data(), nn, c.stream);
+    // ball search of every node among the candidates, ranked by caller ids for ties
+    DevBuf<int32_t> ncand;
     D.num_nbr.alloc(nn);
     D.nbr.alloc(nn * kMaxNbr);
     ncand.alloc(nn);
-    SearchArgs S{nn, c.X.p, D.radius.p, caller.p, G, bstart.p, items.p, D.num_nbr.p, D.nbr.p, ncand.p};
+    SearchArgs S{nn, c.X.p, D.radius.p, c.d_node_iperm.p, G, bstart.p, items.p, D.num_nbr.p, D.nbr.p, ncand.p};
     hipLaunchKernelGGL(k_search, grid1(nn, kSearchBlock), dim3(kSearchBlock), 0, c.stream, S);
     APHIP_HIP(hipGetLastError());
     {
@@ -1144,7 +1455,12 @@ void rk_setup(Ctx &c, double scale, bool two_pass, bool fe_cells) {
                        D.phi.p);
     APHIP_HIP(hipGetLastError());
 
-    rk_cells_and_order(c, two_pass, two_pass);
+    if (nodal) {
+        nodal_subcells(c, sub_start, sub_elems);
+        rk_cells_order(c, false);
+    } else {
+        rk_cells_and_order(c, two_pass, two_pass);
+    }
 }
 
 // Finite-element strain smoothing with element cells (ElementSmoothedTetrahedron4,
@@ -1222,6 +1538,14 @@ void rk_cells_and_order(Ctx &c, bool two_pass, bool elem_cells) {
         }
     }
 
+    D.ncells = ne;
+    rk_cells_order(c, two_pass);
+}
+
+// Reverse neighbor lists and the node order of the cell entries (after the cell CSR).
+void rk_cells_order(Ctx &c, bool two_pass) {
+    CellData &D = c.cells;
+    const int64_t nn = c.nn;
     // reverse neighbor lists: for node J the points i that list it, ascending i,
     // with phi_iJ (the mass scatter and the two-pass force scatter)
     {
@@ -1281,7 +1605,7 @@ void rk_cells_and_order(Ctx &c, bool two_pass, bool elem_cells) {
         }
     }
     if (two_pass) D.g.alloc(3 * std::max<int64_t>(D.nnz, 1));
-    else D.Paos.alloc(kRowLen * std::max<int64_t>(ne, 1));
+    else D.Paos.alloc(kRowLen * std::max<int64_t>(D.ncells, 1));
     APHIP_HIP(hipStreamSynchronize(c.stream));
     D.ready = true;
 }
@@ -1294,9 +1618,9 @@ void rk_cell_force(Ctx &c) {
         hipLaunchKernelGGL(k_rk_gather, dim3((unsigned)nbg), dim3(256), 0, c.stream, G);
         APHIP_HIP(hipGetLastError());
     }
-    if (c.ne == 0) return;
-    const int64_t nbc = padded_groups(c.ne, D.two_pass ? 1 : kCellLanes);
-    CellForceArgs A{c.ne, nbc, D.start.p, D.nodes.p, D.dphi.p, D.volume.p, D.two_pass ? D.u_phys.p : c.u.p,
+    if (D.ncells == 0) return;
+    const int64_t nbc = padded_groups(D.ncells, D.two_pass ? 1 : kCellLanes);
+    CellForceArgs A{D.ncells, nbc, D.start.p, D.nodes.p, D.dphi.p, D.volume.p, D.two_pass ? D.u_phys.p : c.u.p,
                     c.Hnp1().p, c.Pnp1().p, D.g.p, D.entry_pos.p,
                     c.state[1 - c.state_np1].p, c.state[c.state_np1].p,
                     c.mat.params(),
@@ -1304,6 +1628,23 @@ void rk_cell_force(Ctx &c) {
                     c.mat.rate_form() ? c.Hn().p : nullptr, c.mat.rate_form() ? c.Pn().p : nullptr, c.rate_dt,
                     D.Paos.p};
     const dim3 g((unsigned)nbc);
+    if (D.fbar) {
+        hipLaunchKernelGGL((k_fbar_subcells<kCellLanes>), g, dim3(256), 0, c.stream, A, D.Jsub.p);
+        APHIP_HIP(hipGetLastError());
+        FbarArgs F{D.n_fcells, D.ncells, D.fcell_sub.p, D.fcell_volume.p, D.volume.p, D.Jsub.p, c.Hnp1().p,
+                   c.Pnp1().p, D.Pbar.p, D.Paos.p, c.state[1 - c.state_np1].p, c.state[c.state_np1].p,
+                   c.mat.params()};
+        const dim3 gf((unsigned)((D.n_fcells + 255) / 256));
+        switch (c.mat.kind) {
+            case APHIP_LINEAR_ELASTIC: hipLaunchKernelGGL(k_fbar_cells<APHIP_LINEAR_ELASTIC>, gf, dim3(256), 0, c.stream, F); break;
+            case APHIP_NEO_HOOKEAN: hipLaunchKernelGGL(k_fbar_cells<APHIP_NEO_HOOKEAN>, gf, dim3(256), 0, c.stream, F); break;
+            case APHIP_J2: hipLaunchKernelGGL(k_fbar_cells<APHIP_J2>, gf, dim3(256), 0, c.stream, F); break;
+            case APHIP_SVK: hipLaunchKernelGGL(k_fbar_cells<APHIP_SVK>, gf, dim3(256), 0, c.stream, F); break;
+            default: throw Error(APHIP_ERR_UNSUPPORTED, "F-bar takes the linear elastic, neo-Hookean, J2 and SVK materials");
+        }
+        APHIP_HIP(hipGetLastError());
+        return;
+    }
     if (D.two_pass) {
         const double *X = c.X.p;
         const bool hex = c.etype == APHIP_HEX8;
@@ -1371,8 +1712,11 @@ void rk_node_update(Ctx &c, double dt, int mode) {
         APHIP_HIP(hipGetLastError());
         return;
     }
-    const int64_t nb = padded_groups(c.nn, kNodeLanes);
+    // nodes with degrees of freedom only (nodal cells put them first; the others keep
+    // their values, as the reference's integrator runs over the active part)
+    const int64_t nb = padded_groups(c.n_dof, kNodeLanes);
     RkNodeCellArgs N{A, D.ent_cell.p, D.dphi_node.p, D.Paos.p};
+    N.out.nn = c.n_dof;
     N.out.nblocks = nb;
     const dim3 gn((unsigned)nb);
     if (mode == 0) hipLaunchKernelGGL((k_rk_nodes_cells<0, kNodeLanes, kNodeCellBatch>), gn, dim3(256), 0, c.stream, N);
@@ -1387,7 +1731,7 @@ void rk_mass(Ctx &c, double *total) {
     DevBuf<double> mfe;
     mfe.alloc(nn);
     hipLaunchKernelGGL(c.etype == APHIP_HEX8 ? k_rk_mfe<8> : k_rk_mfe<4>, grid1(nn, 256), dim3(256), 0, c.stream, nn, c.node_start.p, c.node_inc.p,
-                       D.volume.p, c.mat.density, mfe.p);
+                       D.nodal ? D.evol.p : D.volume.p, c.mat.density, mfe.p);
     APHIP_HIP(hipGetLastError());
     c.mass.alloc(nn);
     hipLaunchKernelGGL(k_rk_mass, grid1(nn, 256), dim3(256), 0, c.stream, nn, D.num_nbr.p, D.rstart.p, D.rsrc.p,

@@ -43,7 +43,7 @@ EXPORTED = [
     "aphip_internal_force", "aphip_initialize", "aphip_step",
     "aphip_synchronize", "aphip_set_tiling", "aphip_set_graph", "aphip_autotune", "aphip_batch_begin",
     "aphip_batch_add", "aphip_batch_end", "aphip_batch_ranges", "aphip_internal_force_vjp",
-    "aphip_adjoint_forward", "aphip_adjoint_backward", "aphip_rk_setup", "aphip_rk_setup_method", "aphip_strain_smoothing_setup", "aphip_rk_get_neighbors", "aphip_rk_get_function_values",
+    "aphip_adjoint_forward", "aphip_adjoint_backward", "aphip_rk_setup", "aphip_rk_setup_method", "aphip_strain_smoothing_setup", "aphip_rk_nodal_setup", "aphip_rk_get_nodal_cells", "aphip_rk_get_neighbors", "aphip_rk_get_function_values",
     "aphip_rk_get_cells", "aphip_set_profiling", "aphip_get_profile", "aphip_reset_profile",
     "aphip_bandwidth_probe", "aphip_power_method_time_increment", "aphip_power_method_time_increment_noalloc",
     "aphip_results_open", "aphip_results_write", "aphip_results_close",
@@ -175,6 +175,7 @@ class Solver:
         conn = np.ascontiguousarray(conn, dtype=np.int32)
         self.nn, self.ne, self.etype = X.shape[0], conn.shape[0], etype
         _check(self.L.aphip_set_mesh(self.ctx, etype, self.nn, _p(X), self.ne, _p(conn, _i32)))
+        self._nodal = False
         return self
 
     def node_permutation(self):
@@ -375,13 +376,33 @@ class Solver:
     def strain_smoothing_setup(self):
         """Finite-element strain smoothing, one cell per Tet4 (ElementSmoothedTetrahedron4)."""
         _check(self.L.aphip_strain_smoothing_setup(self.ctx))
+        self._nodal = False
         return self
 
     def rk_setup(self, kernel_radius_scale, one_pass=True):
         """RK functions and smoothed cells; one_pass=False selects the two-pass force
         (physical displacement gathered before, force scattered after the cells)."""
         _check(self.L.aphip_rk_setup_method(self.ctx, ctypes.c_double(kernel_radius_scale), 1 if one_pass else 0))
+        self._nodal = False
         return self
+
+    def rk_nodal_setup(self, kernel_radius_scale, subdomains=0, f_bar=False):
+        """Nodal smoothing cells on a thex mesh (0: every hex a subcell, 1: one per cell)."""
+        _check(self.L.aphip_rk_nodal_setup(self.ctx, ctypes.c_double(kernel_radius_scale), int(subdomains),
+                                           1 if f_bar else 0))
+        self._nodal = True
+        return self
+
+    def rk_nodal_cells(self):
+        """(cell_subcells, element_subcell, active, f_bar) of the nodal cells."""
+        nc, ns, fb = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int32()
+        _check(self.L.aphip_rk_get_nodal_cells(self.ctx, ctypes.byref(nc), ctypes.byref(ns), None, None, None,
+                                               ctypes.byref(fb)))
+        cs = np.zeros(nc.value + 1, dtype=np.int64)
+        es = np.zeros(self.ne, dtype=np.int32)
+        act = np.zeros(self.nn, dtype=np.uint8)
+        _check(self.L.aphip_rk_get_nodal_cells(self.ctx, None, None, _p(cs, _i64), _p(es, _i32), _p(act, _u8), None))
+        return cs, es, act, bool(fb.value)
 
     def rk_neighbors(self):
         num = np.zeros(self.nn, dtype=np.int32)
@@ -402,12 +423,19 @@ class Solver:
         return nnz.value
 
     def rk_cells(self):
+        """Cell CSR (start, nodes, dphi, volume): per element in caller order, or, after
+        rk_nodal_setup, per subcell in subcell order."""
         nnz = ctypes.c_int64()
         _check(self.L.aphip_rk_get_cells(self.ctx, ctypes.byref(nnz), None, None, None, None))
-        start = np.zeros(self.ne + 1, dtype=np.int64)
+        ncells = self.ne
+        if getattr(self, "_nodal", False):
+            ns = ctypes.c_int64()
+            _check(self.L.aphip_rk_get_nodal_cells(self.ctx, None, ctypes.byref(ns), None, None, None, None))
+            ncells = ns.value
+        start = np.zeros(ncells + 1, dtype=np.int64)
         nodes = np.zeros(nnz.value, dtype=np.int32)
         dphi = np.zeros((nnz.value, 3))
-        vol = np.zeros(self.ne)
+        vol = np.zeros(ncells)
         _check(self.L.aphip_rk_get_cells(self.ctx, ctypes.byref(nnz), _p(start, _i64), _p(nodes, _i32), _p(dphi),
                                          _p(vol)))
         return start, nodes, dphi, vol

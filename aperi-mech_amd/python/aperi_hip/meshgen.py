@@ -126,6 +126,51 @@ def stable_dt_estimate(X, conn, E, nu, rho):
     return 0.8 * lmin / c
 
 
+def thex(X, tets):
+    """Split every tetrahedron into four hexahedra ("thex"): one hex per tet vertex v, with
+    v, the midpoints of the three edges at v, the centroids of the three faces at v and the
+    tet's centroid.  Nodes: the tet vertices keep their ids 0..n-1 (the lowest ids, so each
+    hex's lowest id is its tet vertex), then the edge midpoints, the face centroids and the
+    tet centroids.  Hex node order: v, m(v,p), c(v,p,q), m(v,q), m(v,r), c(v,p,r), centroid,
+    c(v,q,r) with (p - v, q - v, r - v) right-handed, so every hex has positive volume."""
+    X = np.asarray(X, dtype=np.float64)
+    tets = np.asarray(tets, dtype=np.int64)
+    nv, nt = len(X), len(tets)
+    edges = np.sort(tets[:, [[0, 1], [0, 2], [0, 3], [1, 2], [1, 3], [2, 3]]].reshape(-1, 2), axis=1)
+    uedges, einv = np.unique(edges, axis=0, return_inverse=True)
+    faces = np.sort(tets[:, [[0, 1, 2], [0, 1, 3], [0, 2, 3], [1, 2, 3]]].reshape(-1, 3), axis=1)
+    ufaces, finv = np.unique(faces, axis=0, return_inverse=True)
+    ne_, nf = len(uedges), len(ufaces)
+    Xh = np.concatenate([X, 0.5 * (X[uedges[:, 0]] + X[uedges[:, 1]]),
+                         (X[ufaces[:, 0]] + X[ufaces[:, 1]] + X[ufaces[:, 2]]) / 3.0,
+                         X[tets].mean(axis=1)])
+    einv = einv.reshape(nt, 6) + nv
+    finv = finv.reshape(nt, 4) + nv + ne_
+    cid = nv + ne_ + nf + np.arange(nt)
+    eslot = {(0, 1): 0, (0, 2): 1, (0, 3): 2, (1, 2): 3, (1, 3): 4, (2, 3): 5}
+    fslot = {(0, 1, 2): 0, (0, 1, 3): 1, (0, 2, 3): 2, (1, 2, 3): 3}
+    hexes = np.zeros((4 * nt, 8), dtype=np.int64)
+    for lv in range(4):
+        others = [k for k in range(4) if k != lv]
+        v = tets[:, lv]
+        p, q, r = (tets[:, k] for k in others)
+        det = np.einsum("ij,ij->i", X[p] - X[v], np.cross(X[q] - X[v], X[r] - X[v]))
+        for t in range(nt):
+            a, b, c = others if det[t] > 0 else (others[0], others[2], others[1])
+            E = lambda i, j: einv[t, eslot[tuple(sorted((i, j)))]]
+            F = lambda i, j, k: finv[t, fslot[tuple(sorted((i, j, k)))]]
+            hexes[4 * t + lv] = [tets[t, lv], E(lv, a), F(lv, a, b), E(lv, b), E(lv, c), F(lv, a, c), cid[t],
+                                 F(lv, b, c)]
+    return Xh, hexes.astype(np.int32)
+
+
+def thex_block(n=3, seed=DEFAULT_SEED, jitter_frac=0.1):
+    """A jittered n^3 hex lattice on [0,1]^3, split into Kuhn tets, each tet into four hexes."""
+    X, hc = hex_lattice(n, n, n)
+    X = jitter(X, 1.0 / n, jitter_frac, seed, 1)
+    return thex(X, split_tets(X, hc))
+
+
 # ---------------- named workloads ----------------
 
 def plain_hex_cube(n=20, seed=DEFAULT_SEED):

@@ -333,6 +333,32 @@ int aphip_rk_setup_method(aphip_ctx *ctx, double kernel_radius_scale, int32_t us
  * (include/Element.h:66-69).  The functions read back through aphip_rk_get_* are then the
  * identity (one neighbor, the node itself, value 1). */
 int aphip_strain_smoothing_setup(aphip_ctx *ctx);
+/* Nodal strain smoothing on a "thex" mesh (a Hex8 mesh made by splitting every tetrahedron
+ * of a tet mesh into four hexahedra): the reference's nodal_smoothing_cell integration
+ * (include/MeshLabelerProcessor.h:227-330 active nodes, :690-827 cells and subcells;
+ * src/MaxEdgeLengthProcessor.cpp:54-55 edges to inactive nodes doubled; the search of
+ * include/NeighborSearchProcessor.h:196-260 with the active nodes as the only candidates;
+ * include/SmoothedCellDataProcessor.h:407-675 subcell lists and derivatives).  Every hex's
+ * node of lowest caller id is active (the tet vertices); a mesh in which some hex then holds
+ * another number of active nodes is refused with the reference's message.  One cell per
+ * active node, made of the hexes around it; subdomains 0: every hex its own subcell,
+ * subdomains 1: the whole cell one subcell.  use_f_bar (the reference's strain_smoothing
+ * use_f_bar) applies F-bar over the subcells of each cell
+ * (include/ComputeInternalForceSmoothedCell.h:120-413), which the reference does only when
+ * cells hold more than one subcell (:458-459); F-bar takes the linear elastic, neo-Hookean,
+ * J2 and SVK materials.  Only active nodes carry degrees of freedom: the others keep their
+ * values through aphip_step, and their mass is zero.  Element fields H, P and STATE are per
+ * subcell: every element reads its subcell's values (the reference's output population);
+ * VOLUME is each element's smoothed volume. */
+int aphip_rk_nodal_setup(aphip_ctx *ctx, double kernel_radius_scale, int32_t subdomains, int32_t use_f_bar);
+/* The nodal cells of aphip_rk_nodal_setup: counts, cell_subcells[n_cells+1] (the subcells of
+ * cell c are cell_subcells[c] .. cell_subcells[c+1]-1; cells ordered by their active node's
+ * caller id, subcells by caller element id within a cell), element_subcell[n_elem] (caller
+ * element -> subcell), active[n_nodes] (1 = the node carries degrees of freedom) and whether
+ * F-bar is applied.  Any output may be NULL.  aphip_rk_get_cells then lists the subcells in
+ * this order (start[n_subcells+1], volume[n_subcells]). */
+int aphip_rk_get_nodal_cells(aphip_ctx *ctx, int64_t *n_cells, int64_t *n_subcells, int64_t *cell_subcells,
+                             int32_t *element_subcell, uint8_t *active, int32_t *f_bar);
 /* Neighbor lists in caller numbering: num[n_nodes], nbr[n_nodes x 40] (-1 padded),
  * the 40-cap truncation count (NeighborSearchProcessor.h:307-316). */
 int aphip_rk_get_neighbors(aphip_ctx *ctx, int32_t *num, int32_t *nbr, int64_t *n_truncated);

@@ -90,6 +90,13 @@ static int contains(const double *xi, const double *xj, double rj) {
 
 int or_neighbor_search(int64_t n_nodes, const double *X, const double *radius, int max_neighbors,
                        int brute_force, int32_t *num_neighbors, int32_t *neighbors, int64_t *n_truncated) {
+    return or_neighbor_search_masked(n_nodes, X, radius, max_neighbors, NULL, brute_force, num_neighbors, neighbors,
+                                     n_truncated);
+}
+
+int or_neighbor_search_masked(int64_t n_nodes, const double *X, const double *radius, int max_neighbors,
+                              const uint8_t *range_mask, int brute_force, int32_t *num_neighbors,
+                              int32_t *neighbors, int64_t *n_truncated) {
     int64_t trunc = 0;
     int64_t cap = 256;
     cand_t *c = (cand_t *)malloc(sizeof(cand_t) * cap);
@@ -100,7 +107,7 @@ int or_neighbor_search(int64_t n_nodes, const double *X, const double *radius, i
             if (X[3 * i + d] < lo[d]) lo[d] = X[3 * i + d];
             if (X[3 * i + d] > hi[d]) hi[d] = X[3 * i + d];
         }
-        if (radius[i] > rmax) rmax = radius[i];
+        if ((!range_mask || range_mask[i]) && radius[i] > rmax) rmax = radius[i];
     }
     int64_t nb[3] = {1, 1, 1};
     int64_t *bin_start = NULL, *bin_items = NULL;
@@ -124,10 +131,11 @@ int or_neighbor_search(int64_t n_nodes, const double *X, const double *radius, i
                 if (b[d] >= nb[d]) b[d] = nb[d] - 1;
             }
             binof[i] = (b[2] * nb[1] + b[1]) * nb[0] + b[0];
-            bin_start[binof[i] + 1]++;
+            if (!range_mask || range_mask[i]) bin_start[binof[i] + 1]++;
         }
         for (int64_t b = 0; b < nbins; ++b) bin_start[b + 1] += bin_start[b];
-        for (int64_t i = 0; i < n_nodes; ++i) bin_items[bin_start[binof[i]] + fill[binof[i]]++] = i;
+        for (int64_t i = 0; i < n_nodes; ++i)
+            if (!range_mask || range_mask[i]) bin_items[bin_start[binof[i]] + fill[binof[i]]++] = i;
         free(fill);
         free(binof);
     }
@@ -136,7 +144,7 @@ int or_neighbor_search(int64_t n_nodes, const double *X, const double *radius, i
         int64_t nc = 0;
         if (brute_force) {
             for (int64_t j = 0; j < n_nodes; ++j)
-                if (contains(xi, &X[3 * j], radius[j])) {
+                if ((!range_mask || range_mask[j]) && contains(xi, &X[3 * j], radius[j])) {
                     if (nc == cap) { cap *= 2; c = (cand_t *)realloc(c, sizeof(cand_t) * cap); }
                     c[nc].d2 = tweaked_d2(xi, &X[3 * j]);
                     c[nc].id = (int32_t)j;

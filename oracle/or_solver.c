@@ -9,6 +9,8 @@
  *                      acceleration BC; v = v1/2 + dt/2 a)
  *   node functors      include/ExplicitTimeIntegrator.h:116-126, :267-276, :308-318, :348-357
  *   N/NP1 state swap   Explicit.cpp:393 (UpdateFieldStates) for the element state field
+ *   active nodes       the integrator runs over the active part only (Explicit.cpp:316,
+ *                      ExplicitTimeIntegrator.h:376-379): nodes without DOFs keep their values
  *   updated Lagrangian ExplicitTimeIntegratorUpdated (include/ExplicitTimeIntegrator.h:193-226,
  *                      :466-486): increment dt v1/2, d += increment, x_{n+1} = x_n + increment,
  *                      the force over x_n with the increments and H_n; semi-Lagrangian
@@ -60,6 +62,10 @@ static int compute_force(const or_problem *p, const double *u, const double *sn,
         free(fl);
         return rc;
     }
+    if (p->etype == -1 && p->fbar) /* nodal cells with F-bar (ComputeInternalForceSmoothedCell.h:120-413) */
+        return or_fbar_internal_force(p->n_nodes, p->n_fbar_cells, p->fbar_cell_start, p->fbar_cell_volume,
+                                      p->cell_start, p->cell_nodes, p->cell_dphi, p->cell_volume, &p->mat, u, sn,
+                                      snp1, H, NULL, P, NULL, f, nthreads);
     if (p->etype == -1)
         return or_cell_internal_force(p->n_nodes, p->n_elem, p->cell_start, p->cell_nodes, p->cell_dphi,
                                       p->cell_volume, &p->mat, u, sn, snp1, H, P, f, nthreads);
@@ -94,8 +100,10 @@ int or_solve(const or_problem *p, double dt, int n_steps, int do_initial, double
         rc = compute_force(p, ul ? q.d_inc : u, snp1, snp1, H, P, f, nt);
         if (rc) { free(x_np1); return rc; }
 #pragma omp parallel for schedule(static) num_threads(nt)
-        for (int64_t i = 0; i < N; ++i)
+        for (int64_t i = 0; i < N; ++i) {
+            if (p->active && !p->active[i]) continue;
             for (int d = 0; d < 3; ++d) a[3 * i + d] = f[3 * i + d] / p->mass[i];
+        }
     }
     for (int s = 0; s < n_steps; ++s) {
         /* UpdateFieldStates: N <- NP1 */
@@ -106,6 +114,7 @@ int or_solve(const or_problem *p, double dt, int n_steps, int do_initial, double
         }
 #pragma omp parallel for schedule(static) num_threads(nt)
         for (int64_t i = 0; i < N; ++i) {
+            if (p->active && !p->active[i]) continue; /* nodes without DOFs are not integrated */
             const double dti = p->node_dt ? p->node_dt[i] : dt;
             const double h = 0.5 * dti;
             for (int d = 0; d < 3; ++d) v[3 * i + d] = v[3 * i + d] + h * a[3 * i + d];
@@ -127,6 +136,7 @@ int or_solve(const or_problem *p, double dt, int n_steps, int do_initial, double
         } else {
 #pragma omp parallel for schedule(static) num_threads(nt)
             for (int64_t i = 0; i < N; ++i) {
+                if (p->active && !p->active[i]) continue;
                 const double dti = p->node_dt ? p->node_dt[i] : dt;
                 for (int d = 0; d < 3; ++d) u[3 * i + d] = u[3 * i + d] + dti * v[3 * i + d];
             }
@@ -135,13 +145,16 @@ int or_solve(const or_problem *p, double dt, int n_steps, int do_initial, double
         if (rc) { free(x_np1); return rc; }
         if (ul && !semi) memcpy(q.x_cur, x_np1, sizeof(double) * 3 * (size_t)N);
 #pragma omp parallel for schedule(static) num_threads(nt)
-        for (int64_t i = 0; i < N; ++i)
+        for (int64_t i = 0; i < N; ++i) {
+            if (p->active && !p->active[i]) continue;
             for (int d = 0; d < 3; ++d) a[3 * i + d] = f[3 * i + d] / p->mass[i];
+        }
         if (p->fixed)
             for (int64_t i = 0; i < 3 * N; ++i)
                 if (p->fixed[i]) a[i] = 0.0;
 #pragma omp parallel for schedule(static) num_threads(nt)
         for (int64_t i = 0; i < N; ++i) {
+            if (p->active && !p->active[i]) continue;
             const double dti = p->node_dt ? p->node_dt[i] : dt;
             const double h = 0.5 * dti;
             for (int d = 0; d < 3; ++d) v[3 * i + d] += h * a[3 * i + d];

@@ -129,6 +129,15 @@ typedef struct {
     /* semi-Lagrangian: reference update interval and the caller's step counter */
     int64_t semi_interval;
     int64_t *semi_steps;
+    /* F-bar over nodal cells (etype == -1, fbar = 1): the cells above are the subcells,
+     * grouped into n_fbar_cells cells of consecutive subcells (fbar_cell_start[n+1]) */
+    int32_t fbar;
+    int32_t pad2_;
+    int64_t n_fbar_cells;
+    const int64_t *fbar_cell_start;
+    const double *fbar_cell_volume;
+    /* nodes that carry degrees of freedom (NULL: all); the others are never updated */
+    const uint8_t *active;
 } or_problem;
 
 /* Initial force and acceleration (Explicit.cpp:363-376), then n_steps of the
@@ -182,6 +191,34 @@ int or_rk_gather(int64_t n_nodes, int max_neighbors, const int32_t *num_neighbor
  * without neighbors takes src_i alone (3 components). */
 int or_rk_scatter(int64_t n_nodes, int max_neighbors, const int32_t *num_neighbors, const int32_t *neighbors,
                   const double *phi, const double *src, double *dst);
+
+/* ---- nodal strain smoothing on thex meshes, subcells, F-bar (or_nodal.c) ---- */
+/* active[i] = 1 for the node of lowest id (node_id[i], or i when NULL) of some hex; returns
+ * the number of hexes whose count of active nodes is not exactly one (0 = a valid thex mesh). */
+int64_t or_nodal_active(int64_t n_nodes, int64_t n_hex, const int32_t *hexes, const int64_t *node_id,
+                        uint8_t *active);
+/* hex_cell[e] = rank of the hex's active node among the active nodes (ascending index);
+ * returns the number of cells. */
+int64_t or_nodal_cells(int64_t n_nodes, int64_t n_hex, const int32_t *hexes, const uint8_t *active,
+                       int32_t *hex_cell);
+/* ball search whose candidates are only the nodes with range_mask[j] != 0 (NULL: all) */
+int or_neighbor_search_masked(int64_t n_nodes, const double *X, const double *radius, int max_neighbors,
+                              const uint8_t *range_mask, int brute_force, int32_t *num_neighbors,
+                              int32_t *neighbors, int64_t *n_truncated);
+/* subcells of several elements: sub_start[n_sub+1] ranges into sub_elems */
+int64_t or_rk_subcells_count(int64_t n_sub, const int64_t *sub_start, const int32_t *sub_elems, const int32_t *conn,
+                             int npe, int max_neighbors, const int32_t *num, const int32_t *nbr, int64_t *start);
+int or_rk_subcells_build(int64_t n_nodes, const double *X, int64_t n_sub, const int64_t *sub_start,
+                         const int32_t *sub_elems, const int32_t *conn, int npe, int max_neighbors,
+                         const int32_t *num, const int32_t *nbr, const double *phi, const int64_t *start,
+                         int32_t *nodes, double *dphi, double *sub_volume, double *elem_volume);
+/* F-bar force of cells made of consecutive subcells (cell_sub[n_cells+1]); H, Hbar, P, Pbar
+ * per subcell (row-major, any may be NULL); the state per subcell. */
+int or_fbar_internal_force(int64_t n_nodes, int64_t n_cells, const int64_t *cell_sub, const double *cell_volume,
+                           const int64_t *sub_start, const int32_t *sub_nodes, const double *sub_dphi,
+                           const double *sub_volume, const or_material *mat, const double *u, const double *state_n,
+                           double *state_np1, double *H, double *Hbar, double *P, double *Pbar, double *f,
+                           int nthreads);
 
 #ifdef __cplusplus
 }

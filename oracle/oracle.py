@@ -45,7 +45,9 @@ class Problem(ctypes.Structure):
                 ("cell_start", _i64), ("cell_nodes", _i32), ("cell_dphi", _d), ("cell_volume", _d),
                 ("rk_two_pass", ctypes.c_int32), ("max_neighbors", ctypes.c_int32), ("num_neighbors", _i32),
                 ("neighbors", _i32), ("phi", _d), ("x_cur", _d), ("d_inc", _d),
-                ("semi_interval", ctypes.c_int64), ("semi_steps", _i64)]
+                ("semi_interval", ctypes.c_int64), ("semi_steps", _i64),
+                ("fbar", ctypes.c_int32), ("pad2_", ctypes.c_int32), ("n_fbar_cells", ctypes.c_int64),
+                ("fbar_cell_start", _i64), ("fbar_cell_volume", _d), ("active", _u8)]
 
 
 def build():
@@ -63,6 +65,9 @@ def lib():
         L.or_kernel_value.restype = ctypes.c_double
         L.or_power_method_dt.restype = ctypes.c_double
         L.or_rk_cells_count.restype = ctypes.c_int64
+        L.or_nodal_active.restype = ctypes.c_int64
+        L.or_nodal_cells.restype = ctypes.c_int64
+        L.or_rk_subcells_count.restype = ctypes.c_int64
         _LIB = L
     return _LIB
 
@@ -266,11 +271,14 @@ class Solver:
 
     def __init__(self, etype, X, conn, kind, lam, mu, mass, fixed=None, yield_stress=0.0, hardening=0.0,
                  elem_lambda=None, elem_mu=None, node_dt=None, state_per_qp=0, cells=None, two_pass=None,
-                 damage=None, creep=None, updated=False, semi_interval=0):
+                 damage=None, creep=None, updated=False, semi_interval=0, fbar_cells=None, active=None):
         """updated: the updated Lagrangian formulation (Gaussian elements); semi_interval > 0:
         the semi-Lagrangian one with that reference update interval."""
         """cells: RK smoothed cells (start, nodes, dphi, volume); two_pass: (num, nbr, phi)
-        of the RK functions, with `cells` from rk_cells_two_pass, for the two-pass force."""
+        of the RK functions, with `cells` from rk_cells_two_pass, for the two-pass force.
+        fbar_cells: (cell_start, cell_volume) grouping `cells` (the subcells) into nodal
+        cells for the F-bar force; active: nodes with degrees of freedom (others are not
+        integrated)."""
         self.X = _f64(X)
         self.conn = None if conn is None else _i32a(conn)
         self.mass = _f64(mass)
@@ -311,6 +319,13 @@ class Solver:
             p.x_cur, p.d_inc = _p(self.xc), _p(self.dinc)
             self.semi_steps = np.zeros(1, dtype=np.int64)
             p.semi_interval, p.semi_steps = int(semi_interval), _p(self.semi_steps, _i64)
+        if fbar_cells is not None:
+            self.fcs = np.ascontiguousarray(fbar_cells[0], dtype=np.int64)
+            self.fcv = _f64(fbar_cells[1])
+            p.fbar, p.n_fbar_cells = 1, self.fcs.shape[0] - 1
+            p.fbar_cell_start, p.fbar_cell_volume = _p(self.fcs, _i64), _p(self.fcv)
+        self.active = None if active is None else np.ascontiguousarray(active, dtype=np.uint8)
+        p.active = _p(self.active, _u8)
         p.mass = _p(self.mass)
         p.fixed = _p(self.fixed, _u8)
         p.node_dt = _p(self.node_dt)
@@ -385,14 +400,16 @@ def kernel_radius(max_edge, scale):
     return np.asarray(max_edge) * scale
 
 
-def neighbor_search(X, radius, max_neighbors=MAX_NODE_NUM_NEIGHBORS, brute_force=False):
+def neighbor_search(X, radius, max_neighbors=MAX_NODE_NUM_NEIGHBORS, brute_force=False, range_mask=None):
+    """range_mask: only nodes with a nonzero entry are candidates (the active nodes)."""
     X, radius = _f64(X), _f64(radius)
     n = X.shape[0]
     num = np.zeros(n, dtype=np.int32)
     nbr = np.zeros((n, max_neighbors), dtype=np.int32)
     trunc = ctypes.c_int64()
-    rc = lib().or_neighbor_search(n, _p(X), _p(radius), max_neighbors, 1 if brute_force else 0, _p(num, _i32),
-                                  _p(nbr, _i32), ctypes.byref(trunc))
+    mask = None if range_mask is None else np.ascontiguousarray(range_mask, dtype=np.uint8)
+    rc = lib().or_neighbor_search_masked(n, _p(X), _p(radius), max_neighbors, _p(mask, _u8),
+                                         1 if brute_force else 0, _p(num, _i32), _p(nbr, _i32), ctypes.byref(trunc))
     if rc:
         raise RuntimeError("or_neighbor_search failed")
     return num, nbr, trunc.value
@@ -489,3 +506,106 @@ def rk_mass(conn, elem_volume, density, num, nbr, phi):
     lib().or_rk_mass(n, conn.shape[0], _p(conn, _i32), conn.shape[1], _p(_f64(elem_volume)), ctypes.c_double(density), mx,
                      _p(num, _i32), _p(nbr, _i32), _p(phi), _p(m))
     return m
+
+
+# ---------------- nodal strain smoothing (thex meshes), subcells, F-bar ----------------
+
+def nodal_active(hexes, n_nodes, node_id=None):
+    """(active mask, number of hexes without exactly one active node)."""
+    hexes = _i32a(hexes)
+    act = np.zeros(n_nodes, dtype=np.uint8)
+    ids = None if node_id is None else np.ascontiguousarray(node_id, dtype=np.int64)
+    bad = lib().or_nodal_active(n_nodes, hexes.shape[0], _p(hexes, _i32), _p(ids, _i64), _p(act, _u8))
+    return act, int(bad)
+
+
+def nodal_cells(hexes, active, subdomains):
+    """Cells (one per active node, ascending) and subcells: returns (hex_cell, cell_start,
+    cell_hexes, sub_start, sub_elems, cell_sub), where the hexes of a cell are listed in
+    ascending order, subdomains 0 makes every hex a subcell and 1 every cell one subcell;
+    cell_sub[c]..cell_sub[c+1] are the cell's subcells."""
+    hexes = _i32a(hexes)
+    ne = hexes.shape[0]
+    hc = np.zeros(ne, dtype=np.int32)
+    nc = lib().or_nodal_cells(len(active), ne, _p(hexes, _i32), _p(np.ascontiguousarray(active, dtype=np.uint8), _u8),
+                              _p(hc, _i32))
+    order = np.lexsort((np.arange(ne), hc)).astype(np.int32)
+    cell_start = np.zeros(nc + 1, dtype=np.int64)
+    np.add.at(cell_start, hc + 1, 1)
+    cell_start = np.cumsum(cell_start)
+    if subdomains == 0:
+        sub_start = np.arange(ne + 1, dtype=np.int64)
+        cell_sub = cell_start.copy()
+    else:
+        sub_start = cell_start.copy()
+        cell_sub = np.arange(nc + 1, dtype=np.int64)
+    return hc, cell_start, order, sub_start, order.copy(), cell_sub
+
+
+def rk_subcells(X, conn, sub_start, sub_elems, num, nbr, phi):
+    """Subcell CSR (start, nodes, dphi, sub_volume) and every element's smoothed volume."""
+    X, conn, num, nbr, phi = _f64(X), _i32a(conn), _i32a(num), _i32a(nbr), _f64(phi)
+    sub_start = np.ascontiguousarray(sub_start, dtype=np.int64)
+    sub_elems = _i32a(sub_elems)
+    ns, npe, mx = sub_start.shape[0] - 1, conn.shape[1], nbr.shape[1]
+    start = np.zeros(ns + 1, dtype=np.int64)
+    nnz = lib().or_rk_subcells_count(ns, _p(sub_start, _i64), _p(sub_elems, _i32), _p(conn, _i32), npe, mx,
+                                     _p(num, _i32), _p(nbr, _i32), _p(start, _i64))
+    if nnz < 0:
+        raise RuntimeError("or_rk_subcells_count failed")
+    nodes = np.zeros(nnz, dtype=np.int32)
+    dphi = np.zeros((nnz, 3))
+    vol = np.zeros(ns)
+    evol = np.zeros(conn.shape[0])
+    rc = lib().or_rk_subcells_build(X.shape[0], _p(X), ns, _p(sub_start, _i64), _p(sub_elems, _i32), _p(conn, _i32),
+                                    npe, mx, _p(num, _i32), _p(nbr, _i32), _p(phi), _p(start, _i64), _p(nodes, _i32),
+                                    _p(dphi), _p(vol), _p(evol))
+    if rc:
+        raise RuntimeError("or_rk_subcells_build failed")
+    return (start, nodes, dphi, vol), evol
+
+
+def fbar_internal_force(n_nodes, cells, fbar_cells, kind, lam, mu, u, state_n=None, yield_stress=0.0,
+                        hardening=0.0, nthreads=1):
+    """F-bar force of subcells `cells` grouped by fbar_cells = (cell_sub, cell_volume):
+    returns (f, H, Hbar, P, Pbar, state_np1)."""
+    start, nodes, dphi, vol = cells
+    cs, cv = np.ascontiguousarray(fbar_cells[0], dtype=np.int64), _f64(fbar_cells[1])
+    ns = vol.shape[0]
+    mat = _material(kind, lam, mu, yield_stress, hardening)
+    w = max(state_width(kind), 1)
+    sn = np.zeros(ns * w) if state_n is None else _f64(state_n).copy()
+    snp1 = np.zeros(ns * w)
+    H, Hb, P, Pb, f = np.zeros((ns, 9)), np.zeros((ns, 9)), np.zeros((ns, 9)), np.zeros((ns, 9)), np.zeros((n_nodes, 3))
+    rc = lib().or_fbar_internal_force(n_nodes, cs.shape[0] - 1, _p(cs, _i64), _p(cv),
+                                      _p(np.ascontiguousarray(start, dtype=np.int64), _i64), _p(_i32a(nodes), _i32),
+                                      _p(_f64(dphi)), _p(_f64(vol)), ctypes.byref(mat), _p(_f64(u)), _p(sn), _p(snp1),
+                                      _p(H), _p(Hb), _p(P), _p(Pb), _p(f), nthreads)
+    if rc:
+        raise RuntimeError("or_fbar_internal_force failed")
+    return f, H, Hb, P, Pb, snp1
+
+
+def nodal_setup(X, hexes, scale, subdomains, density=None, node_id=None):
+    """The whole nodal-smoothing set-up on a thex mesh: active nodes, max edge lengths
+    (edges to inactive nodes doubled), radii of the active nodes, search over the active
+    nodes, RK functions at every node, cells and subcells, subcell CSR, RK mass.  Returns a
+    dict."""
+    X, hexes = _f64(X), _i32a(hexes)
+    act, bad = nodal_active(hexes, len(X), node_id)
+    if bad:
+        raise ValueError(f"{bad} hexes without exactly one active node")
+    edge = max_edge_length(X, hexes, active=act)
+    radius = np.where(act.astype(bool), kernel_radius(edge, scale), 0.0)
+    num, nbr, trunc = neighbor_search(X, radius, range_mask=act)
+    phi = rk_function_values(X, radius, num, nbr)
+    hc, cell_start, cell_hexes, sub_start, sub_elems, cell_sub = nodal_cells(hexes, act, subdomains)
+    cells, evol = rk_subcells(X, hexes, sub_start, sub_elems, num, nbr, phi)
+    cell_volume = np.array([evol[cell_hexes[cell_start[c]:cell_start[c + 1]]].sum()
+                            for c in range(len(cell_start) - 1)])
+    out = dict(active=act, edge=edge, radius=radius, num=num, nbr=nbr, trunc=trunc, phi=phi, hex_cell=hc,
+               cell_start=cell_start, cell_hexes=cell_hexes, sub_start=sub_start, sub_elems=sub_elems,
+               cell_sub=cell_sub, cells=cells, elem_volume=evol, cell_volume=cell_volume)
+    if density is not None:
+        out["mass"] = rk_mass(hexes, evol, density, num, nbr, phi)
+    return out

@@ -17,6 +17,7 @@
 #include <cstring>
 #include <functional>
 #include <limits>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -125,6 +126,61 @@ std::vector<int32_t> SplitTets(const MeshBlock &m, const std::vector<int32_t> &h
             tets.insert(tets.end(), t, t + 4);
         }
     return tets;
+}
+
+// "thex": every tet split into four hexes (one per vertex v: v, the three edge midpoints,
+// the three face centroids at v and the centroid), the tet vertices keeping the lowest
+// node ids, so that each hex's lowest id is its tet vertex.
+MeshBlock Thex(const MeshBlock &tets) {
+    MeshBlock m;
+    m.coordinates = tets.coordinates;
+    const std::vector<double> &X = tets.coordinates;
+    std::map<std::vector<int32_t>, int32_t> made;
+    auto point = [&](std::vector<int32_t> key) {
+        std::sort(key.begin(), key.end());
+        auto it = made.find(key);
+        if (it != made.end()) return it->second;
+        const int32_t id = (int32_t)(m.coordinates.size() / 3);
+        for (int d = 0; d < 3; ++d) {
+            double x = 0.0;
+            for (int32_t k : key) x += X[3 * k + d];
+            m.coordinates.push_back(x / key.size());
+        }
+        made[key] = id;
+        return id;
+    };
+    // every edge and face point first, then the centroids
+    const size_t nt = tets.connectivity.size() / 4;
+    for (size_t t = 0; t < nt; ++t) {
+        const int32_t *v = &tets.connectivity[4 * t];
+        for (int a = 0; a < 4; ++a)
+            for (int b = a + 1; b < 4; ++b) point({v[a], v[b]});
+    }
+    for (size_t t = 0; t < nt; ++t) {
+        const int32_t *v = &tets.connectivity[4 * t];
+        for (int a = 0; a < 4; ++a) point({v[a == 0 ? 1 : 0], v[a <= 1 ? 2 : 1], v[a <= 2 ? 3 : 2]});
+    }
+    for (size_t t = 0; t < nt; ++t) {
+        const int32_t *v = &tets.connectivity[4 * t];
+        point({v[0], v[1], v[2], v[3]});
+    }
+    for (size_t t = 0; t < nt; ++t) {
+        const int32_t *v = &tets.connectivity[4 * t];
+        const int32_t c = point({v[0], v[1], v[2], v[3]});
+        for (int lv = 0; lv < 4; ++lv) {
+            int o[3], n = 0;
+            for (int k = 0; k < 4; ++k)
+                if (k != lv) o[n++] = k;
+            // (p - v, q - v, r - v) right-handed
+            const int32_t tv[4] = {v[lv], v[o[0]], v[o[1]], v[o[2]]};
+            if (SignedVolume(X, tv) < 0) std::swap(o[1], o[2]);
+            const int32_t V = v[lv], P = v[o[0]], Q = v[o[1]], R = v[o[2]];
+            const int32_t h[8] = {V, point({V, P}), point({V, P, Q}), point({V, Q}),
+                                  point({V, R}), point({V, P, R}), c, point({V, Q, R})};
+            m.connectivity.insert(m.connectivity.end(), h, h + 8);
+        }
+    }
+    return m;
 }
 
 std::shared_ptr<Material> MakeMaterial(MaterialType type, double E = 200e9, double nu = 0.3, double rho = 7850.0) {
@@ -273,6 +329,18 @@ void CpuCases() {
         p->linear_elastic_properties.youngs_modulus = 1.0;
         p->linear_elastic_properties.poisson_ratio = 0.2;
         EXPECT_TRUE(Throws([&] { CreateMaterial(p); }, "Missing material property"));
+    });
+    Run("CreateElementTest.NodalCellsNeedHexahedra", [] {
+        MeshBlock empty;
+        auto mat = MakeMaterial(LINEAR_ELASTIC);
+        EXPECT_TRUE(Throws(
+            [&] {
+                CreateElement(ElementTopology::Tetrahedron4, ApproximationSpaceType::ReproducingKernel,
+                              IntegrationSchemeType::StrainSmoothing, empty, nullptr, mat, 0.9, true,
+                              LagrangianFormulationType::Total, MeshLabelerParameters{SmoothingCellType::Nodal, 0},
+                              true);
+            },
+            "Use 'thex'"));
     });
     Run("CreateElementTest.UnsupportedCombinations", [] {
         MeshBlock empty;
@@ -648,6 +716,67 @@ double CellElementForceError(const std::shared_ptr<ElementBase> &elem, const Mes
 }
 
 void CellElementCases() {
+    Run("ForceTest.ExplicitShearForceReproducingKernelNodalSubcellsFBar", [] {
+        // ForceTest.cpp:292-304: a thex brick, nodal smoothing cells with every element a
+        // subcell and F-bar; uniform shear u = gamma X_1 e_0: H and P uniform, F-bar changes
+        // nothing (uniform J), zero net force and sum_I f_I (x) x_I = -V P exactly
+        std::vector<int32_t> hexes;
+        MeshBlock lattice = HexLattice(2, 0.15, 31, &hexes);
+        lattice.connectivity = SplitTets(lattice, hexes);
+        const MeshBlock mesh = Thex(lattice);
+        auto mat = MakeMaterial(NEO_HOOKEAN, 70e9, 0.33, 2700.0);
+        MeshLabelerParameters labeler{SmoothingCellType::Nodal, 0};
+        auto elem = std::dynamic_pointer_cast<ElementNodalReproducingKernelHip>(
+            CreateElement(ElementTopology::Hexahedron8, ApproximationSpaceType::ReproducingKernel,
+                          IntegrationSchemeType::StrainSmoothing, mesh, std::make_shared<Device>(), mat, 0.9, true,
+                          LagrangianFormulationType::Total, labeler, /*use_f_bar=*/true));
+        EXPECT_TRUE(elem != nullptr);
+        std::vector<uint8_t> active;
+        EXPECT_TRUE(elem->UsesFBar(&active));
+        const size_t n = mesh.NumNodes(), nv = lattice.NumNodes();
+        for (size_t i = 0; i < n; ++i) EXPECT_TRUE(active[i] == (i < nv ? 1 : 0));  // the tet vertices
+        const double gamma = 1e-3;
+        std::vector<double> u(3 * n, 0.0);
+        for (size_t i = 0; i < n; ++i) u[3 * i] = gamma * mesh.coordinates[3 * i + 1];
+        elem->SetField(APHIP_FIELD_U, u);
+        InternalForceContribution(elem).ComputeForce(0.0, 1e-7);
+        const std::vector<double> H = elem->GetField(APHIP_FIELD_H), P = elem->GetField(APHIP_FIELD_P),
+                                  f = elem->GetField(APHIP_FIELD_F), vol = elem->GetField(APHIP_FIELD_VOLUME);
+        double Hx[9] = {0, gamma, 0, 0, 0, 0, 0, 0, 0}, Px[9];
+        const LinearElasticProperties &le = mat->GetMaterialProperties()->linear_elastic_properties;
+        or_stress(OR_NEO_HOOKEAN, le.lambda, le.shear_modulus, 0.0, 0.0, Hx, nullptr, nullptr, Px);
+        double hmax = 0.0, pmax = 0.0, pscale = 0.0, V = 0.0;
+        for (size_t e = 0; e < elem->NumElements(); ++e) {
+            V += vol[e];
+            for (int c = 0; c < 9; ++c) {
+                hmax = std::max(hmax, std::fabs(H[9 * e + c] - Hx[c]));
+                pmax = std::max(pmax, std::fabs(P[9 * e + c] - Px[c]));
+                pscale = std::max(pscale, std::fabs(Px[c]));
+            }
+        }
+        EXPECT_TRUE(hmax < 1e-14);
+        EXPECT_TRUE(pmax < 1e-12 * pscale);
+        double sum[3] = {0, 0, 0}, moment[9] = {0}, fmax = 0.0;
+        for (size_t i = 0; i < n; ++i)
+            for (int a = 0; a < 3; ++a) {
+                sum[a] += f[3 * i + a];
+                fmax = std::max(fmax, std::fabs(f[3 * i + a]));
+                for (int b = 0; b < 3; ++b) moment[3 * a + b] += f[3 * i + a] * mesh.coordinates[3 * i + b];
+            }
+        for (int c = 0; c < 3; ++c) EXPECT_TRUE(std::fabs(sum[c]) < 1e-11 * fmax);
+        for (int c = 0; c < 9; ++c) EXPECT_TRUE(std::fabs(moment[c] + V * Px[c]) < 1e-11 * V * pscale);
+        // a plain hex lattice is not a refined mesh: the reference's message
+        std::vector<int32_t> h2;
+        MeshBlock plain = HexLattice(2, 0.0, 1, &h2);
+        plain.connectivity = h2;
+        EXPECT_TRUE(Throws(
+            [&] {
+                CreateElement(ElementTopology::Hexahedron8, ApproximationSpaceType::ReproducingKernel,
+                              IntegrationSchemeType::StrainSmoothing, plain, std::make_shared<Device>(), mat, 0.9,
+                              true, LagrangianFormulationType::Total, labeler, true);
+            },
+            "exactly one active node per element"));
+    });
     Run("CreateElementTest.ReproducingKernelOnHex8MatchesOracle", [] {
         // ElementReproducingKernelHex8 (include/ElementReproducingKernel.h:317-378): one smoothed
         // cell per hex, kernel scale 1.5, the one-pass force, J2
