@@ -581,7 +581,7 @@ void adjoint_backward(Ctx &c, const double *ub_T, const double *vb_T, double *E_
                       double *v0_bar) {
     require_adjoint_ready(c);
     AdjointData &D = c.adj;
-    APHIP_REQUIRE(D.ready, "aphip_adjoint_forward has not run");
+    APHIP_REQUIRE(D.ready, "aphip_adjoint_forward has not run since the mesh was set or its nodes renumbered");
     APHIP_REQUIRE(ub_T != nullptr, "u_bar_T is NULL");
     alloc_adjoint(c);
     const int64_t n3 = 3 * c.nn;

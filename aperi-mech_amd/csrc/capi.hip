@@ -255,6 +255,9 @@ void renumber_nodes(Ctx &c, const std::vector<int32_t> &old_of_new) {
     if (c.has_mass) permute_device(c, c.mass, 1, old_of_new);
     if (c.has_fixed) permute_device(c, c.fixed, 1, old_of_new);
     if (c.pm_eig_valid) permute_device(c, c.pm_eig, 3, old_of_new);
+    permute_device(c, c.node_dt, 1, old_of_new);  // ensembles: each specimen's increment
+    // recorded adjoint checkpoints hold node state in the old numbering
+    if (c.adj.ready) c.adj.reset();
     if (c.formulation != APHIP_TOTAL_LAGRANGIAN) {
         permute_device(c, c.xc[0], 3, old_of_new);
         permute_device(c, c.xc[1], 3, old_of_new);
@@ -1186,28 +1189,30 @@ int aphip_autotune(aphip_ctx *c, double dt, int32_t trial_steps, int32_t *chosen
     APHIP_REQUIRE(!c->cells.ready && !(c->flags & APHIP_FLAG_TWO_PASS), "autotuning applies to the tiled element path");
     APHIP_REQUIRE(trial_steps > 0, "trial_steps must be positive");
     APHIP_HIP(hipSetDevice(c->device));
-    // save the simulation state in caller numbering (tilings renumber the nodes)
+    APHIP_REQUIRE(c->formulation == APHIP_TOTAL_LAGRANGIAN, "autotuning applies to the total Lagrangian formulation");
+    // save the simulation state: node fields in caller numbering (tilings renumber the
+    // nodes), element fields (never reordered by a tiling) as they lie, both halves of
+    // every N / NP1 pair
     std::vector<double> su(3 * c->nn), sv(3 * c->nn), sa(3 * c->nn), sf(3 * c->nn);
-    std::vector<double> sH(9 * c->ne), sP(9 * c->ne);
     const int64_t ns = c->state_count();
-    std::vector<double> s0(ns), s1(ns);
     const int parity = c->state_np1;
+    const double rate_dt = c->rate_dt;
     get_node_field(*c, c->u, su.data());
     get_node_field(*c, c->v, sv.data());
     get_node_field(*c, c->a, sa.data());
     get_node_field(*c, c->f, sf.data());
     const int prev_te = c->tiles.te;
-    std::vector<std::vector<double>> ssave;
-    {
-        std::vector<double> h(9 * c->ne);
-        c->Hnp1().download(sH.data(), sH.size(), c->stream);
-        c->Pnp1().download(sP.data(), sP.size(), c->stream);
-        if (ns) {
-            c->state[0].download(s0.data(), ns, c->stream);
-            c->state[1].download(s1.data(), ns, c->stream);
-        }
-        APHIP_HIP(hipStreamSynchronize(c->stream));
-    }
+    struct Saved {
+        DevBuf<double> *b;
+        std::vector<double> h;
+    };
+    std::vector<Saved> esave;
+    for (DevBuf<double> *b : {&c->H, &c->P, &c->H1, &c->P1})
+        if (b->p) esave.push_back({b, std::vector<double>(9 * c->ne)});
+    if (ns)
+        for (DevBuf<double> *b : {&c->state[0], &c->state[1]}) esave.push_back({b, std::vector<double>(ns)});
+    for (auto &s : esave) s.b->download(s.h.data(), s.h.size(), c->stream);
+    APHIP_HIP(hipStreamSynchronize(c->stream));
     struct Cand {
         int te, threads;
         bool pipelined;
@@ -1254,13 +1259,9 @@ int aphip_autotune(aphip_ctx *c, double dt, int32_t trial_steps, int32_t *chosen
     set_node_field(*c, c->v, sv.data());
     set_node_field(*c, c->a, sa.data());
     set_node_field(*c, c->f, sf.data());
-    c->Hnp1().upload(sH.data(), sH.size(), c->stream);
-    c->Pnp1().upload(sP.data(), sP.size(), c->stream);
-    if (ns) {
-        c->state[0].upload(s0.data(), ns, c->stream);
-        c->state[1].upload(s1.data(), ns, c->stream);
-    }
+    for (auto &s : esave) s.b->upload(s.h.data(), s.h.size(), c->stream);
     c->state_np1 = parity;
+    c->rate_dt = rate_dt;
     APHIP_HIP(hipStreamSynchronize(c->stream));
     if (chosen) *chosen = (best.pipelined ? 100000000 : 0) + best.te * 10000 + best.threads;
     APHIP_API_END

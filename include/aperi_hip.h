@@ -402,7 +402,9 @@ int aphip_adjoint_forward(aphip_ctx *ctx, double time_increment, int32_t n_steps
 /* Reverse sweep of the last aphip_adjoint_forward: given the adjoint of the final
  * displacement (and optionally velocity), returns dL/dE_e, dL/dnu_e (n_elem, caller
  * order) and dL/dv0 (n_nodes x 3), recomputing each segment from its checkpoint.
- * The initial acceleration is taken as f(u0)/m.  Leaves the final forward state. */
+ * The initial acceleration is taken as f(u0)/m.  Leaves the final forward state.  A new
+ * tile plan (aphip_set_tiling, aphip_autotune) renumbers the nodes and drops the recording:
+ * the call then fails until aphip_adjoint_forward runs again. */
 int aphip_adjoint_backward(aphip_ctx *ctx, const double *u_bar_T, const double *v_bar_T, double *E_bar,
                            double *nu_bar, double *v0_bar);
 

@@ -195,3 +195,30 @@ def test_vjp_independent_of_tile_shape(gpu_available, etype, big):
             out.append(s.internal_force_vjp(fb))
     for a, b in zip(*out):
         assert np.abs(a - b).max() <= 1e-12 * np.abs(b).max()
+
+
+def test_renumbering_invalidates_the_recording(gpu_available):
+    """A new tile plan renumbers the nodes; checkpoints recorded in the old numbering are
+    dropped, and the backward pass asks for a new forward run instead of using them."""
+    X, conn = _mesh(A.HEX8, 4, 2)
+    v0 = 10.0 * M.normal(4, 3, X.size).reshape(X.shape)
+    dt = 0.5 * M.stable_dt_estimate(X, conn, 200e9, 0.3, RHO)
+    with A.Solver() as s:
+        s.set_mesh(A.HEX8, X, conn)
+        s.set_material(A.NEO_HOOKEAN, RHO, 200e9, 0.3)
+        s.compute_mass()
+        s.set_field(A.FIELD_V, v0)
+        s.initialize()
+        s.adjoint_forward(dt, 6, 3)
+        g1 = s.adjoint_backward(s.get(A.FIELD_U))
+        s.set_tiling(16, 128)
+        with pytest.raises(A.AphipError, match="renumbered"):
+            s.adjoint_backward(s.get(A.FIELD_U))
+        s.set_tiling(64, 64)
+        s.set_field(A.FIELD_V, v0)
+        s.set_field(A.FIELD_U, np.zeros_like(X))
+        s.initialize()
+        s.adjoint_forward(dt, 6, 3)
+        g2 = s.adjoint_backward(s.get(A.FIELD_U))
+    for a, b in zip(g1, g2):
+        assert np.abs(a - b).max() <= 1e-12 * np.abs(b).max()

@@ -119,3 +119,43 @@ def test_autotune_restores_state(gpu_available):
         h.step(dt, 5)
         for f in FIELDS:
             assert rel_err(g.get(f), h.get(f)) < 1e-13
+
+
+def test_autotune_keeps_each_specimens_increment(gpu_available):
+    """Autotuning an ensemble renumbers its nodes; every node must keep its own specimen's
+    time increment, so further steps agree with an untuned ensemble to rounding."""
+    specs = _specimens(6, seed=21)
+    assert len({s["dt"] for s in specs}) == len(specs)
+    with _batch(specs, 3) as g, _batch(specs, 3) as h:
+        for s in (g, h):
+            s.set_graph(False)
+        g.autotune(0.0, 2)
+        g.step(0.0, 8)
+        h.step(0.0, 8)
+        for f in FIELDS:
+            assert rel_err(g.get(f), h.get(f)) < 1e-12, f
+
+
+@pytest.mark.parametrize("kind", [A.J2, A.SVK], ids=["j2", "svk"])
+def test_autotune_restores_hex8_state(gpu_available, kind):
+    X, hc = M.hex_lattice(9, 9, 9)
+    X = M.jitter(X, 1.0 / 9, 0.1, 3, 1)
+    v0 = 40.0 * M.normal(5, 3, X.size).reshape(X.shape)
+    dt = 0.5 * M.stable_dt_estimate(X, hc, 200e9, 0.3, 7850.0)
+    with A.Solver() as g, A.Solver() as h:
+        for s in (g, h):
+            s.set_mesh(A.HEX8, X, hc)
+            s.set_material(kind, 7850.0, 200e9, 0.3, 250e6 if kind == A.J2 else 0.0,
+                           1e9 if kind == A.J2 else 0.0)
+            s.compute_mass()
+            s.set_field(A.FIELD_V, v0)
+            s.initialize()
+            s.step(dt, 4)
+        g.autotune(dt, 2)
+        g.step(dt, 6)
+        h.step(dt, 6)
+        for f in FIELDS:
+            assert rel_err(g.get(f), h.get(f)) < 1e-12, f
+        if kind == A.J2:
+            st = h.get(A.FIELD_STATE)
+            assert st.max() > 0.0 and rel_err(g.get(A.FIELD_STATE), st) < 1e-12
