@@ -94,17 +94,34 @@ def run_gpu(w, kind, steps, warmup):
                 n_step=n_step, finite=ok, dt=dt)
 
 
-def bench_batched(n_specimens=256, steps=50):
+def hex8_bytes_per_update(nn, ne, material_fields=True):
+    """Algorithmic bytes per Hex8 element-update: connectivity 32 B, H 72 B, P 72 B, the
+    per-element E and nu 16 B when the material varies per element, and the node streams
+    (200 B per node) spread over the elements."""
+    return 8 * 4 + 72 + 72 + (16 if material_fields else 0) + NODE_BYTES * nn / ne
+
+
+def roofline(bytes_per_update, updates_per_launch, ms_per_launch, kernel):
+    achieved = bytes_per_update * updates_per_launch / (ms_per_launch * 1e-3) / 1e9
+    return {"bound": "hbm", "bytes_per_update": bytes_per_update, "achieved": achieved, "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "kernel": kernel}
+
+
+def bench_batched(n_specimens=256, steps=50, tune=True):
     """Uneven Hex8 ensemble (edges log-uniform in [8, 48], E ~ U[50, 250] GPa, nu ~ U[0.2, 0.45],
     own dt per specimen): one batched context stepping all specimens (two launches per step,
     replayed from a HIP graph) against the plain loop, one solver per specimen run to
-    completion in turn.  Every field of every specimen must be bitwise identical."""
+    completion in turn.  With the default tile plan on both sides every field of every
+    specimen must be bitwise identical.  Then both sides are autotuned (the batched context
+    once, every plain specimen on its own) and timed again over the same number of steps;
+    different plans change the summation grouping, so those results agree to rounding."""
     import aperi_hip as A
     from aperi_hip import meshgen as M
     t0 = time.time()
     specs = M.ensemble(n_specimens)
     dts = [0.5 * M.stable_dt_estimate(sp["X"], sp["conn"], sp["E"], sp["nu"], sp["rho"]) for sp in specs]
     ne = sum(len(sp["conn"]) for sp in specs)
+    nn = sum(len(sp["X"]) for sp in specs)
     g = A.Solver()
     g.batch_begin(A.HEX8, A.NEO_HOOKEAN)
     for sp, dt in zip(specs, dts):
@@ -125,43 +142,98 @@ def bench_batched(n_specimens=256, steps=50):
     for s in plain:
         s.synchronize()
     log(f"[bench] ensemble set-up {time.time() - t0:.1f}s ({n_specimens} specimens, {ne} Hex8)")
-    g.set_graph(True)
-    g.synchronize()
-    t0 = time.perf_counter()
-    g.step(0.0, steps)  # the increment of each node is its specimen's dt
-    g.synchronize()
-    t_batch = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    for s, dt in zip(plain, dts):
-        s.step(dt, steps)
-        s.synchronize()
-    t_loop = time.perf_counter() - t0
-    identical = True
+
+    def run_batched():
+        g.set_graph(True)
+        g.synchronize()
+        t0 = time.perf_counter()
+        g.step(0.0, steps)  # the increment of each node is its specimen's dt
+        g.synchronize()
+        return time.perf_counter() - t0
+
+    def run_loop():
+        t0 = time.perf_counter()
+        for s, dt in zip(plain, dts):
+            s.step(dt, steps)
+            s.synchronize()
+        return time.perf_counter() - t0
+
+    def kernel_ms():
+        # per-step device time of the batched step from HIP events on the context's stream
+        g.set_graph(False)
+        g.set_profiling(True)
+        g.reset_profile()
+        g.step(0.0, 10)
+        g.synchronize()
+        n0, m0 = g.profile(0)
+        n1, m1 = g.profile(1)
+        g.set_profiling(False)
+        for s, dt in zip(plain, dts):  # keep the loop in step with the batched context
+            s.step(dt, 10)
+        return m0 / max(n0, 1), m1 / max(n1, 1)
+
     fields = (A.FIELD_U, A.FIELD_V, A.FIELD_A, A.FIELD_F, A.FIELD_H, A.FIELD_P)
-    got = {f: g.get(f) for f in fields}
-    for k, s in enumerate(plain):
-        n0, nn, e0, nel = g.batch_ranges(k)
-        for f in fields:
-            lo, hi = (e0, e0 + nel) if f in (A.FIELD_H, A.FIELD_P) else (n0, n0 + nn)
-            if not np.array_equal(got[f][lo:hi], s.get(f)):
-                identical = False
+
+    def compare(bitwise):
+        got = {f: g.get(f) for f in fields}
+        same, worst = True, 0.0
+        for k, s in enumerate(plain):
+            n0, nn_k, e0, nel = g.batch_ranges(k)
+            for f in fields:
+                lo, hi = (e0, e0 + nel) if f in (A.FIELD_H, A.FIELD_P) else (n0, n0 + nn_k)
+                a, b = got[f][lo:hi], s.get(f)
+                same = same and np.array_equal(a, b)
+                worst = max(worst, float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)))
+        return same, worst
+
+    t_batch = run_batched()
+    t_loop = run_loop()
+    identical, _ = compare(True)
+    step_ms, tile_ms = kernel_ms()
+    bpu = hex8_bytes_per_update(nn, ne, material_fields=True)
+    out = {"workload": f"{n_specimens} Hex8 cubes, edges log-uniform in [8,48] ({ne} Hex8, {nn} nodes), "
+                       f"neo-Hookean, per-specimen E, nu and dt, {steps} steps",
+           "elements": ne, "nodes": nn, "steps": steps,
+           "batched_s": t_batch, "plain_loop_s": t_loop, "speedup": t_loop / t_batch,
+           "batched_element_updates_per_s": ne * steps / t_batch,
+           "plain_loop_element_updates_per_s": ne * steps / t_loop,
+           "identical_results": identical,
+           "kernels_ms": {"step": step_ms, "tile_kernel": tile_ms},
+           "roofline": roofline(bpu, ne, step_ms, "batched step (tile kernel + shared-node kernel), HIP events"),
+           "launch": "batched: HIP graph, two launches per step for all specimens; plain: eager, one solver "
+                     "after another; default tile plan (128 Hex8 per workgroup) on both sides"}
+    if tune:
+        t0 = time.time()
+        choice = g.autotune(0.0, 3)
+        loop_choices = {}
+        for s, dt in zip(plain, dts):
+            c = s.autotune(dt, 3)
+            loop_choices[c] = loop_choices.get(c, 0) + 1
+        log(f"[bench] ensemble autotune {time.time() - t0:.1f}s (batched plan {choice})")
+        tb = run_batched()
+        tl = run_loop()
+        _, worst = compare(False)
+        out["autotuned"] = {"batched_s": tb, "plain_loop_s": tl, "speedup": tl / tb,
+                            "batched_plan": choice,
+                            "plain_loop_plans": {str(k): v for k, v in sorted(loop_choices.items())},
+                            "max_rel_diff_after_further_steps": worst,
+                            "plan_code": "elements_per_tile * 10000 + threads"}
+    for s in plain:
         s.close()
     g.close()
-    return {"workload": f"{n_specimens} Hex8 cubes, edges log-uniform in [8,48] ({ne} Hex8), neo-Hookean, "
-                        f"per-specimen E, nu and dt, {steps} steps",
-            "elements": ne, "steps": steps,
-            "batched_s": t_batch, "plain_loop_s": t_loop, "speedup": t_loop / t_batch,
-            "batched_element_updates_per_s": ne * steps / t_batch,
-            "plain_loop_element_updates_per_s": ne * steps / t_loop,
-            "identical_results": identical, "launch": "batched: HIP graph, two launches per step for all "
-                                                       "specimens; plain: eager, one solver after another"}
+    return out
 
 
 def bench_adjoint(n=64, steps=200, interval=20, n_check=32, h=1e-6):
     """Gradient of L = 1/2 |u_T - u*|^2 with respect to every element's E and nu and the
     initial velocity (64^3 Hex8, smooth E/nu fields), 200 steps with checkpoints every 20,
     against the plain forward; 32 random parameters checked by central differences of the
-    same GPU forward (h relative), with L+ - L- formed from the two displacement fields."""
+    same GPU forward (whose parity with the oracle over the 200 steps is a GPU test), with
+    L+ - L- formed from the two displacement fields.  Steps: h |theta| for E_e and nu_e;
+    an initial-velocity component, which can sit arbitrarily close to zero, takes 10 h
+    times the velocity field's amplitude (at h its smallest sensitivities sit at the
+    rounding floor of two 200-step forwards); the figures with h times the amplitude and
+    with h relative to each component's own value are reported beside it."""
     import aperi_hip as A
     from aperi_hip import meshgen as M
     w = M.adjoint_block(n)
@@ -191,6 +263,14 @@ def bench_adjoint(n=64, steps=200, interval=20, n_check=32, h=1e-6):
     s.step(dt, steps)
     s.synchronize()
     t_plain = time.perf_counter() - t0
+    # per-step device time of the forward step (HIP events, eager)
+    s.set_profiling(True)
+    s.reset_profile()
+    s.step(dt, 20)
+    s.synchronize()
+    nfw, mfw = s.profile(0)
+    s.set_profiling(False)
+    fwd_ms = mfw / max(nfw, 1)
     # forward with checkpoints + reverse sweep
     s.set_field(A.FIELD_U, np.zeros_like(X))
     s.set_field(A.FIELD_V, w["v0"])
@@ -210,38 +290,99 @@ def bench_adjoint(n=64, steps=200, interval=20, n_check=32, h=1e-6):
     picks = ([("E", int(i)) for i in rng.choice(len(conn), n_e, replace=False)] +
              [("nu", int(i)) for i in rng.choice(len(conn), n_nu, replace=False)] +
              [("v0", int(i)) for i in rng.choice(X.size, n_check - n_e - n_nu, replace=False)])
-    def central_difference(kind, i, hrel):
+    vscale = float(np.abs(w["v0"]).max())
+
+    def central_difference(kind, i, step):
         E, nu, v0 = w["E"].copy(), w["nu"].copy(), w["v0"].copy()
         arr = {"E": E, "nu": nu, "v0": v0.reshape(-1)}[kind]
         base = arr[i]
-        step = hrel * abs(base)
         arr[i] = base + step
         up = forward(E, nu, v0)
         arr[i] = base - step
         um = forward(E, nu, v0)
         return np.sum((up - um) * (0.5 * (up + um) - u_star)) / (2.0 * step)
 
-    errs, errs10, per_kind = [], [], {}
+    errs, errs_own, errs_h, errs10, per_kind = [], [], [], [], {}
     for kind, i in picks:
         ad = {"E": Eb, "nu": nub, "v0": v0b.reshape(-1)}[kind][i]
-        fd = central_difference(kind, i, h)
-        fd10 = central_difference(kind, i, 10 * h)
+        theta = {"E": w["E"], "nu": w["nu"], "v0": w["v0"].reshape(-1)}[kind][i]
+        # the documented step per kind: h |theta| for E and nu, 10 h x the velocity amplitude
+        # for initial-velocity components (whose smallest sensitivities sit at the rounding
+        # floor of the 200-step forward with h)
+        step = h * abs(theta) if kind in ("E", "nu") else 10 * h * vscale
+        fd = central_difference(kind, i, step)
         errs.append(abs(ad - fd) / max(abs(fd), 1e-300))
-        errs10.append(abs(ad - fd10) / max(abs(fd10), 1e-300))
         per_kind.setdefault(kind, []).append(errs[-1])
+        if kind == "v0":
+            fd_h = central_difference(kind, i, h * vscale)
+            errs_h.append(abs(ad - fd_h) / max(abs(fd_h), 1e-300))
+            fd_own = central_difference(kind, i, h * abs(theta))
+            errs_own.append(abs(ad - fd_own) / max(abs(fd_own), 1e-300))
+        base = abs(theta) if kind in ("E", "nu") else vscale
+        fd10 = central_difference(kind, i, 10 * h * base)
+        errs10.append(abs(ad - fd10) / max(abs(fd10), 1e-300))
     s.close()
     errs, errs10 = np.array(errs), np.array(errs10)
-    return {"workload": f"{n}^3 Hex8 ({len(conn)} el), neo-Hookean with per-element E/nu fields, "
+    ne, nn = len(conn), len(X)
+    bpu_fwd = hex8_bytes_per_update(nn, ne, material_fields=True)
+    # reverse sweep per element-update: the forward's streams again (segment recompute), the
+    # adjoint node vectors lambda_u and lambda_v read and written (4 x 24 B per node) and the
+    # per-element E / nu sensitivities read and written (32 B)
+    bpu_rev = bpu_fwd + 4 * 24 * nn / ne + 32
+    rev_ms = 1e3 * t_bwd / steps
+    return {"workload": f"{n}^3 Hex8 ({ne} el), neo-Hookean with per-element E/nu fields, "
                         f"{steps} steps, checkpoints every {interval}",
             "plain_forward_s": t_plain, "checkpointed_forward_s": t_fwd, "backward_s": t_bwd,
             "gradient_over_forward": (t_fwd + t_bwd) / t_plain,
-            "element_updates_per_s_forward": len(conn) * steps / t_plain,
-            "fd_check": {"params": len(picks), "h_rel": h, "max_rel_err": float(errs.max()),
+            "element_updates_per_s_forward": ne * steps / t_plain,
+            "kernels_ms": {"forward_step": fwd_ms, "reverse_step_wall": rev_ms},
+            "roofline": roofline(bpu_fwd, ne, fwd_ms, "forward step (tile kernel + shared-node kernel), HIP events"),
+            "roofline_reverse": roofline(bpu_rev, ne, rev_ms,
+                                         "reverse sweep per step incl. its segment recompute, wall clock"),
+            "fd_check": {"params": len(picks), "h_rel": h,
+                         "step": "h x |theta| for E_e and nu_e, 10 h x max|v0| for initial-velocity components",
+                         "max_rel_err": float(errs.max()),
                          "median_rel_err": float(np.median(errs)), "within_1e-6": int((errs < 1e-6).sum()),
                          "max_rel_err_by_kind": {k: float(max(v)) for k, v in per_kind.items()},
+                         "v0_step_h_times_amplitude": {"max_rel_err": float(max(errs_h)),
+                                                       "within_1e-6": int((np.array(errs_h) < 1e-6).sum()),
+                                                       "params": len(errs_h)},
+                         "v0_step_relative_to_component": {"max_rel_err": float(max(errs_own)),
+                                                           "within_1e-6": int((np.array(errs_own) < 1e-6).sum()),
+                                                           "params": len(errs_own)},
                          "max_rel_err_h10": float(errs10.max()), "within_1e-6_h10": int((errs10 < 1e-6).sum()),
-                         "reference": "central differences of the same GPU forward (oracle parity of that "
-                                      "forward is tested at 1e-12 per step)"}}
+                         "reference": "central differences of the same GPU forward (its parity with the oracle "
+                                      "over these 200 steps is tested at 1e-10)"}}
+
+
+def bench_hex8_forward(n=64, steps=20):
+    """The plain forward of the adjoint block alone (64^3 Hex8, neo-Hookean with per-element
+    E / nu fields): event-timed step and its roofline; used for kernel traces and counters."""
+    import aperi_hip as A
+    from aperi_hip import meshgen as M
+    w = M.adjoint_block(n)
+    X, conn, rho = w["X"], w["conn"], w["rho"]
+    dt = 0.5 * M.stable_dt_estimate(X, conn, float(w["E"].max()), float(w["nu"].min()), rho)
+    s = A.Solver()
+    s.set_mesh(A.HEX8, X, conn)
+    s.set_material(A.NEO_HOOKEAN, rho, 200e9, 0.3)
+    s.set_material_fields(w["E"], w["nu"])
+    s.compute_mass()
+    s.set_field(A.FIELD_V, w["v0"])
+    s.initialize()
+    s.step(dt, 2)
+    s.set_profiling(True)
+    s.reset_profile()
+    s.step(dt, steps)
+    s.synchronize()
+    n0, m0 = s.profile(0)
+    n1, m1 = s.profile(1)
+    s.close()
+    ne, nn = len(conn), len(X)
+    step_ms = m0 / max(n0, 1)
+    return {"workload": f"{n}^3 Hex8 ({ne} el), neo-Hookean with per-element E/nu, {steps} steps",
+            "kernels_ms": {"step": step_ms, "tile_kernel": m1 / max(n1, 1)},
+            "roofline": roofline(hex8_bytes_per_update(nn, ne, True), ne, step_ms, "step, HIP events")}
 
 
 def bench_rkpm(n_nodes_edge=64, steps=100, scale=1.5, one_pass=True):
@@ -297,7 +438,7 @@ def bench_rkpm(n_nodes_edge=64, steps=100, scale=1.5, one_pass=True):
     return {"workload": f"{n_nodes_edge}^3-node RKPM lattice ({nn} nodes, {ne} Tet4 cells), J2, kernel scale {scale}, "
                         f"{method} force, {steps} steps",
             "setup_s": t_setup, "mean_neighbors": float(num.mean()), "max_neighbors": int(num.max()),
-            "truncated_nodes": int(truncated), "cell_list_mean": lc,
+            "extra_neighbors": int(truncated), "cell_list_mean": lc,
             "cell_updates_per_s": ne * steps / wall, "ms_per_step": 1e3 * wall / steps,
             "kernels_ms": {"step": step_ms, "cell_force" if one_pass else "gather_and_cell_force": m1 / max(n1, 1),
                            "node_gather" if one_pass else "node_sum_and_scatter": m2 / max(n2, 1)},
@@ -306,8 +447,110 @@ def bench_rkpm(n_nodes_edge=64, steps=100, scale=1.5, one_pass=True):
             "yielded_cells": yielded}
 
 
-def cpu_baseline(w, kind, budget_s=12.0):
-    """Oracle (C restatement of the reference data flow) on all host cores."""
+def _affinity():
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
+# The process's CPU share, read at start-up: once libgomp loads with OMP_PROC_BIND set it
+# binds the calling thread to one place, and the affinity mask read afterwards shrinks.
+_CPUS_ALLOWED = _affinity()
+
+
+def host_cpu():
+    """CPU model and the CPUs this process may run on (its affinity mask at start-up)."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return model, _CPUS_ALLOWED
+
+
+def cpu_threads():
+    """Threads for the CPU baseline: every CPU of the affinity mask, capped by
+    OMP_NUM_THREADS when the environment sets one (the host's CPU share)."""
+    _, allowed = host_cpu()
+    env = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    return max(1, min(allowed, env) if env > 0 else allowed)
+
+
+def pin_openmp_threads():
+    """Pinned OpenMP threads (one per core, placed close together).  libgomp reads these
+    when it is loaded, which happens with the oracle library, after this call."""
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    os.environ.setdefault("OMP_PLACES", "cores")
+
+
+def bench_nodal(n=20, steps=100, f_bar=True):
+    """Nodal strain smoothing on a thex mesh, modelled on the reference's Taylor-bar
+    performance inputs (test/performance_tests/aperi-mech/taylor_bar/rkpm_fbar and
+    rkpm_nodal input.yaml: thex mesh, reproducing kernel, elastic (SVK) material with
+    E = 78.2e8 Pa, nu = 0.3, rho = 2700; F-bar: every hex a subcell, kernel scale 0.8;
+    nodal: one subcell per cell, scale 0.9): a jittered n^3 hex lattice split into Kuhn
+    tets, each tet into four hexes, an impact-like initial velocity of -373 m/s in z."""
+    import aperi_hip as A
+    from aperi_hip import meshgen as M
+    Xh, hx = M.thex_block(n)
+    rho, E, nu = 2700.0, 78.2e8, 0.3
+    scale, subdomains = (0.8, 0) if f_bar else (0.9, 1)
+    s = A.Solver()
+    s.set_mesh(A.HEX8, Xh, hx)
+    s.set_material(A.SVK, rho, E, nu)
+    s.synchronize()
+    t0 = time.perf_counter()
+    s.rk_nodal_setup(scale, subdomains, f_bar)
+    s.synchronize()
+    t_setup = time.perf_counter() - t0
+    cs, es, act, fb = s.rk_nodal_cells()
+    nnz = s.rk_nnz()
+    n_sub = int(es.max()) + 1
+    n_act = int(act.sum())
+    s.compute_mass()
+    v0 = np.zeros_like(Xh)
+    v0[:, 2] = -373.0
+    dt = 0.25 * M.stable_dt_estimate(Xh, hx, E, nu, rho)
+    s.set_field(A.FIELD_V, v0)
+    s.initialize()
+    s.step(dt, 5)
+    s.synchronize()
+    s.set_profiling(True)
+    s.reset_profile()
+    t0 = time.perf_counter()
+    s.step(dt, steps)
+    s.synchronize()
+    wall = time.perf_counter() - t0
+    n0, m0 = s.profile(0)
+    n1, m1 = s.profile(1)
+    n2, m2 = s.profile(2)
+    ok = bool(np.isfinite(s.get(A.FIELD_U)).all())
+    s.close()
+    # per subcell-update: entries (4 B id + 24 B grad phi), H and P written (144 B), the
+    # volume read (8 B); the node streams (200 B per node with degrees of freedom)
+    lc = nnz / n_sub
+    bytes_per_update = 28.0 * lc + 152.0 + 200.0 * n_act / n_sub
+    step_ms = m0 / max(n0, 1)
+    return {"workload": f"thex of a {n}^3 Kuhn-tet lattice ({len(hx)} Hex8, {len(Xh)} nodes, {n_act} active), "
+                        f"SVK, kernel scale {scale}, {'F-bar, every hex a subcell' if f_bar else 'one subcell per cell'}, "
+                        f"{steps} steps",
+            "cells": int(len(cs) - 1), "subcells": n_sub, "f_bar": fb, "setup_s": t_setup,
+            "subcell_list_mean": lc, "subcell_updates_per_s": n_sub * steps / wall, "ms_per_step": 1e3 * wall / steps,
+            "kernels_ms": {"step": step_ms, "cell_force": m1 / max(n1, 1), "node_gather": m2 / max(n2, 1)},
+            "roofline": roofline(bytes_per_update, n_sub, step_ms, "one step (cell passes + node kernel + "
+                                                                    "predictor), HIP events"),
+            "finite": ok}
+
+
+def cpu_baseline(w, kind, budget_s=12.0, single_thread=True):
+    """Oracle (C restatement of the reference data flow) on all host cores of the process's
+    CPU share, threads pinned."""
+    pin_openmp_threads()
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     X, conn = w["X"], w["conn"]
@@ -315,7 +558,8 @@ def cpu_baseline(w, kind, budget_s=12.0):
     mass = O.lumped_mass(0, X, conn, w["rho"])
     from aperi_hip import meshgen as M
     dt = 0.5 * M.stable_dt_estimate(X, conn, w["E"], w["nu"], w["rho"])
-    threads = min(os.cpu_count() or 1, 16)
+    threads = cpu_threads()
+    model, allowed = host_cpu()
     ref = O.Solver(0, X, conn, kind, lam, mu, mass)
     ref.v[:] = w["v0"]
     ref.run(dt, 0, nthreads=threads)
@@ -327,13 +571,16 @@ def cpu_baseline(w, kind, budget_s=12.0):
     ref.run(dt, n, nthreads=threads)
     el = time.perf_counter() - t0
     ne = conn.shape[0]
-    t0 = time.perf_counter()
-    ref.run(dt, 1, nthreads=1)
-    one_thread = time.perf_counter() - t0
-    return dict(value=ne * n / el, unit="element-updates/s", cores=threads, kind="port",
-                sample=f"{n} steps of the same 6M-tet neo-Hookean workload after 1 warm step "
-                       f"({el:.1f}s, OpenMP atomic scatter, oracle/liboracle.so)",
-                single_thread_value=ne / one_thread)
+    out = dict(value=ne * n / el, unit="element-updates/s", cores=threads, kind="port",
+               sample=f"{n} steps of the same 6M-tet {'neo-Hookean' if kind == 1 else 'linear-elastic'} workload "
+                      f"after 1 warm step ({el:.1f}s, OpenMP atomic scatter, oracle/liboracle.so)",
+               cpu_model=model, cpus_allowed=allowed, cpus_online=os.cpu_count(),
+               omp_proc_bind=os.environ.get("OMP_PROC_BIND"), omp_places=os.environ.get("OMP_PLACES"))
+    if single_thread:
+        t0 = time.perf_counter()
+        ref.run(dt, 1, nthreads=1)
+        out["single_thread_value"] = ne / (time.perf_counter() - t0)
+    return out
 
 
 def fp64_line(flops_per_step, step_ms):
@@ -440,11 +687,30 @@ def main():
     if not args.no_cpu:
         log("[bench] cpu baseline ...")
         line["cpu_baseline"] = cpu_baseline(w, kind)
+    if not args.no_extras:
+        # the other material of the same block: GPU step and CPU baseline
+        other = A.LINEAR_ELASTIC if kind == A.NEO_HOOKEAN else A.NEO_HOOKEAN
+        oname = "le" if other == A.LINEAR_ELASTIC else "nh"
+        log(f"[bench] {oname} variant ...")
+        ro = run_gpu(w, other, args.steps, args.warmup)
+        leg = {"material": oname, "value": ne * args.steps / ro["wall"], "unit": "element-updates/s",
+               "ms_per_step": 1e3 * ro["wall"] / args.steps,
+               "kernels_ms": {"step": ro["step_ms"], "tile_kernel": ro["force_ms"], "shared_node_kernel": ro["node_ms"]},
+               "roofline": roofline(bytes_per_update, ne, ro["step_ms"],
+                                    "one step = k_tile + k_boundary, HIP events"),
+               "fp64_valu": fp64_line(MEASURED_FP64_FLOPS.get((oname, args.n)), ro["step_ms"]),
+               "traffic": MEASURED_TRAFFIC.get((oname, args.n))}
+        if not args.no_cpu:
+            leg["cpu_baseline"] = cpu_baseline(w, other, budget_s=6.0, single_thread=False)
+        line[f"{oname}_variant"] = leg
+        log(f"[bench] {oname} variant: {leg['value']:.3e} element-updates/s, roofline {leg['roofline']['frac']:.3f}")
     del w
     if not args.no_extras:
         log("[bench] batched ensemble ...")
         line["batched"] = bench_batched()
-        log(f"[bench] batched: {line['batched']['speedup']:.2f}x, identical={line['batched']['identical_results']}")
+        log(f"[bench] batched: {line['batched']['speedup']:.2f}x, identical={line['batched']['identical_results']}, "
+            f"roofline {line['batched']['roofline']['frac']:.3f}, autotuned "
+            f"{line['batched'].get('autotuned', {}).get('speedup', float('nan')):.2f}x")
         log("[bench] rkpm ...")
         line["rkpm"] = bench_rkpm()
         log(f"[bench] rkpm: {line['rkpm']['cell_updates_per_s']:.3e} cell-updates/s, "
@@ -453,6 +719,12 @@ def main():
         line["rkpm_two_pass"] = bench_rkpm(one_pass=False)
         log(f"[bench] rkpm two-pass: {line['rkpm_two_pass']['cell_updates_per_s']:.3e} cell-updates/s, "
             f"roofline {line['rkpm_two_pass']['roofline']['frac']:.3f}")
+        for fb in (True, False):
+            key = "nodal_fbar" if fb else "nodal"
+            log(f"[bench] {key} ...")
+            line[key] = bench_nodal(f_bar=fb)
+            log(f"[bench] {key}: {line[key]['subcell_updates_per_s']:.3e} subcell-updates/s, "
+                f"setup {line[key]['setup_s']:.2f}s, roofline {line[key]['roofline']['frac']:.3f}")
         log("[bench] power method ...")
         line["power_method"] = bench_power(args.n)
         log(f"[bench] power method: {line['power_method']['iterations']} iterations, "
