@@ -158,9 +158,25 @@ struct CellData {
     bool two_pass = false;
     DevBuf<double> u_phys;      // [nn * 3] physical displacement of the last evaluation
     DevBuf<double> f_local;     // [nn * 3] cell force before the scatter
+    // one-pass method on cell tiles (runs of consecutive cells, one wave each)
+    bool tiled = false;
+    int tile_cells = 0;             // cells per tile
+    int tile_maxtn = 0;             // largest tile node set
+    int tile_lpc = 8;               // lanes per cell of the tile kernel
+    int64_t ntiles = 0, tile_total = 0;
+    DevBuf<uint16_t> lidx;          // [nnz] entry -> index in its tile's node set
+    DevBuf<int32_t> t_nstart;       // [ntiles + 1]
+    DevBuf<int32_t> t_node;         // [tile_total] internal node ids, ascending per tile
+    DevBuf<int64_t> nslot_start;    // [nn + 1] slot range of each node
+    DevBuf<int32_t> nslot;          // [tile_total] slots of each node, ascending tile
+    DevBuf<double> slots;           // [tile_total * 3] tile force rows
     void reset() {
         ready = false;
         two_pass = false;
+        tiled = false;
+        tile_cells = tile_maxtn = 0;
+        ntiles = tile_total = 0;
+        lidx.release(); t_nstart.release(); t_node.release(); nslot_start.release(); nslot.release(); slots.release();
         rstart.release(); rsrc.release(); rphi.release(); u_phys.release(); f_local.release();
         ent_cell.release(); dphi_node.release(); Paos.release();
         nnz = n_truncated = 0;

@@ -41,7 +41,9 @@
 //                       list the node, ascending), fused with the central-difference node update
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <numeric>
+#include <thread>
 #include <vector>
 
 #include "element.hpp"
@@ -1054,6 +1056,183 @@ __global__ __launch_bounds__(256) void k_rk_nodes_cells(RkNodeCellArgs A) {
     rk_finish<MODE>(A.out, i, f);
 }
 
+// ---- one-pass method on cell tiles -------------------------------------------
+// A tile is a run of consecutive (Morton-ordered) cells; its node set is the sorted
+// union of their node lists and every entry carries its node's index in that set
+// (uint16).  One wave per tile:
+//   A. stages u of the tile's nodes in LDS and clears one force row per tile node,
+//   B. takes 64 / LPC cells at a time, LPC lanes per cell: each lane loads its
+//      entries (lane j: j, j + LPC, ...; the first K stay in registers), sums its
+//      share of H = sum_t u_node(t) (x) grad phi_t from the staged u, the group
+//      combines the shares with a xor butterfly and every lane of the group
+//      evaluates the stress; the lanes then add the entry forces -V P grad phi_t of
+//      their registered entries into the LDS rows of the entries' nodes,
+//   C. writes the tile's rows to its slots (tile-major, coalesced).
+// Each entry's grad phi is read from HBM once per step: the reference's per-entry
+// atomic scatter (include/ComputeInternalForceSmoothedCell.h:312-351) becomes LDS
+// adds of one wave, in program order, and a node pass that sums each node's slots in
+// ascending tile order and finishes the node.  Both are fixed orders, so the result
+// is the same bits on every run.
+struct RkTileArgs {
+    int64_t ncells, ntiles, nblocks;
+    int cells_per_tile, maxtn;
+    const int64_t *start;
+    const uint16_t *lidx;     // [nnz] entry -> index in its tile's node set
+    const double *dphi;
+    const double *volume;
+    const double *u;
+    const int32_t *t_nstart;  // [ntiles + 1] offsets of the tile node sets
+    const int32_t *t_node;    // [total] internal node ids, ascending within a tile
+    double *slots;            // [total * 3] tile force rows, tile-major
+    double *H, *P;
+    const double *state_n;
+    double *state_np1;
+    MatParams mat;
+    const double *elem_lambda, *elem_mu;
+    const double *Hn, *Pn;
+    double rate_dt;
+};
+
+template <int KIND, int LPC, int K>
+__global__ __launch_bounds__(64) void k_rk_tile(RkTileArgs A) {
+#pragma clang fp contract(off)
+    extern __shared__ __align__(16) double rk_lds[];
+    double *U = rk_lds;
+    double *F = rk_lds + 3 * (size_t)A.maxtn;
+    const int64_t t = xcd_block(A.nblocks);
+    if (t >= A.ntiles) return;
+    const int lane = threadIdx.x;
+    const int tb = A.t_nstart[t], tn = A.t_nstart[t + 1] - tb;
+    for (int ln = lane; ln < tn; ln += 64) {
+        const int64_t gi = A.t_node[tb + ln];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            U[3 * ln + d] = A.u[3 * gi + d];
+            F[3 * ln + d] = 0.0;
+        }
+    }
+    __syncthreads();
+    constexpr int G = 64 / LPC;
+    const int grp = lane / LPC, j = lane % LPC;
+    const int64_t c0 = t * (int64_t)A.cells_per_tile;
+    const int64_t c1 = min(c0 + (int64_t)A.cells_per_tile, A.ncells);
+    for (int64_t cb = c0; cb < c1; cb += G) {
+        const int64_t c_raw = cb + grp;
+        const bool live = c_raw < c1;
+        const int64_t c = live ? c_raw : c1 - 1;  // idle groups recompute the tile's last cell, store nothing
+        const int64_t t0 = A.start[c], t1 = A.start[c + 1];
+        double H[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, P[9];
+        double gr[K][3] = {};
+        int li[K] = {};
+        if (t1 > t0) {
+#pragma unroll
+            for (int r = 0; r < K; ++r) {
+                const int64_t e = min(t0 + j + r * LPC, t1 - 1);
+                li[r] = A.lidx[e];
+                ld3(A.dphi, e, gr[r]);
+            }
+#pragma unroll
+            for (int r = 0; r < K; ++r) {
+                const bool in = t0 + j + r * LPC < t1;
+#pragma unroll
+                for (int b = 0; b < 3; ++b) gr[r][b] = in ? gr[r][b] : 0.0;
+                double uj[3];
+                ld3(U, li[r], uj);
+#pragma unroll
+                for (int a = 0; a < 3; ++a)
+#pragma unroll
+                    for (int b = 0; b < 3; ++b) H[3 * a + b] = __fma_rn(uj[a], gr[r][b], H[3 * a + b]);
+            }
+        }
+        for (int64_t e = t0 + j + K * LPC; e < t1; e += LPC) {  // entries past the registered ones
+            double uj[3], g[3];
+            ld3(U, A.lidx[e], uj);
+            ld3(A.dphi, e, g);
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+#pragma unroll
+                for (int b = 0; b < 3; ++b) H[3 * a + b] = __fma_rn(uj[a], g[b], H[3 * a + b]);
+        }
+#pragma unroll
+        for (int i = 0; i < 9; ++i) H[i] = group_sum<LPC>(H[i]);
+        MatParams m = A.mat;
+        if (A.elem_lambda) {
+            m.lambda = A.elem_lambda[c];
+            m.mu = A.elem_mu[c];
+        }
+        double snp1 = 0.0, aux = 0.0;
+        const double sn = load_state<KIND>(A.state_n, c), sn2 = load_state2<KIND>(A.state_n, c);
+        RateIn rin;
+        load_rate<KIND>(rin, A.Hn, A.Pn, A.ncells, c, A.rate_dt);
+        stress<KIND>(H, m, sn, snp1, aux, P, &rin, sn2);
+        snp1 = cell_damage_marker<KIND>(snp1, m);
+        if (live && j == 0) {
+            store_state<KIND>(A.state_np1, c, snp1, aux);
+#pragma unroll
+            for (int i = 0; i < 9; ++i) {
+                store_once(A.H + i * A.ncells + c, H[i]);
+                store_once(A.P + i * A.ncells + c, P[i]);
+            }
+        }
+        if (!live) continue;
+        const double mv = -A.volume[c];
+#pragma unroll
+        for (int r = 0; r < K; ++r) {
+            if (t0 + j + r * LPC < t1) {
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    const double fa = mv * dot3(P[3 * a], gr[r][0], P[3 * a + 1], gr[r][1], P[3 * a + 2], gr[r][2]);
+                    __hip_atomic_fetch_add(F + 3 * li[r] + a, fa, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                }
+            }
+        }
+        for (int64_t e = t0 + j + K * LPC; e < t1; e += LPC) {
+            double g[3];
+            ld3(A.dphi, e, g);
+            const int lr = A.lidx[e];
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                const double fa = mv * dot3(P[3 * a], g[0], P[3 * a + 1], g[1], P[3 * a + 2], g[2]);
+                __hip_atomic_fetch_add(F + 3 * lr + a, fa, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            }
+        }
+    }
+    __syncthreads();
+    double *dst = A.slots + 3 * (int64_t)tb;
+    for (int k = lane; k < 3 * tn; k += 64) dst[k] = F[k];
+}
+
+struct RkSlotNodeArgs {
+    RkNodeArgs out;
+    const int64_t *nslot_start;  // [n + 1] slot range of each node
+    const int32_t *nslot;        // slots (tile-node positions) of each node, ascending tile
+    const double *slots;
+};
+
+// f_i = sum of node i's tile rows in ascending tile order (lanes stride over them,
+// then the butterfly), then the node update.
+template <int MODE, int LANES>
+__global__ __launch_bounds__(256) void k_rk_slot_nodes(RkSlotNodeArgs A) {
+#pragma clang fp contract(off)
+    const int l = threadIdx.x & (LANES - 1);
+    const int64_t i_raw = xcd_block(A.out.nblocks) * (256 / LANES) + (threadIdx.x / LANES);
+    const bool live = i_raw < A.out.nn;
+    const int64_t i = live ? i_raw : A.out.nn - 1;
+    double f[3] = {0.0, 0.0, 0.0};
+    const int64_t s0 = A.nslot_start[i], s1 = A.nslot_start[i + 1];
+    for (int64_t s = s0 + l; s < s1; s += LANES) {
+        double r[3];
+        ld3(A.slots, A.nslot[s], r);
+        f[0] += r[0];
+        f[1] += r[1];
+        f[2] += r[2];
+    }
+#pragma unroll
+    for (int d = 0; d < 3; ++d) f[d] = group_sum<LANES>(f[d]);
+    if (!live || l != 0) return;
+    rk_finish<MODE>(A.out, i, f);
+}
+
 // ---- two-pass method: gather before, scatter after the cell force -----------
 struct RkGatherArgs {
     int64_t nn, nblocks;
@@ -1542,6 +1721,97 @@ void rk_cells_and_order(Ctx &c, bool two_pass, bool elem_cells) {
     rk_cells_order(c, two_pass);
 }
 
+// LDS of one tile-kernel workgroup (u and force rows of the tile nodes) is kept to
+// 64 KiB so that several tiles share a CU.
+constexpr int64_t kTileLdsMax = 64 * 1024;
+
+// Cell tiles of the one-pass force: runs of tile_cells consecutive cells, the sorted
+// union of their node lists, every entry's index in it, and every node's slots (its
+// positions in the tile-major node sets, ascending tile).  The tile size starts at 64
+// cells (256 for cells of at most four entries; APHIP_RK_TILE_CELLS overrides it) and
+// is halved until every tile's node set fits the LDS budget.
+static void rk_tile_plan(Ctx &c) {
+    CellData &D = c.cells;
+    const int64_t nc = D.ncells, nnz = D.nnz, nn = c.nn;
+    std::vector<int64_t> hs(nc + 1, 0);
+    std::vector<int32_t> hn(std::max<int64_t>(nnz, 1));
+    D.start.download(hs.data(), nc + 1, c.stream);
+    if (nnz) D.nodes.download(hn.data(), nnz, c.stream);
+    APHIP_HIP(hipStreamSynchronize(c.stream));
+    int64_t maxlen = 0;
+    for (int64_t k = 0; k < nc; ++k) maxlen = std::max(maxlen, hs[k + 1] - hs[k]);
+    D.tile_lpc = maxlen <= 4 ? 4 : (maxlen <= 96 ? 8 : 16);
+    int tc = D.tile_lpc == 4 ? 256 : 64;
+    if (const char *env = std::getenv("APHIP_RK_TILE_CELLS")) {
+        const int v = std::atoi(env);
+        if (v >= 1 && v <= 4096) tc = v;
+    }
+    const int nthreads = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    std::vector<uint16_t> lidx(std::max<int64_t>(nnz, 1));
+    std::vector<std::vector<int32_t>> sets;
+    int64_t nt = 0;
+    int maxtn = 0;
+    while (true) {
+        nt = nc ? (nc + tc - 1) / tc : 0;
+        sets.assign(nt, {});
+        std::vector<int> tile_max(nthreads, 0);
+        auto work = [&](int w) {
+            std::vector<int32_t> loc(nn, 0);
+            for (int64_t t = w; t < nt; t += nthreads) {
+                const int64_t e0 = hs[t * tc], e1 = hs[std::min<int64_t>((t + 1) * tc, nc)];
+                std::vector<int32_t> &u = sets[t];
+                u.assign(hn.begin() + e0, hn.begin() + e1);
+                std::sort(u.begin(), u.end());
+                u.erase(std::unique(u.begin(), u.end()), u.end());
+                tile_max[w] = std::max(tile_max[w], (int)u.size());
+                if (u.size() > 65535) continue;  // too large: the caller halves the tile
+                for (size_t k = 0; k < u.size(); ++k) loc[u[k]] = (int32_t)k;
+                for (int64_t e = e0; e < e1; ++e) lidx[e] = (uint16_t)loc[hn[e]];
+            }
+        };
+        std::vector<std::thread> pool;
+        for (int w = 1; w < nthreads; ++w) pool.emplace_back(work, w);
+        work(0);
+        for (auto &th : pool) th.join();
+        maxtn = *std::max_element(tile_max.begin(), tile_max.end());
+        if ((int64_t)maxtn * 48 <= kTileLdsMax || tc == 1) break;
+        tc = std::max(1, tc / 2);
+    }
+    APHIP_REQUIRE((int64_t)maxtn * 48 <= kTileLdsMax, "a smoothing cell names too many nodes for the tile kernel");
+    std::vector<int32_t> tstart(nt + 1, 0);
+    for (int64_t t = 0; t < nt; ++t) {
+        APHIP_REQUIRE((int64_t)tstart[t] + (int64_t)sets[t].size() < (int64_t)INT32_MAX, "too many tile nodes");
+        tstart[t + 1] = tstart[t] + (int32_t)sets[t].size();
+    }
+    const int64_t total = tstart[nt];
+    std::vector<int32_t> tnode(std::max<int64_t>(total, 1));
+    std::vector<int64_t> sstart(nn + 1, 0);
+    for (int64_t t = 0; t < nt; ++t) {
+        std::copy(sets[t].begin(), sets[t].end(), tnode.begin() + tstart[t]);
+        for (int32_t g : sets[t]) sstart[g + 1]++;
+    }
+    for (int64_t i = 0; i < nn; ++i) sstart[i + 1] += sstart[i];
+    std::vector<int32_t> nslot(std::max<int64_t>(total, 1));
+    {
+        std::vector<int64_t> fill(sstart.begin(), sstart.end() - 1);
+        for (int64_t k = 0; k < total; ++k) nslot[fill[tnode[k]]++] = (int32_t)k;  // tiles in ascending order
+    }
+    sets.clear();
+    D.tile_cells = tc;
+    D.tile_maxtn = maxtn;
+    D.ntiles = nt;
+    D.tile_total = total;
+    D.lidx.upload(lidx.data(), lidx.size(), c.stream);
+    D.t_nstart.upload(tstart.data(), tstart.size(), c.stream);
+    D.t_node.upload(tnode.data(), tnode.size(), c.stream);
+    D.nslot_start.upload(sstart.data(), sstart.size(), c.stream);
+    D.nslot.upload(nslot.data(), nslot.size(), c.stream);
+    D.slots.alloc(3 * std::max<int64_t>(total, 1));
+    D.slots.zero(c.stream);
+    APHIP_HIP(hipStreamSynchronize(c.stream));
+    D.tiled = true;
+}
+
 // Reverse neighbor lists and the node order of the cell entries (after the cell CSR).
 void rk_cells_order(Ctx &c, bool two_pass) {
     CellData &D = c.cells;
@@ -1575,8 +1845,15 @@ void rk_cells_order(Ctx &c, bool two_pass) {
         APHIP_HIP(hipStreamSynchronize(c.stream));
     }
 
+    // the one-pass force without F-bar runs on cell tiles; the rest needs the entries
+    // in node order
+    const bool tiled = !two_pass && !D.fbar;
+    if (tiled) {
+        rk_tile_plan(c);
+        D.inc_cell.release();
+    }
     // node -> entries naming it, ascending entry (stable radix sort by node)
-    {
+    if (!tiled) {
         DevBuf<int32_t> vals, keys_sorted, ccount;
         vals.alloc(std::max<int64_t>(D.nnz, 1));
         keys_sorted.alloc(std::max<int64_t>(D.nnz, 1));
@@ -1605,9 +1882,41 @@ void rk_cells_order(Ctx &c, bool two_pass) {
         }
     }
     if (two_pass) D.g.alloc(3 * std::max<int64_t>(D.nnz, 1));
-    else D.Paos.alloc(kRowLen * std::max<int64_t>(D.ncells, 1));
+    else if (!tiled) D.Paos.alloc(kRowLen * std::max<int64_t>(D.ncells, 1));
     APHIP_HIP(hipStreamSynchronize(c.stream));
     D.ready = true;
+}
+
+template <int KIND>
+static void launch_rk_tile(const RkTileArgs &T, int lpc, dim3 g, size_t lds, hipStream_t s) {
+    if (lpc == 4) hipLaunchKernelGGL((k_rk_tile<KIND, 4, 1>), g, dim3(64), lds, s, T);
+    else if (lpc == 8) hipLaunchKernelGGL((k_rk_tile<KIND, 8, 12>), g, dim3(64), lds, s, T);
+    else hipLaunchKernelGGL((k_rk_tile<KIND, 16, 8>), g, dim3(64), lds, s, T);
+}
+
+static void rk_tile_force(Ctx &c) {
+    CellData &D = c.cells;
+    if (D.ntiles == 0) return;
+    const int64_t nbt = ((D.ntiles + kNumXcd - 1) / kNumXcd) * kNumXcd;
+    RkTileArgs T{D.ncells, D.ntiles, nbt, D.tile_cells, D.tile_maxtn, D.start.p, D.lidx.p, D.dphi.p, D.volume.p,
+                 c.u.p, D.t_nstart.p, D.t_node.p, D.slots.p, c.Hnp1().p, c.Pnp1().p,
+                 c.state[1 - c.state_np1].p, c.state[c.state_np1].p, c.mat.params(),
+                 c.has_fields ? c.elem_lambda.p : nullptr, c.has_fields ? c.elem_mu.p : nullptr,
+                 c.mat.rate_form() ? c.Hn().p : nullptr, c.mat.rate_form() ? c.Pn().p : nullptr, c.rate_dt};
+    const dim3 g((unsigned)nbt);
+    const size_t lds = 48 * (size_t)std::max(D.tile_maxtn, 1);
+    const int lpc = D.tile_lpc;
+    switch (c.mat.kind) {
+        case APHIP_LINEAR_ELASTIC: launch_rk_tile<APHIP_LINEAR_ELASTIC>(T, lpc, g, lds, c.stream); break;
+        case APHIP_NEO_HOOKEAN: launch_rk_tile<APHIP_NEO_HOOKEAN>(T, lpc, g, lds, c.stream); break;
+        case APHIP_J2: launch_rk_tile<APHIP_J2>(T, lpc, g, lds, c.stream); break;
+        case APHIP_SVK: launch_rk_tile<APHIP_SVK>(T, lpc, g, lds, c.stream); break;
+        case APHIP_NEO_HOOKEAN_DAMAGE: launch_rk_tile<APHIP_NEO_HOOKEAN_DAMAGE>(T, lpc, g, lds, c.stream); break;
+        case APHIP_POWER_LAW_CREEP: launch_rk_tile<APHIP_POWER_LAW_CREEP>(T, lpc, g, lds, c.stream); break;
+        case APHIP_DRUCKER_PRAGER: launch_rk_tile<APHIP_DRUCKER_PRAGER>(T, lpc, g, lds, c.stream); break;
+        default: throw Error(APHIP_ERR_INVALID, "material not set");
+    }
+    APHIP_HIP(hipGetLastError());
 }
 
 void rk_cell_force(Ctx &c) {
@@ -1619,6 +1928,10 @@ void rk_cell_force(Ctx &c) {
         APHIP_HIP(hipGetLastError());
     }
     if (D.ncells == 0) return;
+    if (D.tiled) {
+        rk_tile_force(c);
+        return;
+    }
     const int64_t nbc = padded_groups(D.ncells, D.two_pass ? 1 : kCellLanes);
     CellForceArgs A{D.ncells, nbc, D.start.p, D.nodes.p, D.dphi.p, D.volume.p, D.two_pass ? D.u_phys.p : c.u.p,
                     c.Hnp1().p, c.Pnp1().p, D.g.p, D.entry_pos.p,
@@ -1714,6 +2027,19 @@ void rk_node_update(Ctx &c, double dt, int mode) {
     }
     // nodes with degrees of freedom only (nodal cells put them first; the others keep
     // their values, as the reference's integrator runs over the active part)
+    if (D.tiled) {
+        constexpr int kSlotLanes = 8;
+        const int64_t nbs = padded_groups(c.n_dof, kSlotLanes);
+        RkSlotNodeArgs N{A, D.nslot_start.p, D.nslot.p, D.slots.p};
+        N.out.nn = c.n_dof;
+        N.out.nblocks = nbs;
+        const dim3 gs((unsigned)nbs);
+        if (mode == 0) hipLaunchKernelGGL((k_rk_slot_nodes<0, kSlotLanes>), gs, dim3(256), 0, c.stream, N);
+        else if (mode == 1) hipLaunchKernelGGL((k_rk_slot_nodes<1, kSlotLanes>), gs, dim3(256), 0, c.stream, N);
+        else hipLaunchKernelGGL((k_rk_slot_nodes<2, kSlotLanes>), gs, dim3(256), 0, c.stream, N);
+        APHIP_HIP(hipGetLastError());
+        return;
+    }
     const int64_t nb = padded_groups(c.n_dof, kNodeLanes);
     RkNodeCellArgs N{A, D.ent_cell.p, D.dphi_node.p, D.Paos.p};
     N.out.nn = c.n_dof;
