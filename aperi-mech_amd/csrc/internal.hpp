@@ -132,10 +132,7 @@ struct CellData {
     DevBuf<int32_t> node_inc;   // unused (kept for layout stability)
     DevBuf<int32_t> entry_pos;  // [nnz] position of each entry in node order (by node, then ascending entry)
     DevBuf<double> g;           // [nnz * 3] two-pass: per-entry force contributions of the last evaluation, node order
-    // one-pass: the node kernel recomputes each entry's force from its cell's P and volume
-    DevBuf<int32_t> ent_cell;   // [nnz] cell of the k-th entry in node order
-    DevBuf<double> dphi_node;   // [nnz * 3] grad phi of the k-th entry in node order
-    DevBuf<double> Paos;        // [ne * 10] P of the last evaluation (row-major) and -V, one row per cell
+    DevBuf<double> Paos;        // F-bar: [ncells * 10] P (row-major) and -V of the last evaluation, one row per subcell
     DevBuf<int64_t> rstart;     // [nn + 1] reverse neighbor lists: points i that list node J
     DevBuf<int32_t> rsrc;       // [nrev] i, ascending per node
     DevBuf<double> rphi;        // [nrev] phi_iJ
@@ -149,6 +146,8 @@ struct CellData {
     DevBuf<double> fcell_volume;    // [n_fcells]
     DevBuf<double> Jsub;            // [ncells] F-bar: J of each subcell
     DevBuf<double> Pbar;            // [ncells * 9] F-bar: stress of H-bar, row-major
+    DevBuf<double> dterm;           // [ncells] F-bar: each subcell's term of the dual pressure
+    DevBuf<int32_t> sub_fcell;      // [ncells] F-bar: nodal cell of each subcell
     DevBuf<double> evol;            // [ne] smoothed volume of every element (nodal cells)
     std::vector<int32_t> sub_of_elem;     // [ne] subcell of each internal element (nodal cells)
     std::vector<int32_t> sub_first_elem;  // [ncells] first internal element of each subcell
@@ -178,11 +177,12 @@ struct CellData {
         ntiles = tile_total = 0;
         lidx.release(); t_nstart.release(); t_node.release(); nslot_start.release(); nslot.release(); slots.release();
         rstart.release(); rsrc.release(); rphi.release(); u_phys.release(); f_local.release();
-        ent_cell.release(); dphi_node.release(); Paos.release();
+        Paos.release();
         nnz = n_truncated = 0;
         ncells = n_fcells = 0;
         nodal = fbar = false;
         fcell_sub.release(); fcell_volume.release(); Jsub.release(); Pbar.release(); evol.release();
+        dterm.release(); sub_fcell.release();
         sub_of_elem.clear(); sub_first_elem.clear(); fcell_sub_h.clear();
         num_nbr.release(); nbr.release(); phi.release(); radius.release(); start.release();
         nodes.release(); dphi.release(); volume.release(); inc_cell.release(); node_start.release();

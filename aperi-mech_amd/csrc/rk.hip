@@ -591,38 +591,17 @@ __global__ __launch_bounds__(256) void k_invert_perm(int64_t n, const int32_t *p
     if (k < n) pos[perm[k]] = (int32_t)k;
 }
 
-// node-order copies: dphi_node[k] = dphi[inc[k]], ent_cell[k] = cell of entry inc[k]
-__global__ __launch_bounds__(256) void k_node_order(int64_t n, const int32_t *inc, const double *dphi,
-                                                    const int32_t *entry_cell, double *dphi_node, int32_t *ent_cell) {
-    const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (k >= n) return;
-    const int64_t t = inc[k];
-    dphi_node[3 * k] = dphi[3 * t];
-    dphi_node[3 * k + 1] = dphi[3 * t + 1];
-    dphi_node[3 * k + 2] = dphi[3 * t + 2];
-    ent_cell[k] = entry_cell[t];
-}
-
 __global__ __launch_bounds__(256) void k_count_keys(int64_t n, const int32_t *keys, int32_t *count) {
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (i < n) atomicAdd(&count[keys[i]], 1);
 }
 
 // ---- smoothed-cell force step --------------------------------------------
-// Cells: a group of 16 lanes per cell.  The lanes stride over the cell's CSR entries
-// (entry t goes to lane t mod 16, ascending within a lane), accumulate their share
-// of H = sum_t u_node(t) (x) grad phi_t with fused multiply-adds and combine the 16
-// partial sums with a xor butterfly; addition is commutative, so every lane ends
-// with the same bits and evaluates the stress update itself (no broadcast).  The
-// cell stores H, P and P once more as one row per cell.
-// Nodes (one-pass): a group of kNodeLanes lanes per node walks its contiguous run of
-// entries in node order (all entries naming node 0, ascending entry, then node 1 ...),
-// each with its grad phi copied into that order and its cell, and sums the entry forces
-// -V P grad phi recomputed from the cell's row (ascending entry, strided over the
-// lanes, then the same butterfly): the deterministic replacement of the reference's
-// per-entry atomic scatter (include/ComputeInternalForceSmoothedCell.h:312-351)
-// without a per-entry force array in HBM.  The two-pass cells (four entries) keep
-// their entry forces in node order (g) for the plain node sum.
+// Cell and node kernels of the smoothed-cell force.  Lanes in groups of 16 stride over
+// a cell's (or node's) entries, accumulate their share of the sum and combine the 16
+// partial sums with a xor butterfly; addition is commutative, so every lane ends with
+// the same bits.  The one-pass force runs on cell tiles (below); the two-pass cells
+// (four entries) keep their entry forces in node order (g) for the plain node sum.
 constexpr int kCellLanes = 16;
 
 struct CellForceArgs {
@@ -641,17 +620,11 @@ struct CellForceArgs {
     const double *elem_lambda, *elem_mu;
     const double *Hn, *Pn;  // rate-form materials: H and P of the previous step
     double rate_dt;
-    double *Paos;           // one-pass: [ne * kRowLen] rows (P row-major, then -V)
+    double *Paos;           // F-bar: [ne * kRowLen] rows (P row-major, then -V)
 };
 
-constexpr int kRowLen = 10;  // doubles per cell row of the one-pass node kernel
-constexpr int kCellCache = 6;  // entries per lane whose grad phi stays in registers
+constexpr int kRowLen = 10;  // doubles per cell row of the F-bar scatter
 constexpr int kNodeBatch = 4;  // incidences per lane whose loads are issued together
-// one-pass node kernel: lanes per node and entries per lane whose loads are issued
-// together (32 x 16 measured fastest on the 64^3-node lattice: 1.06 ms against 1.53 ms
-// for 16 x 4; wider batches or 64 lanes need more registers and are slower)
-constexpr int kNodeLanes = 32;
-constexpr int kNodeCellBatch = 16;
 constexpr int kNumXcd = 8;
 
 // Consecutive logical blocks go to the same XCD (the dispatcher places block b on
@@ -671,86 +644,6 @@ __device__ __forceinline__ double group_sum(double x) {
 }
 
 __device__ __forceinline__ double group_sum16(double x) { return group_sum<kCellLanes>(x); }
-
-template <int KIND, int LANES, int kCache = kCellCache>
-__global__ __launch_bounds__(256) void k_rk_cells(CellForceArgs A) {
-#pragma clang fp contract(off)
-    const int l = threadIdx.x & (LANES - 1);
-    const int64_t c_raw = xcd_block(A.nblocks) * (256 / LANES) + (threadIdx.x / LANES);
-    const bool live = c_raw < A.ne;
-    const int64_t c = live ? c_raw : A.ne - 1;  // idle groups recompute the last cell and store nothing
-    const int64_t t0 = A.start[c], t1 = A.start[c + 1];
-    double H[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, P[9];
-    // The first kCache entries of the lane: every load issued before the first
-    // use (clamped indices, absent entries weighted by zero).  Entries are
-    // accumulated in ascending order either way.
-    double gc[kCache][3] = {};
-    if (t1 > t0) {  // (an empty cell has no entry to clamp to)
-        int32_t nd[kCache];
-#pragma unroll
-        for (int r = 0; r < kCache; ++r) {
-            const int64_t t = min(t0 + l + r * LANES, t1 - 1);
-            nd[r] = A.nodes[t];
-        }
-        double uj[kCache][3];
-#pragma unroll
-        for (int r = 0; r < kCache; ++r) {
-            const int64_t t = min(t0 + l + r * LANES, t1 - 1);
-            ld3(A.u, nd[r], uj[r]);
-            ld3(A.dphi, t, gc[r]);
-        }
-#pragma unroll
-        for (int r = 0; r < kCache; ++r) {
-            const bool in = t0 + l + r * LANES < t1;
-#pragma unroll
-            for (int b = 0; b < 3; ++b) gc[r][b] = in ? gc[r][b] : 0.0;
-#pragma unroll
-            for (int a = 0; a < 3; ++a)
-#pragma unroll
-                for (int b = 0; b < 3; ++b) H[3 * a + b] = __fma_rn(uj[r][a], gc[r][b], H[3 * a + b]);
-        }
-    }
-    for (int64_t t = t0 + l + kCache * LANES; t < t1; t += LANES) {
-        double uj[3], g[3];
-        ld3(A.u, A.nodes[t], uj);
-        ld3(A.dphi, t, g);
-#pragma unroll
-        for (int a = 0; a < 3; ++a)
-#pragma unroll
-            for (int b = 0; b < 3; ++b) H[3 * a + b] = __fma_rn(uj[a], g[b], H[3 * a + b]);
-    }
-#pragma unroll
-    for (int i = 0; i < 9; ++i) H[i] = group_sum<LANES>(H[i]);
-    MatParams m = A.mat;
-    if (A.elem_lambda) {
-        m.lambda = A.elem_lambda[c];
-        m.mu = A.elem_mu[c];
-    }
-    double snp1 = 0.0, aux = 0.0;
-    const double sn = load_state<KIND>(A.state_n, c), sn2 = load_state2<KIND>(A.state_n, c);
-    RateIn rin;
-    load_rate<KIND>(rin, A.Hn, A.Pn, A.ne, c, A.rate_dt);
-    stress<KIND>(H, m, sn, snp1, aux, P, &rin, sn2);
-    snp1 = cell_damage_marker<KIND>(snp1, m);
-    if (live && l == 0) {
-        store_state<KIND>(A.state_np1, c, snp1, aux);
-#pragma unroll
-        for (int i = 0; i < 9; ++i) {
-            store_once(A.H + i * A.ne + c, H[i]);
-            store_once(A.P + i * A.ne + c, P[i]);
-        }
-    }
-    // P once more as one 80-byte row per cell, with -V in its tenth slot, for the node
-    // kernel's gathers (five aligned 16-byte loads per entry)
-    if (live) {
-        for (int q = l; q < kRowLen; q += LANES) {
-            double pl = -A.volume[c];
-#pragma unroll
-            for (int i = 0; i < 9; ++i) pl = q == i ? P[i] : pl;
-            A.Paos[kRowLen * c + q] = pl;
-        }
-    }
-}
 
 // F-bar over nodal cells (ComputeInternalForceSmoothedCell.h:120-413), first pass: the
 // subcell displacement gradient (the one-pass cell accumulation, LANES lanes per subcell)
@@ -796,86 +689,97 @@ struct FbarArgs {
     const double *state_n;
     double *state_np1;
     MatParams mat;
+    const int32_t *sub_cell;       // [ns] nodal cell of each subcell
+    double *dterm;                 // [ns] each subcell's term of its cell's dual pressure
 };
 
-// Second pass, one thread per nodal cell: J-bar = sum J_s V_s / V_cell (ComputeFbar,
-// :120-175), H-bar = H cbrt(J-bar / J_s) + (cbrt(.) - 1) I unless the ratio is within
-// 3 eps of 1, the stress of H-bar (:177-213), the dual pressure and the un-barred
-// stress P = P-bar s + (p - s / (3 J_s) P-bar : F) J_s F^-T (UnBarStress, :215-309), and
-// the node kernel's row (P, -V_s) per subcell.
+// Second and third passes, one thread per subcell s of nodal cell r: J-bar = sum J_k V_k
+// / V_cell over the cell's subcells in order (ComputeFbar, :120-175), H-bar = H
+// cbrt(J-bar / J_s) + (cbrt(.) - 1) I unless the ratio is within 3 eps of 1, the stress
+// of H-bar (:177-213) and the subcell's term of the dual pressure; then the dual pressure
+// (the cell's terms summed in order) and the un-barred stress P = P-bar s + (p - s /
+// (3 J_s) P-bar : F) J_s F^-T (UnBarStress, :215-309) with the node kernel's row (P, -V_s).
+// Every thread re-forms its cell's two sums in the same order, so each subcell sees the
+// same bits a per-cell loop would.
+__device__ __forceinline__ double fbar_jbar(const FbarArgs &A, int64_t r) {
+    const int64_t s0 = A.cell_sub[r], s1 = A.cell_sub[r + 1];
+    double jbar = 0.0;
+    for (int64_t k = s0; k < s1; ++k) jbar = jbar + A.J[k] * A.sub_volume[k];
+    return jbar / A.cell_volume[r];
+}
+
 template <int KIND>
-__global__ __launch_bounds__(256) void k_fbar_cells(FbarArgs A) {
+__global__ __launch_bounds__(256) void k_fbar_stress(FbarArgs A) {
 #pragma clang fp contract(off)
     constexpr double kEps = 2.2204460492503131e-16;
-    const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (r >= A.n_cells) return;
-    const int64_t s0 = A.cell_sub[r], s1 = A.cell_sub[r + 1];
-    const double vc = A.cell_volume[r];
-    double jbar = 0.0;
-    for (int64_t s = s0; s < s1; ++s) jbar = jbar + A.J[s] * A.sub_volume[s];
-    jbar = jbar / vc;
-    for (int64_t s = s0; s < s1; ++s) {
-        double H[9], Hb[9], Pb[9];
+    const int64_t s = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (s >= A.ns) return;
+    const double jbar = fbar_jbar(A, A.sub_cell[s]);
+    double H[9], Hb[9], Pb[9];
 #pragma unroll
-        for (int i = 0; i < 9; ++i) H[i] = A.H[i * A.ns + s];
-        const double ratio = jbar / A.J[s];
-        if (fabs(ratio - 1.0) < 3.0 * kEps) {
+    for (int i = 0; i < 9; ++i) H[i] = A.H[i * A.ns + s];
+    const double ratio = jbar / A.J[s];
+    if (fabs(ratio - 1.0) < 3.0 * kEps) {
 #pragma unroll
-            for (int i = 0; i < 9; ++i) Hb[i] = H[i];
-        } else {
-            const double js = cbrt(ratio);
+        for (int i = 0; i < 9; ++i) Hb[i] = H[i];
+    } else {
+        const double js = cbrt(ratio);
 #pragma unroll
-            for (int i = 0; i < 9; ++i) Hb[i] = H[i] * js;
-            Hb[0] += js - 1.0;
-            Hb[4] += js - 1.0;
-            Hb[8] += js - 1.0;
-        }
-        double snp1 = 0.0, aux = 0.0;
-        const double sn = load_state<KIND>(A.state_n, s);
-        stress<KIND>(Hb, A.mat, sn, snp1, aux, Pb);
-        store_state<KIND>(A.state_np1, s, snp1, aux);
-#pragma unroll
-        for (int i = 0; i < 9; ++i) A.Pbar[9 * s + i] = Pb[i];
+        for (int i = 0; i < 9; ++i) Hb[i] = H[i] * js;
+        Hb[0] += js - 1.0;
+        Hb[4] += js - 1.0;
+        Hb[8] += js - 1.0;
     }
+    double snp1 = 0.0, aux = 0.0;
+    const double sn = load_state<KIND>(A.state_n, s);
+    stress<KIND>(Hb, A.mat, sn, snp1, aux, Pb);
+    store_state<KIND>(A.state_np1, s, snp1, aux);
+#pragma unroll
+    for (int i = 0; i < 9; ++i) A.Pbar[9 * s + i] = Pb[i];
+    const double js = cbrt(jbar / A.J[s]);
+    const double djb = js / (3.0 * jbar);
+    double ph = 0.0;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) ph += Pb[i] * H[i];
+    const double tr = Pb[0] + Pb[4] + Pb[8];
+    A.dterm[s] = djb * A.sub_volume[s] * (ph + tr);
+}
+
+__global__ __launch_bounds__(256) void k_fbar_unbar(FbarArgs A) {
+#pragma clang fp contract(off)
+    const int64_t s = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (s >= A.ns) return;
+    const int64_t r = A.sub_cell[s];
+    const double jbar = fbar_jbar(A, r);
     double dual = 0.0;
-    for (int64_t s = s0; s < s1; ++s) {
-        const double js = cbrt(jbar / A.J[s]);
-        const double djb = js / (3.0 * jbar);
-        double ph = 0.0;
+    for (int64_t k = A.cell_sub[r]; k < A.cell_sub[r + 1]; ++k) dual += A.dterm[k];
+    dual = dual / A.cell_volume[r];
+    const double Js = A.J[s];
+    const double js = cbrt(jbar / Js);
+    double F[9], Fi[9], Pb[9], pf = 0.0;
 #pragma unroll
-        for (int i = 0; i < 9; ++i) ph += A.Pbar[9 * s + i] * A.H[i * A.ns + s];
-        const double tr = A.Pbar[9 * s] + A.Pbar[9 * s + 4] + A.Pbar[9 * s + 8];
-        dual += djb * A.sub_volume[s] * (ph + tr);
+    for (int i = 0; i < 9; ++i) {
+        F[i] = A.H[i * A.ns + s];
+        Pb[i] = A.Pbar[9 * s + i];
     }
-    dual = dual / vc;
-    for (int64_t s = s0; s < s1; ++s) {
-        const double Js = A.J[s];
-        const double js = cbrt(jbar / Js);
-        double F[9], Fi[9], Pb[9], pf = 0.0;
+    F[0] += 1.0;
+    F[4] += 1.0;
+    F[8] += 1.0;
 #pragma unroll
-        for (int i = 0; i < 9; ++i) {
-            F[i] = A.H[i * A.ns + s];
-            Pb[i] = A.Pbar[9 * s + i];
+    for (int i = 0; i < 9; ++i) pf += Pb[i] * F[i];
+    const double djj = -js / (3.0 * Js);
+    const double scalar = (dual + djj * pf) * Js;
+    inverse3(F, Fi);
+    double *row = A.Paos + kRowLen * s;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            const double p = Pb[3 * i + j] * js + scalar * Fi[3 * j + i];
+            A.P[(3 * i + j) * A.ns + s] = p;
+            row[3 * i + j] = p;
         }
-        F[0] += 1.0;
-        F[4] += 1.0;
-        F[8] += 1.0;
-#pragma unroll
-        for (int i = 0; i < 9; ++i) pf += Pb[i] * F[i];
-        const double djj = -js / (3.0 * Js);
-        const double scalar = (dual + djj * pf) * Js;
-        inverse3(F, Fi);
-        double *row = A.Paos + kRowLen * s;
-#pragma unroll
-        for (int i = 0; i < 3; ++i)
-#pragma unroll
-            for (int j = 0; j < 3; ++j) {
-                const double p = Pb[3 * i + j] * js + scalar * Fi[3 * j + i];
-                A.P[(3 * i + j) * A.ns + s] = p;
-                row[3 * i + j] = p;
-            }
-        row[9] = -A.sub_volume[s];
-    }
+    row[9] = -A.sub_volume[s];
 }
 
 // Two-pass cells, one thread per cell: the element's four nodes, their derivative
@@ -1001,59 +905,6 @@ __global__ __launch_bounds__(256) void k_rk_nodes(RkNodeArgs A) {
     for (int d = 0; d < 3; ++d) f[d] = group_sum16(f[d]);
     if (!live || l != 0) return;
     rk_finish<MODE>(A, i, f);
-}
-
-struct RkNodeCellArgs {
-    RkNodeArgs out;
-    const int32_t *ent_cell;   // [nnz] cell of each node-order entry
-    const double *dphi_node;   // [nnz * 3] its grad phi
-    const double *Paos;        // [ne * kRowLen] rows: P row-major, then -V
-};
-
-// One-pass node kernel: f_J = sum over J's node-order entries k of -V_c P_c grad phi_k
-// (c = the entry's cell), the cell kernel's former per-entry expression, in the same
-// order; then the node update.
-template <int MODE, int LANES, int BATCH>
-__global__ __launch_bounds__(256) void k_rk_nodes_cells(RkNodeCellArgs A) {
-#pragma clang fp contract(off)
-    const int l = threadIdx.x & (LANES - 1);
-    const int64_t i_raw = xcd_block(A.out.nblocks) * (256 / LANES) + (threadIdx.x / LANES);
-    const bool live = i_raw < A.out.nn;
-    const int64_t i = live ? i_raw : A.out.nn - 1;
-    double f[3] = {0.0, 0.0, 0.0};
-    const int64_t t0 = A.out.inc_start[i], t1 = A.out.inc_start[i + 1];
-    for (int64_t tb = t0 + l; tb < t1; tb += BATCH * LANES) {
-        int32_t cc[BATCH];
-        double g[BATCH][3];
-#pragma unroll
-        for (int r = 0; r < BATCH; ++r) {
-            const int64_t t = min(tb + r * LANES, t1 - 1);
-            cc[r] = A.ent_cell[t];
-            ld3(A.dphi_node, t, g[r]);
-        }
-#pragma unroll
-        for (int r = 0; r < BATCH; ++r) {
-            const double2 *row = reinterpret_cast<const double2 *>(A.Paos + kRowLen * (int64_t)cc[r]);
-            double Pr[kRowLen];
-#pragma unroll
-            for (int q = 0; q < kRowLen / 2; ++q) {
-                const double2 w = row[q];
-                Pr[2 * q] = w.x;
-                Pr[2 * q + 1] = w.y;
-            }
-            const double mv = Pr[9];
-            const bool in = tb + r * LANES < t1;
-#pragma unroll
-            for (int a = 0; a < 3; ++a) {
-                const double fa = mv * dot3(Pr[3 * a], g[r][0], Pr[3 * a + 1], g[r][1], Pr[3 * a + 2], g[r][2]);
-                f[a] += in ? fa : 0.0;
-            }
-        }
-    }
-#pragma unroll
-    for (int d = 0; d < 3; ++d) f[d] = group_sum<LANES>(f[d]);
-    if (!live || l != 0) return;
-    rk_finish<MODE>(A.out, i, f);
 }
 
 // ---- one-pass method on cell tiles -------------------------------------------
@@ -1194,6 +1045,63 @@ __global__ __launch_bounds__(64) void k_rk_tile(RkTileArgs A) {
             for (int a = 0; a < 3; ++a) {
                 const double fa = mv * dot3(P[3 * a], g[0], P[3 * a + 1], g[1], P[3 * a + 2], g[2]);
                 __hip_atomic_fetch_add(F + 3 * lr + a, fa, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            }
+        }
+    }
+    __syncthreads();
+    double *dst = A.slots + 3 * (int64_t)tb;
+    for (int k = lane; k < 3 * tn; k += 64) dst[k] = F[k];
+}
+
+// F-bar: the entry forces of cells whose stress an earlier pass left in the cell rows
+// (P row-major, then -V), added into the tile's LDS rows like the tile kernel's.
+template <int LPC>
+__global__ __launch_bounds__(64) void k_rk_tile_scatter(RkTileArgs A, const double *rows) {
+#pragma clang fp contract(off)
+    extern __shared__ __align__(16) double rk_lds[];
+    double *F = rk_lds;
+    const int64_t t = xcd_block(A.nblocks);
+    if (t >= A.ntiles) return;
+    const int lane = threadIdx.x;
+    const int tb = A.t_nstart[t], tn = A.t_nstart[t + 1] - tb;
+    for (int k = lane; k < 3 * tn; k += 64) F[k] = 0.0;
+    __syncthreads();
+    constexpr int G = 64 / LPC;
+    constexpr int B = 4;  // entries per lane whose loads are issued together
+    const int grp = lane / LPC, j = lane % LPC;
+    const int64_t c0 = t * (int64_t)A.cells_per_tile;
+    const int64_t c1 = min(c0 + (int64_t)A.cells_per_tile, A.ncells);
+    for (int64_t cb = c0; cb < c1; cb += G) {
+        const int64_t c = cb + grp;
+        if (c >= c1) continue;
+        const double2 *row = reinterpret_cast<const double2 *>(rows + kRowLen * c);
+        double P[kRowLen];
+#pragma unroll
+        for (int q = 0; q < kRowLen / 2; ++q) {
+            const double2 w = row[q];
+            P[2 * q] = w.x;
+            P[2 * q + 1] = w.y;
+        }
+        const double mv = P[9];
+        const int64_t t0 = A.start[c], t1 = A.start[c + 1];
+        for (int64_t eb = t0 + j; eb < t1; eb += B * LPC) {
+            int li[B];
+            double g[B][3];
+#pragma unroll
+            for (int r = 0; r < B; ++r) {
+                const int64_t e = min(eb + r * LPC, t1 - 1);
+                li[r] = A.lidx[e];
+                ld3(A.dphi, e, g[r]);
+            }
+#pragma unroll
+            for (int r = 0; r < B; ++r) {
+                if (eb + r * LPC < t1) {
+#pragma unroll
+                    for (int a = 0; a < 3; ++a) {
+                        const double fa = mv * dot3(P[3 * a], g[r][0], P[3 * a + 1], g[r][1], P[3 * a + 2], g[r][2]);
+                        __hip_atomic_fetch_add(F + 3 * li[r] + a, fa, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                    }
+                }
             }
         }
     }
@@ -1531,6 +1439,11 @@ static void nodal_subcells(Ctx &c, const std::vector<int64_t> &sub_start, const 
     if (D.fbar) {
         D.Jsub.alloc(std::max<int64_t>(ns, 1));
         D.Pbar.alloc(9 * std::max<int64_t>(ns, 1));
+        D.dterm.alloc(std::max<int64_t>(ns, 1));
+        std::vector<int32_t> sc_of(std::max<int64_t>(ns, 1), 0);
+        for (int64_t r = 0; r < D.n_fcells; ++r)
+            for (int64_t sc = D.fcell_sub_h[r]; sc < D.fcell_sub_h[r + 1]; ++sc) sc_of[sc] = (int32_t)r;
+        D.sub_fcell.upload(sc_of.data(), sc_of.size(), c.stream);
     }
     APHIP_HIP(hipStreamSynchronize(c.stream));
 }
@@ -1728,8 +1641,10 @@ constexpr int64_t kTileLdsMax = 64 * 1024;
 // Cell tiles of the one-pass force: runs of tile_cells consecutive cells, the sorted
 // union of their node lists, every entry's index in it, and every node's slots (its
 // positions in the tile-major node sets, ascending tile).  The tile size starts at 64
-// cells (256 for cells of at most four entries; APHIP_RK_TILE_CELLS overrides it) and
-// is halved until every tile's node set fits the LDS budget.
+// cells (256 for cells of at most four entries), is halved while the mesh gives fewer
+// than 2048 tiles, and then until every tile's node set fits the LDS budget.  The lanes
+// per cell follow the longest cell list (4 up to 4 entries, 8 up to 96, else 16).
+// APHIP_RK_TILE_CELLS and APHIP_RK_TILE_LANES override the two choices.
 static void rk_tile_plan(Ctx &c) {
     CellData &D = c.cells;
     const int64_t nc = D.ncells, nnz = D.nnz, nn = c.nn;
@@ -1741,7 +1656,13 @@ static void rk_tile_plan(Ctx &c) {
     int64_t maxlen = 0;
     for (int64_t k = 0; k < nc; ++k) maxlen = std::max(maxlen, hs[k + 1] - hs[k]);
     D.tile_lpc = maxlen <= 4 ? 4 : (maxlen <= 96 ? 8 : 16);
+    if (const char *env = std::getenv("APHIP_RK_TILE_LANES")) {
+        const int v = std::atoi(env);
+        if (v == 4 || v == 8 || v == 16) D.tile_lpc = v;
+    }
     int tc = D.tile_lpc == 4 ? 256 : 64;
+    // small meshes: at least 2048 tiles (eight waves per CU) while a tile keeps 8 cells
+    while (tc > 8 && (nc + tc - 1) / tc < 2048) tc /= 2;
     if (const char *env = std::getenv("APHIP_RK_TILE_CELLS")) {
         const int v = std::atoi(env);
         if (v >= 1 && v <= 4096) tc = v;
@@ -1845,9 +1766,9 @@ void rk_cells_order(Ctx &c, bool two_pass) {
         APHIP_HIP(hipStreamSynchronize(c.stream));
     }
 
-    // the one-pass force without F-bar runs on cell tiles; the rest needs the entries
-    // in node order
-    const bool tiled = !two_pass && !D.fbar;
+    // the one-pass force (F-bar included) runs on cell tiles; the two-pass force keeps
+    // its entry forces in node order
+    const bool tiled = !two_pass;
     if (tiled) {
         rk_tile_plan(c);
         D.inc_cell.release();
@@ -1871,18 +1792,9 @@ void rk_cells_order(Ctx &c, bool two_pass) {
         D.entry_pos.alloc(std::max<int64_t>(D.nnz, 1));
         hipLaunchKernelGGL(k_invert_perm, grid1(D.nnz, 256), dim3(256), 0, c.stream, D.nnz, inc.p, D.entry_pos.p);
         APHIP_HIP(hipGetLastError());
-        if (!two_pass) {
-            D.dphi_node.alloc(3 * std::max<int64_t>(D.nnz, 1));
-            D.ent_cell.alloc(std::max<int64_t>(D.nnz, 1));
-            if (D.nnz) {
-                hipLaunchKernelGGL(k_node_order, grid1(D.nnz, 256), dim3(256), 0, c.stream, D.nnz, inc.p, D.dphi.p,
-                                   D.inc_cell.p, D.dphi_node.p, D.ent_cell.p);
-                APHIP_HIP(hipGetLastError());
-            }
-        }
     }
     if (two_pass) D.g.alloc(3 * std::max<int64_t>(D.nnz, 1));
-    else if (!tiled) D.Paos.alloc(kRowLen * std::max<int64_t>(D.ncells, 1));
+    else if (D.fbar) D.Paos.alloc(kRowLen * std::max<int64_t>(D.ncells, 1));
     APHIP_HIP(hipStreamSynchronize(c.stream));
     D.ready = true;
 }
@@ -1894,16 +1806,33 @@ static void launch_rk_tile(const RkTileArgs &T, int lpc, dim3 g, size_t lds, hip
     else hipLaunchKernelGGL((k_rk_tile<KIND, 16, 8>), g, dim3(64), lds, s, T);
 }
 
+static RkTileArgs rk_tile_args(Ctx &c) {
+    CellData &D = c.cells;
+    const int64_t nbt = ((D.ntiles + kNumXcd - 1) / kNumXcd) * kNumXcd;
+    return RkTileArgs{D.ncells, D.ntiles, nbt, D.tile_cells, D.tile_maxtn, D.start.p, D.lidx.p, D.dphi.p,
+                      D.volume.p, c.u.p, D.t_nstart.p, D.t_node.p, D.slots.p, c.Hnp1().p, c.Pnp1().p,
+                      c.state[1 - c.state_np1].p, c.state[c.state_np1].p, c.mat.params(),
+                      c.has_fields ? c.elem_lambda.p : nullptr, c.has_fields ? c.elem_mu.p : nullptr,
+                      c.mat.rate_form() ? c.Hn().p : nullptr, c.mat.rate_form() ? c.Pn().p : nullptr, c.rate_dt};
+}
+
+static void rk_tile_scatter(Ctx &c, const double *rows) {
+    CellData &D = c.cells;
+    if (D.ntiles == 0) return;
+    const RkTileArgs T = rk_tile_args(c);
+    const dim3 g((unsigned)T.nblocks);
+    const size_t lds = 24 * (size_t)std::max(D.tile_maxtn, 1);
+    if (D.tile_lpc == 4) hipLaunchKernelGGL(k_rk_tile_scatter<4>, g, dim3(64), lds, c.stream, T, rows);
+    else if (D.tile_lpc == 8) hipLaunchKernelGGL(k_rk_tile_scatter<8>, g, dim3(64), lds, c.stream, T, rows);
+    else hipLaunchKernelGGL(k_rk_tile_scatter<16>, g, dim3(64), lds, c.stream, T, rows);
+    APHIP_HIP(hipGetLastError());
+}
+
 static void rk_tile_force(Ctx &c) {
     CellData &D = c.cells;
     if (D.ntiles == 0) return;
-    const int64_t nbt = ((D.ntiles + kNumXcd - 1) / kNumXcd) * kNumXcd;
-    RkTileArgs T{D.ncells, D.ntiles, nbt, D.tile_cells, D.tile_maxtn, D.start.p, D.lidx.p, D.dphi.p, D.volume.p,
-                 c.u.p, D.t_nstart.p, D.t_node.p, D.slots.p, c.Hnp1().p, c.Pnp1().p,
-                 c.state[1 - c.state_np1].p, c.state[c.state_np1].p, c.mat.params(),
-                 c.has_fields ? c.elem_lambda.p : nullptr, c.has_fields ? c.elem_mu.p : nullptr,
-                 c.mat.rate_form() ? c.Hn().p : nullptr, c.mat.rate_form() ? c.Pn().p : nullptr, c.rate_dt};
-    const dim3 g((unsigned)nbt);
+    const RkTileArgs T = rk_tile_args(c);
+    const dim3 g((unsigned)T.nblocks);
     const size_t lds = 48 * (size_t)std::max(D.tile_maxtn, 1);
     const int lpc = D.tile_lpc;
     switch (c.mat.kind) {
@@ -1928,7 +1857,7 @@ void rk_cell_force(Ctx &c) {
         APHIP_HIP(hipGetLastError());
     }
     if (D.ncells == 0) return;
-    if (D.tiled) {
+    if (D.tiled && !D.fbar) {
         rk_tile_force(c);
         return;
     }
@@ -1946,15 +1875,19 @@ void rk_cell_force(Ctx &c) {
         APHIP_HIP(hipGetLastError());
         FbarArgs F{D.n_fcells, D.ncells, D.fcell_sub.p, D.fcell_volume.p, D.volume.p, D.Jsub.p, c.Hnp1().p,
                    c.Pnp1().p, D.Pbar.p, D.Paos.p, c.state[1 - c.state_np1].p, c.state[c.state_np1].p,
-                   c.mat.params()};
-        const dim3 gf((unsigned)((D.n_fcells + 255) / 256));
+                   c.mat.params(), D.sub_fcell.p, D.dterm.p};
+        const dim3 gf((unsigned)((D.ncells + 255) / 256));
         switch (c.mat.kind) {
-            case APHIP_LINEAR_ELASTIC: hipLaunchKernelGGL(k_fbar_cells<APHIP_LINEAR_ELASTIC>, gf, dim3(256), 0, c.stream, F); break;
-            case APHIP_NEO_HOOKEAN: hipLaunchKernelGGL(k_fbar_cells<APHIP_NEO_HOOKEAN>, gf, dim3(256), 0, c.stream, F); break;
-            case APHIP_J2: hipLaunchKernelGGL(k_fbar_cells<APHIP_J2>, gf, dim3(256), 0, c.stream, F); break;
-            case APHIP_SVK: hipLaunchKernelGGL(k_fbar_cells<APHIP_SVK>, gf, dim3(256), 0, c.stream, F); break;
+            case APHIP_LINEAR_ELASTIC: hipLaunchKernelGGL(k_fbar_stress<APHIP_LINEAR_ELASTIC>, gf, dim3(256), 0, c.stream, F); break;
+            case APHIP_NEO_HOOKEAN: hipLaunchKernelGGL(k_fbar_stress<APHIP_NEO_HOOKEAN>, gf, dim3(256), 0, c.stream, F); break;
+            case APHIP_J2: hipLaunchKernelGGL(k_fbar_stress<APHIP_J2>, gf, dim3(256), 0, c.stream, F); break;
+            case APHIP_SVK: hipLaunchKernelGGL(k_fbar_stress<APHIP_SVK>, gf, dim3(256), 0, c.stream, F); break;
             default: throw Error(APHIP_ERR_UNSUPPORTED, "F-bar takes the linear elastic, neo-Hookean, J2 and SVK materials");
         }
+        APHIP_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_fbar_unbar, gf, dim3(256), 0, c.stream, F);
+        APHIP_HIP(hipGetLastError());
+        rk_tile_scatter(c, D.Paos.p);
         APHIP_HIP(hipGetLastError());
         return;
     }
@@ -1981,23 +1914,6 @@ void rk_cell_force(Ctx &c) {
                 break;
             case APHIP_DRUCKER_PRAGER:
                 go(k_rk_cells_two_pass<APHIP_DRUCKER_PRAGER, 4>, k_rk_cells_two_pass<APHIP_DRUCKER_PRAGER, 8>);
-                break;
-            default: throw Error(APHIP_ERR_INVALID, "material not set");
-        }
-    } else {
-        switch (c.mat.kind) {
-            case APHIP_LINEAR_ELASTIC: hipLaunchKernelGGL((k_rk_cells<APHIP_LINEAR_ELASTIC, kCellLanes>), g, dim3(256), 0, c.stream, A); break;
-            case APHIP_NEO_HOOKEAN: hipLaunchKernelGGL((k_rk_cells<APHIP_NEO_HOOKEAN, kCellLanes>), g, dim3(256), 0, c.stream, A); break;
-            case APHIP_J2: hipLaunchKernelGGL((k_rk_cells<APHIP_J2, kCellLanes>), g, dim3(256), 0, c.stream, A); break;
-            case APHIP_SVK: hipLaunchKernelGGL((k_rk_cells<APHIP_SVK, kCellLanes>), g, dim3(256), 0, c.stream, A); break;
-            case APHIP_NEO_HOOKEAN_DAMAGE:
-                hipLaunchKernelGGL((k_rk_cells<APHIP_NEO_HOOKEAN_DAMAGE, kCellLanes>), g, dim3(256), 0, c.stream, A);
-                break;
-            case APHIP_POWER_LAW_CREEP:
-                hipLaunchKernelGGL((k_rk_cells<APHIP_POWER_LAW_CREEP, kCellLanes>), g, dim3(256), 0, c.stream, A);
-                break;
-            case APHIP_DRUCKER_PRAGER:
-                hipLaunchKernelGGL((k_rk_cells<APHIP_DRUCKER_PRAGER, kCellLanes>), g, dim3(256), 0, c.stream, A);
                 break;
             default: throw Error(APHIP_ERR_INVALID, "material not set");
         }
@@ -2040,15 +1956,7 @@ void rk_node_update(Ctx &c, double dt, int mode) {
         APHIP_HIP(hipGetLastError());
         return;
     }
-    const int64_t nb = padded_groups(c.n_dof, kNodeLanes);
-    RkNodeCellArgs N{A, D.ent_cell.p, D.dphi_node.p, D.Paos.p};
-    N.out.nn = c.n_dof;
-    N.out.nblocks = nb;
-    const dim3 gn((unsigned)nb);
-    if (mode == 0) hipLaunchKernelGGL((k_rk_nodes_cells<0, kNodeLanes, kNodeCellBatch>), gn, dim3(256), 0, c.stream, N);
-    else if (mode == 1) hipLaunchKernelGGL((k_rk_nodes_cells<1, kNodeLanes, kNodeCellBatch>), gn, dim3(256), 0, c.stream, N);
-    else hipLaunchKernelGGL((k_rk_nodes_cells<2, kNodeLanes, kNodeCellBatch>), gn, dim3(256), 0, c.stream, N);
-    APHIP_HIP(hipGetLastError());
+    throw Error(APHIP_ERR_INVALID, "one-pass cells without a tile plan");
 }
 
 void rk_mass(Ctx &c, double *total) {

@@ -572,8 +572,9 @@ def cpu_baseline(w, kind, budget_s=12.0, single_thread=True):
     el = time.perf_counter() - t0
     ne = conn.shape[0]
     out = dict(value=ne * n / el, unit="element-updates/s", cores=threads, kind="port",
-               sample=f"{n} steps of the same 6M-tet {'neo-Hookean' if kind == 1 else 'linear-elastic'} workload "
-                      f"after 1 warm step ({el:.1f}s, OpenMP atomic scatter, oracle/liboracle.so)",
+               sample=f"{n} steps of the same {ne}-Tet4 {'neo-Hookean' if kind == 1 else 'linear-elastic'} workload "
+                      f"after 1 warm step ({el:.1f}s, OpenMP atomic scatter, oracle/liboracle.so, run in a "
+                      f"child process of the bench)",
                cpu_model=model, cpus_allowed=allowed, cpus_online=os.cpu_count(),
                omp_proc_bind=os.environ.get("OMP_PROC_BIND"), omp_places=os.environ.get("OMP_PLACES"))
     if single_thread:
@@ -581,6 +582,16 @@ def cpu_baseline(w, kind, budget_s=12.0, single_thread=True):
         ref.run(dt, 1, nthreads=1)
         out["single_thread_value"] = ne / (time.perf_counter() - t0)
     return out
+
+
+def cpu_baseline_child(n, material, budget_s, single_thread):
+    """cpu_baseline in a fresh interpreter (this file with --cpu-baseline-only): the
+    child's start-up affinity is the host's CPU share, and the OpenMP binding stays in it."""
+    import subprocess
+    cmd = [sys.executable, os.path.abspath(__file__), "--cpu-baseline-only", material, "--n", str(n),
+           "--cpu-budget", str(budget_s), "--single-thread", "1" if single_thread else "0"]
+    out = subprocess.run(cmd, check=True, stdout=subprocess.PIPE, text=True).stdout
+    return json.loads(out.strip().splitlines()[-1])
 
 
 def fp64_line(flops_per_step, step_ms):
@@ -633,12 +644,29 @@ def main():
     ap.add_argument("--no-extras", action="store_true", help="skip the ensemble, RKPM, power-method and adjoint measurements")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only time a device copy of BYTES (HBM yardstick) and print it")
+    ap.add_argument("--cpu-baseline-only", default=None, choices=["nh", "le"],
+                    help="only time the oracle on the block (run by the bench itself in a child process)")
+    ap.add_argument("--cpu-budget", type=float, default=12.0)
+    ap.add_argument("--single-thread", type=int, default=1)
     args = ap.parse_args()
+    if args.cpu_baseline_only:
+        okind = 1 if args.cpu_baseline_only == "nh" else 0
+        print(json.dumps(cpu_baseline(workload(args.n), okind, args.cpu_budget, bool(args.single_thread))), flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
               f"(the force step would shard by Morton tile ranges with a shared-node force exchange)",
               file=sys.stderr)
         sys.exit(2)
+    baselines = {}
+    if not args.no_cpu and not args.copy_probe:
+        # The oracle runs in child processes started before this process touches the GPU:
+        # libgomp binds the thread that loads it (OMP_PROC_BIND), which would otherwise pin
+        # this process's host work (set-up threads included) to one core.
+        log("[bench] cpu baseline ...")
+        mats = [args.material] + ([] if args.no_extras else ["le" if args.material == "nh" else "nh"])
+        for k, mat in enumerate(mats):
+            baselines[mat] = cpu_baseline_child(args.n, mat, 12.0 if k == 0 else 6.0, k == 0)
     import aperi_hip as A
     if args.copy_probe:
         ms = A.bandwidth_probe(args.copy_probe, reps=max(args.steps, 1))
@@ -684,9 +712,8 @@ def main():
     }
     if not r["finite"]:
         line["error"] = "non-finite displacement"
-    if not args.no_cpu:
-        log("[bench] cpu baseline ...")
-        line["cpu_baseline"] = cpu_baseline(w, kind)
+    if args.material in baselines:
+        line["cpu_baseline"] = baselines[args.material]
     if not args.no_extras:
         # the other material of the same block: GPU step and CPU baseline
         other = A.LINEAR_ELASTIC if kind == A.NEO_HOOKEAN else A.NEO_HOOKEAN
@@ -700,8 +727,8 @@ def main():
                                     "one step = k_tile + k_boundary, HIP events"),
                "fp64_valu": fp64_line(MEASURED_FP64_FLOPS.get((oname, args.n)), ro["step_ms"]),
                "traffic": MEASURED_TRAFFIC.get((oname, args.n))}
-        if not args.no_cpu:
-            leg["cpu_baseline"] = cpu_baseline(w, other, budget_s=6.0, single_thread=False)
+        if oname in baselines:
+            leg["cpu_baseline"] = baselines[oname]
         line[f"{oname}_variant"] = leg
         log(f"[bench] {oname} variant: {leg['value']:.3e} element-updates/s, roofline {leg['roofline']['frac']:.3f}")
     del w
