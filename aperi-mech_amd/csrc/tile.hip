@@ -223,40 +223,61 @@ __global__ __launch_bounds__(NPE == 4 ? 1024 : (QPAR ? 512 : 256), NPE == 8 && !
     RateIn rin;
     load_rate<KIND>(rin, A.Hn, A.Pn, A.ne, ec, A.rate_dt);
 
-    // A: tile CSR entries and node data into LDS
-    {
-        const int nent = A.te * NPE;  // entries of a full tile; a partial tile reads zeros past its end
-        const uint32_t *src = reinterpret_cast<const uint32_t *>(A.lcsr + t * (int64_t)nent);
-        uint32_t *dst = reinterpret_cast<uint32_t *>(L.ent);
-        if (nent / 2 == 2 * (int)blockDim.x) {  // one element per thread, Tet4: two words each
-            const uint32_t w0 = src[tid], w1 = src[tid + blockDim.x];
-            dst[tid] = w0;
-            dst[tid + blockDim.x] = w1;
-        } else {
-            for (int i = tid; i < nent / 2; i += blockDim.x) dst[i] = src[i];
+    // A: tile CSR entries and node data into LDS.  Every global load of the stage is
+    // issued before the first LDS write: the CSR words and the tile-node indices
+    // together, then the node data of the first NPT node rows of each thread.  (A load
+    // followed by its LDS write in a loop waits for that load, and vmcnt counts in
+    // order, so each iteration would add a full memory latency.)
+    constexpr int EW = NPE == 4 ? 2 : (QPAR ? 1 : 4);  // CSR words per thread: nent / 2 <= EW x blockDim
+    constexpr int NPT = NPE == 4 ? 1 : (QPAR ? 1 : 2); // node rows per thread loaded together
+    const int nw = A.te * NPE / 2;  // words of a full tile; a partial tile reads zeros past its end
+    const uint32_t *ent_src = reinterpret_cast<const uint32_t *>(A.lcsr) + t * (int64_t)nw;
+    uint32_t ew[EW];
+#pragma unroll
+    for (int r = 0; r < EW; ++r) ew[r] = ent_src[min(tid + r * (int)blockDim.x, nw - 1)];
+    int64_t gq[NPT];
+    uint32_t csrq[NPT];
+    int32_t bdq[NPT];
+#pragma unroll
+    for (int q = 0; q < NPT; ++q) {
+        // all loads unconditional (tn_node also lists the interior nodes, as ibase + ln)
+        const int lc = tb + min(tid + q * (int)blockDim.x, tn - 1);
+        gq[q] = A.tn_node[lc];
+        csrq[q] = A.tn_csr[lc];
+        bdq[q] = A.tn_bdst[lc];
+    }
+    double xq[NPT][3], uq[NPT][3], vq[NPT][3], aq[NPT][3], mq[NPT], dq[NPT];
+    unsigned fq[NPT];
+#pragma unroll
+    for (int q = 0; q < NPT; ++q) {
+        const int64_t g = gq[q];
+        ld3(A.X, g, xq[q]);
+        ld3(A.u, g, uq[q]);
+        fq[q] = load_fixed(A, g);
+        mq[q] = 1.0;
+        dq[q] = A.dt;
+        if constexpr (MODE >= 1) mq[q] = A.mass[g];
+        if constexpr (MODE == 2) {
+            ld3(A.v, g, vq[q]);
+            ld3(A.a, g, aq[q]);
+            dq[q] = load_node_dt(A, g);
         }
     }
-    for (int ln = tid; ln < tn; ln += blockDim.x) {
-        // all loads unconditional (tn_node also lists the interior nodes, as ibase + ln)
-        const int64_t g = A.tn_node[tb + ln];
-        const uint32_t csr = A.tn_csr[tb + ln];
-        const int32_t bdst = A.tn_bdst[tb + ln];
-        double x[3], u[3];
-        ld3(A.X, g, x);
-        ld3(A.u, g, u);
-        const unsigned fx = load_fixed(A, g);
-        double mm = 1.0, v[3], a[3], nd = A.dt;
-        if constexpr (MODE >= 1) mm = A.mass[g];
-        if constexpr (MODE == 2) {
-            ld3(A.v, g, v);
-            ld3(A.a, g, a);
-            nd = load_node_dt(A, g);
+    {
+        uint32_t *dst = reinterpret_cast<uint32_t *>(L.ent);
+#pragma unroll
+        for (int r = 0; r < EW; ++r) {
+            const int i = tid + r * (int)blockDim.x;
+            if (i < nw) dst[i] = ew[r];
         }
+    }
+    // Hex8 keeps the corrector data of interior nodes only (the only ones it finishes);
+    // Tet4 writes it for every node (measured faster with its 512-element tiles).
+    auto stage = [&](int ln, uint32_t csr, int32_t bdst, const double *x, const double *u, const double *v,
+                     const double *a, unsigned fx, double mm, double nd) {
         L.csr[ln] = csr;
         L.bdst[ln] = bdst;
         st3(L.X, ln, x);
-        // Hex8 keeps the corrector data of interior nodes only (the only ones it finishes);
-        // Tet4 writes it for every node (measured faster with its 512-element tiles).
         const bool interior = NPE == 4 || ln < nint;
         if (interior) {
             L.fx[ln] = (uint8_t)fx;
@@ -271,6 +292,27 @@ __global__ __launch_bounds__(NPE == 4 ? 1024 : (QPAR ? 512 : 256), NPE == 8 && !
         } else {
             st3(L.U, ln, u);
         }
+    };
+#pragma unroll
+    for (int q = 0; q < NPT; ++q) {
+        const int ln = tid + q * (int)blockDim.x;
+        if (ln < tn) stage(ln, csrq[q], bdq[q], xq[q], uq[q], vq[q], aq[q], fq[q], mq[q], dq[q]);
+    }
+    // rows past NPT x blockDim (tiles with unusually many nodes)
+    for (int ln = tid + NPT * (int)blockDim.x; ln < tn; ln += blockDim.x) {
+        const int64_t g = A.tn_node[tb + ln];
+        double x[3], u[3], v[3], a[3];
+        ld3(A.X, g, x);
+        ld3(A.u, g, u);
+        const unsigned fx = load_fixed(A, g);
+        double mm = 1.0, nd = A.dt;
+        if constexpr (MODE >= 1) mm = A.mass[g];
+        if constexpr (MODE == 2) {
+            ld3(A.v, g, v);
+            ld3(A.a, g, a);
+            nd = load_node_dt(A, g);
+        }
+        stage(ln, A.tn_csr[tb + ln], A.tn_bdst[tb + ln], x, u, v, a, fx, mm, nd);
     }
     // Every load of the wave completes here, on every path (a wave whose lanes all
     // skip the staging loop still has its element prefetch in flight).  Otherwise the
@@ -1017,6 +1059,9 @@ void tile_build(Ctx &c, int te, int threads) {
                       "tile LDS layout exceeds its allocation");
     }
     APHIP_REQUIRE(maxtn <= npe * threads, "tile has more nodes than its gather covers");
+    // the staging loads te * npe / 2 CSR words with at most 2 (Tet4), 4 (Hex8, one element
+    // per thread) or 1 (Hex8, eight lanes per element) per thread
+    APHIP_REQUIRE(te * npe / 2 <= (npe == 4 ? 2 : (threads == te ? 4 : 1)) * threads, "tile CSR exceeds the staging");
     P.use_stream = (c.flags & APHIP_FLAG_PIPELINED) != 0;
     for (int &g : P.stream_grid) g = 0;  // sized (and LDS attribute set) at the first launch
     P.ready = true;
