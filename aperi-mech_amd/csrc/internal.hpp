@@ -220,6 +220,13 @@ struct TilePlan {
     DevBuf<int32_t> bstart;           // [nb + 1] slot range of each shared node
     DevBuf<double> zero;              // [8] zeros: stands in for absent optional arrays
     DevBuf<int32_t> meta;             // [ntiles * 8] packed per-tile scalars of the pipelined step
+    // the tile kernel's copies at a fixed stride per tile (rows past a tile's end repeat
+    // its last row), so their addresses need no per-tile scalar
+    int tpad = 0;                     // node rows per tile (= maxtn)
+    DevBuf<int32_t> tnp_node;         // [ntiles * tpad]
+    DevBuf<uint32_t> tnp_csr;         // [ntiles * tpad]
+    DevBuf<int32_t> tnp_bdst;         // [ntiles * tpad]
+    DevBuf<uint16_t> lconn_p;         // [ntiles * te * npe]
     void reset() {
         ready = false;
         te = threads = maxtn = max_nint = 0;
@@ -230,6 +237,8 @@ struct TilePlan {
         tn_start.release(); tile_e0.release(); tn_ninterior.release(); int_base.release(); tn_node.release(); tn_csr.release();
         tn_bdst.release(); lcsr.release(); lconn.release(); bpart.release(); bstart.release();
         zero.release(); meta.release();
+        tpad = 0;
+        tnp_node.release(); tnp_csr.release(); tnp_bdst.release(); lconn_p.release();
     }
 };
 

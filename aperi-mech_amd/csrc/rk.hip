@@ -954,12 +954,26 @@ __global__ __launch_bounds__(64) void k_rk_tile(RkTileArgs A) {
     if (t >= A.ntiles) return;
     const int lane = threadIdx.x;
     const int tb = A.t_nstart[t], tn = A.t_nstart[t + 1] - tb;
-    for (int ln = lane; ln < tn; ln += 64) {
-        const int64_t gi = A.t_node[tb + ln];
+    // staging in batches of kStage rows per lane, every load of a batch issued before its
+    // first LDS write (a load and its write per iteration would cost a latency each)
+    constexpr int kStage = 8;
+    for (int l0 = lane; l0 < tn; l0 += 64 * kStage) {
+        int64_t gi[kStage];
 #pragma unroll
-        for (int d = 0; d < 3; ++d) {
-            U[3 * ln + d] = A.u[3 * gi + d];
-            F[3 * ln + d] = 0.0;
+        for (int r = 0; r < kStage; ++r) gi[r] = A.t_node[tb + min(l0 + 64 * r, tn - 1)];
+        double ur[kStage][3];
+#pragma unroll
+        for (int r = 0; r < kStage; ++r) ld3(A.u, gi[r], ur[r]);
+#pragma unroll
+        for (int r = 0; r < kStage; ++r) {
+            const int ln = l0 + 64 * r;
+            if (ln < tn) {
+#pragma unroll
+                for (int d = 0; d < 3; ++d) {
+                    U[3 * ln + d] = ur[r][d];
+                    F[3 * ln + d] = 0.0;
+                }
+            }
         }
     }
     __syncthreads();

@@ -63,6 +63,11 @@ struct TileArgs {
     const int32_t *tn_bdst;
     const uint16_t *lcsr;
     const uint16_t *lconn;
+    const int32_t *tnp_node;  // fixed-stride copies of tn_node / tn_csr / tn_bdst / lconn
+    const uint32_t *tnp_csr;
+    const int32_t *tnp_bdst;
+    const uint16_t *lconn_p;
+    int tpad;
     double *bpart;
     double *H, *P;
     const double *Hn, *Pn;  // rate-form materials: H and P of the previous step
@@ -190,38 +195,19 @@ __global__ __launch_bounds__(NPE == 4 ? 1024 : (QPAR ? 512 : 256), NPE == 8 && !
     constexpr int ROUNDS = (NPE == 8 && !QPAR) ? 2 : 1;
     const TileLds L = carve(lds_raw, A.maxtn, NPE == 4 ? A.maxtn : A.max_nint, A.te, ROUNDS, NPE);
     const int tid = threadIdx.x;
-    const int tb = A.tn_start[t];
-    const int tn = A.tn_start[t + 1] - tb;
-    const int nint = A.tn_ninterior[t];
-    const int64_t ibase = A.int_base[t];
-
-    // element-side prefetch: independent of the node staging below
-    const int64_t te0 = A.tile_e0[t];
-    const int nel = A.tile_e0[t + 1] - (int)te0;
     const int le = QPAR ? (tid >> 3) : tid;
-    const int64_t e = te0 + le;
-    const bool has_elem = le < nel;
-    const int64_t ec = te0 + min(le, nel - 1);  // clamped: the loads below need no branch
     int lc[NPE];
-    MatParams m = A.mat;
-    double sn = 0.0;
+    // (le < te always; the fixed-stride copy needs no per-tile scalar)
+    const int64_t ep = t * (int64_t)A.te + le;
     if constexpr (NPE == 4) {
-        const ushort4 c = reinterpret_cast<const ushort4 *>(A.lconn)[ec];
+        const ushort4 c = reinterpret_cast<const ushort4 *>(A.lconn_p)[ep];
         lc[0] = c.x; lc[1] = c.y; lc[2] = c.z; lc[3] = c.w;
     } else {
-        const ushort4 c0 = reinterpret_cast<const ushort4 *>(A.lconn)[2 * ec];
-        const ushort4 c1 = reinterpret_cast<const ushort4 *>(A.lconn)[2 * ec + 1];
+        const ushort4 c0 = reinterpret_cast<const ushort4 *>(A.lconn_p)[2 * ep];
+        const ushort4 c1 = reinterpret_cast<const ushort4 *>(A.lconn_p)[2 * ep + 1];
         lc[0] = c0.x; lc[1] = c0.y; lc[2] = c0.z; lc[3] = c0.w;
         lc[4] = c1.x; lc[5] = c1.y; lc[6] = c1.z; lc[7] = c1.w;
     }
-    load_elem_lame(A, ec, m);
-    double sn2 = 0.0;
-    if constexpr (NPE == 4) {
-        sn = load_state<KIND>(A.state_n, ec);
-        sn2 = load_state2<KIND>(A.state_n, ec);
-    }
-    RateIn rin;
-    load_rate<KIND>(rin, A.Hn, A.Pn, A.ne, ec, A.rate_dt);
 
     // A: tile CSR entries and node data into LDS.  Every global load of the stage is
     // issued before the first LDS write: the CSR words and the tile-node indices
@@ -241,10 +227,10 @@ __global__ __launch_bounds__(NPE == 4 ? 1024 : (QPAR ? 512 : 256), NPE == 8 && !
 #pragma unroll
     for (int q = 0; q < NPT; ++q) {
         // all loads unconditional (tn_node also lists the interior nodes, as ibase + ln)
-        const int lc = tb + min(tid + q * (int)blockDim.x, tn - 1);
-        gq[q] = A.tn_node[lc];
-        csrq[q] = A.tn_csr[lc];
-        bdq[q] = A.tn_bdst[lc];
+        const int64_t row = t * (int64_t)A.tpad + min(tid + q * (int)blockDim.x, A.tpad - 1);
+        gq[q] = A.tnp_node[row];
+        csrq[q] = A.tnp_csr[row];
+        bdq[q] = A.tnp_bdst[row];
     }
     double xq[NPT][3], uq[NPT][3], vq[NPT][3], aq[NPT][3], mq[NPT], dq[NPT];
     unsigned fq[NPT];
@@ -263,6 +249,27 @@ __global__ __launch_bounds__(NPE == 4 ? 1024 : (QPAR ? 512 : 256), NPE == 8 && !
             dq[q] = load_node_dt(A, g);
         }
     }
+    // per-tile scalars, read after the staging loads are out (their addresses need none)
+    const int tb = A.tn_start[t];
+    const int tn = A.tn_start[t + 1] - tb;
+    const int nint = A.tn_ninterior[t];
+    const int64_t ibase = A.int_base[t];
+    const int64_t te0 = A.tile_e0[t];
+    const int nel = A.tile_e0[t + 1] - (int)te0;
+    const int64_t e = te0 + le;
+    const bool has_elem = le < nel;
+    const int64_t ec = te0 + min(le, nel - 1);  // clamped: the loads below need no branch
+    MatParams m = A.mat;
+    double sn = 0.0;
+    load_elem_lame(A, ec, m);
+    double sn2 = 0.0;
+    if constexpr (NPE == 4) {
+        sn = load_state<KIND>(A.state_n, ec);
+        sn2 = load_state2<KIND>(A.state_n, ec);
+    }
+    RateIn rin;
+    load_rate<KIND>(rin, A.Hn, A.Pn, A.ne, ec, A.rate_dt);
+
     {
         uint32_t *dst = reinterpret_cast<uint32_t *>(L.ent);
 #pragma unroll
@@ -1025,6 +1032,29 @@ void tile_build(Ctx &c, int te, int threads) {
     P.tn_bdst.upload(tbdst.data(), tbdst.size(), c.stream);
     P.lcsr.upload(lcsr.data(), lcsr.size(), c.stream);
     P.lconn.upload(lconn.data(), lconn.size(), c.stream);
+    {  // fixed-stride copies for the tile kernel
+        P.tpad = std::max(maxtn, 1);
+        std::vector<int32_t> pn((size_t)nt * P.tpad), pb((size_t)nt * P.tpad);
+        std::vector<uint32_t> pc((size_t)nt * P.tpad);
+        std::vector<uint16_t> pl((size_t)nt * te * npe);
+        for (int64_t t = 0; t < nt; ++t) {
+            const int tn = tstart[t + 1] - tstart[t];
+            for (int r = 0; r < P.tpad; ++r) {
+                const int32_t i = tstart[t] + std::min(r, tn - 1);
+                pn[(size_t)t * P.tpad + r] = tnodes[i];
+                pc[(size_t)t * P.tpad + r] = tcsr[i];
+                pb[(size_t)t * P.tpad + r] = tbdst[i];
+            }
+            const int64_t e0 = te0v[t], nel = te0v[t + 1] - e0;
+            for (int le = 0; le < te; ++le)
+                for (int k = 0; k < npe; ++k)
+                    pl[((size_t)t * te + le) * npe + k] = lconn[(e0 + std::min<int64_t>(le, nel - 1)) * npe + k];
+        }
+        P.tnp_node.upload(pn.data(), pn.size(), c.stream);
+        P.tnp_csr.upload(pc.data(), pc.size(), c.stream);
+        P.tnp_bdst.upload(pb.data(), pb.size(), c.stream);
+        P.lconn_p.upload(pl.data(), pl.size(), c.stream);
+    }
     P.nslot = bstart[nb];
     P.bpart.alloc(3 * std::max<int64_t>(1, (int64_t)bstart[nb]));
     P.bstart.upload(bstart.data(), bstart.size(), c.stream);
@@ -1089,6 +1119,11 @@ void tile_run(Ctx &c, double dt, int mode) {
     A.tn_bdst = P.tn_bdst.p;
     A.lcsr = P.lcsr.p;
     A.lconn = P.lconn.p;
+    A.tnp_node = P.tnp_node.p;
+    A.tnp_csr = P.tnp_csr.p;
+    A.tnp_bdst = P.tnp_bdst.p;
+    A.lconn_p = P.lconn_p.p;
+    A.tpad = P.tpad;
     A.bpart = P.bpart.p;
     A.H = c.Hnp1().p;
     A.P = c.Pnp1().p;
