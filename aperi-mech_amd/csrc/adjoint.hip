@@ -228,6 +228,7 @@ struct AdjArgs {
     const uint32_t *tn_csr;
     const int32_t *tn_bdst;
     const uint16_t *lcsr, *lconn;
+    const double *zero;  // 8 zero doubles: stands in for an absent optional array
     double *bpart;
     double *lam_bar, *mu_bar;
     MatParams mat;
@@ -261,30 +262,33 @@ __global__ __launch_bounds__(NPE == 4 ? 512 : 128) void k_tile_adj(AdjArgs A) {
     const int nel = A.tile_e0[t + 1] - (int)te0;
     const int64_t e = te0 + tid;
     const bool has_elem = tid < nel;
-    {
-        const int nent = A.te * NPE;
-        const uint32_t *src = reinterpret_cast<const uint32_t *>(A.lcsr + t * (int64_t)nent);
-        uint32_t *dst = reinterpret_cast<uint32_t *>(sE);
-        for (int i = tid; i < nent / 2; i += blockDim.x) dst[i] = src[i];
-    }
-    for (int ln = tid; ln < tn; ln += blockDim.x) {
-        const bool interior = ln < nint;
-        const int64_t g = interior ? ibase + ln : (int64_t)A.tn_node[tb + ln];
-        sC[ln] = A.tn_csr[tb + ln];
-        if (!interior) sD[ln] = A.tn_bdst[tb + ln];
-        double x[3], u[3], fb[3];
+    // Staging: the CSR words, then every node row's index and data, all issued before
+    // the first LDS write (a load and its write per iteration would each wait a full
+    // memory latency).  The kernel runs one element per thread (te == blockDim), so a
+    // thread holds NPE / 2 CSR words; the first NPT rows of a thread load together.
+    constexpr int EW = NPE / 2;
+    constexpr int NPT = NPE == 4 ? 1 : 2;
+    const int nw = A.te * NPE / 2;
+    const uint32_t *ent_src = reinterpret_cast<const uint32_t *>(A.lcsr) + t * (int64_t)nw;
+    uint32_t ew[EW];
+#pragma unroll
+    for (int r = 0; r < EW; ++r) ew[r] = ent_src[min(tid + r * (int)blockDim.x, nw - 1)];
+    auto load_row = [&](int ln, double (&x)[3], double (&u)[3], double (&fb)[3]) {
+        const int64_t g = A.tn_node[tb + ln];  // interior nodes are listed as ibase + ln
         ld3(A.X, g, x);
         ld3(A.u1, g, u);
-        st3(sX, ln, x);
-        st3(sU, ln, u);
         if constexpr (MODE == 0) {
             ld3(A.fbar, g, fb);
         } else if constexpr (MODE == 1) {
             double a[3], v[3];
             ld3(A.ab, g, a);
             ld3(A.vb, g, v);
-            const double h = 0.5 * (A.node_dt ? A.node_dt[g] : A.dt);
-            const unsigned fx = A.fixed ? A.fixed[g] : 0u;
+            const double *pd = A.node_dt ? A.node_dt : A.zero;
+            const uint8_t *pf = A.fixed ? A.fixed : reinterpret_cast<const uint8_t *>(A.zero);
+            const double d8 = pd[A.node_dt ? g : 0];
+            const unsigned f8 = pf[A.fixed ? g : 0];
+            const double h = 0.5 * (A.node_dt ? d8 : A.dt);
+            const unsigned fx = A.fixed ? f8 : 0u;
             const double m = A.mass[g];
 #pragma unroll
             for (int d = 0; d < 3; ++d) fb[d] = (fx & (1u << d)) ? 0.0 : __fma_rn(h, v[d], a[d]) / m;
@@ -295,7 +299,42 @@ __global__ __launch_bounds__(NPE == 4 ? 512 : 128) void k_tile_adj(AdjArgs A) {
 #pragma unroll
             for (int d = 0; d < 3; ++d) fb[d] = a[d] / m;
         }
+    };
+    double xq[NPT][3], uq[NPT][3], bq[NPT][3];
+    uint32_t cq[NPT];
+    int32_t dq[NPT];
+#pragma unroll
+    for (int q = 0; q < NPT; ++q) {
+        const int ln = min(tid + q * (int)blockDim.x, tn - 1);  // clamped: unconditional loads
+        cq[q] = A.tn_csr[tb + ln];
+        dq[q] = A.tn_bdst[tb + ln];
+        load_row(ln, xq[q], uq[q], bq[q]);
+    }
+    {
+        uint32_t *dst = reinterpret_cast<uint32_t *>(sE);
+#pragma unroll
+        for (int r = 0; r < EW; ++r) {
+            const int i = tid + r * (int)blockDim.x;
+            if (i < nw) dst[i] = ew[r];
+        }
+    }
+    auto stage = [&](int ln, uint32_t csr, int32_t bdst, const double (&x)[3], const double (&u)[3],
+                     const double (&fb)[3]) {
+        sC[ln] = csr;
+        sD[ln] = bdst;  // -1 for interior nodes, never read for them
+        st3(sX, ln, x);
+        st3(sU, ln, u);
         st3(sB, ln, fb);
+    };
+#pragma unroll
+    for (int q = 0; q < NPT; ++q) {
+        const int ln = tid + q * (int)blockDim.x;
+        if (ln < tn) stage(ln, cq[q], dq[q], xq[q], uq[q], bq[q]);
+    }
+    for (int ln = tid + NPT * (int)blockDim.x; ln < tn; ln += blockDim.x) {
+        double x[3], u[3], fb[3];
+        load_row(ln, x, u, fb);
+        stage(ln, A.tn_csr[tb + ln], A.tn_bdst[tb + ln], x, u, fb);
     }
     __syncthreads();
     if (has_elem) {
@@ -445,6 +484,7 @@ void adj_run(Ctx &c, const double *u1, const double *fbar, int mode, double dt) 
     A.tn_bdst = P.tn_bdst.p;
     A.lcsr = P.lcsr.p;
     A.lconn = P.lconn.p;
+    A.zero = P.zero.p;
     A.bpart = P.bpart.p;
     A.lam_bar = D.lam_bar.p;
     A.mu_bar = D.mu_bar.p;

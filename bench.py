@@ -218,6 +218,23 @@ def bench_batched(n_specimens=256, steps=50, tune=True):
                             "plain_loop_plans": {str(k): v for k, v in sorted(loop_choices.items())},
                             "max_rel_diff_after_further_steps": worst,
                             "plan_code": "elements_per_tile * 10000 + threads"}
+        # the batched context's tuned plan on every plain specimen too, each specimen
+        # restarted from the batched context's state: the same tiles on both sides, hence
+        # bitwise identical results again
+        te, thr = divmod(choice % 100000000, 10000)
+        gu, gv, ga = g.get(A.FIELD_U), g.get(A.FIELD_V), g.get(A.FIELD_A)
+        for k, s in enumerate(plain):
+            n0, nn_k, _, _ = g.batch_ranges(k)
+            s.set_tiling(te, thr)
+            s.set_field(A.FIELD_U, gu[n0:n0 + nn_k])
+            s.set_field(A.FIELD_V, gv[n0:n0 + nn_k])
+            s.set_field(A.FIELD_A, ga[n0:n0 + nn_k])
+        tb2 = run_batched()
+        tl2 = run_loop()
+        same2, _ = compare(True)
+        out["same_tuned_plan"] = {"plan": choice, "batched_s": tb2, "plain_loop_s": tl2, "speedup": tl2 / tb2,
+                                  "identical_results": same2,
+                                  "note": "the batched context's autotuned plan set on every plain specimen"}
     for s in plain:
         s.close()
     g.close()
@@ -737,7 +754,9 @@ def main():
         line["batched"] = bench_batched()
         log(f"[bench] batched: {line['batched']['speedup']:.2f}x, identical={line['batched']['identical_results']}, "
             f"roofline {line['batched']['roofline']['frac']:.3f}, autotuned "
-            f"{line['batched'].get('autotuned', {}).get('speedup', float('nan')):.2f}x")
+            f"{line['batched'].get('autotuned', {}).get('speedup', float('nan')):.2f}x, same tuned plan "
+            f"{line['batched'].get('same_tuned_plan', {}).get('speedup', float('nan')):.2f}x "
+            f"(identical={line['batched'].get('same_tuned_plan', {}).get('identical_results')})")
         log("[bench] rkpm ...")
         line["rkpm"] = bench_rkpm()
         log(f"[bench] rkpm: {line['rkpm']['cell_updates_per_s']:.3e} cell-updates/s, "
