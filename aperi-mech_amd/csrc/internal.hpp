@@ -369,7 +369,10 @@ void tile_build(Ctx &c, int te, int threads);
 // Reorders the internal nodes (old_of_new[i] = current id of new node i) and every
 // node-indexed array of the context (capi.hip).
 void renumber_nodes(Ctx &c, const std::vector<int32_t> &old_of_new);
-void tile_run(Ctx &c, double dt, int mode);
+// One tiled pass (MODE 0 force only, 1 force and a = f/m, 2 the full step).  In MODE 0 the
+// displacement read and the force written may be other arrays than the context's u and f
+// (the two-pass RKPM force: physical displacement in, local force out).
+void tile_run(Ctx &c, double dt, int mode, const double *u_in = nullptr, double *f_out = nullptr);
 
 // ---- adjoint (adjoint.hip) ----
 void adjoint_forward(Ctx &c, double dt, int n_steps, int interval);

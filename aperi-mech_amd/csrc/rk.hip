@@ -1862,6 +1862,16 @@ static void rk_tile_force(Ctx &c) {
     APHIP_HIP(hipGetLastError());
 }
 
+// The two-pass cells of a Tet4 background are the elements themselves with the
+// element's own gradient (the smoothed gradient of a linear tet), so their force is the
+// Gauss Tet4 element force on the physical displacement: the element tile plan computes it
+// with the deterministic LDS gather and no per-entry force array.  Not for the damage
+// material, whose smoothed-cell update marks new failures differently
+// (ComputeInternalForceSmoothedCell.h:219-223).
+static bool two_pass_on_tiles(const Ctx &c) {
+    return c.cells.two_pass && c.etype == APHIP_TET4 && c.tiles.ready && c.mat.kind != APHIP_NEO_HOOKEAN_DAMAGE;
+}
+
 void rk_cell_force(Ctx &c) {
     CellData &D = c.cells;
     if (D.two_pass) {  // physical displacement = Phi d (ExplicitSolver::ComputeForce, Explicit.cpp:148-152)
@@ -1869,6 +1879,13 @@ void rk_cell_force(Ctx &c) {
         RkGatherArgs G{c.nn, nbg, D.num_nbr.p, D.nbr.p, D.phi.p, c.u.p, D.u_phys.p};
         hipLaunchKernelGGL(k_rk_gather, dim3((unsigned)nbg), dim3(256), 0, c.stream, G);
         APHIP_HIP(hipGetLastError());
+        if (two_pass_on_tiles(c)) {
+            const bool prof = c.prof.enabled;  // timed as this force pass, not as tile launches
+            c.prof.enabled = false;
+            tile_run(c, 0.0, 0, D.u_phys.p, D.f_local.p);
+            c.prof.enabled = prof;
+            return;
+        }
     }
     if (D.ncells == 0) return;
     if (D.tiled && !D.fbar) {
@@ -1942,12 +1959,14 @@ void rk_node_update(Ctx &c, double dt, int mode) {
     RkNodeArgs A{c.nn, nbn, D.node_start.p, D.g.p, c.mass.p,
                  c.has_fixed ? c.fixed.p : nullptr, c.f.p, c.a.p, c.v.p, dt};
     if (D.two_pass) {
-        // cell entries -> local force, then f = Phi^T f_local with the node update
-        // (ScatterValues, Explicit.cpp:181-186)
-        RkNodeArgs L = A;
-        L.f = D.f_local.p;
-        hipLaunchKernelGGL(k_rk_nodes<0>, g, dim3(256), 0, c.stream, L);
-        APHIP_HIP(hipGetLastError());
+        // cell entries -> local force (unless the element tiles assembled it), then
+        // f = Phi^T f_local with the node update (ScatterValues, Explicit.cpp:181-186)
+        if (!two_pass_on_tiles(c)) {
+            RkNodeArgs L = A;
+            L.f = D.f_local.p;
+            hipLaunchKernelGGL(k_rk_nodes<0>, g, dim3(256), 0, c.stream, L);
+            APHIP_HIP(hipGetLastError());
+        }
         RkScatterArgs S{A, D.num_nbr.p, D.rstart.p, D.rsrc.p, D.rphi.p, D.f_local.p};
         if (mode == 0) hipLaunchKernelGGL(k_rk_scatter<0>, g, dim3(256), 0, c.stream, S);
         else if (mode == 1) hipLaunchKernelGGL(k_rk_scatter<1>, g, dim3(256), 0, c.stream, S);

@@ -1098,16 +1098,17 @@ void tile_build(Ctx &c, int te, int threads) {
     APHIP_HIP(hipStreamSynchronize(c.stream));
 }
 
-void tile_run(Ctx &c, double dt, int mode) {
+void tile_run(Ctx &c, double dt, int mode, const double *u_in, double *f_out) {
     TilePlan &P = c.tiles;
     APHIP_REQUIRE(P.ready, "tile plan not built");
+    APHIP_REQUIRE(mode == 0 || (u_in == nullptr && f_out == nullptr), "displacement / force arrays only in force mode");
     if (c.ne == 0) return;
     TileArgs A;
     A.X = c.X.p;
-    A.u = c.u.p;
+    A.u = u_in ? const_cast<double *>(u_in) : c.u.p;
     A.v = c.v.p;
     A.a = c.a.p;
-    A.f = c.f.p;
+    A.f = f_out ? f_out : c.f.p;
     A.mass = c.mass.p;
     A.fixed = c.has_fixed ? c.fixed.p : nullptr;
     A.tn_start = P.tn_start.p;
@@ -1161,7 +1162,7 @@ void tile_run(Ctx &c, double dt, int mode) {
     APHIP_HIP(hipGetLastError());
     prof_end(c, 1);
     if (P.nb > 0) {
-        BoundArgs B{P.nb, P.n_interior, P.bstart.p, P.bpart.p, c.u.p, c.v.p, c.a.p, c.f.p, c.mass.p,
+        BoundArgs B{P.nb, P.n_interior, P.bstart.p, P.bpart.p, c.u.p, c.v.p, c.a.p, A.f, c.mass.p,
                     c.has_fixed ? c.fixed.p : nullptr, c.node_dt.p, P.zero.p, dt};
         const dim3 grid((unsigned)((P.nb + 255) / 256));
         prof_begin(c, 2);
