@@ -162,7 +162,9 @@ __device__ void hex8_vjp(const double (&X)[8][3], const double (&U)[8][3], const
 #pragma unroll
     for (int k = 0; k < 8; ++k) ub[k][0] = ub[k][1] = ub[k][2] = 0.0;
     lb = mb = 0.0;
-#pragma unroll
+    // Gauss points one at a time: unrolled, the eight need over 500 registers per lane
+    // (one wave per SIMD, with spills)
+#pragma unroll 1
     for (int q = 0; q < 8; ++q) {
         int sq0, sq1, sq2;
         hex8_sign(q, sq0, sq1, sq2);
