@@ -132,7 +132,6 @@ struct CellData {
     DevBuf<int32_t> node_inc;   // unused (kept for layout stability)
     DevBuf<int32_t> entry_pos;  // [nnz] position of each entry in node order (by node, then ascending entry)
     DevBuf<double> g;           // [nnz * 3] two-pass: per-entry force contributions of the last evaluation, node order
-    DevBuf<double> Paos;        // F-bar: [ncells * 10] P (row-major) and -V of the last evaluation, one row per subcell
     DevBuf<int64_t> rstart;     // [nn + 1] reverse neighbor lists: points i that list node J
     DevBuf<int32_t> rsrc;       // [nrev] i, ascending per node
     DevBuf<double> rphi;        // [nrev] phi_iJ
@@ -144,9 +143,6 @@ struct CellData {
     int64_t n_fcells = 0;           // nodal cells
     DevBuf<int64_t> fcell_sub;      // [n_fcells + 1] subcell range of each nodal cell
     DevBuf<double> fcell_volume;    // [n_fcells]
-    DevBuf<double> Jsub;            // [ncells] F-bar: J of each subcell
-    DevBuf<double> Pbar;            // [ncells * 9] F-bar: stress of H-bar, row-major
-    DevBuf<double> dterm;           // [ncells] F-bar: each subcell's term of the dual pressure
     DevBuf<int32_t> sub_fcell;      // [ncells] F-bar: nodal cell of each subcell
     DevBuf<double> evol;            // [ne] smoothed volume of every element (nodal cells)
     std::vector<int32_t> sub_of_elem;     // [ne] subcell of each internal element (nodal cells)
@@ -164,6 +160,9 @@ struct CellData {
     int tile_lpc = 8;               // lanes per cell of the tile kernel
     int64_t ntiles = 0, tile_total = 0;
     DevBuf<uint16_t> lidx;          // [nnz] entry -> index in its tile's node set
+    int tile_maxsub = 0;            // most cells in one tile
+    DevBuf<int64_t> t_cstart;       // [ntiles + 1] first cell of each tile
+    DevBuf<int32_t> t_fstart;       // F-bar: [ntiles + 1] first nodal cell of each tile
     DevBuf<int32_t> t_nstart;       // [ntiles + 1]
     DevBuf<int32_t> t_node;         // [tile_total] internal node ids, ascending per tile
     DevBuf<int64_t> nslot_start;    // [nn + 1] slot range of each node
@@ -176,13 +175,13 @@ struct CellData {
         tile_cells = tile_maxtn = 0;
         ntiles = tile_total = 0;
         lidx.release(); t_nstart.release(); t_node.release(); nslot_start.release(); nslot.release(); slots.release();
+        t_cstart.release(); t_fstart.release();
+        tile_maxsub = 0;
         rstart.release(); rsrc.release(); rphi.release(); u_phys.release(); f_local.release();
-        Paos.release();
         nnz = n_truncated = 0;
         ncells = n_fcells = 0;
         nodal = fbar = false;
-        fcell_sub.release(); fcell_volume.release(); Jsub.release(); Pbar.release(); evol.release();
-        dterm.release(); sub_fcell.release();
+        fcell_sub.release(); fcell_volume.release(); evol.release(); sub_fcell.release();
         sub_of_elem.clear(); sub_first_elem.clear(); fcell_sub_h.clear();
         num_nbr.release(); nbr.release(); phi.release(); radius.release(); start.release();
         nodes.release(); dphi.release(); volume.release(); inc_cell.release(); node_start.release();

@@ -620,10 +620,8 @@ struct CellForceArgs {
     const double *elem_lambda, *elem_mu;
     const double *Hn, *Pn;  // rate-form materials: H and P of the previous step
     double rate_dt;
-    double *Paos;           // F-bar: [ne * kRowLen] rows (P row-major, then -V)
 };
 
-constexpr int kRowLen = 10;  // doubles per cell row of the F-bar scatter
 constexpr int kNodeBatch = 4;  // incidences per lane whose loads are issued together
 constexpr int kNumXcd = 8;
 
@@ -644,143 +642,6 @@ __device__ __forceinline__ double group_sum(double x) {
 }
 
 __device__ __forceinline__ double group_sum16(double x) { return group_sum<kCellLanes>(x); }
-
-// F-bar over nodal cells (ComputeInternalForceSmoothedCell.h:120-413), first pass: the
-// subcell displacement gradient (the one-pass cell accumulation, LANES lanes per subcell)
-// into the H planes and J = DetApIm1(H) + 1 (:186-224).
-template <int LANES>
-__global__ __launch_bounds__(256) void k_fbar_subcells(CellForceArgs A, double *J) {
-#pragma clang fp contract(off)
-    const int l = threadIdx.x & (LANES - 1);
-    const int64_t c_raw = xcd_block(A.nblocks) * (256 / LANES) + (threadIdx.x / LANES);
-    const bool live = c_raw < A.ne;
-    const int64_t c = live ? c_raw : A.ne - 1;
-    const int64_t t0 = A.start[c], t1 = A.start[c + 1];
-    double H[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    for (int64_t t = t0 + l; t < t1; t += LANES) {
-        double uj[3], g[3];
-        ld3(A.u, A.nodes[t], uj);
-        ld3(A.dphi, t, g);
-#pragma unroll
-        for (int a = 0; a < 3; ++a)
-#pragma unroll
-            for (int b = 0; b < 3; ++b) H[3 * a + b] = __fma_rn(uj[a], g[b], H[3 * a + b]);
-    }
-#pragma unroll
-    for (int i = 0; i < 9; ++i) H[i] = group_sum<LANES>(H[i]);
-    if (live && l == 0) {
-#pragma unroll
-        for (int i = 0; i < 9; ++i) A.H[i * A.ne + c] = H[i];
-        J[c] = det_ap_im1(H) + 1.0;
-    }
-}
-
-struct FbarArgs {
-    int64_t n_cells;               // nodal cells
-    int64_t ns;                    // subcells (the plane stride of H and P)
-    const int64_t *cell_sub;       // [n_cells + 1]
-    const double *cell_volume;     // [n_cells]
-    const double *sub_volume;      // [ns]
-    const double *J;               // [ns]
-    const double *H;               // [9 x ns] planes
-    double *P;                     // [9 x ns] planes: the un-barred stress
-    double *Pbar;                  // [ns x 9] row-major: the stress of H-bar
-    double *Paos;                  // [ns x kRowLen] rows of the node kernel (P, -V)
-    const double *state_n;
-    double *state_np1;
-    MatParams mat;
-    const int32_t *sub_cell;       // [ns] nodal cell of each subcell
-    double *dterm;                 // [ns] each subcell's term of its cell's dual pressure
-};
-
-// Second and third passes, one thread per subcell s of nodal cell r: J-bar = sum J_k V_k
-// / V_cell over the cell's subcells in order (ComputeFbar, :120-175), H-bar = H
-// cbrt(J-bar / J_s) + (cbrt(.) - 1) I unless the ratio is within 3 eps of 1, the stress
-// of H-bar (:177-213) and the subcell's term of the dual pressure; then the dual pressure
-// (the cell's terms summed in order) and the un-barred stress P = P-bar s + (p - s /
-// (3 J_s) P-bar : F) J_s F^-T (UnBarStress, :215-309) with the node kernel's row (P, -V_s).
-// Every thread re-forms its cell's two sums in the same order, so each subcell sees the
-// same bits a per-cell loop would.
-__device__ __forceinline__ double fbar_jbar(const FbarArgs &A, int64_t r) {
-    const int64_t s0 = A.cell_sub[r], s1 = A.cell_sub[r + 1];
-    double jbar = 0.0;
-    for (int64_t k = s0; k < s1; ++k) jbar = jbar + A.J[k] * A.sub_volume[k];
-    return jbar / A.cell_volume[r];
-}
-
-template <int KIND>
-__global__ __launch_bounds__(256) void k_fbar_stress(FbarArgs A) {
-#pragma clang fp contract(off)
-    constexpr double kEps = 2.2204460492503131e-16;
-    const int64_t s = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (s >= A.ns) return;
-    const double jbar = fbar_jbar(A, A.sub_cell[s]);
-    double H[9], Hb[9], Pb[9];
-#pragma unroll
-    for (int i = 0; i < 9; ++i) H[i] = A.H[i * A.ns + s];
-    const double ratio = jbar / A.J[s];
-    if (fabs(ratio - 1.0) < 3.0 * kEps) {
-#pragma unroll
-        for (int i = 0; i < 9; ++i) Hb[i] = H[i];
-    } else {
-        const double js = cbrt(ratio);
-#pragma unroll
-        for (int i = 0; i < 9; ++i) Hb[i] = H[i] * js;
-        Hb[0] += js - 1.0;
-        Hb[4] += js - 1.0;
-        Hb[8] += js - 1.0;
-    }
-    double snp1 = 0.0, aux = 0.0;
-    const double sn = load_state<KIND>(A.state_n, s);
-    stress<KIND>(Hb, A.mat, sn, snp1, aux, Pb);
-    store_state<KIND>(A.state_np1, s, snp1, aux);
-#pragma unroll
-    for (int i = 0; i < 9; ++i) A.Pbar[9 * s + i] = Pb[i];
-    const double js = cbrt(jbar / A.J[s]);
-    const double djb = js / (3.0 * jbar);
-    double ph = 0.0;
-#pragma unroll
-    for (int i = 0; i < 9; ++i) ph += Pb[i] * H[i];
-    const double tr = Pb[0] + Pb[4] + Pb[8];
-    A.dterm[s] = djb * A.sub_volume[s] * (ph + tr);
-}
-
-__global__ __launch_bounds__(256) void k_fbar_unbar(FbarArgs A) {
-#pragma clang fp contract(off)
-    const int64_t s = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (s >= A.ns) return;
-    const int64_t r = A.sub_cell[s];
-    const double jbar = fbar_jbar(A, r);
-    double dual = 0.0;
-    for (int64_t k = A.cell_sub[r]; k < A.cell_sub[r + 1]; ++k) dual += A.dterm[k];
-    dual = dual / A.cell_volume[r];
-    const double Js = A.J[s];
-    const double js = cbrt(jbar / Js);
-    double F[9], Fi[9], Pb[9], pf = 0.0;
-#pragma unroll
-    for (int i = 0; i < 9; ++i) {
-        F[i] = A.H[i * A.ns + s];
-        Pb[i] = A.Pbar[9 * s + i];
-    }
-    F[0] += 1.0;
-    F[4] += 1.0;
-    F[8] += 1.0;
-#pragma unroll
-    for (int i = 0; i < 9; ++i) pf += Pb[i] * F[i];
-    const double djj = -js / (3.0 * Js);
-    const double scalar = (dual + djj * pf) * Js;
-    inverse3(F, Fi);
-    double *row = A.Paos + kRowLen * s;
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-        for (int j = 0; j < 3; ++j) {
-            const double p = Pb[3 * i + j] * js + scalar * Fi[3 * j + i];
-            A.P[(3 * i + j) * A.ns + s] = p;
-            row[3 * i + j] = p;
-        }
-    row[9] = -A.sub_volume[s];
-}
 
 // Two-pass cells, one thread per cell: the element's four nodes, their derivative
 // (B_k V) / V recomputed from the coordinates by the set-up's formula (instead of
@@ -926,7 +787,7 @@ __global__ __launch_bounds__(256) void k_rk_nodes(RkNodeArgs A) {
 // is the same bits on every run.
 struct RkTileArgs {
     int64_t ncells, ntiles, nblocks;
-    int cells_per_tile, maxtn;
+    int maxtn;
     const int64_t *start;
     const uint16_t *lidx;     // [nnz] entry -> index in its tile's node set
     const double *dphi;
@@ -934,6 +795,7 @@ struct RkTileArgs {
     const double *u;
     const int32_t *t_nstart;  // [ntiles + 1] offsets of the tile node sets
     const int32_t *t_node;    // [total] internal node ids, ascending within a tile
+    const int64_t *t_cstart;  // [ntiles + 1] first cell of each tile
     double *slots;            // [total * 3] tile force rows, tile-major
     double *H, *P;
     const double *state_n;
@@ -979,8 +841,7 @@ __global__ __launch_bounds__(64) void k_rk_tile(RkTileArgs A) {
     __syncthreads();
     constexpr int G = 64 / LPC;
     const int grp = lane / LPC, j = lane % LPC;
-    const int64_t c0 = t * (int64_t)A.cells_per_tile;
-    const int64_t c1 = min(c0 + (int64_t)A.cells_per_tile, A.ncells);
+    const int64_t c0 = A.t_cstart[t], c1 = A.t_cstart[t + 1];
     for (int64_t cb = c0; cb < c1; cb += G) {
         const int64_t c_raw = cb + grp;
         const bool live = c_raw < c1;
@@ -1067,48 +928,193 @@ __global__ __launch_bounds__(64) void k_rk_tile(RkTileArgs A) {
     for (int k = lane; k < 3 * tn; k += 64) dst[k] = F[k];
 }
 
-// F-bar: the entry forces of cells whose stress an earlier pass left in the cell rows
-// (P row-major, then -V), added into the tile's LDS rows like the tile kernel's.
-template <int LPC>
-__global__ __launch_bounds__(64) void k_rk_tile_scatter(RkTileArgs A, const double *rows) {
+// F-bar on cell tiles (ComputeInternalForceSmoothedCell.h:120-413): a tile holds whole
+// nodal cells, so every pass of the F-bar force runs inside one wave with the subcell
+// rows in LDS: (1) subcell H (LPC lanes per subcell, butterfly) and J = DetApIm1(H) + 1;
+// (2) one lane per subcell: J-bar of its cell (the cell's J_k V_k summed in order over
+// V_cell, :120-175), H-bar = H cbrt(J-bar / J) + (cbrt(.) - 1) I unless the ratio is
+// within 3 eps of 1, the stress of H-bar (:177-213) and the subcell's term of the dual
+// pressure; (3) one lane per subcell: the dual pressure (the cell's terms in order) and
+// the un-barred P = P-bar s + (p - s / (3 J) P-bar : F) J F^-T (UnBarStress, :215-309);
+// (4) the entry forces -V P grad phi added into the tile's LDS rows, as in k_rk_tile.
+// The entries are read twice (H needs every subcell of a cell before any stress), the
+// second time right after the first by the same wave.
+constexpr int kFbarSubRow = 29;  // LDS doubles per subcell: H 9, J 1, P-bar 9, dual term 1, P 9
+
+struct FbarTileArgs {
+    RkTileArgs T;
+    const int32_t *t_fstart;     // [ntiles + 1] first nodal cell of each tile
+    const int64_t *fcell_sub;    // [n_fcells + 1] subcell range of each nodal cell
+    const double *fcell_volume;  // [n_fcells]
+    const int32_t *sub_fcell;    // [ncells] nodal cell of each subcell
+    int maxsub;                  // most subcells in one tile
+};
+
+template <int KIND, int LPC>
+__global__ __launch_bounds__(64) void k_fbar_tile(FbarTileArgs B) {
 #pragma clang fp contract(off)
+    constexpr double kEps = 2.2204460492503131e-16;
+    const RkTileArgs &A = B.T;
     extern __shared__ __align__(16) double rk_lds[];
-    double *F = rk_lds;
+    double *U = rk_lds;
+    double *F = U + 3 * (size_t)A.maxtn;
+    double *SH = F + 3 * (size_t)A.maxtn;  // [maxsub * 9]
+    double *SJ = SH + 9 * (size_t)B.maxsub;
+    double *SPb = SJ + B.maxsub;           // [maxsub * 9]
+    double *SD = SPb + 9 * (size_t)B.maxsub;
+    double *SP = SD + B.maxsub;            // [maxsub * 9]
     const int64_t t = xcd_block(A.nblocks);
     if (t >= A.ntiles) return;
     const int lane = threadIdx.x;
     const int tb = A.t_nstart[t], tn = A.t_nstart[t + 1] - tb;
-    for (int k = lane; k < 3 * tn; k += 64) F[k] = 0.0;
+    constexpr int kStage = 8;
+    for (int l0 = lane; l0 < tn; l0 += 64 * kStage) {
+        int64_t gi[kStage];
+#pragma unroll
+        for (int r = 0; r < kStage; ++r) gi[r] = A.t_node[tb + min(l0 + 64 * r, tn - 1)];
+        double ur[kStage][3];
+#pragma unroll
+        for (int r = 0; r < kStage; ++r) ld3(A.u, gi[r], ur[r]);
+#pragma unroll
+        for (int r = 0; r < kStage; ++r) {
+            const int ln = l0 + 64 * r;
+            if (ln < tn) {
+#pragma unroll
+                for (int d = 0; d < 3; ++d) {
+                    U[3 * ln + d] = ur[r][d];
+                    F[3 * ln + d] = 0.0;
+                }
+            }
+        }
+    }
     __syncthreads();
     constexpr int G = 64 / LPC;
-    constexpr int B = 4;  // entries per lane whose loads are issued together
     const int grp = lane / LPC, j = lane % LPC;
-    const int64_t c0 = t * (int64_t)A.cells_per_tile;
-    const int64_t c1 = min(c0 + (int64_t)A.cells_per_tile, A.ncells);
+    const int64_t c0 = A.t_cstart[t], c1 = A.t_cstart[t + 1];
+    // (1) subcell H and J
+    for (int64_t cb = c0; cb < c1; cb += G) {
+        const int64_t c_raw = cb + grp;
+        const bool live = c_raw < c1;
+        const int64_t c = live ? c_raw : c1 - 1;
+        const int64_t t0 = A.start[c], t1 = A.start[c + 1];
+        double H[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        for (int64_t e = t0 + j; e < t1; e += LPC) {
+            double uj[3], g[3];
+            ld3(U, A.lidx[e], uj);
+            ld3(A.dphi, e, g);
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+#pragma unroll
+                for (int b = 0; b < 3; ++b) H[3 * a + b] = __fma_rn(uj[a], g[b], H[3 * a + b]);
+        }
+#pragma unroll
+        for (int i = 0; i < 9; ++i) H[i] = group_sum<LPC>(H[i]);
+        if (live && j == 0) {
+            const int sl = (int)(c - c0);
+#pragma unroll
+            for (int i = 0; i < 9; ++i) {
+                SH[9 * sl + i] = H[i];
+                store_once(A.H + i * A.ncells + c, H[i]);
+            }
+            SJ[sl] = det_ap_im1(H) + 1.0;
+        }
+    }
+    __syncthreads();
+    auto jbar_of = [&](int64_t r) {
+        double jb = 0.0;
+        for (int64_t k = B.fcell_sub[r]; k < B.fcell_sub[r + 1]; ++k) jb = jb + SJ[k - c0] * A.volume[k];
+        return jb / B.fcell_volume[r];
+    };
+    // (2) H-bar, its stress, the dual-pressure term
+    for (int64_t s = c0 + lane; s < c1; s += 64) {
+        const int sl = (int)(s - c0);
+        const double jbar = jbar_of(B.sub_fcell[s]);
+        double H[9], Hb[9], Pb[9];
+#pragma unroll
+        for (int i = 0; i < 9; ++i) H[i] = SH[9 * sl + i];
+        const double ratio = jbar / SJ[sl];
+        if (fabs(ratio - 1.0) < 3.0 * kEps) {
+#pragma unroll
+            for (int i = 0; i < 9; ++i) Hb[i] = H[i];
+        } else {
+            const double js = cbrt(ratio);
+#pragma unroll
+            for (int i = 0; i < 9; ++i) Hb[i] = H[i] * js;
+            Hb[0] += js - 1.0;
+            Hb[4] += js - 1.0;
+            Hb[8] += js - 1.0;
+        }
+        double snp1 = 0.0, aux = 0.0;
+        const double sn = load_state<KIND>(A.state_n, s);
+        stress<KIND>(Hb, A.mat, sn, snp1, aux, Pb);
+        store_state<KIND>(A.state_np1, s, snp1, aux);
+#pragma unroll
+        for (int i = 0; i < 9; ++i) SPb[9 * sl + i] = Pb[i];
+        const double js = cbrt(jbar / SJ[sl]);
+        const double djb = js / (3.0 * jbar);
+        double ph = 0.0;
+#pragma unroll
+        for (int i = 0; i < 9; ++i) ph += Pb[i] * H[i];
+        const double tr = Pb[0] + Pb[4] + Pb[8];
+        SD[sl] = djb * A.volume[s] * (ph + tr);
+    }
+    __syncthreads();
+    // (3) the dual pressure and the un-barred stress
+    for (int64_t s = c0 + lane; s < c1; s += 64) {
+        const int sl = (int)(s - c0);
+        const int64_t r = B.sub_fcell[s];
+        const double jbar = jbar_of(r);
+        double dual = 0.0;
+        for (int64_t k = B.fcell_sub[r]; k < B.fcell_sub[r + 1]; ++k) dual += SD[k - c0];
+        dual = dual / B.fcell_volume[r];
+        const double Js = SJ[sl];
+        const double js = cbrt(jbar / Js);
+        double Fm[9], Fi[9], Pb[9], pf = 0.0;
+#pragma unroll
+        for (int i = 0; i < 9; ++i) {
+            Fm[i] = SH[9 * sl + i];
+            Pb[i] = SPb[9 * sl + i];
+        }
+        Fm[0] += 1.0;
+        Fm[4] += 1.0;
+        Fm[8] += 1.0;
+#pragma unroll
+        for (int i = 0; i < 9; ++i) pf += Pb[i] * Fm[i];
+        const double djj = -js / (3.0 * Js);
+        const double scalar = (dual + djj * pf) * Js;
+        inverse3(Fm, Fi);
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int b = 0; b < 3; ++b) {
+                const double p = Pb[3 * a + b] * js + scalar * Fi[3 * b + a];
+                SP[9 * sl + 3 * a + b] = p;
+                store_once(A.P + (3 * a + b) * A.ncells + s, p);
+            }
+    }
+    __syncthreads();
+    // (4) entry forces
+    constexpr int BQ = 4;  // entries per lane whose loads are issued together
     for (int64_t cb = c0; cb < c1; cb += G) {
         const int64_t c = cb + grp;
         if (c >= c1) continue;
-        const double2 *row = reinterpret_cast<const double2 *>(rows + kRowLen * c);
-        double P[kRowLen];
+        const int sl = (int)(c - c0);
+        double P[9];
 #pragma unroll
-        for (int q = 0; q < kRowLen / 2; ++q) {
-            const double2 w = row[q];
-            P[2 * q] = w.x;
-            P[2 * q + 1] = w.y;
-        }
-        const double mv = P[9];
+        for (int i = 0; i < 9; ++i) P[i] = SP[9 * sl + i];
+        const double mv = -A.volume[c];
         const int64_t t0 = A.start[c], t1 = A.start[c + 1];
-        for (int64_t eb = t0 + j; eb < t1; eb += B * LPC) {
-            int li[B];
-            double g[B][3];
+        for (int64_t eb = t0 + j; eb < t1; eb += BQ * LPC) {
+            int li[BQ];
+            double g[BQ][3];
 #pragma unroll
-            for (int r = 0; r < B; ++r) {
+            for (int r = 0; r < BQ; ++r) {
                 const int64_t e = min(eb + r * LPC, t1 - 1);
                 li[r] = A.lidx[e];
                 ld3(A.dphi, e, g[r]);
             }
 #pragma unroll
-            for (int r = 0; r < B; ++r) {
+            for (int r = 0; r < BQ; ++r) {
                 if (eb + r * LPC < t1) {
 #pragma unroll
                     for (int a = 0; a < 3; ++a) {
@@ -1451,9 +1457,6 @@ static void nodal_subcells(Ctx &c, const std::vector<int64_t> &sub_start, const 
     D.fcell_sub.upload(D.fcell_sub_h.data(), D.fcell_sub_h.size(), c.stream);
     D.fcell_volume.upload(vcell.data(), std::max<int64_t>(D.n_fcells, 1), c.stream);
     if (D.fbar) {
-        D.Jsub.alloc(std::max<int64_t>(ns, 1));
-        D.Pbar.alloc(9 * std::max<int64_t>(ns, 1));
-        D.dterm.alloc(std::max<int64_t>(ns, 1));
         std::vector<int32_t> sc_of(std::max<int64_t>(ns, 1), 0);
         for (int64_t r = 0; r < D.n_fcells; ++r)
             for (int64_t sc = D.fcell_sub_h[r]; sc < D.fcell_sub_h[r + 1]; ++sc) sc_of[sc] = (int32_t)r;
@@ -1648,9 +1651,13 @@ void rk_cells_and_order(Ctx &c, bool two_pass, bool elem_cells) {
     rk_cells_order(c, two_pass);
 }
 
-// LDS of one tile-kernel workgroup (u and force rows of the tile nodes) is kept to
-// 64 KiB so that several tiles share a CU.
+// LDS of one tile-kernel workgroup (u and force rows of the tile nodes; F-bar also the
+// subcell rows) is kept to 64 KiB so that several tiles share a CU.
 constexpr int64_t kTileLdsMax = 64 * 1024;
+
+static int64_t tile_lds(int maxtn, int maxsub, bool fbar) {
+    return 48 * (int64_t)std::max(maxtn, 1) + (fbar ? 8 * (int64_t)kFbarSubRow * std::max(maxsub, 1) : 0);
+}
 
 // Cell tiles of the one-pass force: runs of tile_cells consecutive cells, the sorted
 // union of their node lists, every entry's index in it, and every node's slots (its
@@ -1685,15 +1692,37 @@ static void rk_tile_plan(Ctx &c) {
     std::vector<uint16_t> lidx(std::max<int64_t>(nnz, 1));
     std::vector<std::vector<int32_t>> sets;
     int64_t nt = 0;
-    int maxtn = 0;
+    int maxtn = 0, maxsub = 0;
+    std::vector<int64_t> cst;  // first cell of each tile
+    std::vector<int32_t> fst;  // F-bar: first nodal cell of each tile
     while (true) {
-        nt = nc ? (nc + tc - 1) / tc : 0;
+        cst.clear();
+        fst.clear();
+        if (D.fbar) {
+            // whole nodal cells, as many as fit tc subcells (a larger cell gets a tile alone)
+            const std::vector<int64_t> &fs = D.fcell_sub_h;
+            for (int64_t r = 0; r < D.n_fcells;) {
+                cst.push_back(fs[r]);
+                fst.push_back((int32_t)r);
+                int64_t r1 = r + 1;
+                while (r1 < D.n_fcells && fs[r1 + 1] - fs[r] <= tc) ++r1;
+                r = r1;
+            }
+            cst.push_back(nc);
+            fst.push_back((int32_t)D.n_fcells);
+        } else {
+            for (int64_t k = 0; k < nc; k += tc) cst.push_back(k);
+            cst.push_back(nc);
+        }
+        nt = (int64_t)cst.size() - 1;
+        maxsub = 0;
+        for (int64_t t = 0; t < nt; ++t) maxsub = std::max<int64_t>(maxsub, cst[t + 1] - cst[t]);
         sets.assign(nt, {});
         std::vector<int> tile_max(nthreads, 0);
         auto work = [&](int w) {
             std::vector<int32_t> loc(nn, 0);
             for (int64_t t = w; t < nt; t += nthreads) {
-                const int64_t e0 = hs[t * tc], e1 = hs[std::min<int64_t>((t + 1) * tc, nc)];
+                const int64_t e0 = hs[cst[t]], e1 = hs[cst[t + 1]];
                 std::vector<int32_t> &u = sets[t];
                 u.assign(hn.begin() + e0, hn.begin() + e1);
                 std::sort(u.begin(), u.end());
@@ -1709,10 +1738,11 @@ static void rk_tile_plan(Ctx &c) {
         work(0);
         for (auto &th : pool) th.join();
         maxtn = *std::max_element(tile_max.begin(), tile_max.end());
-        if ((int64_t)maxtn * 48 <= kTileLdsMax || tc == 1) break;
+        if (tile_lds(maxtn, maxsub, D.fbar) <= kTileLdsMax || tc == 1) break;
         tc = std::max(1, tc / 2);
     }
-    APHIP_REQUIRE((int64_t)maxtn * 48 <= kTileLdsMax, "a smoothing cell names too many nodes for the tile kernel");
+    APHIP_REQUIRE(tile_lds(maxtn, maxsub, D.fbar) <= kTileLdsMax,
+                  "a smoothing cell names too many nodes (or a nodal cell holds too many subcells) for the tile kernels");
     std::vector<int32_t> tstart(nt + 1, 0);
     for (int64_t t = 0; t < nt; ++t) {
         APHIP_REQUIRE((int64_t)tstart[t] + (int64_t)sets[t].size() < (int64_t)INT32_MAX, "too many tile nodes");
@@ -1734,8 +1764,11 @@ static void rk_tile_plan(Ctx &c) {
     sets.clear();
     D.tile_cells = tc;
     D.tile_maxtn = maxtn;
+    D.tile_maxsub = maxsub;
     D.ntiles = nt;
     D.tile_total = total;
+    D.t_cstart.upload(cst.data(), cst.size(), c.stream);
+    if (D.fbar) D.t_fstart.upload(fst.data(), fst.size(), c.stream);
     D.lidx.upload(lidx.data(), lidx.size(), c.stream);
     D.t_nstart.upload(tstart.data(), tstart.size(), c.stream);
     D.t_node.upload(tnode.data(), tnode.size(), c.stream);
@@ -1808,7 +1841,6 @@ void rk_cells_order(Ctx &c, bool two_pass) {
         APHIP_HIP(hipGetLastError());
     }
     if (two_pass) D.g.alloc(3 * std::max<int64_t>(D.nnz, 1));
-    else if (D.fbar) D.Paos.alloc(kRowLen * std::max<int64_t>(D.ncells, 1));
     APHIP_HIP(hipStreamSynchronize(c.stream));
     D.ready = true;
 }
@@ -1823,22 +1855,35 @@ static void launch_rk_tile(const RkTileArgs &T, int lpc, dim3 g, size_t lds, hip
 static RkTileArgs rk_tile_args(Ctx &c) {
     CellData &D = c.cells;
     const int64_t nbt = ((D.ntiles + kNumXcd - 1) / kNumXcd) * kNumXcd;
-    return RkTileArgs{D.ncells, D.ntiles, nbt, D.tile_cells, D.tile_maxtn, D.start.p, D.lidx.p, D.dphi.p,
-                      D.volume.p, c.u.p, D.t_nstart.p, D.t_node.p, D.slots.p, c.Hnp1().p, c.Pnp1().p,
+    return RkTileArgs{D.ncells, D.ntiles, nbt, D.tile_maxtn, D.start.p, D.lidx.p, D.dphi.p,
+                      D.volume.p, c.u.p, D.t_nstart.p, D.t_node.p, D.t_cstart.p, D.slots.p, c.Hnp1().p, c.Pnp1().p,
                       c.state[1 - c.state_np1].p, c.state[c.state_np1].p, c.mat.params(),
                       c.has_fields ? c.elem_lambda.p : nullptr, c.has_fields ? c.elem_mu.p : nullptr,
                       c.mat.rate_form() ? c.Hn().p : nullptr, c.mat.rate_form() ? c.Pn().p : nullptr, c.rate_dt};
 }
 
-static void rk_tile_scatter(Ctx &c, const double *rows) {
+template <int KIND>
+static void launch_fbar_tile(const FbarTileArgs &B, int lpc, dim3 g, size_t lds, hipStream_t s) {
+    if (lpc == 4) hipLaunchKernelGGL((k_fbar_tile<KIND, 4>), g, dim3(64), lds, s, B);
+    else if (lpc == 8) hipLaunchKernelGGL((k_fbar_tile<KIND, 8>), g, dim3(64), lds, s, B);
+    else hipLaunchKernelGGL((k_fbar_tile<KIND, 16>), g, dim3(64), lds, s, B);
+}
+
+static void rk_fbar_tiles(Ctx &c) {
     CellData &D = c.cells;
     if (D.ntiles == 0) return;
-    const RkTileArgs T = rk_tile_args(c);
-    const dim3 g((unsigned)T.nblocks);
-    const size_t lds = 24 * (size_t)std::max(D.tile_maxtn, 1);
-    if (D.tile_lpc == 4) hipLaunchKernelGGL(k_rk_tile_scatter<4>, g, dim3(64), lds, c.stream, T, rows);
-    else if (D.tile_lpc == 8) hipLaunchKernelGGL(k_rk_tile_scatter<8>, g, dim3(64), lds, c.stream, T, rows);
-    else hipLaunchKernelGGL(k_rk_tile_scatter<16>, g, dim3(64), lds, c.stream, T, rows);
+    const FbarTileArgs B{rk_tile_args(c), D.t_fstart.p, D.fcell_sub.p, D.fcell_volume.p, D.sub_fcell.p,
+                         std::max(D.tile_maxsub, 1)};
+    const dim3 g((unsigned)B.T.nblocks);
+    const size_t lds = (size_t)tile_lds(D.tile_maxtn, D.tile_maxsub, true);
+    const int lpc = D.tile_lpc;
+    switch (c.mat.kind) {
+        case APHIP_LINEAR_ELASTIC: launch_fbar_tile<APHIP_LINEAR_ELASTIC>(B, lpc, g, lds, c.stream); break;
+        case APHIP_NEO_HOOKEAN: launch_fbar_tile<APHIP_NEO_HOOKEAN>(B, lpc, g, lds, c.stream); break;
+        case APHIP_J2: launch_fbar_tile<APHIP_J2>(B, lpc, g, lds, c.stream); break;
+        case APHIP_SVK: launch_fbar_tile<APHIP_SVK>(B, lpc, g, lds, c.stream); break;
+        default: throw Error(APHIP_ERR_UNSUPPORTED, "F-bar takes the linear elastic, neo-Hookean, J2 and SVK materials");
+    }
     APHIP_HIP(hipGetLastError());
 }
 
@@ -1898,28 +1943,10 @@ void rk_cell_force(Ctx &c) {
                     c.state[1 - c.state_np1].p, c.state[c.state_np1].p,
                     c.mat.params(),
                     c.has_fields ? c.elem_lambda.p : nullptr, c.has_fields ? c.elem_mu.p : nullptr,
-                    c.mat.rate_form() ? c.Hn().p : nullptr, c.mat.rate_form() ? c.Pn().p : nullptr, c.rate_dt,
-                    D.Paos.p};
+                    c.mat.rate_form() ? c.Hn().p : nullptr, c.mat.rate_form() ? c.Pn().p : nullptr, c.rate_dt};
     const dim3 g((unsigned)nbc);
     if (D.fbar) {
-        hipLaunchKernelGGL((k_fbar_subcells<kCellLanes>), g, dim3(256), 0, c.stream, A, D.Jsub.p);
-        APHIP_HIP(hipGetLastError());
-        FbarArgs F{D.n_fcells, D.ncells, D.fcell_sub.p, D.fcell_volume.p, D.volume.p, D.Jsub.p, c.Hnp1().p,
-                   c.Pnp1().p, D.Pbar.p, D.Paos.p, c.state[1 - c.state_np1].p, c.state[c.state_np1].p,
-                   c.mat.params(), D.sub_fcell.p, D.dterm.p};
-        const dim3 gf((unsigned)((D.ncells + 255) / 256));
-        switch (c.mat.kind) {
-            case APHIP_LINEAR_ELASTIC: hipLaunchKernelGGL(k_fbar_stress<APHIP_LINEAR_ELASTIC>, gf, dim3(256), 0, c.stream, F); break;
-            case APHIP_NEO_HOOKEAN: hipLaunchKernelGGL(k_fbar_stress<APHIP_NEO_HOOKEAN>, gf, dim3(256), 0, c.stream, F); break;
-            case APHIP_J2: hipLaunchKernelGGL(k_fbar_stress<APHIP_J2>, gf, dim3(256), 0, c.stream, F); break;
-            case APHIP_SVK: hipLaunchKernelGGL(k_fbar_stress<APHIP_SVK>, gf, dim3(256), 0, c.stream, F); break;
-            default: throw Error(APHIP_ERR_UNSUPPORTED, "F-bar takes the linear elastic, neo-Hookean, J2 and SVK materials");
-        }
-        APHIP_HIP(hipGetLastError());
-        hipLaunchKernelGGL(k_fbar_unbar, gf, dim3(256), 0, c.stream, F);
-        APHIP_HIP(hipGetLastError());
-        rk_tile_scatter(c, D.Paos.p);
-        APHIP_HIP(hipGetLastError());
+        rk_fbar_tiles(c);
         return;
     }
     if (D.two_pass) {
