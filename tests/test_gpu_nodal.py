@@ -151,3 +151,38 @@ def test_nodal_refusals(gpu_available):
             s.power_method_time_increment()
         with pytest.raises(A.AphipError, match="nodal cells"):
             s.set_material_fields(np.full(len(hx), E), np.full(len(hx), NU))
+
+
+@pytest.mark.parametrize("tile_cells", ["8", "64"])
+def test_fbar_tiles_are_deterministic_and_size_independent(gpu_available, monkeypatch, tile_cells):
+    """The F-bar force runs one tile kernel per tile of whole nodal cells (at most
+    APHIP_RK_TILE_CELLS subcells, a larger cell alone).  Two runs give the same bits, and
+    the tile size changes nothing beyond rounding: 30 J2 steps match the oracle at 1e-10."""
+    Xh, hx = _thex(3, 23)
+    d = O.nodal_setup(Xh, hx, SCALE, 0, density=RHO)
+    lam, mu = O.lame(E, NU)
+    ref = O.Solver(-1, Xh, None, O.J2, lam, mu, d["mass"], None, cells=d["cells"],
+                   fbar_cells=(d["cell_sub"], d["cell_volume"]), active=d["active"], **J2)
+    dt = 0.5 * M.stable_dt_estimate(Xh, hx, E, NU, RHO)
+    c = Xh - 0.5
+    v0 = (0.03 / (30 * dt)) * np.stack([-0.5 * c[:, 0], -0.5 * c[:, 1], c[:, 2]], axis=1)
+    ref.v[:] = v0
+    ref.run(dt, 30)
+    es = _elem_subcell(d, len(hx))
+    monkeypatch.setenv("APHIP_RK_TILE_CELLS", tile_cells)
+    print(f"F-bar tiles of at most {tile_cells} subcells", flush=True)
+    out = []
+    for _ in range(2):
+        with _gpu(Xh, hx, A.J2, 0, True) as s:
+            s.compute_mass()
+            s.set_field(A.FIELD_V, v0)
+            s.initialize()
+            s.step(dt, 30)
+            out.append([s.get(f) for f in (A.FIELD_U, A.FIELD_V, A.FIELD_F, A.FIELD_P, A.FIELD_STATE)])
+    for a, b in zip(*out):
+        assert np.array_equal(a, b)
+    u, v, f, P, st = out[0]
+    assert ref.state.max() > 0.0
+    for g, r in ((u, ref.u), (v, ref.v), (f, ref.f)):
+        assert rel_err(g, r) < 1e-10
+    assert rel_err(P, ref.P[es]) < 1e-10 and rel_err(st, ref.state[es]) < 1e-10
