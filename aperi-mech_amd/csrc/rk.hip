@@ -1663,7 +1663,9 @@ static int64_t tile_lds(int maxtn, int maxsub, bool fbar) {
 // union of their node lists, every entry's index in it, and every node's slots (its
 // positions in the tile-major node sets, ascending tile).  The tile size starts at 64
 // cells (256 for cells of at most four entries), is halved while the mesh gives fewer
-// than 2048 tiles, and then until every tile's node set fits the LDS budget.  The lanes
+// than 2048 tiles (F-bar, whose tiles do several passes over their subcells: 8192; on a
+// thex mesh of 192,000 subcells 16 instead of 64 subcells per tile took the step from
+// 0.131 to 0.121 ms), and then until every tile's node set fits the LDS budget.  The lanes
 // per cell follow the longest cell list (4 up to 4 entries, 8 up to 96, else 16).
 // APHIP_RK_TILE_CELLS and APHIP_RK_TILE_LANES override the two choices.
 static void rk_tile_plan(Ctx &c) {
@@ -1682,8 +1684,9 @@ static void rk_tile_plan(Ctx &c) {
         if (v == 4 || v == 8 || v == 16) D.tile_lpc = v;
     }
     int tc = D.tile_lpc == 4 ? 256 : 64;
-    // small meshes: at least 2048 tiles (eight waves per CU) while a tile keeps 8 cells
-    while (tc > 8 && (nc + tc - 1) / tc < 2048) tc /= 2;
+    // small meshes: at least 2048 tiles (eight waves per CU; F-bar 8192) while a tile keeps 8 cells
+    const int64_t min_tiles = D.fbar ? 8192 : 2048;
+    while (tc > 8 && (nc + tc - 1) / tc < min_tiles) tc /= 2;
     if (const char *env = std::getenv("APHIP_RK_TILE_CELLS")) {
         const int v = std::atoi(env);
         if (v >= 1 && v <= 4096) tc = v;
