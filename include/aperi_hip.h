@@ -13,7 +13,8 @@
  *     reference's field layout (include/Field.h:129-138): node vectors are
  *     n_nodes x 3 row-major, element tensors n_elem x 9 row-major (3*i+j),
  *     connectivity n_elem x nodes_per_element.  The library renumbers nodes
- *     and elements internally (Morton order) and converts on the way in/out.
+ *     and elements internally (space-filling orders grouped by its tile plan)
+ *     and converts on the way in/out.
  *   - A context owns one HIP stream and all device memory it allocates.  Calls
  *     on one context are stream-ordered and must come from one thread at a time.
  *   - All arithmetic is FP64.
@@ -146,7 +147,7 @@ int aphip_device_count(int32_t *count);
  * and the element volume functor (include/ComputeElementVolumeFunctor.h:47-69). */
 int aphip_set_mesh(aphip_ctx *ctx, int32_t elem_type, int64_t n_nodes, const double *X, int64_t n_elem,
                    const int32_t *conn);
-/* perm[i] = internal (Morton) index of caller node i. */
+/* perm[i] = internal index of caller node i. */
 int aphip_get_node_permutation(aphip_ctx *ctx, int32_t *perm);
 
 /* Material of the block: replaces CreateMaterial (include/Materials/Factory.h:43-160)
