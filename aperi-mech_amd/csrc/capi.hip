@@ -336,7 +336,7 @@ static void default_tiling(Ctx &c) {
 
 // Mesh set-up shared by aphip_set_mesh and the ensemble builder.  node_seg / elem_seg
 // delimit blocks of caller nodes / elements (one block per specimen) that are
-// Morton-ordered independently and never share a tile.
+// ordered independently and never share a tile.
 static void setup_mesh(Ctx &c, int32_t elem_type, int64_t n_nodes, const double *X, int64_t n_elem,
                        const int32_t *conn, const std::vector<int64_t> &node_seg, const std::vector<int64_t> &elem_seg) {
     APHIP_REQUIRE(elem_type == APHIP_TET4 || elem_type == APHIP_HEX8, "element type must be TET4 or HEX8");
