@@ -333,6 +333,13 @@ __global__ __launch_bounds__(NPE == 4 ? 1024 : (QPAR ? 512 : 256), NPE == 8 && !
     // forces (QPAR: this lane's node) stay in registers until their round's LDS pass.
     constexpr int KF = (QPAR && NPE == 8) ? 1 : NPE;
     double fe[KF][3];
+    // Tet4: the element phase stores H and keeps P for the start of the gather phase.
+    // The 144 B per element of H and P, all issued during the element phase, leave a CU
+    // faster than the memory takes them while many CUs are in that phase; split between
+    // the two phases they overlap the gather (tile kernel 274 -> 250 us on the 6M-tet
+    // block; H and P both after the barrier: 277 us; P after the gather: 256 us).  Hex8
+    // tiles, whose element phase is arithmetic-bound, measured the same either way.
+    double pdef[NPE == 4 ? 9 : 1];
     const int te_round = ROUNDS == 1 ? A.te : A.te / ROUNDS;
     const int my_round = ROUNDS == 1 ? 0 : le / te_round;
     // LDS row of entry (element j of the round, slot k).  One element per thread:
@@ -378,7 +385,8 @@ __global__ __launch_bounds__(NPE == 4 ? 1024 : (QPAR ? 512 : 256), NPE == 8 && !
 #pragma unroll
         for (int i = 0; i < 9; ++i) {
             store_once(A.H + i * A.ne + e, H[i]);
-            store_once(A.P + i * A.ne + e, P[i]);
+            if constexpr (NPE == 4) pdef[i] = P[i];
+            else store_once(A.P + i * A.ne + e, P[i]);
         }
         if (my_round == 0) {
 #pragma unroll
@@ -386,6 +394,12 @@ __global__ __launch_bounds__(NPE == 4 ? 1024 : (QPAR ? 512 : 256), NPE == 8 && !
         }
     }
     __syncthreads();
+    if constexpr (NPE == 4) {
+        if (has_elem) {
+#pragma unroll
+            for (int i = 0; i < 9; ++i) store_once(A.P + i * A.ne + e, pdef[i]);
+        }
+    }
 
     // C: deterministic per-node gather from LDS (each node's entries in ascending order:
     // round r adds the entries of elements [r, r + 1) x te / ROUNDS, a contiguous run of
