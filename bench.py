@@ -31,7 +31,7 @@ NODE_BYTES = 200
 # scaled by the same counters on a 1 GiB device copy (fetch x 2.0, write x 1.0); summed
 # over the step's two launches.  profiles/tile_step_tet4_6M_neohookean_hbm_traffic.txt
 # holds the commands and per-kernel figures.  Keyed by (material, hexes per edge).
-MEASURED_TRAFFIC = {("nh", 100): 1368.5e6, ("le", 100): 1367.1e6}
+MEASURED_TRAFFIC = {("nh", 100): 1369.9e6, ("le", 100): 1368.0e6}
 # FP64 flops per step of the same workload from rocprofv3 --pmc SQ_INSTS_VALU_{FMA,MUL,ADD,
 # TRANS}_F64 on the tile kernel (64 lanes x (2 FMA + MUL + ADD + TRANS) per wave
 # instruction) on the tile and shared-node kernels; profiles/tile_step_tet4_6M_neohookean_
