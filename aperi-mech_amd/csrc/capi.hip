@@ -8,6 +8,7 @@
 #include <cstring>
 #include <mutex>
 #include <numeric>
+#include <thread>
 
 #include "internal.hpp"
 
@@ -132,7 +133,9 @@ static std::vector<int32_t> rcb_order(const std::vector<double> &pts, int64_t n)
     const std::vector<uint64_t> code = morton_codes(pts, n);
     std::vector<int32_t> idx(n);
     std::iota(idx.begin(), idx.end(), 0);
-    std::function<void(int64_t, int64_t)> split = [&](int64_t b, int64_t e) {
+    // the halves of the top splits run on threads of their own (disjoint ranges of idx)
+    const int par_depth = std::thread::hardware_concurrency() >= 16 ? 4 : (std::thread::hardware_concurrency() >= 4 ? 2 : 0);
+    std::function<void(int64_t, int64_t, int)> split = [&](int64_t b, int64_t e, int depth) {
         const int64_t m = e - b;
         if (m <= 64) {
             std::sort(idx.begin() + b, idx.begin() + e, [&](int32_t x, int32_t y) {
@@ -157,10 +160,16 @@ static std::vector<int32_t> rcb_order(const std::vector<double> &pts, int64_t n)
             const double px = pts[3 * (int64_t)x + ax], py = pts[3 * (int64_t)y + ax];
             return px != py ? px < py : x < y;
         });
-        split(b, b + k);
-        split(b + k, e);
+        if (depth < par_depth && m > 65536) {
+            std::thread left([&, b, k, depth] { split(b, b + k, depth + 1); });
+            split(b + k, e, depth + 1);
+            left.join();
+        } else {
+            split(b, b + k, depth + 1);
+            split(b + k, e, depth + 1);
+        }
     };
-    split(0, n);
+    split(0, n, 0);
     return idx;
 }
 
