@@ -9,6 +9,7 @@
 #include <mutex>
 #include <numeric>
 #include <thread>
+#include <utility>
 
 #include "internal.hpp"
 
@@ -313,6 +314,7 @@ void renumber_nodes(Ctx &c, const std::vector<int32_t> &old_of_new) {
     if (c.has_mass) permute_device(c, c.mass, 1, old_of_new);
     if (c.has_fixed) permute_device(c, c.fixed, 1, old_of_new);
     if (c.pm_eig_valid) permute_device(c, c.pm_eig, 3, old_of_new);
+    if (c.pm_l_valid) permute_device(c, c.pm_l, 1, old_of_new);
     permute_device(c, c.node_dt, 1, old_of_new);  // ensembles: each specimen's increment
     // recorded adjoint checkpoints hold node state in the old numbering
     if (c.adj.ready) c.adj.reset();
@@ -364,6 +366,8 @@ static void setup_mesh(Ctx &c, int32_t elem_type, int64_t n_nodes, const double 
     c.Href.release();
     c.steps_done = 0;
     c.pm_eig_valid = false;
+    c.pm_l_valid = false;
+    for (DevBuf<double> &b : c.pm_work) b.release();
     c.tiles.reset();
     c.graph_reset();
     c.etype = elem_type;
@@ -416,6 +420,7 @@ static void setup_mesh(Ctx &c, int32_t elem_type, int64_t n_nodes, const double 
         b->zero(c.stream);
     }
     c.pm_eig.alloc(3 * n_nodes);  // the power method's eigenvector, kept between its calls
+    c.pm_l.alloc(n_nodes);        // and its max edge lengths
     c.H.alloc(9 * std::max<int64_t>(n_elem, 1));
     c.P.alloc(9 * std::max<int64_t>(n_elem, 1));
     c.H.zero(c.stream);
@@ -497,20 +502,44 @@ struct PowerMethodResult {
 // never reports oscillation; that outcome is kept.  The first comparison is against
 // prev_dt (1e-16 when 0, :363-364).  Like the reference's processor, the context keeps
 // the final eigenvector and starts the next call from it (:350-361, :468-469); only the
-// first call after the mesh is set draws the random start.  u, f, H, P and the NP1
-// state are restored afterwards, so the call leaves the simulation state untouched.
+// first call after the mesh is set draws the random start.  The force evaluations write H, P
+// and the NP1 state into the call's work buffers (the context's pointers are exchanged for
+// the call and exchanged back on every exit), and u and f are restored afterwards, so the
+// call leaves the simulation state untouched.  The max edge lengths are the context's.
 // Device buffers of one power-method call (internal numbering): the displacement and
-// force on entry, F(u), the iterate, the max edge lengths, the H / P / state on entry and
-// the norm's partial sums.
+// force on entry, F(u), the iterate, the targets of H / P / state and the norm's partial
+// sums.
 struct PowerMethodWork {
-    double *u_in, *f_in, *f0, *v, *l, *H, *P, *st, *partial;
+    double *u_in, *f_in, *f0, *v, *H, *P, *st, *partial;
 };
 
 // doubles each buffer of PowerMethodWork needs for the context's mesh and material
-static std::array<int64_t, 9> power_method_sizes(const Ctx &c) {
+static std::array<int64_t, 8> power_method_sizes(const Ctx &c) {
     const int64_t n3 = 3 * c.nn, nt = c.ne ? 9 * c.ne : 0;
-    return {n3, n3, n3, n3, c.nn, nt, nt, c.state_count(), pm_partials(c)};
+    return {n3, n3, n3, n3, nt, nt, c.state_count(), pm_partials(c)};
 }
+
+// points context buffers at other memory until it goes out of scope
+class Redirect {
+  public:
+    Redirect() = default;
+    Redirect(const Redirect &) = delete;
+    Redirect &operator=(const Redirect &) = delete;
+    void to(double *&ptr, double *target) {
+        saved_[n_++] = {&ptr, ptr};
+        ptr = target;
+    }
+    ~Redirect() {
+        while (n_ > 0) {
+            --n_;
+            *saved_[n_].first = saved_[n_].second;
+        }
+    }
+
+  private:
+    std::array<std::pair<double **, double *>, 3> saved_{};
+    int n_ = 0;
+};
 
 static PowerMethodResult power_method(Ctx &c, const PowerMethodWork &w, double prev_dt, int n_iter, double eps,
                                       double tol, double vtol, uint64_t seed) {
@@ -524,19 +553,22 @@ static PowerMethodResult power_method(Ctx &c, const PowerMethodWork &w, double p
     const int64_t ns = c.state_count();
     APHIP_HIP(hipMemcpyAsync(w.u_in, c.u.p, sizeof(double) * n3, hipMemcpyDeviceToDevice, c.stream));
     APHIP_HIP(hipMemcpyAsync(w.f_in, c.f.p, sizeof(double) * n3, hipMemcpyDeviceToDevice, c.stream));
+    Redirect redirect;
     if (c.ne) {
-        APHIP_HIP(hipMemcpyAsync(w.H, c.Hnp1().p, sizeof(double) * 9 * c.ne, hipMemcpyDeviceToDevice, c.stream));
-        APHIP_HIP(hipMemcpyAsync(w.P, c.Pnp1().p, sizeof(double) * 9 * c.ne, hipMemcpyDeviceToDevice, c.stream));
+        redirect.to(c.Hnp1().p, w.H);
+        redirect.to(c.Pnp1().p, w.P);
     }
-    if (ns)
-        APHIP_HIP(hipMemcpyAsync(w.st, c.state[c.state_np1].p, sizeof(double) * ns, hipMemcpyDeviceToDevice,
-                                 c.stream));
+    if (ns) redirect.to(c.state[c.state_np1].p, w.st);
     c.graph_reset();
-    pm_max_edge(c, w.l);
+    if (!c.pm_l_valid) {
+        pm_max_edge(c, c.pm_l.p);
+        c.pm_l_valid = true;
+    }
+    const double *l = c.pm_l.p;
     if (c.pm_eig_valid)
         APHIP_HIP(hipMemcpyAsync(w.v, c.pm_eig.p, sizeof(double) * n3, hipMemcpyDeviceToDevice, c.stream));
     else
-        pm_start(c, c.d_node_iperm.p, seed, eps, w.l, w.v);
+        pm_start(c, c.d_node_iperm.p, seed, eps, l, w.v);
     internal_force_eval(c);  // F(u)
     APHIP_HIP(hipMemcpyAsync(w.f0, c.f.p, sizeof(double) * n3, hipMemcpyDeviceToDevice, c.stream));
 
@@ -547,9 +579,9 @@ static PowerMethodResult power_method(Ctx &c, const PowerMethodWork &w, double p
     int used = n_iter;
     for (int k = 0; k < n_iter; ++k) {
         dt_n = dt_np1;
-        pm_perturb(c, w.u_in, w.v, w.l, eps);
+        pm_perturb(c, w.u_in, w.v, l, eps);
         internal_force_eval(c);
-        const double lambda = pm_next(c, w.f0, w.l, eps, w.v, w.partial);
+        const double lambda = pm_next(c, w.f0, l, eps, w.v, w.partial);
         pm_scale(c, lambda, w.v);
         dt_np1 = 2.0 / std::sqrt(lambda);
         recent[k % window] = dt_np1;
@@ -592,13 +624,6 @@ static PowerMethodResult power_method(Ctx &c, const PowerMethodWork &w, double p
 
     APHIP_HIP(hipMemcpyAsync(c.u.p, w.u_in, sizeof(double) * n3, hipMemcpyDeviceToDevice, c.stream));
     APHIP_HIP(hipMemcpyAsync(c.f.p, w.f_in, sizeof(double) * n3, hipMemcpyDeviceToDevice, c.stream));
-    if (c.ne) {
-        APHIP_HIP(hipMemcpyAsync(c.Hnp1().p, w.H, sizeof(double) * 9 * c.ne, hipMemcpyDeviceToDevice, c.stream));
-        APHIP_HIP(hipMemcpyAsync(c.Pnp1().p, w.P, sizeof(double) * 9 * c.ne, hipMemcpyDeviceToDevice, c.stream));
-    }
-    if (ns)
-        APHIP_HIP(hipMemcpyAsync(c.state[c.state_np1].p, w.st, sizeof(double) * ns, hipMemcpyDeviceToDevice,
-                                 c.stream));
     APHIP_HIP(hipStreamSynchronize(c.stream));
     return PowerMethodResult{dt, used, converged};
 }
@@ -1131,9 +1156,9 @@ int aphip_power_method_time_increment(aphip_ctx *c, double previous_time_increme
     APHIP_REQUIRE(previous_time_increment >= 0.0, "previous time increment must be >= 0");
     APHIP_HIP(hipSetDevice(c->device));
     const auto n = power_method_sizes(*c);
-    DevBuf<double> b[9];
-    for (int k = 0; k < 9; ++k) b[k].alloc((size_t)n[k]);
-    const PowerMethodWork w{b[0].p, b[1].p, b[2].p, b[3].p, b[4].p, b[5].p, b[6].p, b[7].p, b[8].p};
+    DevBuf<double> *b = c->pm_work;  // kept between calls; alloc keeps a buffer of the same size
+    for (int k = 0; k < 8; ++k) b[k].alloc((size_t)n[k]);
+    const PowerMethodWork w{b[0].p, b[1].p, b[2].p, b[3].p, b[4].p, b[5].p, b[6].p, b[7].p};
     const PowerMethodResult r = power_method(*c, w, previous_time_increment, num_iterations, epsilon, tolerance,
                                              variation_tolerance, seed);
     *time_increment = r.dt;
@@ -1154,18 +1179,18 @@ int aphip_power_method_time_increment_noalloc(aphip_ctx *c, double previous_time
     APHIP_REQUIRE(previous_time_increment >= 0.0, "previous time increment must be >= 0");
     // the work buffers, each on a 256-byte boundary inside this device's scratch
     const auto n = power_method_sizes(*c);
-    size_t off[10] = {0};
-    for (int k = 0; k < 9; ++k) off[k + 1] = off[k] + (((size_t)n[k] * sizeof(double) + 255) & ~size_t(255));
-    if (off[9] > kScratchBytes)
+    size_t off[9] = {0};
+    for (int k = 0; k < 8; ++k) off[k + 1] = off[k] + (((size_t)n[k] * sizeof(double) + 255) & ~size_t(255));
+    if (off[8] > kScratchBytes)
         throw Error(APHIP_ERR_INVALID, "the allocation-free power method holds its work buffers in " +
                                            std::to_string(kScratchBytes >> 20) + " MiB of scratch per GPU; this mesh needs " +
-                                           std::to_string(off[9]) + " bytes (use aphip_power_method_time_increment)");
+                                           std::to_string(off[8]) + " bytes (use aphip_power_method_time_increment)");
     APHIP_HIP(hipSetDevice(c->device));
     unsigned char *base = static_cast<unsigned char *>(scratch_address(c->device));
     APHIP_REQUIRE(base != nullptr, "scratch of this device not resolved");
-    double *p[9];
-    for (int k = 0; k < 9; ++k) p[k] = reinterpret_cast<double *>(base + off[k]);
-    const PowerMethodWork w{p[0], p[1], p[2], p[3], p[4], p[5], p[6], p[7], p[8]};
+    double *p[8];
+    for (int k = 0; k < 8; ++k) p[k] = reinterpret_cast<double *>(base + off[k]);
+    const PowerMethodWork w{p[0], p[1], p[2], p[3], p[4], p[5], p[6], p[7]};
     const PowerMethodResult r = power_method(*c, w, previous_time_increment, num_iterations, epsilon, tolerance,
                                              variation_tolerance, seed);
     *time_increment = r.dt;

@@ -337,9 +337,15 @@ struct Ctx {
 
     std::unique_ptr<exo::Writer> results;  // open Exodus II results file, if any
 
-    // power method: the eigenvector of the last call, the next call's start vector
+    // power method: the eigenvector of the last call, the next call's start vector; the
+    // per-node max edge length, computed on the first call after the mesh is set (it
+    // depends on X and the connectivity only; a node renumbering permutes it); the work
+    // buffers of the allocating entry point, kept between its calls
     DevBuf<double> pm_eig;               // 3 nn, internal numbering, allocated with the mesh
+    DevBuf<double> pm_l;                 // nn, internal numbering, allocated with the mesh
+    DevBuf<double> pm_work[8];
     bool pm_eig_valid = false;
+    bool pm_l_valid = false;
 
     CellData cells;
     TilePlan tiles;

@@ -2,7 +2,7 @@
 // PowerMethodProcessor, include/PowerMethodProcessor.h:150-470): per-node max edge
 // length, the seeded start vector, the perturbation u + v l eps, the next iterate
 // (F(u + v l eps) - F(u)) / (l eps m) and its L2 norm (deterministic block partial
-// sums, added in block order by one thread).  The driver is in capi.hip.
+// sums, added in a fixed order by one workgroup).  The driver is in capi.hip.
 #include <cmath>
 
 #include "internal.hpp"
@@ -91,13 +91,20 @@ __global__ __launch_bounds__(kBlock) void k_pm_next(int64_t nn, const double *f1
     if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
 }
 
-// partial[nb] = partial[0] + partial[1] + ... + partial[nb - 1], added in block order by
-// one thread (the order of a sequential host loop)
-__global__ __launch_bounds__(64) void k_pm_sum(int64_t nb, double *partial) {
-    if (threadIdx.x != 0) return;
+// partial[nb] = the sum of partial[0 .. nb-1] in a fixed order: lane t of one workgroup adds
+// partial[t], partial[t + kSumLanes], ... in turn, then a tree over the lanes
+constexpr int kSumLanes = 1024;
+__global__ __launch_bounds__(kSumLanes) void k_pm_sum(int64_t nb, double *partial) {
+    __shared__ double red[kSumLanes];
     double s = 0.0;
-    for (int64_t b = 0; b < nb; ++b) s += partial[b];
-    partial[nb] = s;
+    for (int64_t b = threadIdx.x; b < nb; b += kSumLanes) s += partial[b];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = kSumLanes / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[nb] = red[0];
 }
 
 __global__ __launch_bounds__(kBlock) void k_pm_scale(int64_t n, double norm, double *v) {
@@ -133,7 +140,7 @@ double pm_next(Ctx &c, const double *f0, const double *l, double eps, double *v,
     hipLaunchKernelGGL(k_pm_next, g, dim3(kBlock), 0, c.stream, c.nn, c.f.p, f0, c.mass.p, l,
                        c.has_fixed ? c.fixed.p : nullptr, eps, v, partial);
     APHIP_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_pm_sum, dim3(1), dim3(64), 0, c.stream, (int64_t)g.x, partial);
+    hipLaunchKernelGGL(k_pm_sum, dim3(1), dim3(kSumLanes), 0, c.stream, (int64_t)g.x, partial);
     APHIP_HIP(hipGetLastError());
     double s = 0.0;
     APHIP_HIP(hipMemcpyAsync(&s, partial + g.x, sizeof(double), hipMemcpyDeviceToHost, c.stream));

@@ -239,7 +239,10 @@ int aphip_step(aphip_ctx *ctx, double time_increment, int32_t n_steps);
  * set starts from a vector uniform in [-1, 1] per DOF (a counter-based generator keyed
  * by `seed` and the caller's DOF index, times epsilon and the node's max edge length);
  * every later call starts from the eigenvector the previous call ended with, as the
- * reference's processor does.  u, f, H, P and the state are restored on return. */
+ * reference's processor does.  u, f, H, P and the state are as before on return.  The
+ * max edge lengths are computed on the first call and kept with the mesh; the call's
+ * work buffers (8 (12 n_nodes + 18 n_elem + state values) bytes) are kept in the context
+ * between calls and released with the mesh. */
 int aphip_power_method_time_increment(aphip_ctx *ctx, double previous_time_increment, int32_t num_iterations,
                                       double epsilon, double tolerance,
                                       double variation_tolerance, uint64_t seed, double *time_increment,
@@ -249,9 +252,9 @@ int aphip_power_method_time_increment(aphip_ctx *ctx, double previous_time_incre
  * algorithm and the same results (it shares the kernels and the driver).  No counterpart
  * in the reference: it is for callers that cannot allocate device memory, such as a time
  * loop under HIP graph capture or a serving loop that forbids hipMalloc.  Its work buffers
- * (the displacement, force, H, P and state on entry, F(u), the iterate, the edge lengths
- * and the norm's partial sums: 8 (13 n_nodes + 18 n_elem + state values) bytes plus a
- * few KiB) live in a fixed 64 MiB array per GPU, held from library load to process end;
+ * (the displacement and force on entry, F(u), the iterate, the targets of H, P and the
+ * state, and the norm's partial sums: 8 (12 n_nodes + 18 n_elem + state values) bytes
+ * plus a few KiB) live in a fixed 64 MiB array per GPU, held from library load to process end;
  * a mesh that needs more is refused with APHIP_ERR_INVALID.  Every call on a GPU shares
  * that GPU's array: calls on contexts of the same GPU must not overlap in time (they
  * would race on it).  The eigenvector kept between calls stays in the context, as for

@@ -65,7 +65,7 @@ def test_no_alloc_power_method_matches_allocating_entry_point(gpu_available):
 
 
 def test_no_alloc_power_method_refuses_a_mesh_beyond_its_scratch(gpu_available):
-    # 50^3 hexes x 6 = 750,000 Tet4: H and P backups alone need 108 MB, more than the 64 MiB
+    # 50^3 hexes x 6 = 750,000 Tet4: the H and P work buffers alone need 108 MB, more than the 64 MiB
     X, hc = M.hex_lattice(50, 50, 50)
     with A.Solver() as s:
         s.set_mesh(A.TET4, X, M.split_tets(X, hc))
