@@ -60,6 +60,34 @@ def test_power_method_matches_oracle(gpu_available, etype, mat):
             assert np.array_equal(s.get(f), b), f
 
 
+def test_power_method_across_node_renumberings(gpu_available):
+    """The max edge lengths and the eigenvector the context keeps between calls follow the
+    node renumbering of a new tile plan: three calls, with a re-plan before the second and
+    the third, match the oracle's three calls in sequence (each starting from the previous
+    call's eigenvector)."""
+    X, conn = _mesh(A.TET4, 6, 23)
+    lam, mu = O.lame(E, NU)
+    mass = O.lumped_mass(A.TET4, X, conn, RHO)
+    ref = O.Solver(A.TET4, X, conn, A.NEO_HOOKEAN, lam, mu, mass)
+    l = O.max_edge_length(X, conn)
+    u = 1e-3 * (2.0 * M.uniform01(9, 1, X.size).reshape(X.shape) - 1.0)
+    eig = {}
+    with A.Solver() as s:
+        s.set_mesh(A.TET4, X, conn)
+        s.set_material(A.NEO_HOOKEAN, RHO, E, NU)
+        s.compute_mass()
+        s.set_field(A.FIELD_U, u)
+        for te in (None, 256, 512):
+            print(f"power method after plan {te}", flush=True)
+            if te is not None:
+                s.set_tiling(te, te)
+            dt_ref, it_ref, cv_ref = ref.power_method(l, u_in=u, eigenvector=eig)
+            dt, it, cv = s.power_method_time_increment()
+            assert (it, cv) == (it_ref, cv_ref), te
+            assert dt == pytest.approx(dt_ref, rel=1e-9), te
+        assert np.array_equal(s.get(A.FIELD_U), u)
+
+
 def test_power_method_rkpm_matches_oracle(gpu_available):
     X, conn = _mesh(A.TET4, 4, 5)
     scale = 1.5
