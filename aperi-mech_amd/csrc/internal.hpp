@@ -216,6 +216,7 @@ struct TilePlan {
     DevBuf<uint16_t> lcsr;            // [ntiles * te * npe] local element * npe + slot
     DevBuf<uint16_t> lconn;           // [ne * npe] local node of each element slot
     DevBuf<double> bpart;             // [slots * 3] tile sums of shared nodes
+    DevBuf<double> upred;             // [nb * 3] shared nodes' predicted displacement for the next step
     DevBuf<int32_t> bstart;           // [nb + 1] slot range of each shared node
     DevBuf<double> zero;              // [8] zeros: stands in for absent optional arrays
     DevBuf<int32_t> meta;             // [ntiles * 8] packed per-tile scalars of the pipelined step
@@ -234,7 +235,7 @@ struct TilePlan {
         lds_bytes = 0;
         for (int &g : stream_grid) g = 0;
         tn_start.release(); tile_e0.release(); tn_ninterior.release(); int_base.release(); tn_node.release(); tn_csr.release();
-        tn_bdst.release(); lcsr.release(); lconn.release(); bpart.release(); bstart.release();
+        tn_bdst.release(); lcsr.release(); lconn.release(); bpart.release(); bstart.release(); upred.release();
         zero.release(); meta.release();
         tpad = 0;
         tnp_node.release(); tnp_csr.release(); tnp_bdst.release(); lconn_p.release();
@@ -377,7 +378,11 @@ void renumber_nodes(Ctx &c, const std::vector<int32_t> &old_of_new);
 // One tiled pass (MODE 0 force only, 1 force and a = f/m, 2 the full step).  In MODE 0 the
 // displacement read and the force written may be other arrays than the context's u and f
 // (the two-pass RKPM force: physical displacement in, local force out).
-void tile_run(Ctx &c, double dt, int mode, const double *u_in = nullptr, double *f_out = nullptr);
+// pred_in (mode 2): shared nodes take the displacement predicted by the previous step's
+// shared-node pass (it must have run with pred_out and the same dt in this same call);
+// pred_out (mode 2): the shared-node pass stores that prediction for the next step.
+void tile_run(Ctx &c, double dt, int mode, const double *u_in = nullptr, double *f_out = nullptr,
+              bool pred_in = false, bool pred_out = false);
 
 // ---- adjoint (adjoint.hip) ----
 void adjoint_forward(Ctx &c, double dt, int n_steps, int interval);

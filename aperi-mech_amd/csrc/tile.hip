@@ -85,6 +85,10 @@ struct TileArgs {
     int te, maxtn;
     int max_nint;  // most interior nodes in one tile (Hex8: LDS rows of the corrector data)
     double dt;
+    // mode 2: the shared nodes' displacement predicted by the previous step's shared-node
+    // pass (row first_shared + b at upred[3 b]), or null: predicted here from u, v, a
+    const double *upred;
+    int64_t first_shared;
 };
 
 // LDS carve-up (bytes), sized on the host (tile_lds_bytes) from maxtn, max_nint, te
@@ -234,19 +238,28 @@ __global__ __launch_bounds__(NPE == 4 ? 1024 : (QPAR ? 512 : 256), NPE == 8 && !
     }
     double xq[NPT][3], uq[NPT][3], vq[NPT][3], aq[NPT][3], mq[NPT], dq[NPT];
     unsigned fq[NPT];
+    bool pq[NPT];  // shared row with a stored prediction: X and the predicted u only
 #pragma unroll
     for (int q = 0; q < NPT; ++q) {
         const int64_t g = gq[q];
+        pq[q] = MODE == 2 && A.upred != nullptr && bdq[q] >= 0;
         ld3(A.X, g, xq[q]);
-        ld3(A.u, g, uq[q]);
-        fq[q] = load_fixed(A, g);
         mq[q] = 1.0;
         dq[q] = A.dt;
-        if constexpr (MODE >= 1) mq[q] = A.mass[g];
-        if constexpr (MODE == 2) {
-            ld3(A.v, g, vq[q]);
-            ld3(A.a, g, aq[q]);
-            dq[q] = load_node_dt(A, g);
+        if (pq[q]) {
+            ld3(A.upred, g - A.first_shared, uq[q]);
+            fq[q] = 0u;
+#pragma unroll
+            for (int d = 0; d < 3; ++d) vq[q][d] = aq[q][d] = 0.0;
+        } else {
+            ld3(A.u, g, uq[q]);
+            fq[q] = load_fixed(A, g);
+            if constexpr (MODE >= 1) mq[q] = A.mass[g];
+            if constexpr (MODE == 2) {
+                ld3(A.v, g, vq[q]);
+                ld3(A.a, g, aq[q]);
+                dq[q] = load_node_dt(A, g);
+            }
         }
     }
     // per-tile scalars, read after the staging loads are out (their addresses need none)
@@ -280,8 +293,10 @@ __global__ __launch_bounds__(NPE == 4 ? 1024 : (QPAR ? 512 : 256), NPE == 8 && !
     }
     // Hex8 keeps the corrector data of interior nodes only (the only ones it finishes);
     // Tet4 writes it for every node (measured faster with its 512-element tiles).
+    // pre: u is already the predicted displacement (a shared row; the corrector data
+    // written for it are never read)
     auto stage = [&](int ln, uint32_t csr, int32_t bdst, const double *x, const double *u, const double *v,
-                     const double *a, unsigned fx, double mm, double nd) {
+                     const double *a, unsigned fx, double mm, double nd, bool pre) {
         L.csr[ln] = csr;
         L.bdst[ln] = bdst;
         st3(L.X, ln, x);
@@ -292,8 +307,8 @@ __global__ __launch_bounds__(NPE == 4 ? 1024 : (QPAR ? 512 : 256), NPE == 8 && !
             L.DT[ln] = nd;
         }
         if constexpr (MODE == 2) {
-            double vh[3], u1[3];
-            cd_predict(u, v, a, fx, nd, vh, u1);
+            double vh[3] = {0.0, 0.0, 0.0}, u1[3] = {u[0], u[1], u[2]};
+            if (!pre) cd_predict(u, v, a, fx, nd, vh, u1);
             st3(L.U, ln, u1);
             if (interior) st3(L.V, ln, vh);
         } else {
@@ -303,7 +318,7 @@ __global__ __launch_bounds__(NPE == 4 ? 1024 : (QPAR ? 512 : 256), NPE == 8 && !
 #pragma unroll
     for (int q = 0; q < NPT; ++q) {
         const int ln = tid + q * (int)blockDim.x;
-        if (ln < tn) stage(ln, csrq[q], bdq[q], xq[q], uq[q], vq[q], aq[q], fq[q], mq[q], dq[q]);
+        if (ln < tn) stage(ln, csrq[q], bdq[q], xq[q], uq[q], vq[q], aq[q], fq[q], mq[q], dq[q], pq[q]);
     }
     // rows past NPT x blockDim (tiles with unusually many nodes)
     for (int ln = tid + NPT * (int)blockDim.x; ln < tn; ln += blockDim.x) {
@@ -319,7 +334,7 @@ __global__ __launch_bounds__(NPE == 4 ? 1024 : (QPAR ? 512 : 256), NPE == 8 && !
             ld3(A.a, g, a);
             nd = load_node_dt(A, g);
         }
-        stage(ln, A.tn_csr[tb + ln], A.tn_bdst[tb + ln], x, u, v, a, fx, mm, nd);
+        stage(ln, A.tn_csr[tb + ln], A.tn_bdst[tb + ln], x, u, v, a, fx, mm, nd, false);
     }
     // Every load of the wave completes here, on every path (a wave whose lanes all
     // skip the staging loop still has its element prefetch in flight).  Otherwise the
@@ -775,6 +790,7 @@ struct BoundArgs {
     const double *node_dt;
     const double *zero;  // 8 zero doubles: stands in for an absent optional array
     double dt;
+    double *upred;  // mode 2: store the next step's predicted displacement (same dt), or null
 };
 
 template <int MODE>
@@ -825,6 +841,11 @@ __global__ __launch_bounds__(256) void k_boundary(BoundArgs A) {
         st3(A.u, g, u1);
         st3(A.v, g, v1);
         st3(A.a, g, a1);
+        if (A.upred) {  // the predictor of the next step from the values just stored
+            double vh2[3], u2[3];
+            cd_predict(u1, v1, a1, fx, dt, vh2, u2);
+            st3(A.upred, b, u2);
+        }
     }
 }
 
@@ -1071,6 +1092,7 @@ void tile_build(Ctx &c, int te, int threads) {
     }
     P.nslot = bstart[nb];
     P.bpart.alloc(3 * std::max<int64_t>(1, (int64_t)bstart[nb]));
+    P.upred.alloc(3 * std::max<int64_t>(1, nb));
     P.bstart.upload(bstart.data(), bstart.size(), c.stream);
     P.zero.alloc(8);
     P.zero.zero(c.stream);
@@ -1112,10 +1134,13 @@ void tile_build(Ctx &c, int te, int threads) {
     APHIP_HIP(hipStreamSynchronize(c.stream));
 }
 
-void tile_run(Ctx &c, double dt, int mode, const double *u_in, double *f_out) {
+void tile_run(Ctx &c, double dt, int mode, const double *u_in, double *f_out, bool pred_in, bool pred_out) {
     TilePlan &P = c.tiles;
     APHIP_REQUIRE(P.ready, "tile plan not built");
     APHIP_REQUIRE(mode == 0 || (u_in == nullptr && f_out == nullptr), "displacement / force arrays only in force mode");
+    APHIP_REQUIRE(mode == 2 || !(pred_in || pred_out), "predicted displacements only in step mode");
+    pred_in = pred_in && P.nb > 0;
+    pred_out = pred_out && P.nb > 0;
     if (c.ne == 0) return;
     TileArgs A;
     A.X = c.X.p;
@@ -1163,6 +1188,8 @@ void tile_run(Ctx &c, double dt, int mode, const double *u_in, double *f_out) {
     A.maxtn = P.maxtn;
     A.max_nint = P.max_nint;
     A.dt = dt;
+    A.upred = pred_in ? P.upred.p : nullptr;
+    A.first_shared = P.n_interior;
     prof_begin(c, 1);
     if (c.npe == 4) {
         if (mode == 0) dispatch_kind<4, 0>(c, A, P.lds_bytes);
@@ -1177,7 +1204,8 @@ void tile_run(Ctx &c, double dt, int mode, const double *u_in, double *f_out) {
     prof_end(c, 1);
     if (P.nb > 0) {
         BoundArgs B{P.nb, P.n_interior, P.bstart.p, P.bpart.p, c.u.p, c.v.p, c.a.p, A.f, c.mass.p,
-                    c.has_fixed ? c.fixed.p : nullptr, c.node_dt.p, P.zero.p, dt};
+                    c.has_fixed ? c.fixed.p : nullptr, c.node_dt.p, P.zero.p, dt,
+                    pred_out ? P.upred.p : nullptr};
         const dim3 grid((unsigned)((P.nb + 255) / 256));
         prof_begin(c, 2);
         if (mode == 0) hipLaunchKernelGGL(k_boundary<0>, grid, dim3(256), 0, c.stream, B);

@@ -220,6 +220,48 @@ def test_tile_results_are_deterministic(gpu_available):
         assert np.array_equal(a, b)  # bitwise: no atomics anywhere in the step
 
 
+@pytest.mark.parametrize("etype,tiling", [(A.TET4, (128, 128)), (A.HEX8, (128, 128)), (A.HEX8, (16, 128))],
+                         ids=["tet4", "hex8", "hex8-gauss-lanes"])
+@pytest.mark.parametrize("mat", ["nh", "j2"])
+def test_steps_of_one_call_match_single_steps(gpu_available, etype, tiling, mat):
+    """Within one aphip_step call the shared-node pass hands the next step's tile kernel
+    the shared nodes' predicted displacement; a call of one step predicts it from u, v
+    and a.  Thirteen single-step calls, one eager call of thirteen steps and one call
+    through the two-step graph give bitwise equal fields (fixed DOFs, many tiles)."""
+    kind, p = MATS[mat]
+    X, conn = _mesh(etype, 8 if etype == A.HEX8 else 6, 12)
+    fixed = np.zeros(X.shape, dtype=np.uint8)
+    fixed[X[:, 0] < 1e-3] = 1
+    fixed[X[:, 2] < 1e-3, 2] = 1
+    vscale = 30.0 if mat == "j2" else 5.0
+    v0 = vscale * (2.0 * M.uniform01(7, 2, X.size).reshape(X.shape) - 1.0)
+    v0[fixed.astype(bool)] = 0.0
+    dt = 0.4 * M.stable_dt_estimate(X, conn, p["E"], p["nu"], p["rho"])
+    flds = [A.FIELD_U, A.FIELD_V, A.FIELD_A, A.FIELD_F, A.FIELD_H, A.FIELD_P]
+    if kind == A.J2:
+        flds.append(A.FIELD_STATE)
+    outs = []
+    for mode in ("single", "eager", "graph"):
+        print(f"one call vs single steps etype={etype} tiling={tiling} mat={mat} mode={mode}", flush=True)
+        with _gpu(etype, X, conn, kind, p) as s:
+            s.set_tiling(*tiling)
+            s.set_fixed(fixed)
+            s.compute_mass()
+            s.set_field(A.FIELD_V, v0)
+            s.initialize()
+            s.set_graph(mode == "graph")
+            if mode == "single":
+                for _ in range(13):
+                    s.step(dt, 1)
+            else:
+                s.step(dt, 13)
+            outs.append([s.get(f) for f in flds])
+    for other in outs[1:]:
+        for f, a, b in zip(flds, outs[0], other):
+            assert np.array_equal(a, b), f
+    assert np.abs(outs[0][0]).max() > 0.0
+
+
 @pytest.mark.parametrize("mat", ["nh", "le", "j2"])
 @pytest.mark.parametrize("te", [64, 128, 256])
 def test_pipelined_step_matches_one_tile_per_workgroup(gpu_available, mat, te):
