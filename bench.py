@@ -31,12 +31,12 @@ NODE_BYTES = 200
 # scaled by the same counters on a 1 GiB device copy (fetch x 2.0, write x 1.0); summed
 # over the step's two launches.  profiles/tile_step_tet4_6M_neohookean_hbm_traffic.txt
 # holds the commands and per-kernel figures.  Keyed by (material, hexes per edge).
-MEASURED_TRAFFIC = {("nh", 100): 1369.9e6, ("le", 100): 1368.0e6}
+MEASURED_TRAFFIC = {("nh", 100): 1339.7e6, ("le", 100): 1337.9e6}
 # FP64 flops per step of the same workload from rocprofv3 --pmc SQ_INSTS_VALU_{FMA,MUL,ADD,
 # TRANS}_F64 on the tile kernel (64 lanes x (2 FMA + MUL + ADD + TRANS) per wave
 # instruction) on the tile and shared-node kernels; profiles/tile_step_tet4_6M_neohookean_
 # fp64_valu.txt.  FP64 vector peak of the MI355X: 78.6 TFLOP/s.
-MEASURED_FP64_FLOPS = {("nh", 100): 3.31e9, ("le", 100): 1.65e9}
+MEASURED_FP64_FLOPS = {("nh", 100): 3.30e9, ("le", 100): 1.64e9}
 FP64_PEAK_TFLOPS = 78.6
 
 
